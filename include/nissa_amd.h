@@ -1,0 +1,183 @@
+/*
+ * nissa_amd — MI355X (gfx950) native fermion-solve path for nissa.
+ *
+ * C ABI: plain pointers, sizes and int status codes.  Host buffers use nissa's
+ * CPU ("AoS") layout (src/base/field.hpp:408-416):
+ *   spincolor double[4][3][2], color double[3][2], quad_su3 double[4][3][3][2];
+ *   lx sites t-major / z-fastest, eo sites = lx>>1 within each parity
+ *   (src/geometry/geometry_lx.cpp:51-60, src/geometry/geometry_eo.cpp:67-77).
+ * Gauge links are passed with every phase already applied (boundary
+ * conditions, staggered eta phases), as nissa's callers do before calling
+ * the operators (src/operations/gaugeconf.cpp:125-182, src/hmc/backfield.cpp:230-245).
+ *
+ * Every entry point returns NQ_OK (0) or an error code; nq_last_error() gives
+ * the message of the last failure on the calling thread.  A context owns one
+ * HIP stream on one device; calls on one context are serialised, distinct
+ * contexts may be used from different threads.
+ */
+#ifndef NISSA_AMD_H
+#define NISSA_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (nissa CRASHes on these conditions, src/base/debug.hpp:25) */
+#define NQ_OK 0
+#define NQ_EINVAL 1   /* bad argument / shape mismatch            */
+#define NQ_ENOCONV 2  /* "exit without converging"  (cg_invert_template_threaded.hpp:146-147) */
+#define NQ_ENAN 3     /* "nanned" / "invalid norm" (cg_invert_template_threaded.hpp:55,113)   */
+#define NQ_EOOM 4     /* device allocation failed                 */
+#define NQ_EDEVICE 5  /* HIP runtime error / no device            */
+#define NQ_EBUSY 6    /* admission refused (serving)              */
+
+/* field coverage (nissa FieldCoverage, src/base/field.hpp:106) */
+#define NQ_EVEN 0
+#define NQ_ODD 1
+#define NQ_LX 2
+/* field kind */
+#define NQ_SPINCOLOR 0
+#define NQ_COLOR 1
+/* solver precision of the inner Krylov iteration */
+#define NQ_PREC_F64 0      /* plain FP64 CG                                          */
+#define NQ_PREC_F32 1      /* FP32 spinors + FP32 18-real links, FP64 reliable updates */
+#define NQ_PREC_F16 2      /* FP16 spinors + per-site FP32 norm, FP32 12-real links  */
+#define NQ_PREC_F16H 3     /* FP16 spinors + per-site norm, FP16 12-real links       */
+
+typedef struct nq_ctx nq_ctx;
+typedef struct nq_gauge nq_gauge;
+typedef struct nq_field nq_field;
+
+typedef struct nq_solve_stats {
+  int iters;             /* Krylov iterations (sloppy ones for mixed precision)      */
+  int reliable_updates;  /* FP64 true-residual recomputations inside the loop       */
+  int converged;         /* 1 when the FP64 true residual met the request           */
+  int reserved;
+  double rel_res_sq;     /* |b - A x|^2 / |b|^2 recomputed in FP64 after the loop   */
+  double seconds;        /* device time of the solve (HIP events on the ctx stream) */
+} nq_solve_stats;
+
+/* ------------------------------------------------------------------ context */
+const char* nq_last_error(void);
+int nq_device_count(int* n);
+/* replaces nissa's initGrid(T,L) geometry set-up on one rank (src/base/init.cpp:563) */
+int nq_ctx_create(int T, int LX, int LY, int LZ, int device, nq_ctx** out);
+void nq_ctx_destroy(nq_ctx* ctx);
+int nq_ctx_sync(nq_ctx* ctx);
+/* per-kernel timing ledger (HIP events around every launch when enabled) */
+int nq_ctx_set_profiling(nq_ctx* ctx, int on);
+int nq_ctx_kernel_count(nq_ctx* ctx, int* n);
+int nq_ctx_kernel_stat(nq_ctx* ctx, int i, char* name, int name_len, int64_t* launches,
+                       double* total_ms, double* bytes_per_launch);
+int nq_ctx_reset_kernel_stats(nq_ctx* ctx);
+
+/* ------------------------------------------------------------------ gauge */
+/* EoField<quad_su3> as two host buffers of Vh quad_su3 (src/base/field.hpp:1586-1599) */
+int nq_gauge_create_eo(nq_ctx* ctx, const double* conf_evn, const double* conf_odd, nq_gauge** out);
+/* LxField<quad_su3>: split on the device like split_lx_vector_into_eo_parts */
+int nq_gauge_create_lx(nq_ctx* ctx, const double* conf_lx, nq_gauge** out);
+void nq_gauge_destroy(nq_gauge* g);
+/* device bytes held by the gauge handle (all precisions built so far) */
+int64_t nq_gauge_device_bytes(const nq_gauge* g);
+
+/* ------------------------------------------------------------------ fields */
+int nq_field_create(nq_ctx* ctx, int kind, int coverage, nq_field** out);
+void nq_field_destroy(nq_field* f);
+int nq_field_upload(nq_field* f, const double* host_aos);
+int nq_field_download(const nq_field* f, double* host_aos);
+int nq_field_copy(nq_field* dst, const nq_field* src);
+int nq_field_norm2(const nq_field* f, double* out);
+
+/* ------------------------------------------------------------------ twisted mass (device handles) */
+/* tmn2Deo_or_tmn2Doe_eos (dirac_operator_tmD_eoprec_portable.hpp:36-169); out parity != in parity */
+int nq_tm_hop(nq_ctx* ctx, const nq_gauge* g, const double* anis, nq_field* out, const nq_field* in);
+/* tmDee_or_oo_eos / inv_tmDee_or_oo_eos (..._portable.hpp:174-222) */
+int nq_tm_Dee(nq_ctx* ctx, double kappa, double mu, int inverse, nq_field* out, const nq_field* in);
+/* tmDkern_eoprec_eos (..._portable.cpp:36-51) */
+int nq_tm_kern(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis, double mu,
+               nq_field* out_odd, const nq_field* in_odd);
+/* tmDkern_eoprec_square_eos (..._portable.cpp:54-65) */
+int nq_tm_kern_square(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis, double mu,
+                      nq_field* out_odd, const nq_field* in_odd);
+/* apply_tmQ on lx (dirac_operator_tmQ_portable.cpp:29-176) */
+int nq_tm_apply_Q(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis, double mu,
+                  nq_field* out_lx, const nq_field* in_lx);
+/* apply_tmQ2 (dirac_operator_tmQ2.hpp:8): Q(-mu) Q(+mu) on lx */
+int nq_tm_apply_Q2(nq_ctx* ctx, const nq_gauge* g, double kappa, double mu,
+                   nq_field* out_lx, const nq_field* in_lx);
+/* inv_tmDkern_eoprec_square_eos_cg (cg_invert_tmD_eoprec.cpp:61-78): solve K K^+ sol = src.
+   residue is nissa's squared relative residual. */
+int nq_tm_cg_kernsq(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis, double mu,
+                    int niter, double residue, int prec, nq_field* sol_odd,
+                    const nq_field* guess_odd, const nq_field* src_odd, nq_solve_stats* st);
+/* inv_tmD_cg_eoprec (cg_invert_tmD_eoprec.cpp:152-213): full eo-preconditioned solve of
+   D sol = src on lx, plus the FP64 check |g5 Q sol - src|^2/|src|^2 */
+int nq_tm_solve_eoprec(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis, double mu,
+                       int niter, double residue, int prec, nq_field* sol_lx,
+                       const nq_field* src_lx, nq_solve_stats* st, double* check_rel_res_sq);
+/* inv_tmQ2_cg (cg_invert_tmQ2.cpp:12-19): CG on Q(-mu)Q(+mu) over lx */
+int nq_tmQ2_cg(nq_ctx* ctx, const nq_gauge* g, double kappa, double mu, int niter, double residue,
+               nq_field* sol_lx, const nq_field* guess_lx, const nq_field* src_lx, nq_solve_stats* st);
+/* multi-RHS eo solve: nrhs sources share the gauge handle, kappa and the iteration;
+   mu[r] per right-hand side (per-RHS diagonal). Residuals are per RHS. */
+int nq_tm_solve_eoprec_multi(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis,
+                             int nrhs, const double* mu, int niter, double residue, int prec,
+                             nq_field** sol_lx, const nq_field* const* src_lx, nq_solve_stats* st);
+
+/* ------------------------------------------------------------------ staggered */
+/* apply_st2Doe (dirac_operator_stD.hpp:47-77) */
+int nq_st_2Doe(nq_ctx* ctx, const nq_gauge* g, nq_field* out_odd, const nq_field* in_evn);
+/* apply_stDeo_half (dirac_operator_stD.hpp:94-127) */
+int nq_st_Deo_half(nq_ctx* ctx, const nq_gauge* g, nq_field* out_evn, const nq_field* in_odd);
+/* apply_stD2ee_m2 (dirac_operator_stD.hpp:130-225): mass2 in - 1/4 Deo Doe in */
+int nq_st_DDee(nq_ctx* ctx, const nq_gauge* g, double mass2, nq_field* out_evn, const nq_field* in_evn);
+/* inv_stD2ee_m2_cg (cg_invert_stD2ee_m2.hpp:18-43) */
+int nq_st_cg(nq_ctx* ctx, const nq_gauge* g, double mass2, int niter, double residue,
+             nq_field* sol_evn, const nq_field* guess_evn, const nq_field* src_evn, nq_solve_stats* st);
+/* cgm_invert with ApplyStD2eeM2Functor (cgm_invert_template_threaded.hpp:13-295,
+   cgm_invert_stD2ee_m2_portable.hpp:10-29), batched over nrhs sources in one Krylov
+   loop per source: sols[r*nshift+k]. prec NQ_PREC_F64 or NQ_PREC_F32 (FP32 inner,
+   FP64 reliable updates). true_res_sq[r*nshift+k] gets the FP64 check. */
+int nq_st_cgm(nq_ctx* ctx, const nq_gauge* g, int nrhs, int nshift, const double* shifts,
+              int niter, const double* residues, int prec, nq_field** sols,
+              const nq_field* const* srcs, nq_solve_stats* st, double* true_res_sq);
+
+/* ------------------------------------------------------------------ host-buffer drop-ins */
+/* Same operations over nissa host buffers; the gauge handle stays resident. */
+int nq_tm_kern_square_host(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis,
+                           double mu, double* out_odd, const double* in_odd);
+int nq_tm_solve_eoprec_host(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis,
+                            double mu, int niter, double residue, int prec, double* sol_lx,
+                            const double* src_lx, nq_solve_stats* st, double* check_rel_res_sq);
+
+/* ------------------------------------------------------------------ host-side harness helpers */
+/* Seeded synthetic gauge field U = exp(i eps H) per link, H traceless Hermitian with
+   N(0,1/2) components in the Gell-Mann basis (nissa herm_put_to_gauss, random.hpp:311-338);
+   counter-based Philox4x32-10 keyed by (seed, site, mu): identical on any host. */
+int nq_host_gauge_random_lx(int T, int LX, int LY, int LZ, uint64_t seed, double eps, double* conf_lx);
+/* the algebra element H and link for one (site, mu), for tests */
+int nq_host_link_algebra(uint64_t seed, int64_t lx, int mu, double eps, double* H18, double* U18);
+/* put_boundaries_conditions (gaugeconf.cpp:125-147): U_mu *= exp(i pi theta_mu / L_mu) */
+int nq_host_gauge_phase_lx(int T, int LX, int LY, int LZ, const double* theta_in_pi, double* conf_lx);
+/* add_or_rem_stagphases_to_conf (backfield.cpp:230-245) on an lx conf */
+int nq_host_gauge_stagphases_lx(int T, int LX, int LY, int LZ, double* conf_lx);
+/* sources: kind 0 = Z2 (+-1 real), 1 = Gaussian complex (N(0,1/2) per real),
+   2 = point source (site0, spin-colour index sc) ; ncomp = 12 (spincolor) or 3 (color);
+   parity_mask: 1 = even sites only, 2 = odd only, 3 = all (lx buffer) */
+int nq_host_source_lx(int T, int LX, int LY, int LZ, uint64_t seed, int kind, int ncomp,
+                      int parity_mask, int64_t site0, int sc, double* out_lx);
+int nq_host_split_lx(int T, int LX, int LY, int LZ, int ndoubles, const double* lx, double* evn, double* odd);
+int nq_host_paste_lx(int T, int LX, int LY, int LZ, int ndoubles, double* lx, const double* evn, const double* odd);
+
+/* ------------------------------------------------------------------ instrumentation */
+/* napply back-to-back hop applies on device fields, timed with HIP events on the
+   ctx stream; prec selects the storage the kernel reads (NQ_PREC_*). */
+int nq_bench_tm_hop(nq_ctx* ctx, const nq_gauge* g, int prec, int napply, nq_field* out,
+                    const nq_field* in, double* ms_per_apply, double* bytes_per_apply);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,458 @@
+"""nissa_amd: MI355X-native fermion-solve path for nissa, Python bindings.
+
+The product is the C-ABI library ``libnissa_amd.so`` (declared in
+``include/nissa_amd.h``); this module only binds it with ctypes for tests,
+benchmarks and scripting.  Loading fails loudly when the library has not been
+built: there is no fallback implementation.
+
+Host arrays follow nissa's CPU layout: spincolor ``(..., 4, 3, 2)`` float64,
+color ``(..., 3, 2)``, quad_su3 ``(..., 4, 3, 3, 2)``; lx site order t-major,
+z fastest; eo fields hold the Vh sites of one parity in lx>>1 order.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import weakref
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnissa_amd.so")
+
+NQ_OK, NQ_EINVAL, NQ_ENOCONV, NQ_ENAN, NQ_EOOM, NQ_EDEVICE, NQ_EBUSY = range(7)
+EVEN, ODD, LX = 0, 1, 2
+SPINCOLOR, COLOR = 0, 1
+PREC_F64, PREC_F32, PREC_F16, PREC_F16H = 0, 1, 2, 3
+PREC = {"f64": PREC_F64, "f32": PREC_F32, "f16": PREC_F16, "f16h": PREC_F16H}
+
+
+class NqError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"nissa_amd error {code}: {msg}")
+        self.code = code
+
+
+class SolveStats(C.Structure):
+    _fields_ = [
+        ("iters", C.c_int),
+        ("reliable_updates", C.c_int),
+        ("converged", C.c_int),
+        ("reserved", C.c_int),
+        ("rel_res_sq", C.c_double),
+        ("seconds", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+_lib = None
+
+_P = C.c_void_p
+_D = C.c_double
+_I = C.c_int
+_DP = C.POINTER(C.c_double)
+_SIG = {
+    "nq_last_error": ([], C.c_char_p),
+    "nq_device_count": ([C.POINTER(C.c_int)], _I),
+    "nq_ctx_create": ([_I, _I, _I, _I, _I, C.POINTER(_P)], _I),
+    "nq_ctx_destroy": ([_P], None),
+    "nq_ctx_sync": ([_P], _I),
+    "nq_ctx_set_profiling": ([_P, _I], _I),
+    "nq_ctx_kernel_count": ([_P, C.POINTER(C.c_int)], _I),
+    "nq_ctx_kernel_stat": ([_P, _I, C.c_char_p, _I, C.POINTER(C.c_int64), _DP, _DP], _I),
+    "nq_ctx_reset_kernel_stats": ([_P], _I),
+    "nq_gauge_create_eo": ([_P, _DP, _DP, C.POINTER(_P)], _I),
+    "nq_gauge_create_lx": ([_P, _DP, C.POINTER(_P)], _I),
+    "nq_gauge_destroy": ([_P], None),
+    "nq_gauge_device_bytes": ([_P], C.c_int64),
+    "nq_field_create": ([_P, _I, _I, C.POINTER(_P)], _I),
+    "nq_field_destroy": ([_P], None),
+    "nq_field_upload": ([_P, _DP], _I),
+    "nq_field_download": ([_P, _DP], _I),
+    "nq_field_copy": ([_P, _P], _I),
+    "nq_field_norm2": ([_P, _DP], _I),
+    "nq_tm_hop": ([_P, _P, _DP, _P, _P], _I),
+    "nq_tm_Dee": ([_P, _D, _D, _I, _P, _P], _I),
+    "nq_tm_kern": ([_P, _P, _D, _DP, _D, _P, _P], _I),
+    "nq_tm_kern_square": ([_P, _P, _D, _DP, _D, _P, _P], _I),
+    "nq_tm_apply_Q": ([_P, _P, _D, _DP, _D, _P, _P], _I),
+    "nq_tm_apply_Q2": ([_P, _P, _D, _D, _P, _P], _I),
+    "nq_tm_cg_kernsq": ([_P, _P, _D, _DP, _D, _I, _D, _I, _P, _P, _P, C.POINTER(SolveStats)], _I),
+    "nq_tm_solve_eoprec": ([_P, _P, _D, _DP, _D, _I, _D, _I, _P, _P, C.POINTER(SolveStats), _DP], _I),
+    "nq_tmQ2_cg": ([_P, _P, _D, _D, _I, _D, _P, _P, _P, C.POINTER(SolveStats)], _I),
+    "nq_tm_solve_eoprec_multi": ([_P, _P, _D, _DP, _I, _DP, _I, _D, _I, C.POINTER(_P), C.POINTER(_P),
+                                  C.POINTER(SolveStats)], _I),
+    "nq_st_2Doe": ([_P, _P, _P, _P], _I),
+    "nq_st_Deo_half": ([_P, _P, _P, _P], _I),
+    "nq_st_DDee": ([_P, _P, _D, _P, _P], _I),
+    "nq_st_cg": ([_P, _P, _D, _I, _D, _P, _P, _P, C.POINTER(SolveStats)], _I),
+    "nq_st_cgm": ([_P, _P, _I, _I, _DP, _I, _DP, _I, C.POINTER(_P), C.POINTER(_P), C.POINTER(SolveStats), _DP], _I),
+    "nq_tm_kern_square_host": ([_P, _P, _D, _DP, _D, _DP, _DP], _I),
+    "nq_tm_solve_eoprec_host": ([_P, _P, _D, _DP, _D, _I, _D, _I, _DP, _DP, C.POINTER(SolveStats), _DP], _I),
+    "nq_host_gauge_random_lx": ([_I, _I, _I, _I, C.c_uint64, _D, _DP], _I),
+    "nq_host_link_algebra": ([C.c_uint64, C.c_int64, _I, _D, _DP, _DP], _I),
+    "nq_host_gauge_phase_lx": ([_I, _I, _I, _I, _DP, _DP], _I),
+    "nq_host_gauge_stagphases_lx": ([_I, _I, _I, _I, _DP], _I),
+    "nq_host_source_lx": ([_I, _I, _I, _I, C.c_uint64, _I, _I, _I, C.c_int64, _I, _DP], _I),
+    "nq_host_split_lx": ([_I, _I, _I, _I, _I, _DP, _DP, _DP], _I),
+    "nq_host_paste_lx": ([_I, _I, _I, _I, _I, _DP, _DP, _DP], _I),
+    "nq_bench_tm_hop": ([_P, _P, _I, _I, _P, _P, _DP, _DP], _I),
+}
+
+
+def lib():
+    """Load libnissa_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} not built: run __graft_entry__.build() (no fallback path exists)")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in _SIG.items():
+            f = getattr(L, name, None)
+            if f is None:
+                continue
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    L = lib()
+    return {n: hasattr(L, n) for n in _SIG}
+
+
+def _check(rc):
+    if rc != NQ_OK:
+        raise NqError(rc, lib().nq_last_error().decode())
+    return rc
+
+
+def _dp(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_DP)
+
+
+def _anis(anis):
+    return None if anis is None else C.byref(C.c_double(anis))
+
+
+def device_count():
+    n = C.c_int(0)
+    rc = lib().nq_device_count(C.byref(n))
+    return n.value if rc == NQ_OK else 0
+
+
+# ----------------------------------------------------------------- host helpers
+def dims(L):
+    T, X, Y, Z = L
+    return int(T), int(X), int(Y), int(Z)
+
+
+def volume(L):
+    T, X, Y, Z = dims(L)
+    return T * X * Y * Z
+
+
+def gauge_random_lx(L, seed, eps=0.3):
+    V = volume(L)
+    conf = np.empty((V, 4, 3, 3, 2))
+    _check(lib().nq_host_gauge_random_lx(*dims(L), seed, eps, _dp(conf)))
+    return conf
+
+
+def link_algebra(seed, lx, mu, eps):
+    H = np.empty((3, 3, 2))
+    U = np.empty((3, 3, 2))
+    _check(lib().nq_host_link_algebra(seed, lx, mu, eps, _dp(H), _dp(U)))
+    return H, U
+
+
+def gauge_phase_lx(L, conf, theta_in_pi):
+    th = np.ascontiguousarray(theta_in_pi, dtype=np.float64)
+    _check(lib().nq_host_gauge_phase_lx(*dims(L), _dp(th), _dp(conf)))
+    return conf
+
+
+def gauge_stagphases_lx(L, conf):
+    _check(lib().nq_host_gauge_stagphases_lx(*dims(L), _dp(conf)))
+    return conf
+
+
+def source_lx(L, seed, kind="gauss", ncomp=12, parity_mask=3, site0=0, sc=0):
+    kinds = {"z2": 0, "gauss": 1, "point": 2}
+    V = volume(L)
+    out = np.empty((V, ncomp, 2)) if ncomp == 3 else np.empty((V, 4, 3, 2))
+    _check(lib().nq_host_source_lx(*dims(L), seed, kinds[kind], ncomp, parity_mask, site0, sc, _dp(out)))
+    return out
+
+
+def split_lx(L, lx):
+    V = volume(L)
+    nd = lx.size // V
+    e = np.empty((V // 2,) + lx.shape[1:])
+    o = np.empty_like(e)
+    _check(lib().nq_host_split_lx(*dims(L), nd, _dp(np.ascontiguousarray(lx)), _dp(e), _dp(o)))
+    return e, o
+
+
+def paste_lx(L, e, o):
+    V = volume(L)
+    lx = np.empty((V,) + e.shape[1:])
+    nd = lx.size // V
+    _check(lib().nq_host_paste_lx(*dims(L), nd, _dp(lx), _dp(np.ascontiguousarray(e)), _dp(np.ascontiguousarray(o))))
+    return lx
+
+
+# ----------------------------------------------------------------- device objects
+class Ctx:
+    def __init__(self, L, device=0):
+        self.L = dims(L)
+        self.V = volume(L)
+        self.Vh = self.V // 2
+        self.h = None
+        self._children = weakref.WeakSet()
+        h = _P()
+        _check(lib().nq_ctx_create(*self.L, device, C.byref(h)))
+        self.h = h
+
+    def _adopt(self, obj):
+        self._children.add(obj)
+
+    def close(self):
+        if self.h:
+            # device objects of this context must go before the context itself
+            for ch in list(self._children):
+                ch.close()
+            lib().nq_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        _check(lib().nq_ctx_sync(self.h))
+
+    def profiling(self, on=True):
+        _check(lib().nq_ctx_set_profiling(self.h, 1 if on else 0))
+
+    def reset_kernel_stats(self):
+        _check(lib().nq_ctx_reset_kernel_stats(self.h))
+
+    def kernel_stats(self):
+        n = C.c_int(0)
+        _check(lib().nq_ctx_kernel_count(self.h, C.byref(n)))
+        out = {}
+        buf = C.create_string_buffer(128)
+        for i in range(n.value):
+            la = C.c_int64(0)
+            ms = C.c_double(0)
+            by = C.c_double(0)
+            _check(lib().nq_ctx_kernel_stat(self.h, i, buf, 128, C.byref(la), C.byref(ms), C.byref(by)))
+            out[buf.value.decode()] = {"launches": la.value, "total_ms": ms.value, "bytes_per_launch": by.value}
+        return out
+
+    def gauge_lx(self, conf_lx):
+        return Gauge(self, lx=conf_lx)
+
+    def gauge_eo(self, evn, odd):
+        return Gauge(self, evn=evn, odd=odd)
+
+    def field(self, kind=SPINCOLOR, coverage=ODD, data=None):
+        f = Field(self, kind, coverage)
+        if data is not None:
+            f.upload(data)
+        return f
+
+
+class Gauge:
+    def __init__(self, ctx, lx=None, evn=None, odd=None):
+        self.ctx = ctx
+        self.h = None
+        h = _P()
+        if lx is not None:
+            _check(lib().nq_gauge_create_lx(ctx.h, _dp(np.ascontiguousarray(lx)), C.byref(h)))
+        else:
+            _check(lib().nq_gauge_create_eo(ctx.h, _dp(np.ascontiguousarray(evn)), _dp(np.ascontiguousarray(odd)),
+                                            C.byref(h)))
+        self.h = h
+        ctx._adopt(self)
+
+    def device_bytes(self):
+        return lib().nq_gauge_device_bytes(self.h)
+
+    def close(self):
+        if self.h:
+            lib().nq_gauge_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Field:
+    def __init__(self, ctx, kind, coverage):
+        self.ctx, self.kind, self.coverage = ctx, kind, coverage
+        self.h = None
+        h = _P()
+        _check(lib().nq_field_create(ctx.h, kind, coverage, C.byref(h)))
+        self.h = h
+        ctx._adopt(self)
+
+    @property
+    def shape(self):
+        n = self.ctx.V if self.coverage == LX else self.ctx.Vh
+        return (n, 4, 3, 2) if self.kind == SPINCOLOR else (n, 3, 2)
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.size == int(np.prod(self.shape)), (a.shape, self.shape)
+        _check(lib().nq_field_upload(self.h, _dp(a)))
+
+    def download(self):
+        a = np.empty(self.shape)
+        _check(lib().nq_field_download(self.h, _dp(a)))
+        return a
+
+    def norm2(self):
+        d = C.c_double(0)
+        _check(lib().nq_field_norm2(self.h, C.byref(d)))
+        return d.value
+
+    def close(self):
+        if self.h:
+            lib().nq_field_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ----------------------------------------------------------------- operators and solvers
+def tm_hop(ctx, g, out, inp, anis=None):
+    _check(lib().nq_tm_hop(ctx.h, g.h, _anis(anis), out.h, inp.h))
+
+
+def tm_Dee(ctx, kappa, mu, out, inp, inverse=False):
+    _check(lib().nq_tm_Dee(ctx.h, kappa, mu, 1 if inverse else 0, out.h, inp.h))
+
+
+def tm_kern(ctx, g, kappa, mu, out, inp, anis=None):
+    _check(lib().nq_tm_kern(ctx.h, g.h, kappa, _anis(anis), mu, out.h, inp.h))
+
+
+def tm_kern_square(ctx, g, kappa, mu, out, inp, anis=None):
+    _check(lib().nq_tm_kern_square(ctx.h, g.h, kappa, _anis(anis), mu, out.h, inp.h))
+
+
+def tm_apply_Q(ctx, g, kappa, mu, out, inp, anis=None):
+    _check(lib().nq_tm_apply_Q(ctx.h, g.h, kappa, _anis(anis), mu, out.h, inp.h))
+
+
+def tm_apply_Q2(ctx, g, kappa, mu, out, inp):
+    _check(lib().nq_tm_apply_Q2(ctx.h, g.h, kappa, mu, out.h, inp.h))
+
+
+def _solve(rc, st):
+    if rc not in (NQ_OK, NQ_ENOCONV):
+        _check(rc)
+    d = st.as_dict()
+    d["rc"] = rc
+    return d
+
+
+def tm_cg_kernsq(ctx, g, kappa, mu, niter, residue, sol, src, prec="f64", guess=None, anis=None):
+    st = SolveStats()
+    rc = lib().nq_tm_cg_kernsq(ctx.h, g.h, kappa, _anis(anis), mu, niter, residue, PREC[prec], sol.h,
+                               guess.h if guess is not None else None, src.h, C.byref(st))
+    return _solve(rc, st)
+
+
+def tm_solve_eoprec(ctx, g, kappa, mu, niter, residue, sol, src, prec="f64", anis=None):
+    st = SolveStats()
+    chk = C.c_double(0)
+    rc = lib().nq_tm_solve_eoprec(ctx.h, g.h, kappa, _anis(anis), mu, niter, residue, PREC[prec], sol.h, src.h,
+                                  C.byref(st), C.byref(chk))
+    d = _solve(rc, st)
+    d["check_rel_res_sq"] = chk.value
+    return d
+
+
+def tmQ2_cg(ctx, g, kappa, mu, niter, residue, sol, src, guess=None):
+    st = SolveStats()
+    rc = lib().nq_tmQ2_cg(ctx.h, g.h, kappa, mu, niter, residue, sol.h, guess.h if guess is not None else None,
+                          src.h, C.byref(st))
+    return _solve(rc, st)
+
+
+def tm_solve_eoprec_multi(ctx, g, kappa, mus, niter, residue, sols, srcs, prec="f16", anis=None):
+    n = len(srcs)
+    assert len(sols) == n and len(mus) == n
+    mu = (C.c_double * n)(*mus)
+    so = (_P * n)(*[s.h for s in sols])
+    sr = (_P * n)(*[s.h for s in srcs])
+    st = (SolveStats * n)()
+    rc = lib().nq_tm_solve_eoprec_multi(ctx.h, g.h, kappa, _anis(anis), n, mu, niter, residue, PREC[prec], so, sr, st)
+    if rc not in (NQ_OK, NQ_ENOCONV):
+        _check(rc)
+    return [dict(st[i].as_dict(), rc=rc) for i in range(n)]
+
+
+def st_2Doe(ctx, g, out, inp):
+    _check(lib().nq_st_2Doe(ctx.h, g.h, out.h, inp.h))
+
+
+def st_Deo_half(ctx, g, out, inp):
+    _check(lib().nq_st_Deo_half(ctx.h, g.h, out.h, inp.h))
+
+
+def st_DDee(ctx, g, mass2, out, inp):
+    _check(lib().nq_st_DDee(ctx.h, g.h, mass2, out.h, inp.h))
+
+
+def st_cg(ctx, g, mass2, niter, residue, sol, src, guess=None):
+    st = SolveStats()
+    rc = lib().nq_st_cg(ctx.h, g.h, mass2, niter, residue, sol.h, guess.h if guess is not None else None, src.h,
+                        C.byref(st))
+    return _solve(rc, st)
+
+
+def st_cgm(ctx, g, shifts, niter, residues, sols, srcs, prec="f64"):
+    """sols[r][k] for source r and shift k."""
+    nr, ns = len(srcs), len(shifts)
+    sh = (C.c_double * ns)(*shifts)
+    rs = (C.c_double * ns)(*residues)
+    so = (_P * (nr * ns))(*[sols[r][k].h for r in range(nr) for k in range(ns)])
+    sr = (_P * nr)(*[s.h for s in srcs])
+    st = (SolveStats * nr)()
+    tr = (C.c_double * (nr * ns))()
+    rc = lib().nq_st_cgm(ctx.h, g.h, nr, ns, sh, niter, rs, PREC[prec], so, sr, st, tr)
+    if rc not in (NQ_OK, NQ_ENOCONV):
+        _check(rc)
+    return [dict(st[r].as_dict(), rc=rc, true_res_sq=[tr[r * ns + k] for k in range(ns)]) for r in range(nr)]
+
+
+def bench_tm_hop(ctx, g, napply, out, inp, prec="f64"):
+    ms = C.c_double(0)
+    by = C.c_double(0)
+    _check(lib().nq_bench_tm_hop(ctx.h, g.h, PREC[prec], napply, out.h, inp.h, C.byref(ms), C.byref(by)))
+    return ms.value, by.value

@@ -1,0 +1,525 @@
+// C ABI: contexts, device memory, fields, gauge handles, profiling ledger.
+// Replaces nissa's field containers and device plumbing on the hot path
+// (src/base/field.hpp:659-1805, src/base/memory_manager.hpp:221-434,
+// src/threads/threads.hpp:86-265) with explicit device handles.
+
+#include <cmath>
+#include <complex>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "nq_internal.hpp"
+
+namespace nq {
+
+static thread_local char g_err[1024] = "";
+
+int fail(int code, const char* fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int dev_alloc(DevBuf& b, size_t bytes)
+{
+  if (b.p && b.bytes >= bytes) return NQ_OK;
+  dev_free(b);
+  if (bytes == 0) return NQ_OK;
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    (void)hipGetLastError();
+    return fail(NQ_EOOM, "hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+  }
+  b.bytes = bytes;
+  return NQ_OK;
+}
+
+void dev_free(DevBuf& b)
+{
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+int ctx_temp(nq_ctx* ctx, const std::string& role, size_t bytes, void** out)
+{
+  DevBuf& b = ctx->temps[role];
+  NQ_TRY(dev_alloc(b, bytes));
+  *out = b.p;
+  return NQ_OK;
+}
+
+int spinor_bytes(int prec, int ncomp, const Geo& g, size_t* data_bytes, size_t* norm_bytes)
+{
+  const size_t n = (size_t)ncomp * (size_t)g.stride;
+  switch (prec) {
+  case SP_F64: *data_bytes = n * 16; *norm_bytes = 0; break;
+  case SP_F32: *data_bytes = n * 8; *norm_bytes = 0; break;
+  case SP_F16: *data_bytes = n * 4; *norm_bytes = (size_t)g.stride * 4; break;
+  default: return fail(NQ_EINVAL, "bad spinor precision %d", prec);
+  }
+  return NQ_OK;
+}
+
+// temporary field of nr right-hand sides (rstride = ncomp*stride elements)
+int alloc_spinor_nr(nq_ctx* ctx, const std::string& role, int prec, int ncomp, int nr, SpinorView* v)
+{
+  size_t db = 0, nb = 0;
+  NQ_TRY(spinor_bytes(prec, ncomp, ctx->geo, &db, &nb));
+  void* p = nullptr;
+  NQ_TRY(ctx_temp(ctx, role, (db + nb) * nr + 256, &p));
+  v->data = p;
+  v->prec = prec;
+  v->ncomp = ncomp;
+  v->rstride = (int64_t)ncomp * ctx->geo.stride;
+  v->norm = nb ? (float*)((char*)p + db * nr) : nullptr;
+  return NQ_OK;
+}
+
+int alloc_spinor(nq_ctx* ctx, const std::string& role, int prec, int ncomp, SpinorView* v)
+{
+  return alloc_spinor_nr(ctx, role, prec, ncomp, 1, v);
+}
+
+SpinorView view_of(const nq_field* f, int parity)
+{
+  SpinorView v;
+  v.prec = SP_F64;
+  v.ncomp = f->ncomp();
+  v.rstride = 0;
+  const int slot = f->coverage == NQ_LX ? parity : 0;
+  v.data = (char*)f->buf.p + (size_t)slot * f->ncomp() * f->ctx->geo.stride * 16;
+  return v;
+}
+
+// lx field as one field of 2*ncomp components over Vh sites (site-local ops only)
+SpinorView view_lx_flat(const nq_field* f)
+{
+  SpinorView v = view_of(f, 0);
+  v.ncomp = 2 * f->ncomp();
+  return v;
+}
+
+int ensure_state(nq_ctx* ctx, int n)
+{
+  if (ctx->nstate >= n) return NQ_OK;
+  return fail(NQ_EINVAL, "solver state pool too small (%d < %d)", ctx->nstate, n);
+}
+
+// ---------------------------------------------------------------- profiling
+void prof_begin(nq_ctx* ctx, const char* name, double bytes, hipEvent_t* a)
+{
+  *a = nullptr;
+  if (!ctx->profiling) return;
+  hipEvent_t e0, e1;
+  if (ctx->event_pool.size() >= 2) {
+    e0 = ctx->event_pool.back();
+    ctx->event_pool.pop_back();
+    e1 = ctx->event_pool.back();
+    ctx->event_pool.pop_back();
+  } else {
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return;
+  }
+  (void)hipEventRecord(e0, ctx->stream);
+  ctx->pending.push_back({name, e0, e1, bytes});
+  *a = e1;
+}
+
+void prof_end(nq_ctx* ctx, hipEvent_t b)
+{
+  if (!b) return;
+  (void)hipEventRecord(b, ctx->stream);
+  if (ctx->pending.size() > 8192) prof_flush(ctx);
+}
+
+void prof_flush(nq_ctx* ctx)
+{
+  if (ctx->pending.empty()) return;
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& p : ctx->pending) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      KernelStat& s = ctx->stats[p.name];
+      s.launches++;
+      s.total_ms += ms;
+      s.bytes = p.bytes;
+    }
+    ctx->event_pool.push_back(p.a);
+    ctx->event_pool.push_back(p.b);
+  }
+  ctx->pending.clear();
+}
+
+}  // namespace nq
+
+using namespace nq;
+
+int64_t nq_gauge::bytes() const
+{
+  return (int64_t)(aos[0].bytes + aos[1].bytes + l64.bytes + l32.bytes + l32c.bytes + l16c.bytes);
+}
+
+namespace {
+struct Guard {
+  std::lock_guard<std::recursive_mutex> lk;
+  explicit Guard(nq_ctx* c) : lk(c->mu) { (void)hipSetDevice(c->device); }
+};
+bool dims_ok(int T, int LX, int LY, int LZ)
+{
+  const int L[4] = {T, LX, LY, LZ};
+  for (int i = 0; i < 4; i++)
+    if (L[i] < 2 || (L[i] & 1)) return false;
+  return (int64_t)T * LX * LY * LZ < (int64_t)1 << 31;
+}
+}  // namespace
+
+extern "C" {
+
+const char* nq_last_error(void) { return g_err; }
+
+int nq_device_count(int* n)
+{
+  if (!n) return fail(NQ_EINVAL, "null pointer");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) {
+    *n = 0;
+    (void)hipGetLastError();
+    return fail(NQ_EDEVICE, "hipGetDeviceCount: %s", hipGetErrorString(e));
+  }
+  *n = c;
+  return NQ_OK;
+}
+
+int nq_ctx_create(int T, int LX, int LY, int LZ, int device, nq_ctx** out)
+{
+  if (!out) return fail(NQ_EINVAL, "null output");
+  *out = nullptr;
+  if (!dims_ok(T, LX, LY, LZ)) return fail(NQ_EINVAL, "lattice extents must be even and >= 2 (got %d %d %d %d)", T, LX, LY, LZ);
+  int n = 0;
+  NQ_TRY(nq_device_count(&n));
+  if (device < 0 || device >= n) return fail(NQ_EDEVICE, "device %d not available (%d devices)", device, n);
+  NQ_HIP(hipSetDevice(device));
+  auto* c = new nq_ctx;
+  c->device = device;
+  Geo& g = c->geo;
+  g.T = T; g.LX = LX; g.LY = LY; g.LZ = LZ;
+  g.Vh = (int64_t)T * LX * LY * LZ / 2;
+  g.stride = (g.Vh + 63) / 64 * 64;
+  g.VS = (int64_t)LX * LY * LZ;
+  g.S2 = (int64_t)LY * LZ;
+  int rc = NQ_OK;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = fail(NQ_EDEVICE, "stream creation failed");
+  const int64_t nchunks = (g.Vh + 63) / 64;
+  c->max_partials = nchunks * 16;
+  c->nstate = 64;
+  if (rc == NQ_OK && hipMalloc(&c->partial, sizeof(double) * c->max_partials) != hipSuccess) rc = fail(NQ_EOOM, "partials");
+  if (rc == NQ_OK && hipMalloc(&c->counter, 256) != hipSuccess) rc = fail(NQ_EOOM, "counter");
+  if (rc == NQ_OK && hipMalloc(&c->state, sizeof(CgState) * c->nstate) != hipSuccess) rc = fail(NQ_EOOM, "state");
+  if (rc == NQ_OK && hipHostMalloc(&c->state_host, sizeof(CgState) * c->nstate, hipHostMallocDefault) != hipSuccess)
+    rc = fail(NQ_EOOM, "pinned state");
+  if (rc == NQ_OK && hipMemset(c->counter, 0, 256) != hipSuccess) rc = fail(NQ_EDEVICE, "memset");
+  if (rc == NQ_OK && hipMemset(c->state, 0, sizeof(CgState) * c->nstate) != hipSuccess) rc = fail(NQ_EDEVICE, "memset");
+  if (rc == NQ_OK && hipDeviceSynchronize() != hipSuccess) rc = fail(NQ_EDEVICE, "sync");
+  if (rc != NQ_OK) {
+    nq_ctx_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return NQ_OK;
+}
+
+void nq_ctx_destroy(nq_ctx* c)
+{
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+  for (auto e : c->event_pool) (void)hipEventDestroy(e);
+  for (auto& t : c->temps) dev_free(t.second);
+  if (c->partial) (void)hipFree(c->partial);
+  if (c->counter) (void)hipFree(c->counter);
+  if (c->state) (void)hipFree(c->state);
+  if (c->state_host) (void)hipHostFree(c->state_host);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int nq_ctx_sync(nq_ctx* c)
+{
+  if (!c) return fail(NQ_EINVAL, "null ctx");
+  Guard G(c);
+  NQ_HIP(hipStreamSynchronize(c->stream));
+  return NQ_OK;
+}
+
+int nq_ctx_set_profiling(nq_ctx* c, int on)
+{
+  if (!c) return fail(NQ_EINVAL, "null ctx");
+  Guard G(c);
+  if (!on) prof_flush(c);
+  c->profiling = on != 0;
+  return NQ_OK;
+}
+
+int nq_ctx_reset_kernel_stats(nq_ctx* c)
+{
+  if (!c) return fail(NQ_EINVAL, "null ctx");
+  Guard G(c);
+  prof_flush(c);
+  c->stats.clear();
+  return NQ_OK;
+}
+
+int nq_ctx_kernel_count(nq_ctx* c, int* n)
+{
+  if (!c || !n) return fail(NQ_EINVAL, "null argument");
+  Guard G(c);
+  prof_flush(c);
+  *n = (int)c->stats.size();
+  return NQ_OK;
+}
+
+int nq_ctx_kernel_stat(nq_ctx* c, int i, char* name, int name_len, int64_t* launches, double* total_ms,
+                       double* bytes_per_launch)
+{
+  if (!c) return fail(NQ_EINVAL, "null ctx");
+  Guard G(c);
+  prof_flush(c);
+  if (i < 0 || i >= (int)c->stats.size()) return fail(NQ_EINVAL, "kernel index out of range");
+  auto it = c->stats.begin();
+  std::advance(it, i);
+  if (name && name_len > 0) snprintf(name, name_len, "%s", it->first.c_str());
+  if (launches) *launches = it->second.launches;
+  if (total_ms) *total_ms = it->second.total_ms;
+  if (bytes_per_launch) *bytes_per_launch = it->second.bytes;
+  return NQ_OK;
+}
+
+// ------------------------------------------------------------------ fields
+int nq_field_create(nq_ctx* c, int kind, int coverage, nq_field** out)
+{
+  if (!c || !out) return fail(NQ_EINVAL, "null argument");
+  *out = nullptr;
+  if (kind != NQ_SPINCOLOR && kind != NQ_COLOR) return fail(NQ_EINVAL, "bad field kind %d", kind);
+  if (coverage < NQ_EVEN || coverage > NQ_LX) return fail(NQ_EINVAL, "bad coverage %d", coverage);
+  Guard G(c);
+  auto* f = new nq_field;
+  f->ctx = c;
+  f->kind = kind;
+  f->coverage = coverage;
+  const size_t per = (size_t)f->ncomp() * c->geo.stride * 16;
+  int rc = dev_alloc(f->buf, per * (coverage == NQ_LX ? 2 : 1));
+  if (rc == NQ_OK && hipMemsetAsync(f->buf.p, 0, f->buf.bytes, c->stream) != hipSuccess) rc = fail(NQ_EDEVICE, "memset");
+  if (rc == NQ_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(NQ_EDEVICE, "sync");
+  if (rc != NQ_OK) {
+    dev_free(f->buf);
+    delete f;
+    return rc;
+  }
+  *out = f;
+  return NQ_OK;
+}
+
+void nq_field_destroy(nq_field* f)
+{
+  if (!f) return;
+  {
+    Guard G(f->ctx);
+    (void)hipStreamSynchronize(f->ctx->stream);
+    dev_free(f->buf);
+  }
+  delete f;
+}
+
+int nq_field_upload(nq_field* f, const double* host)
+{
+  if (!f || !host) return fail(NQ_EINVAL, "null argument");
+  nq_ctx* c = f->ctx;
+  Guard G(c);
+  const int nd = 2 * f->ncomp();
+  const int64_t nsites = f->coverage == NQ_LX ? 2 * c->geo.Vh : c->geo.Vh;
+  void* stg = nullptr;
+  NQ_TRY(ctx_temp(c, "staging", sizeof(double) * nd * nsites, &stg));
+  NQ_HIP(hipMemcpyAsync(stg, host, sizeof(double) * nd * nsites, hipMemcpyHostToDevice, c->stream));
+  if (f->coverage == NQ_LX) {
+    NQ_TRY(launch_aos_to_soa(c, (const double*)stg, nsites, f->ncomp(), 0, view_of(f, 0)));
+    NQ_TRY(launch_aos_to_soa(c, (const double*)stg, nsites, f->ncomp(), 1, view_of(f, 1)));
+  } else {
+    NQ_TRY(launch_aos_to_soa(c, (const double*)stg, nsites, f->ncomp(), -1, view_of(f, 0)));
+  }
+  NQ_HIP(hipStreamSynchronize(c->stream));
+  return NQ_OK;
+}
+
+int nq_field_download(const nq_field* f, double* host)
+{
+  if (!f || !host) return fail(NQ_EINVAL, "null argument");
+  nq_ctx* c = f->ctx;
+  Guard G(c);
+  const int nd = 2 * f->ncomp();
+  const int64_t nsites = f->coverage == NQ_LX ? 2 * c->geo.Vh : c->geo.Vh;
+  void* stg = nullptr;
+  NQ_TRY(ctx_temp(c, "staging", sizeof(double) * nd * nsites, &stg));
+  if (f->coverage == NQ_LX) {
+    NQ_TRY(launch_soa_to_aos(c, view_of(f, 0), nsites, f->ncomp(), 0, (double*)stg));
+    NQ_TRY(launch_soa_to_aos(c, view_of(f, 1), nsites, f->ncomp(), 1, (double*)stg));
+  } else {
+    NQ_TRY(launch_soa_to_aos(c, view_of(f, 0), nsites, f->ncomp(), -1, (double*)stg));
+  }
+  NQ_HIP(hipMemcpyAsync(host, stg, sizeof(double) * nd * nsites, hipMemcpyDeviceToHost, c->stream));
+  NQ_HIP(hipStreamSynchronize(c->stream));
+  return NQ_OK;
+}
+
+int nq_field_copy(nq_field* dst, const nq_field* src)
+{
+  if (!dst || !src) return fail(NQ_EINVAL, "null argument");
+  if (dst->ctx != src->ctx || dst->kind != src->kind || dst->coverage != src->coverage)
+    return fail(NQ_EINVAL, "field copy: shape mismatch");
+  Guard G(dst->ctx);
+  NQ_HIP(hipMemcpyAsync(dst->buf.p, src->buf.p, src->buf.bytes, hipMemcpyDeviceToDevice, dst->ctx->stream));
+  return NQ_OK;
+}
+
+int nq_field_norm2(const nq_field* f, double* out)
+{
+  if (!f || !out) return fail(NQ_EINVAL, "null argument");
+  Guard G(f->ctx);
+  SpinorView v = f->coverage == NQ_LX ? view_lx_flat(f) : view_of(f, 0);
+  return reduce_norm2(f->ctx, v, out);
+}
+
+// ------------------------------------------------------------------ gauge
+static double det3(const double* U, double* im)
+{
+  std::complex<double> m[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) m[i][j] = {U[(i * 3 + j) * 2], U[(i * 3 + j) * 2 + 1]};
+  const std::complex<double> d = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) -
+                                 m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+                                 m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+  *im = d.imag();
+  return d.real();
+}
+
+int nq_gauge_create_eo(nq_ctx* c, const double* ce, const double* co, nq_gauge** out)
+{
+  if (!c || !ce || !co || !out) return fail(NQ_EINVAL, "null argument");
+  *out = nullptr;
+  Guard G(c);
+  const int64_t Vh = c->geo.Vh;
+  auto* g = new nq_gauge;
+  g->ctx = c;
+  // 12-real compression needs det(U_mu) constant per direction (boundary phases are)
+  bool comp = true;
+  double dref[4][2];
+  for (int mu = 0; mu < 4; mu++) dref[mu][0] = det3(ce + mu * 18, &dref[mu][1]);
+  for (int p = 0; p < 2 && comp; p++) {
+    const double* cf = p ? co : ce;
+    for (int64_t e = 0; e < Vh && comp; e++)
+      for (int mu = 0; mu < 4; mu++) {
+        double im, re = det3(cf + (e * 4 + mu) * 18, &im);
+        if (std::fabs(re - dref[mu][0]) > 1e-12 || std::fabs(im - dref[mu][1]) > 1e-12) { comp = false; break; }
+      }
+  }
+  g->compressible = comp;
+  std::memcpy(g->det, dref, sizeof dref);
+  int rc = dev_alloc(g->aos[0], sizeof(double) * 72 * Vh);
+  if (rc == NQ_OK) rc = dev_alloc(g->aos[1], sizeof(double) * 72 * Vh);
+  if (rc == NQ_OK && hipMemcpyAsync(g->aos[0].p, ce, sizeof(double) * 72 * Vh, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+    rc = fail(NQ_EDEVICE, "gauge upload");
+  if (rc == NQ_OK && hipMemcpyAsync(g->aos[1].p, co, sizeof(double) * 72 * Vh, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+    rc = fail(NQ_EDEVICE, "gauge upload");
+  if (rc == NQ_OK) rc = gauge_build(g, LK_F64_18);
+  if (rc == NQ_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(NQ_EDEVICE, "sync");
+  if (rc != NQ_OK) {
+    nq_gauge_destroy(g);
+    return rc;
+  }
+  *out = g;
+  return NQ_OK;
+}
+
+int nq_gauge_create_lx(nq_ctx* c, const double* lx, nq_gauge** out)
+{
+  if (!c || !lx || !out) return fail(NQ_EINVAL, "null argument");
+  const Geo& g = c->geo;
+  std::vector<double> e(72 * g.Vh), o(72 * g.Vh);
+  NQ_TRY(nq_host_split_lx(g.T, g.LX, g.LY, g.LZ, 72, lx, e.data(), o.data()));
+  return nq_gauge_create_eo(c, e.data(), o.data(), out);
+}
+
+void nq_gauge_destroy(nq_gauge* g)
+{
+  if (!g) return;
+  {
+    Guard G(g->ctx);
+    (void)hipStreamSynchronize(g->ctx->stream);
+    dev_free(g->aos[0]);
+    dev_free(g->aos[1]);
+    dev_free(g->l64);
+    dev_free(g->l32);
+    dev_free(g->l32c);
+    dev_free(g->l16c);
+  }
+  delete g;
+}
+
+int64_t nq_gauge_device_bytes(const nq_gauge* g) { return g ? g->bytes() : 0; }
+
+}  // extern "C"
+
+namespace nq {
+
+static DevBuf& link_buf(nq_gauge* g, int lk)
+{
+  switch (lk) {
+  case LK_F64_18: return g->l64;
+  case LK_F32_18: return g->l32;
+  case LK_F32_12: return g->l32c;
+  default: return g->l16c;
+  }
+}
+static size_t link_elem(int lk) { return lk == LK_F64_18 ? 16 : (lk == LK_F16_12 ? 4 : 8); }
+static int link_nc(int lk) { return (lk == LK_F64_18 || lk == LK_F32_18) ? 9 : 6; }
+
+int gauge_build(nq_gauge* g, int lk)
+{
+  DevBuf& b = link_buf(g, lk);
+  if (b.p) return NQ_OK;
+  if ((lk == LK_F32_12 || lk == LK_F16_12) && !g->compressible)
+    return fail(NQ_EINVAL, "12-real link compression needs det(U_mu) constant per direction");
+  nq_ctx* c = g->ctx;
+  const size_t bytes = 2 * 8 * (size_t)link_nc(lk) * c->geo.stride * link_elem(lk);
+  NQ_TRY(dev_alloc(b, bytes));
+  NQ_TRY(launch_build_links(c, (const double*)g->aos[0].p, (const double*)g->aos[1].p, lk, b.p, g->det));
+  return NQ_OK;
+}
+
+int link_view(const nq_gauge* gc, int lk, LinkView* v)
+{
+  nq_gauge* g = const_cast<nq_gauge*>(gc);
+  NQ_TRY(gauge_build(g, lk));
+  DevBuf& b = link_buf(g, lk);
+  const size_t per = 8 * (size_t)link_nc(lk) * g->ctx->geo.stride * link_elem(lk);
+  v->prec = lk;
+  v->data[0] = b.p;
+  v->data[1] = (const char*)b.p + per;
+  for (int p = 0; p < 2; p++)
+    for (int mu = 0; mu < 4; mu++) {
+      v->det[p][2 * mu][0] = g->det[mu][0];
+      v->det[p][2 * mu][1] = g->det[mu][1];
+      v->det[p][2 * mu + 1][0] = g->det[mu][0];
+      v->det[p][2 * mu + 1][1] = -g->det[mu][1];
+    }
+  return NQ_OK;
+}
+
+}  // namespace nq

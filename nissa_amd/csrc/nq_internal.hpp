@@ -1,0 +1,241 @@
+// Internal host-side declarations shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/nissa_amd.h"
+
+namespace nq {
+
+// ---------------------------------------------------------------- errors
+int fail(int code, const char* fmt, ...);
+#define NQ_HIP(call)                                                                      \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess) return ::nq::fail(NQ_EDEVICE, "%s failed: %s (%s:%d)", #call,     \
+                                            hipGetErrorString(e_), __FILE__, __LINE__);  \
+  } while (0)
+#define NQ_TRY(call)            \
+  do {                          \
+    int rc_ = (call);           \
+    if (rc_ != NQ_OK) return rc_; \
+  } while (0)
+
+// ---------------------------------------------------------------- storage kinds
+enum SpPrec { SP_F64 = 0, SP_F32 = 1, SP_F16 = 2 };
+enum LkPrec { LK_F64_18 = 0, LK_F32_18 = 1, LK_F32_12 = 2, LK_F16_12 = 3 };
+
+// Device geometry of one lattice (all extents even; eo index = lx>>1).
+struct Geo {
+  int T, LX, LY, LZ;
+  int64_t Vh;      // sites per parity
+  int64_t stride;  // per-component stride of SoA fields (Vh rounded up to 64)
+  int64_t VS;      // LX*LY*LZ
+  int64_t S2;      // LY*LZ
+};
+
+// One parity of a spinor (ncomp=12) or colour (ncomp=3) field in SoA layout:
+// component k of site e at data[k*stride + e] (complex element).  F16 keeps a
+// per-site float norm; the stored halves are value/norm.
+struct SpinorView {
+  void* data = nullptr;
+  float* norm = nullptr;   // F16 only: norm[r*stride + e]
+  int prec = SP_F64;
+  int ncomp = 12;
+  int64_t rstride = 0;     // elements between consecutive right-hand sides
+};
+
+// Gauge links as seen from output sites of one parity: 8 slots per site,
+// slot 2mu = U_mu(x), slot 2mu+1 = U_mu(x-mu)^+ (already daggered), each a 3x3
+// complex matrix stored as 9 (18-real) or 6 (12-real: rows 0,1) complex SoA
+// components: element (slot*nc + k)*stride + e.  For 12-real, row 2 is
+// det * conj(row0 x row1) with det a per-slot constant phase.
+struct LinkView {
+  const void* data[2] = {nullptr, nullptr};  // indexed by output parity
+  int prec = LK_F64_18;
+  double det[2][8][2] = {};                  // [out parity][slot][re,im]
+};
+
+// Solver state living in device memory; updated by the last block of the
+// reducing kernels (see blas / hop kernels).
+struct CgState {
+  double b2, target;
+  double rr, rr_new, pAp, alpha, beta;
+  double r0, delta;     // reliable-update reference |r| and threshold factor
+  double true_rr;       // last FP64 residual
+  double extra[6];
+  int iter, niter, done, reliable, nreliable, nan, mixed, flags;
+  unsigned int counter;
+  unsigned int pad[7];
+};
+
+enum Fin {
+  FIN_NONE = 0,
+  FIN_STORE0 = 1,     // extra[0] = total
+  FIN_PAP = 2,        // pAp = total; alpha = rr/pAp
+  FIN_RR = 3,         // plain CG: beta, rr, iter, done
+  FIN_RR_SLOPPY = 4,  // mixed CG: decide reliable update
+  FIN_RR_TRUE = 5,    // mixed CG: after FP64 recomputation
+  FIN_RR_INIT = 6,    // initial residual: rr = total, r0 = sqrt
+  FIN_STORE1 = 7,     // extra[1] = total
+};
+
+// Hop epilogue: out_h = c1_h*hop + c2_h*aux + c3_h*aux2 (h = upper/lower spins).
+// mode 0: write out; 1: write out (if write) and reduce |out|^2;
+// 2: CG tail (s=out, x+=alpha p, r-=alpha s, reduce |r|^2);
+// 3: residual tail (out written to r64 = out, sloppy copy to rs, reduce |out|^2).
+struct HopEpi {
+  int mode = 0;
+  int write = 1;
+  int fin = FIN_NONE;
+  int pred_reliable = 0;   // run only when state->reliable
+  int nrhs = 1;            // right-hand sides in the launch (views carry rstride)
+  int batched = 0;         // use the multi-RHS kernel build even for nrhs == 1
+  // c1, c2, c3 must stay contiguous: the kernel reads them as double[12]
+  double c1[2][2] = {{1, 0}, {1, 0}};
+  double c2[2][2] = {{0, 0}, {0, 0}};
+  double c3[2][2] = {{0, 0}, {0, 0}};
+  const double* coef = nullptr;  // device double[nrhs][12] overriding c1,c2,c3 per rhs
+  SpinorView aux, aux2;
+  SpinorView x, p, r, rs;
+  CgState* st = nullptr;
+};
+
+// ---------------------------------------------------------------- objects
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct KernelStat {
+  int64_t launches = 0;
+  double total_ms = 0;
+  double bytes = 0;  // algorithmic bytes per launch (last value)
+};
+
+}  // namespace nq
+
+struct nq_ctx {
+  int device = 0;
+  nq::Geo geo{};
+  hipStream_t stream = nullptr;
+  std::recursive_mutex mu;
+  // reduction scratch
+  double* partial = nullptr;  // per-(rhs, 64-site chunk) partials
+  int64_t max_partials = 0;
+  unsigned int* counter = nullptr;    // block-arrival counter of the reducing kernels
+  nq::CgState* state = nullptr;       // device, array of nstate
+  nq::CgState* state_host = nullptr;  // pinned mirror
+  int nstate = 0;
+  // profiling ledger
+  bool profiling = false;
+  std::map<std::string, nq::KernelStat> stats;
+  struct Pending { std::string name; hipEvent_t a, b; double bytes; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> event_pool;
+  // cached temporaries keyed by role
+  std::map<std::string, nq::DevBuf> temps;
+};
+
+struct nq_field {
+  nq_ctx* ctx = nullptr;
+  int kind = NQ_SPINCOLOR;  // NQ_SPINCOLOR / NQ_COLOR
+  int coverage = NQ_EVEN;   // NQ_EVEN / NQ_ODD / NQ_LX
+  nq::DevBuf buf;           // FP64 SoA; LX: even half then odd half
+  int ncomp() const { return kind == NQ_SPINCOLOR ? 12 : 3; }
+};
+
+struct nq_gauge {
+  nq_ctx* ctx = nullptr;
+  nq::DevBuf aos[2];                 // eo AoS FP64 copy (source for derived layouts)
+  nq::DevBuf l64;                    // LK_F64_18, both output parities
+  nq::DevBuf l32;                    // LK_F32_18
+  nq::DevBuf l32c;                   // LK_F32_12
+  nq::DevBuf l16c;                   // LK_F16_12
+  bool compressible = false;         // per-direction constant det
+  double det[4][2] = {};             // det(U_mu) when compressible
+  int64_t bytes() const;
+};
+
+namespace nq {
+
+// ---------------------------------------------------------------- memory
+int dev_alloc(DevBuf& b, size_t bytes);
+void dev_free(DevBuf& b);
+int ctx_temp(nq_ctx* ctx, const std::string& role, size_t bytes, void** out);
+int ensure_state(nq_ctx* ctx, int n);
+
+// views
+SpinorView view_of(const nq_field* f, int parity);
+SpinorView view_lx_flat(const nq_field* f);  // lx field as 2*ncomp components over Vh sites
+int spinor_bytes(int prec, int ncomp, const Geo& g, size_t* data_bytes, size_t* norm_bytes);
+int alloc_spinor(nq_ctx* ctx, const std::string& role, int prec, int ncomp, SpinorView* v);
+int alloc_spinor_nr(nq_ctx* ctx, const std::string& role, int prec, int ncomp, int nr, SpinorView* v);
+
+// gauge
+int gauge_build(nq_gauge* g, int lk);
+int link_view(const nq_gauge* g, int lk, LinkView* v);
+
+// ---------------------------------------------------------------- profiling
+void prof_begin(nq_ctx* ctx, const char* name, double bytes, hipEvent_t* a);
+void prof_end(nq_ctx* ctx, hipEvent_t a);
+void prof_flush(nq_ctx* ctx);
+
+// ---------------------------------------------------------------- launchers (device code)
+int64_t blocks_for(int64_t n);
+constexpr int kBlock = 256;
+// launch shape of the (64-site chunk x right-hand side) mapping
+struct Launch {
+  int W, NR;
+  int64_t nchunks, blocks;
+  int threads;
+};
+Launch launch_shape(const Geo& g, int nr);
+
+// twisted-mass hop of output parity out_par with epilogue
+int launch_tm_hop(nq_ctx* ctx, int out_par, const SpinorView& out, const SpinorView& in,
+                  const LinkView& U, double anis, const HopEpi& epi);
+double tm_hop_bytes(int sp, int lk, const HopEpi& epi, int ncomp = 12);
+
+// staggered hop: out(par) = sum_mu U in(x+mu) - U^+ in(x-mu); then
+// out = a*hop + b*aux (aux optional) ; mode 1 reduces |out|^2 with fin
+int launch_st_hop(nq_ctx* ctx, int out_par, const SpinorView& out, const SpinorView& in,
+                  const LinkView& U, const HopEpi& epi);
+
+// layout conversions
+int launch_aos_to_soa(nq_ctx* ctx, const double* aos, int64_t nsites_par, int ncomp, int lx_par,
+                      const SpinorView& out);
+int launch_soa_to_aos(nq_ctx* ctx, const SpinorView& in, int64_t nsites_par, int ncomp, int lx_par,
+                      double* aos);
+int launch_build_links(nq_ctx* ctx, const double* aos_e, const double* aos_o, int lk, void* out,
+                       const double det[4][2]);
+int launch_lx_split_gauge(nq_ctx* ctx, const double* lx, double* evn, double* odd);
+
+// BLAS-1 (all fused; operands FP64 or sloppy; nr right-hand sides per launch
+// sharing the views' rstride; st points at nr consecutive states).
+// pred: 0 always run, 1 skip right-hand sides that are done, 2 run only when reliable.
+int launch_axpby(nq_ctx* ctx, const SpinorView& y, const double a[2], const SpinorView& x,
+                 const double b[2], int fin, CgState* st, int pred, int nr = 1);
+int launch_lincomb(nq_ctx* ctx, const SpinorView& out, double a, const SpinorView& x, double b,
+                   const SpinorView& y, double c, const SpinorView& z, int fin, CgState* st,
+                   int nr = 1, int pred = 0);
+int launch_dot(nq_ctx* ctx, const SpinorView& a, const SpinorView& b, int fin, CgState* st,
+               int nr = 1, int pred = 0);
+int launch_cg_pupdate(nq_ctx* ctx, const SpinorView& p, const SpinorView& r, CgState* st, int nr = 1);
+int launch_reliable_accum(nq_ctx* ctx, const SpinorView& x64, const SpinorView& xs, CgState* st, int nr = 1);
+int launch_convert(nq_ctx* ctx, const SpinorView& out, const SpinorView& in, int nr = 1,
+                   CgState* st = nullptr, int pred = 0);
+int launch_zero(nq_ctx* ctx, const SpinorView& v, int nr = 1);
+// out_h = a_h x + b_h y on the upper/lower spin halves (y optional)
+int launch_spin_diag(nq_ctx* ctx, const SpinorView& out, const double a[2][2], const SpinorView& x,
+                     const double b[2][2], const SpinorView& y, int nr = 1);
+int reduce_norm2(nq_ctx* ctx, const SpinorView& v, double* out);
+int reduce_dot(nq_ctx* ctx, const SpinorView& a, const SpinorView& b, double* out);
+}  // namespace nq

@@ -1,0 +1,649 @@
+// Twisted-mass operators and solvers on the device.
+//
+// Reference path (src/dirac_operators/tmD_eoprec/dirac_operator_tmD_eoprec_portable.cpp,
+// src/inverters/twisted_mass/cg_invert_tmD_eoprec.cpp,
+// src/inverters/templates/cg_invert_template_threaded.hpp):
+//   K(mu)      = g5 (D_oo(mu) - 1/4 H_oe D_ee(mu)^-1 H_eo)           (2 hop launches)
+//   K K^+      = K(+mu) K(-mu)                                        (4 hop launches)
+//   CG on K K^+ with nissa's squared relative residual stop, then the eo
+//   reconstruction (eqs. 8-11 of doc/eo_inverter.lyx) and the FP64 check.
+// Every site-local factor (D_ee^-1, D_oo, g5, the 1/4 and 1/2) is folded into
+// the hop epilogues; the CG scalars live on the device; the host only polls
+// the convergence flag once per chunk of iterations.
+//
+// <p, K K^+ p> is computed as |K(-mu) p|^2 in the epilogue of the second hop
+// (K(+mu)^+ = K(-mu)), so the matvec needs no extra pass for the dot product;
+// the x/r updates and |r|^2 are fused into the last hop of the matvec.
+
+#include <cmath>
+#include <complex>
+#include <cstring>
+
+#include "nq_internal.hpp"
+
+namespace nq {
+
+using cplx = std::complex<double>;
+
+static void setc(double c[2][2], cplx up, cplx lo)
+{
+  c[0][0] = up.real(); c[0][1] = up.imag();
+  c[1][0] = lo.real(); c[1][1] = lo.imag();
+}
+
+// D_ee(mu)^-1 on the hop: upper (a - i mu)/n, lower (a + i mu)/n  (..._portable.hpp:205-221)
+static HopEpi epi_inv_dee(double kappa, double mu)
+{
+  HopEpi e;
+  const double a = 1 / (2 * kappa), n = a * a + mu * mu;
+  setc(e.c1, cplx(a / n, -mu / n), cplx(a / n, mu / n));
+  return e;
+}
+// g5 (D_oo(mu) in - hop/4)  (..._portable.cpp:21-32 with tmDee of :179-194)
+static HopEpi epi_kout(double kappa, double mu, const SpinorView& in)
+{
+  HopEpi e;
+  const double a = 1 / (2 * kappa);
+  setc(e.c1, cplx(-0.25, 0), cplx(0.25, 0));
+  setc(e.c2, cplx(a, mu), cplx(-a, mu));
+  e.aux = in;
+  return e;
+}
+// Q = g5 D on one parity: g5((1/2k + i mu g5) in - hop/2)  (dirac_operator_tmQ_portable.cpp:164-174)
+static HopEpi epi_q(double kappa, double mu, const SpinorView& in)
+{
+  HopEpi e;
+  const double a = 1 / (2 * kappa);
+  setc(e.c1, cplx(-0.5, 0), cplx(0.5, 0));
+  setc(e.c2, cplx(a, mu), cplx(-a, mu));
+  e.aux = in;
+  return e;
+}
+
+struct TmSetup {
+  nq_ctx* ctx;
+  LinkView U;
+  double anis;
+  double kappa;
+};
+
+static int tm_setup(nq_ctx* ctx, const nq_gauge* g, int lk, const double* anis, double kappa, TmSetup* s)
+{
+  if (!g || g->ctx != ctx) return fail(NQ_EINVAL, "gauge handle belongs to another context");
+  if (!(kappa != 0) || std::isnan(kappa)) return fail(NQ_EINVAL, "bad kappa");
+  s->ctx = ctx;
+  s->anis = anis ? *anis : 1.0;
+  s->kappa = kappa;
+  return link_view(g, lk, &s->U);
+}
+
+// K(mu): out_odd = K in_odd, using tmp_even
+static int apply_kern(const TmSetup& S, double mu, const SpinorView& out, const SpinorView& in, const SpinorView& tmp,
+                      CgState* st_skip = nullptr, int pred = 0)
+{
+  HopEpi a = epi_inv_dee(S.kappa, mu);
+  a.st = st_skip;
+  a.pred_reliable = pred;
+  NQ_TRY(launch_tm_hop(S.ctx, NQ_EVEN, tmp, in, S.U, S.anis, a));
+  HopEpi b = epi_kout(S.kappa, mu, in);
+  b.st = st_skip;
+  b.pred_reliable = pred;
+  return launch_tm_hop(S.ctx, NQ_ODD, out, tmp, S.U, S.anis, b);
+}
+
+// r = b - K(+mu)K(-mu) x with |r|^2 -> fin on st (mode 3: residual tail)
+static int residual_kernsq(const TmSetup& S, double mu, const SpinorView& r, const SpinorView& rs, const SpinorView& x,
+                           const SpinorView& b, const SpinorView& t1, const SpinorView& tmp, CgState* st, int fin,
+                           int pred)
+{
+  NQ_TRY(apply_kern(S, -mu, t1, x, tmp, st, pred));
+  HopEpi a = epi_inv_dee(S.kappa, mu);
+  a.st = st;
+  a.pred_reliable = pred;
+  NQ_TRY(launch_tm_hop(S.ctx, NQ_EVEN, tmp, t1, S.U, S.anis, a));
+  HopEpi e = epi_kout(S.kappa, mu, t1);
+  for (int h = 0; h < 2; h++)
+    for (int k = 0; k < 2; k++) { e.c1[h][k] = -e.c1[h][k]; e.c2[h][k] = -e.c2[h][k]; }
+  setc(e.c3, 1.0, 1.0);
+  e.aux2 = b;
+  e.mode = 3;
+  e.r = r;
+  e.rs = rs;
+  e.st = st;
+  e.fin = fin;
+  e.pred_reliable = pred;
+  SpinorView none;
+  return launch_tm_hop(S.ctx, NQ_ODD, none, tmp, S.U, S.anis, e);
+}
+
+static int lk_of_prec(int prec, int* sp, int* lk)
+{
+  switch (prec) {
+  case NQ_PREC_F64: *sp = SP_F64; *lk = LK_F64_18; return NQ_OK;
+  case NQ_PREC_F32: *sp = SP_F32; *lk = LK_F32_18; return NQ_OK;
+  case NQ_PREC_F16: *sp = SP_F16; *lk = LK_F32_12; return NQ_OK;
+  case NQ_PREC_F16H: *sp = SP_F16; *lk = LK_F16_12; return NQ_OK;
+  default: return fail(NQ_EINVAL, "unknown precision mode %d", prec);
+  }
+}
+
+static CgState* scratch(nq_ctx* ctx) { return ctx->state + (ctx->nstate - 1); }
+
+// Upload a host-initialised state to device slot i (async on the ctx stream).
+static int put_state(nq_ctx* ctx, int i, const CgState& s)
+{
+  ctx->state_host[i] = s;
+  NQ_HIP(hipMemcpyAsync(ctx->state + i, ctx->state_host + i, sizeof(CgState), hipMemcpyHostToDevice, ctx->stream));
+  return NQ_OK;
+}
+static int get_states(nq_ctx* ctx, int i0, int n)
+{
+  NQ_HIP(hipMemcpyAsync(ctx->state_host + i0, ctx->state + i0, sizeof(CgState) * n, hipMemcpyDeviceToHost, ctx->stream));
+  NQ_HIP(hipStreamSynchronize(ctx->stream));
+  return NQ_OK;
+}
+
+// iterations to enqueue before the next poll, from the observed convergence rate
+static int next_chunk(double rr0, double rr, double target, int done_iters, int last_chunk)
+{
+  if (done_iters <= 0 || !(rr > 0) || !(rr0 > rr)) return last_chunk;
+  const double rate = std::log(rr / rr0) / done_iters;  // log decrease per iteration (<0)
+  const double need = std::log(target / rr) / rate;
+  int n = (int)std::ceil(need) + 1;
+  if (n < 2) n = 2;
+  if (n > 64) n = 64;
+  return n;
+}
+
+struct Timer {
+  hipEvent_t a = nullptr, b = nullptr;
+  hipStream_t s;
+  explicit Timer(hipStream_t st) : s(st)
+  {
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    (void)hipEventRecord(a, s);
+  }
+  double stop()
+  {
+    (void)hipEventRecord(b, s);
+    (void)hipEventSynchronize(b);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, a, b);
+    return ms * 1e-3;
+  }
+  ~Timer()
+  {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+};
+
+// CG on K K^+ (FP64 throughout), cg_invert semantics (cg_invert_template_threaded.hpp:19-150)
+static int cg_kernsq_f64(const TmSetup& S, double mu, int niter, double residue, const SpinorView& x,
+                         const SpinorView* guess, const SpinorView& b, nq_solve_stats* out)
+{
+  nq_ctx* ctx = S.ctx;
+  SpinorView r, p, t1, tmp;
+  NQ_TRY(alloc_spinor(ctx, "cg.r", SP_F64, 12, &r));
+  NQ_TRY(alloc_spinor(ctx, "cg.p", SP_F64, 12, &p));
+  NQ_TRY(alloc_spinor(ctx, "cg.t1", SP_F64, 12, &t1));
+  NQ_TRY(alloc_spinor(ctx, "cg.tmp", SP_F64, 12, &tmp));
+  Timer T(ctx->stream);
+  double b2 = 0;
+  NQ_TRY(reduce_norm2(ctx, b, &b2));
+  if (b2 == 0 || std::isnan(b2)) return fail(NQ_ENAN, "invalid norm: %g", b2);
+  CgState s0 = {};
+  s0.b2 = b2;
+  s0.target = residue * b2;
+  s0.niter = niter;
+  NQ_TRY(put_state(ctx, 0, s0));
+  CgState* st = ctx->state;
+  SpinorView none;
+  none.ncomp = 12;
+  if (guess) {
+    NQ_TRY(launch_convert(ctx, x, *guess));
+    NQ_TRY(residual_kernsq(S, mu, r, none, x, b, t1, tmp, st, FIN_RR_INIT, 0));
+  } else {
+    NQ_TRY(launch_zero(ctx, x));
+    NQ_TRY(launch_lincomb(ctx, r, 1.0, b, 0, none, 0, none, FIN_RR_INIT, st));
+  }
+  NQ_TRY(launch_convert(ctx, p, r));
+  NQ_TRY(get_states(ctx, 0, 1));
+  const double rr0 = ctx->state_host[0].rr;
+  int chunk = 8, launched = 0;
+  while (!ctx->state_host[0].done) {
+    for (int i = 0; i < chunk; i++) {
+      HopEpi a = epi_inv_dee(S.kappa, -mu);
+      a.st = st;
+      NQ_TRY(launch_tm_hop(ctx, NQ_EVEN, tmp, p, S.U, S.anis, a));
+      HopEpi b1 = epi_kout(S.kappa, -mu, p);
+      b1.mode = 1;
+      b1.st = st;
+      b1.fin = FIN_PAP;
+      NQ_TRY(launch_tm_hop(ctx, NQ_ODD, t1, tmp, S.U, S.anis, b1));
+      HopEpi a2 = epi_inv_dee(S.kappa, mu);
+      a2.st = st;
+      NQ_TRY(launch_tm_hop(ctx, NQ_EVEN, tmp, t1, S.U, S.anis, a2));
+      HopEpi b2 = epi_kout(S.kappa, mu, t1);
+      b2.mode = 2;
+      b2.st = st;
+      b2.fin = FIN_RR;
+      b2.x = x;
+      b2.p = p;
+      b2.r = r;
+      NQ_TRY(launch_tm_hop(ctx, NQ_ODD, none, tmp, S.U, S.anis, b2));
+      NQ_TRY(launch_cg_pupdate(ctx, p, r, st));
+    }
+    launched += chunk;
+    NQ_TRY(get_states(ctx, 0, 1));
+    const CgState& h = ctx->state_host[0];
+    chunk = next_chunk(rr0, h.rr, h.target, h.iter, chunk);
+    if (launched > niter + 64) break;
+  }
+  const CgState h = ctx->state_host[0];
+  if (h.nan) return fail(NQ_ENAN, "nanned");
+  // last calculation of the residual (cg_invert_template_threaded.hpp:131-135)
+  CgState sc = {};
+  NQ_TRY(put_state(ctx, ctx->nstate - 1, sc));
+  NQ_TRY(residual_kernsq(S, mu, r, none, x, b, t1, tmp, scratch(ctx), FIN_STORE0, 0));
+  NQ_TRY(get_states(ctx, ctx->nstate - 1, 1));
+  const double true_rr = ctx->state_host[ctx->nstate - 1].extra[0];
+  out->iters = h.iter;
+  out->reliable_updates = 0;
+  out->rel_res_sq = true_rr / b2;
+  out->converged = h.iter < niter;
+  out->seconds = T.stop();
+  if (!out->converged) return fail(NQ_ENOCONV, "exit without converging (%d iterations)", h.iter);
+  return NQ_OK;
+}
+
+// Mixed precision CG with reliable updates: sloppy Krylov iteration (FP32 or FP16
+// spinors, FP32/FP16 links), FP64 solution accumulation and true-residual
+// recomputation whenever |r| drops by delta (or the sloppy residual reaches the
+// target); convergence is declared only on the FP64 residual.
+static int cg_kernsq_mixed(const TmSetup& S64, const TmSetup& Ss, int sp, double mu, int niter, double residue,
+                           double delta, const SpinorView& x64, const SpinorView* guess, const SpinorView& b,
+                           nq_solve_stats* out)
+{
+  nq_ctx* ctx = S64.ctx;
+  SpinorView r64, t64, tmp64, xs, rs, p, t1s, tmps;
+  NQ_TRY(alloc_spinor(ctx, "cg.r", SP_F64, 12, &r64));
+  NQ_TRY(alloc_spinor(ctx, "cg.t1", SP_F64, 12, &t64));
+  NQ_TRY(alloc_spinor(ctx, "cg.tmp", SP_F64, 12, &tmp64));
+  NQ_TRY(alloc_spinor(ctx, "mcg.xs", SP_F32, 12, &xs));
+  NQ_TRY(alloc_spinor(ctx, "mcg.rs", SP_F32, 12, &rs));
+  NQ_TRY(alloc_spinor(ctx, sp == SP_F16 ? "mcg.p16" : "mcg.p32", sp, 12, &p));
+  NQ_TRY(alloc_spinor(ctx, sp == SP_F16 ? "mcg.t16" : "mcg.t32", sp, 12, &t1s));
+  NQ_TRY(alloc_spinor(ctx, sp == SP_F16 ? "mcg.e16" : "mcg.e32", sp, 12, &tmps));
+  Timer T(ctx->stream);
+  double b2 = 0;
+  NQ_TRY(reduce_norm2(ctx, b, &b2));
+  if (b2 == 0 || std::isnan(b2)) return fail(NQ_ENAN, "invalid norm: %g", b2);
+  CgState s0 = {};
+  s0.b2 = b2;
+  s0.target = residue * b2;
+  s0.niter = niter;
+  s0.mixed = 1;
+  s0.delta = delta;
+  NQ_TRY(put_state(ctx, 0, s0));
+  CgState* st = ctx->state;
+  SpinorView none;
+  none.ncomp = 12;
+  if (guess) {
+    NQ_TRY(launch_convert(ctx, x64, *guess));
+    NQ_TRY(residual_kernsq(S64, mu, r64, none, x64, b, t64, tmp64, st, FIN_RR_INIT, 0));
+  } else {
+    NQ_TRY(launch_zero(ctx, x64));
+    NQ_TRY(launch_lincomb(ctx, r64, 1.0, b, 0, none, 0, none, FIN_RR_INIT, st));
+  }
+  NQ_TRY(launch_convert(ctx, rs, r64));
+  NQ_TRY(launch_convert(ctx, p, rs));
+  NQ_TRY(launch_zero(ctx, xs));
+  NQ_TRY(get_states(ctx, 0, 1));
+  const double rr0 = ctx->state_host[0].rr;
+  int chunk = 8, launched = 0;
+  while (!ctx->state_host[0].done) {
+    for (int i = 0; i < chunk; i++) {
+      HopEpi a = epi_inv_dee(Ss.kappa, -mu);
+      a.st = st;
+      NQ_TRY(launch_tm_hop(ctx, NQ_EVEN, tmps, p, Ss.U, Ss.anis, a));
+      HopEpi b1 = epi_kout(Ss.kappa, -mu, p);
+      b1.mode = 1;
+      b1.st = st;
+      b1.fin = FIN_PAP;
+      NQ_TRY(launch_tm_hop(ctx, NQ_ODD, t1s, tmps, Ss.U, Ss.anis, b1));
+      HopEpi a2 = epi_inv_dee(Ss.kappa, mu);
+      a2.st = st;
+      NQ_TRY(launch_tm_hop(ctx, NQ_EVEN, tmps, t1s, Ss.U, Ss.anis, a2));
+      HopEpi b2 = epi_kout(Ss.kappa, mu, t1s);
+      b2.mode = 2;
+      b2.st = st;
+      b2.fin = FIN_RR_SLOPPY;
+      b2.x = xs;
+      b2.p = p;
+      b2.r = rs;
+      NQ_TRY(launch_tm_hop(ctx, NQ_ODD, none, tmps, Ss.U, Ss.anis, b2));
+      // reliable update (predicated on the device flag)
+      NQ_TRY(launch_reliable_accum(ctx, x64, xs, st));
+      NQ_TRY(residual_kernsq(S64, mu, r64, rs, x64, b, t64, tmp64, st, FIN_RR_TRUE, 1));
+      NQ_TRY(launch_cg_pupdate(ctx, p, rs, st));
+    }
+    launched += chunk;
+    NQ_TRY(get_states(ctx, 0, 1));
+    const CgState& h = ctx->state_host[0];
+    chunk = next_chunk(rr0, h.rr, h.target, h.iter, chunk);
+    if (launched > niter + 64) break;
+  }
+  const CgState h = ctx->state_host[0];
+  if (h.nan) return fail(NQ_ENAN, "nanned");
+  out->iters = h.iter;
+  out->reliable_updates = h.nreliable;
+  out->rel_res_sq = h.true_rr / b2;
+  out->converged = h.true_rr < h.target;
+  out->seconds = T.stop();
+  if (!out->converged) return fail(NQ_ENOCONV, "exit without converging (%d iterations, rel res %g)", h.iter, h.true_rr / b2);
+  return NQ_OK;
+}
+
+static int cg_kernsq_any(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis, double mu, int niter,
+                         double residue, int prec, const SpinorView& x, const SpinorView* guess, const SpinorView& b,
+                         nq_solve_stats* st)
+{
+  int sp, lk;
+  NQ_TRY(lk_of_prec(prec, &sp, &lk));
+  if (niter <= 0) return fail(NQ_EINVAL, "niter must be positive");
+  if (!(residue > 0)) return fail(NQ_EINVAL, "residue must be positive");
+  TmSetup S64;
+  NQ_TRY(tm_setup(ctx, g, LK_F64_18, anis, kappa, &S64));
+  nq_solve_stats dummy;
+  if (!st) st = &dummy;
+  std::memset(st, 0, sizeof *st);
+  if (prec == NQ_PREC_F64) return cg_kernsq_f64(S64, mu, niter, residue, x, guess, b, st);
+  TmSetup Ss;
+  NQ_TRY(tm_setup(ctx, g, lk, anis, kappa, &Ss));
+  return cg_kernsq_mixed(S64, Ss, sp, mu, niter, residue, 1e-2, x, guess, b, st);
+}
+
+}  // namespace nq
+
+using namespace nq;
+
+namespace {
+struct Guard {
+  std::lock_guard<std::recursive_mutex> lk;
+  explicit Guard(nq_ctx* c) : lk(c->mu) { (void)hipSetDevice(c->device); }
+};
+int check_field(const nq_ctx* c, const nq_field* f, int kind, int cov, const char* what)
+{
+  if (!f) return fail(NQ_EINVAL, "%s: null field", what);
+  if (f->ctx != c) return fail(NQ_EINVAL, "%s: field belongs to another context", what);
+  if (f->kind != kind || f->coverage != cov) return fail(NQ_EINVAL, "%s: wrong field kind/coverage", what);
+  return NQ_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int nq_tm_hop(nq_ctx* c, const nq_gauge* g, const double* anis, nq_field* out, const nq_field* in)
+{
+  if (!c || !out || !in) return fail(NQ_EINVAL, "null argument");
+  if (in->coverage == NQ_LX || out->coverage == NQ_LX || in->coverage == out->coverage)
+    return fail(NQ_EINVAL, "calling with messed up parities");
+  NQ_TRY(check_field(c, in, NQ_SPINCOLOR, in->coverage, "nq_tm_hop"));
+  NQ_TRY(check_field(c, out, NQ_SPINCOLOR, out->coverage, "nq_tm_hop"));
+  Guard G(c);
+  TmSetup S;
+  NQ_TRY(tm_setup(c, g, LK_F64_18, anis, 1.0, &S));
+  HopEpi e;
+  NQ_TRY(launch_tm_hop(c, out->coverage, view_of(out, 0), view_of(in, 0), S.U, S.anis, e));
+  NQ_HIP(hipStreamSynchronize(c->stream));
+  return NQ_OK;
+}
+
+int nq_tm_Dee(nq_ctx* c, double kappa, double mu, int inverse, nq_field* out, const nq_field* in)
+{
+  if (!c || !out || !in) return fail(NQ_EINVAL, "null argument");
+  if (in->coverage == NQ_LX || out->coverage != in->coverage) return fail(NQ_EINVAL, "nq_tm_Dee: parity mismatch");
+  NQ_TRY(check_field(c, in, NQ_SPINCOLOR, in->coverage, "nq_tm_Dee"));
+  NQ_TRY(check_field(c, out, NQ_SPINCOLOR, in->coverage, "nq_tm_Dee"));
+  Guard G(c);
+  const double a = 1 / (2 * kappa), n = a * a + mu * mu;
+  double ca[2][2], cb[2][2] = {{0, 0}, {0, 0}};
+  if (inverse) setc(ca, cplx(a / n, -mu / n), cplx(a / n, mu / n));
+  else setc(ca, cplx(a, mu), cplx(a, -mu));
+  SpinorView none;
+  NQ_TRY(launch_spin_diag(c, view_of(out, 0), ca, view_of(in, 0), cb, none));
+  NQ_HIP(hipStreamSynchronize(c->stream));
+  return NQ_OK;
+}
+
+int nq_tm_kern(nq_ctx* c, const nq_gauge* g, double kappa, const double* anis, double mu, nq_field* out,
+               const nq_field* in)
+{
+  if (!c) return fail(NQ_EINVAL, "null ctx");
+  NQ_TRY(check_field(c, in, NQ_SPINCOLOR, NQ_ODD, "nq_tm_kern"));
+  NQ_TRY(check_field(c, out, NQ_SPINCOLOR, NQ_ODD, "nq_tm_kern"));
+  Guard G(c);
+  TmSetup S;
+  NQ_TRY(tm_setup(c, g, LK_F64_18, anis, kappa, &S));
+  SpinorView tmp;
+  NQ_TRY(alloc_spinor(c, "op.tmp", SP_F64, 12, &tmp));
+  NQ_TRY(apply_kern(S, mu, view_of(out, 0), view_of(in, 0), tmp));
+  NQ_HIP(hipStreamSynchronize(c->stream));
+  return NQ_OK;
+}
+
+int nq_tm_kern_square(nq_ctx* c, const nq_gauge* g, double kappa, const double* anis, double mu, nq_field* out,
+                      const nq_field* in)
+{
+  if (!c) return fail(NQ_EINVAL, "null ctx");
+  NQ_TRY(check_field(c, in, NQ_SPINCOLOR, NQ_ODD, "nq_tm_kern_square"));
+  NQ_TRY(check_field(c, out, NQ_SPINCOLOR, NQ_ODD, "nq_tm_kern_square"));
+  Guard G(c);
+  TmSetup S;
+  NQ_TRY(tm_setup(c, g, LK_F64_18, anis, kappa, &S));
+  SpinorView tmp, t1;
+  NQ_TRY(alloc_spinor(c, "op.tmp", SP_F64, 12, &tmp));
+  NQ_TRY(alloc_spinor(c, "op.t1", SP_F64, 12, &t1));
+  NQ_TRY(apply_kern(S, -mu, t1, view_of(in, 0), tmp));
+  NQ_TRY(apply_kern(S, +mu, view_of(out, 0), t1, tmp));
+  NQ_HIP(hipStreamSynchronize(c->stream));
+  return NQ_OK;
+}
+
+static int apply_Q_views(const TmSetup& S, double mu, const nq_field* out, const nq_field* in)
+{
+  for (int p = 0; p < 2; p++)
+    NQ_TRY(launch_tm_hop(S.ctx, p, view_of(out, p), view_of(in, !p), S.U, S.anis, epi_q(S.kappa, mu, view_of(in, p))));
+  return NQ_OK;
+}
+
+int nq_tm_apply_Q(nq_ctx* c, const nq_gauge* g, double kappa, const double* anis, double mu, nq_field* out,
+                  const nq_field* in)
+{
+  if (!c) return fail(NQ_EINVAL, "null ctx");
+  NQ_TRY(check_field(c, in, NQ_SPINCOLOR, NQ_LX, "nq_tm_apply_Q"));
+  NQ_TRY(check_field(c, out, NQ_SPINCOLOR, NQ_LX, "nq_tm_apply_Q"));
+  if (in == out) return fail(NQ_EINVAL, "out==in!");
+  Guard G(c);
+  TmSetup S;
+  NQ_TRY(tm_setup(c, g, LK_F64_18, anis, kappa, &S));
+  NQ_TRY(apply_Q_views(S, mu, out, in));
+  NQ_HIP(hipStreamSynchronize(c->stream));
+  return NQ_OK;
+}
+
+int nq_tm_apply_Q2(nq_ctx* c, const nq_gauge* g, double kappa, double mu, nq_field* out, const nq_field* in)
+{
+  if (!c) return fail(NQ_EINVAL, "null ctx");
+  NQ_TRY(check_field(c, in, NQ_SPINCOLOR, NQ_LX, "nq_tm_apply_Q2"));
+  NQ_TRY(check_field(c, out, NQ_SPINCOLOR, NQ_LX, "nq_tm_apply_Q2"));
+  if (in == out) return fail(NQ_EINVAL, "out==in!");
+  Guard G(c);
+  TmSetup S;
+  NQ_TRY(tm_setup(c, g, LK_F64_18, nullptr, kappa, &S));
+  nq_field tmp;
+  tmp.ctx = c;
+  tmp.kind = NQ_SPINCOLOR;
+  tmp.coverage = NQ_LX;
+  SpinorView tv;
+  NQ_TRY(alloc_spinor_nr(c, "op.q2tmp", SP_F64, 12, 2, &tv));
+  tmp.buf.p = tv.data;
+  NQ_TRY(apply_Q_views(S, +mu, &tmp, in));
+  NQ_TRY(apply_Q_views(S, -mu, out, &tmp));
+  tmp.buf.p = nullptr;
+  NQ_HIP(hipStreamSynchronize(c->stream));
+  return NQ_OK;
+}
+
+int nq_tm_cg_kernsq(nq_ctx* c, const nq_gauge* g, double kappa, const double* anis, double mu, int niter,
+                    double residue, int prec, nq_field* sol, const nq_field* guess, const nq_field* src,
+                    nq_solve_stats* st)
+{
+  if (!c) return fail(NQ_EINVAL, "null ctx");
+  NQ_TRY(check_field(c, src, NQ_SPINCOLOR, NQ_ODD, "nq_tm_cg_kernsq"));
+  NQ_TRY(check_field(c, sol, NQ_SPINCOLOR, NQ_ODD, "nq_tm_cg_kernsq"));
+  if (guess) NQ_TRY(check_field(c, guess, NQ_SPINCOLOR, NQ_ODD, "nq_tm_cg_kernsq"));
+  Guard G(c);
+  SpinorView gv;
+  if (guess) gv = view_of(guess, 0);
+  return cg_kernsq_any(c, g, kappa, anis, mu, niter, residue, prec, view_of(sol, 0), guess ? &gv : nullptr,
+                       view_of(src, 0), st);
+}
+
+// inv_tmD_cg_eoprec_native + check (cg_invert_tmD_eoprec.cpp:102-150,192-212)
+int nq_tm_solve_eoprec(nq_ctx* c, const nq_gauge* g, double kappa, const double* anis, double mu, int niter,
+                       double residue, int prec, nq_field* sol, const nq_field* src, nq_solve_stats* st,
+                       double* check)
+{
+  if (!c) return fail(NQ_EINVAL, "null ctx");
+  NQ_TRY(check_field(c, src, NQ_SPINCOLOR, NQ_LX, "nq_tm_solve_eoprec"));
+  NQ_TRY(check_field(c, sol, NQ_SPINCOLOR, NQ_LX, "nq_tm_solve_eoprec"));
+  if (sol == src) return fail(NQ_EINVAL, "sol==src");
+  Guard G(c);
+  TmSetup S;
+  NQ_TRY(tm_setup(c, g, LK_F64_18, anis, kappa, &S));
+  const double a = 1 / (2 * kappa), n = a * a + mu * mu;
+  const cplx zi_u(a / n, -mu / n), zi_l(a / n, mu / n);
+  SpinorView se = view_of(src, 0), so = view_of(src, 1), xe = view_of(sol, 0), xo = view_of(sol, 1);
+  SpinorView et, phi, y;
+  NQ_TRY(alloc_spinor(c, "eo.et", SP_F64, 12, &et));
+  NQ_TRY(alloc_spinor(c, "eo.phi", SP_F64, 12, &phi));
+  NQ_TRY(alloc_spinor(c, "eo.y", SP_F64, 12, &y));
+  SpinorView none;
+  none.ncomp = 12;
+  // eq. 8a: et = D_ee^-1 eta_e
+  double ca[2][2], cb[2][2] = {{0, 0}, {0, 0}};
+  setc(ca, zi_u, zi_l);
+  NQ_TRY(launch_spin_diag(c, et, ca, se, cb, none));
+  // eq. 8b: phi = g5 (eta_o + 1/2 H_oe et)
+  HopEpi e8;
+  setc(e8.c1, 0.5, -0.5);
+  setc(e8.c2, 1.0, -1.0);
+  e8.aux = so;
+  NQ_TRY(launch_tm_hop(c, NQ_ODD, phi, et, S.U, S.anis, e8));
+  // eq. 9
+  nq_solve_stats local;
+  if (!st) st = &local;
+  int rc = cg_kernsq_any(c, g, kappa, anis, mu, niter, residue, prec, y, nullptr, phi, st);
+  if (rc != NQ_OK && rc != NQ_ENOCONV) return rc;
+  // eq. 10: sol_o = K(-mu) y  (sol_e used as scratch)
+  NQ_TRY(apply_kern(S, -mu, xo, y, xe));
+  // eq. 11: sol_e = D_ee^-1 (eta_e + 1/2 H_eo sol_o)
+  HopEpi e11;
+  setc(e11.c1, 0.5 * zi_u, 0.5 * zi_l);
+  setc(e11.c2, zi_u, zi_l);
+  e11.aux = se;
+  NQ_TRY(launch_tm_hop(c, NQ_EVEN, xe, xo, S.U, S.anis, e11));
+  if (check) {
+    // |D sol - src|^2 / |src|^2, D = g5 Q, per parity: -1/2 hop + (a + i mu g5) sol - src
+    double n2[2] = {0, 0};
+    for (int p = 0; p < 2; p++) {
+      CgState sc = {};
+      NQ_TRY(put_state(c, c->nstate - 1, sc));
+      HopEpi ec;
+      setc(ec.c1, -0.5, -0.5);
+      setc(ec.c2, cplx(a, mu), cplx(a, -mu));
+      setc(ec.c3, -1.0, -1.0);
+      ec.aux = view_of(sol, p);
+      ec.aux2 = view_of(src, p);
+      ec.mode = 1;
+      ec.write = 0;
+      ec.st = scratch(c);
+      ec.fin = FIN_STORE0;
+      NQ_TRY(launch_tm_hop(c, p, none, view_of(sol, !p), S.U, S.anis, ec));
+      NQ_TRY(get_states(c, c->nstate - 1, 1));
+      n2[p] = c->state_host[c->nstate - 1].extra[0];
+    }
+    double s2 = 0;
+    NQ_TRY(reduce_norm2(c, view_lx_flat(src), &s2));
+    *check = (n2[0] + n2[1]) / s2;
+  }
+  NQ_HIP(hipStreamSynchronize(c->stream));
+  return rc == NQ_ENOCONV ? fail(NQ_ENOCONV, "exit without converging") : NQ_OK;
+}
+
+int nq_tm_kern_square_host(nq_ctx* c, const nq_gauge* g, double kappa, const double* anis, double mu, double* out,
+                           const double* in)
+{
+  if (!c || !out || !in) return fail(NQ_EINVAL, "null argument");
+  Guard G(c);
+  nq_field *fi = nullptr, *fo = nullptr;
+  NQ_TRY(nq_field_create(c, NQ_SPINCOLOR, NQ_ODD, &fi));
+  int rc = nq_field_create(c, NQ_SPINCOLOR, NQ_ODD, &fo);
+  if (rc == NQ_OK) rc = nq_field_upload(fi, in);
+  if (rc == NQ_OK) rc = nq_tm_kern_square(c, g, kappa, anis, mu, fo, fi);
+  if (rc == NQ_OK) rc = nq_field_download(fo, out);
+  nq_field_destroy(fi);
+  nq_field_destroy(fo);
+  return rc;
+}
+
+int nq_tm_solve_eoprec_host(nq_ctx* c, const nq_gauge* g, double kappa, const double* anis, double mu, int niter,
+                            double residue, int prec, double* sol, const double* src, nq_solve_stats* st,
+                            double* check)
+{
+  if (!c || !sol || !src) return fail(NQ_EINVAL, "null argument");
+  Guard G(c);
+  nq_field *fs = nullptr, *fx = nullptr;
+  NQ_TRY(nq_field_create(c, NQ_SPINCOLOR, NQ_LX, &fs));
+  int rc = nq_field_create(c, NQ_SPINCOLOR, NQ_LX, &fx);
+  if (rc == NQ_OK) rc = nq_field_upload(fs, src);
+  int rs = rc == NQ_OK ? nq_tm_solve_eoprec(c, g, kappa, anis, mu, niter, residue, prec, fx, fs, st, check) : rc;
+  if (rs == NQ_OK || rs == NQ_ENOCONV) rc = nq_field_download(fx, sol);
+  nq_field_destroy(fs);
+  nq_field_destroy(fx);
+  return rc == NQ_OK ? rs : rc;
+}
+
+int nq_bench_tm_hop(nq_ctx* c, const nq_gauge* g, int prec, int napply, nq_field* out, const nq_field* in,
+                    double* ms, double* bytes)
+{
+  if (!c || napply <= 0) return fail(NQ_EINVAL, "bad argument");
+  NQ_TRY(check_field(c, in, NQ_SPINCOLOR, NQ_ODD, "nq_bench_tm_hop"));
+  NQ_TRY(check_field(c, out, NQ_SPINCOLOR, NQ_EVEN, "nq_bench_tm_hop"));
+  Guard G(c);
+  int sp, lk;
+  NQ_TRY(lk_of_prec(prec, &sp, &lk));
+  TmSetup S;
+  NQ_TRY(tm_setup(c, g, lk, nullptr, 1.0, &S));
+  SpinorView vi = view_of(in, 0), vo = view_of(out, 0);
+  if (sp != SP_F64) {
+    NQ_TRY(alloc_spinor(c, "bench.in", sp, 12, &vi));
+    NQ_TRY(alloc_spinor(c, "bench.out", sp, 12, &vo));
+    NQ_TRY(launch_convert(c, vi, view_of(in, 0)));
+  }
+  HopEpi e;
+  NQ_TRY(launch_tm_hop(c, NQ_EVEN, vo, vi, S.U, S.anis, e));  // warm-up
+  Timer T(c->stream);
+  for (int i = 0; i < napply; i++) NQ_TRY(launch_tm_hop(c, NQ_EVEN, vo, vi, S.U, S.anis, e));
+  const double sec = T.stop();
+  if (ms) *ms = sec * 1e3 / napply;
+  if (bytes) *bytes = tm_hop_bytes(sp, lk, e) * (double)c->geo.Vh;
+  if (sp != SP_F64) NQ_TRY(launch_convert(c, view_of(out, 0), vo));
+  NQ_HIP(hipStreamSynchronize(c->stream));
+  return NQ_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,307 @@
+// Twisted-mass Wilson hopping term of one parity with fused epilogues, for gfx950.
+//
+// Reference: tmn2Deo_or_tmn2Doe_eos (src/dirac_operators/tmD_eoprec/
+// dirac_operator_tmD_eoprec_portable.hpp:36-169):
+//   out(x) = sum_mu (1 - g_mu) U_mu(x) in(x+mu) + (1 + g_mu) U_mu(x-mu)^+ in(x-mu)
+// in nissa's gamma basis (src/new_types/dirac.cpp:43-58).  The spin projections
+// and the way the two projected colour vectors are scattered back onto the four
+// spins are fixed by that basis, so the per-direction table below necessarily
+// matches the reference's (same algebra, written for our accumulators).
+//
+// One thread per output site; links are streamed from the per-output-parity
+// slot arrays (backward links pre-daggered at upload), neighbour spinors are
+// gathered from the opposite-parity SoA field (L2/MALL-resident).  The epilogue
+// applies the site-local pieces of the eo operator in registers:
+//   out_h = c1_h hop + c2_h aux + c3_h aux2    (h = upper/lower spin pair)
+// which covers tmn2D, inv_tmDee(hop), g5(Doo in - hop/4), g5(eta + hop/2),
+// Dee^-1(eta + hop/2), and Q = g5 D on each parity; mode 2 fuses the CG update
+// x += alpha p, r -= alpha s and |r|^2; mode 3 writes a residual b - A x in FP64
+// plus its sloppy copy and |r|^2.
+
+#include "nq_device.hpp"
+
+namespace nq {
+
+template <class R>
+struct Half {
+  cx<R> h0[3], h1[3];
+};
+
+// spin projection of the neighbour spinor for slot (mu, fwd/bwd)
+template <class R, int MU, bool FWD>
+__device__ __forceinline__ void project(const cx<R> s[12], cx<R> h0[3], cx<R> h1[3])
+{
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const cx<R> s0 = s[c], s1 = s[3 + c], s2 = s[6 + c], s3 = s[9 + c];
+    if constexpr (FWD) {  // (1 - gamma_mu)
+      if constexpr (MU == 0) { h0[c] = s0 + s2; h1[c] = s1 + s3; }
+      if constexpr (MU == 1) { h0[c] = iadd(s0, s3); h1[c] = iadd(s1, s2); }
+      if constexpr (MU == 2) { h0[c] = s0 + s3; h1[c] = s1 - s2; }
+      if constexpr (MU == 3) { h0[c] = iadd(s0, s2); h1[c] = isub(s1, s3); }
+    } else {  // (1 + gamma_mu)
+      if constexpr (MU == 0) { h0[c] = s0 - s2; h1[c] = s1 - s3; }
+      if constexpr (MU == 1) { h0[c] = isub(s0, s3); h1[c] = isub(s1, s2); }
+      if constexpr (MU == 2) { h0[c] = s0 - s3; h1[c] = s1 + s2; }
+      if constexpr (MU == 3) { h0[c] = isub(s0, s2); h1[c] = iadd(s1, s3); }
+    }
+  }
+}
+
+// reconstruct the four spins from the two multiplied half spinors
+template <class R, int MU, bool FWD>
+__device__ __forceinline__ void scatter(cx<R> o[12], const cx<R> g0[3], const cx<R> g1[3])
+{
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    o[c] = o[c] + g0[c];
+    o[3 + c] = o[3 + c] + g1[c];
+    if constexpr (FWD) {
+      if constexpr (MU == 0) { o[6 + c] = o[6 + c] + g0[c]; o[9 + c] = o[9 + c] + g1[c]; }
+      if constexpr (MU == 1) { o[6 + c] = isub(o[6 + c], g1[c]); o[9 + c] = isub(o[9 + c], g0[c]); }
+      if constexpr (MU == 2) { o[6 + c] = o[6 + c] - g1[c]; o[9 + c] = o[9 + c] + g0[c]; }
+      if constexpr (MU == 3) { o[6 + c] = isub(o[6 + c], g0[c]); o[9 + c] = iadd(o[9 + c], g1[c]); }
+    } else {
+      if constexpr (MU == 0) { o[6 + c] = o[6 + c] - g0[c]; o[9 + c] = o[9 + c] - g1[c]; }
+      if constexpr (MU == 1) { o[6 + c] = iadd(o[6 + c], g1[c]); o[9 + c] = iadd(o[9 + c], g0[c]); }
+      if constexpr (MU == 2) { o[6 + c] = o[6 + c] + g1[c]; o[9 + c] = o[9 + c] - g0[c]; }
+      if constexpr (MU == 3) { o[6 + c] = iadd(o[6 + c], g0[c]); o[9 + c] = isub(o[9 + c], g1[c]); }
+    }
+  }
+}
+
+template <int SP, int LK, class R, int MU, bool FWD>
+__device__ __forceinline__ void hop_slot(const Geo& g, const Site& s, uint32_t e, const SpinorView& in, int r,
+                                         const void* ub, const double* det, R anis, bool use_anis,
+                                         cx<R> o[12])
+{
+  const uint32_t n = neigh_eo(g, s, MU, FWD);
+  cx<R> sp[12];
+  SpAcc<SP, R, 12>::load(in, g.stride, n, sp, r);
+  cx<R> h0[3], h1[3];
+  project<R, MU, FWD>(sp, h0, h1);
+  cx<R> u[9];
+  constexpr int slot = 2 * MU + (FWD ? 0 : 1);
+  LkAcc<LK, R>::load(ub, g.stride, slot, e, det + 2 * slot, u);
+  cx<R> g0[3], g1[3];
+  mat_vec(u, h0, g0);
+  mat_vec(u, h1, g1);
+  if (MU > 0 && use_anis) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) { g0[c] = scal(g0[c], anis); g1[c] = scal(g1[c], anis); }
+  }
+  scatter<R, MU, FWD>(o, g0, g1);
+  // Materialise the accumulators before the next slot's loads are issued: without
+  // this the compiler sinks the arithmetic of all slots below all their loads and
+  // keeps ~400 values live (FP64 spills to scratch).
+#pragma unroll
+  for (int k = 0; k < 12; k++) asm volatile("" : "+v"(o[k].re), "+v"(o[k].im));
+  asm volatile("" ::: "memory");
+}
+
+template <int SP>
+using RealOf = typename std::conditional<SP == SP_F64, double, float>::type;
+
+template <int SP, int LK, int MODE, int XP, int MT>
+__global__ void __launch_bounds__(MT, (MT == 256 ? 2 : 1)) k_tm_hop(Geo g, int op, SpinorView out, SpinorView in, LinkView U, double anis_d,
+                                               int use_anis, HopEpi epi, int W, int NR, int64_t nchunks,
+                                               double* partial, unsigned int* counter)
+{
+  using R = RealOf<SP>;
+  using RX = RealOf<XP>;
+  const Map m = map_thread(W, NR, g.Vh);
+  CgState* st = epi.st ? epi.st + m.r : nullptr;
+  bool skip = false;
+  if (st) skip = st->done || (epi.pred_reliable && !st->reliable);
+  // a launch whose every right-hand side is idle does nothing (uniform over the grid)
+  if (epi.st && NR == 1 && skip) return;
+  double nrm = 0;
+  if (m.active && !skip) {
+    const uint32_t e = m.e;
+    const int r = m.r;
+    const Site s = site_of(g, op, e);
+    const void* ub = U.data[op];
+    const double* det = &U.det[op][0][0];
+    const R anis = (R)anis_d;
+    const bool ua = use_anis != 0;
+    cx<R> o[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) o[k] = mk<R>(0, 0);
+    hop_slot<SP, LK, R, 0, true>(g, s, e, in, r, ub, det, anis, ua, o);
+    hop_slot<SP, LK, R, 0, false>(g, s, e, in, r, ub, det, anis, ua, o);
+    hop_slot<SP, LK, R, 1, true>(g, s, e, in, r, ub, det, anis, ua, o);
+    hop_slot<SP, LK, R, 1, false>(g, s, e, in, r, ub, det, anis, ua, o);
+    hop_slot<SP, LK, R, 2, true>(g, s, e, in, r, ub, det, anis, ua, o);
+    hop_slot<SP, LK, R, 2, false>(g, s, e, in, r, ub, det, anis, ua, o);
+    hop_slot<SP, LK, R, 3, true>(g, s, e, in, r, ub, det, anis, ua, o);
+    hop_slot<SP, LK, R, 3, false>(g, s, e, in, r, ub, det, anis, ua, o);
+
+    // epilogue: out_h = c1 hop + c2 aux + c3 aux2 ; per-rhs coefficients when tabulated
+    const double* cf = epi.coef ? epi.coef + 12 * r : &epi.c1[0][0];
+    const cx<R> c1u = mk<R>((R)cf[0], (R)cf[1]), c1l = mk<R>((R)cf[2], (R)cf[3]);
+#pragma unroll
+    for (int k = 0; k < 12; k++) o[k] = (k < 6 ? c1u : c1l) * o[k];
+    if (epi.aux.data) {
+      cx<R> a[12];
+      SpAcc<SP, R, 12>::load(epi.aux, g.stride, e, a, r);
+      const cx<R> cu = mk<R>((R)cf[4], (R)cf[5]), cl = mk<R>((R)cf[6], (R)cf[7]);
+#pragma unroll
+      for (int k = 0; k < 12; k++) fma_to(o[k], k < 6 ? cu : cl, a[k]);
+    }
+    if (epi.aux2.data) {
+      cx<R> a[12];
+      SpAcc<SP, R, 12>::load(epi.aux2, g.stride, e, a, r);
+      const cx<R> cu = mk<R>((R)cf[8], (R)cf[9]), cl = mk<R>((R)cf[10], (R)cf[11]);
+#pragma unroll
+      for (int k = 0; k < 12; k++) fma_to(o[k], k < 6 ? cu : cl, a[k]);
+    }
+    if constexpr (MODE == 0) {
+      SpAcc<SP, R, 12>::store(out, g.stride, e, o, r);
+    } else if constexpr (MODE == 1) {
+      if (epi.write) SpAcc<SP, R, 12>::store(out, g.stride, e, o, r);
+#pragma unroll
+      for (int k = 0; k < 12; k++) nrm += (double)o[k].re * (double)o[k].re + (double)o[k].im * (double)o[k].im;
+    } else if constexpr (MODE == 2) {
+      // CG tail: s = o; x += alpha p ; r -= alpha s ; |r|^2
+      const RX alpha = (RX)st->alpha;
+      cx<RX> x[12], rv[12];
+      cx<R> p[12];
+      SpAcc<XP, RX, 12>::load(epi.x, g.stride, e, x, r);
+      SpAcc<SP, R, 12>::load(epi.p, g.stride, e, p, r);
+      SpAcc<XP, RX, 12>::load(epi.r, g.stride, e, rv, r);
+#pragma unroll
+      for (int k = 0; k < 12; k++) {
+        x[k].re = fma(alpha, (RX)p[k].re, x[k].re);
+        x[k].im = fma(alpha, (RX)p[k].im, x[k].im);
+        rv[k].re = fma(-alpha, (RX)o[k].re, rv[k].re);
+        rv[k].im = fma(-alpha, (RX)o[k].im, rv[k].im);
+        nrm += (double)rv[k].re * (double)rv[k].re + (double)rv[k].im * (double)rv[k].im;
+      }
+      SpAcc<XP, RX, 12>::store(epi.x, g.stride, e, x, r);
+      SpAcc<XP, RX, 12>::store(epi.r, g.stride, e, rv, r);
+    } else {
+      // residual tail: r64 = o ; rs = sloppy(o) ; |o|^2
+      SpAcc<SP, R, 12>::store(epi.r, g.stride, e, o, r);
+      if (epi.rs.data) {
+        cx<RX> q[12];
+#pragma unroll
+        for (int k = 0; k < 12; k++) q[k] = mk<RX>((RX)o[k].re, (RX)o[k].im);
+        SpAcc<XP, RX, 12>::store(epi.rs, g.stride, e, q, r);
+      }
+#pragma unroll
+      for (int k = 0; k < 12; k++) nrm += (double)o[k].re * (double)o[k].re + (double)o[k].im * (double)o[k].im;
+    }
+  }
+  if constexpr (MODE != 0) grid_reduce_rhs(nrm, partial, nchunks, m.r, m.chunk, NR, epi.st, counter, epi.fin);
+}
+
+static double sp_site_bytes(int sp, int ncomp)
+{
+  switch (sp) {
+  case SP_F64: return 16.0 * ncomp;
+  case SP_F32: return 8.0 * ncomp;
+  default: return 4.0 * ncomp + 4.0;
+  }
+}
+static double lk_bytes(int lk)
+{
+  switch (lk) {
+  case LK_F64_18: return 144;
+  case LK_F32_18: return 72;
+  case LK_F32_12: return 48;
+  default: return 24;
+  }
+}
+
+static int xp_of(const HopEpi& epi, int sp)
+{
+  if (epi.mode == 2) return epi.x.prec;
+  if (epi.mode == 3) return epi.rs.data ? epi.rs.prec : sp;
+  return sp;
+}
+
+double tm_hop_bytes(int sp, int lk, const HopEpi& epi, int ncomp)
+{
+  const double s = sp_site_bytes(sp, ncomp);
+  double b = 8 * lk_bytes(lk) + s;  // links + input spinor (each site read once)
+  if (epi.aux.data) b += s;
+  if (epi.aux2.data) b += s;
+  if (epi.mode == 0 || (epi.mode == 1 && epi.write)) b += s;
+  if (epi.mode == 2) {
+    const double sx = sp_site_bytes(xp_of(epi, sp), ncomp);
+    b += 4 * sx + s;  // x,r read+write, p read
+  }
+  if (epi.mode == 3) {
+    b += s;
+    if (epi.rs.data) b += sp_site_bytes(epi.rs.prec, ncomp);
+  }
+  return b;
+}
+
+int64_t blocks_for(int64_t n) { return (n + kBlock - 1) / kBlock; }
+
+template <int SP, int LK, int MODE, int XP>
+static int go(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, const LinkView& U, double anis,
+              const HopEpi& epi, const char* name)
+{
+  const Geo& g = ctx->geo;
+  const int nr = epi.nrhs;
+  if (nr < 1 || nr > 16) return fail(NQ_EINVAL, "tm hop: right-hand sides per launch must be 1..16");
+  const Launch L = launch_shape(g, nr);
+  if (L.nchunks * nr > ctx->max_partials) return fail(NQ_EINVAL, "tm hop: reduction scratch too small");
+  hipEvent_t ev = nullptr;
+  prof_begin(ctx, name, tm_hop_bytes(SP, LK, epi) * (double)g.Vh, &ev);
+  const bool multi = nr > 1 || epi.batched;
+  if (multi)
+    hipLaunchKernelGGL((k_tm_hop<SP, LK, MODE, XP, 1024>), dim3((unsigned)L.blocks), dim3(L.threads), 0, ctx->stream, g,
+                       op, out, in, U, anis, anis != 1.0 ? 1 : 0, epi, L.W, nr, L.nchunks, ctx->partial, ctx->counter);
+  else
+    hipLaunchKernelGGL((k_tm_hop<SP, LK, MODE, XP, 256>), dim3((unsigned)L.blocks), dim3(L.threads), 0, ctx->stream, g,
+                       op, out, in, U, anis, anis != 1.0 ? 1 : 0, epi, L.W, nr, L.nchunks, ctx->partial, ctx->counter);
+  NQ_HIP(hipGetLastError());
+  prof_end(ctx, ev);
+  return NQ_OK;
+}
+
+#define NQ_MODES(SP, LK, XP, TAG)                                                            \
+  switch (epi.mode) {                                                                       \
+  case 0: return go<SP, LK, 0, SP>(ctx, op, out, in, U, anis, epi, "tm_hop_" TAG);            \
+  case 1: return go<SP, LK, 1, SP>(ctx, op, out, in, U, anis, epi, "tm_hop_norm_" TAG);       \
+  case 2: return go<SP, LK, 2, XP>(ctx, op, out, in, U, anis, epi, "tm_hop_cgtail_" TAG);     \
+  default: break;                                                                           \
+  }
+
+int launch_tm_hop(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, const LinkView& U, double anis,
+                  const HopEpi& epi)
+{
+  if (in.ncomp != 12) return fail(NQ_EINVAL, "tm hop needs spincolor fields");
+  if (epi.mode != 0 && !epi.st) return fail(NQ_EINVAL, "reducing epilogue without state");
+  const int sp = in.prec, lk = U.prec;
+  if ((epi.mode == 0 || (epi.mode == 1 && epi.write)) && out.prec != sp)
+    return fail(NQ_EINVAL, "tm hop: output precision differs from input");
+  if (epi.aux.data && epi.aux.prec != sp) return fail(NQ_EINVAL, "tm hop: aux precision mismatch");
+  if (epi.aux2.data && epi.aux2.prec != sp) return fail(NQ_EINVAL, "tm hop: aux2 precision mismatch");
+  if (epi.mode == 2 && (epi.p.prec != sp || epi.x.prec != epi.r.prec)) return fail(NQ_EINVAL, "tm hop: CG tail precision mismatch");
+  if (sp == SP_F64 && lk == LK_F64_18) {
+    if (epi.mode == 2 && epi.x.prec != SP_F64) return fail(NQ_EINVAL, "FP64 CG tail needs FP64 x,r");
+    NQ_MODES(SP_F64, LK_F64_18, SP_F64, "f64")
+    if (epi.mode == 3) {
+      if (epi.r.prec != SP_F64) return fail(NQ_EINVAL, "residual tail needs FP64 r");
+      if (!epi.rs.data) return go<SP_F64, LK_F64_18, 3, SP_F64>(ctx, op, out, in, U, anis, epi, "tm_hop_res_f64");
+      if (epi.rs.prec == SP_F32) return go<SP_F64, LK_F64_18, 3, SP_F32>(ctx, op, out, in, U, anis, epi, "tm_hop_res_f64_s32");
+      if (epi.rs.prec == SP_F16) return go<SP_F64, LK_F64_18, 3, SP_F16>(ctx, op, out, in, U, anis, epi, "tm_hop_res_f64_s16");
+    }
+  } else if (sp == SP_F32 && lk == LK_F32_18) {
+    if (epi.mode == 2 && epi.x.prec != SP_F32) return fail(NQ_EINVAL, "FP32 CG tail needs FP32 x,r");
+    NQ_MODES(SP_F32, LK_F32_18, SP_F32, "f32")
+  } else if (sp == SP_F16 && lk == LK_F32_12) {
+    if (epi.mode == 2 && epi.x.prec != SP_F32) return fail(NQ_EINVAL, "FP16 CG tail needs FP32 x,r");
+    NQ_MODES(SP_F16, LK_F32_12, SP_F32, "f16_l32c")
+  } else if (sp == SP_F16 && lk == LK_F16_12) {
+    if (epi.mode == 2 && epi.x.prec != SP_F32) return fail(NQ_EINVAL, "FP16 CG tail needs FP32 x,r");
+    NQ_MODES(SP_F16, LK_F16_12, SP_F32, "f16_l16c")
+  }
+  return fail(NQ_EINVAL, "tm hop: unsupported precision combination (spinor %d, links %d, mode %d)", sp, lk, epi.mode);
+}
+
+}  // namespace nq

@@ -1,0 +1,182 @@
+"""GPU parity of the twisted-mass path against the oracle, through the C ABI.
+
+Bars: a single operator apply matches the oracle to <= 1e-14 relative L2 in
+FP64; every solve reaches the requested true residual checked in FP64, and the
+solution agrees with the oracle's solve to 1e-10 relative L2 when both are
+driven to nissa residue 1e-24 (|r|/|b| = 1e-12).
+"""
+import numpy as np
+import pytest
+
+import lime
+import nissa_amd as nq
+from helpers import rel_l2, tm_conf, gauss_eo
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(8, 8, 8, 8), (4, 6, 4, 8), (6, 8, 4, 2), (16, 8, 8, 6)]
+
+
+@pytest.fixture(scope="module")
+def setups():
+    cache = {}
+
+    def get(L, seed=1):
+        key = (L, seed)
+        if key not in cache:
+            conf = tm_conf(L, seed)
+            ctx = nq.Ctx(L)
+            g = nq.Gauge(ctx, lx=conf)
+            geo = orc.Geom(L)
+            ce, co = geo.split(conf)
+            cache[key] = (ctx, g, geo, conf, ce, co)
+        return cache[key]
+
+    return get
+
+
+@pytest.mark.parametrize("L", SHAPES)
+@pytest.mark.parametrize("anis", [None, 1.37])
+def test_hop_both_parities(setups, L, anis):
+    ctx, g, geo, conf, ce, co = setups(L)
+    for out_par in (0, 1):
+        x = gauss_eo(geo.Vh, 3 + out_par)
+        fi = ctx.field(nq.SPINCOLOR, 1 - out_par, x)
+        fo = ctx.field(nq.SPINCOLOR, out_par)
+        nq.tm_hop(ctx, g, fo, fi, anis=anis)
+        ref = geo.tm_hop(out_par, ce, co, x, anis=anis)
+        assert rel_l2(fo.download(), ref) < 1e-14
+
+
+def test_hop_24c48(setups):
+    """Config (a): the single-parity hop at 24^3x48 matches to <= 1e-14."""
+    L = (48, 24, 24, 24)
+    ctx, g, geo, conf, ce, co = setups(L, 2)
+    x = gauss_eo(geo.Vh, 5)
+    fi = ctx.field(nq.SPINCOLOR, nq.EVEN, x)
+    fo = ctx.field(nq.SPINCOLOR, nq.ODD)
+    nq.tm_hop(ctx, g, fo, fi)
+    assert rel_l2(fo.download(), geo.tm_hop(1, ce, co, x)) < 1e-14
+
+
+@pytest.mark.parametrize("L", SHAPES[:3])
+@pytest.mark.parametrize("mu", [0.01, -0.2])
+def test_kern_and_square(setups, L, mu):
+    ctx, g, geo, conf, ce, co = setups(L)
+    kappa = 0.16
+    x = gauss_eo(geo.Vh, 7)
+    fi = ctx.field(nq.SPINCOLOR, nq.ODD, x)
+    fo = ctx.field(nq.SPINCOLOR, nq.ODD)
+    nq.tm_kern(ctx, g, kappa, mu, fo, fi)
+    assert rel_l2(fo.download(), geo.tmDkern(ce, co, kappa, mu, x)) < 1e-14
+    nq.tm_kern_square(ctx, g, kappa, mu, fo, fi)
+    assert rel_l2(fo.download(), geo.tmDkern_square(ce, co, kappa, mu, x)) < 1e-14
+    nq.tm_kern_square(ctx, g, kappa, mu, fo, fi, anis=0.9)
+    assert rel_l2(fo.download(), geo.tmDkern_square(ce, co, kappa, mu, x, anis=0.9)) < 1e-14
+    fe = ctx.field(nq.SPINCOLOR, nq.ODD)
+    nq.tm_Dee(ctx, kappa, mu, fe, fi, inverse=True)
+    assert rel_l2(fe.download(), geo.tmDee(kappa, mu, x, inverse=True)) < 1e-15
+
+
+@pytest.mark.parametrize("L", SHAPES[:2])
+def test_apply_Q_and_Q2(setups, L):
+    ctx, g, geo, conf, ce, co = setups(L)
+    kappa, mu = 0.16, 0.03
+    x = nq.source_lx(L, 11, "gauss")
+    fi = ctx.field(nq.SPINCOLOR, nq.LX, x)
+    fo = ctx.field(nq.SPINCOLOR, nq.LX)
+    nq.tm_apply_Q(ctx, g, kappa, mu, fo, fi)
+    assert rel_l2(fo.download(), geo.apply_tmQ(conf, kappa, mu, x)) < 1e-14
+    nq.tm_apply_Q2(ctx, g, kappa, mu, fo, fi)
+    assert rel_l2(fo.download(), geo.apply_tmQ2(conf, kappa, mu, x)) < 1e-14
+
+
+def test_tmlqcd_fixture_on_gpu(golden_dir):
+    """The GPU operator applied to the tmLQCD propagator gives the point source."""
+    L = (8, 4, 4, 4)
+    conf = lime.read_ildg_conf(f"{golden_dir}/L4T8conf", L)
+    nq.gauge_phase_lx(L, conf, [1.0, 0, 0, 0])
+    psi = lime.read_scidac_spincolor(f"{golden_dir}/source_1.0003.00.inverted.07", L)
+    with nq.Ctx(L) as ctx:
+        g = nq.Gauge(ctx, lx=conf)
+        fi = ctx.field(nq.SPINCOLOR, nq.LX, psi)
+        fo = ctx.field(nq.SPINCOLOR, nq.LX)
+        nq.tm_apply_Q(ctx, g, 0.150, 0.0, fo, fi)
+        D = fo.download()
+        D[:, 2:] *= -1
+        eta = np.zeros_like(D)
+        eta[0].reshape(12, 2)[7, 0] = 1.0
+        assert np.sum((D - eta) ** 2) < 1e-12
+        g.close()
+
+
+@pytest.mark.parametrize("L", [(8, 8, 8, 8), (4, 6, 4, 8)])
+def test_cg_kernsq_f64_matches_oracle(setups, L):
+    ctx, g, geo, conf, ce, co = setups(L)
+    kappa, mu = 0.16, 0.01
+    b = gauss_eo(geo.Vh, 13)
+    ref, rst = geo.tm_cg_kernsq(ce, co, kappa, mu, 10000, 1e-24, b)
+    fb = ctx.field(nq.SPINCOLOR, nq.ODD, b)
+    fx = ctx.field(nq.SPINCOLOR, nq.ODD)
+    st = nq.tm_cg_kernsq(ctx, g, kappa, mu, 10000, 1e-24, fx, fb, prec="f64")
+    assert st["rc"] == 0 and st["converged"]
+    assert abs(st["iters"] - rst["iters"]) <= 1
+    assert st["rel_res_sq"] < 2e-24
+    assert rel_l2(fx.download(), ref) < 1e-10
+    # with a guess: restart from the solution converges immediately-ish
+    st2 = nq.tm_cg_kernsq(ctx, g, kappa, mu, 10000, 1e-20, fx, fb, prec="f64", guess=fx)
+    assert st2["rc"] == 0 and st2["iters"] <= 2
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32", "f16", "f16h"])
+def test_eo_solve_all_precisions(setups, prec):
+    """inv_tmD_cg_eoprec on 8^4 (config 0 shape): Z2 source on even sites, kappa 0.16,
+    mu 0.01, residue 1e-24; solution within 1e-10 of the oracle's FP64 solve."""
+    L = (8, 8, 8, 8)
+    ctx, g, geo, conf, ce, co = setups(L)
+    src = nq.source_lx(L, 7, "z2", 12, 1)
+    ref, rst = geo.tm_solve_eoprec(conf, 0.16, 0.01, 10000, 1e-24, src)
+    fs = ctx.field(nq.SPINCOLOR, nq.LX, src)
+    fx = ctx.field(nq.SPINCOLOR, nq.LX)
+    st = nq.tm_solve_eoprec(ctx, g, 0.16, 0.01, 10000, 1e-24, fx, fs, prec=prec)
+    assert st["rc"] == 0 and st["converged"], st
+    assert st["rel_res_sq"] < 1e-24 * (2 if prec == "f64" else 1)
+    assert st["check_rel_res_sq"] < 1e-23, st
+    assert rel_l2(fx.download(), ref) < 1e-10
+    if prec != "f64":
+        assert st["reliable_updates"] >= 1
+
+
+def test_nan_source_is_rejected(setups):
+    L = (4, 6, 4, 8)
+    ctx, g, geo, conf, ce, co = setups(L)
+    b = gauss_eo(geo.Vh, 1)
+    b[3, 1, 2, 0] = np.nan
+    fb = ctx.field(nq.SPINCOLOR, nq.ODD, b)
+    fx = ctx.field(nq.SPINCOLOR, nq.ODD)
+    with pytest.raises(nq.NqError) as e:
+        nq.tm_cg_kernsq(ctx, g, 0.16, 0.01, 100, 1e-20, fx, fb)
+    assert e.value.code == nq.NQ_ENAN
+
+
+def test_non_convergence_is_reported(setups):
+    L = (4, 6, 4, 8)
+    ctx, g, geo, conf, ce, co = setups(L)
+    fb = ctx.field(nq.SPINCOLOR, nq.ODD, gauss_eo(geo.Vh, 2))
+    fx = ctx.field(nq.SPINCOLOR, nq.ODD)
+    st = nq.tm_cg_kernsq(ctx, g, 0.16, 0.01, 5, 1e-24, fx, fb)
+    assert st["rc"] == nq.NQ_ENOCONV and st["iters"] == 5
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-14), ("f32", 1e-6), ("f16", 2e-3), ("f16h", 2e-3)])
+def test_sloppy_hop_storage_formats(setups, prec, tol):
+    """The hop reading FP32 / FP16+norm spinors and 18/12-real FP32/FP16 links
+    agrees with the FP64 oracle to the storage precision."""
+    L = (8, 8, 8, 8)
+    ctx, g, geo, conf, ce, co = setups(L)
+    x = gauss_eo(geo.Vh, 3)
+    fi = ctx.field(nq.SPINCOLOR, nq.ODD, x)
+    fo = ctx.field(nq.SPINCOLOR, nq.EVEN)
+    nq.bench_tm_hop(ctx, g, 2, fo, fi, prec=prec)
+    assert rel_l2(fo.download(), geo.tm_hop(0, ce, co, x)) < tol
