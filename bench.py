@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""Benchmark of the twisted-mass fermion solve on one MI355X.
+
+Workload (one "step"): a full even-odd preconditioned twisted-mass solve
+(inv_tmD_cg_eoprec: source preparation, CG on K K^+, reconstruction) on a 32^3x64
+lattice, kappa=0.16, mu=0.003, seeded synthetic SU(3) gauge field U=exp(i 0.3 H)
+with the antiperiodic time phase, Gaussian spincolor source, to a true relative
+residual |r|/|b| = 1e-10 (nissa residue 1e-20) checked in FP64.  The inner
+Krylov iteration runs on FP16 spinors with per-site norms and 12-real FP32
+links, with FP64 reliable updates.  Gauge field and source are resident in HBM
+before the timed region.
+
+Also reported: the plain FP64 solve (same workload), the FP64 single-parity hop
+("Dslash") bandwidth over 1000 applies, the roofline of the dominant kernel of
+the timed workload (HIP-event ledger kept by the library), and the CPU oracle
+timed on the host cores on a bounded sample of the FP64 solve.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "tmQ2 CG solves/s @32^3x64, 1e-10 (mixed vs FP64); Dslash GB/s; serve p99 lat"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--lattice", default="64,32,32,32", help="T,LX,LY,LZ")
+    ap.add_argument("--prec", default="f16", choices=["f64", "f32", "f16", "f16h"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-iters", type=int, default=6, help="oracle CG iterations timed for the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}): this build has no multi-GPU path; "
+              "the solve would shard by request/RHS across GPUs (see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import nissa_amd as nq
+
+    L = tuple(int(x) for x in args.lattice.split(","))
+    kappa, mu, residue = 0.16, 0.003, 1e-20
+    V = int(np.prod(L))
+    log(f"[bench] lattice {L} V={V}: generating gauge field")
+    t0 = time.time()
+    conf = nq.gauge_random_lx(L, 3, 0.3)
+    nq.gauge_phase_lx(L, conf, [1.0, 0, 0, 0])
+    src = nq.source_lx(L, 17, "gauss")
+    log(f"[bench] inputs ready in {time.time() - t0:.1f}s")
+
+    ctx = nq.Ctx(L, 0)
+    g = nq.Gauge(ctx, lx=conf)
+    fs = ctx.field(nq.SPINCOLOR, nq.LX, src)
+    fx = ctx.field(nq.SPINCOLOR, nq.LX)
+
+    def solve(prec):
+        st = nq.tm_solve_eoprec(ctx, g, kappa, mu, 100000, residue, fx, fs, prec=prec)
+        if st["rc"] != 0 or st["check_rel_res_sq"] > 2 * residue:
+            raise RuntimeError(f"solve failed: {st}")
+        return st
+
+    # warm-up (also builds the sloppy link copies)
+    for _ in range(max(args.warmup, 0)):
+        st = solve(args.prec)
+        log(f"[bench] warmup {args.prec}: iters={st['iters']} reliable={st['reliable_updates']} "
+            f"check={st['check_rel_res_sq']:.2e} t={st['seconds']*1e3:.1f}ms")
+    ctx.sync()
+    t0 = time.perf_counter()
+    stats = [solve(args.prec) for _ in range(args.steps)]
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    solves_per_s = args.steps / elapsed
+    log(f"[bench] timed {args.steps} {args.prec} solves: {elapsed*1e3/args.steps:.2f} ms/solve "
+        f"({solves_per_s:.2f} solves/s), iters={stats[-1]['iters']}")
+
+    # plain FP64 solve of the same workload
+    solve("f64")
+    ctx.sync()
+    t0 = time.perf_counter()
+    n64 = max(1, min(3, args.steps))
+    st64 = [solve("f64") for _ in range(n64)]
+    ctx.sync()
+    t64 = (time.perf_counter() - t0) / n64
+    log(f"[bench] FP64 solve: {t64*1e3:.2f} ms, iters={st64[-1]['iters']}")
+
+    # roofline of the dominant kernel over a profiled pass identical to the timed region
+    ctx.reset_kernel_stats()
+    ctx.profiling(True)
+    for _ in range(args.steps):
+        solve(args.prec)
+    ctx.profiling(False)
+    ks = ctx.kernel_stats()
+    dom_name, dom = max(ks.items(), key=lambda kv: kv[1]["total_ms"])
+    avg_ms = dom["total_ms"] / dom["launches"]
+    achieved = dom["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9
+    total_ms = sum(v["total_ms"] for v in ks.values())
+    for name, v in sorted(ks.items(), key=lambda kv: -kv[1]["total_ms"]):
+        a = v["total_ms"] / v["launches"]
+        log(f"[bench]   {name:28s} {v['launches']:6d} x {a*1e3:8.1f} us = {v['total_ms']/args.steps:7.2f} ms/solve "
+            f"{v['bytes_per_launch']/(a*1e-3)/1e9:7.0f} GB/s")
+    log(f"[bench] dominant kernel {dom_name}: {dom['launches']} launches, avg {avg_ms*1e3:.1f} us, "
+        f"{achieved:.0f} GB/s, {100*dom['total_ms']/total_ms:.0f}% of kernel time")
+
+    # Dslash: FP64 single-parity hop, 1000 applies
+    fo = ctx.field(nq.SPINCOLOR, nq.ODD)
+    fe = ctx.field(nq.SPINCOLOR, nq.EVEN)
+    ms_hop, bytes_hop = nq.bench_tm_hop(ctx, g, 1000, fe, fo, prec="f64")
+    dslash_gbs = bytes_hop / (ms_hop * 1e-3) / 1e9
+    ms_hop16, bytes_hop16 = nq.bench_tm_hop(ctx, g, 1000, fe, fo, prec=args.prec)
+    log(f"[bench] Dslash FP64: {ms_hop*1e3:.1f} us/apply, {dslash_gbs:.0f} GB/s; "
+        f"{args.prec}: {ms_hop16*1e3:.1f} us/apply, {bytes_hop16/(ms_hop16*1e-3)/1e9:.0f} GB/s")
+
+    # CPU baseline: the oracle's FP64 CG on the same inputs, bounded sample
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(L, conf, src, kappa, mu, args.cpu_iters, st64[-1]["iters"])
+
+    out = {
+        "metric": METRIC,
+        "value": round(solves_per_s, 4),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"f16": "f32 compute, f16 spinor storage, f64 reliable updates",
+                  "f16h": "f32 compute, f16 spinor+link storage, f64 reliable updates",
+                  "f32": "f32 compute, f64 reliable updates", "f64": "f64"}[args.prec],
+        "data": "synthetic (seeded SU(3) U=exp(i 0.3 H), antiperiodic t; Gaussian source)",
+        "config": {
+            "workload": f"tm eo-preconditioned CG solve {L[1]}^3x{L[0]}, kappa=0.16, mu=0.003, "
+                        f"true residual 1e-10 (nissa residue 1e-20), inner {args.prec}",
+            "lattice": list(L),
+            "iters": stats[-1]["iters"],
+            "reliable_updates": stats[-1]["reliable_updates"],
+            "true_rel_res_sq": stats[-1]["check_rel_res_sq"],
+            "parallelism": "1 GPU",
+        },
+        "fp64": {"ms_per_solve": round(t64 * 1e3, 3), "solves_per_s": round(1 / t64, 4), "iters": st64[-1]["iters"],
+                 "mixed_speedup": round(solves_per_s * t64, 3)},
+        "dslash": {"fp64_GBs": round(dslash_gbs, 1), "fp64_us": round(ms_hop * 1e3, 2),
+                   "fp64_frac_of_peak": round(dslash_gbs / HBM_PEAK_GBS, 4),
+                   "bytes_per_apply": bytes_hop, "sloppy_us": round(ms_hop16 * 1e3, 2),
+                   "sloppy_GBs": round(bytes_hop16 / (ms_hop16 * 1e-3) / 1e9, 1)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom_name,
+                     "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": dom["bytes_per_launch"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    for f in (fs, fx, fo, fe):
+        f.close()
+    g.close()
+    ctx.close()
+
+
+def cpu_baseline(L, conf, src, kappa, mu, iters, fp64_iters):
+    """Time `iters` iterations of the oracle's FP64 CG on K K^+ (the reference's
+    unfused pass sequence) at the full lattice and scale to a complete solve of
+    fp64_iters iterations plus the eo preparation/reconstruction (2 K applies)."""
+    import numpy as np
+
+    from oracle import oracle as orc
+
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    geo = orc.Geom(L)
+    ce, co = geo.split(conf)
+    se, so = geo.split(src)
+    phi = so.copy()  # a representative odd source for the CG sample
+    t0 = time.perf_counter()
+    geo.tmDkern_square(ce, co, kappa, mu, phi)
+    t_apply = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, st = geo.tm_cg_kernsq(ce, co, kappa, mu, iters, 1e-20, phi)
+    t_cg = time.perf_counter() - t0
+    t_iter = t_cg / max(1, st["iters"] + 2)  # the sample also does the initial and final residual matvecs
+    t_solve = t_iter * fp64_iters + 2 * t_apply
+    log(f"[bench] CPU oracle: K K^+ apply {t_apply:.2f}s, {st['iters']} CG iterations in {t_cg:.2f}s "
+        f"-> {t_solve:.1f}s per {fp64_iters}-iteration solve on {cores} threads")
+    return {"value": round(1.0 / t_solve, 6), "unit": "solves/s", "cores": cores, "kind": "port",
+            "sample": f"oracle FP64 CG on K K^+ at {L[1]}^3x{L[0]}: {st['iters']} iterations timed "
+                      f"({t_cg:.1f}s) plus one apply, scaled to the {fp64_iters}-iteration FP64 solve"}
+
+
+if __name__ == "__main__":
+    main()

@@ -7,6 +7,7 @@
 #include <complex>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -212,19 +213,27 @@ int nq_ctx_create(int T, int LX, int LY, int LZ, int device, nq_ctx** out)
   g.T = T; g.LX = LX; g.LY = LY; g.LZ = LZ;
   g.Vh = (int64_t)T * LX * LY * LZ / 2;
   g.stride = (g.Vh + 63) / 64 * 64;
-  g.VS = (int64_t)LX * LY * LZ;
-  g.S2 = (int64_t)LY * LZ;
+  // t tile of the device site order (tuning knob NQ_TTILE in {2,4,8}; default 8)
+  int tb = 8;
+  if (const char* e = getenv("NQ_TTILE")) tb = atoi(e);
+  if (tb != 2 && tb != 4 && tb != 8) tb = 8;
+  while (T % tb) tb /= 2;
+  g.TB = tb;
+  g.logTB = g.TB == 8 ? 3 : (g.TB == 4 ? 2 : 1);
+  g.NTH = T / g.TB;
+  g.Sz = g.TB;
+  g.Sy = (int64_t)g.TB * LZ;
+  g.Sx = (int64_t)g.TB * LZ * LY;
+  g.Sth = (int64_t)g.TB * LZ * LY * LX;
   int rc = NQ_OK;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = fail(NQ_EDEVICE, "stream creation failed");
   const int64_t nchunks = (g.Vh + 63) / 64;
   c->max_partials = nchunks * 16;
   c->nstate = 64;
   if (rc == NQ_OK && hipMalloc(&c->partial, sizeof(double) * c->max_partials) != hipSuccess) rc = fail(NQ_EOOM, "partials");
-  if (rc == NQ_OK && hipMalloc(&c->counter, 256) != hipSuccess) rc = fail(NQ_EOOM, "counter");
   if (rc == NQ_OK && hipMalloc(&c->state, sizeof(CgState) * c->nstate) != hipSuccess) rc = fail(NQ_EOOM, "state");
   if (rc == NQ_OK && hipHostMalloc(&c->state_host, sizeof(CgState) * c->nstate, hipHostMallocDefault) != hipSuccess)
     rc = fail(NQ_EOOM, "pinned state");
-  if (rc == NQ_OK && hipMemset(c->counter, 0, 256) != hipSuccess) rc = fail(NQ_EDEVICE, "memset");
   if (rc == NQ_OK && hipMemset(c->state, 0, sizeof(CgState) * c->nstate) != hipSuccess) rc = fail(NQ_EDEVICE, "memset");
   if (rc == NQ_OK && hipDeviceSynchronize() != hipSuccess) rc = fail(NQ_EDEVICE, "sync");
   if (rc != NQ_OK) {
@@ -244,7 +253,6 @@ void nq_ctx_destroy(nq_ctx* c)
   for (auto e : c->event_pool) (void)hipEventDestroy(e);
   for (auto& t : c->temps) dev_free(t.second);
   if (c->partial) (void)hipFree(c->partial);
-  if (c->counter) (void)hipFree(c->counter);
   if (c->state) (void)hipFree(c->state);
   if (c->state_host) (void)hipHostFree(c->state_host);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -349,10 +357,10 @@ int nq_field_upload(nq_field* f, const double* host)
   NQ_TRY(ctx_temp(c, "staging", sizeof(double) * nd * nsites, &stg));
   NQ_HIP(hipMemcpyAsync(stg, host, sizeof(double) * nd * nsites, hipMemcpyHostToDevice, c->stream));
   if (f->coverage == NQ_LX) {
-    NQ_TRY(launch_aos_to_soa(c, (const double*)stg, nsites, f->ncomp(), 0, view_of(f, 0)));
-    NQ_TRY(launch_aos_to_soa(c, (const double*)stg, nsites, f->ncomp(), 1, view_of(f, 1)));
+    NQ_TRY(launch_aos_to_soa(c, (const double*)stg, f->ncomp(), 0, 1, view_of(f, 0)));
+    NQ_TRY(launch_aos_to_soa(c, (const double*)stg, f->ncomp(), 1, 1, view_of(f, 1)));
   } else {
-    NQ_TRY(launch_aos_to_soa(c, (const double*)stg, nsites, f->ncomp(), -1, view_of(f, 0)));
+    NQ_TRY(launch_aos_to_soa(c, (const double*)stg, f->ncomp(), f->coverage, 0, view_of(f, 0)));
   }
   NQ_HIP(hipStreamSynchronize(c->stream));
   return NQ_OK;
@@ -368,10 +376,10 @@ int nq_field_download(const nq_field* f, double* host)
   void* stg = nullptr;
   NQ_TRY(ctx_temp(c, "staging", sizeof(double) * nd * nsites, &stg));
   if (f->coverage == NQ_LX) {
-    NQ_TRY(launch_soa_to_aos(c, view_of(f, 0), nsites, f->ncomp(), 0, (double*)stg));
-    NQ_TRY(launch_soa_to_aos(c, view_of(f, 1), nsites, f->ncomp(), 1, (double*)stg));
+    NQ_TRY(launch_soa_to_aos(c, view_of(f, 0), f->ncomp(), 0, 1, (double*)stg));
+    NQ_TRY(launch_soa_to_aos(c, view_of(f, 1), f->ncomp(), 1, 1, (double*)stg));
   } else {
-    NQ_TRY(launch_soa_to_aos(c, view_of(f, 0), nsites, f->ncomp(), -1, (double*)stg));
+    NQ_TRY(launch_soa_to_aos(c, view_of(f, 0), f->ncomp(), f->coverage, 0, (double*)stg));
   }
   NQ_HIP(hipMemcpyAsync(host, stg, sizeof(double) * nd * nsites, hipMemcpyDeviceToHost, c->stream));
   NQ_HIP(hipStreamSynchronize(c->stream));

@@ -34,7 +34,6 @@ struct BlasArgs {
   int W, NR;
   int64_t nchunks;
   double* partial;
-  unsigned int* counter;
   CgState* st;
   int fin;
   int pred;  // 0 always, 1 skip rhs when done, 2 skip unless reliable
@@ -45,7 +44,7 @@ __device__ __forceinline__ bool skip_rhs(const BlasArgs& a, int r)
   if (!a.st || !a.pred) return false;
   const CgState* s = a.st + r;
   if (s->done) return true;
-  return a.pred == 2 && !s->reliable;
+  return (a.pred == 2 && !s->reliable) || (a.pred == 3 && s->reliable);
 }
 
 // y = a x + b y ; optional |y|^2 -> fin
@@ -74,7 +73,7 @@ __global__ void __launch_bounds__(1024) k_axpby(BlasArgs A, SpinorView y, double
     }
     store_any<NC>(y, A.g.stride, m.e, yv, m.r);
   }
-  if constexpr (RED) grid_reduce_rhs(nrm, A.partial, A.nchunks, m.r, m.chunk, A.NR, A.st, A.counter, A.fin);
+  if constexpr (RED) store_partial(nrm, A.partial, A.nchunks, m.r, m.chunk);
 }
 
 // out = a x + b y + c z (real coefficients; y/z may be absent; out may be absent);
@@ -105,7 +104,7 @@ __global__ void __launch_bounds__(1024) k_lincomb(BlasArgs A, SpinorView out, do
     for (int k = 0; k < NC; k++) nrm += o[k].re * o[k].re + o[k].im * o[k].im;
     if (out.data) store_any<NC>(out, A.g.stride, m.e, o, m.r);
   }
-  if constexpr (RED) grid_reduce_rhs(nrm, A.partial, A.nchunks, m.r, m.chunk, A.NR, A.st, A.counter, A.fin);
+  if constexpr (RED) store_partial(nrm, A.partial, A.nchunks, m.r, m.chunk);
 }
 
 // Re <a, b> -> fin
@@ -121,16 +120,17 @@ __global__ void __launch_bounds__(1024) k_dot(BlasArgs A, SpinorView a, SpinorVi
 #pragma unroll
     for (int k = 0; k < NC; k++) acc += av[k].re * bv[k].re + av[k].im * bv[k].im;
   }
-  grid_reduce_rhs(acc, A.partial, A.nchunks, m.r, m.chunk, A.NR, A.st, A.counter, A.fin);
+  store_partial(acc, A.partial, A.nchunks, m.r, m.chunk);
 }
 
-// p = r + beta p, beta from state[r]; skipped once that right-hand side is done
+// p = r + beta p, beta from state[r]; skipped once that right-hand side is done or
+// while it waits for a reliable update
 template <int NC>
 __global__ void __launch_bounds__(1024) k_pupdate(BlasArgs A, SpinorView p, SpinorView r)
 {
   const Map m = map_thread(A.W, A.NR, A.g.Vh);
   const CgState* s = A.st + m.r;
-  if (s->done || !m.active) return;
+  if (s->done || s->reliable || !m.active) return;
   const double beta = s->beta;
   cx<double> pv[NC], rv[NC];
   load_any<NC>(p, A.g.stride, m.e, pv, m.r);
@@ -193,6 +193,50 @@ __global__ void __launch_bounds__(1024) k_spin_diag(BlasArgs A, SpinorView out, 
   store_any<12>(out, A.g.stride, m.e, o, m.r);
 }
 
+// One block: for each right-hand side still running, sum its chunk partials with
+// 256 lanes (lane v takes chunks v, v+256, ... in four interleaved chains),
+// combine the lanes by a fixed pairwise tree and finalise its state.
+__global__ void __launch_bounds__(256) k_finalize(const double* partial, int64_t nchunks, int nr, CgState* st, int fin,
+                                                  int pred_reliable)
+{
+  __shared__ double vl[256];
+  const int vt = threadIdx.x;
+  for (int rr = 0; rr < nr; rr++) {
+    CgState* s = st + rr;
+    if (s->done || (pred_reliable == 1 && !s->reliable) || (pred_reliable == 2 && s->reliable)) continue;
+    const double* pr = partial + (int64_t)rr * nchunks;
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    int64_t c = vt;
+    for (; c + 768 < nchunks; c += 1024) {
+      a0 += pr[c];
+      a1 += pr[c + 256];
+      a2 += pr[c + 512];
+      a3 += pr[c + 768];
+    }
+    for (; c < nchunks; c += 256) a0 += pr[c];
+    vl[vt] = (a0 + a1) + (a2 + a3);
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+      if (vt < h) vl[vt] += vl[vt + h];
+      __syncthreads();
+    }
+    if (vt == 0) finalize(s, fin, vl[0]);
+    __syncthreads();
+  }
+}
+
+int launch_finalize(nq_ctx* ctx, int nr, CgState* st, int fin, int pred_reliable)
+{
+  if (!st || fin == FIN_NONE) return fail(NQ_EINVAL, "finalize without state");
+  const int64_t nchunks = (ctx->geo.Vh + 63) / 64;
+  hipEvent_t ev = nullptr;
+  prof_begin(ctx, "reduce_finalize", 8.0 * nchunks * nr, &ev);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, ctx->stream, ctx->partial, nchunks, nr, st, fin, pred_reliable);
+  NQ_HIP(hipGetLastError());
+  prof_end(ctx, ev);
+  return NQ_OK;
+}
+
 // ------------------------------------------------------------------ launchers
 Launch launch_shape(const Geo& g, int nr)
 {
@@ -226,7 +270,6 @@ static int make_args(nq_ctx* ctx, int nr, CgState* st, int fin, int pred, BlasAr
   A->NR = nr;
   A->nchunks = L->nchunks;
   A->partial = ctx->partial;
-  A->counter = ctx->counter;
   A->st = st;
   A->fin = fin;
   A->pred = pred;
@@ -256,6 +299,7 @@ int launch_axpby(nq_ctx* ctx, const SpinorView& y, const double a[2], const Spin
   }
   NQ_HIP(hipGetLastError());
   prof_end(ctx, ev);
+  if (red) NQ_TRY(launch_finalize(ctx, nr, st, fin, pred == 2 ? 1 : (pred == 3 ? 2 : 0)));
   return NQ_OK;
 }
 
@@ -282,6 +326,7 @@ int launch_lincomb(nq_ctx* ctx, const SpinorView& out, double a, const SpinorVie
   }
   NQ_HIP(hipGetLastError());
   prof_end(ctx, ev);
+  if (red) NQ_TRY(launch_finalize(ctx, nr, st, fin, pred == 2 ? 1 : (pred == 3 ? 2 : 0)));
   return NQ_OK;
 }
 
@@ -298,6 +343,7 @@ int launch_dot(nq_ctx* ctx, const SpinorView& a, const SpinorView& b, int fin, C
   else hipLaunchKernelGGL(k_dot<3>, grid, blk, 0, ctx->stream, A, a, b);
   NQ_HIP(hipGetLastError());
   prof_end(ctx, ev);
+  NQ_TRY(launch_finalize(ctx, nr, st, fin, pred == 2 ? 1 : (pred == 3 ? 2 : 0)));
   return NQ_OK;
 }
 

@@ -2,8 +2,9 @@
 // layouts, and construction of the per-output-parity link slot arrays.
 //
 // nissa AoS (src/base/field.hpp:408-416): site-major, internal degrees
-// [id][ic][ri] for spincolor, [mu][ic1][ic2][ri] for quad_su3.  On the device a
-// field of one parity is SoA over complex components (stride = padded Vh), so
+// [id][ic][ri] for spincolor, [mu][ic1][ic2][ri] for quad_su3, sites in nissa's
+// lx / eo order.  On the device a field of one parity is SoA over complex
+// components (stride = padded Vh) in the device site order (t tiled, see Geo), so
 // a wavefront reading one component of 64 consecutive sites issues one 1 KiB
 // (FP64) coalesced transaction.  Links are re-laid per output parity p as 8
 // slots (U_mu(x), U_mu(x-mu)^+) so the hop streams them without gathers.
@@ -12,12 +13,20 @@
 
 namespace nq {
 
+// host index of device site e of parity par: nissa lx (host_lx) or nissa eo = lx>>1
+__device__ __forceinline__ int64_t host_index(const Geo& g, int par, uint32_t e, int host_lx)
+{
+  const Site s = site_of(g, par, e);
+  const int64_t lx = nissa_lx(g, s.t, s.x, s.y, s.z);
+  return host_lx ? lx : lx >> 1;
+}
+
 template <int NC>
-__global__ void __launch_bounds__(kBlock) k_aos_to_soa(Geo g, const double* aos, int lx_par, SpinorView out)
+__global__ void __launch_bounds__(kBlock) k_aos_to_soa(Geo g, const double* aos, int par, int host_lx, SpinorView out)
 {
   const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
   if (e >= (uint32_t)g.Vh) return;
-  const uint32_t src = lx_par < 0 ? e : site_of(g, lx_par, e).lx;
+  const int64_t src = host_index(g, par, e, host_lx);
   const double* p = aos + (int64_t)src * 2 * NC;
   cx<double> v[NC];
 #pragma unroll
@@ -26,11 +35,11 @@ __global__ void __launch_bounds__(kBlock) k_aos_to_soa(Geo g, const double* aos,
 }
 
 template <int NC>
-__global__ void __launch_bounds__(kBlock) k_soa_to_aos(Geo g, SpinorView in, int lx_par, double* aos)
+__global__ void __launch_bounds__(kBlock) k_soa_to_aos(Geo g, SpinorView in, int par, int host_lx, double* aos)
 {
   const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
   if (e >= (uint32_t)g.Vh) return;
-  const uint32_t dst = lx_par < 0 ? e : site_of(g, lx_par, e).lx;
+  const int64_t dst = host_index(g, par, e, host_lx);
   cx<double> v[NC];
   SpAcc<SP_F64, double, NC>::load(in, g.stride, e, v);
   double* p = aos + (int64_t)dst * 2 * NC;
@@ -38,26 +47,26 @@ __global__ void __launch_bounds__(kBlock) k_soa_to_aos(Geo g, SpinorView in, int
   for (int k = 0; k < NC; k++) { p[2 * k] = v[k].re; p[2 * k + 1] = v[k].im; }
 }
 
-int launch_aos_to_soa(nq_ctx* ctx, const double* aos, int64_t, int ncomp, int lx_par, const SpinorView& out)
+int launch_aos_to_soa(nq_ctx* ctx, const double* aos, int ncomp, int par, int host_lx, const SpinorView& out)
 {
   if (out.prec != SP_F64) return fail(NQ_EINVAL, "upload target must be FP64");
   const int64_t nb = blocks_for(ctx->geo.Vh);
   if (ncomp == 12)
-    hipLaunchKernelGGL(k_aos_to_soa<12>, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, ctx->geo, aos, lx_par, out);
+    hipLaunchKernelGGL(k_aos_to_soa<12>, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, ctx->geo, aos, par, host_lx, out);
   else
-    hipLaunchKernelGGL(k_aos_to_soa<3>, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, ctx->geo, aos, lx_par, out);
+    hipLaunchKernelGGL(k_aos_to_soa<3>, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, ctx->geo, aos, par, host_lx, out);
   NQ_HIP(hipGetLastError());
   return NQ_OK;
 }
 
-int launch_soa_to_aos(nq_ctx* ctx, const SpinorView& in, int64_t, int ncomp, int lx_par, double* aos)
+int launch_soa_to_aos(nq_ctx* ctx, const SpinorView& in, int ncomp, int par, int host_lx, double* aos)
 {
   if (in.prec != SP_F64) return fail(NQ_EINVAL, "download source must be FP64");
   const int64_t nb = blocks_for(ctx->geo.Vh);
   if (ncomp == 12)
-    hipLaunchKernelGGL(k_soa_to_aos<12>, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, ctx->geo, in, lx_par, aos);
+    hipLaunchKernelGGL(k_soa_to_aos<12>, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, ctx->geo, in, par, host_lx, aos);
   else
-    hipLaunchKernelGGL(k_soa_to_aos<3>, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, ctx->geo, in, lx_par, aos);
+    hipLaunchKernelGGL(k_soa_to_aos<3>, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, ctx->geo, in, par, host_lx, aos);
   NQ_HIP(hipGetLastError());
   return NQ_OK;
 }
@@ -81,17 +90,18 @@ __global__ void __launch_bounds__(kBlock) k_build_links(Geo g, const double* ae,
   if (e >= (uint32_t)g.Vh) return;
   const double* conf[2] = {ae, ao};
   const Site s = site_of(g, op, e);
+  const int64_t ne = nissa_lx(g, s.t, s.x, s.y, s.z) >> 1;
   const int64_t par_off = (int64_t)op * 8 * NC * g.stride;
 #pragma unroll
   for (int mu = 0; mu < 4; mu++) {
     // forward: U_mu(x) as stored
-    const double* Uf = conf[op] + ((int64_t)e * 4 + mu) * 18;
+    const double* Uf = conf[op] + (ne * 4 + mu) * 18;
     const int64_t bf = par_off + (int64_t)(2 * mu) * NC * g.stride + e;
 #pragma unroll
     for (int k = 0; k < NC; k++) put_link<LK>(out, bf + k * g.stride, Uf[2 * k], Uf[2 * k + 1]);
     // backward: U_mu(x-mu)^+ , element (i,j) = conj(U(j,i))
-    const uint32_t n = neigh_eo(g, s, mu, false);
-    const double* Ub = conf[!op] + ((int64_t)n * 4 + mu) * 18;
+    const int64_t n = nissa_neigh_eo(g, s, mu, false);
+    const double* Ub = conf[!op] + (n * 4 + mu) * 18;
     const int64_t bb = par_off + (int64_t)(2 * mu + 1) * NC * g.stride + e;
 #pragma unroll
     for (int k = 0; k < NC; k++) {

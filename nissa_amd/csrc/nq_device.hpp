@@ -47,46 +47,65 @@ __device__ __forceinline__ void fma_to(cx<R>& acc, cx<R> a, cx<R> b)
 
 // ---------------------------------------------------------------- geometry
 struct Site {
-  int t, x, y, z;
-  uint32_t lx;
+  int t, x, y, z, th, tl;
+  uint32_t dlx;  // rank in the device site order
 };
 
 __device__ __forceinline__ Site site_of(const Geo& g, int par, uint32_t e)
 {
   Site s;
   const uint32_t q = 2u * e;
-  uint32_t r = q / (uint32_t)g.LZ;
-  const int z0 = (int)(q - r * (uint32_t)g.LZ);
-  const uint32_t r2 = r / (uint32_t)g.LY;
-  s.y = (int)(r - r2 * (uint32_t)g.LY);
+  const int tl0 = (int)(q & (uint32_t)(g.TB - 1));
+  uint32_t r = q >> g.logTB;
+  const uint32_t r1 = r / (uint32_t)g.LZ;
+  s.z = (int)(r - r1 * (uint32_t)g.LZ);
+  const uint32_t r2 = r1 / (uint32_t)g.LY;
+  s.y = (int)(r1 - r2 * (uint32_t)g.LY);
   const uint32_t r3 = r2 / (uint32_t)g.LX;
   s.x = (int)(r2 - r3 * (uint32_t)g.LX);
-  s.t = (int)r3;
-  const int b = (par + s.t + s.x + s.y) & 1;
-  s.z = z0 + b;
-  s.lx = q + (uint32_t)b;
+  s.th = (int)r3;
+  const int b = (par + tl0 + s.x + s.y + s.z) & 1;
+  s.tl = tl0 + b;
+  s.t = s.th * g.TB + s.tl;
+  s.dlx = q + (uint32_t)b;
   return s;
 }
 
-// eo index of the neighbour in direction mu (0=t..3=z), up (fwd) or down
+// nissa lx index (t-major, z fastest) of a site
+__device__ __forceinline__ int64_t nissa_lx(const Geo& g, int t, int x, int y, int z)
+{
+  return (((int64_t)t * g.LX + x) * g.LY + y) * g.LZ + z;
+}
+
+// device eo index of the neighbour in direction mu (0=t..3=z), up (fwd) or down
 __device__ __forceinline__ uint32_t neigh_eo(const Geo& g, const Site& s, int mu, bool up)
 {
-  int64_t lx = s.lx;
+  int64_t d = s.dlx;
   switch (mu) {
   case 0:
-    lx += up ? (s.t == g.T - 1 ? -(int64_t)(g.T - 1) * g.VS : g.VS) : (s.t == 0 ? (int64_t)(g.T - 1) * g.VS : -g.VS);
+    if (up) d += s.tl == g.TB - 1 ? (1 - g.TB) + (s.th == g.NTH - 1 ? -(int64_t)(g.NTH - 1) * g.Sth : g.Sth) : 1;
+    else d += s.tl == 0 ? (g.TB - 1) + (s.th == 0 ? (int64_t)(g.NTH - 1) * g.Sth : -g.Sth) : -1;
     break;
   case 1:
-    lx += up ? (s.x == g.LX - 1 ? -(int64_t)(g.LX - 1) * g.S2 : g.S2) : (s.x == 0 ? (int64_t)(g.LX - 1) * g.S2 : -g.S2);
+    d += up ? (s.x == g.LX - 1 ? -(int64_t)(g.LX - 1) * g.Sx : g.Sx) : (s.x == 0 ? (int64_t)(g.LX - 1) * g.Sx : -g.Sx);
     break;
   case 2:
-    lx += up ? (s.y == g.LY - 1 ? -(int64_t)(g.LY - 1) * g.LZ : g.LZ) : (s.y == 0 ? (int64_t)(g.LY - 1) * g.LZ : -(int64_t)g.LZ);
+    d += up ? (s.y == g.LY - 1 ? -(int64_t)(g.LY - 1) * g.Sy : g.Sy) : (s.y == 0 ? (int64_t)(g.LY - 1) * g.Sy : -g.Sy);
     break;
   default:
-    lx += up ? (s.z == g.LZ - 1 ? -(g.LZ - 1) : 1) : (s.z == 0 ? (g.LZ - 1) : -1);
+    d += up ? (s.z == g.LZ - 1 ? -(int64_t)(g.LZ - 1) * g.Sz : g.Sz) : (s.z == 0 ? (int64_t)(g.LZ - 1) * g.Sz : -g.Sz);
     break;
   }
-  return (uint32_t)(lx >> 1);
+  return (uint32_t)(d >> 1);
+}
+
+// nissa eo index (lx>>1) of the neighbour (layout conversion only)
+__device__ __forceinline__ int64_t nissa_neigh_eo(const Geo& g, const Site& s, int mu, bool up)
+{
+  int c[4] = {s.t, s.x, s.y, s.z};
+  const int L[4] = {g.T, g.LX, g.LY, g.LZ};
+  c[mu] = up ? (c[mu] + 1 == L[mu] ? 0 : c[mu] + 1) : (c[mu] == 0 ? L[mu] - 1 : c[mu] - 1);
+  return nissa_lx(g, c[0], c[1], c[2], c[3]) >> 1;
 }
 
 // ---------------------------------------------------------------- buffer loads
@@ -105,21 +124,27 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, in
 // first lets the compiler narrow a 64-bit load to a single dword)
 typedef double nq_dv2 __attribute__((ext_vector_type(2)));
 typedef float nq_fv2 __attribute__((ext_vector_type(2)));
+// CP = cache policy bits of the load: 0 default, 2 non-temporal (streamed data
+// that should not displace reusable lines, used for the links)
+template <int CP = 0>
 __device__ __forceinline__ double2 bload_d2(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
 {
-  const nq_dv2 v = __builtin_bit_cast(nq_dv2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+  const nq_dv2 v = __builtin_bit_cast(nq_dv2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, CP));
   return make_double2(v.x, v.y);
 }
+template <int CP = 0>
 __device__ __forceinline__ float2 bload_f2(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
 {
-  const nq_fv2 v = __builtin_bit_cast(nq_fv2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  const nq_fv2 v = __builtin_bit_cast(nq_fv2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, CP));
   return make_float2(v.x, v.y);
 }
+template <int CP = 0>
 __device__ __forceinline__ float2 bload_h2(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
 {
-  const unsigned v = __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+  const unsigned v = __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, CP);
   return __half22float2(__builtin_bit_cast(__half2, v));
 }
+constexpr int kLinkCP = 2;
 __device__ __forceinline__ float bload_f1(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
 {
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
@@ -127,7 +152,10 @@ __device__ __forceinline__ float bload_f1(__amdgpu_buffer_rsrc_t r, uint32_t vof
 
 // ---------------------------------------------------------------- spinor access
 // Load NC complex components of site e; compute type R.
-template <int SP, class R, int NC>
+// CP: cache policy of the accesses (0 default; kNT for data used once by the
+// kernel, which should not displace the neighbour spinors from L2)
+constexpr int kNT = 2;
+template <int SP, class R, int NC, int CP = 0>
 struct SpAcc {
   static constexpr int EB = SP == SP_F64 ? 16 : (SP == SP_F32 ? 8 : 4);  // bytes per complex
   // component k of site e of right-hand side r lives at data[r*rstride + k*stride + e]
@@ -139,13 +167,13 @@ struct SpAcc {
     if constexpr (SP == SP_F64) {
 #pragma unroll
       for (int k = 0; k < NC; k++) {
-        const double2 d = bload_d2(rs, vo, (uint32_t)(k * stride * EB));
+        const double2 d = bload_d2<CP>(rs, vo, (uint32_t)(k * stride * EB));
         s[k] = mk<R>((R)d.x, (R)d.y);
       }
     } else if constexpr (SP == SP_F32) {
 #pragma unroll
       for (int k = 0; k < NC; k++) {
-        const float2 d = bload_f2(rs, vo, (uint32_t)(k * stride * EB));
+        const float2 d = bload_f2<CP>(rs, vo, (uint32_t)(k * stride * EB));
         s[k] = mk<R>((R)d.x, (R)d.y);
       }
     } else {
@@ -153,7 +181,7 @@ struct SpAcc {
       const R nrm = (R)bload_f1(rn, e * 4, 0);
 #pragma unroll
       for (int k = 0; k < NC; k++) {
-        const float2 d = bload_h2(rs, vo, (uint32_t)(k * stride * EB));
+        const float2 d = bload_h2<CP>(rs, vo, (uint32_t)(k * stride * EB));
         s[k] = mk<R>((R)d.x * nrm, (R)d.y * nrm);
       }
     }
@@ -163,13 +191,21 @@ struct SpAcc {
   {
     const int64_t o = (int64_t)r * v.rstride + e;
     if constexpr (SP == SP_F64) {
-      double2* p = (double2*)v.data + o;
+      nq_dv2* p = (nq_dv2*)v.data + o;
 #pragma unroll
-      for (int k = 0; k < NC; k++) p[k * stride] = make_double2((double)s[k].re, (double)s[k].im);
+      for (int k = 0; k < NC; k++) {
+        const nq_dv2 d = {(double)s[k].re, (double)s[k].im};
+        if constexpr (CP == kNT) __builtin_nontemporal_store(d, p + k * stride);
+        else p[k * stride] = d;
+      }
     } else if constexpr (SP == SP_F32) {
-      float2* p = (float2*)v.data + o;
+      nq_fv2* p = (nq_fv2*)v.data + o;
 #pragma unroll
-      for (int k = 0; k < NC; k++) p[k * stride] = make_float2((float)s[k].re, (float)s[k].im);
+      for (int k = 0; k < NC; k++) {
+        const nq_fv2 d = {(float)s[k].re, (float)s[k].im};
+        if constexpr (CP == kNT) __builtin_nontemporal_store(d, p + k * stride);
+        else p[k * stride] = d;
+      }
     } else {
       float m = 0.f;
 #pragma unroll
@@ -199,20 +235,20 @@ struct LkAcc {
     if constexpr (LK == LK_F64_18) {
 #pragma unroll
       for (int k = 0; k < 9; k++) {
-        const double2 d = bload_d2(rs, vo, (uint32_t)(k * stride * EB));
+        const double2 d = bload_d2<kLinkCP>(rs, vo, (uint32_t)(k * stride * EB));
         u[k] = mk<R>((R)d.x, (R)d.y);
       }
     } else if constexpr (LK == LK_F32_18) {
 #pragma unroll
       for (int k = 0; k < 9; k++) {
-        const float2 d = bload_f2(rs, vo, (uint32_t)(k * stride * EB));
+        const float2 d = bload_f2<kLinkCP>(rs, vo, (uint32_t)(k * stride * EB));
         u[k] = mk<R>((R)d.x, (R)d.y);
       }
     } else {
 #pragma unroll
       for (int k = 0; k < 6; k++) {
-        const float2 d = LK == LK_F32_12 ? bload_f2(rs, vo, (uint32_t)(k * stride * EB))
-                                         : bload_h2(rs, vo, (uint32_t)(k * stride * EB));
+        const float2 d = LK == LK_F32_12 ? bload_f2<kLinkCP>(rs, vo, (uint32_t)(k * stride * EB))
+                                         : bload_h2<kLinkCP>(rs, vo, (uint32_t)(k * stride * EB));
         u[k] = mk<R>((R)d.x, (R)d.y);
       }
       // row 2 = det * conj(row0 x row1)
@@ -248,22 +284,6 @@ __device__ __forceinline__ double wave_sum(double v)
   return v;
 }
 
-// Block sum in fixed order (wavefront butterflies then waves in order).
-template <int BS>
-__device__ __forceinline__ double block_sum(double v, double* lds)
-{
-  v = wave_sum(v);
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  if (l == 0) lds[w] = v;
-  __syncthreads();
-  double t = 0;
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < BS / 64; i++) t += lds[i];
-  }
-  return t;  // valid in thread 0
-}
-
 __device__ inline void finalize(CgState* st, int fin, double total)
 {
   switch (fin) {
@@ -284,6 +304,10 @@ __device__ inline void finalize(CgState* st, int fin, double total)
     break;
   }
   case FIN_RR_SLOPPY: {
+    // mixed CG: when |r| has dropped by delta since the last FP64 residual (or the
+    // sloppy residual reached the target, or the iteration budget is spent) raise
+    // `reliable`: the sloppy kernels then idle until the host has run the FP64
+    // reliable update, whose finalisation (FIN_RR_TRUE) clears it again.
     const double rr_new = total;
     st->iter += 1;
     st->rr_new = rr_new;
@@ -291,7 +315,6 @@ __device__ inline void finalize(CgState* st, int fin, double total)
     if (sqrt(rr_new) < st->delta * st->r0 || rr_new < st->target || st->iter >= st->niter) {
       st->reliable = 1;
     } else {
-      st->reliable = 0;
       st->beta = rr_new / st->rr;
       st->rr = rr_new;
     }
@@ -330,55 +353,42 @@ struct Map {
   uint32_t e;
   bool active;
 };
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2);
+// the logical block order gives every XCD one contiguous range of chunks, so
+// neighbouring sites are processed by the same XCD.
+__device__ __forceinline__ int64_t xcd_block()
+{
+  const uint32_t b = blockIdx.x, nb = gridDim.x;
+  const uint32_t xcd = b & 7u, i = b >> 3, q8 = nb >> 3, r8 = nb & 7u;
+  return (int64_t)xcd * q8 + (xcd < r8 ? xcd : r8) + i;
+}
+
 __device__ __forceinline__ Map map_thread(int W, int NR, int64_t Vh)
 {
   Map m;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the wave index is uniform across the wavefront: keep r and chunk in SGPRs so
+  // that buffer descriptors built from them stay scalar
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   m.r = wave % NR;
-  m.chunk = (int64_t)blockIdx.x * W + wave / NR;
+  m.chunk = xcd_block() * W + wave / NR;
   const int64_t e = m.chunk * 64 + lane;
   m.active = e < Vh;
   m.e = (uint32_t)e;
   return m;
 }
 
-// Deterministic grid reduction over 64-site chunks.  Every wavefront holds the
-// partial of one (right-hand side r, chunk c) pair, stores it at
-// partial[r*nchunks + c], and the last block to arrive sums, for each r, the
-// chunk partials with one wavefront in a fixed pattern (lane i takes chunks
-// i, i+64, ...; then butterflies) and finalises state[r].  The summation order
-// therefore depends only on the lattice, never on how many right-hand sides
-// share the launch or on the block shape.
-__device__ __forceinline__ void grid_reduce_rhs(double v, double* partial, int64_t nchunks, int r, int64_t chunk,
-                                                int nr, CgState* st, unsigned int* counter, int fin)
+// Deterministic reductions over 64-site chunks.  Every wavefront holds the
+// partial of one (right-hand side r, chunk c) pair and stores it at
+// partial[r*nchunks + c] with a plain store; a one-block finalisation kernel
+// launched right after (k_finalize) sums the chunk partials of each r in a fixed
+// pattern and updates state[r].  No atomics or device-scope fences are needed:
+// the kernel boundary publishes the partials.  The summation order depends only
+// on the lattice, never on how many right-hand sides share the launch or on the
+// block shape, so a result is bit-identical alone or in a batch.
+__device__ __forceinline__ void store_partial(double v, double* partial, int64_t nchunks, int r, int64_t chunk)
 {
-  __shared__ int am_last;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
   v = wave_sum(v);
-  if (lane == 0 && r >= 0 && chunk < nchunks) {
-    __hip_atomic_store(&partial[(int64_t)r * nchunks + chunk], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    am_last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (am_last) {
-    __threadfence();
-    for (int rr = wave; rr < nr; rr += nwaves) {
-      CgState* s = st + rr;
-      if (s->done) continue;
-      double acc = 0;
-      for (int64_t c = lane; c < nchunks; c += 64)
-        acc += __hip_atomic_load(&partial[(int64_t)rr * nchunks + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      acc = wave_sum(acc);
-      if (lane == 0) finalize(s, fin, acc);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  if ((threadIdx.x & 63) == 0 && chunk < nchunks) partial[(int64_t)r * nchunks + chunk] = v;
 }
 
 }  // namespace nq

@@ -32,13 +32,20 @@ int fail(int code, const char* fmt, ...);
 enum SpPrec { SP_F64 = 0, SP_F32 = 1, SP_F16 = 2 };
 enum LkPrec { LK_F64_18 = 0, LK_F32_18 = 1, LK_F32_12 = 2, LK_F16_12 = 3 };
 
-// Device geometry of one lattice (all extents even; eo index = lx>>1).
+// Device geometry of one lattice (all extents even).
+// Device site order: t is split as t = th*TB + tl (TB in {2,4,8} dividing T) and
+// sites are ordered lexicographically in (th, x, y, z, tl), tl fastest; within a
+// parity the device index is that order's rank >> 1 (TB even).  Compared with
+// nissa's (t, x, y, z) order this brings the most frequent neighbour distance
+// down from LX*LY*LZ/2 to TB*LY*LZ/2 sites, so neighbour spinors are re-read from
+// L2 instead of the fabric.  Host buffers keep nissa's order; the conversion
+// kernels translate.
 struct Geo {
   int T, LX, LY, LZ;
-  int64_t Vh;      // sites per parity
-  int64_t stride;  // per-component stride of SoA fields (Vh rounded up to 64)
-  int64_t VS;      // LX*LY*LZ
-  int64_t S2;      // LY*LZ
+  int TB, logTB, NTH;  // t tile, log2, number of tiles
+  int64_t Vh;          // sites per parity
+  int64_t stride;      // per-component stride of SoA fields (Vh rounded up to 64)
+  int64_t Sz, Sy, Sx, Sth;  // device-order strides (tl stride is 1)
 };
 
 // One parity of a spinor (ncomp=12) or colour (ncomp=3) field in SoA layout:
@@ -63,8 +70,8 @@ struct LinkView {
   double det[2][8][2] = {};                  // [out parity][slot][re,im]
 };
 
-// Solver state living in device memory; updated by the last block of the
-// reducing kernels (see blas / hop kernels).
+// Solver state living in device memory; updated by the finalisation kernel
+// that follows each reducing launch.
 struct CgState {
   double b2, target;
   double rr, rr_new, pAp, alpha, beta;
@@ -72,8 +79,6 @@ struct CgState {
   double true_rr;       // last FP64 residual
   double extra[6];
   int iter, niter, done, reliable, nreliable, nan, mixed, flags;
-  unsigned int counter;
-  unsigned int pad[7];
 };
 
 enum Fin {
@@ -95,7 +100,8 @@ struct HopEpi {
   int mode = 0;
   int write = 1;
   int fin = FIN_NONE;
-  int pred_reliable = 0;   // run only when state->reliable
+  int pred_reliable = 0;   // 0: skip when done; 1: run only when state->reliable;
+                           // 2: run only when !state->reliable (sloppy iteration)
   int nrhs = 1;            // right-hand sides in the launch (views carry rstride)
   int batched = 0;         // use the multi-RHS kernel build even for nrhs == 1
   // c1, c2, c3 must stay contiguous: the kernel reads them as double[12]
@@ -130,7 +136,6 @@ struct nq_ctx {
   // reduction scratch
   double* partial = nullptr;  // per-(rhs, 64-site chunk) partials
   int64_t max_partials = 0;
-  unsigned int* counter = nullptr;    // block-arrival counter of the reducing kernels
   nq::CgState* state = nullptr;       // device, array of nstate
   nq::CgState* state_host = nullptr;  // pinned mirror
   int nstate = 0;
@@ -210,17 +215,18 @@ int launch_st_hop(nq_ctx* ctx, int out_par, const SpinorView& out, const SpinorV
                   const LinkView& U, const HopEpi& epi);
 
 // layout conversions
-int launch_aos_to_soa(nq_ctx* ctx, const double* aos, int64_t nsites_par, int ncomp, int lx_par,
-                      const SpinorView& out);
-int launch_soa_to_aos(nq_ctx* ctx, const SpinorView& in, int64_t nsites_par, int ncomp, int lx_par,
-                      double* aos);
+// host buffer in nissa order (host_lx: lx-indexed buffer, else eo-indexed) <->
+// device parity `par` of a field
+int launch_aos_to_soa(nq_ctx* ctx, const double* aos, int ncomp, int par, int host_lx, const SpinorView& out);
+int launch_soa_to_aos(nq_ctx* ctx, const SpinorView& in, int ncomp, int par, int host_lx, double* aos);
 int launch_build_links(nq_ctx* ctx, const double* aos_e, const double* aos_o, int lk, void* out,
                        const double det[4][2]);
 int launch_lx_split_gauge(nq_ctx* ctx, const double* lx, double* evn, double* odd);
 
 // BLAS-1 (all fused; operands FP64 or sloppy; nr right-hand sides per launch
 // sharing the views' rstride; st points at nr consecutive states).
-// pred: 0 always run, 1 skip right-hand sides that are done, 2 run only when reliable.
+// pred: 0 always run, 1 skip right-hand sides that are done, 2 run only when reliable,
+// 3 skip when done or reliable.
 int launch_axpby(nq_ctx* ctx, const SpinorView& y, const double a[2], const SpinorView& x,
                  const double b[2], int fin, CgState* st, int pred, int nr = 1);
 int launch_lincomb(nq_ctx* ctx, const SpinorView& out, double a, const SpinorView& x, double b,
@@ -237,5 +243,8 @@ int launch_zero(nq_ctx* ctx, const SpinorView& v, int nr = 1);
 int launch_spin_diag(nq_ctx* ctx, const SpinorView& out, const double a[2][2], const SpinorView& x,
                      const double b[2][2], const SpinorView& y, int nr = 1);
 int reduce_norm2(nq_ctx* ctx, const SpinorView& v, double* out);
+// sum the per-chunk partials of the last reducing launch into state[0..nr) with fin;
+// pred_reliable as in HopEpi (the same condition the reducing launch used)
+int launch_finalize(nq_ctx* ctx, int nr, CgState* st, int fin, int pred_reliable);
 int reduce_dot(nq_ctx* ctx, const SpinorView& a, const SpinorView& b, double* out);
 }  // namespace nq

@@ -15,6 +15,7 @@
 // (K(+mu)^+ = K(-mu)), so the matvec needs no extra pass for the dot product;
 // the x/r updates and |r|^2 are fused into the last hop of the matvec.
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -301,38 +302,54 @@ static int cg_kernsq_mixed(const TmSetup& S64, const TmSetup& Ss, int sp, double
   NQ_TRY(launch_convert(ctx, p, rs));
   NQ_TRY(launch_zero(ctx, xs));
   NQ_TRY(get_states(ctx, 0, 1));
-  const double rr0 = ctx->state_host[0].rr;
+  const double rr_init = ctx->state_host[0].rr;
+  // The sloppy iterations idle (device flag) once a reliable update is due; the host
+  // polls after chunks sized from the observed convergence rate to end where the
+  // next update is expected, and then runs the FP64 update unconditionally.
   int chunk = 8, launched = 0;
   while (!ctx->state_host[0].done) {
     for (int i = 0; i < chunk; i++) {
       HopEpi a = epi_inv_dee(Ss.kappa, -mu);
       a.st = st;
+      a.pred_reliable = 2;
       NQ_TRY(launch_tm_hop(ctx, NQ_EVEN, tmps, p, Ss.U, Ss.anis, a));
       HopEpi b1 = epi_kout(Ss.kappa, -mu, p);
       b1.mode = 1;
       b1.st = st;
       b1.fin = FIN_PAP;
+      b1.pred_reliable = 2;
       NQ_TRY(launch_tm_hop(ctx, NQ_ODD, t1s, tmps, Ss.U, Ss.anis, b1));
       HopEpi a2 = epi_inv_dee(Ss.kappa, mu);
       a2.st = st;
+      a2.pred_reliable = 2;
       NQ_TRY(launch_tm_hop(ctx, NQ_EVEN, tmps, t1s, Ss.U, Ss.anis, a2));
       HopEpi b2 = epi_kout(Ss.kappa, mu, t1s);
       b2.mode = 2;
       b2.st = st;
       b2.fin = FIN_RR_SLOPPY;
+      b2.pred_reliable = 2;
       b2.x = xs;
       b2.p = p;
       b2.r = rs;
       NQ_TRY(launch_tm_hop(ctx, NQ_ODD, none, tmps, Ss.U, Ss.anis, b2));
-      // reliable update (predicated on the device flag)
-      NQ_TRY(launch_reliable_accum(ctx, x64, xs, st));
-      NQ_TRY(residual_kernsq(S64, mu, r64, rs, x64, b, t64, tmp64, st, FIN_RR_TRUE, 1));
       NQ_TRY(launch_cg_pupdate(ctx, p, rs, st));
     }
     launched += chunk;
     NQ_TRY(get_states(ctx, 0, 1));
+    if (ctx->state_host[0].reliable && !ctx->state_host[0].done) {
+      // reliable update: x64 += xs, xs = 0; r64 = b - A x64 in FP64; rs = r64; p = rs + beta p
+      NQ_TRY(launch_reliable_accum(ctx, x64, xs, st));
+      NQ_TRY(residual_kernsq(S64, mu, r64, rs, x64, b, t64, tmp64, st, FIN_RR_TRUE, 1));
+      NQ_TRY(launch_cg_pupdate(ctx, p, rs, st));
+      NQ_TRY(get_states(ctx, 0, 1));
+    }
     const CgState& h = ctx->state_host[0];
-    chunk = next_chunk(rr0, h.rr, h.target, h.iter, chunk);
+    if (h.nan || h.done) break;
+    // iterations until |r| is expected to fall below delta * r0 (or the target)
+    const double goal = std::max(h.delta * h.delta * h.r0 * h.r0, h.target);
+    const double slope = h.iter > 0 && h.rr < rr_init ? std::log(h.rr / rr_init) / h.iter : -0.3;
+    int need = (int)std::ceil(std::log(goal / h.rr) / slope);
+    chunk = std::max(1, std::min(32, need));
     if (launched > niter + 64) break;
   }
   const CgState h = ctx->state_host[0];

@@ -70,7 +70,7 @@ __device__ __forceinline__ void scatter(cx<R> o[12], const cx<R> g0[3], const cx
   }
 }
 
-template <int SP, int LK, class R, int MU, bool FWD>
+template <int SP, int LK, class R, int MU, bool FWD, bool PIN>
 __device__ __forceinline__ void hop_slot(const Geo& g, const Site& s, uint32_t e, const SpinorView& in, int r,
                                          const void* ub, const double* det, R anis, bool use_anis,
                                          cx<R> o[12])
@@ -91,28 +91,32 @@ __device__ __forceinline__ void hop_slot(const Geo& g, const Site& s, uint32_t e
     for (int c = 0; c < 3; c++) { g0[c] = scal(g0[c], anis); g1[c] = scal(g1[c], anis); }
   }
   scatter<R, MU, FWD>(o, g0, g1);
-  // Materialise the accumulators before the next slot's loads are issued: without
-  // this the compiler sinks the arithmetic of all slots below all their loads and
-  // keeps ~400 values live (FP64 spills to scratch).
+  // Materialise the accumulators before the next group of slots issues its loads:
+  // without this the compiler sinks the arithmetic of all eight slots below all
+  // their loads and keeps ~400 values live (FP64 spills to scratch).  Groups of
+  // four slots keep enough loads in flight per wait (measured best on gfx950).
+  if constexpr (PIN) {
 #pragma unroll
-  for (int k = 0; k < 12; k++) asm volatile("" : "+v"(o[k].re), "+v"(o[k].im));
-  asm volatile("" ::: "memory");
+    for (int k = 0; k < 12; k++) asm volatile("" : "+v"(o[k].re), "+v"(o[k].im));
+    asm volatile("" ::: "memory");
+  }
 }
 
 template <int SP>
 using RealOf = typename std::conditional<SP == SP_F64, double, float>::type;
 
 template <int SP, int LK, int MODE, int XP, int MT>
-__global__ void __launch_bounds__(MT, (MT == 256 ? 2 : 1)) k_tm_hop(Geo g, int op, SpinorView out, SpinorView in, LinkView U, double anis_d,
+__global__ void __launch_bounds__(MT, (MT == 256 && SP != SP_F64 ? 2 : 1)) k_tm_hop(Geo g, int op, SpinorView out, SpinorView in, LinkView U, double anis_d,
                                                int use_anis, HopEpi epi, int W, int NR, int64_t nchunks,
-                                               double* partial, unsigned int* counter)
+                                               double* partial)
 {
   using R = RealOf<SP>;
   using RX = RealOf<XP>;
   const Map m = map_thread(W, NR, g.Vh);
   CgState* st = epi.st ? epi.st + m.r : nullptr;
   bool skip = false;
-  if (st) skip = st->done || (epi.pred_reliable && !st->reliable);
+  if (st)
+    skip = st->done || (epi.pred_reliable == 1 && !st->reliable) || (epi.pred_reliable == 2 && st->reliable);
   // a launch whose every right-hand side is idle does nothing (uniform over the grid)
   if (epi.st && NR == 1 && skip) return;
   double nrm = 0;
@@ -127,38 +131,49 @@ __global__ void __launch_bounds__(MT, (MT == 256 ? 2 : 1)) k_tm_hop(Geo g, int o
     cx<R> o[12];
 #pragma unroll
     for (int k = 0; k < 12; k++) o[k] = mk<R>(0, 0);
-    hop_slot<SP, LK, R, 0, true>(g, s, e, in, r, ub, det, anis, ua, o);
-    hop_slot<SP, LK, R, 0, false>(g, s, e, in, r, ub, det, anis, ua, o);
-    hop_slot<SP, LK, R, 1, true>(g, s, e, in, r, ub, det, anis, ua, o);
-    hop_slot<SP, LK, R, 1, false>(g, s, e, in, r, ub, det, anis, ua, o);
-    hop_slot<SP, LK, R, 2, true>(g, s, e, in, r, ub, det, anis, ua, o);
-    hop_slot<SP, LK, R, 2, false>(g, s, e, in, r, ub, det, anis, ua, o);
-    hop_slot<SP, LK, R, 3, true>(g, s, e, in, r, ub, det, anis, ua, o);
-    hop_slot<SP, LK, R, 3, false>(g, s, e, in, r, ub, det, anis, ua, o);
+    hop_slot<SP, LK, R, 0, true, false>(g, s, e, in, r, ub, det, anis, ua, o);
+    hop_slot<SP, LK, R, 0, false, false>(g, s, e, in, r, ub, det, anis, ua, o);
+    hop_slot<SP, LK, R, 1, true, false>(g, s, e, in, r, ub, det, anis, ua, o);
+    hop_slot<SP, LK, R, 1, false, true>(g, s, e, in, r, ub, det, anis, ua, o);
+    hop_slot<SP, LK, R, 2, true, false>(g, s, e, in, r, ub, det, anis, ua, o);
+    hop_slot<SP, LK, R, 2, false, false>(g, s, e, in, r, ub, det, anis, ua, o);
+    hop_slot<SP, LK, R, 3, true, false>(g, s, e, in, r, ub, det, anis, ua, o);
+    hop_slot<SP, LK, R, 3, false, true>(g, s, e, in, r, ub, det, anis, ua, o);
 
     // epilogue: out_h = c1 hop + c2 aux + c3 aux2 ; per-rhs coefficients when tabulated
-    const double* cf = epi.coef ? epi.coef + 12 * r : &epi.c1[0][0];
+    double cf[12];
+    if (epi.coef) {
+#pragma unroll
+      for (int k = 0; k < 12; k++) cf[k] = epi.coef[12 * r + k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        cf[k] = (&epi.c1[0][0])[k];
+        cf[4 + k] = (&epi.c2[0][0])[k];
+        cf[8 + k] = (&epi.c3[0][0])[k];
+      }
+    }
     const cx<R> c1u = mk<R>((R)cf[0], (R)cf[1]), c1l = mk<R>((R)cf[2], (R)cf[3]);
 #pragma unroll
     for (int k = 0; k < 12; k++) o[k] = (k < 6 ? c1u : c1l) * o[k];
     if (epi.aux.data) {
       cx<R> a[12];
-      SpAcc<SP, R, 12>::load(epi.aux, g.stride, e, a, r);
+      SpAcc<SP, R, 12, kNT>::load(epi.aux, g.stride, e, a, r);
       const cx<R> cu = mk<R>((R)cf[4], (R)cf[5]), cl = mk<R>((R)cf[6], (R)cf[7]);
 #pragma unroll
       for (int k = 0; k < 12; k++) fma_to(o[k], k < 6 ? cu : cl, a[k]);
     }
     if (epi.aux2.data) {
       cx<R> a[12];
-      SpAcc<SP, R, 12>::load(epi.aux2, g.stride, e, a, r);
+      SpAcc<SP, R, 12, kNT>::load(epi.aux2, g.stride, e, a, r);
       const cx<R> cu = mk<R>((R)cf[8], (R)cf[9]), cl = mk<R>((R)cf[10], (R)cf[11]);
 #pragma unroll
       for (int k = 0; k < 12; k++) fma_to(o[k], k < 6 ? cu : cl, a[k]);
     }
     if constexpr (MODE == 0) {
-      SpAcc<SP, R, 12>::store(out, g.stride, e, o, r);
+      SpAcc<SP, R, 12, kNT>::store(out, g.stride, e, o, r);
     } else if constexpr (MODE == 1) {
-      if (epi.write) SpAcc<SP, R, 12>::store(out, g.stride, e, o, r);
+      if (epi.write) SpAcc<SP, R, 12, kNT>::store(out, g.stride, e, o, r);
 #pragma unroll
       for (int k = 0; k < 12; k++) nrm += (double)o[k].re * (double)o[k].re + (double)o[k].im * (double)o[k].im;
     } else if constexpr (MODE == 2) {
@@ -166,9 +181,9 @@ __global__ void __launch_bounds__(MT, (MT == 256 ? 2 : 1)) k_tm_hop(Geo g, int o
       const RX alpha = (RX)st->alpha;
       cx<RX> x[12], rv[12];
       cx<R> p[12];
-      SpAcc<XP, RX, 12>::load(epi.x, g.stride, e, x, r);
-      SpAcc<SP, R, 12>::load(epi.p, g.stride, e, p, r);
-      SpAcc<XP, RX, 12>::load(epi.r, g.stride, e, rv, r);
+      SpAcc<XP, RX, 12, kNT>::load(epi.x, g.stride, e, x, r);
+      SpAcc<SP, R, 12, kNT>::load(epi.p, g.stride, e, p, r);
+      SpAcc<XP, RX, 12, kNT>::load(epi.r, g.stride, e, rv, r);
 #pragma unroll
       for (int k = 0; k < 12; k++) {
         x[k].re = fma(alpha, (RX)p[k].re, x[k].re);
@@ -177,22 +192,22 @@ __global__ void __launch_bounds__(MT, (MT == 256 ? 2 : 1)) k_tm_hop(Geo g, int o
         rv[k].im = fma(-alpha, (RX)o[k].im, rv[k].im);
         nrm += (double)rv[k].re * (double)rv[k].re + (double)rv[k].im * (double)rv[k].im;
       }
-      SpAcc<XP, RX, 12>::store(epi.x, g.stride, e, x, r);
-      SpAcc<XP, RX, 12>::store(epi.r, g.stride, e, rv, r);
+      SpAcc<XP, RX, 12, kNT>::store(epi.x, g.stride, e, x, r);
+      SpAcc<XP, RX, 12, kNT>::store(epi.r, g.stride, e, rv, r);
     } else {
       // residual tail: r64 = o ; rs = sloppy(o) ; |o|^2
-      SpAcc<SP, R, 12>::store(epi.r, g.stride, e, o, r);
+      SpAcc<SP, R, 12, kNT>::store(epi.r, g.stride, e, o, r);
       if (epi.rs.data) {
         cx<RX> q[12];
 #pragma unroll
         for (int k = 0; k < 12; k++) q[k] = mk<RX>((RX)o[k].re, (RX)o[k].im);
-        SpAcc<XP, RX, 12>::store(epi.rs, g.stride, e, q, r);
+        SpAcc<XP, RX, 12, kNT>::store(epi.rs, g.stride, e, q, r);
       }
 #pragma unroll
       for (int k = 0; k < 12; k++) nrm += (double)o[k].re * (double)o[k].re + (double)o[k].im * (double)o[k].im;
     }
   }
-  if constexpr (MODE != 0) grid_reduce_rhs(nrm, partial, nchunks, m.r, m.chunk, NR, epi.st, counter, epi.fin);
+  if constexpr (MODE != 0) store_partial(nrm, partial, nchunks, m.r, m.chunk);
 }
 
 static double sp_site_bytes(int sp, int ncomp)
@@ -254,12 +269,13 @@ static int go(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, 
   const bool multi = nr > 1 || epi.batched;
   if (multi)
     hipLaunchKernelGGL((k_tm_hop<SP, LK, MODE, XP, 1024>), dim3((unsigned)L.blocks), dim3(L.threads), 0, ctx->stream, g,
-                       op, out, in, U, anis, anis != 1.0 ? 1 : 0, epi, L.W, nr, L.nchunks, ctx->partial, ctx->counter);
+                       op, out, in, U, anis, anis != 1.0 ? 1 : 0, epi, L.W, nr, L.nchunks, ctx->partial);
   else
     hipLaunchKernelGGL((k_tm_hop<SP, LK, MODE, XP, 256>), dim3((unsigned)L.blocks), dim3(L.threads), 0, ctx->stream, g,
-                       op, out, in, U, anis, anis != 1.0 ? 1 : 0, epi, L.W, nr, L.nchunks, ctx->partial, ctx->counter);
+                       op, out, in, U, anis, anis != 1.0 ? 1 : 0, epi, L.W, nr, L.nchunks, ctx->partial);
   NQ_HIP(hipGetLastError());
   prof_end(ctx, ev);
+  if (MODE != 0) NQ_TRY(launch_finalize(ctx, nr, epi.st, epi.fin, epi.pred_reliable));
   return NQ_OK;
 }
 

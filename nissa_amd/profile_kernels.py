@@ -1,0 +1,62 @@
+"""Kernel driver for rocprofv3 runs: a fixed number of launches of one kernel
+family at a given lattice, with inputs resident on the device.
+
+  python -m nissa_amd.profile_kernels --what hop --prec f64 --n 20
+  python -m nissa_amd.profile_kernels --what copy --n 20          # calibration copy
+
+The copy case runs the site-local FP64 kernel of nq_tm_Dee (blas_spin_diag: reads
+192 B and writes 192 B per site, nothing else) so that hardware byte counters can
+be scaled against a streaming kernel whose HBM traffic is known exactly.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--what", default="hop", choices=["hop", "copy", "solve"])
+    ap.add_argument("--prec", default="f64", choices=["f64", "f32", "f16", "f16h"])
+    ap.add_argument("--n", type=int, default=20)
+    ap.add_argument("--lattice", default="64,32,32,32")
+    a = ap.parse_args()
+    import numpy as np
+
+    import nissa_amd as nq
+
+    L = tuple(int(x) for x in a.lattice.split(","))
+    conf = nq.gauge_random_lx(L, 3, 0.3)
+    nq.gauge_phase_lx(L, conf, [1.0, 0, 0, 0])
+    ctx = nq.Ctx(L)
+    g = nq.Gauge(ctx, lx=conf)
+    Vh = ctx.Vh
+    x = np.random.default_rng(1).normal(size=(Vh, 4, 3, 2))
+    fo = ctx.field(nq.SPINCOLOR, nq.ODD, x)
+    fe = ctx.field(nq.SPINCOLOR, nq.EVEN)
+    if a.what == "hop":
+        ms, by = nq.bench_tm_hop(ctx, g, a.n, fe, fo, prec=a.prec)
+        print(f"hop {a.prec}: {ms*1e3:.1f} us/apply, {by/1e6:.1f} MB algorithmic, {by/(ms*1e-3)/1e9:.0f} GB/s")
+    elif a.what == "copy":
+        fo2 = ctx.field(nq.SPINCOLOR, nq.ODD)
+        for _ in range(a.n):
+            nq.tm_Dee(ctx, 0.16, 0.003, fo2, fo)
+        ctx.sync()
+        print(f"copy: {a.n} x ({Vh*192/1e6:.1f} MB read + {Vh*192/1e6:.1f} MB written)")
+    else:
+        src = nq.source_lx(L, 17, "gauss")
+        fs = ctx.field(nq.SPINCOLOR, nq.LX, src)
+        fx = ctx.field(nq.SPINCOLOR, nq.LX)
+        for _ in range(a.n):
+            st = nq.tm_solve_eoprec(ctx, g, 0.16, 0.003, 100000, 1e-20, fx, fs, prec=a.prec)
+        print(f"solve {a.prec}: {st}")
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()
