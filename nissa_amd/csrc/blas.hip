@@ -130,7 +130,7 @@ __global__ void __launch_bounds__(1024) k_pupdate(BlasArgs A, SpinorView p, Spin
 {
   const Map m = map_thread(A.W, A.NR, A.g.Vh);
   const CgState* s = A.st + m.r;
-  if (s->done || s->reliable || !m.active) return;
+  if (s->done || !m.active || (A.pred == 2 ? !s->reliable : s->reliable)) return;
   const double beta = s->beta;
   cx<double> pv[NC], rv[NC];
   load_any<NC>(p, A.g.stride, m.e, pv, m.r);
@@ -141,6 +141,34 @@ __global__ void __launch_bounds__(1024) k_pupdate(BlasArgs A, SpinorView p, Spin
     pv[k].im = fma(beta, pv[k].im, rv[k].im);
   }
   store_any<NC>(p, A.g.stride, m.e, pv, m.r);
+}
+
+// x += alpha p ; r -= alpha s ; |r|^2 (plain CG update, cg_invert_template_threaded.hpp:82-102)
+template <int NC>
+__global__ void __launch_bounds__(1024) k_cg_update(BlasArgs A, SpinorView x, SpinorView p, SpinorView r, SpinorView s)
+{
+  const Map m = map_thread(A.W, A.NR, A.g.Vh);
+  const CgState* S = A.st + m.r;
+  double nrm = 0;
+  if (m.active && !S->done && !S->reliable) {
+    const double alpha = S->alpha;
+    cx<double> xv[NC], pv[NC], rv[NC], sv[NC];
+    load_any<NC>(x, A.g.stride, m.e, xv, m.r);
+    load_any<NC>(p, A.g.stride, m.e, pv, m.r);
+    load_any<NC>(r, A.g.stride, m.e, rv, m.r);
+    load_any<NC>(s, A.g.stride, m.e, sv, m.r);
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      xv[k].re = fma(alpha, pv[k].re, xv[k].re);
+      xv[k].im = fma(alpha, pv[k].im, xv[k].im);
+      rv[k].re = fma(-alpha, sv[k].re, rv[k].re);
+      rv[k].im = fma(-alpha, sv[k].im, rv[k].im);
+      nrm += rv[k].re * rv[k].re + rv[k].im * rv[k].im;
+    }
+    store_any<NC>(x, A.g.stride, m.e, xv, m.r);
+    store_any<NC>(r, A.g.stride, m.e, rv, m.r);
+  }
+  store_partial(nrm, A.partial, A.nchunks, m.r, m.chunk);
 }
 
 // x64 += xs ; xs = 0 (reliable update), only for right-hand sides flagged reliable
@@ -347,11 +375,11 @@ int launch_dot(nq_ctx* ctx, const SpinorView& a, const SpinorView& b, int fin, C
   return NQ_OK;
 }
 
-int launch_cg_pupdate(nq_ctx* ctx, const SpinorView& p, const SpinorView& r, CgState* st, int nr)
+int launch_cg_pupdate(nq_ctx* ctx, const SpinorView& p, const SpinorView& r, CgState* st, int nr, int pred)
 {
   BlasArgs A;
   Launch L;
-  NQ_TRY(make_args(ctx, nr, st, FIN_NONE, 1, &A, &L));
+  NQ_TRY(make_args(ctx, nr, st, FIN_NONE, pred == 2 ? 2 : 1, &A, &L));
   hipEvent_t ev = nullptr;
   prof_begin(ctx, "blas_pupdate", nr * (2 * site_bytes(p) + site_bytes(r)) * (double)ctx->geo.Vh, &ev);
   const dim3 grid((unsigned)L.blocks), blk(L.threads);
@@ -360,6 +388,37 @@ int launch_cg_pupdate(nq_ctx* ctx, const SpinorView& p, const SpinorView& r, CgS
   else hipLaunchKernelGGL(k_pupdate<3>, grid, blk, 0, ctx->stream, A, p, r);
   NQ_HIP(hipGetLastError());
   prof_end(ctx, ev);
+  return NQ_OK;
+}
+
+int launch_cg_update(nq_ctx* ctx, const SpinorView& x, const SpinorView& p, const SpinorView& r, const SpinorView& s,
+                     CgState* st, int fin, int nr)
+{
+  BlasArgs A;
+  Launch L;
+  NQ_TRY(make_args(ctx, nr, st, fin, 1, &A, &L));
+  hipEvent_t ev = nullptr;
+  prof_begin(ctx, "blas_cg_update",
+             nr * (2 * site_bytes(x) + site_bytes(p) + 2 * site_bytes(r) + site_bytes(s)) * (double)ctx->geo.Vh, &ev);
+  const dim3 grid((unsigned)L.blocks), blk(L.threads);
+  if (x.ncomp == 24) hipLaunchKernelGGL(k_cg_update<24>, grid, blk, 0, ctx->stream, A, x, p, r, s);
+  else if (x.ncomp == 12) hipLaunchKernelGGL(k_cg_update<12>, grid, blk, 0, ctx->stream, A, x, p, r, s);
+  else hipLaunchKernelGGL(k_cg_update<3>, grid, blk, 0, ctx->stream, A, x, p, r, s);
+  NQ_HIP(hipGetLastError());
+  prof_end(ctx, ev);
+  return launch_finalize(ctx, nr, st, fin, 2);
+}
+
+__global__ void k_clear_reliable(CgState* st, int nr)
+{
+  if ((int)threadIdx.x < nr) st[threadIdx.x].reliable = 0;
+}
+
+int launch_clear_reliable(nq_ctx* ctx, CgState* st, int nr)
+{
+  if (nr < 1 || nr > 64) return fail(NQ_EINVAL, "clear_reliable: bad count");
+  hipLaunchKernelGGL(k_clear_reliable, dim3(1), dim3(64), 0, ctx->stream, st, nr);
+  NQ_HIP(hipGetLastError());
   return NQ_OK;
 }
 

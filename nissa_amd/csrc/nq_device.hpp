@@ -320,14 +320,15 @@ __device__ inline void finalize(CgState* st, int fin, double total)
     }
     break;
   }
-  case FIN_RR_TRUE: {
+  case FIN_RR_TRUE:
+  case FIN_RR_TRUE_HOLD: {
     const double rr_true = total;
     st->nreliable += 1;
     st->true_rr = rr_true;
     st->beta = rr_true / st->rr;
     st->rr = rr_true;
     st->r0 = sqrt(rr_true);
-    st->reliable = 0;
+    if (fin == FIN_RR_TRUE) st->reliable = 0;
     if (isnan(rr_true)) { st->nan = 1; st->done = 1; }
     if (rr_true < st->target || st->iter >= st->niter) st->done = 1;
     break;

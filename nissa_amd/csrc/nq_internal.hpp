@@ -81,6 +81,18 @@ struct CgState {
   int iter, niter, done, reliable, nreliable, nan, mixed, flags;
 };
 
+constexpr int kMaxShift = 16;
+
+// Multi-shift CG state of one source (cgm_invert recurrences); lives in device
+// memory, updated by the CGM finalisation kernels.
+struct CgmState {
+  double b2, rr, betaa, betap, alpha, rfrf, pap, pad0;
+  double shift[kMaxShift], target[kMaxShift], zps[kMaxShift], zas[kMaxShift], zfs[kMaxShift];
+  double betas[kMaxShift], alphas[kMaxShift], res[kMaxShift];
+  int run[kMaxShift];
+  int done, nshift, nrun, iter, niter, nan, pad1[2];
+};
+
 enum Fin {
   FIN_NONE = 0,
   FIN_STORE0 = 1,     // extra[0] = total
@@ -90,6 +102,7 @@ enum Fin {
   FIN_RR_TRUE = 5,    // mixed CG: after FP64 recomputation
   FIN_RR_INIT = 6,    // initial residual: rr = total, r0 = sqrt
   FIN_STORE1 = 7,     // extra[1] = total
+  FIN_RR_TRUE_HOLD = 8,  // as FIN_RR_TRUE, but `reliable` stays raised (batched updates)
 };
 
 // Hop epilogue: out_h = c1_h*hop + c2_h*aux + c3_h*aux2 (h = upper/lower spins).
@@ -209,10 +222,19 @@ int launch_tm_hop(nq_ctx* ctx, int out_par, const SpinorView& out, const SpinorV
                   const LinkView& U, double anis, const HopEpi& epi);
 double tm_hop_bytes(int sp, int lk, const HopEpi& epi, int ncomp = 12);
 
-// staggered hop: out(par) = sum_mu U in(x+mu) - U^+ in(x-mu); then
-// out = a*hop + b*aux (aux optional) ; mode 1 reduces |out|^2 with fin
-int launch_st_hop(nq_ctx* ctx, int out_par, const SpinorView& out, const SpinorView& in,
-                  const LinkView& U, const HopEpi& epi);
+// staggered hop of output parity op: out = c1*hop + c2*aux (per-rhs (c1,c2) from
+// coef_dev when given). mode 0 write; 1 |out|^2; 4 Re<aux,out> (reduced into st with fin).
+// skip_flags (optional): rhs r is skipped when any of skip_flags[r*skip_stride + i],
+// i < skip_n, is set (skip_n = 2 on &CgState::done also skips while a reliable update
+// is due, and the finalisation then leaves such states alone).
+int launch_st_hop(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, const LinkView& U, double c1,
+                  double c2, const double* coef_dev, const SpinorView& aux, int mode, int write, int nr, int fin,
+                  CgState* st, const int* skip_flags = nullptr, int skip_stride = 0, int skip_n = 1);
+// one multi-shift CG iteration for nr sources (states: device CgmState[nr]);
+// sol/ps hold the shifts of each source at element offset k*sstride
+int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, const SpinorView& t,
+                         const SpinorView& r, const SpinorView& sol, const SpinorView& ps, int64_t sstride,
+                         void* states, int nr);
 
 // layout conversions
 // host buffer in nissa order (host_lx: lx-indexed buffer, else eo-indexed) <->
@@ -234,7 +256,15 @@ int launch_lincomb(nq_ctx* ctx, const SpinorView& out, double a, const SpinorVie
                    int nr = 1, int pred = 0);
 int launch_dot(nq_ctx* ctx, const SpinorView& a, const SpinorView& b, int fin, CgState* st,
                int nr = 1, int pred = 0);
-int launch_cg_pupdate(nq_ctx* ctx, const SpinorView& p, const SpinorView& r, CgState* st, int nr = 1);
+// p = r + beta p; pred 1: skip done or reliable-pending right-hand sides; 2: only the
+// reliable-pending ones (after FIN_RR_TRUE_HOLD)
+int launch_cg_pupdate(nq_ctx* ctx, const SpinorView& p, const SpinorView& r, CgState* st, int nr = 1, int pred = 1);
+// state[i].reliable = 0 for i < nr
+int launch_clear_reliable(nq_ctx* ctx, CgState* st, int nr);
+// x += alpha p ; r -= alpha s ; |r|^2 -> fin  (alpha from state, skipped when done or
+// while a reliable update is due)
+int launch_cg_update(nq_ctx* ctx, const SpinorView& x, const SpinorView& p, const SpinorView& r,
+                     const SpinorView& s, CgState* st, int fin, int nr = 1);
 int launch_reliable_accum(nq_ctx* ctx, const SpinorView& x64, const SpinorView& xs, CgState* st, int nr = 1);
 int launch_convert(nq_ctx* ctx, const SpinorView& out, const SpinorView& in, int nr = 1,
                    CgState* st = nullptr, int pred = 0);

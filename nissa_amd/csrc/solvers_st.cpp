@@ -1,0 +1,491 @@
+// Staggered operators and solvers on the device.
+//
+// Reference: src/dirac_operators/stD/dirac_operator_stD.hpp (apply_st2Doe,
+// apply_stDeo_half, apply_stD2ee_m2), src/inverters/staggered/cg_invert_stD2ee_m2.hpp
+// (CG on m2 - 1/4 Deo Doe) and cgm_invert_template_threaded.hpp with
+// ApplyStD2eeM2Functor (multi-shift CG, one Krylov space per source for all shifts).
+//
+// The multi-shift solve is batched over sources: one launch of each CGM kernel
+// serves every source, so each loaded link is reused across right-hand sides.
+// Mixed precision (NQ_PREC_F32): the CGM runs on FP32 fields and links to a
+// residual the FP32 iteration can reach, then every (source, shift) system is
+// refined by a batched CG with FP32 inner iterations and FP64 reliable updates
+// that is warm-started from the CGM solution; convergence is declared on the
+// FP64 true residual of each shifted system.
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "nq_internal.hpp"
+
+namespace nq {
+
+namespace {
+struct Guard {
+  std::lock_guard<std::recursive_mutex> lk;
+  explicit Guard(nq_ctx* c) : lk(c->mu) { (void)hipSetDevice(c->device); }
+};
+int check_color(const nq_ctx* c, const nq_field* f, int cov, const char* what)
+{
+  if (!f) return fail(NQ_EINVAL, "%s: null field", what);
+  if (f->ctx != c) return fail(NQ_EINVAL, "%s: field belongs to another context", what);
+  if (f->kind != NQ_COLOR || f->coverage != cov) return fail(NQ_EINVAL, "%s: wrong field kind/coverage", what);
+  return NQ_OK;
+}
+int put_states(nq_ctx* ctx, int i0, int n, const CgState* s)
+{
+  std::memcpy(ctx->state_host + i0, s, sizeof(CgState) * n);
+  NQ_HIP(hipMemcpyAsync(ctx->state + i0, ctx->state_host + i0, sizeof(CgState) * n, hipMemcpyHostToDevice, ctx->stream));
+  return NQ_OK;
+}
+int get_states(nq_ctx* ctx, int i0, int n)
+{
+  NQ_HIP(hipMemcpyAsync(ctx->state_host + i0, ctx->state + i0, sizeof(CgState) * n, hipMemcpyDeviceToHost, ctx->stream));
+  NQ_HIP(hipStreamSynchronize(ctx->stream));
+  return NQ_OK;
+}
+constexpr int kStateInts = (int)(sizeof(CgState) / sizeof(int));
+SpinorView shifted(const SpinorView& v, int64_t elems)
+{
+  SpinorView o = v;
+  const int eb = v.prec == SP_F64 ? 16 : (v.prec == SP_F32 ? 8 : 4);
+  o.data = (char*)v.data + elems * eb;
+  return o;
+}
+}  // namespace
+
+// A(m2) x = m2 x - 1/4 Deo Doe x on nr right-hand sides (t: odd scratch)
+static int apply_DDee(nq_ctx* ctx, const LinkView& U, double mass2, const double* coef, const SpinorView& out,
+                      const SpinorView& in, const SpinorView& t, int nr)
+{
+  SpinorView none;
+  none.ncomp = 3;
+  NQ_TRY(launch_st_hop(ctx, NQ_ODD, t, in, U, 1.0, 0.0, nullptr, none, 0, 1, nr, FIN_NONE, nullptr));
+  const bool aux = coef || mass2 != 0;
+  return launch_st_hop(ctx, NQ_EVEN, out, t, U, -0.25, mass2, coef, aux ? in : none, 0, 1, nr, FIN_NONE, nullptr);
+}
+
+// Batched CG on A(sigma_i) x_i = b_i for nr systems (per-system shift in coef[2i+1]).
+// prec NQ_PREC_F64: plain CG (cg_invert semantics); NQ_PREC_F32: FP32 inner iteration
+// with FP64 reliable updates.  x64 holds the guess on entry (use_guess) and the
+// solution on exit; residues[i] is the target of |r_i|^2/|b_i|^2.  States: ctx->state[0..nr).
+static int st_cg_batched(nq_ctx* ctx, const nq_gauge* g, int nr, const double* shifts, int niter,
+                         const double* residues, int prec, const SpinorView& x64, const SpinorView& b, bool use_guess,
+                         nq_solve_stats* st)
+{
+  LinkView U64, Us;
+  NQ_TRY(link_view(g, LK_F64_18, &U64));
+  const bool mixed = prec != NQ_PREC_F64;
+  if (mixed) NQ_TRY(link_view(g, LK_F32_18, &Us));
+  double* coef = nullptr;
+  NQ_TRY(ctx_temp(ctx, "stcg.coef", sizeof(double) * 2 * 16, (void**)&coef));
+  std::vector<double> hc(2 * nr);
+  for (int i = 0; i < nr; i++) { hc[2 * i] = -0.25; hc[2 * i + 1] = shifts[i]; }
+  NQ_HIP(hipMemcpyAsync(coef, hc.data(), sizeof(double) * 2 * nr, hipMemcpyHostToDevice, ctx->stream));
+  SpinorView r64, t64, s64, xs, rs, p, t, s;
+  NQ_TRY(alloc_spinor_nr(ctx, "stcg.r", SP_F64, 3, nr, &r64));
+  NQ_TRY(alloc_spinor_nr(ctx, "stcg.t", SP_F64, 3, nr, &t64));
+  NQ_TRY(alloc_spinor_nr(ctx, "stcg.s", SP_F64, 3, nr, &s64));
+  if (mixed) {
+    NQ_TRY(alloc_spinor_nr(ctx, "stcg.xs", SP_F32, 3, nr, &xs));
+    NQ_TRY(alloc_spinor_nr(ctx, "stcg.rs", SP_F32, 3, nr, &rs));
+    NQ_TRY(alloc_spinor_nr(ctx, "stcg.p32", SP_F32, 3, nr, &p));
+    NQ_TRY(alloc_spinor_nr(ctx, "stcg.t32", SP_F32, 3, nr, &t));
+    NQ_TRY(alloc_spinor_nr(ctx, "stcg.s32", SP_F32, 3, nr, &s));
+  } else {
+    NQ_TRY(alloc_spinor_nr(ctx, "stcg.p", SP_F64, 3, nr, &p));
+    t = t64;
+    s = s64;
+    xs = x64;
+    rs = r64;
+  }
+  SpinorView none;
+  none.ncomp = 3;
+  // |b|^2 per system
+  std::vector<CgState> h0(nr);
+  for (int i = 0; i < nr; i++) h0[i] = CgState{};
+  NQ_TRY(put_states(ctx, 0, nr, h0.data()));
+  NQ_TRY(launch_lincomb(ctx, none, 1.0, b, 0, none, 0, none, FIN_STORE0, ctx->state, nr));
+  NQ_TRY(get_states(ctx, 0, nr));
+  std::vector<double> b2(nr);
+  for (int i = 0; i < nr; i++) {
+    b2[i] = ctx->state_host[i].extra[0];
+    if (b2[i] == 0 || std::isnan(b2[i])) return fail(NQ_ENAN, "invalid norm: %g", b2[i]);
+    CgState s0 = {};
+    s0.b2 = b2[i];
+    s0.target = residues[i] * b2[i];
+    s0.niter = niter;
+    s0.mixed = mixed;
+    s0.delta = 1e-2;
+    h0[i] = s0;
+  }
+  NQ_TRY(put_states(ctx, 0, nr, h0.data()));
+  CgState* S = ctx->state;
+  // r = b - A x
+  if (use_guess) {
+    NQ_TRY(apply_DDee(ctx, U64, 0, coef, s64, x64, t64, nr));
+    NQ_TRY(launch_lincomb(ctx, r64, 1.0, b, -1.0, s64, 0, none, FIN_RR_INIT, S, nr));
+  } else {
+    NQ_TRY(launch_zero(ctx, x64, nr));
+    NQ_TRY(launch_lincomb(ctx, r64, 1.0, b, 0, none, 0, none, FIN_RR_INIT, S, nr));
+  }
+  if (mixed) {
+    NQ_TRY(launch_convert(ctx, rs, r64, nr));
+    NQ_TRY(launch_zero(ctx, xs, nr));
+  }
+  NQ_TRY(launch_convert(ctx, p, rs, nr));
+  NQ_TRY(get_states(ctx, 0, nr));
+  const LinkView& Ui = mixed ? Us : U64;
+  int chunk = 8, launched = 0;
+  auto all_done = [&]() {
+    for (int i = 0; i < nr; i++)
+      if (!ctx->state_host[i].done) return false;
+    return true;
+  };
+  while (!all_done()) {
+    for (int it = 0; it < chunk; it++) {
+      // t = Doe p ; s = sigma p - 1/4 Deo t with <p, s> -> pAp (skip idle systems)
+      NQ_TRY(launch_st_hop(ctx, NQ_ODD, t, p, Ui, 1.0, 0.0, nullptr, none, 0, 1, nr, FIN_NONE, nullptr, &S->done,
+                           kStateInts, 2));
+      NQ_TRY(launch_st_hop(ctx, NQ_EVEN, s, t, Ui, -0.25, 0.0, coef, p, 4, 1, nr, FIN_PAP, S, &S->done, kStateInts, 2));
+      NQ_TRY(launch_cg_update(ctx, xs, p, rs, s, S, mixed ? FIN_RR_SLOPPY : FIN_RR, nr));
+      NQ_TRY(launch_cg_pupdate(ctx, p, rs, S, nr));
+    }
+    launched += chunk;
+    NQ_TRY(get_states(ctx, 0, nr));
+    bool any_rel = false;
+    for (int i = 0; i < nr; i++) any_rel |= ctx->state_host[i].reliable && !ctx->state_host[i].done;
+    if (mixed && any_rel) {
+      // reliable update of the flagged systems: x64 += xs, xs = 0; r64 = b - A x64 in
+      // FP64; rs = r64; p = rs + beta p (beta from the true residual); then clear flags
+      NQ_TRY(launch_reliable_accum(ctx, x64, xs, S, nr));
+      NQ_TRY(apply_DDee(ctx, U64, 0, coef, s64, x64, t64, nr));
+      NQ_TRY(launch_lincomb(ctx, r64, 1.0, b, -1.0, s64, 0, none, FIN_RR_TRUE_HOLD, S, nr, 2));
+      NQ_TRY(launch_convert(ctx, rs, r64, nr, S, 2));
+      NQ_TRY(launch_cg_pupdate(ctx, p, rs, S, nr, 2));
+      NQ_TRY(launch_clear_reliable(ctx, S, nr));
+      NQ_TRY(get_states(ctx, 0, nr));
+    }
+    bool stop = true;
+    int need = 2;
+    for (int i = 0; i < nr; i++) {
+      const CgState& h = ctx->state_host[i];
+      if (h.nan) return fail(NQ_ENAN, "nanned");
+      if (h.done) continue;
+      stop = false;
+      const double goal = mixed ? std::max(h.delta * h.delta * h.r0 * h.r0, h.target) : h.target;
+      const double slope = h.iter > 0 && h.rr < h.b2 ? std::log(h.rr / h.b2) / h.iter : -0.1;
+      need = std::max(need, (int)std::ceil(std::log(goal / h.rr) / slope));
+    }
+    if (stop) break;
+    chunk = std::max(2, std::min(64, need));
+    if (launched > niter + 64) break;
+  }
+  int rc = NQ_OK;
+  for (int i = 0; i < nr; i++) {
+    const CgState& h = ctx->state_host[i];
+    st[i].iters = h.iter;
+    st[i].reliable_updates = h.nreliable;
+    st[i].converged = mixed ? h.true_rr < h.target : h.rr < h.target;
+    if (!st[i].converged) rc = NQ_ENOCONV;
+  }
+  // FP64 true residual of every system
+  std::vector<CgState> z(nr);
+  NQ_TRY(put_states(ctx, 0, nr, z.data()));
+  NQ_TRY(apply_DDee(ctx, U64, 0, coef, s64, x64, t64, nr));
+  NQ_TRY(launch_lincomb(ctx, none, 1.0, b, -1.0, s64, 0, none, FIN_STORE0, S, nr));
+  NQ_TRY(get_states(ctx, 0, nr));
+  for (int i = 0; i < nr; i++) st[i].rel_res_sq = ctx->state_host[i].extra[0] / b2[i];
+  if (rc != NQ_OK) return fail(rc, "exit without converging");
+  return NQ_OK;
+}
+
+// Batched multi-shift CG (cgm_invert) for nr <= 16 sources; sol/ps laid out
+// [source][shift][colour][site].  States: device CgmState[nr] in a ctx temp.
+static int st_cgm_batched(nq_ctx* ctx, const LinkView& U, int nr, int nshift, const double* shifts, int niter,
+                          const double* residues, const SpinorView& b, const SpinorView& sol, int sp,
+                          std::vector<int>* iters)
+{
+  const int64_t csz = 3 * ctx->geo.stride;
+  SpinorView p, t, r, ps;
+  NQ_TRY(alloc_spinor_nr(ctx, sp == SP_F64 ? "cgm.p" : "cgm.p32", sp, 3, nr, &p));
+  NQ_TRY(alloc_spinor_nr(ctx, sp == SP_F64 ? "cgm.t" : "cgm.t32", sp, 3, nr, &t));
+  NQ_TRY(alloc_spinor_nr(ctx, sp == SP_F64 ? "cgm.r" : "cgm.r32", sp, 3, nr, &r));
+  NQ_TRY(alloc_spinor_nr(ctx, sp == SP_F64 ? "cgm.ps" : "cgm.ps32", sp, 3, nr * nshift, &ps));
+  ps.rstride = csz * nshift;
+  CgmState* dst = nullptr;
+  CgmState* hst = nullptr;
+  NQ_TRY(ctx_temp(ctx, "cgm.state", sizeof(CgmState) * 16, (void**)&dst));
+  NQ_HIP(hipHostMalloc((void**)&hst, sizeof(CgmState) * 16, hipHostMallocDefault));
+  struct HostFree {
+    CgmState* p;
+    ~HostFree() { (void)hipHostFree(p); }
+  } hf{hst};
+  SpinorView none;
+  none.ncomp = 3;
+  // |b|^2 per source (CgState scratch slots)
+  std::vector<CgState> z(nr);
+  NQ_TRY(put_states(ctx, 0, nr, z.data()));
+  NQ_TRY(launch_lincomb(ctx, none, 1.0, b, 0, none, 0, none, FIN_STORE0, ctx->state, nr));
+  NQ_TRY(get_states(ctx, 0, nr));
+  for (int i = 0; i < nr; i++) {
+    CgmState s = {};
+    s.b2 = ctx->state_host[i].extra[0];
+    if (s.b2 == 0 || std::isnan(s.b2)) return fail(NQ_ENAN, "invalid norm: %g", s.b2);
+    s.rr = s.b2;
+    s.betaa = 1;
+    s.alpha = 0;
+    s.nshift = nshift;
+    s.nrun = nshift;
+    s.niter = niter;
+    for (int k = 0; k < nshift; k++) {
+      s.shift[k] = shifts[k];
+      s.target[k] = residues[k];
+      s.zps[k] = s.zas[k] = 1;
+      s.alphas[k] = 0;
+      s.run[k] = 1;
+    }
+    hst[i] = s;
+  }
+  NQ_HIP(hipMemcpyAsync(dst, hst, sizeof(CgmState) * nr, hipMemcpyHostToDevice, ctx->stream));
+  // sol_k = 0 ; ps_k = b ; p = r = b
+  for (int k = 0; k < nshift; k++) {
+    NQ_TRY(launch_zero(ctx, shifted(sol, k * csz), nr));
+    NQ_TRY(launch_convert(ctx, shifted(ps, k * csz), b, nr));
+  }
+  NQ_TRY(launch_convert(ctx, p, b, nr));
+  NQ_TRY(launch_convert(ctx, r, b, nr));
+  int chunk = 16, launched = 0;
+  for (;;) {
+    for (int it = 0; it < chunk; it++) NQ_TRY(launch_cgm_iteration(ctx, U, p, t, r, sol, ps, csz, dst, nr));
+    launched += chunk;
+    NQ_HIP(hipMemcpyAsync(hst, dst, sizeof(CgmState) * nr, hipMemcpyDeviceToHost, ctx->stream));
+    NQ_HIP(hipStreamSynchronize(ctx->stream));
+    bool all = true;
+    int need = 4;
+    for (int i = 0; i < nr; i++) {
+      if (hst[i].nan) return fail(NQ_ENAN, "nanned");
+      if (hst[i].done) continue;
+      all = false;
+      double worst = 0;
+      for (int k = 0; k < nshift; k++)
+        if (hst[i].run[k]) worst = std::max(worst, std::log(hst[i].res[k] / hst[i].target[k]));
+      const double slope = hst[i].iter > 0 ? std::log(hst[i].rr / hst[i].b2) / hst[i].iter : -0.05;
+      if (slope < 0) need = std::max(need, (int)std::ceil(-worst / slope));
+    }
+    if (all || launched > niter + 64) break;
+    chunk = std::max(4, std::min(128, need));
+  }
+  if (iters) {
+    iters->resize(nr);
+    for (int i = 0; i < nr; i++) (*iters)[i] = hst[i].iter;
+  }
+  for (int i = 0; i < nr; i++)
+    if (!hst[i].done || hst[i].iter >= niter) return fail(NQ_ENOCONV, "exit without converging");
+  return NQ_OK;
+}
+
+}  // namespace nq
+
+using namespace nq;
+
+extern "C" {
+
+int nq_st_2Doe(nq_ctx* c, const nq_gauge* g, nq_field* out, const nq_field* in)
+{
+  if (!c || !g) return fail(NQ_EINVAL, "null argument");
+  NQ_TRY(check_color(c, in, NQ_EVEN, "nq_st_2Doe"));
+  NQ_TRY(check_color(c, out, NQ_ODD, "nq_st_2Doe"));
+  Guard G(c);
+  LinkView U;
+  NQ_TRY(link_view(g, LK_F64_18, &U));
+  SpinorView none;
+  none.ncomp = 3;
+  NQ_TRY(launch_st_hop(c, NQ_ODD, view_of(out, 0), view_of(in, 0), U, 1.0, 0.0, nullptr, none, 0, 1, 1, FIN_NONE, nullptr));
+  NQ_HIP(hipStreamSynchronize(c->stream));
+  return NQ_OK;
+}
+
+int nq_st_Deo_half(nq_ctx* c, const nq_gauge* g, nq_field* out, const nq_field* in)
+{
+  if (!c || !g) return fail(NQ_EINVAL, "null argument");
+  NQ_TRY(check_color(c, in, NQ_ODD, "nq_st_Deo_half"));
+  NQ_TRY(check_color(c, out, NQ_EVEN, "nq_st_Deo_half"));
+  Guard G(c);
+  LinkView U;
+  NQ_TRY(link_view(g, LK_F64_18, &U));
+  SpinorView none;
+  none.ncomp = 3;
+  NQ_TRY(launch_st_hop(c, NQ_EVEN, view_of(out, 0), view_of(in, 0), U, 0.25, 0.0, nullptr, none, 0, 1, 1, FIN_NONE,
+                       nullptr));
+  NQ_HIP(hipStreamSynchronize(c->stream));
+  return NQ_OK;
+}
+
+int nq_st_DDee(nq_ctx* c, const nq_gauge* g, double mass2, nq_field* out, const nq_field* in)
+{
+  if (!c || !g) return fail(NQ_EINVAL, "null argument");
+  NQ_TRY(check_color(c, in, NQ_EVEN, "nq_st_DDee"));
+  NQ_TRY(check_color(c, out, NQ_EVEN, "nq_st_DDee"));
+  if (out == in) return fail(NQ_EINVAL, "out==in!");
+  Guard G(c);
+  LinkView U;
+  NQ_TRY(link_view(g, LK_F64_18, &U));
+  SpinorView t;
+  NQ_TRY(alloc_spinor(c, "st.t", SP_F64, 3, &t));
+  NQ_TRY(apply_DDee(c, U, mass2, nullptr, view_of(out, 0), view_of(in, 0), t, 1));
+  NQ_HIP(hipStreamSynchronize(c->stream));
+  return NQ_OK;
+}
+
+int nq_st_cg(nq_ctx* c, const nq_gauge* g, double mass2, int niter, double residue, nq_field* sol,
+             const nq_field* guess, const nq_field* src, nq_solve_stats* st)
+{
+  if (!c || !g) return fail(NQ_EINVAL, "null argument");
+  NQ_TRY(check_color(c, src, NQ_EVEN, "nq_st_cg"));
+  NQ_TRY(check_color(c, sol, NQ_EVEN, "nq_st_cg"));
+  if (guess) NQ_TRY(check_color(c, guess, NQ_EVEN, "nq_st_cg"));
+  if (niter <= 0 || !(residue > 0)) return fail(NQ_EINVAL, "bad niter/residue");
+  Guard G(c);
+  nq_solve_stats local;
+  if (!st) st = &local;
+  std::memset(st, 0, sizeof *st);
+  hipEvent_t a, b;
+  (void)hipEventCreate(&a);
+  (void)hipEventCreate(&b);
+  (void)hipEventRecord(a, c->stream);
+  if (guess && guess != sol) NQ_TRY(launch_convert(c, view_of(sol, 0), view_of(guess, 0)));
+  const int rc = st_cg_batched(c, g, 1, &mass2, niter, &residue, NQ_PREC_F64, view_of(sol, 0), view_of(src, 0),
+                               guess != nullptr, st);
+  (void)hipEventRecord(b, c->stream);
+  (void)hipEventSynchronize(b);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, a, b);
+  st->seconds = ms * 1e-3;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return rc;
+}
+
+int nq_st_cgm(nq_ctx* c, const nq_gauge* g, int nrhs, int nshift, const double* shifts, int niter,
+              const double* residues, int prec, nq_field** sols, const nq_field* const* srcs, nq_solve_stats* st,
+              double* true_res_sq)
+{
+  if (!c || !g || !shifts || !residues || !sols || !srcs) return fail(NQ_EINVAL, "null argument");
+  if (nrhs < 1 || nshift < 1 || nshift > kMaxShift) return fail(NQ_EINVAL, "need 1..%d shifts and >= 1 source", kMaxShift);
+  if (prec != NQ_PREC_F64 && prec != NQ_PREC_F32) return fail(NQ_EINVAL, "CGM supports NQ_PREC_F64 and NQ_PREC_F32");
+  for (int i = 0; i < nrhs; i++) {
+    NQ_TRY(check_color(c, srcs[i], NQ_EVEN, "nq_st_cgm"));
+    for (int k = 0; k < nshift; k++) NQ_TRY(check_color(c, sols[i * nshift + k], NQ_EVEN, "nq_st_cgm"));
+  }
+  Guard G(c);
+  hipEvent_t ea, eb;
+  (void)hipEventCreate(&ea);
+  (void)hipEventCreate(&eb);
+  (void)hipEventRecord(ea, c->stream);
+  LinkView U64, U32;
+  NQ_TRY(link_view(g, LK_F64_18, &U64));
+  const bool mixed = prec == NQ_PREC_F32;
+  if (mixed) NQ_TRY(link_view(g, LK_F32_18, &U32));
+  const int64_t csz = 3 * c->geo.stride;
+  int rc = NQ_OK;
+  for (int r0 = 0; r0 < nrhs; r0 += 16) {
+    const int nr = std::min(16, nrhs - r0);
+    SpinorView b, sol;
+    NQ_TRY(alloc_spinor_nr(c, "cgm.b", SP_F64, 3, nr, &b));
+    for (int i = 0; i < nr; i++) NQ_TRY(launch_convert(c, shifted(b, i * csz), view_of(srcs[r0 + i], 0)));
+    const int sp = mixed ? SP_F32 : SP_F64;
+    SpinorView bs = b;
+    if (mixed) {
+      NQ_TRY(alloc_spinor_nr(c, "cgm.b32", SP_F32, 3, nr, &bs));
+      NQ_TRY(launch_convert(c, bs, b, nr));
+    }
+    NQ_TRY(alloc_spinor_nr(c, mixed ? "cgm.sol32" : "cgm.sol", sp, 3, nr * nshift, &sol));
+    sol.rstride = csz * nshift;
+    std::vector<double> res(nshift);
+    for (int k = 0; k < nshift; k++) res[k] = mixed ? std::max(residues[k], 1e-11) : residues[k];
+    std::vector<int> its;
+    int rci = st_cgm_batched(c, mixed ? U32 : U64, nr, nshift, shifts, niter, res.data(), bs, sol, sp, &its);
+    if (rci != NQ_OK && rci != NQ_ENOCONV) return rci;
+    if (rci == NQ_ENOCONV) rc = rci;
+    // copy out; FP64 solutions per (source, shift)
+    for (int i = 0; i < nr; i++)
+      for (int k = 0; k < nshift; k++)
+        NQ_TRY(launch_convert(c, view_of(sols[(r0 + i) * nshift + k], 0), shifted(sol, i * csz * nshift + k * csz)));
+    for (int i = 0; i < nr; i++)
+      if (st) {
+        st[r0 + i] = nq_solve_stats{};
+        st[r0 + i].iters = its.empty() ? 0 : its[i];
+      }
+    // check / refine every (source, shift) system with the FP64 operator, 16 at a time
+    const int nsys = nr * nshift;
+    for (int s0 = 0; s0 < nsys; s0 += 16) {
+      const int ns = std::min(16, nsys - s0);
+      SpinorView xb, bb;
+      NQ_TRY(alloc_spinor_nr(c, "cgm.xb", SP_F64, 3, ns, &xb));
+      NQ_TRY(alloc_spinor_nr(c, "cgm.bb", SP_F64, 3, ns, &bb));
+      std::vector<double> sh(ns), rs(ns);
+      for (int j = 0; j < ns; j++) {
+        const int i = (s0 + j) / nshift, k = (s0 + j) % nshift;
+        sh[j] = shifts[k];
+        rs[j] = residues[k];
+        NQ_TRY(launch_convert(c, shifted(xb, j * csz), view_of(sols[(r0 + i) * nshift + k], 0)));
+        NQ_TRY(launch_convert(c, shifted(bb, j * csz), view_of(srcs[r0 + i], 0)));
+      }
+      std::vector<nq_solve_stats> sst(ns);
+      if (mixed) {
+        const int rr = st_cg_batched(c, g, ns, sh.data(), niter, rs.data(), NQ_PREC_F32, xb, bb, true, sst.data());
+        if (rr != NQ_OK && rr != NQ_ENOCONV) return rr;
+        if (rr == NQ_ENOCONV) rc = rr;
+        for (int j = 0; j < ns; j++) {
+          const int i = (s0 + j) / nshift, k = (s0 + j) % nshift;
+          NQ_TRY(launch_convert(c, view_of(sols[(r0 + i) * nshift + k], 0), shifted(xb, j * csz)));
+          if (st) {
+            st[r0 + i].reliable_updates += sst[j].reliable_updates;
+            st[r0 + i].iters += sst[j].iters;
+          }
+        }
+      }
+      // FP64 true residual |A(sigma) x - b|^2 / |b|^2
+      double* coef = nullptr;
+      NQ_TRY(ctx_temp(c, "cgm.coef", sizeof(double) * 32, (void**)&coef));
+      std::vector<double> hc(2 * ns);
+      for (int j = 0; j < ns; j++) { hc[2 * j] = -0.25; hc[2 * j + 1] = sh[j]; }
+      NQ_HIP(hipMemcpyAsync(coef, hc.data(), sizeof(double) * 2 * ns, hipMemcpyHostToDevice, c->stream));
+      SpinorView tt, ss;
+      NQ_TRY(alloc_spinor_nr(c, "cgm.tt", SP_F64, 3, ns, &tt));
+      NQ_TRY(alloc_spinor_nr(c, "cgm.ss", SP_F64, 3, ns, &ss));
+      NQ_TRY(apply_DDee(c, U64, 0, coef, ss, xb, tt, ns));
+      std::vector<CgState> z(ns);
+      NQ_TRY(put_states(c, 0, ns, z.data()));
+      SpinorView none;
+      none.ncomp = 3;
+      NQ_TRY(launch_lincomb(c, none, 1.0, bb, -1.0, ss, 0, none, FIN_STORE0, c->state, ns));
+      NQ_TRY(launch_lincomb(c, none, 1.0, bb, 0, none, 0, none, FIN_STORE1, c->state, ns));
+      NQ_TRY(get_states(c, 0, ns));
+      for (int j = 0; j < ns; j++) {
+        const int i = (s0 + j) / nshift, k = (s0 + j) % nshift;
+        const double tr = c->state_host[j].extra[0] / c->state_host[j].extra[1];
+        if (true_res_sq) true_res_sq[(r0 + i) * nshift + k] = tr;
+        if (st) st[r0 + i].rel_res_sq = std::max(st[r0 + i].rel_res_sq, tr);
+      }
+    }
+  }
+  (void)hipEventRecord(eb, c->stream);
+  (void)hipEventSynchronize(eb);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, ea, eb);
+  if (st)
+    for (int i = 0; i < nrhs; i++) {
+      st[i].seconds = ms * 1e-3;
+      st[i].converged = rc == NQ_OK;
+    }
+  (void)hipEventDestroy(ea);
+  (void)hipEventDestroy(eb);
+  if (rc != NQ_OK) return fail(rc, "exit without converging");
+  return NQ_OK;
+}
+
+}  // extern "C"

@@ -382,6 +382,311 @@ static int cg_kernsq_any(nq_ctx* ctx, const nq_gauge* g, double kappa, const dou
   return cg_kernsq_mixed(S64, Ss, sp, mu, niter, residue, 1e-2, x, guess, b, st);
 }
 
+static int apply_Q_views(const TmSetup& S, double mu, const nq_field* out, const nq_field* in)
+{
+  for (int p = 0; p < 2; p++)
+    NQ_TRY(launch_tm_hop(S.ctx, p, view_of(out, p), view_of(in, !p), S.U, S.anis, epi_q(S.kappa, mu, view_of(in, p))));
+  return NQ_OK;
+}
+
+// ---------------------------------------------------------------- batched solves
+// Coefficient tables of the four K K^+ epilogues (and the residual one) for nr
+// right-hand sides with per-RHS mu; one table is double[16][12] (c1, c2, c3).
+enum { TB_INVDEE_M = 0, TB_KOUT_M = 1, TB_INVDEE_P = 2, TB_KOUT_P = 3, TB_RES = 4, TB_N = 5 };
+
+static void put_coef(double* t, const HopEpi& e)
+{
+  for (int h = 0; h < 2; h++)
+    for (int k = 0; k < 2; k++) {
+      t[2 * h + k] = e.c1[h][k];
+      t[4 + 2 * h + k] = e.c2[h][k];
+      t[8 + 2 * h + k] = e.c3[h][k];
+    }
+}
+
+static int upload_kernsq_coefs(nq_ctx* ctx, double kappa, int nr, const double* mus, double** dev)
+{
+  NQ_TRY(ctx_temp(ctx, "bcg.coef", sizeof(double) * TB_N * 16 * 12, (void**)dev));
+  std::vector<double> h(TB_N * 16 * 12, 0.0);
+  SpinorView none;
+  for (int r = 0; r < nr; r++) {
+    const double mu = mus[r];
+    put_coef(&h[(TB_INVDEE_M * 16 + r) * 12], epi_inv_dee(kappa, -mu));
+    put_coef(&h[(TB_KOUT_M * 16 + r) * 12], epi_kout(kappa, -mu, none));
+    put_coef(&h[(TB_INVDEE_P * 16 + r) * 12], epi_inv_dee(kappa, mu));
+    put_coef(&h[(TB_KOUT_P * 16 + r) * 12], epi_kout(kappa, mu, none));
+    HopEpi e = epi_kout(kappa, mu, none);
+    for (int a = 0; a < 2; a++)
+      for (int k = 0; k < 2; k++) { e.c1[a][k] = -e.c1[a][k]; e.c2[a][k] = -e.c2[a][k]; }
+    setc(e.c3, 1.0, 1.0);
+    put_coef(&h[(TB_RES * 16 + r) * 12], e);
+  }
+  NQ_HIP(hipMemcpyAsync(*dev, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+  return NQ_OK;
+}
+
+static HopEpi bepi(const double* coef, int table, int nr, CgState* st, int pred)
+{
+  HopEpi e;
+  e.coef = coef + table * 16 * 12;
+  e.nrhs = nr;
+  e.batched = 1;
+  e.st = st;
+  e.pred_reliable = pred;
+  return e;
+}
+
+// r = b - K(+mu_r)K(-mu_r) x for nr right-hand sides (mode-3 tail; rs optional)
+static int residual_kernsq_batched(const TmSetup& S, const double* coef, int nr, const SpinorView& r,
+                                   const SpinorView& rs, const SpinorView& x, const SpinorView& b, const SpinorView& t1,
+                                   const SpinorView& tmp, CgState* st, int fin, int pred)
+{
+  nq_ctx* ctx = S.ctx;
+  NQ_TRY(launch_tm_hop(ctx, NQ_EVEN, tmp, x, S.U, S.anis, bepi(coef, TB_INVDEE_M, nr, st, pred)));
+  HopEpi k1 = bepi(coef, TB_KOUT_M, nr, st, pred);
+  k1.aux = x;
+  NQ_TRY(launch_tm_hop(ctx, NQ_ODD, t1, tmp, S.U, S.anis, k1));
+  NQ_TRY(launch_tm_hop(ctx, NQ_EVEN, tmp, t1, S.U, S.anis, bepi(coef, TB_INVDEE_P, nr, st, pred)));
+  HopEpi e = bepi(coef, TB_RES, nr, st, pred);
+  e.aux = t1;
+  e.aux2 = b;
+  e.mode = 3;
+  e.r = r;
+  e.rs = rs;
+  e.fin = fin;
+  SpinorView none;
+  none.ncomp = 12;
+  return launch_tm_hop(ctx, NQ_ODD, none, tmp, S.U, S.anis, e);
+}
+
+// CG on K(+mu_r) K(-mu_r) y_r = b_r for nr <= 16 right-hand sides sharing the gauge
+// field and kappa.  Every launch serves all right-hand sides; a right-hand side that
+// has converged (or waits for its reliable update) idles in the kernels.  The
+// arithmetic of one right-hand side does not depend on the others in the batch
+// (per-RHS coefficient rows, per-RHS states, per-(RHS, 64-site chunk) reduction
+// partials), so a solve gives the same bits alone or in any batch.
+// prec NQ_PREC_F64: FP64 CG; otherwise sloppy iterations with FP64 reliable updates.
+static int cg_kernsq_batched(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis, int nr,
+                             const double* mus, int niter, double residue, int prec, const SpinorView& x64,
+                             const SpinorView& b, nq_solve_stats* out)
+{
+  int sp, lk;
+  NQ_TRY(lk_of_prec(prec, &sp, &lk));
+  if (nr < 1 || nr > 16) return fail(NQ_EINVAL, "batched CG: 1..16 right-hand sides");
+  const bool mixed = prec != NQ_PREC_F64;
+  TmSetup S64, Ss;
+  NQ_TRY(tm_setup(ctx, g, LK_F64_18, anis, kappa, &S64));
+  if (mixed) NQ_TRY(tm_setup(ctx, g, lk, anis, kappa, &Ss));
+  else Ss = S64;
+  double* coef = nullptr;
+  NQ_TRY(upload_kernsq_coefs(ctx, kappa, nr, mus, &coef));
+  SpinorView r64, t64, tmp64, xs, rs, p, t1s, tmps;
+  NQ_TRY(alloc_spinor_nr(ctx, "bcg.r", SP_F64, 12, nr, &r64));
+  NQ_TRY(alloc_spinor_nr(ctx, "bcg.t1", SP_F64, 12, nr, &t64));
+  NQ_TRY(alloc_spinor_nr(ctx, "bcg.tmp", SP_F64, 12, nr, &tmp64));
+  if (mixed) {
+    NQ_TRY(alloc_spinor_nr(ctx, "bcg.xs", SP_F32, 12, nr, &xs));
+    NQ_TRY(alloc_spinor_nr(ctx, "bcg.rs", SP_F32, 12, nr, &rs));
+    NQ_TRY(alloc_spinor_nr(ctx, sp == SP_F16 ? "bcg.p16" : "bcg.p32", sp, 12, nr, &p));
+    NQ_TRY(alloc_spinor_nr(ctx, sp == SP_F16 ? "bcg.t16" : "bcg.t32", sp, 12, nr, &t1s));
+    NQ_TRY(alloc_spinor_nr(ctx, sp == SP_F16 ? "bcg.e16" : "bcg.e32", sp, 12, nr, &tmps));
+  } else {
+    NQ_TRY(alloc_spinor_nr(ctx, "bcg.p", SP_F64, 12, nr, &p));
+    xs = x64;
+    rs = r64;
+    t1s = t64;
+    tmps = tmp64;
+  }
+  Timer T(ctx->stream);
+  CgState* S = ctx->state;
+  SpinorView none;
+  none.ncomp = 12;
+  std::vector<CgState> hs(nr);
+  for (int i = 0; i < nr; i++) {
+    hs[i] = CgState{};
+    put_state(ctx, i, hs[i]);
+  }
+  NQ_TRY(launch_lincomb(ctx, none, 1.0, b, 0, none, 0, none, FIN_STORE0, S, nr));
+  NQ_TRY(get_states(ctx, 0, nr));
+  std::vector<double> b2(nr);
+  for (int i = 0; i < nr; i++) {
+    b2[i] = ctx->state_host[i].extra[0];
+    if (b2[i] == 0 || std::isnan(b2[i])) return fail(NQ_ENAN, "invalid norm: %g (right-hand side %d)", b2[i], i);
+    CgState s0 = {};
+    s0.b2 = b2[i];
+    s0.target = residue * b2[i];
+    s0.niter = niter;
+    s0.mixed = mixed;
+    s0.delta = 1e-2;
+    NQ_TRY(put_state(ctx, i, s0));
+  }
+  NQ_TRY(launch_zero(ctx, x64, nr));
+  NQ_TRY(launch_lincomb(ctx, r64, 1.0, b, 0, none, 0, none, FIN_RR_INIT, S, nr));
+  if (mixed) {
+    NQ_TRY(launch_convert(ctx, rs, r64, nr));
+    NQ_TRY(launch_zero(ctx, xs, nr));
+  }
+  NQ_TRY(launch_convert(ctx, p, rs, nr));
+  NQ_TRY(get_states(ctx, 0, nr));
+  std::vector<double> rr0(nr);
+  for (int i = 0; i < nr; i++) rr0[i] = ctx->state_host[i].rr;
+  const int pred = mixed ? 2 : 0;
+  int chunk = 8, launched = 0;
+  for (;;) {
+    for (int it = 0; it < chunk; it++) {
+      NQ_TRY(launch_tm_hop(ctx, NQ_EVEN, tmps, p, Ss.U, Ss.anis, bepi(coef, TB_INVDEE_M, nr, S, pred)));
+      HopEpi b1 = bepi(coef, TB_KOUT_M, nr, S, pred);
+      b1.aux = p;
+      b1.mode = 1;
+      b1.fin = FIN_PAP;
+      NQ_TRY(launch_tm_hop(ctx, NQ_ODD, t1s, tmps, Ss.U, Ss.anis, b1));
+      NQ_TRY(launch_tm_hop(ctx, NQ_EVEN, tmps, t1s, Ss.U, Ss.anis, bepi(coef, TB_INVDEE_P, nr, S, pred)));
+      HopEpi b2e = bepi(coef, TB_KOUT_P, nr, S, pred);
+      b2e.aux = t1s;
+      b2e.mode = 2;
+      b2e.fin = mixed ? FIN_RR_SLOPPY : FIN_RR;
+      b2e.x = xs;
+      b2e.p = p;
+      b2e.r = rs;
+      NQ_TRY(launch_tm_hop(ctx, NQ_ODD, none, tmps, Ss.U, Ss.anis, b2e));
+      NQ_TRY(launch_cg_pupdate(ctx, p, rs, S, nr));
+    }
+    launched += chunk;
+    NQ_TRY(get_states(ctx, 0, nr));
+    bool any_rel = false;
+    for (int i = 0; i < nr; i++) any_rel |= ctx->state_host[i].reliable && !ctx->state_host[i].done;
+    if (any_rel) {
+      // reliable update of the flagged right-hand sides: x64 += xs, xs = 0;
+      // r64 = b - A x64 in FP64, rs = r64; p = rs + beta p; then clear the flags
+      NQ_TRY(launch_reliable_accum(ctx, x64, xs, S, nr));
+      NQ_TRY(residual_kernsq_batched(S64, coef, nr, r64, rs, x64, b, t64, tmp64, S, FIN_RR_TRUE_HOLD, 1));
+      NQ_TRY(launch_cg_pupdate(ctx, p, rs, S, nr, 2));
+      NQ_TRY(launch_clear_reliable(ctx, S, nr));
+      NQ_TRY(get_states(ctx, 0, nr));
+    }
+    bool all = true;
+    int next = 64;
+    for (int i = 0; i < nr; i++) {
+      const CgState& h = ctx->state_host[i];
+      if (h.nan) return fail(NQ_ENAN, "nanned (right-hand side %d)", i);
+      if (h.done) continue;
+      all = false;
+      const double goal = mixed ? std::max(h.delta * h.delta * h.r0 * h.r0, h.target) : h.target;
+      next = std::min(next, next_chunk(rr0[i], h.rr, goal, h.iter, chunk));
+    }
+    if (all || launched > niter + 64) break;
+    chunk = std::max(2, next);
+  }
+  // FP64 true residual of every right-hand side (states 16..16+nr as scratch)
+  CgState* sc = ctx->state + 16;
+  for (int i = 0; i < nr; i++) NQ_TRY(put_state(ctx, 16 + i, CgState{}));
+  NQ_TRY(residual_kernsq_batched(S64, coef, nr, r64, SpinorView{}, x64, b, t64, tmp64, sc, FIN_STORE0, 0));
+  NQ_TRY(get_states(ctx, 0, 16 + nr));
+  const double secs = T.stop();
+  int rc = NQ_OK;
+  for (int i = 0; i < nr; i++) {
+    const CgState& h = ctx->state_host[i];
+    out[i] = nq_solve_stats{};
+    out[i].iters = h.iter;
+    out[i].reliable_updates = h.nreliable;
+    out[i].rel_res_sq = ctx->state_host[16 + i].extra[0] / b2[i];
+    out[i].converged = mixed ? h.true_rr < h.target : h.iter < niter;
+    out[i].seconds = secs;
+    if (!out[i].converged) rc = NQ_ENOCONV;
+  }
+  if (rc != NQ_OK) return fail(rc, "exit without converging");
+  return NQ_OK;
+}
+
+// CG on Q(-mu) Q(+mu) over lx (inv_tmQ2_cg), FP64, cg_invert semantics
+static int tmQ2_cg_f64(const TmSetup& S, double mu, int niter, double residue, const nq_field* sol,
+                       const nq_field* guess, const nq_field* src, nq_solve_stats* out)
+{
+  nq_ctx* ctx = S.ctx;
+  Timer T(ctx->stream);
+  // lx temporaries as fields (even half then odd half) so parity views apply
+  SpinorView rv, pv, tv, sv;
+  NQ_TRY(alloc_spinor_nr(ctx, "q2.r", SP_F64, 12, 2, &rv));
+  NQ_TRY(alloc_spinor_nr(ctx, "q2.p", SP_F64, 12, 2, &pv));
+  NQ_TRY(alloc_spinor_nr(ctx, "q2.t", SP_F64, 12, 2, &tv));
+  NQ_TRY(alloc_spinor_nr(ctx, "q2.s", SP_F64, 12, 2, &sv));
+  auto as_field = [&](const SpinorView& v) {
+    nq_field f;
+    f.ctx = ctx;
+    f.kind = NQ_SPINCOLOR;
+    f.coverage = NQ_LX;
+    f.buf.p = v.data;
+    return f;
+  };
+  nq_field fr = as_field(rv), fp = as_field(pv), ft = as_field(tv), fs = as_field(sv);
+  struct Release {
+    nq_field* f[4];
+    ~Release()
+    {
+      for (auto* x : f) x->buf.p = nullptr;
+    }
+  } rel{{&fr, &fp, &ft, &fs}};
+  const SpinorView x = view_lx_flat(sol), b = view_lx_flat(src), r = view_lx_flat(&fr), p = view_lx_flat(&fp),
+                   s = view_lx_flat(&fs);
+  SpinorView none;
+  none.ncomp = 24;
+  CgState* st = ctx->state;
+  NQ_TRY(put_state(ctx, 0, CgState{}));
+  NQ_TRY(launch_lincomb(ctx, none, 1.0, b, 0, none, 0, none, FIN_STORE0, st));
+  NQ_TRY(get_states(ctx, 0, 1));
+  const double b2 = ctx->state_host[0].extra[0];
+  if (b2 == 0 || std::isnan(b2)) return fail(NQ_ENAN, "invalid norm: %g", b2);
+  CgState s0 = {};
+  s0.b2 = b2;
+  s0.target = residue * b2;
+  s0.niter = niter;
+  NQ_TRY(put_state(ctx, 0, s0));
+  auto q2 = [&](const nq_field* out_f, const nq_field* in_f) -> int {
+    NQ_TRY(apply_Q_views(S, +mu, &ft, in_f));
+    return apply_Q_views(S, -mu, out_f, &ft);
+  };
+  if (guess) {
+    if (guess != sol) NQ_TRY(launch_convert(ctx, x, view_lx_flat(guess)));
+    NQ_TRY(q2(&fs, sol));
+    NQ_TRY(launch_lincomb(ctx, r, 1.0, b, -1.0, s, 0, none, FIN_RR_INIT, st));
+  } else {
+    NQ_TRY(launch_zero(ctx, x));
+    NQ_TRY(launch_lincomb(ctx, r, 1.0, b, 0, none, 0, none, FIN_RR_INIT, st));
+  }
+  NQ_TRY(launch_convert(ctx, p, r));
+  NQ_TRY(get_states(ctx, 0, 1));
+  const double rr0 = ctx->state_host[0].rr;
+  int chunk = 8, launched = 0;
+  while (!ctx->state_host[0].done) {
+    for (int i = 0; i < chunk; i++) {
+      // s = Q(-mu) Q(+mu) p ; <p, s> = |Q(+mu) p|^2
+      NQ_TRY(apply_Q_views(S, +mu, &ft, &fp));
+      NQ_TRY(launch_lincomb(ctx, none, 1.0, view_lx_flat(&ft), 0, none, 0, none, FIN_PAP, st, 1, 1));
+      NQ_TRY(apply_Q_views(S, -mu, &fs, &ft));
+      NQ_TRY(launch_cg_update(ctx, x, p, r, s, st, FIN_RR));
+      NQ_TRY(launch_cg_pupdate(ctx, p, r, st));
+    }
+    launched += chunk;
+    NQ_TRY(get_states(ctx, 0, 1));
+    const CgState& h = ctx->state_host[0];
+    chunk = next_chunk(rr0, h.rr, h.target, h.iter, chunk);
+    if (launched > niter + 64) break;
+  }
+  const CgState h = ctx->state_host[0];
+  if (h.nan) return fail(NQ_ENAN, "nanned");
+  // last calculation of the residual
+  NQ_TRY(q2(&fs, sol));
+  NQ_TRY(put_state(ctx, 1, CgState{}));
+  NQ_TRY(launch_lincomb(ctx, none, 1.0, b, -1.0, s, 0, none, FIN_STORE0, st + 1));
+  NQ_TRY(get_states(ctx, 1, 1));
+  out->iters = h.iter;
+  out->rel_res_sq = ctx->state_host[1].extra[0] / b2;
+  out->converged = h.iter < niter;
+  out->seconds = T.stop();
+  if (!out->converged) return fail(NQ_ENOCONV, "exit without converging (%d iterations)", h.iter);
+  return NQ_OK;
+}
+
 }  // namespace nq
 
 using namespace nq;
@@ -469,12 +774,6 @@ int nq_tm_kern_square(nq_ctx* c, const nq_gauge* g, double kappa, const double* 
   return NQ_OK;
 }
 
-static int apply_Q_views(const TmSetup& S, double mu, const nq_field* out, const nq_field* in)
-{
-  for (int p = 0; p < 2; p++)
-    NQ_TRY(launch_tm_hop(S.ctx, p, view_of(out, p), view_of(in, !p), S.U, S.anis, epi_q(S.kappa, mu, view_of(in, p))));
-  return NQ_OK;
-}
 
 int nq_tm_apply_Q(nq_ctx* c, const nq_gauge* g, double kappa, const double* anis, double mu, nq_field* out,
                   const nq_field* in)
@@ -598,6 +897,92 @@ int nq_tm_solve_eoprec(nq_ctx* c, const nq_gauge* g, double kappa, const double*
     *check = (n2[0] + n2[1]) / s2;
   }
   NQ_HIP(hipStreamSynchronize(c->stream));
+  return rc == NQ_ENOCONV ? fail(NQ_ENOCONV, "exit without converging") : NQ_OK;
+}
+
+int nq_tmQ2_cg(nq_ctx* c, const nq_gauge* g, double kappa, double mu, int niter, double residue, nq_field* sol,
+               const nq_field* guess, const nq_field* src, nq_solve_stats* st)
+{
+  if (!c) return fail(NQ_EINVAL, "null ctx");
+  NQ_TRY(check_field(c, src, NQ_SPINCOLOR, NQ_LX, "nq_tmQ2_cg"));
+  NQ_TRY(check_field(c, sol, NQ_SPINCOLOR, NQ_LX, "nq_tmQ2_cg"));
+  if (guess) NQ_TRY(check_field(c, guess, NQ_SPINCOLOR, NQ_LX, "nq_tmQ2_cg"));
+  if (sol == src) return fail(NQ_EINVAL, "sol==src");
+  if (niter <= 0 || !(residue > 0)) return fail(NQ_EINVAL, "bad niter/residue");
+  Guard G(c);
+  TmSetup S;
+  NQ_TRY(tm_setup(c, g, LK_F64_18, nullptr, kappa, &S));
+  nq_solve_stats local;
+  if (!st) st = &local;
+  std::memset(st, 0, sizeof *st);
+  return tmQ2_cg_f64(S, mu, niter, residue, sol, guess, src, st);
+}
+
+// inv_tmD_cg_eoprec for several sources at once: eqs. 8 and 10-11 per source, the
+// K K^+ CG batched over up to 16 sources per group
+int nq_tm_solve_eoprec_multi(nq_ctx* c, const nq_gauge* g, double kappa, const double* anis, int nrhs,
+                             const double* mus, int niter, double residue, int prec, nq_field** sols,
+                             const nq_field* const* srcs, nq_solve_stats* st)
+{
+  if (!c || !mus || !sols || !srcs) return fail(NQ_EINVAL, "null argument");
+  if (nrhs < 1) return fail(NQ_EINVAL, "need at least one source");
+  if (niter <= 0 || !(residue > 0)) return fail(NQ_EINVAL, "bad niter/residue");
+  for (int i = 0; i < nrhs; i++) {
+    NQ_TRY(check_field(c, srcs[i], NQ_SPINCOLOR, NQ_LX, "nq_tm_solve_eoprec_multi"));
+    NQ_TRY(check_field(c, sols[i], NQ_SPINCOLOR, NQ_LX, "nq_tm_solve_eoprec_multi"));
+    for (int j = 0; j < nrhs; j++)
+      if (sols[i] == srcs[j]) return fail(NQ_EINVAL, "solution aliases a source");
+  }
+  Guard G(c);
+  TmSetup S;
+  NQ_TRY(tm_setup(c, g, LK_F64_18, anis, kappa, &S));
+  const double a = 1 / (2 * kappa);
+  const int64_t fsz = 12 * c->geo.stride;  // elements of one parity of one source
+  std::vector<nq_solve_stats> local(nrhs);
+  int rc = NQ_OK;
+  for (int r0 = 0; r0 < nrhs; r0 += 16) {
+    const int nr = std::min(16, nrhs - r0);
+    SpinorView et, phi, y;
+    NQ_TRY(alloc_spinor(c, "eo.et", SP_F64, 12, &et));
+    NQ_TRY(alloc_spinor_nr(c, "beo.phi", SP_F64, 12, nr, &phi));
+    NQ_TRY(alloc_spinor_nr(c, "beo.y", SP_F64, 12, nr, &y));
+    SpinorView none;
+    none.ncomp = 12;
+    for (int i = 0; i < nr; i++) {
+      const double mu = mus[r0 + i], n = a * a + mu * mu;
+      double ca[2][2], cb[2][2] = {{0, 0}, {0, 0}};
+      setc(ca, cplx(a / n, -mu / n), cplx(a / n, mu / n));
+      NQ_TRY(launch_spin_diag(c, et, ca, view_of(srcs[r0 + i], 0), cb, none));
+      HopEpi e8;
+      setc(e8.c1, 0.5, -0.5);
+      setc(e8.c2, 1.0, -1.0);
+      e8.aux = view_of(srcs[r0 + i], 1);
+      SpinorView ph = phi;
+      ph.data = (char*)phi.data + i * fsz * 16;
+      ph.rstride = 0;
+      NQ_TRY(launch_tm_hop(c, NQ_ODD, ph, et, S.U, S.anis, e8));
+    }
+    const int rci = cg_kernsq_batched(c, g, kappa, anis, nr, mus + r0, niter, residue, prec, y, phi, &local[r0]);
+    if (rci != NQ_OK && rci != NQ_ENOCONV) return rci;
+    if (rci == NQ_ENOCONV) rc = rci;
+    for (int i = 0; i < nr; i++) {
+      const double mu = mus[r0 + i], n = a * a + mu * mu;
+      const cplx zi_u(a / n, -mu / n), zi_l(a / n, mu / n);
+      nq_field* sol = sols[r0 + i];
+      SpinorView yi = y;
+      yi.data = (char*)y.data + i * fsz * 16;
+      yi.rstride = 0;
+      NQ_TRY(apply_kern(S, -mu, view_of(sol, 1), yi, view_of(sol, 0)));
+      HopEpi e11;
+      setc(e11.c1, 0.5 * zi_u, 0.5 * zi_l);
+      setc(e11.c2, zi_u, zi_l);
+      e11.aux = view_of(srcs[r0 + i], 0);
+      NQ_TRY(launch_tm_hop(c, NQ_EVEN, view_of(sol, 0), view_of(sol, 1), S.U, S.anis, e11));
+    }
+  }
+  NQ_HIP(hipStreamSynchronize(c->stream));
+  if (st)
+    for (int i = 0; i < nrhs; i++) st[i] = local[i];
   return rc == NQ_ENOCONV ? fail(NQ_ENOCONV, "exit without converging") : NQ_OK;
 }
 

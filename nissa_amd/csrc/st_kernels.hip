@@ -1,0 +1,349 @@
+// Staggered operator and multi-shift CG kernels for gfx950.
+//
+// Reference: apply_st2Doe / apply_stDeo_half / apply_stD2ee_m2
+// (src/dirac_operators/stD/dirac_operator_stD.hpp:47-225) and cgm_invert
+// (src/inverters/templates/cgm_invert_template_threaded.hpp:13-295).
+//   hop(x) = sum_mu U_mu(x) in(x+mu) - U_mu(x-mu)^+ in(x-mu)   (staggered phases in U)
+//   A      = m2 - 1/4 Deo Doe  on even sites
+// Links come from the same per-output-parity slot arrays as the twisted-mass
+// hop (slot 2mu+1 already holds U_mu(x-mu)^+).  Colour fields are SoA with three
+// complex components; all kernels take the (64-site chunk x right-hand side)
+// mapping so one launch serves every source of a batch.
+//
+// CGM iteration (per source, shifts k in the running set):
+//   C1: t = Doe p,                |t|^2 -> pap = |t|^2/4   (= <p, A(0) p>, Doe^+ = -Deo)
+//   C2: s = -1/4 Deo t; sol_k -= beta_k ps_k; r += betaa s;  |r|^2
+//   C3: p = r + alpha p; ps_k = zeta_k r + alpha_k ps_k
+// The scalar recurrences of the reference (zeta/beta/alpha per shift, the
+// per-shift residual rr*zeta_k^2/|b|^2 < res_k stop) run in the finalisation
+// kernels on the device.
+
+#include "nq_device.hpp"
+
+namespace nq {
+
+template <int SP>
+using RealOfS = typename std::conditional<SP == SP_F64, double, float>::type;
+
+template <int SP, int LK, class R, int MU, bool FWD, bool PIN>
+__device__ __forceinline__ void st_slot(const Geo& g, const Site& s, uint32_t e, const SpinorView& in, int r,
+                                        const void* ub, cx<R> o[3])
+{
+  const uint32_t n = neigh_eo(g, s, MU, FWD);
+  cx<R> v[3];
+  SpAcc<SP, R, 3>::load(in, g.stride, n, v, r);
+  cx<R> u[9];
+  constexpr int slot = 2 * MU + (FWD ? 0 : 1);
+  const double det[2] = {1, 0};
+  LkAcc<LK, R>::load(ub, g.stride, slot, e, det, u);
+  cx<R> w[3];
+  mat_vec(u, v, w);
+#pragma unroll
+  for (int c = 0; c < 3; c++) o[c] = FWD ? o[c] + w[c] : o[c] - w[c];
+  if constexpr (PIN) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) asm volatile("" : "+v"(o[c].re), "+v"(o[c].im));
+    asm volatile("" ::: "memory");
+  }
+}
+
+template <int SP, int LK, class R>
+__device__ __forceinline__ void st_hop_site(const Geo& g, int op, uint32_t e, const SpinorView& in, int r,
+                                            const LinkView& U, cx<R> o[3])
+{
+  const Site s = site_of(g, op, e);
+  const void* ub = U.data[op];
+#pragma unroll
+  for (int c = 0; c < 3; c++) o[c] = mk<R>(0, 0);
+  st_slot<SP, LK, R, 0, true, false>(g, s, e, in, r, ub, o);
+  st_slot<SP, LK, R, 0, false, false>(g, s, e, in, r, ub, o);
+  st_slot<SP, LK, R, 1, true, false>(g, s, e, in, r, ub, o);
+  st_slot<SP, LK, R, 1, false, true>(g, s, e, in, r, ub, o);
+  st_slot<SP, LK, R, 2, true, false>(g, s, e, in, r, ub, o);
+  st_slot<SP, LK, R, 2, false, false>(g, s, e, in, r, ub, o);
+  st_slot<SP, LK, R, 3, true, false>(g, s, e, in, r, ub, o);
+  st_slot<SP, LK, R, 3, false, true>(g, s, e, in, r, ub, o);
+}
+
+// out = c1 hop + c2 aux (real coefficients, per rhs from coef[2r], coef[2r+1] when
+// tabulated). MODE 0: write; 1: write (if write) + |out|^2; 4: write (if write) + Re<aux,out>.
+template <int SP, int LK, int MODE>
+__global__ void __launch_bounds__(1024) k_st_hop(Geo g, int op, SpinorView out, SpinorView in, LinkView U, double c1,
+                                                 double c2, const double* coef, SpinorView aux, int write, int W,
+                                                 int NR, int64_t nchunks, double* partial, const int* skip_flags,
+                                                 int skip_stride, int skip_n)
+{
+  using R = RealOfS<SP>;
+  const Map m = map_thread(W, NR, g.Vh);
+  const bool skip = skip_flags && (skip_flags[m.r * skip_stride] || (skip_n > 1 && skip_flags[m.r * skip_stride + 1]));
+  double red = 0;
+  if (m.active && !skip) {
+    cx<R> o[3];
+    st_hop_site<SP, LK, R>(g, op, m.e, in, m.r, U, o);
+    const double a = coef ? coef[2 * m.r] : c1, b = coef ? coef[2 * m.r + 1] : c2;
+#pragma unroll
+    for (int c = 0; c < 3; c++) o[c] = scal(o[c], (R)a);
+    if (aux.data) {
+      cx<R> x[3];
+      SpAcc<SP, R, 3, kNT>::load(aux, g.stride, m.e, x, m.r);
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        o[c].re = fma((R)b, x[c].re, o[c].re);
+        o[c].im = fma((R)b, x[c].im, o[c].im);
+        if constexpr (MODE == 4) red += (double)x[c].re * o[c].re + (double)x[c].im * o[c].im;
+      }
+    }
+    if (MODE == 0 || write) SpAcc<SP, R, 3, kNT>::store(out, g.stride, m.e, o, m.r);
+    if constexpr (MODE == 1) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) red += (double)o[c].re * o[c].re + (double)o[c].im * o[c].im;
+    }
+  }
+  if constexpr (MODE != 0) store_partial(red, partial, nchunks, m.r, m.chunk);
+}
+
+// ------------------------------------------------------------------ CGM
+// C2: s = -1/4 Deo t (even output, never stored) ; sol_k -= betas_k ps_k ; r += betaa s ; |r|^2
+// sol/ps of shift k of rhs r: base + r*rstride + k*sstride (elements)
+template <int SP, int LK>
+__global__ void __launch_bounds__(1024) k_cgm_hop2(Geo g, SpinorView t, LinkView U, SpinorView rv, SpinorView sol,
+                                                   SpinorView ps, int64_t sstride, CgmState* st, int W, int NR,
+                                                   int64_t nchunks, double* partial)
+{
+  using R = RealOfS<SP>;
+  const Map m = map_thread(W, NR, g.Vh);
+  const CgmState* S = st + m.r;
+  double red = 0;
+  if (m.active && !S->done) {
+    cx<R> o[3];
+    st_hop_site<SP, LK, R>(g, 0, m.e, t, m.r, U, o);
+    const R betaa = (R)S->betaa;
+    for (int k = 0; k < S->nshift; k++) {
+      if (!S->run[k]) continue;
+      SpinorView sk = sol, pk = ps;
+      sk.data = (char*)sol.data + k * sstride * (SP == SP_F64 ? 16 : 8);
+      pk.data = (char*)ps.data + k * sstride * (SP == SP_F64 ? 16 : 8);
+      cx<R> x[3], q[3];
+      SpAcc<SP, R, 3, kNT>::load(sk, g.stride, m.e, x, m.r);
+      SpAcc<SP, R, 3, kNT>::load(pk, g.stride, m.e, q, m.r);
+      const R b = (R)S->betas[k];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        x[c].re = fma(-b, q[c].re, x[c].re);
+        x[c].im = fma(-b, q[c].im, x[c].im);
+      }
+      SpAcc<SP, R, 3, kNT>::store(sk, g.stride, m.e, x, m.r);
+    }
+    cx<R> rr[3];
+    SpAcc<SP, R, 3, kNT>::load(rv, g.stride, m.e, rr, m.r);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const cx<R> sv = scal(o[c], (R)-0.25);
+      rr[c].re = fma(betaa, sv.re, rr[c].re);
+      rr[c].im = fma(betaa, sv.im, rr[c].im);
+      red += (double)rr[c].re * rr[c].re + (double)rr[c].im * rr[c].im;
+    }
+    SpAcc<SP, R, 3, kNT>::store(rv, g.stride, m.e, rr, m.r);
+  }
+  store_partial(red, partial, nchunks, m.r, m.chunk);
+}
+
+// C3: p = r + alpha p ; ps_k = zfs_k r + alphas_k ps_k for running shifts
+template <int SP>
+__global__ void __launch_bounds__(1024) k_cgm_update(Geo g, SpinorView rv, SpinorView p, SpinorView ps,
+                                                     int64_t sstride, CgmState* st, int W, int NR)
+{
+  using R = RealOfS<SP>;
+  const Map m = map_thread(W, NR, g.Vh);
+  const CgmState* S = st + m.r;
+  if (!m.active || S->done) return;
+  cx<R> rr[3], pp[3];
+  SpAcc<SP, R, 3>::load(rv, g.stride, m.e, rr, m.r);
+  SpAcc<SP, R, 3>::load(p, g.stride, m.e, pp, m.r);
+  const R a = (R)S->alpha;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    pp[c].re = fma(a, pp[c].re, rr[c].re);
+    pp[c].im = fma(a, pp[c].im, rr[c].im);
+  }
+  SpAcc<SP, R, 3>::store(p, g.stride, m.e, pp, m.r);
+  for (int k = 0; k < S->nshift; k++) {
+    if (!S->run[k]) continue;
+    SpinorView pk = ps;
+    pk.data = (char*)ps.data + k * sstride * (SP == SP_F64 ? 16 : 8);
+    cx<R> q[3];
+    SpAcc<SP, R, 3, kNT>::load(pk, g.stride, m.e, q, m.r);
+    const R z = (R)S->zfs[k], al = (R)S->alphas[k];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      q[c].re = fma(al, q[c].re, z * rr[c].re);
+      q[c].im = fma(al, q[c].im, z * rr[c].im);
+    }
+    SpAcc<SP, R, 3, kNT>::store(pk, g.stride, m.e, q, m.r);
+  }
+}
+
+enum { FIN_CGM_PAP = 1, FIN_CGM_RFRF = 2 };
+
+// recurrences of cgm_invert_template_threaded.hpp:118-141 (pap) and :183-252 (rfrf)
+__device__ inline void cgm_finalize(CgmState* s, int fin, double total)
+{
+  if (fin == FIN_CGM_PAP) {
+    const double pap = 0.25 * total;
+    s->pap = pap;
+    s->betap = s->betaa;
+    s->betaa = -s->rr / pap;
+    for (int k = 0; k < s->nshift; k++) {
+      if (!s->run[k]) continue;
+      const double ratio = s->betap / (s->betaa * s->alpha * (1 - s->zas[k] / s->zps[k]) +
+                                       s->betap * (1 - s->shift[k] * s->betaa));
+      if (isnan(ratio)) { s->nan = 1; s->done = 1; }
+      s->zfs[k] = s->zas[k] * ratio;
+      s->betas[k] = s->betaa * ratio;
+    }
+  } else {
+    const double rfrf = total;
+    s->rfrf = rfrf;
+    s->alpha = rfrf / s->rr;
+    for (int k = 0; k < s->nshift; k++) {
+      if (!s->run[k]) continue;
+      s->alphas[k] = s->alpha * s->zfs[k] * s->betas[k] / (s->zas[k] * s->betaa);
+      s->zps[k] = s->zas[k];
+      s->zas[k] = s->zfs[k];
+    }
+    s->rr = rfrf;
+    s->iter += 1;
+    for (int k = 0; k < s->nshift; k++) {
+      if (!s->run[k]) continue;
+      s->res[k] = s->rr * s->zfs[k] * s->zfs[k] / s->b2;
+      if (s->res[k] < s->target[k]) {
+        s->run[k] = 0;
+        s->nrun -= 1;
+      }
+    }
+    if (isnan(rfrf)) { s->nan = 1; s->done = 1; }
+    if (s->nrun <= 0 || s->iter >= s->niter) s->done = 1;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_cgm_finalize(const double* partial, int64_t nchunks, int nr, CgmState* st,
+                                                      int fin)
+{
+  __shared__ double vl[256];
+  const int vt = threadIdx.x;
+  for (int rr = 0; rr < nr; rr++) {
+    CgmState* s = st + rr;
+    if (s->done) continue;
+    const double* pr = partial + (int64_t)rr * nchunks;
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    int64_t c = vt;
+    for (; c + 768 < nchunks; c += 1024) {
+      a0 += pr[c];
+      a1 += pr[c + 256];
+      a2 += pr[c + 512];
+      a3 += pr[c + 768];
+    }
+    for (; c < nchunks; c += 256) a0 += pr[c];
+    vl[vt] = (a0 + a1) + (a2 + a3);
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+      if (vt < h) vl[vt] += vl[vt + h];
+      __syncthreads();
+    }
+    if (vt == 0) cgm_finalize(s, fin, vl[0]);
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+static double csite(int sp) { return sp == SP_F64 ? 48.0 : 24.0; }
+static double lbytes(int lk) { return lk == LK_F64_18 ? 144.0 : 72.0; }
+
+int launch_st_hop(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, const LinkView& U, double c1,
+                  double c2, const double* coef_dev, const SpinorView& aux, int mode, int write, int nr, int fin,
+                  CgState* st, const int* skip_flags, int skip_stride, int skip_n)
+{
+  const Geo& g = ctx->geo;
+  if (in.ncomp != 3) return fail(NQ_EINVAL, "staggered hop needs colour fields");
+  if (nr < 1 || nr > 16) return fail(NQ_EINVAL, "right-hand sides per launch must be 1..16");
+  const Launch L = launch_shape(g, nr);
+  const int sp = in.prec, lk = U.prec;
+  if ((mode == 0 || write) && out.prec != sp) return fail(NQ_EINVAL, "st hop: output precision mismatch");
+  if (aux.data && aux.prec != sp) return fail(NQ_EINVAL, "st hop: aux precision mismatch");
+  hipEvent_t ev = nullptr;
+  const double by = (8 * lbytes(lk) + csite(sp) * (1 + (aux.data ? 1 : 0) + ((mode == 0 || write) ? 1 : 0))) * nr;
+  prof_begin(ctx, mode == 0 ? "st_hop" : "st_hop_red", by * (double)g.Vh, &ev);
+  const dim3 grid((unsigned)L.blocks), blk(L.threads);
+#define NQ_ST(SPV, LKV)                                                                                            \
+  switch (mode) {                                                                                                  \
+  case 0: hipLaunchKernelGGL((k_st_hop<SPV, LKV, 0>), grid, blk, 0, ctx->stream, g, op, out, in, U, c1, c2, coef_dev, \
+                             aux, write, L.W, nr, L.nchunks, ctx->partial, skip_flags, skip_stride, skip_n); break;         \
+  case 1: hipLaunchKernelGGL((k_st_hop<SPV, LKV, 1>), grid, blk, 0, ctx->stream, g, op, out, in, U, c1, c2, coef_dev, \
+                             aux, write, L.W, nr, L.nchunks, ctx->partial, skip_flags, skip_stride, skip_n); break;         \
+  default: hipLaunchKernelGGL((k_st_hop<SPV, LKV, 4>), grid, blk, 0, ctx->stream, g, op, out, in, U, c1, c2, coef_dev, \
+                              aux, write, L.W, nr, L.nchunks, ctx->partial, skip_flags, skip_stride, skip_n); break;        \
+  }
+  if (sp == SP_F64 && lk == LK_F64_18) { NQ_ST(SP_F64, LK_F64_18) }
+  else if (sp == SP_F32 && lk == LK_F32_18) { NQ_ST(SP_F32, LK_F32_18) }
+  else return fail(NQ_EINVAL, "st hop: unsupported precision combination");
+#undef NQ_ST
+  NQ_HIP(hipGetLastError());
+  prof_end(ctx, ev);
+  if (mode != 0 && fin != FIN_NONE) NQ_TRY(launch_finalize(ctx, nr, st, fin, skip_n > 1 ? 2 : 0));
+  return NQ_OK;
+}
+
+int cgm_state_bytes() { return (int)sizeof(CgmState); }
+
+int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, const SpinorView& t,
+                         const SpinorView& r, const SpinorView& sol, const SpinorView& ps, int64_t sstride,
+                         void* states, int nr)
+{
+  const Geo& g = ctx->geo;
+  const Launch L = launch_shape(g, nr);
+  CgmState* st = (CgmState*)states;
+  const int sp = p.prec, lk = U.prec;
+  const dim3 grid((unsigned)L.blocks), blk(L.threads);
+  // C1: t = Doe p ; |t|^2
+  SpinorView none;
+  none.ncomp = 3;
+  hipEvent_t ev = nullptr;
+  prof_begin(ctx, "cgm_hop1", (8 * lbytes(lk) + 2 * csite(sp)) * nr * (double)g.Vh, &ev);
+  if (sp == SP_F64)
+    hipLaunchKernelGGL((k_st_hop<SP_F64, LK_F64_18, 1>), grid, blk, 0, ctx->stream, g, 1, t, p, U, 1.0, 0.0, nullptr,
+                       none, 1, L.W, nr, L.nchunks, ctx->partial, (const int*)&st->done,
+                       (int)(sizeof(CgmState) / sizeof(int)), 1);
+  else
+    hipLaunchKernelGGL((k_st_hop<SP_F32, LK_F32_18, 1>), grid, blk, 0, ctx->stream, g, 1, t, p, U, 1.0, 0.0, nullptr,
+                       none, 1, L.W, nr, L.nchunks, ctx->partial, (const int*)&st->done,
+                       (int)(sizeof(CgmState) / sizeof(int)), 1);
+  NQ_HIP(hipGetLastError());
+  prof_end(ctx, ev);
+  hipLaunchKernelGGL(k_cgm_finalize, dim3(1), dim3(256), 0, ctx->stream, ctx->partial, L.nchunks, nr, st,
+                     (int)FIN_CGM_PAP);
+  NQ_HIP(hipGetLastError());
+  // C2
+  prof_begin(ctx, "cgm_hop2_update", (8 * lbytes(lk) + 3 * csite(sp)) * nr * (double)g.Vh, &ev);
+  if (sp == SP_F64)
+    hipLaunchKernelGGL((k_cgm_hop2<SP_F64, LK_F64_18>), grid, blk, 0, ctx->stream, g, t, U, r, sol, ps, sstride, st,
+                       L.W, nr, L.nchunks, ctx->partial);
+  else
+    hipLaunchKernelGGL((k_cgm_hop2<SP_F32, LK_F32_18>), grid, blk, 0, ctx->stream, g, t, U, r, sol, ps, sstride, st,
+                       L.W, nr, L.nchunks, ctx->partial);
+  NQ_HIP(hipGetLastError());
+  prof_end(ctx, ev);
+  hipLaunchKernelGGL(k_cgm_finalize, dim3(1), dim3(256), 0, ctx->stream, ctx->partial, L.nchunks, nr, st,
+                     (int)FIN_CGM_RFRF);
+  NQ_HIP(hipGetLastError());
+  // C3
+  prof_begin(ctx, "cgm_update", 4 * csite(sp) * nr * (double)g.Vh, &ev);
+  if (sp == SP_F64)
+    hipLaunchKernelGGL(k_cgm_update<SP_F64>, grid, blk, 0, ctx->stream, g, r, p, ps, sstride, st, L.W, nr);
+  else
+    hipLaunchKernelGGL(k_cgm_update<SP_F32>, grid, blk, 0, ctx->stream, g, r, p, ps, sstride, st, L.W, nr);
+  NQ_HIP(hipGetLastError());
+  prof_end(ctx, ev);
+  return NQ_OK;
+}
+
+}  // namespace nq

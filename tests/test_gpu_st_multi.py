@@ -1,0 +1,171 @@
+"""GPU parity of the staggered operators, staggered CG / multi-shift CGM, the
+legacy lx CG on Q^2 and the multi-source eo solve, against the oracle through
+the C ABI.
+
+Bars: operator applies match the FP64 oracle to <= 1e-14 relative L2; solves
+reach the requested FP64 true residual and agree with the oracle's solution to
+1e-10 relative L2 when both run to residue 1e-24; a solve gives the same bits
+alone and inside a batch of other sources (per-source arithmetic and
+reductions do not depend on the batch).
+"""
+import numpy as np
+import pytest
+
+import nissa_amd as nq
+from helpers import gauss_eo, rel_l2, st_conf, tm_conf
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+ST_SHAPES = [(8, 8, 8, 8), (4, 6, 4, 8), (6, 4, 8, 2)]
+
+
+@pytest.fixture(scope="module")
+def st_setups():
+    cache = {}
+
+    def get(L, seed=1):
+        key = (L, seed)
+        if key not in cache:
+            conf = st_conf(L, seed)
+            ctx = nq.Ctx(L)
+            g = nq.Gauge(ctx, lx=conf)
+            geo = orc.Geom(L)
+            ce, co = geo.split(conf)
+            cache[key] = (ctx, g, geo, conf, ce, co)
+        return cache[key]
+
+    return get
+
+
+@pytest.mark.parametrize("L", ST_SHAPES)
+def test_st_operators(st_setups, L):
+    ctx, g, geo, conf, ce, co = st_setups(L)
+    xe = gauss_eo(geo.Vh, 11, ncomp=3)
+    xo = gauss_eo(geo.Vh, 12, ncomp=3)
+    fe = ctx.field(nq.COLOR, nq.EVEN, xe)
+    fo = ctx.field(nq.COLOR, nq.ODD, xo)
+    oo = ctx.field(nq.COLOR, nq.ODD)
+    oe = ctx.field(nq.COLOR, nq.EVEN)
+    nq.st_2Doe(ctx, g, oo, fe)
+    assert rel_l2(oo.download(), geo.st_2Doe(ce, co, xe)) < 1e-14
+    nq.st_Deo_half(ctx, g, oe, fo)
+    assert rel_l2(oe.download(), geo.st_Deo_half(ce, co, xo)) < 1e-14
+    for m2 in (0.0, 0.01, 0.37):
+        nq.st_DDee(ctx, g, m2, oe, fe)
+        assert rel_l2(oe.download(), geo.stD2ee_m2(ce, co, m2, xe)) < 1e-14
+
+
+def test_st_DDee_is_hermitian_positive(st_setups):
+    """<y, A x> = <A y, x> and <x, A x> >= m2 |x|^2 on the device operator."""
+    L = (8, 8, 8, 8)
+    ctx, g, geo, conf, ce, co = st_setups(L)
+    x = gauss_eo(geo.Vh, 21, ncomp=3)
+    y = gauss_eo(geo.Vh, 22, ncomp=3)
+    fx, fy = ctx.field(nq.COLOR, nq.EVEN, x), ctx.field(nq.COLOR, nq.EVEN, y)
+    ax, ay = ctx.field(nq.COLOR, nq.EVEN), ctx.field(nq.COLOR, nq.EVEN)
+    m2 = 0.04
+    nq.st_DDee(ctx, g, m2, ax, fx)
+    nq.st_DDee(ctx, g, m2, ay, fy)
+    cx = lambda a: a[..., 0] + 1j * a[..., 1]
+    lhs = np.vdot(cx(y), cx(ax.download()))
+    rhs = np.vdot(cx(ay.download()), cx(x))
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+    assert np.vdot(cx(x), cx(ax.download())).real >= m2 * np.vdot(cx(x), cx(x)).real
+
+
+@pytest.mark.parametrize("L", ST_SHAPES[:2])
+def test_st_cg_matches_oracle(st_setups, L):
+    ctx, g, geo, conf, ce, co = st_setups(L)
+    src = gauss_eo(geo.Vh, 31, ncomp=3)
+    m2 = 0.0025
+    ref, rst = geo.st_cg(ce, co, m2, 10000, 1e-24, src)
+    fb = ctx.field(nq.COLOR, nq.EVEN, src)
+    fx = ctx.field(nq.COLOR, nq.EVEN)
+    st = nq.st_cg(ctx, g, m2, 10000, 1e-24, fx, fb)
+    assert st["converged"] and st["rel_res_sq"] < 2e-24
+    assert rel_l2(fx.download(), ref) < 1e-10
+    assert abs(st["iters"] - rst["iters"]) <= 2
+    # warm start from the solution converges at once
+    st2 = nq.st_cg(ctx, g, m2, 10000, 1e-20, fx, fb, guess=fx)
+    assert st2["converged"] and st2["iters"] <= 2
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_st_cgm_matches_oracle(st_setups, prec):
+    """Multi-shift CG (cgm_invert with ApplyStD2eeM2) on 8^4, three sources and
+    four shifts batched in one call; every (source, shift) reaches its residue in
+    FP64 and agrees with the oracle's CGM solution."""
+    L = (8, 8, 8, 8)
+    ctx, g, geo, conf, ce, co = st_setups(L)
+    shifts = [0.0004, 0.003, 0.02, 0.25]
+    residues = [1e-24] * 4
+    srcs = [gauss_eo(geo.Vh, 40 + i, ncomp=3) for i in range(3)]
+    fsrc = [ctx.field(nq.COLOR, nq.EVEN, s) for s in srcs]
+    fsol = [[ctx.field(nq.COLOR, nq.EVEN) for _ in shifts] for _ in srcs]
+    res = nq.st_cgm(ctx, g, shifts, 20000, residues, fsol, fsrc, prec=prec)
+    for i, s in enumerate(srcs):
+        assert res[i]["rc"] == nq.NQ_OK
+        ref, info = geo.st_cgm(ce, co, shifts, 20000, residues, s)
+        for k in range(len(shifts)):
+            assert res[i]["true_res_sq"][k] < 1e-23
+            assert rel_l2(fsol[i][k].download(), ref[k]) < 1e-10, (i, k)
+
+
+def test_st_cgm_batch_equals_alone(st_setups):
+    """A source solved alone and inside a batch of three gives identical bits."""
+    L = (4, 6, 4, 8)
+    ctx, g, geo, conf, ce, co = st_setups(L)
+    shifts = [0.001, 0.05]
+    srcs = [gauss_eo(geo.Vh, 60 + i, ncomp=3) for i in range(3)]
+    fsrc = [ctx.field(nq.COLOR, nq.EVEN, s) for s in srcs]
+    batch = [[ctx.field(nq.COLOR, nq.EVEN) for _ in shifts] for _ in srcs]
+    nq.st_cgm(ctx, g, shifts, 20000, [1e-22, 1e-22], batch, fsrc)
+    alone = [[ctx.field(nq.COLOR, nq.EVEN) for _ in shifts]]
+    nq.st_cgm(ctx, g, shifts, 20000, [1e-22, 1e-22], alone, fsrc[1:2])
+    for k in range(len(shifts)):
+        assert np.array_equal(alone[0][k].download(), batch[1][k].download())
+
+
+def test_tmQ2_cg_matches_oracle():
+    L = (4, 6, 4, 8)
+    conf = tm_conf(L, 3)
+    ctx = nq.Ctx(L)
+    g = nq.Gauge(ctx, lx=conf)
+    geo = orc.Geom(L)
+    src = nq.source_lx(L, 5, "gauss")
+    kappa, mu = 0.16, 0.02
+    ref, rst = geo.tmQ2_cg(conf, kappa, mu, 10000, 1e-24, src)
+    fs = ctx.field(nq.SPINCOLOR, nq.LX, src)
+    fx = ctx.field(nq.SPINCOLOR, nq.LX)
+    st = nq.tmQ2_cg(ctx, g, kappa, mu, 10000, 1e-24, fx, fs)
+    assert st["converged"] and st["rel_res_sq"] < 2e-24
+    assert rel_l2(fx.download(), ref) < 1e-10
+    assert abs(st["iters"] - rst["iters"]) <= 2
+    ctx.close()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32", "f16"])
+def test_multi_rhs_eo_solve(prec):
+    """Four sources with four twisted masses solved in one batched call: each
+    solution matches the oracle's eo solve, and the one solved alone through the
+    same entry point gives the same bits as inside the batch."""
+    L = (8, 8, 8, 8)
+    conf = tm_conf(L, 4)
+    ctx = nq.Ctx(L)
+    g = nq.Gauge(ctx, lx=conf)
+    geo = orc.Geom(L)
+    kappa = 0.16
+    mus = [0.003, 0.005, 0.01, 0.02]
+    srcs = [nq.source_lx(L, 70 + i, "z2", parity_mask=1) for i in range(4)]
+    fsrc = [ctx.field(nq.SPINCOLOR, nq.LX, s) for s in srcs]
+    fsol = [ctx.field(nq.SPINCOLOR, nq.LX) for _ in srcs]
+    res = nq.tm_solve_eoprec_multi(ctx, g, kappa, mus, 20000, 1e-24, fsol, fsrc, prec=prec)
+    for i in range(4):
+        assert res[i]["rc"] == nq.NQ_OK and res[i]["converged"]
+        ref, _ = geo.tm_solve_eoprec(conf, kappa, mus[i], 20000, 1e-24, srcs[i])
+        assert rel_l2(fsol[i].download(), ref) < 1e-10, i
+    alone = [ctx.field(nq.SPINCOLOR, nq.LX)]
+    nq.tm_solve_eoprec_multi(ctx, g, kappa, mus[2:3], 20000, 1e-24, alone, fsrc[2:3], prec=prec)
+    assert np.array_equal(alone[0].download(), fsol[2].download())
+    ctx.close()
