@@ -159,7 +159,7 @@ def main():
                    "bytes_per_apply": bytes_hop, "sloppy_us": round(ms_hop16 * 1e3, 2),
                    "sloppy_GBs": round(bytes_hop16 / (ms_hop16 * 1e-3) / 1e9, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom_name,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_of(dom_name, L), "kernel": dom_name,
                      "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": dom["bytes_per_launch"]},
         "cpu_baseline": cpu,
     }
@@ -168,6 +168,25 @@ def main():
         f.close()
     g.close()
     ctx.close()
+
+
+# rocprofv3 byte-counter summaries (FETCH_SIZE x calibration + WRITE_SIZE per dispatch) kept in
+# profiles/, per ledger kernel name, for the 32^3x64 workload
+PROFILED = {"tm_hop_f64": "tm_hop_f64_32c64.txt", "tm_hop_f16_l32c": "tm_hop_f16_32c64.txt"}
+
+
+def traffic_of(kernel, L):
+    """HBM bytes per launch of `kernel` from the committed rocprof summary, or None."""
+    f = PROFILED.get(kernel)
+    if f is None or tuple(L) != (64, 32, 32, 32):
+        return None
+    try:
+        for line in open(os.path.join(ROOT, "profiles", f)):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split()[1])
+    except OSError:
+        return None
+    return None
 
 
 def cpu_baseline(L, conf, src, kappa, mu, iters, fp64_iters):
