@@ -31,7 +31,7 @@ __device__ __forceinline__ void store_any(const SpinorView& v, int64_t stride, u
 
 struct BlasArgs {
   Geo g;
-  int W, NR;
+  Shape s;
   int64_t nchunks;
   double* partial;
   CgState* st;
@@ -49,10 +49,10 @@ __device__ __forceinline__ bool skip_rhs(const BlasArgs& a, int r)
 
 // y = a x + b y ; optional |y|^2 -> fin
 template <int NC, bool RED>
-__global__ void __launch_bounds__(1024) k_axpby(BlasArgs A, SpinorView y, double ar, double ai, SpinorView x, double br,
+__global__ void __launch_bounds__(256) k_axpby(BlasArgs A, SpinorView y, double ar, double ai, SpinorView x, double br,
                                                 double bi)
 {
-  const Map m = map_thread(A.W, A.NR, A.g.Vh);
+  const Map m = map_thread(A.s, A.g.Vh);
   const bool skip = skip_rhs(A, m.r);
   double nrm = 0;
   if (m.active && !skip) {
@@ -79,10 +79,10 @@ __global__ void __launch_bounds__(1024) k_axpby(BlasArgs A, SpinorView y, double
 // out = a x + b y + c z (real coefficients; y/z may be absent; out may be absent);
 // optional |out|^2 -> fin
 template <int NC, bool RED>
-__global__ void __launch_bounds__(1024) k_lincomb(BlasArgs A, SpinorView out, double a, SpinorView x, double b,
+__global__ void __launch_bounds__(256) k_lincomb(BlasArgs A, SpinorView out, double a, SpinorView x, double b,
                                                   SpinorView y, double c, SpinorView z)
 {
-  const Map m = map_thread(A.W, A.NR, A.g.Vh);
+  const Map m = map_thread(A.s, A.g.Vh);
   const bool skip = skip_rhs(A, m.r);
   double nrm = 0;
   if (m.active && !skip) {
@@ -109,9 +109,9 @@ __global__ void __launch_bounds__(1024) k_lincomb(BlasArgs A, SpinorView out, do
 
 // Re <a, b> -> fin
 template <int NC>
-__global__ void __launch_bounds__(1024) k_dot(BlasArgs A, SpinorView a, SpinorView b)
+__global__ void __launch_bounds__(256) k_dot(BlasArgs A, SpinorView a, SpinorView b)
 {
-  const Map m = map_thread(A.W, A.NR, A.g.Vh);
+  const Map m = map_thread(A.s, A.g.Vh);
   double acc = 0;
   if (m.active && !skip_rhs(A, m.r)) {
     cx<double> av[NC], bv[NC];
@@ -126,9 +126,9 @@ __global__ void __launch_bounds__(1024) k_dot(BlasArgs A, SpinorView a, SpinorVi
 // p = r + beta p, beta from state[r]; skipped once that right-hand side is done or
 // while it waits for a reliable update
 template <int NC>
-__global__ void __launch_bounds__(1024) k_pupdate(BlasArgs A, SpinorView p, SpinorView r)
+__global__ void __launch_bounds__(256) k_pupdate(BlasArgs A, SpinorView p, SpinorView r)
 {
-  const Map m = map_thread(A.W, A.NR, A.g.Vh);
+  const Map m = map_thread(A.s, A.g.Vh);
   const CgState* s = A.st + m.r;
   if (s->done || !m.active || (A.pred == 2 ? !s->reliable : s->reliable)) return;
   const double beta = s->beta;
@@ -145,9 +145,9 @@ __global__ void __launch_bounds__(1024) k_pupdate(BlasArgs A, SpinorView p, Spin
 
 // x += alpha p ; r -= alpha s ; |r|^2 (plain CG update, cg_invert_template_threaded.hpp:82-102)
 template <int NC>
-__global__ void __launch_bounds__(1024) k_cg_update(BlasArgs A, SpinorView x, SpinorView p, SpinorView r, SpinorView s)
+__global__ void __launch_bounds__(256) k_cg_update(BlasArgs A, SpinorView x, SpinorView p, SpinorView r, SpinorView s)
 {
-  const Map m = map_thread(A.W, A.NR, A.g.Vh);
+  const Map m = map_thread(A.s, A.g.Vh);
   const CgState* S = A.st + m.r;
   double nrm = 0;
   if (m.active && !S->done && !S->reliable) {
@@ -173,9 +173,9 @@ __global__ void __launch_bounds__(1024) k_cg_update(BlasArgs A, SpinorView x, Sp
 
 // x64 += xs ; xs = 0 (reliable update), only for right-hand sides flagged reliable
 template <int NC>
-__global__ void __launch_bounds__(1024) k_reliable_accum(BlasArgs A, SpinorView x64, SpinorView xs)
+__global__ void __launch_bounds__(256) k_reliable_accum(BlasArgs A, SpinorView x64, SpinorView xs)
 {
-  const Map m = map_thread(A.W, A.NR, A.g.Vh);
+  const Map m = map_thread(A.s, A.g.Vh);
   const CgState* s = A.st + m.r;
   if (s->done || !s->reliable || !m.active) return;
   cx<double> a[NC], b[NC];
@@ -189,9 +189,9 @@ __global__ void __launch_bounds__(1024) k_reliable_accum(BlasArgs A, SpinorView 
 
 // out = in (precision conversion); in absent -> zero
 template <int NC>
-__global__ void __launch_bounds__(1024) k_convert(BlasArgs A, SpinorView out, SpinorView in)
+__global__ void __launch_bounds__(256) k_convert(BlasArgs A, SpinorView out, SpinorView in)
 {
-  const Map m = map_thread(A.W, A.NR, A.g.Vh);
+  const Map m = map_thread(A.s, A.g.Vh);
   if (!m.active || skip_rhs(A, m.r)) return;
   cx<double> v[NC];
   if (in.data) load_any<NC>(in, A.g.stride, m.e, v, m.r);
@@ -203,11 +203,11 @@ __global__ void __launch_bounds__(1024) k_convert(BlasArgs A, SpinorView out, Sp
 
 // out_h = a_h x + b_h y for the upper (spins 0,1) and lower (spins 2,3) halves of a
 // spincolor: the site-local (1/2k + i mu g5)^(+-1) pieces of the tm operator
-__global__ void __launch_bounds__(1024) k_spin_diag(BlasArgs A, SpinorView out, SpinorView x, SpinorView y, double aur,
+__global__ void __launch_bounds__(256) k_spin_diag(BlasArgs A, SpinorView out, SpinorView x, SpinorView y, double aur,
                                                     double aui, double alr, double ali, double bur, double bui,
                                                     double blr, double bli)
 {
-  const Map m = map_thread(A.W, A.NR, A.g.Vh);
+  const Map m = map_thread(A.s, A.g.Vh);
   if (!m.active || skip_rhs(A, m.r)) return;
   cx<double> o[12], t[12];
   load_any<12>(x, A.g.stride, m.e, t, m.r);
@@ -269,11 +269,13 @@ int launch_finalize(nq_ctx* ctx, int nr, CgState* st, int fin, int pred_reliable
 Launch launch_shape(const Geo& g, int nr)
 {
   Launch L;
-  L.NR = nr;
-  L.W = nr >= 4 ? 1 : (nr == 1 ? 4 : 2);
+  L.s.NR = nr;
+  L.s.NRB = nr < 4 ? nr : 4;
+  L.s.NRG = (nr + L.s.NRB - 1) / L.s.NRB;
+  L.s.W = L.s.NRB == 1 ? 4 : (L.s.NRB == 2 ? 2 : 1);
   L.nchunks = (g.Vh + 63) / 64;
-  L.blocks = (L.nchunks + L.W - 1) / L.W;
-  L.threads = 64 * L.W * nr;
+  L.blocks = (L.nchunks + L.s.W - 1) / L.s.W * L.s.NRG;
+  L.threads = 64 * L.s.W * L.s.NRB;
   return L;
 }
 
@@ -294,8 +296,7 @@ static int make_args(nq_ctx* ctx, int nr, CgState* st, int fin, int pred, BlasAr
   if (L->nchunks * nr > ctx->max_partials) return fail(NQ_EINVAL, "reduction scratch too small");
   if ((fin != FIN_NONE || pred) && !st) return fail(NQ_EINVAL, "reduction without solver state");
   A->g = ctx->geo;
-  A->W = L->W;
-  A->NR = nr;
+  A->s = L->s;
   A->nchunks = L->nchunks;
   A->partial = ctx->partial;
   A->st = st;

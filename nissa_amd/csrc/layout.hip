@@ -71,13 +71,16 @@ int launch_soa_to_aos(nq_ctx* ctx, const SpinorView& in, int ncomp, int par, int
   return NQ_OK;
 }
 
-// store one complex element of a link slot array in the target precision
+// store element k of site e of one link slot array (packed records, see Pack)
 template <int LK>
-__device__ __forceinline__ void put_link(void* base, int64_t idx, double re, double im)
+__device__ __forceinline__ void put_link(void* slot_base, int k, int64_t e, int64_t stride, double re, double im)
 {
-  if constexpr (LK == LK_F64_18) ((double2*)base)[idx] = make_double2(re, im);
-  else if constexpr (LK == LK_F32_18 || LK == LK_F32_12) ((float2*)base)[idx] = make_float2((float)re, (float)im);
-  else ((__half2*)base)[idx] = __floats2half2_rn((float)re, (float)im);
+  constexpr int NC = (LK == LK_F64_18 || LK == LK_F32_18) ? 9 : 6;
+  constexpr int EB = LK == LK_F64_18 ? 16 : (LK == LK_F16_12 ? 4 : 8);
+  char* p = (char*)slot_base + Pack<EB, NC>::offset(k, e, stride);
+  if constexpr (EB == 16) *(double2*)p = make_double2(re, im);
+  else if constexpr (EB == 8) *(float2*)p = make_float2((float)re, (float)im);
+  else *(__half2*)p = __floats2half2_rn((float)re, (float)im);
 }
 
 // out layout: [op][slot][nc][stride]
@@ -91,22 +94,24 @@ __global__ void __launch_bounds__(kBlock) k_build_links(Geo g, const double* ae,
   const double* conf[2] = {ae, ao};
   const Site s = site_of(g, op, e);
   const int64_t ne = nissa_lx(g, s.t, s.x, s.y, s.z) >> 1;
-  const int64_t par_off = (int64_t)op * 8 * NC * g.stride;
+  constexpr int EB = LK == LK_F64_18 ? 16 : (LK == LK_F16_12 ? 4 : 8);
+  const int64_t slot_bytes = (int64_t)NC * g.stride * EB;
+  char* par_base = (char*)out + (int64_t)op * 8 * slot_bytes;
 #pragma unroll
   for (int mu = 0; mu < 4; mu++) {
     // forward: U_mu(x) as stored
     const double* Uf = conf[op] + (ne * 4 + mu) * 18;
-    const int64_t bf = par_off + (int64_t)(2 * mu) * NC * g.stride + e;
+    char* bf = par_base + (int64_t)(2 * mu) * slot_bytes;
 #pragma unroll
-    for (int k = 0; k < NC; k++) put_link<LK>(out, bf + k * g.stride, Uf[2 * k], Uf[2 * k + 1]);
+    for (int k = 0; k < NC; k++) put_link<LK>(bf, k, e, g.stride, Uf[2 * k], Uf[2 * k + 1]);
     // backward: U_mu(x-mu)^+ , element (i,j) = conj(U(j,i))
     const int64_t n = nissa_neigh_eo(g, s, mu, false);
     const double* Ub = conf[!op] + (n * 4 + mu) * 18;
-    const int64_t bb = par_off + (int64_t)(2 * mu + 1) * NC * g.stride + e;
+    char* bb = par_base + (int64_t)(2 * mu + 1) * slot_bytes;
 #pragma unroll
     for (int k = 0; k < NC; k++) {
       const int i = k / 3, j = k % 3;
-      put_link<LK>(out, bb + k * g.stride, Ub[2 * (j * 3 + i)], -Ub[2 * (j * 3 + i) + 1]);
+      put_link<LK>(bb, k, e, g.stride, Ub[2 * (j * 3 + i)], -Ub[2 * (j * 3 + i) + 1]);
     }
   }
 }

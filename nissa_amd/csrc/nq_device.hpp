@@ -150,72 +150,159 @@ __device__ __forceinline__ float bload_f1(__amdgpu_buffer_rsrc_t r, uint32_t vof
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
 }
 
+// ---------------------------------------------------------------- packed records
+// Fields and link slots store NC complex elements of EB bytes per site in 16-byte
+// records: element k belongs to group q = k / G (G = 16/EB elements per record).
+// Group q < NG = NC/G is an array of 16-byte records over the sites; the NL = NC%G
+// leftover elements form one array of NL*EB-byte records after them:
+//   group q of site e:   base + q*stride*16 + e*16 + (k%G)*EB
+//   leftover of site e:  base + NG*stride*16 + e*NL*EB + (k-NG*G)*EB
+// A field of NC elements occupies NC*EB*stride bytes in any precision.  FP64
+// (G = 1) is plain structure-of-arrays; FP32 packs 2 and FP16 4 complex numbers
+// per record, so every lane issues 16-byte loads in all precisions.
+template <int EB, int NC>
+struct Pack {
+  static constexpr int G = 16 / EB, NG = NC / G, NL = NC % G;
+  __host__ __device__ static constexpr int64_t offset(int k, int64_t e, int64_t stride)
+  {
+    return k < NG * G ? (int64_t)(k / G) * stride * 16 + e * 16 + (k % G) * EB
+                      : (int64_t)NG * stride * 16 + e * NL * EB + (k - NG * G) * EB;
+  }
+};
+
+typedef float nq_fv4 __attribute__((ext_vector_type(4)));
+typedef _Float16 nq_hv2 __attribute__((ext_vector_type(2)));
+typedef _Float16 nq_hv4 __attribute__((ext_vector_type(4)));
+typedef _Float16 nq_hv8 __attribute__((ext_vector_type(8)));
+typedef unsigned int nq_u3 __attribute__((ext_vector_type(3)));
+
+// load NC complex elements of site e from a packed array at `base` (num_records
+// `bytes`); element type selected by EB: 16 double, 8 float, 4 half
+template <int EB, int NC, int CP, class R>
+__device__ __forceinline__ void load_packed(const void* base, int64_t bytes, int64_t stride, uint32_t e, cx<R>* s)
+{
+  using P = Pack<EB, NC>;
+  const __amdgpu_buffer_rsrc_t rs = make_rsrc(base, bytes);
+#pragma unroll
+  for (int q = 0; q < P::NG; q++) {
+    const nq_u4 u = __builtin_amdgcn_raw_buffer_load_b128(rs, e * 16u, (uint32_t)(q * stride * 16), CP);
+    if constexpr (EB == 16) {
+      const nq_dv2 d = __builtin_bit_cast(nq_dv2, u);
+      s[q] = mk<R>((R)d.x, (R)d.y);
+    } else if constexpr (EB == 8) {
+      const nq_fv4 f = __builtin_bit_cast(nq_fv4, u);
+      s[2 * q] = mk<R>((R)f.x, (R)f.y);
+      s[2 * q + 1] = mk<R>((R)f.z, (R)f.w);
+    } else {
+      const nq_hv8 h = __builtin_bit_cast(nq_hv8, u);
+#pragma unroll
+      for (int j = 0; j < 4; j++) s[4 * q + j] = mk<R>((R)(float)h[2 * j], (R)(float)h[2 * j + 1]);
+    }
+  }
+  if constexpr (P::NL > 0) {
+    const uint32_t so = (uint32_t)(P::NG * stride * 16), vo = e * (uint32_t)(P::NL * EB);
+    constexpr int k0 = P::NG * P::G;
+    if constexpr (EB == 8) {  // one float complex
+      const nq_fv2 f = __builtin_bit_cast(nq_fv2, __builtin_amdgcn_raw_buffer_load_b64(rs, vo, so, CP));
+      s[k0] = mk<R>((R)f.x, (R)f.y);
+    } else if constexpr (P::NL == 1) {
+      const nq_hv2 h = __builtin_bit_cast(nq_hv2, __builtin_amdgcn_raw_buffer_load_b32(rs, vo, so, CP));
+      s[k0] = mk<R>((R)(float)h.x, (R)(float)h.y);
+    } else if constexpr (P::NL == 2) {
+      const nq_hv4 h = __builtin_bit_cast(nq_hv4, __builtin_amdgcn_raw_buffer_load_b64(rs, vo, so, CP));
+      s[k0] = mk<R>((R)(float)h[0], (R)(float)h[1]);
+      s[k0 + 1] = mk<R>((R)(float)h[2], (R)(float)h[3]);
+    } else {
+      const nq_u3 u = __builtin_amdgcn_raw_buffer_load_b96(rs, vo, so, CP);
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        const nq_hv2 h = __builtin_bit_cast(nq_hv2, (unsigned)u[j]);
+        s[k0 + j] = mk<R>((R)(float)h.x, (R)(float)h.y);
+      }
+    }
+  }
+}
+
+template <class T, int CP>
+__device__ __forceinline__ void put(T* p, const T& v)
+{
+  if constexpr (CP == 2) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
+// store NC complex elements of site e (FP16 values already scaled)
+template <int EB, int NC, int CP, class R>
+__device__ __forceinline__ void store_packed(void* base, int64_t stride, uint32_t e, const cx<R>* s, R scale = (R)1)
+{
+  using P = Pack<EB, NC>;
+  char* b = (char*)base;
+#pragma unroll
+  for (int q = 0; q < P::NG; q++) {
+    char* p = b + (int64_t)q * stride * 16 + (int64_t)e * 16;
+    if constexpr (EB == 16) {
+      put<nq_dv2, CP>((nq_dv2*)p, nq_dv2{(double)s[q].re, (double)s[q].im});
+    } else if constexpr (EB == 8) {
+      put<nq_fv4, CP>((nq_fv4*)p, nq_fv4{(float)s[2 * q].re, (float)s[2 * q].im, (float)s[2 * q + 1].re,
+                                         (float)s[2 * q + 1].im});
+    } else {
+      nq_hv8 h;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        h[2 * j] = (_Float16)((float)s[4 * q + j].re * (float)scale);
+        h[2 * j + 1] = (_Float16)((float)s[4 * q + j].im * (float)scale);
+      }
+      put<nq_u4, CP>((nq_u4*)p, __builtin_bit_cast(nq_u4, h));
+    }
+  }
+  if constexpr (P::NL > 0) {
+    constexpr int k0 = P::NG * P::G;
+    char* p = b + (int64_t)P::NG * stride * 16 + (int64_t)e * (P::NL * EB);
+    if constexpr (EB == 8) {
+      put<nq_fv2, CP>((nq_fv2*)p, nq_fv2{(float)s[k0].re, (float)s[k0].im});
+    } else {
+#pragma unroll
+      for (int j = 0; j < P::NL; j++) {
+        const nq_hv2 h = {(_Float16)((float)s[k0 + j].re * (float)scale), (_Float16)((float)s[k0 + j].im * (float)scale)};
+        put<unsigned, CP>((unsigned*)p + j, __builtin_bit_cast(unsigned, h));
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------- spinor access
-// Load NC complex components of site e; compute type R.
+// Load NC complex components of site e of right-hand side r; compute type R.
 // CP: cache policy of the accesses (0 default; kNT for data used once by the
-// kernel, which should not displace the neighbour spinors from L2)
+// kernel, which should not displace the neighbour spinors from L2).
+// FP16 fields carry one FP32 norm per site and right-hand side (the largest
+// |component|), stored at norm[r*stride + e]; the halves hold component/norm.
 constexpr int kNT = 2;
 template <int SP, class R, int NC, int CP = 0>
 struct SpAcc {
   static constexpr int EB = SP == SP_F64 ? 16 : (SP == SP_F32 ? 8 : 4);  // bytes per complex
-  // component k of site e of right-hand side r lives at data[r*rstride + k*stride + e]
   __device__ __forceinline__ static void load(const SpinorView& v, int64_t stride, uint32_t e, cx<R> s[NC], int r = 0)
   {
     const char* base = (const char*)v.data + (int64_t)r * v.rstride * EB;
-    const __amdgpu_buffer_rsrc_t rs = make_rsrc(base, (int64_t)NC * stride * EB);
-    const uint32_t vo = e * EB;
-    if constexpr (SP == SP_F64) {
-#pragma unroll
-      for (int k = 0; k < NC; k++) {
-        const double2 d = bload_d2<CP>(rs, vo, (uint32_t)(k * stride * EB));
-        s[k] = mk<R>((R)d.x, (R)d.y);
-      }
-    } else if constexpr (SP == SP_F32) {
-#pragma unroll
-      for (int k = 0; k < NC; k++) {
-        const float2 d = bload_f2<CP>(rs, vo, (uint32_t)(k * stride * EB));
-        s[k] = mk<R>((R)d.x, (R)d.y);
-      }
-    } else {
+    load_packed<EB, NC, CP, R>(base, (int64_t)NC * stride * EB, stride, e, s);
+    if constexpr (SP == SP_F16) {
       const __amdgpu_buffer_rsrc_t rn = make_rsrc(v.norm + (int64_t)r * stride, stride * 4);
       const R nrm = (R)bload_f1(rn, e * 4, 0);
 #pragma unroll
-      for (int k = 0; k < NC; k++) {
-        const float2 d = bload_h2<CP>(rs, vo, (uint32_t)(k * stride * EB));
-        s[k] = mk<R>((R)d.x * nrm, (R)d.y * nrm);
-      }
+      for (int k = 0; k < NC; k++) s[k] = mk<R>(s[k].re * nrm, s[k].im * nrm);
     }
   }
   __device__ __forceinline__ static void store(const SpinorView& v, int64_t stride, uint32_t e, const cx<R> s[NC],
                                                int r = 0)
   {
-    const int64_t o = (int64_t)r * v.rstride + e;
-    if constexpr (SP == SP_F64) {
-      nq_dv2* p = (nq_dv2*)v.data + o;
-#pragma unroll
-      for (int k = 0; k < NC; k++) {
-        const nq_dv2 d = {(double)s[k].re, (double)s[k].im};
-        if constexpr (CP == kNT) __builtin_nontemporal_store(d, p + k * stride);
-        else p[k * stride] = d;
-      }
-    } else if constexpr (SP == SP_F32) {
-      nq_fv2* p = (nq_fv2*)v.data + o;
-#pragma unroll
-      for (int k = 0; k < NC; k++) {
-        const nq_fv2 d = {(float)s[k].re, (float)s[k].im};
-        if constexpr (CP == kNT) __builtin_nontemporal_store(d, p + k * stride);
-        else p[k * stride] = d;
-      }
-    } else {
+    char* base = (char*)v.data + (int64_t)r * v.rstride * EB;
+    if constexpr (SP == SP_F16) {
       float m = 0.f;
 #pragma unroll
       for (int k = 0; k < NC; k++) m = fmaxf(m, fmaxf(fabsf((float)s[k].re), fabsf((float)s[k].im)));
       const float inv = m > 0.f ? 1.0f / m : 0.f;
-      __half2* p = (__half2*)v.data + o;
-#pragma unroll
-      for (int k = 0; k < NC; k++)
-        p[k * stride] = __floats2half2_rn((float)s[k].re * inv, (float)s[k].im * inv);
+      store_packed<EB, NC, CP, R>(base, stride, e, s, (R)inv);
       v.norm[(int64_t)r * stride + e] = m;
+    } else {
+      store_packed<EB, NC, CP, R>(base, stride, e, s);
     }
   }
 };
@@ -225,32 +312,13 @@ template <int LK, class R>
 struct LkAcc {
   static constexpr int NC = (LK == LK_F64_18 || LK == LK_F32_18) ? 9 : 6;
   static constexpr int EB = LK == LK_F64_18 ? 16 : (LK == LK_F16_12 ? 4 : 8);
-  // U as u[3][3]; the slot array starts at base + slot*NC*stride elements
+  // U as u[3][3]; slot s is a packed array of NC elements at base + s*NC*stride*EB bytes
   __device__ __forceinline__ static void load(const void* base, int64_t stride, int slot, uint32_t e,
                                               const double* det, cx<R> u[9])
   {
     const char* sb = (const char*)base + (int64_t)slot * NC * stride * EB;
-    const __amdgpu_buffer_rsrc_t rs = make_rsrc(sb, (int64_t)NC * stride * EB);
-    const uint32_t vo = e * EB;
-    if constexpr (LK == LK_F64_18) {
-#pragma unroll
-      for (int k = 0; k < 9; k++) {
-        const double2 d = bload_d2<kLinkCP>(rs, vo, (uint32_t)(k * stride * EB));
-        u[k] = mk<R>((R)d.x, (R)d.y);
-      }
-    } else if constexpr (LK == LK_F32_18) {
-#pragma unroll
-      for (int k = 0; k < 9; k++) {
-        const float2 d = bload_f2<kLinkCP>(rs, vo, (uint32_t)(k * stride * EB));
-        u[k] = mk<R>((R)d.x, (R)d.y);
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < 6; k++) {
-        const float2 d = LK == LK_F32_12 ? bload_f2<kLinkCP>(rs, vo, (uint32_t)(k * stride * EB))
-                                         : bload_h2<kLinkCP>(rs, vo, (uint32_t)(k * stride * EB));
-        u[k] = mk<R>((R)d.x, (R)d.y);
-      }
+    load_packed<EB, NC, kLinkCP, R>(sb, (int64_t)NC * stride * EB, stride, e, u);
+    if constexpr (NC == 6) {
       // row 2 = det * conj(row0 x row1)
       const cx<R> dt = mk<R>((R)det[0], (R)det[1]);
 #pragma unroll
@@ -345,9 +413,9 @@ __device__ inline void finalize(CgState* st, int fin, double total)
 }
 
 // Thread mapping shared by all field kernels: a block covers W chunks of 64
-// consecutive sites for each of NR right-hand sides; wavefront w handles
-// right-hand side w % NR of chunk w / NR.  Launch: 64*W*NR threads,
-// ceil(nchunks/W) blocks.
+// consecutive sites for NRB right-hand sides of group g (Shape); wavefront w
+// handles right-hand side g*NRB + w % NRB of chunk w / NRB.  Launch: 64*W*NRB
+// threads, ceil(nchunks/W)*NRG blocks.
 struct Map {
   int r;
   int64_t chunk;
@@ -364,14 +432,24 @@ __device__ __forceinline__ int64_t xcd_block()
   return (int64_t)xcd * q8 + (xcd < r8 ? xcd : r8) + i;
 }
 
-__device__ __forceinline__ Map map_thread(int W, int NR, int64_t Vh)
+__device__ __forceinline__ Map map_thread(const Shape& S, int64_t Vh)
 {
   Map m;
   // the wave index is uniform across the wavefront: keep r and chunk in SGPRs so
   // that buffer descriptors built from them stay scalar
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  m.r = wave % NR;
-  m.chunk = xcd_block() * W + wave / NR;
+  const int64_t lb = xcd_block();
+  const int rg = (int)(lb % S.NRG);
+  m.r = rg * S.NRB + wave % S.NRB;
+  m.chunk = (lb / S.NRG) * S.W + wave / S.NRB;
+  if (m.r >= S.NR) {
+    // padding wavefront of the last right-hand-side group: no site, no partial
+    m.r = S.NR - 1;
+    m.chunk = INT64_MAX / 128;
+    m.e = 0;
+    m.active = false;
+    return m;
+  }
   const int64_t e = m.chunk * 64 + lane;
   m.active = e < Vh;
   m.e = (uint32_t)e;

@@ -210,8 +210,15 @@ void prof_flush(nq_ctx* ctx);
 int64_t blocks_for(int64_t n);
 constexpr int kBlock = 256;
 // launch shape of the (64-site chunk x right-hand side) mapping
+// Launch shape of the field kernels: a workgroup covers W chunks of 64 sites for
+// NRB right-hand sides (W*NRB <= 4 wavefronts); the NR right-hand sides of a launch
+// are split into NRG groups of NRB, the groups of one chunk range in adjacent
+// workgroups so that they read the same links close in time.
+struct Shape {
+  int W, NRB, NRG, NR;
+};
 struct Launch {
-  int W, NR;
+  Shape s;
   int64_t nchunks, blocks;
   int threads;
 };

@@ -68,13 +68,13 @@ __device__ __forceinline__ void st_hop_site(const Geo& g, int op, uint32_t e, co
 // out = c1 hop + c2 aux (real coefficients, per rhs from coef[2r], coef[2r+1] when
 // tabulated). MODE 0: write; 1: write (if write) + |out|^2; 4: write (if write) + Re<aux,out>.
 template <int SP, int LK, int MODE>
-__global__ void __launch_bounds__(1024) k_st_hop(Geo g, int op, SpinorView out, SpinorView in, LinkView U, double c1,
-                                                 double c2, const double* coef, SpinorView aux, int write, int W,
-                                                 int NR, int64_t nchunks, double* partial, const int* skip_flags,
+__global__ void __launch_bounds__(256) k_st_hop(Geo g, int op, SpinorView out, SpinorView in, LinkView U, double c1,
+                                                 double c2, const double* coef, SpinorView aux, int write, Shape S,
+                                                 int64_t nchunks, double* partial, const int* skip_flags,
                                                  int skip_stride, int skip_n)
 {
   using R = RealOfS<SP>;
-  const Map m = map_thread(W, NR, g.Vh);
+  const Map m = map_thread(S, g.Vh);
   const bool skip = skip_flags && (skip_flags[m.r * skip_stride] || (skip_n > 1 && skip_flags[m.r * skip_stride + 1]));
   double red = 0;
   if (m.active && !skip) {
@@ -106,12 +106,12 @@ __global__ void __launch_bounds__(1024) k_st_hop(Geo g, int op, SpinorView out, 
 // C2: s = -1/4 Deo t (even output, never stored) ; sol_k -= betas_k ps_k ; r += betaa s ; |r|^2
 // sol/ps of shift k of rhs r: base + r*rstride + k*sstride (elements)
 template <int SP, int LK>
-__global__ void __launch_bounds__(1024) k_cgm_hop2(Geo g, SpinorView t, LinkView U, SpinorView rv, SpinorView sol,
-                                                   SpinorView ps, int64_t sstride, CgmState* st, int W, int NR,
+__global__ void __launch_bounds__(256) k_cgm_hop2(Geo g, SpinorView t, LinkView U, SpinorView rv, SpinorView sol,
+                                                   SpinorView ps, int64_t sstride, CgmState* st, Shape SH,
                                                    int64_t nchunks, double* partial)
 {
   using R = RealOfS<SP>;
-  const Map m = map_thread(W, NR, g.Vh);
+  const Map m = map_thread(SH, g.Vh);
   const CgmState* S = st + m.r;
   double red = 0;
   if (m.active && !S->done) {
@@ -150,11 +150,11 @@ __global__ void __launch_bounds__(1024) k_cgm_hop2(Geo g, SpinorView t, LinkView
 
 // C3: p = r + alpha p ; ps_k = zfs_k r + alphas_k ps_k for running shifts
 template <int SP>
-__global__ void __launch_bounds__(1024) k_cgm_update(Geo g, SpinorView rv, SpinorView p, SpinorView ps,
-                                                     int64_t sstride, CgmState* st, int W, int NR)
+__global__ void __launch_bounds__(256) k_cgm_update(Geo g, SpinorView rv, SpinorView p, SpinorView ps,
+                                                     int64_t sstride, CgmState* st, Shape SH)
 {
   using R = RealOfS<SP>;
-  const Map m = map_thread(W, NR, g.Vh);
+  const Map m = map_thread(SH, g.Vh);
   const CgmState* S = st + m.r;
   if (!m.active || S->done) return;
   cx<R> rr[3], pp[3];
@@ -277,11 +277,11 @@ int launch_st_hop(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& 
 #define NQ_ST(SPV, LKV)                                                                                            \
   switch (mode) {                                                                                                  \
   case 0: hipLaunchKernelGGL((k_st_hop<SPV, LKV, 0>), grid, blk, 0, ctx->stream, g, op, out, in, U, c1, c2, coef_dev, \
-                             aux, write, L.W, nr, L.nchunks, ctx->partial, skip_flags, skip_stride, skip_n); break;         \
+                             aux, write, L.s, L.nchunks, ctx->partial, skip_flags, skip_stride, skip_n); break;         \
   case 1: hipLaunchKernelGGL((k_st_hop<SPV, LKV, 1>), grid, blk, 0, ctx->stream, g, op, out, in, U, c1, c2, coef_dev, \
-                             aux, write, L.W, nr, L.nchunks, ctx->partial, skip_flags, skip_stride, skip_n); break;         \
+                             aux, write, L.s, L.nchunks, ctx->partial, skip_flags, skip_stride, skip_n); break;         \
   default: hipLaunchKernelGGL((k_st_hop<SPV, LKV, 4>), grid, blk, 0, ctx->stream, g, op, out, in, U, c1, c2, coef_dev, \
-                              aux, write, L.W, nr, L.nchunks, ctx->partial, skip_flags, skip_stride, skip_n); break;        \
+                              aux, write, L.s, L.nchunks, ctx->partial, skip_flags, skip_stride, skip_n); break;        \
   }
   if (sp == SP_F64 && lk == LK_F64_18) { NQ_ST(SP_F64, LK_F64_18) }
   else if (sp == SP_F32 && lk == LK_F32_18) { NQ_ST(SP_F32, LK_F32_18) }
@@ -311,11 +311,11 @@ int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, co
   prof_begin(ctx, "cgm_hop1", (8 * lbytes(lk) + 2 * csite(sp)) * nr * (double)g.Vh, &ev);
   if (sp == SP_F64)
     hipLaunchKernelGGL((k_st_hop<SP_F64, LK_F64_18, 1>), grid, blk, 0, ctx->stream, g, 1, t, p, U, 1.0, 0.0, nullptr,
-                       none, 1, L.W, nr, L.nchunks, ctx->partial, (const int*)&st->done,
+                       none, 1, L.s, L.nchunks, ctx->partial, (const int*)&st->done,
                        (int)(sizeof(CgmState) / sizeof(int)), 1);
   else
     hipLaunchKernelGGL((k_st_hop<SP_F32, LK_F32_18, 1>), grid, blk, 0, ctx->stream, g, 1, t, p, U, 1.0, 0.0, nullptr,
-                       none, 1, L.W, nr, L.nchunks, ctx->partial, (const int*)&st->done,
+                       none, 1, L.s, L.nchunks, ctx->partial, (const int*)&st->done,
                        (int)(sizeof(CgmState) / sizeof(int)), 1);
   NQ_HIP(hipGetLastError());
   prof_end(ctx, ev);
@@ -326,10 +326,10 @@ int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, co
   prof_begin(ctx, "cgm_hop2_update", (8 * lbytes(lk) + 3 * csite(sp)) * nr * (double)g.Vh, &ev);
   if (sp == SP_F64)
     hipLaunchKernelGGL((k_cgm_hop2<SP_F64, LK_F64_18>), grid, blk, 0, ctx->stream, g, t, U, r, sol, ps, sstride, st,
-                       L.W, nr, L.nchunks, ctx->partial);
+                       L.s, L.nchunks, ctx->partial);
   else
     hipLaunchKernelGGL((k_cgm_hop2<SP_F32, LK_F32_18>), grid, blk, 0, ctx->stream, g, t, U, r, sol, ps, sstride, st,
-                       L.W, nr, L.nchunks, ctx->partial);
+                       L.s, L.nchunks, ctx->partial);
   NQ_HIP(hipGetLastError());
   prof_end(ctx, ev);
   hipLaunchKernelGGL(k_cgm_finalize, dim3(1), dim3(256), 0, ctx->stream, ctx->partial, L.nchunks, nr, st,
@@ -338,9 +338,9 @@ int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, co
   // C3
   prof_begin(ctx, "cgm_update", 4 * csite(sp) * nr * (double)g.Vh, &ev);
   if (sp == SP_F64)
-    hipLaunchKernelGGL(k_cgm_update<SP_F64>, grid, blk, 0, ctx->stream, g, r, p, ps, sstride, st, L.W, nr);
+    hipLaunchKernelGGL(k_cgm_update<SP_F64>, grid, blk, 0, ctx->stream, g, r, p, ps, sstride, st, L.s);
   else
-    hipLaunchKernelGGL(k_cgm_update<SP_F32>, grid, blk, 0, ctx->stream, g, r, p, ps, sstride, st, L.W, nr);
+    hipLaunchKernelGGL(k_cgm_update<SP_F32>, grid, blk, 0, ctx->stream, g, r, p, ps, sstride, st, L.s);
   NQ_HIP(hipGetLastError());
   prof_end(ctx, ev);
   return NQ_OK;

@@ -105,20 +105,22 @@ __device__ __forceinline__ void hop_slot(const Geo& g, const Site& s, uint32_t e
 template <int SP>
 using RealOf = typename std::conditional<SP == SP_F64, double, float>::type;
 
-template <int SP, int LK, int MODE, int XP, int MT>
-__global__ void __launch_bounds__(MT, (MT == 256 && SP != SP_F64 ? 2 : 1)) k_tm_hop(Geo g, int op, SpinorView out, SpinorView in, LinkView U, double anis_d,
-                                               int use_anis, HopEpi epi, int W, int NR, int64_t nchunks,
+// workgroups of at most 256 threads; FP64 builds get the full 256-VGPR budget
+// (one wavefront per SIMD per workgroup), the sloppy builds two workgroups per CU
+template <int SP, int LK, int MODE, int XP>
+__global__ void __launch_bounds__(256, (SP != SP_F64 ? 2 : 1)) k_tm_hop(Geo g, int op, SpinorView out, SpinorView in, LinkView U, double anis_d,
+                                               int use_anis, HopEpi epi, Shape S, int64_t nchunks,
                                                double* partial)
 {
   using R = RealOf<SP>;
   using RX = RealOf<XP>;
-  const Map m = map_thread(W, NR, g.Vh);
+  const Map m = map_thread(S, g.Vh);
   CgState* st = epi.st ? epi.st + m.r : nullptr;
   bool skip = false;
   if (st)
     skip = st->done || (epi.pred_reliable == 1 && !st->reliable) || (epi.pred_reliable == 2 && st->reliable);
   // a launch whose every right-hand side is idle does nothing (uniform over the grid)
-  if (epi.st && NR == 1 && skip) return;
+  if (epi.st && S.NR == 1 && skip) return;
   double nrm = 0;
   if (m.active && !skip) {
     const uint32_t e = m.e;
@@ -266,13 +268,8 @@ static int go(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, 
   if (L.nchunks * nr > ctx->max_partials) return fail(NQ_EINVAL, "tm hop: reduction scratch too small");
   hipEvent_t ev = nullptr;
   prof_begin(ctx, name, tm_hop_bytes(SP, LK, epi) * (double)g.Vh, &ev);
-  const bool multi = nr > 1 || epi.batched;
-  if (multi)
-    hipLaunchKernelGGL((k_tm_hop<SP, LK, MODE, XP, 1024>), dim3((unsigned)L.blocks), dim3(L.threads), 0, ctx->stream, g,
-                       op, out, in, U, anis, anis != 1.0 ? 1 : 0, epi, L.W, nr, L.nchunks, ctx->partial);
-  else
-    hipLaunchKernelGGL((k_tm_hop<SP, LK, MODE, XP, 256>), dim3((unsigned)L.blocks), dim3(L.threads), 0, ctx->stream, g,
-                       op, out, in, U, anis, anis != 1.0 ? 1 : 0, epi, L.W, nr, L.nchunks, ctx->partial);
+  hipLaunchKernelGGL((k_tm_hop<SP, LK, MODE, XP>), dim3((unsigned)L.blocks), dim3(L.threads), 0, ctx->stream, g, op, out,
+                     in, U, anis, anis != 1.0 ? 1 : 0, epi, L.s, L.nchunks, ctx->partial);
   NQ_HIP(hipGetLastError());
   prof_end(ctx, ev);
   if (MODE != 0) NQ_TRY(launch_finalize(ctx, nr, epi.st, epi.fin, epi.pred_reliable));
