@@ -41,6 +41,9 @@ def main():
     ap.add_argument("--prec", default="f16", choices=["f64", "f32", "f16", "f16h"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-iters", type=int, default=6, help="oracle CG iterations timed for the CPU baseline")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-requests", type=int, default=60, help="requests in the Poisson stream")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="request arrival rate (1/s)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}): this build has no multi-GPU path; "
@@ -123,6 +126,14 @@ def main():
     log(f"[bench] Dslash FP64: {ms_hop*1e3:.1f} us/apply, {dslash_gbs:.0f} GB/s; "
         f"{args.prec}: {ms_hop16*1e3:.1f} us/apply, {bytes_hop16/(ms_hop16*1e-3)/1e9:.0f} GB/s")
 
+    serve = None
+    if not args.no_serve:
+        for f in (fs, fx, fo, fe):
+            f.close()
+        g.close()
+        ctx.close()
+        serve = serve_bench(args)
+
     # CPU baseline: the oracle's FP64 CG on the same inputs, bounded sample
     cpu = None
     if not args.no_cpu:
@@ -161,13 +172,90 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_of(dom_name, L), "kernel": dom_name,
                      "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": dom["bytes_per_launch"]},
+        "serve": serve,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    for f in (fs, fx, fo, fe):
-        f.close()
-    g.close()
-    ctx.close()
+    if args.no_serve:
+        for f in (fs, fx, fo, fe):
+            f.close()
+        g.close()
+        ctx.close()
+
+
+def serve_bench(args):
+    """Seeded Poisson stream of propagator requests (12 spin-colour point sources,
+    one of four twisted masses, kappa 0.16, true residual 1e-10) over three resident
+    volumes in a 50/30/20 mix, served with continuous batching and FP16 inner
+    iterations; p50/p99 latency from submission to the last source's FP64 check,
+    aggregate solves/s, and the one-at-a-time FP64 solve rate for the same mix."""
+    import numpy as np
+
+    import nissa_amd as nq
+
+    vols = [(32, 16, 16, 16), (48, 24, 24, 24), (64, 32, 32, 32)]
+    mix = [0.5, 0.3, 0.2]
+    mus = [0.003, 0.005, 0.01, 0.02]
+    kappa, residue = 0.16, 1e-20
+    confs = []
+    for i, L in enumerate(vols):
+        c = nq.gauge_random_lx(L, 5 + i, 0.3)
+        nq.gauge_phase_lx(L, c, [1.0, 0, 0, 0])
+        confs.append(c)
+    # one-at-a-time FP64 reference: one point-source solve per volume and mass
+    t64 = []
+    for L, conf in zip(vols, confs):
+        ctx = nq.Ctx(L, 0)
+        g = nq.Gauge(ctx, lx=conf)
+        fs = ctx.field(nq.SPINCOLOR, nq.LX, nq.source_lx(L, 0, "point", site0=0, sc=0))
+        fx = ctx.field(nq.SPINCOLOR, nq.LX)
+        nq.tm_solve_eoprec(ctx, g, kappa, mus[0], 100000, residue, fx, fs, prec="f64")
+        ts = [nq.tm_solve_eoprec(ctx, g, kappa, mu, 100000, residue, fx, fs, prec="f64")["seconds"] for mu in mus]
+        t64.append(float(np.mean(ts)))
+        fs.close(), fx.close(), g.close(), ctx.close()
+    srv = nq.Server(max_groups=1)
+    vid = [srv.add_volume(L, c) for L, c in zip(vols, confs)]
+    # warm-up: one request per volume builds the batch groups and link copies
+    for v in vid:
+        srv.wait(srv.submit(v, kappa, mus[0], residue, prec="f16", point_site=0))
+    rng = np.random.default_rng(2026)
+    n = args.serve_requests
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, size=n))
+    which = rng.choice(len(vols), size=n, p=mix)
+    mu_i = rng.integers(0, len(mus), size=n)
+    sites = [int(rng.integers(0, int(np.prod(vols[w])))) for w in which]
+    t0 = time.perf_counter()
+    ids = []
+    for i in range(n):
+        dt = arrivals[i] - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        ids.append(srv.submit(vid[which[i]], kappa, mus[mu_i[i]], residue, prec="f16", point_site=sites[i]))
+    res = [srv.wait(i) for i in ids]
+    srv.close()
+    bad = [r for r in res if r["status"] != 0 or r["max_rel_res_sq"] > 2 * residue]
+    if bad:
+        log(f"[bench] serve: {len(bad)} failed requests, statuses {sorted(set(r['status'] for r in bad))}, "
+            f"worst check {max(r['max_rel_res_sq'] for r in bad):.2e}")
+    lat = np.array([r["t_done"] - r["t_submit"] for r in res])
+    span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
+    nsol = 12 * n
+    fp64_one_at_a_time = nsol / float(sum(12 * t64[w] for w in which))
+    out = {
+        "requests": n, "rate_per_s": args.serve_rate, "sources_per_request": 12,
+        "volumes": ["%dx%d^3" % (L[0], L[1]) for L in vols], "mix": mix, "masses": mus,
+        "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+        "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+        "solves_per_s": round(nsol / span, 2),
+        "fp64_one_at_a_time_solves_per_s": round(fp64_one_at_a_time, 2),
+        "fp64_solve_s_per_volume": [round(t, 4) for t in t64],
+        "failed": len(bad), "worst_rel_res_sq": float(max(r["max_rel_res_sq"] for r in res)),
+        "inner": "f16 spinors + FP32 12-real links, FP64 reliable updates",
+    }
+    log(f"[bench] serve: {n} requests at {args.serve_rate}/s: p50 {out['p50_latency_s']}s p99 "
+        f"{out['p99_latency_s']}s, {out['solves_per_s']} solves/s (FP64 one at a time: "
+        f"{out['fp64_one_at_a_time_solves_per_s']}), failed {len(bad)}")
+    return out
 
 
 # rocprofv3 byte-counter summaries (FETCH_SIZE x calibration + WRITE_SIZE per dispatch) kept in

@@ -171,6 +171,49 @@ int nq_host_source_lx(int T, int LX, int LY, int LZ, uint64_t seed, int kind, in
 int nq_host_split_lx(int T, int LX, int LY, int LZ, int ndoubles, const double* lx, double* evn, double* odd);
 int nq_host_paste_lx(int T, int LX, int LY, int LZ, int ndoubles, double* lx, const double* evn, const double* odd);
 
+/* ------------------------------------------------------------------ serving */
+/* Propagator-solve server (the caller of this path: the propagator driver solving 12
+   spin-colour sources per mass on one configuration, projects/ib/prop.cpp:41-106,925-964).
+   Gauge fields stay resident per volume; sources of queued requests join running
+   batches of 16 slots as slots free up (continuous batching, per-source mu); a batch
+   group is created only when its device footprint fits the HBM budget; volumes run on
+   their own streams and threads.  A solution is bit-identical alone and in any mix. */
+typedef struct nq_server nq_server;
+typedef struct nq_serve_request {
+  int volume;               /* id from nq_server_add_volume                             */
+  int prec;                 /* NQ_PREC_* of the inner iteration                         */
+  double kappa, mu;
+  double residue;           /* nissa residue (squared relative), checked in FP64 per source */
+  int niter;
+  int nsrc;                 /* number of sources                                        */
+  int src_kind;             /* 0: host lx spincolor buffers src[k];
+                               1: point sources at lx site site0, spin-colour k = 0..nsrc-1 */
+  int reserved;
+  int64_t site0;
+  const double* const* src; /* src_kind 0: nsrc host buffers, read when the source starts */
+  double* const* sol;       /* nsrc host lx buffers written when each source finishes, or NULL */
+} nq_serve_request;
+typedef struct nq_serve_result {
+  int64_t id;
+  int status;               /* NQ_OK, or the first error of its sources (NQ_EBUSY: refused) */
+  int nsrc;
+  int iters_max;            /* largest Krylov iteration count over the sources          */
+  int reliable_updates;     /* summed over the sources                                  */
+  double max_rel_res_sq;    /* worst FP64 |D sol - src|^2/|src|^2 over the sources        */
+  double t_submit, t_admit, t_done;  /* seconds on the server clock                     */
+} nq_serve_result;
+/* hbm_budget_bytes <= 0: free device memory minus 4 GiB; max_groups_per_volume 1..3 */
+int nq_server_create(int device, int64_t hbm_budget_bytes, int max_groups_per_volume, nq_server** out);
+/* an lx conf (phases applied) that stays resident; returns the volume id */
+int nq_server_add_volume(nq_server* s, int T, int LX, int LY, int LZ, const double* conf_lx, int* volume_id);
+/* non-blocking; host source/solution buffers must stay valid until the request is done */
+int nq_server_submit(nq_server* s, const nq_serve_request* req, int64_t* id);
+/* blocks until request `id` is done (NQ_EBUSY on timeout); releases the request */
+int nq_server_wait(nq_server* s, int64_t id, double timeout_s, nq_serve_result* out);
+double nq_server_clock(const nq_server* s);
+int64_t nq_server_device_bytes(const nq_server* s);  /* resident gauges + admitted groups */
+void nq_server_destroy(nq_server* s);
+
 /* ------------------------------------------------------------------ instrumentation */
 /* napply back-to-back hop applies on device fields, timed with HIP events on the
    ctx stream; prec selects the storage the kernel reads (NQ_PREC_*). */

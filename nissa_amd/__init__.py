@@ -47,6 +47,40 @@ class SolveStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class ServeRequest(C.Structure):
+    _fields_ = [
+        ("volume", C.c_int),
+        ("prec", C.c_int),
+        ("kappa", C.c_double),
+        ("mu", C.c_double),
+        ("residue", C.c_double),
+        ("niter", C.c_int),
+        ("nsrc", C.c_int),
+        ("src_kind", C.c_int),
+        ("reserved", C.c_int),
+        ("site0", C.c_int64),
+        ("src", C.c_void_p),
+        ("sol", C.c_void_p),
+    ]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [
+        ("id", C.c_int64),
+        ("status", C.c_int),
+        ("nsrc", C.c_int),
+        ("iters_max", C.c_int),
+        ("reliable_updates", C.c_int),
+        ("max_rel_res_sq", C.c_double),
+        ("t_submit", C.c_double),
+        ("t_admit", C.c_double),
+        ("t_done", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _lib = None
 
 _P = C.c_void_p
@@ -54,6 +88,13 @@ _D = C.c_double
 _I = C.c_int
 _DP = C.POINTER(C.c_double)
 _SIG = {
+    "nq_server_create": ([_I, C.c_int64, _I, C.POINTER(_P)], _I),
+    "nq_server_add_volume": ([_P, _I, _I, _I, _I, _DP, C.POINTER(_I)], _I),
+    "nq_server_submit": ([_P, C.POINTER(ServeRequest), C.POINTER(C.c_int64)], _I),
+    "nq_server_wait": ([_P, C.c_int64, _D, C.POINTER(ServeResult)], _I),
+    "nq_server_clock": ([_P], _D),
+    "nq_server_device_bytes": ([_P], C.c_int64),
+    "nq_server_destroy": ([_P], None),
     "nq_last_error": ([], C.c_char_p),
     "nq_device_count": ([C.POINTER(C.c_int)], _I),
     "nq_ctx_create": ([_I, _I, _I, _I, _I, C.POINTER(_P)], _I),
@@ -456,3 +497,69 @@ def bench_tm_hop(ctx, g, napply, out, inp, prec="f64"):
     by = C.c_double(0)
     _check(lib().nq_bench_tm_hop(ctx.h, g.h, PREC[prec], napply, out.h, inp.h, C.byref(ms), C.byref(by)))
     return ms.value, by.value
+
+
+class Server:
+    """Propagator-solve server (nq_server_*): resident gauge fields per volume,
+    continuous batching of the sources of queued requests."""
+
+    def __init__(self, device=0, hbm_budget=0, max_groups=1):
+        h = _P()
+        _check(lib().nq_server_create(device, int(hbm_budget), max_groups, C.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def add_volume(self, L, conf_lx):
+        conf = np.ascontiguousarray(conf_lx, dtype=np.float64)
+        vid = C.c_int()
+        _check(lib().nq_server_add_volume(self.h, *dims(L), _dp(conf), C.byref(vid)))
+        return vid.value
+
+    def submit(self, volume, kappa, mu, residue, prec="f16", niter=100000, point_site=None, nsrc=12,
+               srcs=None, sols=None):
+        """Point sources (spin-colour 0..nsrc-1 at lx site point_site) or host sources
+        `srcs` (list of lx arrays); `sols` (list of lx arrays) receive the solutions."""
+        r = ServeRequest()
+        r.volume, r.prec, r.kappa, r.mu, r.residue, r.niter = volume, PREC[prec], kappa, mu, residue, niter
+        keep = []
+        if srcs is not None:
+            srcs = [np.ascontiguousarray(x, dtype=np.float64) for x in srcs]
+            r.nsrc, r.src_kind = len(srcs), 0
+            arr = (_DP * len(srcs))(*[_dp(x) for x in srcs])
+            r.src = C.cast(arr, C.c_void_p)
+            keep += [srcs, arr]
+        else:
+            r.nsrc, r.src_kind, r.site0 = nsrc, 1, int(point_site or 0)
+        if sols is not None:
+            assert len(sols) == r.nsrc and all(x.flags.c_contiguous and x.dtype == np.float64 for x in sols)
+            arr = (_DP * len(sols))(*[_dp(x) for x in sols])
+            r.sol = C.cast(arr, C.c_void_p)
+            keep += [sols, arr]
+        rid = C.c_int64()
+        _check(lib().nq_server_submit(self.h, C.byref(r), C.byref(rid)))
+        self._keep[rid.value] = keep
+        return rid.value
+
+    def wait(self, rid, timeout=600.0):
+        res = ServeResult()
+        _check(lib().nq_server_wait(self.h, rid, timeout, C.byref(res)))
+        self._keep.pop(rid, None)
+        return res.as_dict()
+
+    def clock(self):
+        return lib().nq_server_clock(self.h)
+
+    def device_bytes(self):
+        return lib().nq_server_device_bytes(self.h)
+
+    def close(self):
+        if self.h:
+            lib().nq_server_destroy(self.h)
+            self.h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

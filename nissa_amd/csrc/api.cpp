@@ -163,7 +163,7 @@ using namespace nq;
 
 int64_t nq_gauge::bytes() const
 {
-  return (int64_t)(aos[0].bytes + aos[1].bytes + l64.bytes + l32.bytes + l32c.bytes + l16c.bytes);
+  return (int64_t)(aos[0].bytes + aos[1].bytes + l64.bytes + l32.bytes + l32c.bytes + l16c.bytes + l64c.bytes);
 }
 
 namespace {
@@ -476,6 +476,7 @@ void nq_gauge_destroy(nq_gauge* g)
     dev_free(g->l32);
     dev_free(g->l32c);
     dev_free(g->l16c);
+    dev_free(g->l64c);
   }
   delete g;
 }
@@ -492,17 +493,18 @@ static DevBuf& link_buf(nq_gauge* g, int lk)
   case LK_F64_18: return g->l64;
   case LK_F32_18: return g->l32;
   case LK_F32_12: return g->l32c;
+  case LK_F64_12: return g->l64c;
   default: return g->l16c;
   }
 }
-static size_t link_elem(int lk) { return lk == LK_F64_18 ? 16 : (lk == LK_F16_12 ? 4 : 8); }
-static int link_nc(int lk) { return (lk == LK_F64_18 || lk == LK_F32_18) ? 9 : 6; }
+static size_t link_elem(int lk) { return (size_t)lk_eb(lk); }
+static int link_nc(int lk) { return lk_nc(lk); }
 
 int gauge_build(nq_gauge* g, int lk)
 {
   DevBuf& b = link_buf(g, lk);
   if (b.p) return NQ_OK;
-  if ((lk == LK_F32_12 || lk == LK_F16_12) && !g->compressible)
+  if (lk_nc(lk) == 6 && !g->compressible)
     return fail(NQ_EINVAL, "12-real link compression needs det(U_mu) constant per direction");
   nq_ctx* c = g->ctx;
   const size_t bytes = 2 * 8 * (size_t)link_nc(lk) * c->geo.stride * link_elem(lk);

@@ -224,33 +224,15 @@ __global__ void __launch_bounds__(256) k_spin_diag(BlasArgs A, SpinorView out, S
 // One block: for each right-hand side still running, sum its chunk partials with
 // 256 lanes (lane v takes chunks v, v+256, ... in four interleaved chains),
 // combine the lanes by a fixed pairwise tree and finalise its state.
-__global__ void __launch_bounds__(256) k_finalize(const double* partial, int64_t nchunks, int nr, CgState* st, int fin,
-                                                  int pred_reliable)
+__global__ void __launch_bounds__(1024) k_finalize(const double* partial, int64_t nchunks, CgState* st, int fin,
+                                                   int pred_reliable)
 {
-  __shared__ double vl[256];
-  const int vt = threadIdx.x;
-  for (int rr = 0; rr < nr; rr++) {
-    CgState* s = st + rr;
-    if (s->done || (pred_reliable == 1 && !s->reliable) || (pred_reliable == 2 && s->reliable)) continue;
-    const double* pr = partial + (int64_t)rr * nchunks;
-    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-    int64_t c = vt;
-    for (; c + 768 < nchunks; c += 1024) {
-      a0 += pr[c];
-      a1 += pr[c + 256];
-      a2 += pr[c + 512];
-      a3 += pr[c + 768];
-    }
-    for (; c < nchunks; c += 256) a0 += pr[c];
-    vl[vt] = (a0 + a1) + (a2 + a3);
-    __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-      if (vt < h) vl[vt] += vl[vt + h];
-      __syncthreads();
-    }
-    if (vt == 0) finalize(s, fin, vl[0]);
-    __syncthreads();
-  }
+  __shared__ double vl[1024];
+  CgState* s = st + blockIdx.x;
+  // uniform over the workgroup
+  if (s->done || (pred_reliable == 1 && !s->reliable) || (pred_reliable == 2 && s->reliable)) return;
+  const double tot = block_sum_partials(partial + (int64_t)blockIdx.x * nchunks, nchunks, vl);
+  if (threadIdx.x == 0) finalize(s, fin, tot);
 }
 
 int launch_finalize(nq_ctx* ctx, int nr, CgState* st, int fin, int pred_reliable)
@@ -259,7 +241,7 @@ int launch_finalize(nq_ctx* ctx, int nr, CgState* st, int fin, int pred_reliable
   const int64_t nchunks = (ctx->geo.Vh + 63) / 64;
   hipEvent_t ev = nullptr;
   prof_begin(ctx, "reduce_finalize", 8.0 * nchunks * nr, &ev);
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, ctx->stream, ctx->partial, nchunks, nr, st, fin, pred_reliable);
+  hipLaunchKernelGGL(k_finalize, dim3(nr), dim3(1024), 0, ctx->stream, ctx->partial, nchunks, st, fin, pred_reliable);
   NQ_HIP(hipGetLastError());
   prof_end(ctx, ev);
   return NQ_OK;

@@ -75,8 +75,8 @@ int launch_soa_to_aos(nq_ctx* ctx, const SpinorView& in, int ncomp, int par, int
 template <int LK>
 __device__ __forceinline__ void put_link(void* slot_base, int k, int64_t e, int64_t stride, double re, double im)
 {
-  constexpr int NC = (LK == LK_F64_18 || LK == LK_F32_18) ? 9 : 6;
-  constexpr int EB = LK == LK_F64_18 ? 16 : (LK == LK_F16_12 ? 4 : 8);
+  constexpr int NC = lk_nc(LK);
+  constexpr int EB = lk_eb(LK);
   char* p = (char*)slot_base + Pack<EB, NC>::offset(k, e, stride);
   if constexpr (EB == 16) *(double2*)p = make_double2(re, im);
   else if constexpr (EB == 8) *(float2*)p = make_float2((float)re, (float)im);
@@ -87,14 +87,14 @@ __device__ __forceinline__ void put_link(void* slot_base, int k, int64_t e, int6
 template <int LK>
 __global__ void __launch_bounds__(kBlock) k_build_links(Geo g, const double* ae, const double* ao, void* out)
 {
-  constexpr int NC = (LK == LK_F64_18 || LK == LK_F32_18) ? 9 : 6;
+  constexpr int NC = lk_nc(LK);
   const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
   const int op = blockIdx.y;
   if (e >= (uint32_t)g.Vh) return;
   const double* conf[2] = {ae, ao};
   const Site s = site_of(g, op, e);
   const int64_t ne = nissa_lx(g, s.t, s.x, s.y, s.z) >> 1;
-  constexpr int EB = LK == LK_F64_18 ? 16 : (LK == LK_F16_12 ? 4 : 8);
+  constexpr int EB = lk_eb(LK);
   const int64_t slot_bytes = (int64_t)NC * g.stride * EB;
   char* par_base = (char*)out + (int64_t)op * 8 * slot_bytes;
 #pragma unroll
@@ -124,6 +124,7 @@ int launch_build_links(nq_ctx* ctx, const double* ae, const double* ao, int lk, 
   case LK_F64_18: hipLaunchKernelGGL(k_build_links<LK_F64_18>, grid, dim3(kBlock), 0, ctx->stream, ctx->geo, ae, ao, out); break;
   case LK_F32_18: hipLaunchKernelGGL(k_build_links<LK_F32_18>, grid, dim3(kBlock), 0, ctx->stream, ctx->geo, ae, ao, out); break;
   case LK_F32_12: hipLaunchKernelGGL(k_build_links<LK_F32_12>, grid, dim3(kBlock), 0, ctx->stream, ctx->geo, ae, ao, out); break;
+  case LK_F64_12: hipLaunchKernelGGL(k_build_links<LK_F64_12>, grid, dim3(kBlock), 0, ctx->stream, ctx->geo, ae, ao, out); break;
   default: hipLaunchKernelGGL(k_build_links<LK_F16_12>, grid, dim3(kBlock), 0, ctx->stream, ctx->geo, ae, ao, out); break;
   }
   NQ_HIP(hipGetLastError());

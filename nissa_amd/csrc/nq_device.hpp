@@ -298,6 +298,9 @@ struct SpAcc {
       float m = 0.f;
 #pragma unroll
       for (int k = 0; k < NC; k++) m = fmaxf(m, fmaxf(fabsf((float)s[k].re), fabsf((float)s[k].im)));
+      // sites whose largest component is below 1e-30 are stored as zero: 1/m would
+      // overflow for denormal m (fields far from a point source decay that far)
+      if (!(m >= 1e-30f)) m = 0.f;
       const float inv = m > 0.f ? 1.0f / m : 0.f;
       store_packed<EB, NC, CP, R>(base, stride, e, s, (R)inv);
       v.norm[(int64_t)r * stride + e] = m;
@@ -310,8 +313,8 @@ struct SpAcc {
 // ---------------------------------------------------------------- link access
 template <int LK, class R>
 struct LkAcc {
-  static constexpr int NC = (LK == LK_F64_18 || LK == LK_F32_18) ? 9 : 6;
-  static constexpr int EB = LK == LK_F64_18 ? 16 : (LK == LK_F16_12 ? 4 : 8);
+  static constexpr int NC = lk_nc(LK);
+  static constexpr int EB = lk_eb(LK);
   // U as u[3][3]; slot s is a packed array of NC elements at base + s*NC*stride*EB bytes
   __device__ __forceinline__ static void load(const void* base, int64_t stride, int slot, uint32_t e,
                                               const double* det, cx<R> u[9])
@@ -401,6 +404,14 @@ __device__ inline void finalize(CgState* st, int fin, double total)
     if (rr_true < st->target || st->iter >= st->niter) st->done = 1;
     break;
   }
+  case FIN_INIT_B2:
+    st->b2 = total;
+    st->target *= total;
+    st->rr = total;
+    st->r0 = sqrt(total);
+    st->true_rr = total;
+    if (!(total > 0.0) || isnan(total)) { st->nan = 1; st->done = 1; }
+    break;
   case FIN_RR_INIT:
     st->rr = total;
     st->r0 = sqrt(total);
@@ -468,6 +479,30 @@ __device__ __forceinline__ void store_partial(double v, double* partial, int64_t
 {
   v = wave_sum(v);
   if ((threadIdx.x & 63) == 0 && chunk < nchunks) partial[(int64_t)r * nchunks + chunk] = v;
+}
+
+// One workgroup per right-hand side: 1024 lanes each sum a fixed strided subset of
+// the chunk partials (4 interleaved chains), then a fixed LDS tree down to one
+// wavefront and a fixed shuffle tree.  The order depends only on nchunks.
+__device__ __forceinline__ double block_sum_partials(const double* pr, int64_t nchunks, double* vl)
+{
+  const int vt = threadIdx.x;
+  double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  int64_t c = vt;
+  for (; c + 3072 < nchunks; c += 4096) {
+    a0 += pr[c];
+    a1 += pr[c + 1024];
+    a2 += pr[c + 2048];
+    a3 += pr[c + 3072];
+  }
+  for (; c < nchunks; c += 1024) a0 += pr[c];
+  vl[vt] = (a0 + a1) + (a2 + a3);
+  __syncthreads();
+  for (int h = 512; h >= 64; h >>= 1) {
+    if (vt < h) vl[vt] += vl[vt + h];
+    __syncthreads();
+  }
+  return vt < 64 ? wave_sum(vl[vt]) : 0.0;
 }
 
 }  // namespace nq

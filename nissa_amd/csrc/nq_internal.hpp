@@ -30,7 +30,10 @@ int fail(int code, const char* fmt, ...);
 
 // ---------------------------------------------------------------- storage kinds
 enum SpPrec { SP_F64 = 0, SP_F32 = 1, SP_F16 = 2 };
-enum LkPrec { LK_F64_18 = 0, LK_F32_18 = 1, LK_F32_12 = 2, LK_F16_12 = 3 };
+enum LkPrec { LK_F64_18 = 0, LK_F32_18 = 1, LK_F32_12 = 2, LK_F16_12 = 3, LK_F64_12 = 4 };
+// complex elements stored per link (12-real: rows 0 and 1) and bytes per element
+constexpr int lk_nc(int lk) { return (lk == LK_F64_18 || lk == LK_F32_18) ? 9 : 6; }
+constexpr int lk_eb(int lk) { return (lk == LK_F64_18 || lk == LK_F64_12) ? 16 : (lk == LK_F16_12 ? 4 : 8); }
 
 // Device geometry of one lattice (all extents even).
 // Device site order: t is split as t = th*TB + tl (TB in {2,4,8} dividing T) and
@@ -103,6 +106,7 @@ enum Fin {
   FIN_RR_INIT = 6,    // initial residual: rr = total, r0 = sqrt
   FIN_STORE1 = 7,     // extra[1] = total
   FIN_RR_TRUE_HOLD = 8,  // as FIN_RR_TRUE, but `reliable` stays raised (batched updates)
+  FIN_INIT_B2 = 9,    // start of a solve on r = b: b2 = rr = total, target *= total
 };
 
 // Hop epilogue: out_h = c1_h*hop + c2_h*aux + c3_h*aux2 (h = upper/lower spins).
@@ -177,6 +181,7 @@ struct nq_gauge {
   nq::DevBuf l32;                    // LK_F32_18
   nq::DevBuf l32c;                   // LK_F32_12
   nq::DevBuf l16c;                   // LK_F16_12
+  nq::DevBuf l64c;                   // LK_F64_12
   bool compressible = false;         // per-direction constant det
   double det[4][2] = {};             // det(U_mu) when compressible
   int64_t bytes() const;

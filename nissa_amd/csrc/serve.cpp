@@ -1,0 +1,401 @@
+// Propagator-solve server: continuous batching of eo twisted-mass solves.
+//
+// The caller of this path in nissa is the propagator driver, which solves 12
+// spin-colour sources per (mass, kappa) on one gauge configuration
+// (projects/ib/prop.cpp:41-106,925-964, through inv_tmD_cg_eoprec).  Here such
+// requests arrive over time for several lattice volumes:
+//  * every volume has an engine: its own context (device stream), its gauge field
+//    resident in HBM, and one host thread that drives its batches;
+//  * a batch group is a TmBatch of 16 slots; a queued source takes any free slot of
+//    a group with the same kappa and inner precision, at the next chunk boundary,
+//    while the other slots keep iterating (continuous batching); sources with
+//    different twisted masses share the launches;
+//  * a new group is created only when its device footprint fits the HBM budget
+//    (admission by footprint); otherwise the sources wait for a slot;
+//  * engines of different volumes run on their own streams and threads, so their
+//    kernels overlap on the device.
+// A source's arithmetic is independent of the other slots and of the slot index
+// (TmBatch), so its solution is bit-identical alone and in any mix.
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "nq_internal.hpp"
+#include "tm_batch.hpp"
+
+using namespace nq;
+
+namespace {
+
+struct Req {
+  int64_t id = 0;
+  nq_serve_request r{};
+  std::vector<const double*> src;
+  std::vector<double*> sol;
+  double t_submit = 0, t_admit = -1, t_done = -1;
+  int remaining = 0;
+  int status = NQ_OK;
+  int iters_max = 0, nrel = 0;
+  double max_chk = 0;
+  bool done = false;
+};
+
+struct RhsRef {
+  std::shared_ptr<Req> req;
+  int k = 0;
+};
+
+struct Group {
+  TmBatch b;
+  double kappa = 0;
+  int prec = 0;
+  RhsRef slot[TmBatch::kMaxSlots];
+  bool pending[TmBatch::kMaxSlots] = {};  // assigned, not started yet
+  nq_field* src[TmBatch::kMaxSlots] = {};
+  nq_field* sol[TmBatch::kMaxSlots] = {};
+  int64_t bytes = 0;
+  bool ready = false;
+  ~Group()
+  {
+    for (int i = 0; i < TmBatch::kMaxSlots; i++) {
+      nq_field_destroy(src[i]);
+      nq_field_destroy(sol[i]);
+    }
+  }
+  bool has_free() const
+  {
+    for (int i = 0; i < TmBatch::kMaxSlots; i++)
+      if (!slot[i].req) return true;
+    return false;
+  }
+  bool busy() const
+  {
+    for (int i = 0; i < TmBatch::kMaxSlots; i++)
+      if (slot[i].req) return true;
+    return false;
+  }
+};
+
+struct Engine {
+  int id = 0;
+  int dims[4] = {};
+  nq_ctx* ctx = nullptr;
+  nq_gauge* g = nullptr;
+  std::deque<RhsRef> queue;
+  std::vector<std::unique_ptr<Group>> groups;
+  std::thread th;
+};
+
+}  // namespace
+
+struct nq_server {
+  int device = 0;
+  int64_t budget = 0;
+  int max_groups = 1;
+  std::mutex mu;
+  std::condition_variable cv_done;
+  std::vector<std::unique_ptr<Engine>> engines;
+  std::vector<std::unique_ptr<std::condition_variable>> cv_work;
+  std::map<int64_t, std::shared_ptr<Req>> reqs;
+  int64_t next_id = 1;
+  int64_t used = 0;
+  bool stop = false;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+};
+
+namespace {
+
+int64_t group_bytes(const nq_ctx* c, int prec)
+{
+  const int64_t lx = 2 * 12 * 16 * c->geo.stride;  // one FP64 lx spincolor field
+  return TmBatch::kMaxSlots * (TmBatch::bytes_per_slot(c->geo, prec) + 2 * lx) + (64 << 20);
+}
+
+// point source: 1 at lx site `site` and spin-colour component sc (id*3+ic), 0 elsewhere
+int point_source(nq_ctx* c, nq_field* f, int64_t site, int sc)
+{
+  NQ_TRY(launch_zero(c, view_lx_flat(f)));
+  const Geo& g = c->geo;
+  int64_t r = site;
+  const int z = (int)(r % g.LZ);
+  r /= g.LZ;
+  const int y = (int)(r % g.LY);
+  r /= g.LY;
+  const int x = (int)(r % g.LX);
+  const int t = (int)(r / g.LX);
+  const int64_t dlx = (int64_t)(t / g.TB) * g.Sth + x * g.Sx + y * g.Sy + z * g.Sz + (t % g.TB);
+  const int par = (t + x + y + z) & 1;
+  static const double one[2] = {1.0, 0.0};
+  char* p = (char*)f->buf.p + ((int64_t)(par * 12 + sc) * g.stride + (dlx >> 1)) * 16;
+  NQ_HIP(hipMemcpyAsync(p, one, sizeof one, hipMemcpyHostToDevice, c->stream));
+  return NQ_OK;
+}
+
+void finish_source(nq_server* s, Req& q, int k, int rc, const nq_solve_stats& st)
+{
+  q.iters_max = std::max(q.iters_max, st.iters);
+  q.nrel += st.reliable_updates;
+  q.max_chk = std::max(q.max_chk, st.rel_res_sq);
+  if (rc != NQ_OK && q.status == NQ_OK) q.status = rc;
+  (void)k;
+  if (--q.remaining == 0) {
+    q.t_done = s->now();
+    q.done = true;
+  }
+}
+
+void engine_loop(nq_server* s, Engine* E)
+{
+  (void)hipSetDevice(s->device);
+  nq_ctx* c = E->ctx;
+  std::condition_variable& cv = *s->cv_work[E->id];
+  for (;;) {
+    std::vector<std::pair<Group*, int>> starts;
+    std::vector<Group*> fresh;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      if (s->stop) return;
+      // admission: queued sources into free slots of compatible groups
+      while (!E->queue.empty()) {
+        RhsRef& q = E->queue.front();
+        Group* sel = nullptr;
+        for (auto& gp : E->groups)
+          if (gp->kappa == q.req->r.kappa && gp->prec == q.req->r.prec && gp->has_free()) {
+            sel = gp.get();
+            break;
+          }
+        if (!sel && (int)E->groups.size() < std::min(s->max_groups, TmBatch::kMaxGroups)) {
+          const int64_t need = group_bytes(c, q.req->r.prec);
+          if (s->used + need <= s->budget) {
+            auto gp = std::make_unique<Group>();
+            gp->kappa = q.req->r.kappa;
+            gp->prec = q.req->r.prec;
+            gp->bytes = need;
+            s->used += need;
+            sel = gp.get();
+            fresh.push_back(sel);
+            E->groups.push_back(std::move(gp));
+          }
+        }
+        if (!sel) break;  // wait for a slot or for memory
+        int i = 0;
+        while (sel->slot[i].req) i++;
+        sel->slot[i] = q;
+        sel->pending[i] = true;
+        if (q.req->t_admit < 0) q.req->t_admit = s->now();
+        starts.push_back({sel, i});
+        E->queue.pop_front();
+      }
+      bool active = !starts.empty();
+      for (auto& gp : E->groups) active |= gp->busy();
+      if (!active) {
+        if (E->queue.empty()) {
+          cv.wait(lk, [&] { return s->stop || !E->queue.empty(); });
+        } else if (E->groups.empty()) {
+          // not even one batch group fits the HBM budget: refuse the queued sources
+          while (!E->queue.empty()) {
+            RhsRef q = E->queue.front();
+            E->queue.pop_front();
+            if (q.req->t_admit < 0) q.req->t_admit = s->now();
+            finish_source(s, *q.req, q.k, NQ_EBUSY, nq_solve_stats{});
+            if (q.req->done) s->cv_done.notify_all();
+          }
+        } else {
+          // incompatible work waits for a group of its own kappa/precision
+          cv.wait_for(lk, std::chrono::milliseconds(5));
+        }
+        continue;
+      }
+    }
+    // new groups: device set-up (group index = position in the engine)
+    for (Group* gp : fresh) {
+      int gi = 0;
+      while (E->groups[gi].get() != gp) gi++;
+      int rc = gp->b.setup(c, E->g, gp->kappa, nullptr, gp->prec, gi);
+      for (int i = 0; rc == NQ_OK && i < TmBatch::kMaxSlots; i++) {
+        rc = nq_field_create(c, NQ_SPINCOLOR, NQ_LX, &gp->src[i]);
+        if (rc == NQ_OK) rc = nq_field_create(c, NQ_SPINCOLOR, NQ_LX, &gp->sol[i]);
+      }
+      gp->ready = rc == NQ_OK;
+    }
+    // start the admitted sources
+    for (auto& [gp, i] : starts) {
+      RhsRef ref = gp->slot[i];
+      Req& q = *ref.req;
+      int rc = gp->ready ? NQ_OK : NQ_EOOM;
+      if (rc == NQ_OK)
+        rc = q.r.src_kind == 1 ? point_source(c, gp->src[i], q.r.site0, ref.k) : nq_field_upload(gp->src[i], q.src[ref.k]);
+      if (rc == NQ_OK)
+        rc = gp->b.start(i, q.r.mu, q.r.residue, q.r.niter, view_of(gp->src[i], 0), view_of(gp->src[i], 1));
+      gp->pending[i] = false;
+      if (rc != NQ_OK) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        finish_source(s, q, ref.k, rc, nq_solve_stats{});
+        gp->slot[i] = RhsRef{};
+        if (q.done) s->cv_done.notify_all();
+      }
+    }
+    // iterate every group that has work, then poll and retire finished sources
+    for (auto& gp : E->groups)
+      if (gp->ready && gp->b.nactive() > 0) (void)gp->b.run(std::min(gp->b.suggested_chunk(), 32));
+    for (auto& gp : E->groups) {
+      if (!gp->ready || gp->b.nactive() == 0) continue;
+      std::vector<int> fin;
+      int prc = gp->b.poll(&fin);
+      if (prc != NQ_OK) {
+        // device error: fail every source of the group
+        for (int i = 0; i < TmBatch::kMaxSlots; i++)
+          if (gp->slot[i].req && gp->b.active(i)) fin.push_back(i);
+      }
+      for (int i : fin) {
+        RhsRef ref = gp->slot[i];
+        Req& q = *ref.req;
+        nq_solve_stats st{};
+        double chk = 0;
+        int rc = gp->b.finish(i, view_of(gp->src[i], 0), view_of(gp->src[i], 1), view_of(gp->sol[i], 0),
+                              view_of(gp->sol[i], 1), &st, &chk);
+        if (prc != NQ_OK) rc = prc;
+        if (rc != NQ_ENAN && q.sol[ref.k]) {
+          const int d = nq_field_download(gp->sol[i], q.sol[ref.k]);
+          if (rc == NQ_OK) rc = d;
+        }
+        std::lock_guard<std::mutex> lk(s->mu);
+        finish_source(s, q, ref.k, rc, st);
+        gp->slot[i] = RhsRef{};
+        if (q.done) s->cv_done.notify_all();
+      }
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int nq_server_create(int device, int64_t hbm_budget_bytes, int max_groups_per_volume, nq_server** out)
+{
+  if (!out) return fail(NQ_EINVAL, "null argument");
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return fail(NQ_EDEVICE, "no device %d", device);
+  auto s = std::make_unique<nq_server>();
+  s->device = device;
+  s->max_groups = std::max(1, std::min(max_groups_per_volume, TmBatch::kMaxGroups));
+  if (hbm_budget_bytes <= 0) {
+    size_t fr = 0, tot = 0;
+    NQ_HIP(hipSetDevice(device));
+    NQ_HIP(hipMemGetInfo(&fr, &tot));
+    hbm_budget_bytes = (int64_t)fr - ((int64_t)4 << 30);
+  }
+  s->budget = hbm_budget_bytes;
+  *out = s.release();
+  return NQ_OK;
+}
+
+int nq_server_add_volume(nq_server* s, int T, int LX, int LY, int LZ, const double* conf_lx, int* volume_id)
+{
+  if (!s || !conf_lx || !volume_id) return fail(NQ_EINVAL, "null argument");
+  auto E = std::make_unique<Engine>();
+  E->dims[0] = T, E->dims[1] = LX, E->dims[2] = LY, E->dims[3] = LZ;
+  NQ_TRY(nq_ctx_create(T, LX, LY, LZ, s->device, &E->ctx));
+  int rc = nq_gauge_create_lx(E->ctx, conf_lx, &E->g);
+  if (rc != NQ_OK) {
+    nq_ctx_destroy(E->ctx);
+    return rc;
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->used += nq_gauge_device_bytes(E->g);
+  E->id = (int)s->engines.size();
+  *volume_id = E->id;
+  s->cv_work.push_back(std::make_unique<std::condition_variable>());
+  Engine* e = E.get();
+  s->engines.push_back(std::move(E));
+  e->th = std::thread(engine_loop, s, e);
+  return NQ_OK;
+}
+
+int nq_server_submit(nq_server* s, const nq_serve_request* r, int64_t* id)
+{
+  if (!s || !r) return fail(NQ_EINVAL, "null argument");
+  if (r->nsrc < 1) return fail(NQ_EINVAL, "request without sources");
+  if (r->src_kind == 1 && r->nsrc > 12) return fail(NQ_EINVAL, "point sources: at most 12 spin-colour components");
+  if (r->src_kind == 0 && !r->src) return fail(NQ_EINVAL, "host sources missing");
+  if (r->src_kind != 0 && r->src_kind != 1) return fail(NQ_EINVAL, "unknown source kind %d", r->src_kind);
+  if (r->prec < NQ_PREC_F64 || r->prec > NQ_PREC_F16H) return fail(NQ_EINVAL, "unknown precision %d", r->prec);
+  if (!(r->residue > 0) || r->niter <= 0 || !(r->kappa != 0)) return fail(NQ_EINVAL, "bad kappa/residue/niter");
+  auto q = std::make_shared<Req>();
+  q->r = *r;
+  for (int k = 0; k < r->nsrc; k++) {
+    if (r->src_kind == 0 && !r->src[k]) return fail(NQ_EINVAL, "null source %d", k);
+    q->src.push_back(r->src_kind == 0 ? r->src[k] : nullptr);
+    q->sol.push_back(r->sol ? r->sol[k] : nullptr);
+  }
+  q->r.src = nullptr;
+  q->r.sol = nullptr;
+  q->remaining = r->nsrc;
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (r->volume < 0 || r->volume >= (int)s->engines.size()) return fail(NQ_EINVAL, "unknown volume %d", r->volume);
+  q->id = s->next_id++;
+  q->t_submit = s->now();
+  s->reqs[q->id] = q;
+  Engine* E = s->engines[r->volume].get();
+  for (int k = 0; k < r->nsrc; k++) E->queue.push_back(RhsRef{q, k});
+  s->cv_work[r->volume]->notify_one();
+  if (id) *id = q->id;
+  return NQ_OK;
+}
+
+int nq_server_wait(nq_server* s, int64_t id, double timeout_s, nq_serve_result* out)
+{
+  if (!s || !out) return fail(NQ_EINVAL, "null argument");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return fail(NQ_EINVAL, "unknown request %lld", (long long)id);
+  std::shared_ptr<Req> q = it->second;
+  const auto until = std::chrono::steady_clock::now() + std::chrono::duration<double>(timeout_s);
+  if (!s->cv_done.wait_until(lk, until, [&] { return q->done; })) return fail(NQ_EBUSY, "request %lld still running", (long long)id);
+  out->id = q->id;
+  out->status = q->status;
+  out->nsrc = q->r.nsrc;
+  out->iters_max = q->iters_max;
+  out->reliable_updates = q->nrel;
+  out->max_rel_res_sq = q->max_chk;
+  out->t_submit = q->t_submit;
+  out->t_admit = q->t_admit;
+  out->t_done = q->t_done;
+  s->reqs.erase(id);
+  return NQ_OK;
+}
+
+double nq_server_clock(const nq_server* s) { return s ? s->now() : 0.0; }
+
+int64_t nq_server_device_bytes(const nq_server* s) { return s ? s->used : 0; }
+
+void nq_server_destroy(nq_server* s)
+{
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+    for (auto& cv : s->cv_work) cv->notify_all();
+  }
+  for (auto& E : s->engines)
+    if (E->th.joinable()) E->th.join();
+  for (auto& E : s->engines) {
+    E->groups.clear();
+    nq_gauge_destroy(E->g);
+    nq_ctx_destroy(E->ctx);
+  }
+  delete s;
+}
+
+}  // extern "C"

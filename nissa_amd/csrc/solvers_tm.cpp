@@ -21,6 +21,7 @@
 #include <cstring>
 
 #include "nq_internal.hpp"
+#include "tm_batch.hpp"
 
 namespace nq {
 
@@ -76,6 +77,15 @@ static int tm_setup(nq_ctx* ctx, const nq_gauge* g, int lk, const double* anis, 
   s->anis = anis ? *anis : 1.0;
   s->kappa = kappa;
   return link_view(g, lk, &s->U);
+}
+
+// FP64 links of the work around a sloppy solve (reliable updates, eo preparation and
+// reconstruction): 12-real when the gauge field allows exact compression (two rows
+// stored, the third rebuilt in FP64), which cuts the FP64 hop's link bytes by a third;
+// the plain FP64 solver and the final FP64 check keep the 18-real links.
+static int lk64_for(int prec, const nq_gauge* g)
+{
+  return (prec != NQ_PREC_F64 && g->compressible) ? LK_F64_12 : LK_F64_18;
 }
 
 // K(mu): out_odd = K in_odd, using tmp_even
@@ -372,7 +382,7 @@ static int cg_kernsq_any(nq_ctx* ctx, const nq_gauge* g, double kappa, const dou
   if (niter <= 0) return fail(NQ_EINVAL, "niter must be positive");
   if (!(residue > 0)) return fail(NQ_EINVAL, "residue must be positive");
   TmSetup S64;
-  NQ_TRY(tm_setup(ctx, g, LK_F64_18, anis, kappa, &S64));
+  NQ_TRY(tm_setup(ctx, g, lk64_for(prec, g), anis, kappa, &S64));
   nq_solve_stats dummy;
   if (!st) st = &dummy;
   std::memset(st, 0, sizeof *st);
@@ -459,142 +469,298 @@ static int residual_kernsq_batched(const TmSetup& S, const double* coef, int nr,
   return launch_tm_hop(ctx, NQ_ODD, none, tmp, S.U, S.anis, e);
 }
 
-// CG on K(+mu_r) K(-mu_r) y_r = b_r for nr <= 16 right-hand sides sharing the gauge
-// field and kappa.  Every launch serves all right-hand sides; a right-hand side that
-// has converged (or waits for its reliable update) idles in the kernels.  The
-// arithmetic of one right-hand side does not depend on the others in the batch
-// (per-RHS coefficient rows, per-RHS states, per-(RHS, 64-site chunk) reduction
-// partials), so a solve gives the same bits alone or in any batch.
-// prec NQ_PREC_F64: FP64 CG; otherwise sloppy iterations with FP64 reliable updates.
-static int cg_kernsq_batched(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis, int nr,
-                             const double* mus, int niter, double residue, int prec, const SpinorView& x64,
-                             const SpinorView& b, nq_solve_stats* out)
+// ---------------------------------------------------------------- TmBatch
+int64_t TmBatch::bytes_per_slot(const Geo& g, int prec)
 {
-  int sp, lk;
-  NQ_TRY(lk_of_prec(prec, &sp, &lk));
-  if (nr < 1 || nr > 16) return fail(NQ_EINVAL, "batched CG: 1..16 right-hand sides");
-  const bool mixed = prec != NQ_PREC_F64;
-  TmSetup S64, Ss;
-  NQ_TRY(tm_setup(ctx, g, LK_F64_18, anis, kappa, &S64));
-  if (mixed) NQ_TRY(tm_setup(ctx, g, lk, anis, kappa, &Ss));
-  else Ss = S64;
-  double* coef = nullptr;
-  NQ_TRY(upload_kernsq_coefs(ctx, kappa, nr, mus, &coef));
-  SpinorView r64, t64, tmp64, xs, rs, p, t1s, tmps;
-  NQ_TRY(alloc_spinor_nr(ctx, "bcg.r", SP_F64, 12, nr, &r64));
-  NQ_TRY(alloc_spinor_nr(ctx, "bcg.t1", SP_F64, 12, nr, &t64));
-  NQ_TRY(alloc_spinor_nr(ctx, "bcg.tmp", SP_F64, 12, nr, &tmp64));
-  if (mixed) {
-    NQ_TRY(alloc_spinor_nr(ctx, "bcg.xs", SP_F32, 12, nr, &xs));
-    NQ_TRY(alloc_spinor_nr(ctx, "bcg.rs", SP_F32, 12, nr, &rs));
-    NQ_TRY(alloc_spinor_nr(ctx, sp == SP_F16 ? "bcg.p16" : "bcg.p32", sp, 12, nr, &p));
-    NQ_TRY(alloc_spinor_nr(ctx, sp == SP_F16 ? "bcg.t16" : "bcg.t32", sp, 12, nr, &t1s));
-    NQ_TRY(alloc_spinor_nr(ctx, sp == SP_F16 ? "bcg.e16" : "bcg.e32", sp, 12, nr, &tmps));
+  const int64_t s = g.stride;
+  int64_t b = 5 * 12 * 16 * s;  // b, x64, r64, t64, tmp64
+  if (prec != NQ_PREC_F64) {
+    b += 2 * 12 * 8 * s;                                                       // xs, rs
+    b += 3 * (prec == NQ_PREC_F32 ? 12 * 8 * s : 12 * 4 * s + 4 * s);          // p, t1, tmp
   } else {
-    NQ_TRY(alloc_spinor_nr(ctx, "bcg.p", SP_F64, 12, nr, &p));
-    xs = x64;
-    rs = r64;
-    t1s = t64;
-    tmps = tmp64;
+    b += 12 * 16 * s;  // p
   }
-  Timer T(ctx->stream);
-  CgState* S = ctx->state;
+  return b;
+}
+
+int TmBatch::setup(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis, int prec, int group)
+{
+  if (group < 0 || group >= kMaxGroups) return fail(NQ_EINVAL, "batch group %d out of range", group);
+  NQ_TRY(lk_of_prec(prec, &sp_, &lk_));
+  ctx_ = ctx;
+  g_ = g;
+  kappa_ = kappa;
+  has_anis_ = anis != nullptr;
+  anis_ = anis ? *anis : 1.0;
+  prec_ = prec;
+  group_ = group;
+  mixed_ = prec != NQ_PREC_F64;
+  S_ = ctx->state + 16 * group;
+  TmSetup chk;
+  NQ_TRY(tm_setup(ctx, g, lk64_for(prec, g), anis, kappa, &chk));  // validates kappa / builds links
+  if (mixed_) NQ_TRY(tm_setup(ctx, g, lk_, anis, kappa, &chk));
+  const std::string t = "tb" + std::to_string(group) + ".";
+  const int n = kMaxSlots;
+  NQ_TRY(ctx_temp(ctx, t + "coef", sizeof(double) * TB_N * 16 * 12, (void**)&coef_));
+  hcoef_.assign(TB_N * 16 * 12, 0.0);
+  NQ_TRY(alloc_spinor_nr(ctx, t + "b", SP_F64, 12, n, &b_));
+  NQ_TRY(alloc_spinor_nr(ctx, t + "x", SP_F64, 12, n, &x64_));
+  NQ_TRY(alloc_spinor_nr(ctx, t + "r", SP_F64, 12, n, &r64_));
+  NQ_TRY(alloc_spinor_nr(ctx, t + "t1", SP_F64, 12, n, &t64_));
+  NQ_TRY(alloc_spinor_nr(ctx, t + "tmp", SP_F64, 12, n, &tmp64_));
+  NQ_TRY(alloc_spinor(ctx, t + "et", SP_F64, 12, &et_));
+  if (mixed_) {
+    NQ_TRY(alloc_spinor_nr(ctx, t + "xs", SP_F32, 12, n, &xs_));
+    NQ_TRY(alloc_spinor_nr(ctx, t + "rs", SP_F32, 12, n, &rs_));
+    NQ_TRY(alloc_spinor_nr(ctx, t + "p" + std::to_string(sp_), sp_, 12, n, &p_));
+    NQ_TRY(alloc_spinor_nr(ctx, t + "t" + std::to_string(sp_), sp_, 12, n, &t1s_));
+    NQ_TRY(alloc_spinor_nr(ctx, t + "e" + std::to_string(sp_), sp_, 12, n, &tmps_));
+  } else {
+    NQ_TRY(alloc_spinor_nr(ctx, t + "p", SP_F64, 12, n, &p_));
+    xs_ = x64_;
+    rs_ = r64_;
+    t1s_ = t64_;
+    tmps_ = tmp64_;
+  }
+  // every slot starts idle
+  for (int i = 0; i < n; i++) {
+    CgState s = {};
+    s.done = 1;
+    ctx->state_host[16 * group + i] = s;
+    active_[i] = false;
+  }
+  NQ_HIP(hipMemcpyAsync(S_, ctx->state_host + 16 * group, sizeof(CgState) * n, hipMemcpyHostToDevice, ctx->stream));
+  NQ_HIP(hipStreamSynchronize(ctx->stream));
+  chunk_ = 8;
+  return NQ_OK;
+}
+
+int TmBatch::nactive() const
+{
+  int n = 0;
+  for (int i = 0; i < kMaxSlots; i++) n += active_[i];
+  return n;
+}
+
+int TmBatch::hi() const
+{
+  int h = 0;
+  for (int i = 0; i < kMaxSlots; i++)
+    if (active_[i]) h = i + 1;
+  return h;
+}
+
+int TmBatch::upload_row(int slot, double mu)
+{
+  SpinorView none;
+  double* h = hcoef_.data();
+  put_coef(&h[(TB_INVDEE_M * 16 + slot) * 12], epi_inv_dee(kappa_, -mu));
+  put_coef(&h[(TB_KOUT_M * 16 + slot) * 12], epi_kout(kappa_, -mu, none));
+  put_coef(&h[(TB_INVDEE_P * 16 + slot) * 12], epi_inv_dee(kappa_, mu));
+  put_coef(&h[(TB_KOUT_P * 16 + slot) * 12], epi_kout(kappa_, mu, none));
+  HopEpi e = epi_kout(kappa_, mu, none);
+  for (int a = 0; a < 2; a++)
+    for (int k = 0; k < 2; k++) { e.c1[a][k] = -e.c1[a][k]; e.c2[a][k] = -e.c2[a][k]; }
+  setc(e.c3, 1.0, 1.0);
+  put_coef(&h[(TB_RES * 16 + slot) * 12], e);
+  for (int t = 0; t < TB_N; t++)
+    NQ_HIP(hipMemcpyAsync(coef_ + (t * 16 + slot) * 12, &h[(t * 16 + slot) * 12], sizeof(double) * 12,
+                          hipMemcpyHostToDevice, ctx_->stream));
+  return NQ_OK;
+}
+
+static SpinorView slot_view(const SpinorView& v, int slot)
+{
+  SpinorView o = v;
+  const int eb = v.prec == SP_F64 ? 16 : (v.prec == SP_F32 ? 8 : 4);
+  o.data = (char*)v.data + (int64_t)slot * v.rstride * eb;
+  if (v.norm) o.norm = v.norm + (int64_t)slot * (v.rstride / v.ncomp);
+  return o;
+}
+
+int TmBatch::start(int slot, double mu, double residue, int niter, const SpinorView& src_e, const SpinorView& src_o)
+{
+  if (slot < 0 || slot >= kMaxSlots) return fail(NQ_EINVAL, "slot %d out of range", slot);
+  if (active_[slot]) return fail(NQ_EINVAL, "slot %d is busy", slot);
+  if (niter <= 0 || !(residue > 0)) return fail(NQ_EINVAL, "bad niter/residue");
+  nq_ctx* c = ctx_;
+  TmSetup S;
+  NQ_TRY(tm_setup(c, g_, lk64_for(prec_, g_), has_anis_ ? &anis_ : nullptr, kappa_, &S));
+  NQ_TRY(upload_row(slot, mu));
+  mu_[slot] = mu;
+  const double a = 1 / (2 * kappa_), n = a * a + mu * mu;
   SpinorView none;
   none.ncomp = 12;
-  std::vector<CgState> hs(nr);
+  // eq. 8a/8b: b = g5 (eta_o + 1/2 H_oe D_ee^-1 eta_e)
+  double ca[2][2], cb[2][2] = {{0, 0}, {0, 0}};
+  setc(ca, cplx(a / n, -mu / n), cplx(a / n, mu / n));
+  NQ_TRY(launch_spin_diag(c, et_, ca, src_e, cb, none));
+  HopEpi e8;
+  setc(e8.c1, 0.5, -0.5);
+  setc(e8.c2, 1.0, -1.0);
+  e8.aux = src_o;
+  const SpinorView bs = slot_view(b_, slot);
+  NQ_TRY(launch_tm_hop(c, NQ_ODD, bs, et_, S.U, S.anis, e8));
+  // solver state of the slot; FIN_INIT_B2 completes it from |b|^2 on the device
+  CgState s0 = {};
+  s0.target = residue;
+  s0.niter = niter;
+  s0.mixed = mixed_;
+  s0.delta = 1e-2;
+  c->state_host[16 * group_ + slot] = s0;
+  NQ_HIP(hipMemcpyAsync(S_ + slot, c->state_host + 16 * group_ + slot, sizeof(CgState), hipMemcpyHostToDevice,
+                        c->stream));
+  NQ_TRY(launch_lincomb(c, slot_view(r64_, slot), 1.0, bs, 0, none, 0, none, FIN_INIT_B2, S_ + slot, 1));
+  NQ_TRY(launch_zero(c, slot_view(x64_, slot)));
+  if (mixed_) {
+    NQ_TRY(launch_convert(c, slot_view(rs_, slot), slot_view(r64_, slot)));
+    NQ_TRY(launch_zero(c, slot_view(xs_, slot)));
+  }
+  NQ_TRY(launch_convert(c, slot_view(p_, slot), slot_view(rs_, slot)));
+  active_[slot] = true;
+  niter_[slot] = niter;
+  launched_[slot] = 0;
+  return NQ_OK;
+}
+
+int TmBatch::run(int iters)
+{
+  const int nr = hi();
+  if (nr == 0) return NQ_OK;
+  nq_ctx* c = ctx_;
+  TmSetup Ss;
+  NQ_TRY(tm_setup(c, g_, mixed_ ? lk_ : LK_F64_18, has_anis_ ? &anis_ : nullptr, kappa_, &Ss));
+  const int pred = mixed_ ? 2 : 0;
+  SpinorView none;
+  none.ncomp = 12;
+  for (int it = 0; it < iters; it++) {
+    NQ_TRY(launch_tm_hop(c, NQ_EVEN, tmps_, p_, Ss.U, Ss.anis, bepi(coef_, TB_INVDEE_M, nr, S_, pred)));
+    HopEpi b1 = bepi(coef_, TB_KOUT_M, nr, S_, pred);
+    b1.aux = p_;
+    b1.mode = 1;
+    b1.fin = FIN_PAP;
+    NQ_TRY(launch_tm_hop(c, NQ_ODD, t1s_, tmps_, Ss.U, Ss.anis, b1));
+    NQ_TRY(launch_tm_hop(c, NQ_EVEN, tmps_, t1s_, Ss.U, Ss.anis, bepi(coef_, TB_INVDEE_P, nr, S_, pred)));
+    HopEpi b2e = bepi(coef_, TB_KOUT_P, nr, S_, pred);
+    b2e.aux = t1s_;
+    b2e.mode = 2;
+    b2e.fin = mixed_ ? FIN_RR_SLOPPY : FIN_RR;
+    b2e.x = xs_;
+    b2e.p = p_;
+    b2e.r = rs_;
+    NQ_TRY(launch_tm_hop(c, NQ_ODD, none, tmps_, Ss.U, Ss.anis, b2e));
+    NQ_TRY(launch_cg_pupdate(c, p_, rs_, S_, nr));
+  }
+  for (int i = 0; i < nr; i++)
+    if (active_[i]) launched_[i] += iters;
+  return NQ_OK;
+}
+
+int TmBatch::poll(std::vector<int>* finished)
+{
+  const int nr = hi();
+  if (nr == 0) return NQ_OK;
+  nq_ctx* c = ctx_;
+  const int base = 16 * group_;
+  NQ_TRY(get_states(c, base, nr));
+  bool any_rel = false;
+  for (int i = 0; i < nr; i++) any_rel |= active_[i] && c->state_host[base + i].reliable && !c->state_host[base + i].done;
+  if (any_rel) {
+    // reliable update of the flagged slots: x64 += xs, xs = 0; r64 = b - A x64 in FP64,
+    // rs = r64; p = rs + beta p; then clear the flags
+    TmSetup S64;
+    NQ_TRY(tm_setup(c, g_, lk64_for(prec_, g_), has_anis_ ? &anis_ : nullptr, kappa_, &S64));
+    NQ_TRY(launch_reliable_accum(c, x64_, xs_, S_, nr));
+    NQ_TRY(residual_kernsq_batched(S64, coef_, nr, r64_, rs_, x64_, b_, t64_, tmp64_, S_, FIN_RR_TRUE_HOLD, 1));
+    NQ_TRY(launch_cg_pupdate(c, p_, rs_, S_, nr, 2));
+    NQ_TRY(launch_clear_reliable(c, S_, nr));
+    NQ_TRY(get_states(c, base, nr));
+  }
+  int next = 64;
   for (int i = 0; i < nr; i++) {
-    hs[i] = CgState{};
-    put_state(ctx, i, hs[i]);
-  }
-  NQ_TRY(launch_lincomb(ctx, none, 1.0, b, 0, none, 0, none, FIN_STORE0, S, nr));
-  NQ_TRY(get_states(ctx, 0, nr));
-  std::vector<double> b2(nr);
-  for (int i = 0; i < nr; i++) {
-    b2[i] = ctx->state_host[i].extra[0];
-    if (b2[i] == 0 || std::isnan(b2[i])) return fail(NQ_ENAN, "invalid norm: %g (right-hand side %d)", b2[i], i);
-    CgState s0 = {};
-    s0.b2 = b2[i];
-    s0.target = residue * b2[i];
-    s0.niter = niter;
-    s0.mixed = mixed;
-    s0.delta = 1e-2;
-    NQ_TRY(put_state(ctx, i, s0));
-  }
-  NQ_TRY(launch_zero(ctx, x64, nr));
-  NQ_TRY(launch_lincomb(ctx, r64, 1.0, b, 0, none, 0, none, FIN_RR_INIT, S, nr));
-  if (mixed) {
-    NQ_TRY(launch_convert(ctx, rs, r64, nr));
-    NQ_TRY(launch_zero(ctx, xs, nr));
-  }
-  NQ_TRY(launch_convert(ctx, p, rs, nr));
-  NQ_TRY(get_states(ctx, 0, nr));
-  std::vector<double> rr0(nr);
-  for (int i = 0; i < nr; i++) rr0[i] = ctx->state_host[i].rr;
-  const int pred = mixed ? 2 : 0;
-  int chunk = 8, launched = 0;
-  for (;;) {
-    for (int it = 0; it < chunk; it++) {
-      NQ_TRY(launch_tm_hop(ctx, NQ_EVEN, tmps, p, Ss.U, Ss.anis, bepi(coef, TB_INVDEE_M, nr, S, pred)));
-      HopEpi b1 = bepi(coef, TB_KOUT_M, nr, S, pred);
-      b1.aux = p;
-      b1.mode = 1;
-      b1.fin = FIN_PAP;
-      NQ_TRY(launch_tm_hop(ctx, NQ_ODD, t1s, tmps, Ss.U, Ss.anis, b1));
-      NQ_TRY(launch_tm_hop(ctx, NQ_EVEN, tmps, t1s, Ss.U, Ss.anis, bepi(coef, TB_INVDEE_P, nr, S, pred)));
-      HopEpi b2e = bepi(coef, TB_KOUT_P, nr, S, pred);
-      b2e.aux = t1s;
-      b2e.mode = 2;
-      b2e.fin = mixed ? FIN_RR_SLOPPY : FIN_RR;
-      b2e.x = xs;
-      b2e.p = p;
-      b2e.r = rs;
-      NQ_TRY(launch_tm_hop(ctx, NQ_ODD, none, tmps, Ss.U, Ss.anis, b2e));
-      NQ_TRY(launch_cg_pupdate(ctx, p, rs, S, nr));
+    if (!active_[i]) continue;
+    const CgState& h = c->state_host[base + i];
+    // a slot ends when its state says so; the launch count bounds the wait (every
+    // launched iteration either advances the state or idles a finished slot)
+    if (h.done || launched_[i] > 2 * niter_[i] + 64) {
+      finished->push_back(i);
+      continue;
     }
-    launched += chunk;
-    NQ_TRY(get_states(ctx, 0, nr));
-    bool any_rel = false;
-    for (int i = 0; i < nr; i++) any_rel |= ctx->state_host[i].reliable && !ctx->state_host[i].done;
-    if (any_rel) {
-      // reliable update of the flagged right-hand sides: x64 += xs, xs = 0;
-      // r64 = b - A x64 in FP64, rs = r64; p = rs + beta p; then clear the flags
-      NQ_TRY(launch_reliable_accum(ctx, x64, xs, S, nr));
-      NQ_TRY(residual_kernsq_batched(S64, coef, nr, r64, rs, x64, b, t64, tmp64, S, FIN_RR_TRUE_HOLD, 1));
-      NQ_TRY(launch_cg_pupdate(ctx, p, rs, S, nr, 2));
-      NQ_TRY(launch_clear_reliable(ctx, S, nr));
-      NQ_TRY(get_states(ctx, 0, nr));
-    }
-    bool all = true;
-    int next = 64;
-    for (int i = 0; i < nr; i++) {
-      const CgState& h = ctx->state_host[i];
-      if (h.nan) return fail(NQ_ENAN, "nanned (right-hand side %d)", i);
-      if (h.done) continue;
-      all = false;
-      const double goal = mixed ? std::max(h.delta * h.delta * h.r0 * h.r0, h.target) : h.target;
-      next = std::min(next, next_chunk(rr0[i], h.rr, goal, h.iter, chunk));
-    }
-    if (all || launched > niter + 64) break;
-    chunk = std::max(2, next);
+    const double goal = mixed_ ? std::max(h.delta * h.delta * h.r0 * h.r0, h.target) : h.target;
+    next = std::min(next, next_chunk(h.b2, h.rr, goal, h.iter, chunk_));
   }
-  // FP64 true residual of every right-hand side (states 16..16+nr as scratch)
-  CgState* sc = ctx->state + 16;
-  for (int i = 0; i < nr; i++) NQ_TRY(put_state(ctx, 16 + i, CgState{}));
-  NQ_TRY(residual_kernsq_batched(S64, coef, nr, r64, SpinorView{}, x64, b, t64, tmp64, sc, FIN_STORE0, 0));
-  NQ_TRY(get_states(ctx, 0, 16 + nr));
-  const double secs = T.stop();
-  int rc = NQ_OK;
-  for (int i = 0; i < nr; i++) {
-    const CgState& h = ctx->state_host[i];
-    out[i] = nq_solve_stats{};
-    out[i].iters = h.iter;
-    out[i].reliable_updates = h.nreliable;
-    out[i].rel_res_sq = ctx->state_host[16 + i].extra[0] / b2[i];
-    out[i].converged = mixed ? h.true_rr < h.target : h.iter < niter;
-    out[i].seconds = secs;
-    if (!out[i].converged) rc = NQ_ENOCONV;
+  chunk_ = std::max(2, next);
+  return NQ_OK;
+}
+
+int TmBatch::finish(int slot, const SpinorView& src_e, const SpinorView& src_o, const SpinorView& sol_e,
+                    const SpinorView& sol_o, nq_solve_stats* st, double* check)
+{
+  if (slot < 0 || slot >= kMaxSlots || !active_[slot]) return fail(NQ_EINVAL, "slot %d is not active", slot);
+  nq_ctx* c = ctx_;
+  active_[slot] = false;
+  const CgState h = c->state_host[16 * group_ + slot];
+  if (!h.done) {
+    // stopped by the launch bound: park the slot so that later launches skip it
+    CgState& hs = c->state_host[16 * group_ + slot];
+    hs.done = 1;
+    NQ_HIP(hipMemcpyAsync(S_ + slot, &hs, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
   }
-  if (rc != NQ_OK) return fail(rc, "exit without converging");
+  nq_solve_stats out = {};
+  out.iters = h.iter;
+  out.reliable_updates = h.nreliable;
+  if (h.nan) {
+    if (st) *st = out;
+    return fail(NQ_ENAN, h.b2 > 0 ? "nanned" : "invalid norm");
+  }
+  const double mu = mu_[slot];
+  TmSetup S;
+  NQ_TRY(tm_setup(c, g_, lk64_for(prec_, g_), has_anis_ ? &anis_ : nullptr, kappa_, &S));
+  const double a = 1 / (2 * kappa_), n = a * a + mu * mu;
+  const cplx zi_u(a / n, -mu / n), zi_l(a / n, mu / n);
+  // eq. 10: sol_o = K(-mu) y ; eq. 11: sol_e = D_ee^-1 (eta_e + 1/2 H_eo sol_o)
+  NQ_TRY(apply_kern(S, -mu, sol_o, slot_view(x64_, slot), sol_e));
+  HopEpi e11;
+  setc(e11.c1, 0.5 * zi_u, 0.5 * zi_l);
+  setc(e11.c2, zi_u, zi_l);
+  e11.aux = src_e;
+  NQ_TRY(launch_tm_hop(c, NQ_EVEN, sol_e, sol_o, S.U, S.anis, e11));
+  // FP64 check with the 18-real links: |D sol - src|^2 / |src|^2
+  TmSetup Sc;
+  NQ_TRY(tm_setup(c, g_, LK_F64_18, has_anis_ ? &anis_ : nullptr, kappa_, &Sc));
+  CgState* sc = c->state + 48 + 2 * group_;
+  c->state_host[48 + 2 * group_] = CgState{};
+  c->state_host[49 + 2 * group_] = CgState{};
+  NQ_HIP(hipMemcpyAsync(sc, c->state_host + 48 + 2 * group_, 2 * sizeof(CgState), hipMemcpyHostToDevice, c->stream));
+  const SpinorView sol_v[2] = {sol_e, sol_o}, src_v[2] = {src_e, src_o};
+  SpinorView none;
+  none.ncomp = 12;
+  for (int p = 0; p < 2; p++) {
+    HopEpi ec;
+    setc(ec.c1, -0.5, -0.5);
+    setc(ec.c2, cplx(a, mu), cplx(a, -mu));
+    setc(ec.c3, -1.0, -1.0);
+    ec.aux = sol_v[p];
+    ec.aux2 = src_v[p];
+    ec.mode = 1;
+    ec.write = 0;
+    ec.st = sc;
+    ec.fin = p == 0 ? FIN_STORE0 : FIN_STORE1;
+    NQ_TRY(launch_tm_hop(c, p, none, sol_v[!p], Sc.U, Sc.anis, ec));
+  }
+  CgState* sn = c->state + 48 + 2 * group_ + 1;
+  NQ_TRY(launch_lincomb(c, none, 1.0, src_e, 0, none, 0, none, FIN_STORE0, sn));
+  NQ_TRY(launch_lincomb(c, none, 1.0, src_o, 0, none, 0, none, FIN_STORE1, sn + 0));
+  NQ_TRY(get_states(c, 48 + 2 * group_, 2));
+  // state 48+2g: extra[0] = |.|^2 even, extra[1] = |.|^2 odd; state 49+2g: |src_e|^2, |src_o|^2
+  const CgState& r0 = c->state_host[48 + 2 * group_];
+  const CgState& r1 = c->state_host[49 + 2 * group_];
+  const double chk = (r0.extra[0] + r0.extra[1]) / (r1.extra[0] + r1.extra[1]);
+  if (check) *check = chk;
+  out.rel_res_sq = chk;
+  out.converged = mixed_ ? h.true_rr < h.target : h.iter < h.niter;
+  if (st) *st = out;
+  if (!out.converged) return fail(NQ_ENOCONV, "exit without converging (%d iterations)", h.iter);
   return NQ_OK;
 }
 
@@ -839,7 +1005,7 @@ int nq_tm_solve_eoprec(nq_ctx* c, const nq_gauge* g, double kappa, const double*
   if (sol == src) return fail(NQ_EINVAL, "sol==src");
   Guard G(c);
   TmSetup S;
-  NQ_TRY(tm_setup(c, g, LK_F64_18, anis, kappa, &S));
+  NQ_TRY(tm_setup(c, g, lk64_for(prec, g), anis, kappa, &S));
   const double a = 1 / (2 * kappa), n = a * a + mu * mu;
   const cplx zi_u(a / n, -mu / n), zi_l(a / n, mu / n);
   SpinorView se = view_of(src, 0), so = view_of(src, 1), xe = view_of(sol, 0), xo = view_of(sol, 1);
@@ -874,6 +1040,9 @@ int nq_tm_solve_eoprec(nq_ctx* c, const nq_gauge* g, double kappa, const double*
   NQ_TRY(launch_tm_hop(c, NQ_EVEN, xe, xo, S.U, S.anis, e11));
   if (check) {
     // |D sol - src|^2 / |src|^2, D = g5 Q, per parity: -1/2 hop + (a + i mu g5) sol - src
+    // (18-real FP64 links: nissa's operator as stored)
+    TmSetup Sc;
+    NQ_TRY(tm_setup(c, g, LK_F64_18, anis, kappa, &Sc));
     double n2[2] = {0, 0};
     for (int p = 0; p < 2; p++) {
       CgState sc = {};
@@ -888,7 +1057,7 @@ int nq_tm_solve_eoprec(nq_ctx* c, const nq_gauge* g, double kappa, const double*
       ec.write = 0;
       ec.st = scratch(c);
       ec.fin = FIN_STORE0;
-      NQ_TRY(launch_tm_hop(c, p, none, view_of(sol, !p), S.U, S.anis, ec));
+      NQ_TRY(launch_tm_hop(c, p, none, view_of(sol, !p), Sc.U, Sc.anis, ec));
       NQ_TRY(get_states(c, c->nstate - 1, 1));
       n2[p] = c->state_host[c->nstate - 1].extra[0];
     }
@@ -934,51 +1103,35 @@ int nq_tm_solve_eoprec_multi(nq_ctx* c, const nq_gauge* g, double kappa, const d
       if (sols[i] == srcs[j]) return fail(NQ_EINVAL, "solution aliases a source");
   }
   Guard G(c);
-  TmSetup S;
-  NQ_TRY(tm_setup(c, g, LK_F64_18, anis, kappa, &S));
-  const double a = 1 / (2 * kappa);
-  const int64_t fsz = 12 * c->geo.stride;  // elements of one parity of one source
+  TmBatch B;
+  NQ_TRY(B.setup(c, g, kappa, anis, prec, 0));
   std::vector<nq_solve_stats> local(nrhs);
-  int rc = NQ_OK;
-  for (int r0 = 0; r0 < nrhs; r0 += 16) {
-    const int nr = std::min(16, nrhs - r0);
-    SpinorView et, phi, y;
-    NQ_TRY(alloc_spinor(c, "eo.et", SP_F64, 12, &et));
-    NQ_TRY(alloc_spinor_nr(c, "beo.phi", SP_F64, 12, nr, &phi));
-    NQ_TRY(alloc_spinor_nr(c, "beo.y", SP_F64, 12, nr, &y));
-    SpinorView none;
-    none.ncomp = 12;
-    for (int i = 0; i < nr; i++) {
-      const double mu = mus[r0 + i], n = a * a + mu * mu;
-      double ca[2][2], cb[2][2] = {{0, 0}, {0, 0}};
-      setc(ca, cplx(a / n, -mu / n), cplx(a / n, mu / n));
-      NQ_TRY(launch_spin_diag(c, et, ca, view_of(srcs[r0 + i], 0), cb, none));
-      HopEpi e8;
-      setc(e8.c1, 0.5, -0.5);
-      setc(e8.c2, 1.0, -1.0);
-      e8.aux = view_of(srcs[r0 + i], 1);
-      SpinorView ph = phi;
-      ph.data = (char*)phi.data + i * fsz * 16;
-      ph.rstride = 0;
-      NQ_TRY(launch_tm_hop(c, NQ_ODD, ph, et, S.U, S.anis, e8));
+  std::vector<int> owner(TmBatch::kMaxSlots, -1);
+  int rc = NQ_OK, next = 0, ndone = 0;
+  // sources enter free slots as earlier ones finish
+  auto fill = [&]() -> int {
+    for (int s = 0; s < TmBatch::kMaxSlots && next < nrhs; s++)
+      if (!B.active(s)) {
+        NQ_TRY(B.start(s, mus[next], residue, niter, view_of(srcs[next], 0), view_of(srcs[next], 1)));
+        owner[s] = next++;
+      }
+    return NQ_OK;
+  };
+  NQ_TRY(fill());
+  while (ndone < nrhs) {
+    NQ_TRY(B.run(B.suggested_chunk()));
+    std::vector<int> fin;
+    NQ_TRY(B.poll(&fin));
+    for (int s : fin) {
+      const int i = owner[s];
+      const int r = B.finish(s, view_of(srcs[i], 0), view_of(srcs[i], 1), view_of(sols[i], 0), view_of(sols[i], 1),
+                             &local[i], nullptr);
+      if (r == NQ_ENOCONV) rc = r;
+      else if (r != NQ_OK) return r;
+      owner[s] = -1;
+      ndone++;
     }
-    const int rci = cg_kernsq_batched(c, g, kappa, anis, nr, mus + r0, niter, residue, prec, y, phi, &local[r0]);
-    if (rci != NQ_OK && rci != NQ_ENOCONV) return rci;
-    if (rci == NQ_ENOCONV) rc = rci;
-    for (int i = 0; i < nr; i++) {
-      const double mu = mus[r0 + i], n = a * a + mu * mu;
-      const cplx zi_u(a / n, -mu / n), zi_l(a / n, mu / n);
-      nq_field* sol = sols[r0 + i];
-      SpinorView yi = y;
-      yi.data = (char*)y.data + i * fsz * 16;
-      yi.rstride = 0;
-      NQ_TRY(apply_kern(S, -mu, view_of(sol, 1), yi, view_of(sol, 0)));
-      HopEpi e11;
-      setc(e11.c1, 0.5 * zi_u, 0.5 * zi_l);
-      setc(e11.c2, zi_u, zi_l);
-      e11.aux = view_of(srcs[r0 + i], 0);
-      NQ_TRY(launch_tm_hop(c, NQ_EVEN, view_of(sol, 0), view_of(sol, 1), S.U, S.anis, e11));
-    }
+    NQ_TRY(fill());
   }
   NQ_HIP(hipStreamSynchronize(c->stream));
   if (st)

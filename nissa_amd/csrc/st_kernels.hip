@@ -226,33 +226,13 @@ __device__ inline void cgm_finalize(CgmState* s, int fin, double total)
   }
 }
 
-__global__ void __launch_bounds__(256) k_cgm_finalize(const double* partial, int64_t nchunks, int nr, CgmState* st,
-                                                      int fin)
+__global__ void __launch_bounds__(1024) k_cgm_finalize(const double* partial, int64_t nchunks, CgmState* st, int fin)
 {
-  __shared__ double vl[256];
-  const int vt = threadIdx.x;
-  for (int rr = 0; rr < nr; rr++) {
-    CgmState* s = st + rr;
-    if (s->done) continue;
-    const double* pr = partial + (int64_t)rr * nchunks;
-    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-    int64_t c = vt;
-    for (; c + 768 < nchunks; c += 1024) {
-      a0 += pr[c];
-      a1 += pr[c + 256];
-      a2 += pr[c + 512];
-      a3 += pr[c + 768];
-    }
-    for (; c < nchunks; c += 256) a0 += pr[c];
-    vl[vt] = (a0 + a1) + (a2 + a3);
-    __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-      if (vt < h) vl[vt] += vl[vt + h];
-      __syncthreads();
-    }
-    if (vt == 0) cgm_finalize(s, fin, vl[0]);
-    __syncthreads();
-  }
+  __shared__ double vl[1024];
+  CgmState* s = st + blockIdx.x;
+  if (s->done) return;  // uniform over the workgroup
+  const double tot = block_sum_partials(partial + (int64_t)blockIdx.x * nchunks, nchunks, vl);
+  if (threadIdx.x == 0) cgm_finalize(s, fin, tot);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -319,7 +299,7 @@ int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, co
                        (int)(sizeof(CgmState) / sizeof(int)), 1);
   NQ_HIP(hipGetLastError());
   prof_end(ctx, ev);
-  hipLaunchKernelGGL(k_cgm_finalize, dim3(1), dim3(256), 0, ctx->stream, ctx->partial, L.nchunks, nr, st,
+  hipLaunchKernelGGL(k_cgm_finalize, dim3(nr), dim3(1024), 0, ctx->stream, ctx->partial, L.nchunks, st,
                      (int)FIN_CGM_PAP);
   NQ_HIP(hipGetLastError());
   // C2
@@ -332,7 +312,7 @@ int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, co
                        L.s, L.nchunks, ctx->partial);
   NQ_HIP(hipGetLastError());
   prof_end(ctx, ev);
-  hipLaunchKernelGGL(k_cgm_finalize, dim3(1), dim3(256), 0, ctx->stream, ctx->partial, L.nchunks, nr, st,
+  hipLaunchKernelGGL(k_cgm_finalize, dim3(nr), dim3(1024), 0, ctx->stream, ctx->partial, L.nchunks, st,
                      (int)FIN_CGM_RFRF);
   NQ_HIP(hipGetLastError());
   // C3

@@ -226,6 +226,7 @@ static double lk_bytes(int lk)
   case LK_F64_18: return 144;
   case LK_F32_18: return 72;
   case LK_F32_12: return 48;
+  case LK_F64_12: return 96;
   default: return 24;
   }
 }
@@ -303,6 +304,15 @@ int launch_tm_hop(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& 
       if (!epi.rs.data) return go<SP_F64, LK_F64_18, 3, SP_F64>(ctx, op, out, in, U, anis, epi, "tm_hop_res_f64");
       if (epi.rs.prec == SP_F32) return go<SP_F64, LK_F64_18, 3, SP_F32>(ctx, op, out, in, U, anis, epi, "tm_hop_res_f64_s32");
       if (epi.rs.prec == SP_F16) return go<SP_F64, LK_F64_18, 3, SP_F16>(ctx, op, out, in, U, anis, epi, "tm_hop_res_f64_s16");
+    }
+  } else if (sp == SP_F64 && lk == LK_F64_12) {
+    if (epi.mode == 2 && epi.x.prec != SP_F64) return fail(NQ_EINVAL, "FP64 CG tail needs FP64 x,r");
+    NQ_MODES(SP_F64, LK_F64_12, SP_F64, "f64_l64c")
+    if (epi.mode == 3) {
+      if (epi.r.prec != SP_F64) return fail(NQ_EINVAL, "residual tail needs FP64 r");
+      if (!epi.rs.data) return go<SP_F64, LK_F64_12, 3, SP_F64>(ctx, op, out, in, U, anis, epi, "tm_hop_res_f64_l64c");
+      if (epi.rs.prec == SP_F32) return go<SP_F64, LK_F64_12, 3, SP_F32>(ctx, op, out, in, U, anis, epi, "tm_hop_res_f64_l64c_s32");
+      if (epi.rs.prec == SP_F16) return go<SP_F64, LK_F64_12, 3, SP_F16>(ctx, op, out, in, U, anis, epi, "tm_hop_res_f64_l64c_s16");
     }
   } else if (sp == SP_F32 && lk == LK_F32_18) {
     if (epi.mode == 2 && epi.x.prec != SP_F32) return fail(NQ_EINVAL, "FP32 CG tail needs FP32 x,r");
