@@ -219,6 +219,10 @@ void nq_server_destroy(nq_server* s);
    ctx stream; prec selects the storage the kernel reads (NQ_PREC_*). */
 int nq_bench_tm_hop(nq_ctx* ctx, const nq_gauge* g, int prec, int napply, nq_field* out,
                     const nq_field* in, double* ms_per_apply, double* bytes_per_apply);
+/* the same for nrhs right-hand sides (copies of `in`) per launch, as the batched solves
+   run it; bytes count the links once and the spinors per right-hand side */
+int nq_bench_tm_hop_multi(nq_ctx* ctx, const nq_gauge* g, int prec, int nrhs, int napply,
+                          const nq_field* in, double* ms_per_apply, double* bytes_per_apply);
 
 #ifdef __cplusplus
 }

@@ -88,6 +88,7 @@ _D = C.c_double
 _I = C.c_int
 _DP = C.POINTER(C.c_double)
 _SIG = {
+    "nq_bench_tm_hop_multi": ([_P, _P, _I, _I, _I, _P, _DP, _DP], _I),
     "nq_server_create": ([_I, C.c_int64, _I, C.POINTER(_P)], _I),
     "nq_server_add_volume": ([_P, _I, _I, _I, _I, _DP, C.POINTER(_I)], _I),
     "nq_server_submit": ([_P, C.POINTER(ServeRequest), C.POINTER(C.c_int64)], _I),
@@ -490,6 +491,13 @@ def st_cgm(ctx, g, shifts, niter, residues, sols, srcs, prec="f64"):
     if rc not in (NQ_OK, NQ_ENOCONV):
         _check(rc)
     return [dict(st[r].as_dict(), rc=rc, true_res_sq=[tr[r * ns + k] for k in range(ns)]) for r in range(nr)]
+
+
+def bench_tm_hop_multi(ctx, g, nrhs, napply, inp, prec="f16"):
+    ms = C.c_double(0)
+    by = C.c_double(0)
+    _check(lib().nq_bench_tm_hop_multi(ctx.h, g.h, PREC[prec], nrhs, napply, inp.h, C.byref(ms), C.byref(by)))
+    return ms.value, by.value
 
 
 def bench_tm_hop(ctx, g, napply, out, inp, prec="f64"):

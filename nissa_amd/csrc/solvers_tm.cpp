@@ -1201,4 +1201,31 @@ int nq_bench_tm_hop(nq_ctx* c, const nq_gauge* g, int prec, int napply, nq_field
   return NQ_OK;
 }
 
+int nq_bench_tm_hop_multi(nq_ctx* c, const nq_gauge* g, int prec, int nrhs, int napply, const nq_field* in,
+                          double* ms, double* bytes)
+{
+  if (!c || napply <= 0 || nrhs < 1 || nrhs > 16) return fail(NQ_EINVAL, "bad argument");
+  NQ_TRY(check_field(c, in, NQ_SPINCOLOR, NQ_ODD, "nq_bench_tm_hop_multi"));
+  Guard G(c);
+  int sp, lk;
+  NQ_TRY(lk_of_prec(prec, &sp, &lk));
+  TmSetup S;
+  NQ_TRY(tm_setup(c, g, lk, nullptr, 1.0, &S));
+  SpinorView vi, vo;
+  NQ_TRY(alloc_spinor_nr(c, "bench.min", sp, 12, nrhs, &vi));
+  NQ_TRY(alloc_spinor_nr(c, "bench.mout", sp, 12, nrhs, &vo));
+  for (int r = 0; r < nrhs; r++) NQ_TRY(launch_convert(c, slot_view(vi, r), view_of(in, 0)));
+  HopEpi e;
+  e.nrhs = nrhs;
+  e.batched = 1;
+  NQ_TRY(launch_tm_hop(c, NQ_EVEN, vo, vi, S.U, S.anis, e));  // warm-up
+  Timer T(c->stream);
+  for (int i = 0; i < napply; i++) NQ_TRY(launch_tm_hop(c, NQ_EVEN, vo, vi, S.U, S.anis, e));
+  const double sec = T.stop();
+  if (ms) *ms = sec * 1e3 / napply;
+  if (bytes) *bytes = tm_hop_bytes(sp, lk, e) * (double)c->geo.Vh;
+  NQ_HIP(hipStreamSynchronize(c->stream));
+  return NQ_OK;
+}
+
 }  // extern "C"

@@ -105,6 +105,89 @@ __device__ __forceinline__ void hop_slot(const Geo& g, const Site& s, uint32_t e
 template <int SP>
 using RealOf = typename std::conditional<SP == SP_F64, double, float>::type;
 
+// Site-local epilogue of one right-hand side r at output site e (shared by the
+// one-RHS and the register-blocked kernels, so both give the same bits):
+//   out_h = c1_h hop + c2_h aux + c3_h aux2, then the mode's tail; returns the
+// site's contribution to the reduced norm (modes 1-3).
+template <int SP, int MODE, int XP, class R>
+__device__ __forceinline__ double hop_epilogue(const Geo& g, const HopEpi& epi, const SpinorView& out, uint32_t e,
+                                               int r, CgState* st, cx<R> o[12])
+{
+  using RX = RealOf<XP>;
+  double nrm = 0;
+  double cf[12];
+  if (epi.coef) {
+#pragma unroll
+    for (int k = 0; k < 12; k++) cf[k] = epi.coef[12 * r + k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      cf[k] = (&epi.c1[0][0])[k];
+      cf[4 + k] = (&epi.c2[0][0])[k];
+      cf[8 + k] = (&epi.c3[0][0])[k];
+    }
+  }
+  const cx<R> c1u = mk<R>((R)cf[0], (R)cf[1]), c1l = mk<R>((R)cf[2], (R)cf[3]);
+#pragma unroll
+  for (int k = 0; k < 12; k++) o[k] = (k < 6 ? c1u : c1l) * o[k];
+  if (epi.aux.data) {
+    cx<R> a[12];
+    SpAcc<SP, R, 12, kNT>::load(epi.aux, g.stride, e, a, r);
+    const cx<R> cu = mk<R>((R)cf[4], (R)cf[5]), cl = mk<R>((R)cf[6], (R)cf[7]);
+#pragma unroll
+    for (int k = 0; k < 12; k++) fma_to(o[k], k < 6 ? cu : cl, a[k]);
+  }
+  if (epi.aux2.data) {
+    cx<R> a[12];
+    SpAcc<SP, R, 12, kNT>::load(epi.aux2, g.stride, e, a, r);
+    const cx<R> cu = mk<R>((R)cf[8], (R)cf[9]), cl = mk<R>((R)cf[10], (R)cf[11]);
+#pragma unroll
+    for (int k = 0; k < 12; k++) fma_to(o[k], k < 6 ? cu : cl, a[k]);
+  }
+  if constexpr (MODE == 0) {
+    SpAcc<SP, R, 12, kNT>::store(out, g.stride, e, o, r);
+  } else if constexpr (MODE == 1) {
+    if (epi.write) SpAcc<SP, R, 12, kNT>::store(out, g.stride, e, o, r);
+#pragma unroll
+    for (int k = 0; k < 12; k++) nrm += (double)o[k].re * (double)o[k].re + (double)o[k].im * (double)o[k].im;
+  } else if constexpr (MODE == 2) {
+    // CG tail: s = o; x += alpha p ; r -= alpha s ; |r|^2
+    const RX alpha = (RX)st->alpha;
+    cx<RX> x[12], rv[12];
+    cx<R> p[12];
+    SpAcc<XP, RX, 12, kNT>::load(epi.x, g.stride, e, x, r);
+    SpAcc<SP, R, 12, kNT>::load(epi.p, g.stride, e, p, r);
+    SpAcc<XP, RX, 12, kNT>::load(epi.r, g.stride, e, rv, r);
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+      x[k].re = fma(alpha, (RX)p[k].re, x[k].re);
+      x[k].im = fma(alpha, (RX)p[k].im, x[k].im);
+      rv[k].re = fma(-alpha, (RX)o[k].re, rv[k].re);
+      rv[k].im = fma(-alpha, (RX)o[k].im, rv[k].im);
+      nrm += (double)rv[k].re * (double)rv[k].re + (double)rv[k].im * (double)rv[k].im;
+    }
+    SpAcc<XP, RX, 12, kNT>::store(epi.x, g.stride, e, x, r);
+    SpAcc<XP, RX, 12, kNT>::store(epi.r, g.stride, e, rv, r);
+  } else {
+    // residual tail: r64 = o ; rs = sloppy(o) ; |o|^2
+    SpAcc<SP, R, 12, kNT>::store(epi.r, g.stride, e, o, r);
+    if (epi.rs.data) {
+      cx<RX> q[12];
+#pragma unroll
+      for (int k = 0; k < 12; k++) q[k] = mk<RX>((RX)o[k].re, (RX)o[k].im);
+      SpAcc<XP, RX, 12, kNT>::store(epi.rs, g.stride, e, q, r);
+    }
+#pragma unroll
+    for (int k = 0; k < 12; k++) nrm += (double)o[k].re * (double)o[k].re + (double)o[k].im * (double)o[k].im;
+  }
+  return nrm;
+}
+
+__device__ __forceinline__ bool rhs_idle(const CgState* st, int pred)
+{
+  return st->done || (pred == 1 && !st->reliable) || (pred == 2 && st->reliable);
+}
+
 // workgroups of at most 256 threads; FP64 builds get the full 256-VGPR budget
 // (one wavefront per SIMD per workgroup), the sloppy builds two workgroups per CU
 template <int SP, int LK, int MODE, int XP>
@@ -113,12 +196,9 @@ __global__ void __launch_bounds__(256, (SP != SP_F64 ? 2 : 1)) k_tm_hop(Geo g, i
                                                double* partial)
 {
   using R = RealOf<SP>;
-  using RX = RealOf<XP>;
   const Map m = map_thread(S, g.Vh);
   CgState* st = epi.st ? epi.st + m.r : nullptr;
-  bool skip = false;
-  if (st)
-    skip = st->done || (epi.pred_reliable == 1 && !st->reliable) || (epi.pred_reliable == 2 && st->reliable);
+  const bool skip = st && rhs_idle(st, epi.pred_reliable);
   // a launch whose every right-hand side is idle does nothing (uniform over the grid)
   if (epi.st && S.NR == 1 && skip) return;
   double nrm = 0;
@@ -141,75 +221,94 @@ __global__ void __launch_bounds__(256, (SP != SP_F64 ? 2 : 1)) k_tm_hop(Geo g, i
     hop_slot<SP, LK, R, 2, false, false>(g, s, e, in, r, ub, det, anis, ua, o);
     hop_slot<SP, LK, R, 3, true, false>(g, s, e, in, r, ub, det, anis, ua, o);
     hop_slot<SP, LK, R, 3, false, true>(g, s, e, in, r, ub, det, anis, ua, o);
-
-    // epilogue: out_h = c1 hop + c2 aux + c3 aux2 ; per-rhs coefficients when tabulated
-    double cf[12];
-    if (epi.coef) {
-#pragma unroll
-      for (int k = 0; k < 12; k++) cf[k] = epi.coef[12 * r + k];
-    } else {
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        cf[k] = (&epi.c1[0][0])[k];
-        cf[4 + k] = (&epi.c2[0][0])[k];
-        cf[8 + k] = (&epi.c3[0][0])[k];
-      }
-    }
-    const cx<R> c1u = mk<R>((R)cf[0], (R)cf[1]), c1l = mk<R>((R)cf[2], (R)cf[3]);
-#pragma unroll
-    for (int k = 0; k < 12; k++) o[k] = (k < 6 ? c1u : c1l) * o[k];
-    if (epi.aux.data) {
-      cx<R> a[12];
-      SpAcc<SP, R, 12, kNT>::load(epi.aux, g.stride, e, a, r);
-      const cx<R> cu = mk<R>((R)cf[4], (R)cf[5]), cl = mk<R>((R)cf[6], (R)cf[7]);
-#pragma unroll
-      for (int k = 0; k < 12; k++) fma_to(o[k], k < 6 ? cu : cl, a[k]);
-    }
-    if (epi.aux2.data) {
-      cx<R> a[12];
-      SpAcc<SP, R, 12, kNT>::load(epi.aux2, g.stride, e, a, r);
-      const cx<R> cu = mk<R>((R)cf[8], (R)cf[9]), cl = mk<R>((R)cf[10], (R)cf[11]);
-#pragma unroll
-      for (int k = 0; k < 12; k++) fma_to(o[k], k < 6 ? cu : cl, a[k]);
-    }
-    if constexpr (MODE == 0) {
-      SpAcc<SP, R, 12, kNT>::store(out, g.stride, e, o, r);
-    } else if constexpr (MODE == 1) {
-      if (epi.write) SpAcc<SP, R, 12, kNT>::store(out, g.stride, e, o, r);
-#pragma unroll
-      for (int k = 0; k < 12; k++) nrm += (double)o[k].re * (double)o[k].re + (double)o[k].im * (double)o[k].im;
-    } else if constexpr (MODE == 2) {
-      // CG tail: s = o; x += alpha p ; r -= alpha s ; |r|^2
-      const RX alpha = (RX)st->alpha;
-      cx<RX> x[12], rv[12];
-      cx<R> p[12];
-      SpAcc<XP, RX, 12, kNT>::load(epi.x, g.stride, e, x, r);
-      SpAcc<SP, R, 12, kNT>::load(epi.p, g.stride, e, p, r);
-      SpAcc<XP, RX, 12, kNT>::load(epi.r, g.stride, e, rv, r);
-#pragma unroll
-      for (int k = 0; k < 12; k++) {
-        x[k].re = fma(alpha, (RX)p[k].re, x[k].re);
-        x[k].im = fma(alpha, (RX)p[k].im, x[k].im);
-        rv[k].re = fma(-alpha, (RX)o[k].re, rv[k].re);
-        rv[k].im = fma(-alpha, (RX)o[k].im, rv[k].im);
-        nrm += (double)rv[k].re * (double)rv[k].re + (double)rv[k].im * (double)rv[k].im;
-      }
-      SpAcc<XP, RX, 12, kNT>::store(epi.x, g.stride, e, x, r);
-      SpAcc<XP, RX, 12, kNT>::store(epi.r, g.stride, e, rv, r);
-    } else {
-      // residual tail: r64 = o ; rs = sloppy(o) ; |o|^2
-      SpAcc<SP, R, 12, kNT>::store(epi.r, g.stride, e, o, r);
-      if (epi.rs.data) {
-        cx<RX> q[12];
-#pragma unroll
-        for (int k = 0; k < 12; k++) q[k] = mk<RX>((RX)o[k].re, (RX)o[k].im);
-        SpAcc<XP, RX, 12, kNT>::store(epi.rs, g.stride, e, q, r);
-      }
-#pragma unroll
-      for (int k = 0; k < 12; k++) nrm += (double)o[k].re * (double)o[k].re + (double)o[k].im * (double)o[k].im;
-    }
+    nrm = hop_epilogue<SP, MODE, XP, R>(g, epi, out, e, r, st, o);
   }
   if constexpr (MODE != 0) store_partial(nrm, partial, nchunks, m.r, m.chunk);
+}
+
+// ---------------------------------------------------------------- register-blocked RHS
+// One slot of the hop for RB right-hand sides r0..r0+nrb-1 of the same site: the
+// link is loaded once and applied to each right-hand side's projected neighbour.
+template <int SP, int LK, class R, int MU, bool FWD, int RB>
+__device__ __forceinline__ void hop_slot_rb(const Geo& g, const Site& s, uint32_t e, const SpinorView& in, int r0,
+                                            int nrb, unsigned busy, const void* ub, const double* det, R anis,
+                                            bool use_anis, cx<R> o[RB][12])
+{
+  const uint32_t n = neigh_eo(g, s, MU, FWD);
+  cx<R> u[9];
+  constexpr int slot = 2 * MU + (FWD ? 0 : 1);
+  LkAcc<LK, R>::load(ub, g.stride, slot, e, det + 2 * slot, u);
+#pragma unroll
+  for (int b = 0; b < RB; b++) {
+    if (b >= nrb || !((busy >> b) & 1u)) continue;  // uniform over the wavefront
+    cx<R> sp[12];
+    SpAcc<SP, R, 12>::load(in, g.stride, n, sp, r0 + b);
+    cx<R> h0[3], h1[3];
+    project<R, MU, FWD>(sp, h0, h1);
+    cx<R> g0[3], g1[3];
+    mat_vec(u, h0, g0);
+    mat_vec(u, h1, g1);
+    if (MU > 0 && use_anis) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) { g0[c] = scal(g0[c], anis); g1[c] = scal(g1[c], anis); }
+    }
+    scatter<R, MU, FWD>(o[b], g0, g1);
+#pragma unroll
+    for (int k = 0; k < 12; k++) asm volatile("" : "+v"(o[b][k].re), "+v"(o[b][k].im));
+  }
+  asm volatile("" ::: "memory");
+}
+
+// Multi-RHS hop for the sloppy precisions: wavefront w of a workgroup takes chunk
+// 4*cb + w and the RB right-hand sides of group rg (Shape: W = 4, NRB = 1, NR = NRG
+// = number of RB-groups).  Link traffic and link loads per right-hand side drop by RB.
+template <int SP, int LK, int MODE, int XP, int RB>
+__global__ void __launch_bounds__(256) k_tm_hop_rb(Geo g, int op, SpinorView out, SpinorView in, LinkView U,
+                                                   double anis_d, int use_anis, HopEpi epi, Shape S, int nr,
+                                                   int64_t nchunks, double* partial)
+{
+  using R = RealOf<SP>;
+  const Map m = map_thread(S, g.Vh);
+  const int r0 = m.r * RB, nrb = min(RB, nr - r0);
+  unsigned busy = 0;
+#pragma unroll
+  for (int b = 0; b < RB; b++)
+    if (b < nrb && !(epi.st && rhs_idle(epi.st + r0 + b, epi.pred_reliable))) busy |= 1u << b;
+  // right-hand sides that idle here are skipped by the finalisation as well
+  if (!busy || !m.active) {
+    if constexpr (MODE != 0) {
+#pragma unroll
+      for (int b = 0; b < RB; b++)
+        if (b < nrb) store_partial(0.0, partial, nchunks, r0 + b, m.chunk);
+    }
+    return;
+  }
+  const uint32_t e = m.e;
+  const Site s = site_of(g, op, e);
+  const void* ub = U.data[op];
+  const double* det = &U.det[op][0][0];
+  const R anis = (R)anis_d;
+  const bool ua = use_anis != 0;
+  cx<R> o[RB][12];
+#pragma unroll
+  for (int b = 0; b < RB; b++)
+#pragma unroll
+    for (int k = 0; k < 12; k++) o[b][k] = mk<R>(0, 0);
+  hop_slot_rb<SP, LK, R, 0, true, RB>(g, s, e, in, r0, nrb, busy, ub, det, anis, ua, o);
+  hop_slot_rb<SP, LK, R, 0, false, RB>(g, s, e, in, r0, nrb, busy, ub, det, anis, ua, o);
+  hop_slot_rb<SP, LK, R, 1, true, RB>(g, s, e, in, r0, nrb, busy, ub, det, anis, ua, o);
+  hop_slot_rb<SP, LK, R, 1, false, RB>(g, s, e, in, r0, nrb, busy, ub, det, anis, ua, o);
+  hop_slot_rb<SP, LK, R, 2, true, RB>(g, s, e, in, r0, nrb, busy, ub, det, anis, ua, o);
+  hop_slot_rb<SP, LK, R, 2, false, RB>(g, s, e, in, r0, nrb, busy, ub, det, anis, ua, o);
+  hop_slot_rb<SP, LK, R, 3, true, RB>(g, s, e, in, r0, nrb, busy, ub, det, anis, ua, o);
+  hop_slot_rb<SP, LK, R, 3, false, RB>(g, s, e, in, r0, nrb, busy, ub, det, anis, ua, o);
+#pragma unroll
+  for (int b = 0; b < RB; b++) {
+    if (b >= nrb) continue;
+    double nrm = 0;
+    if ((busy >> b) & 1u) nrm = hop_epilogue<SP, MODE, XP, R>(g, epi, out, e, r0 + b, epi.st ? epi.st + r0 + b : nullptr, o[b]);
+    if constexpr (MODE != 0) store_partial(nrm, partial, nchunks, r0 + b, m.chunk);
+  }
 }
 
 static double sp_site_bytes(int sp, int ncomp)
@@ -240,8 +339,9 @@ static int xp_of(const HopEpi& epi, int sp)
 
 double tm_hop_bytes(int sp, int lk, const HopEpi& epi, int ncomp)
 {
+  // per output site: the 8 links once for all right-hand sides, everything else per RHS
   const double s = sp_site_bytes(sp, ncomp);
-  double b = 8 * lk_bytes(lk) + s;  // links + input spinor (each site read once)
+  double b = s;  // input spinor (each site read once)
   if (epi.aux.data) b += s;
   if (epi.aux2.data) b += s;
   if (epi.mode == 0 || (epi.mode == 1 && epi.write)) b += s;
@@ -253,7 +353,7 @@ double tm_hop_bytes(int sp, int lk, const HopEpi& epi, int ncomp)
     b += s;
     if (epi.rs.data) b += sp_site_bytes(epi.rs.prec, ncomp);
   }
-  return b;
+  return 8 * lk_bytes(lk) + b * epi.nrhs;
 }
 
 int64_t blocks_for(int64_t n) { return (n + kBlock - 1) / kBlock; }
@@ -269,6 +369,26 @@ static int go(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, 
   if (L.nchunks * nr > ctx->max_partials) return fail(NQ_EINVAL, "tm hop: reduction scratch too small");
   hipEvent_t ev = nullptr;
   prof_begin(ctx, name, tm_hop_bytes(SP, LK, epi) * (double)g.Vh, &ev);
+  if constexpr (SP != SP_F64 && MODE != 3) {
+    if (epi.batched || nr > 1) {
+      // register-blocked right-hand sides (always for batched solves, so that a
+      // right-hand side runs the same code alone and in a batch); two per wavefront
+      // measured fastest at 32^3x64 (FP16, 12 RHS: 71 us per RHS vs 76 with four,
+      // 78 with one, 96 for the single-RHS kernel)
+      constexpr int RB = 2;
+      Shape sh;
+      sh.W = 4;
+      sh.NRB = 1;
+      sh.NRG = sh.NR = (nr + RB - 1) / RB;
+      const int64_t blocks = (L.nchunks + 3) / 4 * sh.NRG;
+      hipLaunchKernelGGL((k_tm_hop_rb<SP, LK, MODE, XP, RB>), dim3((unsigned)blocks), dim3(256), 0, ctx->stream, g, op,
+                         out, in, U, anis, anis != 1.0 ? 1 : 0, epi, sh, nr, L.nchunks, ctx->partial);
+      NQ_HIP(hipGetLastError());
+      prof_end(ctx, ev);
+      if (MODE != 0) NQ_TRY(launch_finalize(ctx, nr, epi.st, epi.fin, epi.pred_reliable));
+      return NQ_OK;
+    }
+  }
   hipLaunchKernelGGL((k_tm_hop<SP, LK, MODE, XP>), dim3((unsigned)L.blocks), dim3(L.threads), 0, ctx->stream, g, op, out,
                      in, U, anis, anis != 1.0 ? 1 : 0, epi, L.s, L.nchunks, ctx->partial);
   NQ_HIP(hipGetLastError());

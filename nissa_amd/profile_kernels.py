@@ -21,7 +21,8 @@ if ROOT not in sys.path:
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--what", default="hop", choices=["hop", "copy", "solve"])
+    ap.add_argument("--what", default="hop", choices=["hop", "hopmulti", "copy", "solve", "multisolve"])
+    ap.add_argument("--nrhs", type=int, default=12)
     ap.add_argument("--prec", default="f64", choices=["f64", "f32", "f16", "f16h"])
     ap.add_argument("--n", type=int, default=20)
     ap.add_argument("--lattice", default="64,32,32,32")
@@ -42,6 +43,21 @@ def main():
     if a.what == "hop":
         ms, by = nq.bench_tm_hop(ctx, g, a.n, fe, fo, prec=a.prec)
         print(f"hop {a.prec}: {ms*1e3:.1f} us/apply, {by/1e6:.1f} MB algorithmic, {by/(ms*1e-3)/1e9:.0f} GB/s")
+    elif a.what == "hopmulti":
+        ms, by = nq.bench_tm_hop_multi(ctx, g, a.nrhs, a.n, fo, prec=a.prec)
+        print(f"hop x{a.nrhs} {a.prec}: {ms*1e3:.1f} us/apply ({ms*1e3/a.nrhs:.1f} us per RHS), "
+              f"{by/1e6:.1f} MB algorithmic, {by/(ms*1e-3)/1e9:.0f} GB/s")
+    elif a.what == "multisolve":
+        srcs = [ctx.field(nq.SPINCOLOR, nq.LX, nq.source_lx(L, 30 + i, "gauss")) for i in range(a.nrhs)]
+        sols = [ctx.field(nq.SPINCOLOR, nq.LX) for _ in range(a.nrhs)]
+        mus = [[0.003, 0.005, 0.01, 0.02][i % 4] for i in range(a.nrhs)]
+        for _ in range(a.n):
+            import time as _t
+            t0 = _t.perf_counter()
+            res = nq.tm_solve_eoprec_multi(ctx, g, 0.16, mus, 100000, 1e-20, sols, srcs, prec=a.prec)
+            dt = _t.perf_counter() - t0
+            print(f"multisolve x{a.nrhs} {a.prec}: {dt*1e3:.1f} ms ({dt*1e3/a.nrhs:.2f} ms per source), "
+                  f"iters {[r['iters'] for r in res]}, worst check {max(r['rel_res_sq'] for r in res):.2e}")
     elif a.what == "copy":
         fo2 = ctx.field(nq.SPINCOLOR, nq.ODD)
         for _ in range(a.n):
