@@ -42,6 +42,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-iters", type=int, default=6, help="oracle CG iterations timed for the CPU baseline")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-stag", action="store_true", help="skip the staggered multi-shift leg")
     ap.add_argument("--serve-requests", type=int, default=60, help="requests in the Poisson stream")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="request arrival rate (1/s)")
     args = ap.parse_args()
@@ -127,12 +128,16 @@ def main():
         f"{args.prec}: {ms_hop16*1e3:.1f} us/apply, {bytes_hop16/(ms_hop16*1e-3)/1e9:.0f} GB/s")
 
     serve = None
-    if not args.no_serve:
+    stag = None
+    if not (args.no_serve and args.no_stag):
         for f in (fs, fx, fo, fe):
             f.close()
         g.close()
         ctx.close()
-        serve = serve_bench(args)
+        if not args.no_serve:
+            serve = serve_bench(args)
+        if not args.no_stag:
+            stag = stag_bench(args)
 
     # CPU baseline: the oracle's FP64 CG on the same inputs, bounded sample
     cpu = None
@@ -173,14 +178,70 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_of(dom_name, L), "kernel": dom_name,
                      "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": dom["bytes_per_launch"]},
         "serve": serve,
+        "staggered_cgm": stag,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    if args.no_serve:
+    if args.no_serve and args.no_stag:
         for f in (fs, fx, fo, fe):
             f.close()
         g.close()
         ctx.close()
+
+
+def stag_bench(args):
+    """Batched multi-shift staggered solve at 32^3x64: 12 Gaussian even colour sources,
+    12 shifts m^2 + 10^(-4+5k/11) (m = 0.0025) on the massless m^2 - 1/4 Deo Doe,
+    every (source, shift) to a true relative residual 1e-10 checked in FP64; FP32
+    CGM batched over the sources plus batched refinement with FP64 reliable updates,
+    against the FP64 CGM of one source (x12 for the sequential rate)."""
+    import numpy as np
+
+    import nissa_amd as nq
+
+    L = (64, 32, 32, 32)
+    conf = nq.gauge_random_lx(L, 4, 0.3)
+    nq.gauge_stagphases_lx(L, conf)
+    nq.gauge_phase_lx(L, conf, [-1.0, 0, 0, 0])
+    m = 0.0025
+    shifts = [m * m + 10 ** (-4 + 5 * k / 11) for k in range(12)]
+    residue = 1e-20
+    ctx = nq.Ctx(L, 0)
+    g = nq.Gauge(ctx, lx=conf)
+    rng = np.random.default_rng(44)
+    Vh = ctx.Vh
+    srcs = [ctx.field(nq.COLOR, nq.EVEN, rng.normal(size=(Vh, 3, 2)) * np.sqrt(0.5)) for _ in range(12)]
+    sols = [[ctx.field(nq.COLOR, nq.EVEN) for _ in shifts] for _ in srcs]
+    # FP64 CGM of one source (the reference's cgm_invert on the device)
+    t0 = time.perf_counter()
+    r64 = nq.st_cgm(ctx, g, shifts, 100000, [residue] * 12, sols[:1], srcs[:1], prec="f64")
+    t64 = time.perf_counter() - t0
+    # batched FP32-inner solve of all 12 sources
+    t0 = time.perf_counter()
+    r32 = nq.st_cgm(ctx, g, shifts, 100000, [residue] * 12, sols, srcs, prec="f32")
+    t32 = time.perf_counter() - t0
+    worst64 = max(r64[0]["true_res_sq"])
+    worst32 = max(max(r["true_res_sq"]) for r in r32)
+    ok = all(r["rc"] == 0 for r in r32) and r64[0]["rc"] == 0 and worst32 < 2 * residue
+    for f in srcs:
+        f.close()
+    for row in sols:
+        for f in row:
+            f.close()
+    g.close()
+    ctx.close()
+    out = {
+        "lattice": list(L), "sources": 12, "shifts": 12, "mass": m,
+        "fp64_cgm_one_source_s": round(t64, 3), "fp64_cgm_iters": r64[0]["iters"],
+        "fp64_sequential_12_sources_s": round(12 * t64, 3),
+        "batched_f32_s": round(t32, 3), "batched_iters_per_source": r32[0]["iters"],
+        "sources_per_s": round(12 / t32, 3), "sequential_fp64_sources_per_s": round(1 / t64, 3),
+        "speedup_vs_sequential_fp64": round(12 * t64 / t32, 3),
+        "worst_true_rel_res_sq": float(worst32), "fp64_worst_true_rel_res_sq": float(worst64), "ok": bool(ok),
+    }
+    log(f"[bench] staggered CGM 12 sources x 12 shifts: batched {t32:.2f}s vs 12 x FP64 {12*t64:.2f}s "
+        f"(x{out['speedup_vs_sequential_fp64']}), worst residue {worst32:.2e}")
+    return out
 
 
 def serve_bench(args):

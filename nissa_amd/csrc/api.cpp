@@ -221,10 +221,23 @@ int nq_ctx_create(int T, int LX, int LY, int LZ, int device, nq_ctx** out)
   g.TB = tb;
   g.logTB = g.TB == 8 ? 3 : (g.TB == 4 ? 2 : 1);
   g.NTH = T / g.TB;
+  // x/y patch of the device site order (tuning knob NQ_XYTILE "TXxTY", powers of two)
+  int tx = 1, ty = 1;
+  if (const char* e = getenv("NQ_XYTILE")) sscanf(e, "%dx%d", &tx, &ty);
+  auto pow2 = [](int v) { return v >= 1 && v <= 16 && (v & (v - 1)) == 0; };
+  if (!pow2(tx)) tx = 1;
+  if (!pow2(ty)) ty = 1;
+  while (LX % tx) tx /= 2;
+  while (LY % ty) ty /= 2;
+  auto lg = [](int v) { int l = 0; while ((1 << l) < v) l++; return l; };
+  g.TX = tx; g.logTX = lg(tx); g.NXT = LX / tx;
+  g.TY = ty; g.logTY = lg(ty); g.NYT = LY / ty;
   g.Sz = g.TB;
-  g.Sy = (int64_t)g.TB * LZ;
-  g.Sx = (int64_t)g.TB * LZ * LY;
-  g.Sth = (int64_t)g.TB * LZ * LY * LX;
+  g.Syi = (int64_t)g.TB * LZ;
+  g.Sxi = g.Syi * g.TY;
+  g.Syt = g.Sxi * g.TX;
+  g.Sxt = g.Syt * g.NYT;
+  g.Sth = g.Sxt * g.NXT;
   int rc = NQ_OK;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = fail(NQ_EDEVICE, "stream creation failed");
   const int64_t nchunks = (g.Vh + 63) / 64;

@@ -47,7 +47,7 @@ __device__ __forceinline__ void fma_to(cx<R>& acc, cx<R> a, cx<R> b)
 
 // ---------------------------------------------------------------- geometry
 struct Site {
-  int t, x, y, z, th, tl;
+  int t, x, y, z, th, tl, xt, xi, yt, yi;
   uint32_t dlx;  // rank in the device site order
 };
 
@@ -59,11 +59,17 @@ __device__ __forceinline__ Site site_of(const Geo& g, int par, uint32_t e)
   uint32_t r = q >> g.logTB;
   const uint32_t r1 = r / (uint32_t)g.LZ;
   s.z = (int)(r - r1 * (uint32_t)g.LZ);
-  const uint32_t r2 = r1 / (uint32_t)g.LY;
-  s.y = (int)(r1 - r2 * (uint32_t)g.LY);
-  const uint32_t r3 = r2 / (uint32_t)g.LX;
-  s.x = (int)(r2 - r3 * (uint32_t)g.LX);
-  s.th = (int)r3;
+  s.yi = (int)(r1 & (uint32_t)(g.TY - 1));
+  const uint32_t r2 = r1 >> g.logTY;
+  s.xi = (int)(r2 & (uint32_t)(g.TX - 1));
+  const uint32_t r3 = r2 >> g.logTX;
+  const uint32_t r4 = r3 / (uint32_t)g.NYT;
+  s.yt = (int)(r3 - r4 * (uint32_t)g.NYT);
+  const uint32_t r5 = r4 / (uint32_t)g.NXT;
+  s.xt = (int)(r4 - r5 * (uint32_t)g.NXT);
+  s.th = (int)r5;
+  s.x = s.xt * g.TX + s.xi;
+  s.y = s.yt * g.TY + s.yi;
   const int b = (par + tl0 + s.x + s.y + s.z) & 1;
   s.tl = tl0 + b;
   s.t = s.th * g.TB + s.tl;
@@ -87,10 +93,12 @@ __device__ __forceinline__ uint32_t neigh_eo(const Geo& g, const Site& s, int mu
     else d += s.tl == 0 ? (g.TB - 1) + (s.th == 0 ? (int64_t)(g.NTH - 1) * g.Sth : -g.Sth) : -1;
     break;
   case 1:
-    d += up ? (s.x == g.LX - 1 ? -(int64_t)(g.LX - 1) * g.Sx : g.Sx) : (s.x == 0 ? (int64_t)(g.LX - 1) * g.Sx : -g.Sx);
+    if (up) d += s.xi < g.TX - 1 ? g.Sxi : -(int64_t)(g.TX - 1) * g.Sxi + (s.xt == g.NXT - 1 ? -(int64_t)(g.NXT - 1) * g.Sxt : g.Sxt);
+    else d += s.xi > 0 ? -g.Sxi : (int64_t)(g.TX - 1) * g.Sxi + (s.xt == 0 ? (int64_t)(g.NXT - 1) * g.Sxt : -g.Sxt);
     break;
   case 2:
-    d += up ? (s.y == g.LY - 1 ? -(int64_t)(g.LY - 1) * g.Sy : g.Sy) : (s.y == 0 ? (int64_t)(g.LY - 1) * g.Sy : -g.Sy);
+    if (up) d += s.yi < g.TY - 1 ? g.Syi : -(int64_t)(g.TY - 1) * g.Syi + (s.yt == g.NYT - 1 ? -(int64_t)(g.NYT - 1) * g.Syt : g.Syt);
+    else d += s.yi > 0 ? -g.Syi : (int64_t)(g.TY - 1) * g.Syi + (s.yt == 0 ? (int64_t)(g.NYT - 1) * g.Syt : -g.Syt);
     break;
   default:
     d += up ? (s.z == g.LZ - 1 ? -(int64_t)(g.LZ - 1) * g.Sz : g.Sz) : (s.z == 0 ? (int64_t)(g.LZ - 1) * g.Sz : -g.Sz);

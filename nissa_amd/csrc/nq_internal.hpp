@@ -43,13 +43,27 @@ constexpr int lk_eb(int lk) { return (lk == LK_F64_18 || lk == LK_F64_12) ? 16 :
 // down from LX*LY*LZ/2 to TB*LY*LZ/2 sites, so neighbour spinors are re-read from
 // L2 instead of the fabric.  Host buffers keep nissa's order; the conversion
 // kernels translate.
+// Device site order: (th, xt, yt, xi, yi, z, tl) from slowest to fastest, with
+// t = th*TB + tl, x = xt*TX + xi, y = yt*TY + yi; the eo index of a site is its rank
+// in this order divided by two.  Tiling t keeps the +-t neighbours of a wavefront in
+// the same rows; tiling x and y in TX x TY patches keeps the +-x/+-y neighbour rows
+// close in processing order, so they are still in L2 when they are needed.
 struct Geo {
   int T, LX, LY, LZ;
   int TB, logTB, NTH;  // t tile, log2, number of tiles
+  int TX, logTX, NXT;  // x tile, log2, number of tiles
+  int TY, logTY, NYT;  // y tile, log2, number of tiles
   int64_t Vh;          // sites per parity
   int64_t stride;      // per-component stride of SoA fields (Vh rounded up to 64)
-  int64_t Sz, Sy, Sx, Sth;  // device-order strides (tl stride is 1)
+  int64_t Sz, Syi, Sxi, Syt, Sxt, Sth;  // device-order strides (tl stride is 1)
 };
+
+// rank of lattice site (t, x, y, z) in the device order (both parities)
+__host__ __device__ inline int64_t device_rank(const Geo& g, int t, int x, int y, int z)
+{
+  return (int64_t)(t >> g.logTB) * g.Sth + (int64_t)(x >> g.logTX) * g.Sxt + (int64_t)(y >> g.logTY) * g.Syt +
+         (int64_t)(x & (g.TX - 1)) * g.Sxi + (int64_t)(y & (g.TY - 1)) * g.Syi + (int64_t)z * g.Sz + (t & (g.TB - 1));
+}
 
 // One parity of a spinor (ncomp=12) or colour (ncomp=3) field in SoA layout:
 // component k of site e at data[k*stride + e] (complex element).  F16 keeps a
@@ -164,6 +178,10 @@ struct nq_ctx {
   std::vector<hipEvent_t> event_pool;
   // cached temporaries keyed by role
   std::map<std::string, nq::DevBuf> temps;
+  // autotuned launch choices (key: kernel family and shape), e.g. right-hand sides
+  // per wavefront of the batched hop; filled on first use
+  std::map<std::string, int> tuned;
+  bool capturing = false;  // a HIP graph capture is open on the stream: no tuning, no ledger
 };
 
 struct nq_field {

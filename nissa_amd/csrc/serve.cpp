@@ -132,7 +132,7 @@ int point_source(nq_ctx* c, nq_field* f, int64_t site, int sc)
   r /= g.LY;
   const int x = (int)(r % g.LX);
   const int t = (int)(r / g.LX);
-  const int64_t dlx = (int64_t)(t / g.TB) * g.Sth + x * g.Sx + y * g.Sy + z * g.Sz + (t % g.TB);
+  const int64_t dlx = device_rank(g, t, x, y, z);
   const int par = (t + x + y + z) & 1;
   static const double one[2] = {1.0, 0.0};
   char* p = (char*)f->buf.p + ((int64_t)(par * 12 + sc) * g.stride + (dlx >> 1)) * 16;

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <cstring>
 
 #include "nq_internal.hpp"
@@ -623,10 +624,8 @@ int TmBatch::start(int slot, double mu, double residue, int niter, const SpinorV
   return NQ_OK;
 }
 
-int TmBatch::run(int iters)
+int TmBatch::enqueue(int nr, int iters)
 {
-  const int nr = hi();
-  if (nr == 0) return NQ_OK;
   nq_ctx* c = ctx_;
   TmSetup Ss;
   NQ_TRY(tm_setup(c, g_, mixed_ ? lk_ : LK_F64_18, has_anis_ ? &anis_ : nullptr, kappa_, &Ss));
@@ -651,6 +650,62 @@ int TmBatch::run(int iters)
     NQ_TRY(launch_tm_hop(c, NQ_ODD, none, tmps_, Ss.U, Ss.anis, b2e));
     NQ_TRY(launch_cg_pupdate(c, p_, rs_, S_, nr));
   }
+  return NQ_OK;
+}
+
+// Replays of captured iteration chunks.  Every launch of an iteration has fixed
+// arguments (batch buffers, coefficient rows and states of this batch), so a chunk
+// of 2^k iterations for a given number of slots is captured once into a HIP graph
+// and replayed; a chunk of any length is a sum of such graphs.  The first chunk of a
+// slot count runs eagerly (it also settles the autotuned launch shapes); the launch
+// ledger, when enabled, needs eager launches too.  NQ_GRAPHS=0 disables graphs.
+int TmBatch::run_graphs(int nr, int iters)
+{
+  nq_ctx* c = ctx_;
+  for (int k = 6; k >= 0; k--) {
+    const int n = 1 << k;
+    while (iters >= n) {
+      const int key = nr * 128 + n;
+      auto it = graphs_.find(key);
+      if (it == graphs_.end()) {
+        hipGraph_t gr = nullptr;
+        NQ_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+        c->capturing = true;
+        const int rc = enqueue(nr, n);
+        c->capturing = false;
+        const hipError_t e = hipStreamEndCapture(c->stream, &gr);
+        if (rc != NQ_OK) {
+          if (gr) (void)hipGraphDestroy(gr);
+          return rc;
+        }
+        if (e != hipSuccess) return fail(NQ_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
+        hipGraphExec_t ex = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (ei != hipSuccess) return fail(NQ_EDEVICE, "graph instantiation failed: %s", hipGetErrorString(ei));
+        it = graphs_.emplace(key, ex).first;
+      }
+      NQ_HIP(hipGraphLaunch(it->second, c->stream));
+      iters -= n;
+    }
+  }
+  return NQ_OK;
+}
+
+TmBatch::~TmBatch()
+{
+  for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+}
+
+int TmBatch::run(int iters)
+{
+  const int nr = hi();
+  if (nr == 0 || iters <= 0) return NQ_OK;
+  const char* env = getenv("NQ_GRAPHS");
+  const bool graphs = !(env && env[0] == '0') && !ctx_->profiling && eager_done_.count(nr);
+  if (graphs) NQ_TRY(run_graphs(nr, iters));
+  else NQ_TRY(enqueue(nr, iters));
+  eager_done_.insert(nr);
   for (int i = 0; i < nr; i++)
     if (active_[i]) launched_[i] += iters;
   return NQ_OK;

@@ -11,6 +11,8 @@
 // any mix.
 #pragma once
 
+#include <map>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -20,6 +22,10 @@ namespace nq {
 
 class TmBatch {
  public:
+  TmBatch() = default;
+  TmBatch(const TmBatch&) = delete;
+  TmBatch& operator=(const TmBatch&) = delete;
+  ~TmBatch();
   static constexpr int kMaxSlots = 16;
   static constexpr int kMaxGroups = 3;  // state ranges [16*group, 16*group+16) of the ctx
 
@@ -48,6 +54,8 @@ class TmBatch {
  private:
   int hi() const;
   int upload_row(int slot, double mu);
+  int enqueue(int nr, int iters);
+  int run_graphs(int nr, int iters);
 
   nq_ctx* ctx_ = nullptr;
   const nq_gauge* g_ = nullptr;
@@ -64,6 +72,8 @@ class TmBatch {
   int niter_[kMaxSlots] = {};
   int launched_[kMaxSlots] = {};  // iterations enqueued since the slot started
   int chunk_ = 8;
+  std::map<int, hipGraphExec_t> graphs_;  // (slots, 2^k iterations) -> captured chunk
+  std::set<int> eager_done_;              // slot counts that ran eagerly once
 };
 
 }  // namespace nq

@@ -18,6 +18,9 @@
 // x += alpha p, r -= alpha s and |r|^2; mode 3 writes a residual b - A x in FP64
 // plus its sloppy copy and |r|^2.
 
+#include <cstdlib>
+#include <string>
+
 #include "nq_device.hpp"
 
 namespace nq {
@@ -358,6 +361,84 @@ double tm_hop_bytes(int sp, int lk, const HopEpi& epi, int ncomp)
 
 int64_t blocks_for(int64_t n) { return (n + kBlock - 1) / kBlock; }
 
+// launch of the register-blocked kernel with RB right-hand sides per wavefront
+template <int SP, int LK, int MODE, int XP>
+static int launch_rb(nq_ctx* ctx, int RB, int op, const SpinorView& out, const SpinorView& in, const LinkView& U,
+                     double anis, const HopEpi& epi, int nr)
+{
+  const Geo& g = ctx->geo;
+  Shape sh;
+  sh.W = 4;
+  sh.NRB = 1;
+  sh.NRG = sh.NR = (nr + RB - 1) / RB;
+  const int64_t nchunks = (g.Vh + 63) / 64;
+  const dim3 grid((unsigned)((nchunks + 3) / 4 * sh.NRG)), blk(256);
+  const int ua = anis != 1.0 ? 1 : 0;
+  switch (RB) {
+  case 1: hipLaunchKernelGGL((k_tm_hop_rb<SP, LK, MODE, XP, 1>), grid, blk, 0, ctx->stream, g, op, out, in, U, anis, ua, epi, sh, nr, nchunks, ctx->partial); break;
+  case 2: hipLaunchKernelGGL((k_tm_hop_rb<SP, LK, MODE, XP, 2>), grid, blk, 0, ctx->stream, g, op, out, in, U, anis, ua, epi, sh, nr, nchunks, ctx->partial); break;
+  default: hipLaunchKernelGGL((k_tm_hop_rb<SP, LK, MODE, XP, 4>), grid, blk, 0, ctx->stream, g, op, out, in, U, anis, ua, epi, sh, nr, nchunks, ctx->partial); break;
+  }
+  NQ_HIP(hipGetLastError());
+  return NQ_OK;
+}
+
+// Autotuner of the right-hand sides per wavefront of the batched sloppy hop: on the
+// first batched launch of a (precision, batch size) it times the write-only kernel
+// (no solver state, output into a scratch buffer) with 1, 2 and 4 right-hand sides
+// per wavefront on the caller's input and links, and keeps the fastest for every
+// mode of that precision and batch size.  The arithmetic of a right-hand side is the
+// same for every choice.  NQ_TUNE=0 or an open graph capture use RB = 2.
+template <int SP, int LK, int XP>
+static int tuned_rb(nq_ctx* ctx, int op, const SpinorView& in, const LinkView& U, double anis, const HopEpi& epi,
+                    int nr, int* rb)
+{
+  const std::string key = "tm_hop_rb/" + std::to_string(SP) + "/" + std::to_string(LK) + "/" + std::to_string(nr);
+  auto it = ctx->tuned.find(key);
+  if (it != ctx->tuned.end()) {
+    *rb = it->second;
+    return NQ_OK;
+  }
+  const char* env = getenv("NQ_TUNE");
+  if (ctx->capturing || (env && env[0] == '0')) {
+    *rb = 2;
+    return NQ_OK;
+  }
+  SpinorView scratch;
+  NQ_TRY(alloc_spinor_nr(ctx, "tune.out" + std::to_string(SP), SP, 12, nr, &scratch));
+  HopEpi e0;
+  e0.nrhs = nr;
+  e0.batched = 1;
+  e0.coef = epi.coef;
+  const bool prof = ctx->profiling;
+  ctx->profiling = false;
+  hipEvent_t a, b;
+  NQ_HIP(hipEventCreate(&a));
+  NQ_HIP(hipEventCreate(&b));
+  float best = 1e30f;
+  int best_rb = 2;
+  for (int cand : {1, 2, 4}) {
+    if (cand > 1 && cand > nr) continue;
+    NQ_TRY((launch_rb<SP, LK, 0, XP>(ctx, cand, op, scratch, in, U, anis, e0, nr)));  // warm-up
+    NQ_HIP(hipEventRecord(a, ctx->stream));
+    for (int i = 0; i < 3; i++) NQ_TRY((launch_rb<SP, LK, 0, XP>(ctx, cand, op, scratch, in, U, anis, e0, nr)));
+    NQ_HIP(hipEventRecord(b, ctx->stream));
+    NQ_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    NQ_HIP(hipEventElapsedTime(&ms, a, b));
+    if (ms < best * 0.98f) {  // prefer the smaller block unless clearly slower
+      best = ms;
+      best_rb = cand;
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  ctx->profiling = prof;
+  ctx->tuned[key] = best_rb;
+  *rb = best_rb;
+  return NQ_OK;
+}
+
 template <int SP, int LK, int MODE, int XP>
 static int go(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, const LinkView& U, double anis,
               const HopEpi& epi, const char* name)
@@ -368,27 +449,21 @@ static int go(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, 
   const Launch L = launch_shape(g, nr);
   if (L.nchunks * nr > ctx->max_partials) return fail(NQ_EINVAL, "tm hop: reduction scratch too small");
   hipEvent_t ev = nullptr;
-  prof_begin(ctx, name, tm_hop_bytes(SP, LK, epi) * (double)g.Vh, &ev);
   if constexpr (SP != SP_F64 && MODE != 3) {
     if (epi.batched || nr > 1) {
       // register-blocked right-hand sides (always for batched solves, so that a
-      // right-hand side runs the same code alone and in a batch); two per wavefront
-      // measured fastest at 32^3x64 (FP16, 12 RHS: 71 us per RHS vs 76 with four,
-      // 78 with one, 96 for the single-RHS kernel)
-      constexpr int RB = 2;
-      Shape sh;
-      sh.W = 4;
-      sh.NRB = 1;
-      sh.NRG = sh.NR = (nr + RB - 1) / RB;
-      const int64_t blocks = (L.nchunks + 3) / 4 * sh.NRG;
-      hipLaunchKernelGGL((k_tm_hop_rb<SP, LK, MODE, XP, RB>), dim3((unsigned)blocks), dim3(256), 0, ctx->stream, g, op,
-                         out, in, U, anis, anis != 1.0 ? 1 : 0, epi, sh, nr, L.nchunks, ctx->partial);
-      NQ_HIP(hipGetLastError());
+      // right-hand side runs the same code alone and in a batch); RB right-hand
+      // sides per wavefront, chosen by the autotuner for this precision and batch
+      int RB = 0;
+      NQ_TRY((tuned_rb<SP, LK, XP>(ctx, op, in, U, anis, epi, nr, &RB)));
+      prof_begin(ctx, name, tm_hop_bytes(SP, LK, epi) * (double)g.Vh, &ev);
+      NQ_TRY((launch_rb<SP, LK, MODE, XP>(ctx, RB, op, out, in, U, anis, epi, nr)));
       prof_end(ctx, ev);
       if (MODE != 0) NQ_TRY(launch_finalize(ctx, nr, epi.st, epi.fin, epi.pred_reliable));
       return NQ_OK;
     }
   }
+  prof_begin(ctx, name, tm_hop_bytes(SP, LK, epi) * (double)g.Vh, &ev);
   hipLaunchKernelGGL((k_tm_hop<SP, LK, MODE, XP>), dim3((unsigned)L.blocks), dim3(L.threads), 0, ctx->stream, g, op, out,
                      in, U, anis, anis != 1.0 ? 1 : 0, epi, L.s, L.nchunks, ctx->partial);
   NQ_HIP(hipGetLastError());

@@ -83,3 +83,45 @@ def test_phases_and_sources():
     assert pt.sum() == 1.0 and pt[5].reshape(12, 2)[7, 0] == 1.0
     ga = nq.source_lx(L, 3, "gauss", 3, 3)
     assert abs(np.var(ga) - 0.5) < 0.1
+
+
+def test_entry_points_fail_cleanly_without_a_device():
+    """Without a usable GPU every entry point that needs one returns an error code
+    (no CPU fallback, no crash): contexts, servers."""
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(nq.NqError) as e:
+        nq.Ctx((4, 4, 4, 4))
+    assert e.value.code in (nq.NQ_EDEVICE, nq.NQ_EINVAL)
+    with pytest.raises(nq.NqError) as e:
+        nq.Server()
+    assert e.value.code == nq.NQ_EDEVICE
+
+
+def test_abi_structs_match_the_header(tmp_path):
+    """The ctypes mirrors of the structures crossing the ABI have the C compiler's
+    layout of include/nissa_amd.h (field offsets and sizes)."""
+    import ctypes as C
+    import subprocess
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    structs = {"nq_solve_stats": nq.SolveStats, "nq_serve_request": nq.ServeRequest,
+               "nq_serve_result": nq.ServeResult}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "nissa_amd.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(root, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
