@@ -144,6 +144,14 @@ int nq_st_cgm(nq_ctx* ctx, const nq_gauge* g, int nrhs, int nshift, const double
               int niter, const double* residues, int prec, nq_field** sols,
               const nq_field* const* srcs, nq_solve_stats* st, double* true_res_sq);
 
+/* summ_src_and_all_inv_stD2ee_m2_cgm_portable (cgm_invert_stD2ee_m2_portable.hpp:31-55):
+   chi_r = cons src_r + sum_k weights[k] x_{r,k},  (m2-less D2ee + poles[k]) x_{r,k} = src_r,
+   one CGM over the rational approximation's poles per source (rat_approx_t,
+   rat_approx.hpp:16-60), residue for every pole; nrhs sources batched. */
+int nq_st_cgm_rat_sum(nq_ctx* ctx, const nq_gauge* g, int nrhs, int degree, double cons,
+                      const double* poles, const double* weights, int niter, double residue, int prec,
+                      nq_field** chis, const nq_field* const* srcs, nq_solve_stats* st);
+
 /* ------------------------------------------------------------------ host-buffer drop-ins */
 /* Same operations over nissa host buffers; the gauge handle stays resident. */
 int nq_tm_kern_square_host(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis,

@@ -88,6 +88,7 @@ _D = C.c_double
 _I = C.c_int
 _DP = C.POINTER(C.c_double)
 _SIG = {
+    "nq_st_cgm_rat_sum": ([_P, _P, _I, _I, _D, _DP, _DP, _I, _D, _I, _P, _P, _P], _I),
     "nq_bench_tm_hop_multi": ([_P, _P, _I, _I, _I, _P, _DP, _DP], _I),
     "nq_server_create": ([_I, C.c_int64, _I, C.POINTER(_P)], _I),
     "nq_server_add_volume": ([_P, _I, _I, _I, _I, _DP, C.POINTER(_I)], _I),
@@ -491,6 +492,21 @@ def st_cgm(ctx, g, shifts, niter, residues, sols, srcs, prec="f64"):
     if rc not in (NQ_OK, NQ_ENOCONV):
         _check(rc)
     return [dict(st[r].as_dict(), rc=rc, true_res_sq=[tr[r * ns + k] for k in range(ns)]) for r in range(nr)]
+
+
+def st_cgm_rat_sum(ctx, g, cons, poles, weights, niter, residue, chis, srcs, prec="f64"):
+    """chi_r = cons src_r + sum_k weights[k] (D2ee + poles[k])^-1 src_r for every source."""
+    nr, d = len(srcs), len(poles)
+    assert len(weights) == d and len(chis) == nr
+    po = (C.c_double * d)(*poles)
+    we = (C.c_double * d)(*weights)
+    ch = (_P * nr)(*[x.h for x in chis])
+    sr = (_P * nr)(*[x.h for x in srcs])
+    st = (SolveStats * nr)()
+    rc = lib().nq_st_cgm_rat_sum(ctx.h, g.h, nr, d, cons, po, we, niter, residue, PREC[prec], ch, sr, st)
+    if rc not in (NQ_OK, NQ_ENOCONV):
+        _check(rc)
+    return [dict(st[r].as_dict(), rc=rc) for r in range(nr)]
 
 
 def bench_tm_hop_multi(ctx, g, nrhs, napply, inp, prec="f16"):

@@ -184,6 +184,24 @@ typedef _Float16 nq_hv4 __attribute__((ext_vector_type(4)));
 typedef _Float16 nq_hv8 __attribute__((ext_vector_type(8)));
 typedef unsigned int nq_u3 __attribute__((ext_vector_type(3)));
 
+// the G = 16/EB complex numbers of one 16-byte record
+template <int EB, class R>
+__device__ __forceinline__ void decode_record(const nq_u4 u, cx<R>* s)
+{
+  if constexpr (EB == 16) {
+    const nq_dv2 d = __builtin_bit_cast(nq_dv2, u);
+    s[0] = mk<R>((R)d.x, (R)d.y);
+  } else if constexpr (EB == 8) {
+    const nq_fv4 f = __builtin_bit_cast(nq_fv4, u);
+    s[0] = mk<R>((R)f.x, (R)f.y);
+    s[1] = mk<R>((R)f.z, (R)f.w);
+  } else {
+    const nq_hv8 h = __builtin_bit_cast(nq_hv8, u);
+#pragma unroll
+    for (int j = 0; j < 4; j++) s[j] = mk<R>((R)(float)h[2 * j], (R)(float)h[2 * j + 1]);
+  }
+}
+
 // load NC complex elements of site e from a packed array at `base` (num_records
 // `bytes`); element type selected by EB: 16 double, 8 float, 4 half
 template <int EB, int NC, int CP, class R>
@@ -192,21 +210,9 @@ __device__ __forceinline__ void load_packed(const void* base, int64_t bytes, int
   using P = Pack<EB, NC>;
   const __amdgpu_buffer_rsrc_t rs = make_rsrc(base, bytes);
 #pragma unroll
-  for (int q = 0; q < P::NG; q++) {
-    const nq_u4 u = __builtin_amdgcn_raw_buffer_load_b128(rs, e * 16u, (uint32_t)(q * stride * 16), CP);
-    if constexpr (EB == 16) {
-      const nq_dv2 d = __builtin_bit_cast(nq_dv2, u);
-      s[q] = mk<R>((R)d.x, (R)d.y);
-    } else if constexpr (EB == 8) {
-      const nq_fv4 f = __builtin_bit_cast(nq_fv4, u);
-      s[2 * q] = mk<R>((R)f.x, (R)f.y);
-      s[2 * q + 1] = mk<R>((R)f.z, (R)f.w);
-    } else {
-      const nq_hv8 h = __builtin_bit_cast(nq_hv8, u);
-#pragma unroll
-      for (int j = 0; j < 4; j++) s[4 * q + j] = mk<R>((R)(float)h[2 * j], (R)(float)h[2 * j + 1]);
-    }
-  }
+  for (int q = 0; q < P::NG; q++)
+    decode_record<EB, R>(__builtin_amdgcn_raw_buffer_load_b128(rs, e * 16u, (uint32_t)(q * stride * 16), CP),
+                         s + q * P::G);
   if constexpr (P::NL > 0) {
     const uint32_t so = (uint32_t)(P::NG * stride * 16), vo = e * (uint32_t)(P::NL * EB);
     constexpr int k0 = P::NG * P::G;
@@ -314,6 +320,55 @@ struct SpAcc {
       v.norm[(int64_t)r * stride + e] = m;
     } else {
       store_packed<EB, NC, CP, R>(base, stride, e, s);
+    }
+  }
+};
+
+// ---------------------------------------------------------------- LDS staging
+// A workgroup of kStageSites = 256 threads covers 256 consecutive sites of one
+// parity; it stages the records of the opposite-parity field at the same 256
+// positions in LDS ([record][site], 16-byte entries, plus the FP16 norms), and its
+// hops read every neighbour that falls inside that range from LDS instead of
+// memory: the +-z neighbours (also across the z boundary, the rows are whole), the
+// +-t neighbours inside a time tile and the +-y neighbours between the block's rows.
+// Out-of-range neighbours are loaded from memory as usual.  The decoded values are
+// the same either way (same records, same decoding).
+constexpr int kStageSites = 256;
+
+template <int SP, int NC>
+struct Stage {
+  static constexpr int EB = SP == SP_F64 ? 16 : (SP == SP_F32 ? 8 : 4);
+  static constexpr int NG = Pack<EB, NC>::NG;
+  static_assert(Pack<EB, NC>::NL == 0, "LDS staging needs whole 16-byte records");
+  static constexpr int BYTES = NG * kStageSites * 16 + (SP == SP_F16 ? kStageSites * 4 : 0);
+  // thread t stages site e (valid: e < Vh) of right-hand side r
+  __device__ __forceinline__ static void put(char* lds, const SpinorView& v, int64_t stride, uint32_t e, bool valid,
+                                             int t, int r)
+  {
+    const char* base = (const char*)v.data + (int64_t)r * v.rstride * EB;
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc(base, (int64_t)NC * stride * EB);
+#pragma unroll
+    for (int q = 0; q < NG; q++) {
+      nq_u4 u = {0u, 0u, 0u, 0u};
+      if (valid) u = __builtin_amdgcn_raw_buffer_load_b128(rs, e * 16u, (uint32_t)(q * stride * 16), 0);
+      *(nq_u4*)(lds + (q * kStageSites + t) * 16) = u;
+    }
+    if constexpr (SP == SP_F16) {
+      float n = 0.f;
+      if (valid) n = v.norm[(int64_t)r * stride + e];
+      ((float*)(lds + NG * kStageSites * 16))[t] = n;
+    }
+  }
+  // site at offset d (< kStageSites) of the staged range
+  template <class R>
+  __device__ __forceinline__ static void get(const char* lds, int d, cx<R> s[NC])
+  {
+#pragma unroll
+    for (int q = 0; q < NG; q++) decode_record<EB, R>(*(const nq_u4*)(lds + (q * kStageSites + d) * 16), s + q * (16 / EB));
+    if constexpr (SP == SP_F16) {
+      const R nrm = (R)((const float*)(lds + NG * kStageSites * 16))[d];
+#pragma unroll
+      for (int k = 0; k < NC; k++) s[k] = mk<R>(s[k].re * nrm, s[k].im * nrm);
     }
   }
 };

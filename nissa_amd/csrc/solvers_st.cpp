@@ -420,6 +420,8 @@ int nq_st_cgm(nq_ctx* c, const nq_gauge* g, int nrhs, int nshift, const double* 
         st[r0 + i].iters = its.empty() ? 0 : its[i];
       }
     // check / refine every (source, shift) system with the FP64 operator, 16 at a time
+    // systems in shift-major order: a group of 16 holds neighbouring shifts, so the
+    // slow small-shift systems share groups instead of stretching every group
     const int nsys = nr * nshift;
     for (int s0 = 0; s0 < nsys; s0 += 16) {
       const int ns = std::min(16, nsys - s0);
@@ -428,7 +430,7 @@ int nq_st_cgm(nq_ctx* c, const nq_gauge* g, int nrhs, int nshift, const double* 
       NQ_TRY(alloc_spinor_nr(c, "cgm.bb", SP_F64, 3, ns, &bb));
       std::vector<double> sh(ns), rs(ns);
       for (int j = 0; j < ns; j++) {
-        const int i = (s0 + j) / nshift, k = (s0 + j) % nshift;
+        const int k = (s0 + j) / nr, i = (s0 + j) % nr;
         sh[j] = shifts[k];
         rs[j] = residues[k];
         NQ_TRY(launch_convert(c, shifted(xb, j * csz), view_of(sols[(r0 + i) * nshift + k], 0)));
@@ -440,7 +442,7 @@ int nq_st_cgm(nq_ctx* c, const nq_gauge* g, int nrhs, int nshift, const double* 
         if (rr != NQ_OK && rr != NQ_ENOCONV) return rr;
         if (rr == NQ_ENOCONV) rc = rr;
         for (int j = 0; j < ns; j++) {
-          const int i = (s0 + j) / nshift, k = (s0 + j) % nshift;
+          const int k = (s0 + j) / nr, i = (s0 + j) % nr;
           NQ_TRY(launch_convert(c, view_of(sols[(r0 + i) * nshift + k], 0), shifted(xb, j * csz)));
           if (st) {
             st[r0 + i].reliable_updates += sst[j].reliable_updates;
@@ -466,7 +468,7 @@ int nq_st_cgm(nq_ctx* c, const nq_gauge* g, int nrhs, int nshift, const double* 
       NQ_TRY(launch_lincomb(c, none, 1.0, bb, 0, none, 0, none, FIN_STORE1, c->state, ns));
       NQ_TRY(get_states(c, 0, ns));
       for (int j = 0; j < ns; j++) {
-        const int i = (s0 + j) / nshift, k = (s0 + j) % nshift;
+        const int k = (s0 + j) / nr, i = (s0 + j) % nr;
         const double tr = c->state_host[j].extra[0] / c->state_host[j].extra[1];
         if (true_res_sq) true_res_sq[(r0 + i) * nshift + k] = tr;
         if (st) st[r0 + i].rel_res_sq = std::max(st[r0 + i].rel_res_sq, tr);
@@ -484,6 +486,55 @@ int nq_st_cgm(nq_ctx* c, const nq_gauge* g, int nrhs, int nshift, const double* 
     }
   (void)hipEventDestroy(ea);
   (void)hipEventDestroy(eb);
+  if (rc != NQ_OK) return fail(rc, "exit without converging");
+  return NQ_OK;
+}
+
+int nq_st_cgm_rat_sum(nq_ctx* c, const nq_gauge* g, int nrhs, int degree, double cons, const double* poles,
+                      const double* weights, int niter, double residue, int prec, nq_field** chis,
+                      const nq_field* const* srcs, nq_solve_stats* st)
+{
+  if (!c || !g || !poles || !weights || !chis || !srcs) return fail(NQ_EINVAL, "null argument");
+  if (nrhs < 1 || degree < 1 || degree > kMaxShift) return fail(NQ_EINVAL, "need 1..%d poles and >= 1 source", kMaxShift);
+  for (int i = 0; i < nrhs; i++) {
+    NQ_TRY(check_color(c, srcs[i], NQ_EVEN, "nq_st_cgm_rat_sum"));
+    NQ_TRY(check_color(c, chis[i], NQ_EVEN, "nq_st_cgm_rat_sum"));
+    for (int j = 0; j < nrhs; j++)
+      if (chis[i] == srcs[j]) return fail(NQ_EINVAL, "chi aliases a source");
+  }
+  Guard G(c);
+  // the CGM solutions of every (source, pole) live in context temporaries
+  std::vector<nq_field> xs((size_t)nrhs * degree);
+  std::vector<nq_field*> xp(xs.size());
+  SpinorView xbuf;
+  NQ_TRY(alloc_spinor_nr(c, "rat.x", SP_F64, 3, (int)xs.size(), &xbuf));
+  for (size_t j = 0; j < xs.size(); j++) {
+    xs[j].ctx = c;
+    xs[j].kind = NQ_COLOR;
+    xs[j].coverage = NQ_EVEN;
+    xs[j].buf.p = (char*)xbuf.data + j * xbuf.rstride * 16;
+    xp[j] = &xs[j];
+  }
+  struct Release {
+    std::vector<nq_field>& v;
+    ~Release()
+    {
+      for (auto& f : v) f.buf.p = nullptr;
+    }
+  } rel{xs};
+  std::vector<double> res(degree, residue);
+  const int rc = nq_st_cgm(c, g, nrhs, degree, poles, niter, res.data(), prec, xp.data(), srcs, st, nullptr);
+  if (rc != NQ_OK && rc != NQ_ENOCONV) return rc;
+  SpinorView none;
+  none.ncomp = 3;
+  for (int i = 0; i < nrhs; i++) {
+    const SpinorView chi = view_of(chis[i], 0);
+    NQ_TRY(launch_lincomb(c, chi, cons, view_of(srcs[i], 0), 0, none, 0, none, FIN_NONE, nullptr));
+    for (int k = 0; k < degree; k++)
+      NQ_TRY(launch_lincomb(c, chi, 1.0, chi, weights[k], view_of(xp[(size_t)i * degree + k], 0), 0, none, FIN_NONE,
+                            nullptr));
+  }
+  NQ_HIP(hipStreamSynchronize(c->stream));
   if (rc != NQ_OK) return fail(rc, "exit without converging");
   return NQ_OK;
 }

@@ -73,14 +73,17 @@ __device__ __forceinline__ void scatter(cx<R> o[12], const cx<R> g0[3], const cx
   }
 }
 
+// lds: staged records of sites [lds_e0, lds_e0 + kStageSites) of `in`, or null
 template <int SP, int LK, class R, int MU, bool FWD, bool PIN>
 __device__ __forceinline__ void hop_slot(const Geo& g, const Site& s, uint32_t e, const SpinorView& in, int r,
                                          const void* ub, const double* det, R anis, bool use_anis,
-                                         cx<R> o[12])
+                                         cx<R> o[12], const char* lds = nullptr, uint32_t lds_e0 = 0)
 {
   const uint32_t n = neigh_eo(g, s, MU, FWD);
   cx<R> sp[12];
-  SpAcc<SP, R, 12>::load(in, g.stride, n, sp, r);
+  const uint32_t d = n - lds_e0;
+  if (lds && d < (uint32_t)kStageSites) Stage<SP, 12>::get(lds, (int)d, sp);
+  else SpAcc<SP, R, 12>::load(in, g.stride, n, sp, r);
   cx<R> h0[3], h1[3];
   project<R, MU, FWD>(sp, h0, h1);
   cx<R> u[9];
@@ -229,6 +232,49 @@ __global__ void __launch_bounds__(256, (SP != SP_F64 ? 2 : 1)) k_tm_hop(Geo g, i
   if constexpr (MODE != 0) store_partial(nrm, partial, nchunks, m.r, m.chunk);
 }
 
+// One right-hand side, neighbours staged in LDS (Stage): the workgroup's four
+// wavefronts cover 256 consecutive output sites (Shape W = 4, NRB = 1).
+template <int SP, int LK, int MODE, int XP>
+__global__ void __launch_bounds__(256, (SP != SP_F64 ? 2 : 1)) k_tm_hop_lds(Geo g, int op, SpinorView out,
+                                                                        SpinorView in, LinkView U, double anis_d,
+                                                                        int use_anis, HopEpi epi, Shape S,
+                                                                        int64_t nchunks, double* partial)
+{
+  using R = RealOf<SP>;
+  __shared__ __attribute__((aligned(16))) char lds[Stage<SP, 12>::BYTES];
+  const Map m = map_thread(S, g.Vh);
+  CgState* st = epi.st ? epi.st + m.r : nullptr;
+  const bool skip = st && rhs_idle(st, epi.pred_reliable);
+  if (epi.st && skip) return;  // uniform: one right-hand side per launch
+  const uint32_t e0 = (uint32_t)((m.chunk - (threadIdx.x >> 6)) * 64);  // first site of the workgroup
+  const int t = (int)threadIdx.x;
+  Stage<SP, 12>::put(lds, in, g.stride, e0 + t, e0 + t < (uint64_t)g.Vh, t, m.r);
+  __syncthreads();
+  double nrm = 0;
+  if (m.active) {
+    const uint32_t e = m.e;
+    const int r = m.r;
+    const Site s = site_of(g, op, e);
+    const void* ub = U.data[op];
+    const double* det = &U.det[op][0][0];
+    const R anis = (R)anis_d;
+    const bool ua = use_anis != 0;
+    cx<R> o[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) o[k] = mk<R>(0, 0);
+    hop_slot<SP, LK, R, 0, true, false>(g, s, e, in, r, ub, det, anis, ua, o, lds, e0);
+    hop_slot<SP, LK, R, 0, false, false>(g, s, e, in, r, ub, det, anis, ua, o, lds, e0);
+    hop_slot<SP, LK, R, 1, true, false>(g, s, e, in, r, ub, det, anis, ua, o, lds, e0);
+    hop_slot<SP, LK, R, 1, false, true>(g, s, e, in, r, ub, det, anis, ua, o, lds, e0);
+    hop_slot<SP, LK, R, 2, true, false>(g, s, e, in, r, ub, det, anis, ua, o, lds, e0);
+    hop_slot<SP, LK, R, 2, false, false>(g, s, e, in, r, ub, det, anis, ua, o, lds, e0);
+    hop_slot<SP, LK, R, 3, true, false>(g, s, e, in, r, ub, det, anis, ua, o, lds, e0);
+    hop_slot<SP, LK, R, 3, false, true>(g, s, e, in, r, ub, det, anis, ua, o, lds, e0);
+    nrm = hop_epilogue<SP, MODE, XP, R>(g, epi, out, e, r, st, o);
+  }
+  if constexpr (MODE != 0) store_partial(nrm, partial, nchunks, m.r, m.chunk);
+}
+
 // ---------------------------------------------------------------- register-blocked RHS
 // One slot of the hop for RB right-hand sides r0..r0+nrb-1 of the same site: the
 // link is loaded once and applied to each right-hand side's projected neighbour.
@@ -362,18 +408,27 @@ double tm_hop_bytes(int sp, int lk, const HopEpi& epi, int ncomp)
 int64_t blocks_for(int64_t n) { return (n + kBlock - 1) / kBlock; }
 
 // launch of the register-blocked kernel with RB right-hand sides per wavefront
+// (RB = 0: the one-RHS kernel with the field-kernel shape)
 template <int SP, int LK, int MODE, int XP>
 static int launch_rb(nq_ctx* ctx, int RB, int op, const SpinorView& out, const SpinorView& in, const LinkView& U,
                      double anis, const HopEpi& epi, int nr)
 {
   const Geo& g = ctx->geo;
+  const int ua = anis != 1.0 ? 1 : 0;
+  if (RB <= 0) {
+    // one right-hand side per wavefront (the kernel of the unbatched operators)
+    const Launch L = launch_shape(g, nr);
+    hipLaunchKernelGGL((k_tm_hop<SP, LK, MODE, XP>), dim3((unsigned)L.blocks), dim3(L.threads), 0, ctx->stream, g, op,
+                       out, in, U, anis, ua, epi, L.s, L.nchunks, ctx->partial);
+    NQ_HIP(hipGetLastError());
+    return NQ_OK;
+  }
   Shape sh;
   sh.W = 4;
   sh.NRB = 1;
   sh.NRG = sh.NR = (nr + RB - 1) / RB;
   const int64_t nchunks = (g.Vh + 63) / 64;
   const dim3 grid((unsigned)((nchunks + 3) / 4 * sh.NRG)), blk(256);
-  const int ua = anis != 1.0 ? 1 : 0;
   switch (RB) {
   case 1: hipLaunchKernelGGL((k_tm_hop_rb<SP, LK, MODE, XP, 1>), grid, blk, 0, ctx->stream, g, op, out, in, U, anis, ua, epi, sh, nr, nchunks, ctx->partial); break;
   case 2: hipLaunchKernelGGL((k_tm_hop_rb<SP, LK, MODE, XP, 2>), grid, blk, 0, ctx->stream, g, op, out, in, U, anis, ua, epi, sh, nr, nchunks, ctx->partial); break;
@@ -385,10 +440,12 @@ static int launch_rb(nq_ctx* ctx, int RB, int op, const SpinorView& out, const S
 
 // Autotuner of the right-hand sides per wavefront of the batched sloppy hop: on the
 // first batched launch of a (precision, batch size) it times the write-only kernel
-// (no solver state, output into a scratch buffer) with 1, 2 and 4 right-hand sides
-// per wavefront on the caller's input and links, and keeps the fastest for every
-// mode of that precision and batch size.  The arithmetic of a right-hand side is the
-// same for every choice.  NQ_TUNE=0 or an open graph capture use RB = 2.
+// (no solver state, output into a scratch buffer) with one right-hand side per
+// wavefront (the unbatched kernel, RB = 0, or the blocked one with RB = 1) and with
+// 2 and 4 per wavefront, on the caller's input and links, and keeps the fastest for
+// every mode of that precision and batch size.  All choices run the same per-RHS
+// arithmetic (hop_epilogue is shared, the slot sums run in the same order), so the
+// choice never changes a result.  NQ_TUNE=0 or an open graph capture use RB = 2.
 template <int SP, int LK, int XP>
 static int tuned_rb(nq_ctx* ctx, int op, const SpinorView& in, const LinkView& U, double anis, const HopEpi& epi,
                     int nr, int* rb)
@@ -417,7 +474,7 @@ static int tuned_rb(nq_ctx* ctx, int op, const SpinorView& in, const LinkView& U
   NQ_HIP(hipEventCreate(&b));
   float best = 1e30f;
   int best_rb = 2;
-  for (int cand : {1, 2, 4}) {
+  for (int cand : {0, 1, 2, 4}) {
     if (cand > 1 && cand > nr) continue;
     NQ_TRY((launch_rb<SP, LK, 0, XP>(ctx, cand, op, scratch, in, U, anis, e0, nr)));  // warm-up
     NQ_HIP(hipEventRecord(a, ctx->stream));
@@ -464,8 +521,15 @@ static int go(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, 
     }
   }
   prof_begin(ctx, name, tm_hop_bytes(SP, LK, epi) * (double)g.Vh, &ev);
-  hipLaunchKernelGGL((k_tm_hop<SP, LK, MODE, XP>), dim3((unsigned)L.blocks), dim3(L.threads), 0, ctx->stream, g, op, out,
-                     in, U, anis, anis != 1.0 ? 1 : 0, epi, L.s, L.nchunks, ctx->partial);
+  // FP64: neighbours staged in LDS (measured 4 % faster at 32^3x64; the sloppy
+  // kernels measured slower with staging and read their neighbours from memory)
+  const char* lenv = getenv("NQ_LDS");
+  if (SP == SP_F64 && nr == 1 && L.s.W == 4 && !(lenv && lenv[0] == '0'))
+    hipLaunchKernelGGL((k_tm_hop_lds<SP, LK, MODE, XP>), dim3((unsigned)L.blocks), dim3(L.threads), 0, ctx->stream, g, op,
+                       out, in, U, anis, anis != 1.0 ? 1 : 0, epi, L.s, L.nchunks, ctx->partial);
+  else
+    hipLaunchKernelGGL((k_tm_hop<SP, LK, MODE, XP>), dim3((unsigned)L.blocks), dim3(L.threads), 0, ctx->stream, g, op,
+                       out, in, U, anis, anis != 1.0 ? 1 : 0, epi, L.s, L.nchunks, ctx->partial);
   NQ_HIP(hipGetLastError());
   prof_end(ctx, ev);
   if (MODE != 0) NQ_TRY(launch_finalize(ctx, nr, epi.st, epi.fin, epi.pred_reliable));

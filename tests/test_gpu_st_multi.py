@@ -191,3 +191,27 @@ def test_multi_rhs_graph_replays_match_eager(monkeypatch):
     for a, b in zip(out["0"], out["1"]):
         assert np.array_equal(a, b)
     ctx.close()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_st_rational_sum_closed_form(st_setups, prec):
+    """summ_src_and_all_inv_stD2ee_m2_cgm: with the partial fractions of
+    1/((x+a)(x+b)) = (1/(b-a)) (1/(x+a) - 1/(x+b)) (cons 0, poles a, b, weights
+    +-1/(b-a)), chi = (A+a)^-1 (A+b)^-1 eta, so (A+b)(A+a) chi = eta up to the solve
+    tolerance; and cons only adds cons*eta."""
+    L = (8, 8, 8, 8)
+    ctx, g, geo, conf, ce, co = st_setups(L)
+    a, b = 0.01, 0.2
+    srcs = [gauss_eo(geo.Vh, 100 + i, ncomp=3) for i in range(2)]
+    fsrc = [ctx.field(nq.COLOR, nq.EVEN, s) for s in srcs]
+    chis = [ctx.field(nq.COLOR, nq.EVEN) for _ in srcs]
+    res = nq.st_cgm_rat_sum(ctx, g, 0.0, [a, b], [1 / (b - a), -1 / (b - a)], 20000, 1e-24, chis, fsrc, prec=prec)
+    assert all(r["rc"] == nq.NQ_OK for r in res)
+    for s, chi in zip(srcs, chis):
+        y = geo.stD2ee_m2(ce, co, a, chi.download())
+        z = geo.stD2ee_m2(ce, co, b, y)
+        assert rel_l2(z, s) < 1e-9
+    chis2 = [ctx.field(nq.COLOR, nq.EVEN) for _ in srcs]
+    nq.st_cgm_rat_sum(ctx, g, 0.75, [a, b], [1 / (b - a), -1 / (b - a)], 20000, 1e-24, chis2, fsrc, prec=prec)
+    for s, c1, c2 in zip(srcs, chis, chis2):
+        assert rel_l2(c2.download() - c1.download(), 0.75 * s) < 1e-9
