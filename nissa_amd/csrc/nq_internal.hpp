@@ -102,12 +102,18 @@ constexpr int kMaxShift = 16;
 
 // Multi-shift CG state of one source (cgm_invert recurrences); lives in device
 // memory, updated by the CGM finalisation kernels.
+// Mixed precision (mixed = 1): the sloppy solutions are flushed into FP64 and the
+// Krylov residual is replaced by the FP64 residual of the hardest running shift
+// (kref, the smallest running shift) whenever that shift's squared residual has
+// dropped by delta2 since the last such update.  reliable = 1: flagged at the end of an iteration
+// whose p/ps update still runs; 2: idle until the host's update clears it.
 struct CgmState {
-  double b2, rr, betaa, betap, alpha, rfrf, pap, pad0;
+  double b2, rr, betaa, betap, alpha, rfrf, pap, rel_ref, delta2, pad0;
   double shift[kMaxShift], target[kMaxShift], zps[kMaxShift], zas[kMaxShift], zfs[kMaxShift];
   double betas[kMaxShift], alphas[kMaxShift], res[kMaxShift];
   int run[kMaxShift];
-  int done, nshift, nrun, iter, niter, nan, pad1[2];
+  int done, reliable;  // adjacent: the sloppy kernels skip a source when either is set
+  int nshift, nrun, iter, niter, nan, mixed, kref, nreliable, pad1[2];
 };
 
 enum Fin {
@@ -260,6 +266,10 @@ double tm_hop_bytes(int sp, int lk, const HopEpi& epi, int ncomp = 12);
 int launch_st_hop(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, const LinkView& U, double c1,
                   double c2, const double* coef_dev, const SpinorView& aux, int mode, int write, int nr, int fin,
                   CgState* st, const int* skip_flags = nullptr, int skip_stride = 0, int skip_n = 1);
+// sol64 += sol ; sol = 0 for every shift of the sources (only those waiting for a
+// reliable update when only_reliable)
+int launch_cgm_flush(nq_ctx* ctx, const SpinorView& sol64, const SpinorView& sol, int64_t sstride, int nshift,
+                     void* states, int nr, int only_reliable);
 // one multi-shift CG iteration for nr sources (states: device CgmState[nr]);
 // sol/ps hold the shifts of each source at element offset k*sstride
 int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, const SpinorView& t,

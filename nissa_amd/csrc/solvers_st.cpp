@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -204,10 +205,14 @@ static int st_cg_batched(nq_ctx* ctx, const nq_gauge* g, int nr, const double* s
 
 // Batched multi-shift CG (cgm_invert) for nr <= 16 sources; sol/ps laid out
 // [source][shift][colour][site].  States: device CgmState[nr] in a ctx temp.
+// With x64 (mixed precision): sol is the FP32 accumulator since the last reliable
+// update and x64 the FP64 solution; b64/U64 give the FP64 residual of the update.
 static int st_cgm_batched(nq_ctx* ctx, const LinkView& U, int nr, int nshift, const double* shifts, int niter,
                           const double* residues, const SpinorView& b, const SpinorView& sol, int sp,
-                          std::vector<int>* iters)
+                          std::vector<int>* iters, const SpinorView* x64 = nullptr, const SpinorView* b64 = nullptr,
+                          const LinkView* U64 = nullptr, std::vector<int>* nrel = nullptr)
 {
+  const bool mixed = x64 != nullptr;
   const int64_t csz = 3 * ctx->geo.stride;
   SpinorView p, t, r, ps;
   NQ_TRY(alloc_spinor_nr(ctx, sp == SP_F64 ? "cgm.p" : "cgm.p32", sp, 3, nr, &p));
@@ -240,6 +245,10 @@ static int st_cgm_batched(nq_ctx* ctx, const LinkView& U, int nr, int nshift, co
     s.nshift = nshift;
     s.nrun = nshift;
     s.niter = niter;
+    s.mixed = mixed;
+    s.rel_ref = 1.0;
+    s.delta2 = 1e-4;
+    if (const char* e = getenv("NQ_CGM_DELTA2")) s.delta2 = atof(e);
     for (int k = 0; k < nshift; k++) {
       s.shift[k] = shifts[k];
       s.target[k] = residues[k];
@@ -257,12 +266,50 @@ static int st_cgm_batched(nq_ctx* ctx, const LinkView& U, int nr, int nshift, co
   }
   NQ_TRY(launch_convert(ctx, p, b, nr));
   NQ_TRY(launch_convert(ctx, r, b, nr));
+  SpinorView t64, s64;
+  if (mixed) {
+    for (int k = 0; k < nshift; k++) NQ_TRY(launch_zero(ctx, shifted(*x64, k * csz), nr));
+    NQ_TRY(alloc_spinor(ctx, "cgm.t64", SP_F64, 3, &t64));
+    NQ_TRY(alloc_spinor(ctx, "cgm.s64", SP_F64, 3, &s64));
+  }
   int chunk = 16, launched = 0;
   for (;;) {
     for (int it = 0; it < chunk; it++) NQ_TRY(launch_cgm_iteration(ctx, U, p, t, r, sol, ps, csz, dst, nr));
     launched += chunk;
     NQ_HIP(hipMemcpyAsync(hst, dst, sizeof(CgmState) * nr, hipMemcpyDeviceToHost, ctx->stream));
     NQ_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<int> upd;
+    for (int i = 0; i < nr; i++)
+      if (mixed && hst[i].reliable && !hst[i].done) upd.push_back(i);
+    if (!upd.empty()) {
+      // reliable update: flush the FP32 accumulators into FP64; replace the Krylov
+      // residual by the FP64 residual of the hardest running shift kref,
+      // r = (b - A(sigma_kref) x_kref) / zeta_kref, which it equals in exact arithmetic
+      NQ_TRY(launch_cgm_flush(ctx, *x64, sol, csz, nshift, dst, nr, 1));
+      for (int i : upd) {
+        const int kr = hst[i].kref;
+        const double zeta = hst[i].zas[kr];
+        const SpinorView xk = shifted(*x64, (int64_t)i * x64->rstride + kr * csz);
+        NQ_TRY(launch_st_hop(ctx, NQ_ODD, t64, xk, *U64, 1.0, 0.0, nullptr, none, 0, 1, 1, FIN_NONE, nullptr));
+        NQ_TRY(launch_st_hop(ctx, NQ_EVEN, s64, t64, *U64, -0.25, shifts[kr], nullptr, xk, 0, 1, 1, FIN_NONE, nullptr));
+        CgState z0 = {};
+        NQ_TRY(put_states(ctx, i, 1, &z0));
+        NQ_TRY(launch_lincomb(ctx, shifted(r, (int64_t)i * r.rstride), 1.0 / zeta, shifted(*b64, (int64_t)i * b64->rstride),
+                              -1.0 / zeta, s64, 0, none, FIN_STORE0, ctx->state + i, 1));
+      }
+      NQ_TRY(get_states(ctx, 0, nr));
+      for (int i : upd) {
+        CgmState& h = hst[i];
+        h.rr = ctx->state_host[i].extra[0];
+        for (int k = 0; k < nshift; k++)
+          if (h.run[k]) h.res[k] = h.rr * h.zas[k] * h.zas[k] / h.b2;
+        h.rel_ref = h.res[h.kref];
+        h.reliable = 0;
+        h.nreliable += 1;
+        if (std::isnan(h.rr)) return fail(NQ_ENAN, "nanned");
+      }
+      NQ_HIP(hipMemcpyAsync(dst, hst, sizeof(CgmState) * nr, hipMemcpyHostToDevice, ctx->stream));
+    }
     bool all = true;
     int need = 4;
     for (int i = 0; i < nr; i++) {
@@ -278,9 +325,14 @@ static int st_cgm_batched(nq_ctx* ctx, const LinkView& U, int nr, int nshift, co
     if (all || launched > niter + 64) break;
     chunk = std::max(4, std::min(128, need));
   }
+  if (mixed) NQ_TRY(launch_cgm_flush(ctx, *x64, sol, csz, nshift, dst, nr, 0));
   if (iters) {
     iters->resize(nr);
     for (int i = 0; i < nr; i++) (*iters)[i] = hst[i].iter;
+  }
+  if (nrel) {
+    nrel->resize(nr);
+    for (int i = 0; i < nr; i++) (*nrel)[i] = hst[i].nreliable;
   }
   for (int i = 0; i < nr; i++)
     if (!hst[i].done || hst[i].iter >= niter) return fail(NQ_ENOCONV, "exit without converging");
@@ -404,45 +456,89 @@ int nq_st_cgm(nq_ctx* c, const nq_gauge* g, int nrhs, int nshift, const double* 
     }
     NQ_TRY(alloc_spinor_nr(c, mixed ? "cgm.sol32" : "cgm.sol", sp, 3, nr * nshift, &sol));
     sol.rstride = csz * nshift;
-    std::vector<double> res(nshift);
-    for (int k = 0; k < nshift; k++) res[k] = mixed ? std::max(residues[k], 1e-11) : residues[k];
-    std::vector<int> its;
-    int rci = st_cgm_batched(c, mixed ? U32 : U64, nr, nshift, shifts, niter, res.data(), bs, sol, sp, &its);
+    SpinorView x64 = sol;
+    if (mixed) {
+      NQ_TRY(alloc_spinor_nr(c, "cgm.x64", SP_F64, 3, nr * nshift, &x64));
+      x64.rstride = csz * nshift;
+    }
+    std::vector<int> its, nrel;
+    int rci = st_cgm_batched(c, mixed ? U32 : U64, nr, nshift, shifts, niter, residues, bs, sol, sp, &its,
+                             mixed ? &x64 : nullptr, &b, &U64, &nrel);
     if (rci != NQ_OK && rci != NQ_ENOCONV) return rci;
     if (rci == NQ_ENOCONV) rc = rci;
     // copy out; FP64 solutions per (source, shift)
     for (int i = 0; i < nr; i++)
       for (int k = 0; k < nshift; k++)
-        NQ_TRY(launch_convert(c, view_of(sols[(r0 + i) * nshift + k], 0), shifted(sol, i * csz * nshift + k * csz)));
+        NQ_TRY(launch_convert(c, view_of(sols[(r0 + i) * nshift + k], 0), shifted(x64, i * csz * nshift + k * csz)));
     for (int i = 0; i < nr; i++)
       if (st) {
         st[r0 + i] = nq_solve_stats{};
         st[r0 + i].iters = its.empty() ? 0 : its[i];
+        st[r0 + i].reliable_updates = nrel.empty() ? 0 : nrel[i];
       }
-    // check / refine every (source, shift) system with the FP64 operator, 16 at a time
-    // systems in shift-major order: a group of 16 holds neighbouring shifts, so the
-    // slow small-shift systems share groups instead of stretching every group
-    const int nsys = nr * nshift;
-    for (int s0 = 0; s0 < nsys; s0 += 16) {
-      const int ns = std::min(16, nsys - s0);
-      SpinorView xb, bb;
-      NQ_TRY(alloc_spinor_nr(c, "cgm.xb", SP_F64, 3, ns, &xb));
-      NQ_TRY(alloc_spinor_nr(c, "cgm.bb", SP_F64, 3, ns, &bb));
-      std::vector<double> sh(ns), rs(ns);
-      for (int j = 0; j < ns; j++) {
-        const int k = (s0 + j) / nr, i = (s0 + j) % nr;
-        sh[j] = shifts[k];
-        rs[j] = residues[k];
-        NQ_TRY(launch_convert(c, shifted(xb, j * csz), view_of(sols[(r0 + i) * nshift + k], 0)));
-        NQ_TRY(launch_convert(c, shifted(bb, j * csz), view_of(srcs[r0 + i], 0)));
+    // FP64 check |A(sigma) x - b|^2 / |b|^2 of every (source, shift) system, 16 at a
+    // time; in mixed precision the systems that miss their residue are refined by a
+    // batched FP32 CG with FP64 reliable updates (warm start) and checked again
+    std::vector<double> tr((size_t)nr * nshift, 0.0);
+    auto check = [&](const std::vector<int>& sys) -> int {
+      for (size_t s0 = 0; s0 < sys.size(); s0 += 16) {
+        const int ns = (int)std::min<size_t>(16, sys.size() - s0);
+        SpinorView xb, bb, tt, ss;
+        NQ_TRY(alloc_spinor_nr(c, "cgm.xb", SP_F64, 3, ns, &xb));
+        NQ_TRY(alloc_spinor_nr(c, "cgm.bb", SP_F64, 3, ns, &bb));
+        NQ_TRY(alloc_spinor_nr(c, "cgm.tt", SP_F64, 3, ns, &tt));
+        NQ_TRY(alloc_spinor_nr(c, "cgm.ss", SP_F64, 3, ns, &ss));
+        double* coef = nullptr;
+        NQ_TRY(ctx_temp(c, "cgm.coef", sizeof(double) * 32, (void**)&coef));
+        std::vector<double> hc(2 * ns);
+        for (int j = 0; j < ns; j++) {
+          const int q = sys[s0 + j], i = q / nshift, k = q % nshift;
+          hc[2 * j] = -0.25;
+          hc[2 * j + 1] = shifts[k];
+          NQ_TRY(launch_convert(c, shifted(xb, j * csz), view_of(sols[(r0 + i) * nshift + k], 0)));
+          NQ_TRY(launch_convert(c, shifted(bb, j * csz), view_of(srcs[r0 + i], 0)));
+        }
+        NQ_HIP(hipMemcpyAsync(coef, hc.data(), sizeof(double) * 2 * ns, hipMemcpyHostToDevice, c->stream));
+        NQ_TRY(apply_DDee(c, U64, 0, coef, ss, xb, tt, ns));
+        std::vector<CgState> z(ns);
+        NQ_TRY(put_states(c, 0, ns, z.data()));
+        SpinorView none;
+        none.ncomp = 3;
+        NQ_TRY(launch_lincomb(c, none, 1.0, bb, -1.0, ss, 0, none, FIN_STORE0, c->state, ns));
+        NQ_TRY(launch_lincomb(c, none, 1.0, bb, 0, none, 0, none, FIN_STORE1, c->state, ns));
+        NQ_TRY(get_states(c, 0, ns));
+        for (int j = 0; j < ns; j++) tr[sys[s0 + j]] = c->state_host[j].extra[0] / c->state_host[j].extra[1];
       }
-      std::vector<nq_solve_stats> sst(ns);
-      if (mixed) {
+      return NQ_OK;
+    };
+    std::vector<int> all;
+    for (int q = 0; q < nr * nshift; q++) all.push_back(q);
+    NQ_TRY(check(all));
+    if (mixed) {
+      // failing systems in shift-major order, so groups hold neighbouring shifts
+      std::vector<int> bad;
+      for (int k = 0; k < nshift; k++)
+        for (int i = 0; i < nr; i++)
+          if (!(tr[i * nshift + k] < residues[k])) bad.push_back(i * nshift + k);
+      for (size_t s0 = 0; s0 < bad.size(); s0 += 16) {
+        const int ns = (int)std::min<size_t>(16, bad.size() - s0);
+        SpinorView xb, bb;
+        NQ_TRY(alloc_spinor_nr(c, "cgm.rxb", SP_F64, 3, ns, &xb));
+        NQ_TRY(alloc_spinor_nr(c, "cgm.rbb", SP_F64, 3, ns, &bb));
+        std::vector<double> sh(ns), rs(ns);
+        for (int j = 0; j < ns; j++) {
+          const int q = bad[s0 + j], i = q / nshift, k = q % nshift;
+          sh[j] = shifts[k];
+          rs[j] = residues[k];
+          NQ_TRY(launch_convert(c, shifted(xb, j * csz), view_of(sols[(r0 + i) * nshift + k], 0)));
+          NQ_TRY(launch_convert(c, shifted(bb, j * csz), view_of(srcs[r0 + i], 0)));
+        }
+        std::vector<nq_solve_stats> sst(ns);
         const int rr = st_cg_batched(c, g, ns, sh.data(), niter, rs.data(), NQ_PREC_F32, xb, bb, true, sst.data());
         if (rr != NQ_OK && rr != NQ_ENOCONV) return rr;
         if (rr == NQ_ENOCONV) rc = rr;
         for (int j = 0; j < ns; j++) {
-          const int k = (s0 + j) / nr, i = (s0 + j) % nr;
+          const int q = bad[s0 + j], i = q / nshift, k = q % nshift;
           NQ_TRY(launch_convert(c, view_of(sols[(r0 + i) * nshift + k], 0), shifted(xb, j * csz)));
           if (st) {
             st[r0 + i].reliable_updates += sst[j].reliable_updates;
@@ -450,30 +546,14 @@ int nq_st_cgm(nq_ctx* c, const nq_gauge* g, int nrhs, int nshift, const double* 
           }
         }
       }
-      // FP64 true residual |A(sigma) x - b|^2 / |b|^2
-      double* coef = nullptr;
-      NQ_TRY(ctx_temp(c, "cgm.coef", sizeof(double) * 32, (void**)&coef));
-      std::vector<double> hc(2 * ns);
-      for (int j = 0; j < ns; j++) { hc[2 * j] = -0.25; hc[2 * j + 1] = sh[j]; }
-      NQ_HIP(hipMemcpyAsync(coef, hc.data(), sizeof(double) * 2 * ns, hipMemcpyHostToDevice, c->stream));
-      SpinorView tt, ss;
-      NQ_TRY(alloc_spinor_nr(c, "cgm.tt", SP_F64, 3, ns, &tt));
-      NQ_TRY(alloc_spinor_nr(c, "cgm.ss", SP_F64, 3, ns, &ss));
-      NQ_TRY(apply_DDee(c, U64, 0, coef, ss, xb, tt, ns));
-      std::vector<CgState> z(ns);
-      NQ_TRY(put_states(c, 0, ns, z.data()));
-      SpinorView none;
-      none.ncomp = 3;
-      NQ_TRY(launch_lincomb(c, none, 1.0, bb, -1.0, ss, 0, none, FIN_STORE0, c->state, ns));
-      NQ_TRY(launch_lincomb(c, none, 1.0, bb, 0, none, 0, none, FIN_STORE1, c->state, ns));
-      NQ_TRY(get_states(c, 0, ns));
-      for (int j = 0; j < ns; j++) {
-        const int k = (s0 + j) / nr, i = (s0 + j) % nr;
-        const double tr = c->state_host[j].extra[0] / c->state_host[j].extra[1];
-        if (true_res_sq) true_res_sq[(r0 + i) * nshift + k] = tr;
-        if (st) st[r0 + i].rel_res_sq = std::max(st[r0 + i].rel_res_sq, tr);
-      }
+      if (!bad.empty()) NQ_TRY(check(bad));
     }
+    for (int i = 0; i < nr; i++)
+      for (int k = 0; k < nshift; k++) {
+        const double t = tr[i * nshift + k];
+        if (true_res_sq) true_res_sq[(r0 + i) * nshift + k] = t;
+        if (st) st[r0 + i].rel_res_sq = std::max(st[r0 + i].rel_res_sq, t);
+      }
   }
   (void)hipEventRecord(eb, c->stream);
   (void)hipEventSynchronize(eb);

@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(256) k_cgm_hop2(Geo g, SpinorView t, LinkView 
   const Map m = map_thread(SH, g.Vh);
   const CgmState* S = st + m.r;
   double red = 0;
-  if (m.active && !S->done) {
+  if (m.active && !S->done && !S->reliable) {
     cx<R> o[3];
     st_hop_site<SP, LK, R>(g, 0, m.e, t, m.r, U, o);
     const R betaa = (R)S->betaa;
@@ -156,7 +156,9 @@ __global__ void __launch_bounds__(256) k_cgm_update(Geo g, SpinorView rv, Spinor
   using R = RealOfS<SP>;
   const Map m = map_thread(SH, g.Vh);
   const CgmState* S = st + m.r;
-  if (!m.active || S->done) return;
+  // reliable == 1: the update was flagged in this iteration, whose p/ps update
+  // still runs; 2: waiting for the host's reliable update
+  if (!m.active || S->done || S->reliable == 2) return;
   cx<R> rr[3], pp[3];
   SpAcc<SP, R, 3>::load(rv, g.stride, m.e, rr, m.r);
   SpAcc<SP, R, 3>::load(p, g.stride, m.e, pp, m.r);
@@ -223,6 +225,13 @@ __device__ inline void cgm_finalize(CgmState* s, int fin, double total)
     }
     if (isnan(rfrf)) { s->nan = 1; s->done = 1; }
     if (s->nrun <= 0 || s->iter >= s->niter) s->done = 1;
+    if (s->mixed && !s->done) {
+      int kr = -1;
+      for (int k = 0; k < s->nshift; k++)
+        if (s->run[k] && (kr < 0 || s->shift[k] < s->shift[kr])) kr = k;
+      s->kref = kr;
+      if (kr >= 0 && s->res[kr] < s->delta2 * s->rel_ref) s->reliable = 1;
+    }
   }
 }
 
@@ -231,6 +240,12 @@ __global__ void __launch_bounds__(1024) k_cgm_finalize(const double* partial, in
   __shared__ double vl[1024];
   CgmState* s = st + blockIdx.x;
   if (s->done) return;  // uniform over the workgroup
+  if (s->reliable) {
+    // first finalisation after the flag: the flagged iteration is complete
+    // (its p/ps update has run); later launches idle until the host's update
+    if (threadIdx.x == 0) s->reliable = 2;
+    return;
+  }
   const double tot = block_sum_partials(partial + (int64_t)blockIdx.x * nchunks, nchunks, vl);
   if (threadIdx.x == 0) cgm_finalize(s, fin, tot);
 }
@@ -273,6 +288,53 @@ int launch_st_hop(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& 
   return NQ_OK;
 }
 
+// sol64_k += sol_k ; sol_k = 0 for all shifts k of source r
+template <int SP>
+__global__ void __launch_bounds__(256) k_cgm_flush(Geo g, SpinorView x64, SpinorView sol, int64_t sstride, int nshift,
+                                                   const CgmState* st, Shape SH, int only_reliable)
+{
+  using R = RealOfS<SP>;
+  const Map m = map_thread(SH, g.Vh);
+  const CgmState* S = st + m.r;
+  if (!m.active || (only_reliable && (!S->reliable || S->done))) return;
+  for (int k = 0; k < nshift; k++) {
+    SpinorView xk = x64, sk = sol;
+    xk.data = (char*)x64.data + k * sstride * 16;
+    sk.data = (char*)sol.data + k * sstride * (SP == SP_F64 ? 16 : 8);
+    cx<double> a[3];
+    cx<R> b[3];
+    SpAcc<SP_F64, double, 3>::load(xk, g.stride, m.e, a, m.r);
+    SpAcc<SP, R, 3>::load(sk, g.stride, m.e, b, m.r);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      a[c] = a[c] + mk<double>((double)b[c].re, (double)b[c].im);
+      b[c] = mk<R>(0, 0);
+    }
+    SpAcc<SP_F64, double, 3>::store(xk, g.stride, m.e, a, m.r);
+    SpAcc<SP, R, 3>::store(sk, g.stride, m.e, b, m.r);
+  }
+}
+
+int launch_cgm_flush(nq_ctx* ctx, const SpinorView& x64, const SpinorView& sol, int64_t sstride, int nshift,
+                     void* states, int nr, int only_reliable)
+{
+  const Geo& g = ctx->geo;
+  const Launch L = launch_shape(g, nr);
+  if (x64.prec != SP_F64) return fail(NQ_EINVAL, "cgm flush: FP64 accumulators expected");
+  const dim3 grid((unsigned)L.blocks), blk(L.threads);
+  hipEvent_t ev = nullptr;
+  prof_begin(ctx, "cgm_flush", (sol.prec == SP_F64 ? 64.0 : 56.0) * nshift * nr * (double)g.Vh, &ev);
+  if (sol.prec == SP_F64)
+    hipLaunchKernelGGL(k_cgm_flush<SP_F64>, grid, blk, 0, ctx->stream, g, x64, sol, sstride, nshift,
+                       (const CgmState*)states, L.s, only_reliable);
+  else
+    hipLaunchKernelGGL(k_cgm_flush<SP_F32>, grid, blk, 0, ctx->stream, g, x64, sol, sstride, nshift,
+                       (const CgmState*)states, L.s, only_reliable);
+  NQ_HIP(hipGetLastError());
+  prof_end(ctx, ev);
+  return NQ_OK;
+}
+
 int cgm_state_bytes() { return (int)sizeof(CgmState); }
 
 int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, const SpinorView& t,
@@ -292,11 +354,11 @@ int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, co
   if (sp == SP_F64)
     hipLaunchKernelGGL((k_st_hop<SP_F64, LK_F64_18, 1>), grid, blk, 0, ctx->stream, g, 1, t, p, U, 1.0, 0.0, nullptr,
                        none, 1, L.s, L.nchunks, ctx->partial, (const int*)&st->done,
-                       (int)(sizeof(CgmState) / sizeof(int)), 1);
+                       (int)(sizeof(CgmState) / sizeof(int)), 2);
   else
     hipLaunchKernelGGL((k_st_hop<SP_F32, LK_F32_18, 1>), grid, blk, 0, ctx->stream, g, 1, t, p, U, 1.0, 0.0, nullptr,
                        none, 1, L.s, L.nchunks, ctx->partial, (const int*)&st->done,
-                       (int)(sizeof(CgmState) / sizeof(int)), 1);
+                       (int)(sizeof(CgmState) / sizeof(int)), 2);
   NQ_HIP(hipGetLastError());
   prof_end(ctx, ev);
   hipLaunchKernelGGL(k_cgm_finalize, dim3(nr), dim3(1024), 0, ctx->stream, ctx->partial, L.nchunks, st,

@@ -21,7 +21,7 @@ if ROOT not in sys.path:
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--what", default="hop", choices=["hop", "hopmulti", "copy", "solve", "multisolve"])
+    ap.add_argument("--what", default="hop", choices=["hop", "hopmulti", "copy", "solve", "multisolve", "stagcgm"])
     ap.add_argument("--nrhs", type=int, default=12)
     ap.add_argument("--prec", default="f64", choices=["f64", "f32", "f16", "f16h"])
     ap.add_argument("--n", type=int, default=20)
@@ -58,6 +58,25 @@ def main():
             dt = _t.perf_counter() - t0
             print(f"multisolve x{a.nrhs} {a.prec}: {dt*1e3:.1f} ms ({dt*1e3/a.nrhs:.2f} ms per source), "
                   f"iters {[r['iters'] for r in res]}, worst check {max(r['rel_res_sq'] for r in res):.2e}")
+    elif a.what == "stagcgm":
+        import time as _t
+        sconf = nq.gauge_random_lx(L, 4, 0.3)
+        nq.gauge_stagphases_lx(L, sconf)
+        nq.gauge_phase_lx(L, sconf, [-1.0, 0, 0, 0])
+        gs = nq.Gauge(ctx, lx=sconf)
+        shifts = [0.0025 ** 2 + 10 ** (-4 + 5 * k / 11) for k in range(12)]
+        rng = np.random.default_rng(44)
+        srcs = [ctx.field(nq.COLOR, nq.EVEN, rng.normal(size=(Vh, 3, 2)) * np.sqrt(0.5)) for _ in range(a.nrhs)]
+        sols = [[ctx.field(nq.COLOR, nq.EVEN) for _ in shifts] for _ in srcs]
+        ctx.profiling(a.n > 0)
+        t0 = _t.perf_counter()
+        res = nq.st_cgm(ctx, gs, shifts, 100000, [1e-20] * 12, sols, srcs, prec=a.prec)
+        dt = _t.perf_counter() - t0
+        ctx.profiling(False)
+        print(f"stagcgm x{a.nrhs} {a.prec}: {dt:.3f} s, iters {res[0]['iters']}, "
+              f"worst {max(max(r['true_res_sq']) for r in res):.2e}")
+        for name, v in sorted(ctx.kernel_stats().items(), key=lambda kv: -kv[1]["total_ms"])[:10]:
+            print(f"   {name:28s} {v['launches']:7d} x {v['total_ms']/v['launches']*1e3:8.1f} us = {v['total_ms']:9.1f} ms")
     elif a.what == "copy":
         fo2 = ctx.field(nq.SPINCOLOR, nq.ODD)
         for _ in range(a.n):
