@@ -100,6 +100,22 @@ def main():
     t64 = (time.perf_counter() - t0) / n64
     log(f"[bench] FP64 solve: {t64*1e3:.2f} ms, iters={st64[-1]['iters']}")
 
+    # the other inner-precision variants of the same workload
+    variants = {}
+    for pv in ("f32", "f16", "f16h"):
+        if pv == args.prec:
+            continue
+        solve(pv)
+        ctx.sync()
+        t0 = time.perf_counter()
+        sv = [solve(pv) for _ in range(3)]
+        ctx.sync()
+        tv = (time.perf_counter() - t0) / 3
+        variants[pv] = {"ms_per_solve": round(tv * 1e3, 3), "iters": sv[-1]["iters"],
+                        "reliable_updates": sv[-1]["reliable_updates"],
+                        "true_rel_res_sq": sv[-1]["check_rel_res_sq"]}
+        log(f"[bench] {pv} solve: {tv*1e3:.2f} ms, iters={sv[-1]['iters']}")
+
     # roofline of the dominant kernel over a profiled pass identical to the timed region
     ctx.reset_kernel_stats()
     ctx.profiling(True)
@@ -170,6 +186,8 @@ def main():
         },
         "fp64": {"ms_per_solve": round(t64 * 1e3, 3), "solves_per_s": round(1 / t64, 4), "iters": st64[-1]["iters"],
                  "mixed_speedup": round(solves_per_s * t64, 3)},
+        "other_inner_precisions": {k: dict(v, speedup_vs_fp64=round(t64 * 1e3 / v["ms_per_solve"], 3))
+                                   for k, v in variants.items()},
         "dslash": {"fp64_GBs": round(dslash_gbs, 1), "fp64_us": round(ms_hop * 1e3, 2),
                    "fp64_frac_of_peak": round(dslash_gbs / HBM_PEAK_GBS, 4),
                    "bytes_per_apply": bytes_hop, "sloppy_us": round(ms_hop16 * 1e3, 2),

@@ -120,6 +120,16 @@ int nq_tm_solve_eoprec(nq_ctx* ctx, const nq_gauge* g, double kappa, const doubl
 /* inv_tmQ2_cg (cg_invert_tmQ2.cpp:12-19): CG on Q(-mu)Q(+mu) over lx */
 int nq_tmQ2_cg(nq_ctx* ctx, const nq_gauge* g, double kappa, double mu, int niter, double residue,
                nq_field* sol_lx, const nq_field* guess_lx, const nq_field* src_lx, nq_solve_stats* st);
+/* inv_tmQ2_cgm (cgm_invert_tmQ2.cpp:46-53): multi-mass CG on Q(-m_k)Q(+m_k) = Q(0)^2 + m_k^2
+   over lx (shifts m_k^2 of the mu = 0 operator), sols[k] per mass; true_res_sq[k] gets the
+   FP64 check |Q(-m)Q(+m) x_k - src|^2/|src|^2.  nmass <= 16. */
+int nq_tmQ2_cgm(nq_ctx* ctx, const nq_gauge* g, double kappa, int nmass, const double* masses, int niter,
+                const double* residues, nq_field** sols_lx, const nq_field* src_lx, nq_solve_stats* st,
+                double* true_res_sq);
+/* inv_tmDQ_cgm (cgm_invert_tmQ2.cpp:55-75): the same for the source g5 src */
+int nq_tmDQ_cgm(nq_ctx* ctx, const nq_gauge* g, double kappa, int nmass, const double* masses, int niter,
+                const double* residues, nq_field** sols_lx, const nq_field* src_lx, nq_solve_stats* st,
+                double* true_res_sq);
 /* multi-RHS eo solve: nrhs sources share the gauge handle, kappa and the iteration;
    mu[r] per right-hand side (per-RHS diagonal). Residuals are per RHS. */
 int nq_tm_solve_eoprec_multi(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis,

@@ -88,6 +88,8 @@ _D = C.c_double
 _I = C.c_int
 _DP = C.POINTER(C.c_double)
 _SIG = {
+    "nq_tmQ2_cgm": ([_P, _P, _D, _I, _DP, _I, _DP, _P, _P, _P, _DP], _I),
+    "nq_tmDQ_cgm": ([_P, _P, _D, _I, _DP, _I, _DP, _P, _P, _P, _DP], _I),
     "nq_st_cgm_rat_sum": ([_P, _P, _I, _I, _D, _DP, _DP, _I, _D, _I, _P, _P, _P], _I),
     "nq_bench_tm_hop_multi": ([_P, _P, _I, _I, _I, _P, _DP, _DP], _I),
     "nq_server_create": ([_I, C.c_int64, _I, C.POINTER(_P)], _I),
@@ -492,6 +494,22 @@ def st_cgm(ctx, g, shifts, niter, residues, sols, srcs, prec="f64"):
     if rc not in (NQ_OK, NQ_ENOCONV):
         _check(rc)
     return [dict(st[r].as_dict(), rc=rc, true_res_sq=[tr[r * ns + k] for k in range(ns)]) for r in range(nr)]
+
+
+def tmQ2_cgm(ctx, g, kappa, masses, niter, residues, sols, src, g5=False):
+    """inv_tmQ2_cgm (or inv_tmDQ_cgm with g5=True): sols[k] per mass; returns stats and
+    the FP64 true residual per mass."""
+    n = len(masses)
+    ma = (C.c_double * n)(*masses)
+    rs = (C.c_double * n)(*residues)
+    so = (_P * n)(*[x.h for x in sols])
+    st = SolveStats()
+    tr = (C.c_double * n)()
+    fn = lib().nq_tmDQ_cgm if g5 else lib().nq_tmQ2_cgm
+    rc = fn(ctx.h, g.h, kappa, n, ma, niter, rs, so, src.h, C.byref(st), tr)
+    if rc not in (NQ_OK, NQ_ENOCONV):
+        _check(rc)
+    return dict(st.as_dict(), rc=rc, true_res_sq=list(tr))
 
 
 def st_cgm_rat_sum(ctx, g, cons, poles, weights, niter, residue, chis, srcs, prec="f64"):

@@ -276,7 +276,7 @@ static int make_args(nq_ctx* ctx, int nr, CgState* st, int fin, int pred, BlasAr
   if (nr < 1 || nr > 16) return fail(NQ_EINVAL, "right-hand sides per launch must be 1..16");
   *L = launch_shape(ctx->geo, nr);
   if (L->nchunks * nr > ctx->max_partials) return fail(NQ_EINVAL, "reduction scratch too small");
-  if ((fin != FIN_NONE || pred) && !st) return fail(NQ_EINVAL, "reduction without solver state");
+  if (((fin != FIN_NONE && fin != FIN_PARTIAL) || pred) && !st) return fail(NQ_EINVAL, "reduction without solver state");
   A->g = ctx->geo;
   A->s = L->s;
   A->nchunks = L->nchunks;
@@ -310,7 +310,7 @@ int launch_axpby(nq_ctx* ctx, const SpinorView& y, const double a[2], const Spin
   }
   NQ_HIP(hipGetLastError());
   prof_end(ctx, ev);
-  if (red) NQ_TRY(launch_finalize(ctx, nr, st, fin, pred == 2 ? 1 : (pred == 3 ? 2 : 0)));
+  if (red && fin != FIN_PARTIAL) NQ_TRY(launch_finalize(ctx, nr, st, fin, pred == 2 ? 1 : (pred == 3 ? 2 : 0)));
   return NQ_OK;
 }
 
@@ -337,7 +337,7 @@ int launch_lincomb(nq_ctx* ctx, const SpinorView& out, double a, const SpinorVie
   }
   NQ_HIP(hipGetLastError());
   prof_end(ctx, ev);
-  if (red) NQ_TRY(launch_finalize(ctx, nr, st, fin, pred == 2 ? 1 : (pred == 3 ? 2 : 0)));
+  if (red && fin != FIN_PARTIAL) NQ_TRY(launch_finalize(ctx, nr, st, fin, pred == 2 ? 1 : (pred == 3 ? 2 : 0)));
   return NQ_OK;
 }
 

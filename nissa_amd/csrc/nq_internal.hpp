@@ -127,6 +127,7 @@ enum Fin {
   FIN_STORE1 = 7,     // extra[1] = total
   FIN_RR_TRUE_HOLD = 8,  // as FIN_RR_TRUE, but `reliable` stays raised (batched updates)
   FIN_INIT_B2 = 9,    // start of a solve on r = b: b2 = rr = total, target *= total
+  FIN_PARTIAL = 10,   // leave the per-chunk partials for a caller's own finalisation
 };
 
 // Hop epilogue: out_h = c1_h*hop + c2_h*aux + c3_h*aux2 (h = upper/lower spins).
@@ -270,6 +271,15 @@ int launch_st_hop(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& 
 // reliable update when only_reliable)
 int launch_cgm_flush(nq_ctx* ctx, const SpinorView& sol64, const SpinorView& sol, int64_t sstride, int nshift,
                      void* states, int nr, int only_reliable);
+// generic multi-shift CG pieces for lx spincolor fields (one source, FP64):
+// finalisation of the reduction just launched (fin 1: <p,A p> = pap_scale * sum,
+// 2: |r|^2), v2: sol_k -= betas_k ps_k, r += betaa s, |r|^2 (+ finalisation),
+// v3: p = r + alpha p, ps_k = zfs_k r + alphas_k ps_k
+int launch_cgm_finalize(nq_ctx* ctx, void* states, int nr, int fin, double pap_scale);
+int launch_cgm_vec2(nq_ctx* ctx, const SpinorView& s, const SpinorView& r, const SpinorView& sol, const SpinorView& ps,
+                    int64_t sstride, void* states);
+int launch_cgm_vec3(nq_ctx* ctx, const SpinorView& r, const SpinorView& p, const SpinorView& ps, int64_t sstride,
+                    void* states);
 // one multi-shift CG iteration for nr sources (states: device CgmState[nr]);
 // sol/ps hold the shifts of each source at element offset k*sstride
 int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, const SpinorView& t,

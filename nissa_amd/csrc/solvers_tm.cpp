@@ -908,6 +908,125 @@ static int tmQ2_cg_f64(const TmSetup& S, double mu, int niter, double residue, c
   return NQ_OK;
 }
 
+// Multi-mass CG on Q(-m_k)Q(+m_k) = Q(0)^2 + m_k^2 over lx (inv_tmQ2_cgm,
+// cgm_invert_tmQ2.cpp:46-53 with cgm_invert's recurrences): the matvec is Q(0)^2 (four
+// hop launches), <p, Q(0)^2 p> = |Q(0) p|^2, the shifts are m_k^2.  FP64.
+static int tmQ2_cgm_f64(nq_ctx* ctx, const nq_gauge* g, double kappa, int nmass, const double* masses, int niter,
+                        const double* residues, nq_field** sols, const nq_field* src, nq_solve_stats* st,
+                        double* true_res)
+{
+  TmSetup S;
+  NQ_TRY(tm_setup(ctx, g, LK_F64_18, nullptr, kappa, &S));
+  const int64_t fsz = 24 * ctx->geo.stride;  // elements of one lx field
+  SpinorView rv, pv, tv, sv, ps, xs;
+  NQ_TRY(alloc_spinor_nr(ctx, "q2.r", SP_F64, 12, 2, &rv));
+  NQ_TRY(alloc_spinor_nr(ctx, "q2.p", SP_F64, 12, 2, &pv));
+  NQ_TRY(alloc_spinor_nr(ctx, "q2.t", SP_F64, 12, 2, &tv));
+  NQ_TRY(alloc_spinor_nr(ctx, "q2.s", SP_F64, 12, 2, &sv));
+  NQ_TRY(alloc_spinor_nr(ctx, "q2m.ps", SP_F64, 24, nmass, &ps));
+  NQ_TRY(alloc_spinor_nr(ctx, "q2m.x", SP_F64, 24, nmass, &xs));
+  auto as_field = [&](void* p) {
+    nq_field f;
+    f.ctx = ctx;
+    f.kind = NQ_SPINCOLOR;
+    f.coverage = NQ_LX;
+    f.buf.p = p;
+    return f;
+  };
+  nq_field fr = as_field(rv.data), fp = as_field(pv.data), ft = as_field(tv.data), fs = as_field(sv.data);
+  struct Release {
+    nq_field* f[4];
+    ~Release()
+    {
+      for (auto* x : f) x->buf.p = nullptr;
+    }
+  } rel{{&fr, &fp, &ft, &fs}};
+  const SpinorView b = view_lx_flat(src), r = view_lx_flat(&fr), p = view_lx_flat(&fp), t = view_lx_flat(&ft),
+                   s = view_lx_flat(&fs);
+  SpinorView none;
+  none.ncomp = 24;
+  Timer T(ctx->stream);
+  NQ_TRY(put_state(ctx, 0, CgState{}));
+  NQ_TRY(launch_lincomb(ctx, none, 1.0, b, 0, none, 0, none, FIN_STORE0, ctx->state));
+  NQ_TRY(get_states(ctx, 0, 1));
+  const double b2 = ctx->state_host[0].extra[0];
+  if (b2 == 0 || std::isnan(b2)) return fail(NQ_ENAN, "invalid norm: %g", b2);
+  CgmState* dst = nullptr;
+  CgmState* hst = nullptr;
+  NQ_TRY(ctx_temp(ctx, "q2m.state", sizeof(CgmState), (void**)&dst));
+  NQ_HIP(hipHostMalloc((void**)&hst, sizeof(CgmState), hipHostMallocDefault));
+  struct HostFree {
+    CgmState* p;
+    ~HostFree() { (void)hipHostFree(p); }
+  } hf{hst};
+  CgmState h0 = {};
+  h0.b2 = b2;
+  h0.rr = b2;
+  h0.betaa = 1;
+  h0.nshift = nmass;
+  h0.nrun = nmass;
+  h0.niter = niter;
+  for (int k = 0; k < nmass; k++) {
+    h0.shift[k] = masses[k] * masses[k];
+    h0.target[k] = residues[k];
+    h0.zps[k] = h0.zas[k] = 1;
+    h0.run[k] = 1;
+  }
+  *hst = h0;
+  NQ_HIP(hipMemcpyAsync(dst, hst, sizeof(CgmState), hipMemcpyHostToDevice, ctx->stream));
+  for (int k = 0; k < nmass; k++) {
+    SpinorView xk = xs, pk = ps;
+    xk.data = (char*)xs.data + k * fsz * 16;
+    pk.data = (char*)ps.data + k * fsz * 16;
+    NQ_TRY(launch_zero(ctx, xk));
+    NQ_TRY(launch_convert(ctx, pk, b));
+  }
+  NQ_TRY(launch_convert(ctx, p, b));
+  NQ_TRY(launch_convert(ctx, r, b));
+  int chunk = 8, launched = 0;
+  for (;;) {
+    for (int it = 0; it < chunk; it++) {
+      NQ_TRY(apply_Q_views(S, 0.0, &ft, &fp));  // t = Q(0) p
+      NQ_TRY(launch_lincomb(ctx, none, 1.0, t, 0, none, 0, none, FIN_PARTIAL, nullptr));
+      NQ_TRY(launch_cgm_finalize(ctx, dst, 1, 1, 1.0));  // pap = |t|^2
+      NQ_TRY(apply_Q_views(S, 0.0, &fs, &ft));  // s = Q(0)^2 p
+      NQ_TRY(launch_cgm_vec2(ctx, s, r, xs, ps, fsz, dst));
+      NQ_TRY(launch_cgm_vec3(ctx, r, p, ps, fsz, dst));
+    }
+    launched += chunk;
+    NQ_HIP(hipMemcpyAsync(hst, dst, sizeof(CgmState), hipMemcpyDeviceToHost, ctx->stream));
+    NQ_HIP(hipStreamSynchronize(ctx->stream));
+    if (hst->nan) return fail(NQ_ENAN, "nanned");
+    if (hst->done || launched > niter + 64) break;
+    chunk = std::min(64, chunk * 2);
+  }
+  const int iters = hst->iter;
+  // copy out and the FP64 check |(Q(-m)Q(+m)) x_k - b|^2 / |b|^2
+  double worst = 0;
+  for (int k = 0; k < nmass; k++) {
+    SpinorView xk = xs;
+    xk.data = (char*)xs.data + k * fsz * 16;
+    NQ_TRY(launch_convert(ctx, view_lx_flat(sols[k]), xk));
+    NQ_TRY(apply_Q_views(S, +masses[k], &ft, sols[k]));
+    NQ_TRY(apply_Q_views(S, -masses[k], &fs, &ft));
+    NQ_TRY(put_state(ctx, 0, CgState{}));
+    NQ_TRY(launch_lincomb(ctx, none, 1.0, s, -1.0, b, 0, none, FIN_STORE0, ctx->state));
+    NQ_TRY(get_states(ctx, 0, 1));
+    const double tr = ctx->state_host[0].extra[0] / b2;
+    if (true_res) true_res[k] = tr;
+    worst = std::max(worst, tr);
+  }
+  if (st) {
+    *st = nq_solve_stats{};
+    st->iters = iters;
+    st->rel_res_sq = worst;
+    st->converged = iters < niter;
+    st->seconds = T.stop();
+  }
+  if (iters >= niter) return fail(NQ_ENOCONV, "exit without converging");
+  return NQ_OK;
+}
+
 }  // namespace nq
 
 using namespace nq;
@@ -1192,6 +1311,45 @@ int nq_tm_solve_eoprec_multi(nq_ctx* c, const nq_gauge* g, double kappa, const d
   if (st)
     for (int i = 0; i < nrhs; i++) st[i] = local[i];
   return rc == NQ_ENOCONV ? fail(NQ_ENOCONV, "exit without converging") : NQ_OK;
+}
+
+int nq_tmQ2_cgm(nq_ctx* c, const nq_gauge* g, double kappa, int nmass, const double* masses, int niter,
+                const double* residues, nq_field** sols, const nq_field* src, nq_solve_stats* st, double* true_res)
+{
+  if (!c || !masses || !residues || !sols) return fail(NQ_EINVAL, "null argument");
+  if (nmass < 1 || nmass > kMaxShift) return fail(NQ_EINVAL, "need 1..%d masses", kMaxShift);
+  if (niter <= 0) return fail(NQ_EINVAL, "bad niter");
+  NQ_TRY(check_field(c, src, NQ_SPINCOLOR, NQ_LX, "nq_tmQ2_cgm"));
+  for (int k = 0; k < nmass; k++) {
+    NQ_TRY(check_field(c, sols[k], NQ_SPINCOLOR, NQ_LX, "nq_tmQ2_cgm"));
+    if (sols[k] == src) return fail(NQ_EINVAL, "sol==src");
+  }
+  Guard G(c);
+  return tmQ2_cgm_f64(c, g, kappa, nmass, masses, niter, residues, sols, src, st, true_res);
+}
+
+// inv_tmDQ_cgm (cgm_invert_tmQ2.cpp:55-75): the same on g5 src
+int nq_tmDQ_cgm(nq_ctx* c, const nq_gauge* g, double kappa, int nmass, const double* masses, int niter,
+                const double* residues, nq_field** sols, const nq_field* src, nq_solve_stats* st, double* true_res)
+{
+  if (!c) return fail(NQ_EINVAL, "null ctx");
+  NQ_TRY(check_field(c, src, NQ_SPINCOLOR, NQ_LX, "nq_tmDQ_cgm"));
+  Guard G(c);
+  SpinorView gv;
+  NQ_TRY(alloc_spinor_nr(c, "q2m.g5src", SP_F64, 12, 2, &gv));
+  nq_field g5src;
+  g5src.ctx = c;
+  g5src.kind = NQ_SPINCOLOR;
+  g5src.coverage = NQ_LX;
+  g5src.buf.p = gv.data;
+  double ca[2][2], cb[2][2] = {{0, 0}, {0, 0}};
+  setc(ca, 1.0, -1.0);
+  SpinorView none;
+  int rc = NQ_OK;
+  for (int p = 0; p < 2 && rc == NQ_OK; p++) rc = launch_spin_diag(c, view_of(&g5src, p), ca, view_of(src, p), cb, none);
+  if (rc == NQ_OK) rc = nq_tmQ2_cgm(c, g, kappa, nmass, masses, niter, residues, sols, &g5src, st, true_res);
+  g5src.buf.p = nullptr;
+  return rc;
 }
 
 int nq_tm_kern_square_host(nq_ctx* c, const nq_gauge* g, double kappa, const double* anis, double mu, double* out,

@@ -188,10 +188,10 @@ __global__ void __launch_bounds__(256) k_cgm_update(Geo g, SpinorView rv, Spinor
 enum { FIN_CGM_PAP = 1, FIN_CGM_RFRF = 2 };
 
 // recurrences of cgm_invert_template_threaded.hpp:118-141 (pap) and :183-252 (rfrf)
-__device__ inline void cgm_finalize(CgmState* s, int fin, double total)
+__device__ inline void cgm_finalize(CgmState* s, int fin, double total, double pap_scale)
 {
   if (fin == FIN_CGM_PAP) {
-    const double pap = 0.25 * total;
+    const double pap = pap_scale * total;
     s->pap = pap;
     s->betap = s->betaa;
     s->betaa = -s->rr / pap;
@@ -235,7 +235,8 @@ __device__ inline void cgm_finalize(CgmState* s, int fin, double total)
   }
 }
 
-__global__ void __launch_bounds__(1024) k_cgm_finalize(const double* partial, int64_t nchunks, CgmState* st, int fin)
+__global__ void __launch_bounds__(1024) k_cgm_finalize(const double* partial, int64_t nchunks, CgmState* st, int fin,
+                                                       double pap_scale)
 {
   __shared__ double vl[1024];
   CgmState* s = st + blockIdx.x;
@@ -247,7 +248,7 @@ __global__ void __launch_bounds__(1024) k_cgm_finalize(const double* partial, in
     return;
   }
   const double tot = block_sum_partials(partial + (int64_t)blockIdx.x * nchunks, nchunks, vl);
-  if (threadIdx.x == 0) cgm_finalize(s, fin, tot);
+  if (threadIdx.x == 0) cgm_finalize(s, fin, tot, pap_scale);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -335,6 +336,121 @@ int launch_cgm_flush(nq_ctx* ctx, const SpinorView& x64, const SpinorView& sol, 
   return NQ_OK;
 }
 
+// ------------------------------------------------------------------ generic CGM vectors
+// For operators whose matvec is launched separately (the lx twisted-mass Q^2):
+//   v2: sol_k -= betas_k ps_k (running k) ; r += betaa s ; |r|^2
+//   v3: p = r + alpha p ; ps_k = zfs_k r + alphas_k ps_k (running k)
+// FP64 fields of NC complex components; one source (state st[0]).
+template <int NC>
+__global__ void __launch_bounds__(256) k_cgm_vec2(Geo g, SpinorView s, SpinorView r, SpinorView sol, SpinorView ps,
+                                                  int64_t sstride, const CgmState* st, Shape SH, int64_t nchunks,
+                                                  double* partial)
+{
+  const Map m = map_thread(SH, g.Vh);
+  double red = 0;
+  if (m.active && !st->done) {
+    for (int k = 0; k < st->nshift; k++) {
+      if (!st->run[k]) continue;
+      SpinorView sk = sol, pk = ps;
+      sk.data = (char*)sol.data + k * sstride * 16;
+      pk.data = (char*)ps.data + k * sstride * 16;
+      cx<double> x[NC], q[NC];
+      SpAcc<SP_F64, double, NC, kNT>::load(sk, g.stride, m.e, x);
+      SpAcc<SP_F64, double, NC, kNT>::load(pk, g.stride, m.e, q);
+      const double b = st->betas[k];
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        x[c].re = fma(-b, q[c].re, x[c].re);
+        x[c].im = fma(-b, q[c].im, x[c].im);
+      }
+      SpAcc<SP_F64, double, NC, kNT>::store(sk, g.stride, m.e, x);
+    }
+    cx<double> rv[NC], sv[NC];
+    SpAcc<SP_F64, double, NC>::load(r, g.stride, m.e, rv);
+    SpAcc<SP_F64, double, NC, kNT>::load(s, g.stride, m.e, sv);
+    const double ba = st->betaa;
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      rv[c].re = fma(ba, sv[c].re, rv[c].re);
+      rv[c].im = fma(ba, sv[c].im, rv[c].im);
+      red += rv[c].re * rv[c].re + rv[c].im * rv[c].im;
+    }
+    SpAcc<SP_F64, double, NC>::store(r, g.stride, m.e, rv);
+  }
+  store_partial(red, partial, nchunks, 0, m.chunk);
+}
+
+template <int NC>
+__global__ void __launch_bounds__(256) k_cgm_vec3(Geo g, SpinorView r, SpinorView p, SpinorView ps, int64_t sstride,
+                                                  const CgmState* st, Shape SH)
+{
+  const Map m = map_thread(SH, g.Vh);
+  if (!m.active || st->done) return;
+  cx<double> rv[NC], pv[NC];
+  SpAcc<SP_F64, double, NC>::load(r, g.stride, m.e, rv);
+  SpAcc<SP_F64, double, NC>::load(p, g.stride, m.e, pv);
+  const double a = st->alpha;
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    pv[c].re = fma(a, pv[c].re, rv[c].re);
+    pv[c].im = fma(a, pv[c].im, rv[c].im);
+  }
+  SpAcc<SP_F64, double, NC>::store(p, g.stride, m.e, pv);
+  for (int k = 0; k < st->nshift; k++) {
+    if (!st->run[k]) continue;
+    SpinorView pk = ps;
+    pk.data = (char*)ps.data + k * sstride * 16;
+    cx<double> q[NC];
+    SpAcc<SP_F64, double, NC, kNT>::load(pk, g.stride, m.e, q);
+    const double z = st->zfs[k], al = st->alphas[k];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      q[c].re = fma(al, q[c].re, z * rv[c].re);
+      q[c].im = fma(al, q[c].im, z * rv[c].im);
+    }
+    SpAcc<SP_F64, double, NC, kNT>::store(pk, g.stride, m.e, q);
+  }
+}
+
+int launch_cgm_finalize(nq_ctx* ctx, void* states, int nr, int fin, double pap_scale)
+{
+  const int64_t nchunks = (ctx->geo.Vh + 63) / 64;
+  hipLaunchKernelGGL(k_cgm_finalize, dim3(nr), dim3(1024), 0, ctx->stream, ctx->partial, nchunks, (CgmState*)states,
+                     fin == 1 ? (int)FIN_CGM_PAP : (int)FIN_CGM_RFRF, pap_scale);
+  NQ_HIP(hipGetLastError());
+  return NQ_OK;
+}
+
+int launch_cgm_vec2(nq_ctx* ctx, const SpinorView& s, const SpinorView& r, const SpinorView& sol, const SpinorView& ps,
+                    int64_t sstride, void* states)
+{
+  const Geo& g = ctx->geo;
+  const Launch L = launch_shape(g, 1);
+  if (r.ncomp != 24) return fail(NQ_EINVAL, "cgm vectors: lx spincolor fields expected");
+  hipEvent_t ev = nullptr;
+  prof_begin(ctx, "cgm_vec2", 0.0, &ev);
+  hipLaunchKernelGGL(k_cgm_vec2<24>, dim3((unsigned)L.blocks), dim3(L.threads), 0, ctx->stream, g, s, r, sol, ps, sstride,
+                     (const CgmState*)states, L.s, L.nchunks, ctx->partial);
+  NQ_HIP(hipGetLastError());
+  prof_end(ctx, ev);
+  return launch_cgm_finalize(ctx, states, 1, 2, 1.0);
+}
+
+int launch_cgm_vec3(nq_ctx* ctx, const SpinorView& r, const SpinorView& p, const SpinorView& ps, int64_t sstride,
+                    void* states)
+{
+  const Geo& g = ctx->geo;
+  const Launch L = launch_shape(g, 1);
+  if (r.ncomp != 24) return fail(NQ_EINVAL, "cgm vectors: lx spincolor fields expected");
+  hipEvent_t ev = nullptr;
+  prof_begin(ctx, "cgm_vec3", 0.0, &ev);
+  hipLaunchKernelGGL(k_cgm_vec3<24>, dim3((unsigned)L.blocks), dim3(L.threads), 0, ctx->stream, g, r, p, ps, sstride,
+                     (const CgmState*)states, L.s);
+  NQ_HIP(hipGetLastError());
+  prof_end(ctx, ev);
+  return NQ_OK;
+}
+
 int cgm_state_bytes() { return (int)sizeof(CgmState); }
 
 int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, const SpinorView& t,
@@ -362,7 +478,7 @@ int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, co
   NQ_HIP(hipGetLastError());
   prof_end(ctx, ev);
   hipLaunchKernelGGL(k_cgm_finalize, dim3(nr), dim3(1024), 0, ctx->stream, ctx->partial, L.nchunks, st,
-                     (int)FIN_CGM_PAP);
+                     (int)FIN_CGM_PAP, 0.25);
   NQ_HIP(hipGetLastError());
   // C2
   prof_begin(ctx, "cgm_hop2_update", (8 * lbytes(lk) + 3 * csite(sp)) * nr * (double)g.Vh, &ev);
@@ -375,7 +491,7 @@ int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, co
   NQ_HIP(hipGetLastError());
   prof_end(ctx, ev);
   hipLaunchKernelGGL(k_cgm_finalize, dim3(nr), dim3(1024), 0, ctx->stream, ctx->partial, L.nchunks, st,
-                     (int)FIN_CGM_RFRF);
+                     (int)FIN_CGM_RFRF, 0.25);
   NQ_HIP(hipGetLastError());
   // C3
   prof_begin(ctx, "cgm_update", 4 * csite(sp) * nr * (double)g.Vh, &ev);

@@ -440,12 +440,13 @@ static int launch_rb(nq_ctx* ctx, int RB, int op, const SpinorView& out, const S
 
 // Autotuner of the right-hand sides per wavefront of the batched sloppy hop: on the
 // first batched launch of a (precision, batch size) it times the write-only kernel
-// (no solver state, output into a scratch buffer) with one right-hand side per
-// wavefront (the unbatched kernel, RB = 0, or the blocked one with RB = 1) and with
-// 2 and 4 per wavefront, on the caller's input and links, and keeps the fastest for
-// every mode of that precision and batch size.  All choices run the same per-RHS
-// arithmetic (hop_epilogue is shared, the slot sums run in the same order), so the
-// choice never changes a result.  NQ_TUNE=0 or an open graph capture use RB = 2.
+// (no solver state, output into a scratch buffer) with RB = 1, 2 and 4 right-hand
+// sides per wavefront on the caller's input and links, and keeps the fastest for
+// every mode of that precision and batch size.  All RB build the same per-RHS code
+// (one unrolled copy per right-hand side), so the choice never changes a result; the
+// one-RHS kernel (RB = 0 in launch_rb) is not a candidate: its FP16 code schedules
+// the same arithmetic differently and does not give the same bits.  NQ_TUNE=0 or an
+// open graph capture use RB = 2.
 template <int SP, int LK, int XP>
 static int tuned_rb(nq_ctx* ctx, int op, const SpinorView& in, const LinkView& U, double anis, const HopEpi& epi,
                     int nr, int* rb)
@@ -474,7 +475,7 @@ static int tuned_rb(nq_ctx* ctx, int op, const SpinorView& in, const LinkView& U
   NQ_HIP(hipEventCreate(&b));
   float best = 1e30f;
   int best_rb = 2;
-  for (int cand : {0, 1, 2, 4}) {
+  for (int cand : {1, 2, 4}) {
     if (cand > 1 && cand > nr) continue;
     NQ_TRY((launch_rb<SP, LK, 0, XP>(ctx, cand, op, scratch, in, U, anis, e0, nr)));  // warm-up
     NQ_HIP(hipEventRecord(a, ctx->stream));

@@ -215,3 +215,28 @@ def test_st_rational_sum_closed_form(st_setups, prec):
     nq.st_cgm_rat_sum(ctx, g, 0.75, [a, b], [1 / (b - a), -1 / (b - a)], 20000, 1e-24, chis2, fsrc, prec=prec)
     for s, c1, c2 in zip(srcs, chis, chis2):
         assert rel_l2(c2.download() - c1.download(), 0.75 * s) < 1e-9
+
+
+@pytest.mark.parametrize("g5", [False, True])
+def test_tmQ2_cgm_matches_oracle(g5):
+    """inv_tmQ2_cgm / inv_tmDQ_cgm: multi-mass CG on Q(0)^2 + m_k^2 over lx; every mass
+    reaches its residue in FP64 and matches the oracle's CGM solution."""
+    L = (4, 6, 4, 8)
+    conf = tm_conf(L, 21)
+    ctx = nq.Ctx(L)
+    g = nq.Gauge(ctx, lx=conf)
+    geo = orc.Geom(L)
+    src = nq.source_lx(L, 22, "gauss")
+    masses = [0.01, 0.05, 0.2]
+    res = [1e-24] * 3
+    fs = ctx.field(nq.SPINCOLOR, nq.LX, src)
+    fx = [ctx.field(nq.SPINCOLOR, nq.LX) for _ in masses]
+    st = nq.tmQ2_cgm(ctx, g, 0.16, masses, 20000, res, fx, fs, g5=g5)
+    assert st["rc"] == nq.NQ_OK and max(st["true_res_sq"]) < 2e-24
+    osrc = src.copy()
+    if g5:
+        osrc[:, 2:] *= -1
+    ref, info = geo.tmQ2_cgm(conf, 0.16, masses, 20000, res, osrc)
+    for k in range(3):
+        assert rel_l2(fx[k].download(), ref[k]) < 1e-10, k
+    ctx.close()
