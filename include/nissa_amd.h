@@ -143,6 +143,17 @@ int nq_st_2Doe(nq_ctx* ctx, const nq_gauge* g, nq_field* out_odd, const nq_field
 int nq_st_Deo_half(nq_ctx* ctx, const nq_gauge* g, nq_field* out_evn, const nq_field* in_odd);
 /* apply_stD2ee_m2 (dirac_operator_stD.hpp:130-225): mass2 in - 1/4 Deo Doe in */
 int nq_st_DDee(nq_ctx* ctx, const nq_gauge* g, double mass2, nq_field* out_evn, const nq_field* in_evn);
+/* evn_apply_stD / odd_apply_stD (dag = 0) and their _dag (dag = 1), apply_stD
+   (dirac_operator_stD.hpp:16-41,254-291): D = m + 1/2 hop per parity; a NULL output
+   skips that parity */
+int nq_st_apply_D(nq_ctx* ctx, const nq_gauge* g, double m, int dag, nq_field* out_evn, nq_field* out_odd,
+                  const nq_field* in_evn, const nq_field* in_odd);
+/* inv_evn_stD_cg (sol_odd NULL) / inv_stD_cg (cg_invert_evn_stD.hpp:17-31,
+   cg_invert_stD.hpp:14-88): CG on m^2 - 1/4 Deo Doe for D^+ src, odd part rebuilt,
+   check |D sol - src|^2/|src|^2 (full solve) */
+int nq_st_solve_D(nq_ctx* ctx, const nq_gauge* g, double m, int niter, double residue, nq_field* sol_evn,
+                  nq_field* sol_odd, const nq_field* guess_evn, const nq_field* src_evn,
+                  const nq_field* src_odd, nq_solve_stats* st, double* check_rel_res_sq);
 /* inv_stD2ee_m2_cg (cg_invert_stD2ee_m2.hpp:18-43) */
 int nq_st_cg(nq_ctx* ctx, const nq_gauge* g, double mass2, int niter, double residue,
              nq_field* sol_evn, const nq_field* guess_evn, const nq_field* src_evn, nq_solve_stats* st);

@@ -88,6 +88,8 @@ _D = C.c_double
 _I = C.c_int
 _DP = C.POINTER(C.c_double)
 _SIG = {
+    "nq_st_apply_D": ([_P, _P, _D, _I, _P, _P, _P, _P], _I),
+    "nq_st_solve_D": ([_P, _P, _D, _I, _D, _P, _P, _P, _P, _P, _P, _DP], _I),
     "nq_tmQ2_cgm": ([_P, _P, _D, _I, _DP, _I, _DP, _P, _P, _P, _DP], _I),
     "nq_tmDQ_cgm": ([_P, _P, _D, _I, _DP, _I, _DP, _P, _P, _P, _DP], _I),
     "nq_st_cgm_rat_sum": ([_P, _P, _I, _I, _D, _DP, _DP, _I, _D, _I, _P, _P, _P], _I),
@@ -494,6 +496,21 @@ def st_cgm(ctx, g, shifts, niter, residues, sols, srcs, prec="f64"):
     if rc not in (NQ_OK, NQ_ENOCONV):
         _check(rc)
     return [dict(st[r].as_dict(), rc=rc, true_res_sq=[tr[r * ns + k] for k in range(ns)]) for r in range(nr)]
+
+
+def st_apply_D(ctx, g, m, out_e, out_o, in_e, in_o, dag=False):
+    _check(lib().nq_st_apply_D(ctx.h, g.h, m, int(dag), out_e.h if out_e is not None else None,
+                               out_o.h if out_o is not None else None, in_e.h, in_o.h))
+
+
+def st_solve_D(ctx, g, m, niter, residue, sol_e, sol_o, src_e, src_o, guess=None):
+    st = SolveStats()
+    chk = C.c_double(0)
+    rc = lib().nq_st_solve_D(ctx.h, g.h, m, niter, residue, sol_e.h, sol_o.h if sol_o is not None else None,
+                             guess.h if guess is not None else None, src_e.h, src_o.h, C.byref(st), C.byref(chk))
+    out = _solve(rc, st)
+    out["check_rel_res_sq"] = chk.value
+    return out
 
 
 def tmQ2_cgm(ctx, g, kappa, masses, niter, residues, sols, src, g5=False):

@@ -570,6 +570,91 @@ int nq_st_cgm(nq_ctx* c, const nq_gauge* g, int nrhs, int nshift, const double* 
   return NQ_OK;
 }
 
+// evn_apply_stD / odd_apply_stD (and their _dag, sign = -1) and apply_stD
+// (dirac_operator_stD.hpp:16-41,254-291): D = m + 1/2 hop on each parity
+// (2 * apply_stDeo_half = 1/2 hop on the even sites, 1/2 apply_st2Doe on the odd).
+// A null output skips that parity.
+int nq_st_apply_D(nq_ctx* c, const nq_gauge* g, double m, int dag, nq_field* out_evn, nq_field* out_odd,
+                  const nq_field* in_evn, const nq_field* in_odd)
+{
+  if (!c || !g) return fail(NQ_EINVAL, "null argument");
+  NQ_TRY(check_color(c, in_evn, NQ_EVEN, "nq_st_apply_D"));
+  NQ_TRY(check_color(c, in_odd, NQ_ODD, "nq_st_apply_D"));
+  if (out_evn) NQ_TRY(check_color(c, out_evn, NQ_EVEN, "nq_st_apply_D"));
+  if (out_odd) NQ_TRY(check_color(c, out_odd, NQ_ODD, "nq_st_apply_D"));
+  if (out_evn == in_evn || out_odd == in_odd) return fail(NQ_EINVAL, "out==in!");
+  Guard G(c);
+  LinkView U;
+  NQ_TRY(link_view(g, LK_F64_18, &U));
+  const double h = dag ? -0.5 : 0.5;
+  if (out_evn)
+    NQ_TRY(launch_st_hop(c, NQ_EVEN, view_of(out_evn, 0), view_of(in_odd, 0), U, h, m, nullptr, view_of(in_evn, 0), 0,
+                         1, 1, FIN_NONE, nullptr));
+  if (out_odd)
+    NQ_TRY(launch_st_hop(c, NQ_ODD, view_of(out_odd, 0), view_of(in_evn, 0), U, h, m, nullptr, view_of(in_odd, 0), 0,
+                         1, 1, FIN_NONE, nullptr));
+  NQ_HIP(hipStreamSynchronize(c->stream));
+  return NQ_OK;
+}
+
+// inv_evn_stD_cg / inv_stD_cg (cg_invert_evn_stD.hpp:17-31, cg_invert_stD.hpp:14-88):
+// sol_e = (m^2 - 1/4 Deo Doe)^-1 (D^+ src)_e by CG, then (full solve, sol_odd given)
+// sol_o = (src_o - 1/2 hop_o sol_e) / m and the check |D sol - src|^2 / |src|^2.
+int nq_st_solve_D(nq_ctx* c, const nq_gauge* g, double m, int niter, double residue, nq_field* sol_evn,
+                  nq_field* sol_odd, const nq_field* guess_evn, const nq_field* src_evn, const nq_field* src_odd,
+                  nq_solve_stats* st, double* check_rel_res_sq)
+{
+  if (!c || !g) return fail(NQ_EINVAL, "null argument");
+  if (!(m != 0)) return fail(NQ_EINVAL, "mass must be non-zero");
+  NQ_TRY(check_color(c, src_evn, NQ_EVEN, "nq_st_solve_D"));
+  NQ_TRY(check_color(c, src_odd, NQ_ODD, "nq_st_solve_D"));
+  NQ_TRY(check_color(c, sol_evn, NQ_EVEN, "nq_st_solve_D"));
+  if (sol_odd) NQ_TRY(check_color(c, sol_odd, NQ_ODD, "nq_st_solve_D"));
+  Guard G(c);
+  LinkView U;
+  NQ_TRY(link_view(g, LK_F64_18, &U));
+  // temp = evn_apply_stD_dag(src) = m src_e - 1/2 hop_e src_o
+  nq_field tmp;
+  tmp.ctx = c;
+  tmp.kind = NQ_COLOR;
+  tmp.coverage = NQ_EVEN;
+  SpinorView tv;
+  NQ_TRY(alloc_spinor(c, "stD.tmp", SP_F64, 3, &tv));
+  tmp.buf.p = tv.data;  // borrowed scratch, owned by the context
+  NQ_TRY(launch_st_hop(c, NQ_EVEN, tv, view_of(src_odd, 0), U, -0.5, m, nullptr, view_of(src_evn, 0), 0, 1, 1,
+                       FIN_NONE, nullptr));
+  int rc = nq_st_cg(c, g, m * m, niter, residue, sol_evn, guess_evn, &tmp, st);
+  if (rc != NQ_OK && rc != NQ_ENOCONV) return rc;
+  if (sol_odd) {
+    NQ_TRY(launch_st_hop(c, NQ_ODD, view_of(sol_odd, 0), view_of(sol_evn, 0), U, -0.5 / m, 1.0 / m, nullptr,
+                         view_of(src_odd, 0), 0, 1, 1, FIN_NONE, nullptr));
+    if (check_rel_res_sq) {
+      // |D sol - src|^2 per parity: (m sol_p + 1/2 hop_p sol_!p) - src_p
+      SpinorView none;
+      none.ncomp = 3;
+      SpinorView de, dd;
+      NQ_TRY(alloc_spinor(c, "stD.de", SP_F64, 3, &de));
+      NQ_TRY(alloc_spinor(c, "stD.do", SP_F64, 3, &dd));
+      NQ_TRY(launch_st_hop(c, NQ_EVEN, de, view_of(sol_odd, 0), U, 0.5, m, nullptr, view_of(sol_evn, 0), 0, 1, 1,
+                           FIN_NONE, nullptr));
+      NQ_TRY(launch_st_hop(c, NQ_ODD, dd, view_of(sol_evn, 0), U, 0.5, m, nullptr, view_of(sol_odd, 0), 0, 1, 1,
+                           FIN_NONE, nullptr));
+      std::vector<CgState> z(2);
+      NQ_TRY(put_states(c, 0, 2, z.data()));
+      NQ_TRY(launch_lincomb(c, none, 1.0, de, -1.0, view_of(src_evn, 0), 0, none, FIN_STORE0, c->state));
+      NQ_TRY(launch_lincomb(c, none, 1.0, dd, -1.0, view_of(src_odd, 0), 0, none, FIN_STORE1, c->state));
+      NQ_TRY(launch_lincomb(c, none, 1.0, view_of(src_evn, 0), 0, none, 0, none, FIN_STORE0, c->state + 1));
+      NQ_TRY(launch_lincomb(c, none, 1.0, view_of(src_odd, 0), 0, none, 0, none, FIN_STORE1, c->state + 1));
+      NQ_TRY(get_states(c, 0, 2));
+      *check_rel_res_sq = (c->state_host[0].extra[0] + c->state_host[0].extra[1]) /
+                          (c->state_host[1].extra[0] + c->state_host[1].extra[1]);
+    }
+  }
+  NQ_HIP(hipStreamSynchronize(c->stream));
+  if (rc != NQ_OK) return fail(rc, "exit without converging");
+  return NQ_OK;
+}
+
 int nq_st_cgm_rat_sum(nq_ctx* c, const nq_gauge* g, int nrhs, int degree, double cons, const double* poles,
                       const double* weights, int niter, double residue, int prec, nq_field** chis,
                       const nq_field* const* srcs, nq_solve_stats* st)

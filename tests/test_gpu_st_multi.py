@@ -240,3 +240,55 @@ def test_tmQ2_cgm_matches_oracle(g5):
     for k in range(3):
         assert rel_l2(fx[k].download(), ref[k]) < 1e-10, k
     ctx.close()
+
+
+@pytest.mark.parametrize("L", ST_SHAPES[:2])
+def test_st_apply_D_matches_oracle(st_setups, L):
+    """evn/odd_apply_stD, their _dag and apply_stD (dirac_operator_stD.hpp:16-41,
+    254-291) against the oracle; D^+(m) = -D(-m) gives the dagger reference."""
+    ctx, g, geo, conf, ce, co = st_setups(L)
+    xe = gauss_eo(geo.Vh, 71, ncomp=3)
+    xo = gauss_eo(geo.Vh, 72, ncomp=3)
+    fe, fo = ctx.field(nq.COLOR, nq.EVEN, xe), ctx.field(nq.COLOR, nq.ODD, xo)
+    oe, oo = ctx.field(nq.COLOR, nq.EVEN), ctx.field(nq.COLOR, nq.ODD)
+    m = 0.07
+    re, ro = geo.apply_stD(ce, co, m, xe, xo)
+    nq.st_apply_D(ctx, g, m, oe, oo, fe, fo)
+    assert rel_l2(oe.download(), re) < 1e-14 and rel_l2(oo.download(), ro) < 1e-14
+    de, do = geo.apply_stD(ce, co, -m, xe, xo)
+    nq.st_apply_D(ctx, g, m, oe, oo, fe, fo, dag=True)
+    assert rel_l2(oe.download(), -de) < 1e-14 and rel_l2(oo.download(), -do) < 1e-14
+    # one parity only: the other output is left untouched
+    oo.upload(np.zeros((geo.Vh, 3, 2)))
+    nq.st_apply_D(ctx, g, m, oe, None, fe, fo)
+    assert rel_l2(oe.download(), re) < 1e-14 and not oo.download().any()
+    nq.st_apply_D(ctx, g, m, None, oo, fe, fo, dag=True)
+    assert rel_l2(oo.download(), -do) < 1e-14
+
+
+@pytest.mark.parametrize("L", ST_SHAPES[:2])
+def test_st_solve_D_matches_oracle(st_setups, L):
+    """inv_evn_stD_cg and inv_stD_cg (cg_invert_evn_stD.hpp:17-31,
+    cg_invert_stD.hpp:14-88): the even solve of (m^2 - 1/4 Deo Doe) on D^+ src, the
+    rebuilt odd part, and the full residual |D sol - src|^2/|src|^2."""
+    ctx, g, geo, conf, ce, co = st_setups(L)
+    m = 0.05
+    se = gauss_eo(geo.Vh, 81, ncomp=3)
+    so = gauss_eo(geo.Vh, 82, ncomp=3)
+    de, _ = geo.apply_stD(ce, co, -m, se, so)
+    ref_e, _ = geo.st_cg(ce, co, m * m, 10000, 1e-24, -de)
+    ref_o = (so - 0.5 * geo.st_2Doe(ce, co, ref_e)) / m
+    fe, fo = ctx.field(nq.COLOR, nq.EVEN, se), ctx.field(nq.COLOR, nq.ODD, so)
+    xe, xo = ctx.field(nq.COLOR, nq.EVEN), ctx.field(nq.COLOR, nq.ODD)
+    st = nq.st_solve_D(ctx, g, m, 10000, 1e-24, xe, xo, fe, fo)
+    assert st["converged"] and st["rel_res_sq"] < 2e-24
+    assert rel_l2(xe.download(), ref_e) < 1e-10 and rel_l2(xo.download(), ref_o) < 1e-10
+    assert st["check_rel_res_sq"] < 1e-21
+    # the solution solves D x = src in the oracle's own operator
+    ae, ao = geo.apply_stD(ce, co, m, xe.download(), xo.download())
+    r2 = np.sum((ae - se) ** 2) + np.sum((ao - so) ** 2)
+    assert abs(r2 / (np.sum(se ** 2) + np.sum(so ** 2)) - st["check_rel_res_sq"]) < 1e-3 * st["check_rel_res_sq"] + 1e-30
+    # even-only solve (inv_evn_stD_cg) gives the same even bits
+    ye = ctx.field(nq.COLOR, nq.EVEN)
+    st2 = nq.st_solve_D(ctx, g, m, 10000, 1e-24, ye, None, fe, fo)
+    assert st2["converged"] and np.array_equal(ye.download(), xe.download())

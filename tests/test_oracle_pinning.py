@@ -228,6 +228,30 @@ def test_staggered_antihermiticity():
     assert abs(lhs + 4 * rhs) / abs(lhs) < 1e-13
 
 
+def test_staggered_full_D_schur_and_antihermiticity():
+    """apply_stD (dirac_operator_stD.hpp:16-41,254-291) against the pinned pieces:
+    D(m) on (x_e, 0) is (m x_e, 1/2 2Doe x_e); D^+ D = -D(-m) D(m) restricted to the
+    even sites is stD2ee_m2(m^2) (the operator inv_stD_cg inverts); D - m is
+    anti-hermitian on the full lattice."""
+    L = (4, 6, 4, 8)
+    g = orc.Geom(L)
+    ce, co = g.split(st_conf(L, 37))
+    rng = np.random.default_rng(5)
+    xe, xo = rng.normal(size=(g.Vh, 3, 2)), rng.normal(size=(g.Vh, 3, 2))
+    ye, yo = rng.normal(size=(g.Vh, 3, 2)), rng.normal(size=(g.Vh, 3, 2))
+    z = np.zeros_like(xe)
+    m = 0.3
+    de, do = g.apply_stD(ce, co, m, xe, z)
+    assert rel_l2(de, m * xe) < 1e-15 and rel_l2(do, 0.5 * g.st_2Doe(ce, co, xe)) < 1e-14
+    ee, _ = g.apply_stD(ce, co, -m, de, do)
+    assert rel_l2(-ee, g.stD2ee_m2(ce, co, m * m, xe)) < 1e-13
+    he, ho = g.apply_stD(ce, co, 0.0, xe, xo)
+    ke, ko = g.apply_stD(ce, co, 0.0, ye, yo)
+    lhs = _cdot(ye, he) + _cdot(yo, ho)
+    rhs = _cdot(ke, xe) + _cdot(ko, xo)
+    assert abs(lhs + rhs) / abs(lhs) < 1e-13
+
+
 def _dense(apply, n_in, shape):
     cols = []
     for k in range(n_in):
