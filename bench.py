@@ -43,6 +43,7 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=6, help="oracle CG iterations timed for the CPU baseline")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-stag", action="store_true", help="skip the staggered multi-shift leg")
+    ap.add_argument("--no-24", action="store_true", help="skip the 24^3x48 FP64 Dslash and solve leg")
     ap.add_argument("--serve-requests", type=int, default=60, help="requests in the Poisson stream")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="request arrival rate (1/s)")
     args = ap.parse_args()
@@ -145,11 +146,14 @@ def main():
 
     serve = None
     stag = None
-    if not (args.no_serve and args.no_stag):
+    lat24 = None
+    if not (args.no_serve and args.no_stag and args.no_24):
         for f in (fs, fx, fo, fe):
             f.close()
         g.close()
         ctx.close()
+        if not args.no_24:
+            lat24 = lattice24_bench()
         if not args.no_serve:
             serve = serve_bench(args)
         if not args.no_stag:
@@ -195,16 +199,58 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_of(dom_name, L), "kernel": dom_name,
                      "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": dom["bytes_per_launch"]},
+        "lattice_24c48_fp64": lat24,
         "serve": serve,
         "staggered_cgm": stag,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    if args.no_serve and args.no_stag:
+    if args.no_serve and args.no_stag and args.no_24:
         for f in (fs, fx, fo, fe):
             f.close()
         g.close()
         ctx.close()
+
+
+def lattice24_bench():
+    """The FP64 drop-in at 24^3x48 (kappa 0.16, mu 0.005, same gauge construction,
+    seed 2): 1000 single-parity FP64 hops timed for GB/s, and one FP64 solve of a
+    Gaussian source to nissa residue 1e-24 (|r|/|b| = 1e-12) checked in FP64."""
+    import nissa_amd as nq
+
+    L = (48, 24, 24, 24)
+    kappa, mu, residue = 0.16, 0.005, 1e-24
+    conf = nq.gauge_random_lx(L, 2, 0.3)
+    nq.gauge_phase_lx(L, conf, [1.0, 0, 0, 0])
+    src = nq.source_lx(L, 18, "gauss")
+    ctx = nq.Ctx(L, 0)
+    g = nq.Gauge(ctx, lx=conf)
+    fo = ctx.field(nq.SPINCOLOR, nq.ODD)
+    fe = ctx.field(nq.SPINCOLOR, nq.EVEN)
+    nq.bench_tm_hop(ctx, g, 20, fe, fo, prec="f64")
+    ms_hop, bytes_hop = nq.bench_tm_hop(ctx, g, 1000, fe, fo, prec="f64")
+    fs = ctx.field(nq.SPINCOLOR, nq.LX, src)
+    fx = ctx.field(nq.SPINCOLOR, nq.LX)
+    nq.tm_solve_eoprec(ctx, g, kappa, mu, 100000, residue, fx, fs, prec="f64")
+    ctx.sync()
+    t0 = time.perf_counter()
+    st = nq.tm_solve_eoprec(ctx, g, kappa, mu, 100000, residue, fx, fs, prec="f64")
+    ctx.sync()
+    t = time.perf_counter() - t0
+    for f in (fs, fx, fo, fe):
+        f.close()
+    g.close()
+    ctx.close()
+    gbs = bytes_hop / (ms_hop * 1e-3) / 1e9
+    out = {"lattice": list(L), "kappa": kappa, "mu": mu,
+           "dslash_fp64_us": round(ms_hop * 1e3, 2), "dslash_fp64_GBs": round(gbs, 1),
+           "dslash_fp64_frac_of_peak": round(gbs / HBM_PEAK_GBS, 4), "dslash_applies": 1000,
+           "fp64_solve_ms": round(t * 1e3, 3), "fp64_solve_iters": st["iters"],
+           "fp64_solve_true_rel_res_sq": st["check_rel_res_sq"], "residue": residue,
+           "ok": bool(st["rc"] == 0 and st["check_rel_res_sq"] < 2 * residue)}
+    log(f"[bench] 24^3x48: Dslash FP64 {ms_hop*1e3:.1f} us ({gbs:.0f} GB/s), FP64 solve to 1e-24 "
+        f"{t*1e3:.1f} ms in {st['iters']} iterations, check {st['check_rel_res_sq']:.2e}")
+    return out
 
 
 def stag_bench(args):

@@ -213,16 +213,22 @@ int nq_ctx_create(int T, int LX, int LY, int LZ, int device, nq_ctx** out)
   g.T = T; g.LX = LX; g.LY = LY; g.LZ = LZ;
   g.Vh = (int64_t)T * LX * LY * LZ / 2;
   g.stride = (g.Vh + 63) / 64 * 64;
-  // t tile of the device site order (tuning knob NQ_TTILE in {2,4,8}; default 8)
-  int tb = 8;
-  if (const char* e = getenv("NQ_TTILE")) tb = atoi(e);
-  if (tb != 2 && tb != 4 && tb != 8) tb = 8;
-  while (T % tb) tb /= 2;
+  // t tile of the device site order: the largest even divisor of T up to 64, or
+  // up to NQ_TTILE when set.  Whole t rows in one tile keep every t neighbour in
+  // the lines the wavefront already reads: a t tile boundary sends a quarter of
+  // the lanes' t-neighbour loads to lines of a far tile (of another XCD), which
+  // costs ~10% of the hop at 32^3x64 with tiles of 8.
+  int tbmax = 64;
+  if (const char* e = getenv("NQ_TTILE")) tbmax = atoi(e);
+  if (tbmax < 2) tbmax = 64;
+  int tb = 2;
+  for (int d = 2; d <= std::min(T, tbmax); d += 2)
+    if (T % d == 0) tb = d;
   g.TB = tb;
-  g.logTB = g.TB == 8 ? 3 : (g.TB == 4 ? 2 : 1);
   g.NTH = T / g.TB;
-  // x/y patch of the device site order (tuning knob NQ_XYTILE "TXxTY", powers of two)
-  int tx = 1, ty = 1;
+  // x/y patch of the device site order (tuning knob NQ_XYTILE "TXxTY", powers of two;
+  // default 4x1: x neighbours of a patch are 1/4 of a y row of sites apart)
+  int tx = 4, ty = 1;
   if (const char* e = getenv("NQ_XYTILE")) sscanf(e, "%dx%d", &tx, &ty);
   auto pow2 = [](int v) { return v >= 1 && v <= 16 && (v & (v - 1)) == 0; };
   if (!pow2(tx)) tx = 1;

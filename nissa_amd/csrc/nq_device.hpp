@@ -55,8 +55,8 @@ __device__ __forceinline__ Site site_of(const Geo& g, int par, uint32_t e)
 {
   Site s;
   const uint32_t q = 2u * e;
-  const int tl0 = (int)(q & (uint32_t)(g.TB - 1));
-  uint32_t r = q >> g.logTB;
+  uint32_t r = q / (uint32_t)g.TB;
+  const int tl0 = (int)(q - r * (uint32_t)g.TB);
   const uint32_t r1 = r / (uint32_t)g.LZ;
   s.z = (int)(r - r1 * (uint32_t)g.LZ);
   s.yi = (int)(r1 & (uint32_t)(g.TY - 1));

@@ -50,7 +50,7 @@ constexpr int lk_eb(int lk) { return (lk == LK_F64_18 || lk == LK_F64_12) ? 16 :
 // close in processing order, so they are still in L2 when they are needed.
 struct Geo {
   int T, LX, LY, LZ;
-  int TB, logTB, NTH;  // t tile, log2, number of tiles
+  int TB, NTH;         // t tile (even, divides T), number of tiles
   int TX, logTX, NXT;  // x tile, log2, number of tiles
   int TY, logTY, NYT;  // y tile, log2, number of tiles
   int64_t Vh;          // sites per parity
@@ -61,8 +61,8 @@ struct Geo {
 // rank of lattice site (t, x, y, z) in the device order (both parities)
 __host__ __device__ inline int64_t device_rank(const Geo& g, int t, int x, int y, int z)
 {
-  return (int64_t)(t >> g.logTB) * g.Sth + (int64_t)(x >> g.logTX) * g.Sxt + (int64_t)(y >> g.logTY) * g.Syt +
-         (int64_t)(x & (g.TX - 1)) * g.Sxi + (int64_t)(y & (g.TY - 1)) * g.Syi + (int64_t)z * g.Sz + (t & (g.TB - 1));
+  return (int64_t)(t / g.TB) * g.Sth + (int64_t)(x >> g.logTX) * g.Sxt + (int64_t)(y >> g.logTY) * g.Syt +
+         (int64_t)(x & (g.TX - 1)) * g.Sxi + (int64_t)(y & (g.TY - 1)) * g.Syi + (int64_t)z * g.Sz + (t % g.TB);
 }
 
 // One parity of a spinor (ncomp=12) or colour (ncomp=3) field in SoA layout:
