@@ -124,9 +124,11 @@ __global__ void __launch_bounds__(256) k_dot(BlasArgs A, SpinorView a, SpinorVie
 }
 
 // p = r + beta p, beta from state[r]; skipped once that right-hand side is done or
-// while it waits for a reliable update
+// while it waits for a reliable update.  With x given, first x += alpha p (the CG
+// solution update moved out of the matvec tail, so that p is read once per
+// iteration)
 template <int NC>
-__global__ void __launch_bounds__(256) k_pupdate(BlasArgs A, SpinorView p, SpinorView r)
+__global__ void __launch_bounds__(256) k_pupdate(BlasArgs A, SpinorView p, SpinorView r, SpinorView x)
 {
   const Map m = map_thread(A.s, A.g.Vh);
   const CgState* s = A.st + m.r;
@@ -135,6 +137,17 @@ __global__ void __launch_bounds__(256) k_pupdate(BlasArgs A, SpinorView p, Spino
   cx<double> pv[NC], rv[NC];
   load_any<NC>(p, A.g.stride, m.e, pv, m.r);
   load_any<NC>(r, A.g.stride, m.e, rv, m.r);
+  if (x.data) {
+    const double alpha = s->alpha;
+    cx<double> xv[NC];
+    load_any<NC>(x, A.g.stride, m.e, xv, m.r);
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      xv[k].re = fma(alpha, pv[k].re, xv[k].re);
+      xv[k].im = fma(alpha, pv[k].im, xv[k].im);
+    }
+    store_any<NC>(x, A.g.stride, m.e, xv, m.r);
+  }
 #pragma unroll
   for (int k = 0; k < NC; k++) {
     pv[k].re = fma(beta, pv[k].re, rv[k].re);
@@ -173,7 +186,7 @@ __global__ void __launch_bounds__(256) k_cg_update(BlasArgs A, SpinorView x, Spi
 
 // x64 += xs ; xs = 0 (reliable update), only for right-hand sides flagged reliable
 template <int NC>
-__global__ void __launch_bounds__(256) k_reliable_accum(BlasArgs A, SpinorView x64, SpinorView xs)
+__global__ void __launch_bounds__(256) k_reliable_accum(BlasArgs A, SpinorView x64, SpinorView xs, SpinorView p)
 {
   const Map m = map_thread(A.s, A.g.Vh);
   const CgState* s = A.st + m.r;
@@ -181,6 +194,17 @@ __global__ void __launch_bounds__(256) k_reliable_accum(BlasArgs A, SpinorView x
   cx<double> a[NC], b[NC];
   load_any<NC>(x64, A.g.stride, m.e, a, m.r);
   load_any<NC>(xs, A.g.stride, m.e, b, m.r);
+  if (p.data) {
+    // the last iteration's alpha p, not yet folded into xs
+    const double alpha = s->alpha;
+    cx<double> pv[NC];
+    load_any<NC>(p, A.g.stride, m.e, pv, m.r);
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      b[k].re = fma(alpha, pv[k].re, b[k].re);
+      b[k].im = fma(alpha, pv[k].im, b[k].im);
+    }
+  }
 #pragma unroll
   for (int k = 0; k < NC; k++) { a[k] = a[k] + b[k]; b[k] = mk(0.0, 0.0); }
   store_any<NC>(x64, A.g.stride, m.e, a, m.r);
@@ -358,17 +382,21 @@ int launch_dot(nq_ctx* ctx, const SpinorView& a, const SpinorView& b, int fin, C
   return NQ_OK;
 }
 
-int launch_cg_pupdate(nq_ctx* ctx, const SpinorView& p, const SpinorView& r, CgState* st, int nr, int pred)
+int launch_cg_pupdate(nq_ctx* ctx, const SpinorView& p, const SpinorView& r, CgState* st, int nr, int pred,
+                      const SpinorView& x)
 {
   BlasArgs A;
   Launch L;
   NQ_TRY(make_args(ctx, nr, st, FIN_NONE, pred == 2 ? 2 : 1, &A, &L));
+  if (x.data && x.ncomp != p.ncomp) return fail(NQ_EINVAL, "pupdate: component mismatch");
   hipEvent_t ev = nullptr;
-  prof_begin(ctx, "blas_pupdate", nr * (2 * site_bytes(p) + site_bytes(r)) * (double)ctx->geo.Vh, &ev);
+  const double xb = x.data ? 2.0 * site_bytes(x) : 0.0;
+  prof_begin(ctx, x.data ? "blas_xpupdate" : "blas_pupdate",
+             nr * (2 * site_bytes(p) + site_bytes(r) + xb) * (double)ctx->geo.Vh, &ev);
   const dim3 grid((unsigned)L.blocks), blk(L.threads);
-  if (p.ncomp == 24) hipLaunchKernelGGL(k_pupdate<24>, grid, blk, 0, ctx->stream, A, p, r);
-  else if (p.ncomp == 12) hipLaunchKernelGGL(k_pupdate<12>, grid, blk, 0, ctx->stream, A, p, r);
-  else hipLaunchKernelGGL(k_pupdate<3>, grid, blk, 0, ctx->stream, A, p, r);
+  if (p.ncomp == 24) hipLaunchKernelGGL(k_pupdate<24>, grid, blk, 0, ctx->stream, A, p, r, x);
+  else if (p.ncomp == 12) hipLaunchKernelGGL(k_pupdate<12>, grid, blk, 0, ctx->stream, A, p, r, x);
+  else hipLaunchKernelGGL(k_pupdate<3>, grid, blk, 0, ctx->stream, A, p, r, x);
   NQ_HIP(hipGetLastError());
   prof_end(ctx, ev);
   return NQ_OK;
@@ -405,16 +433,18 @@ int launch_clear_reliable(nq_ctx* ctx, CgState* st, int nr)
   return NQ_OK;
 }
 
-int launch_reliable_accum(nq_ctx* ctx, const SpinorView& x64, const SpinorView& xs, CgState* st, int nr)
+int launch_reliable_accum(nq_ctx* ctx, const SpinorView& x64, const SpinorView& xs, CgState* st, int nr,
+                          const SpinorView& p)
 {
   BlasArgs A;
   Launch L;
   NQ_TRY(make_args(ctx, nr, st, FIN_NONE, 2, &A, &L));
   hipEvent_t ev = nullptr;
-  prof_begin(ctx, "blas_reliable_accum", nr * 2 * (site_bytes(x64) + site_bytes(xs)) * (double)ctx->geo.Vh, &ev);
+  prof_begin(ctx, "blas_reliable_accum",
+             nr * (2 * (site_bytes(x64) + site_bytes(xs)) + (p.data ? site_bytes(p) : 0)) * (double)ctx->geo.Vh, &ev);
   const dim3 grid((unsigned)L.blocks), blk(L.threads);
-  if (x64.ncomp == 12) hipLaunchKernelGGL(k_reliable_accum<12>, grid, blk, 0, ctx->stream, A, x64, xs);
-  else hipLaunchKernelGGL(k_reliable_accum<3>, grid, blk, 0, ctx->stream, A, x64, xs);
+  if (x64.ncomp == 12) hipLaunchKernelGGL(k_reliable_accum<12>, grid, blk, 0, ctx->stream, A, x64, xs, p);
+  else hipLaunchKernelGGL(k_reliable_accum<3>, grid, blk, 0, ctx->stream, A, x64, xs, p);
   NQ_HIP(hipGetLastError());
   prof_end(ctx, ev);
   return NQ_OK;

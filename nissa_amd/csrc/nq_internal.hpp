@@ -308,14 +308,16 @@ int launch_dot(nq_ctx* ctx, const SpinorView& a, const SpinorView& b, int fin, C
                int nr = 1, int pred = 0);
 // p = r + beta p; pred 1: skip done or reliable-pending right-hand sides; 2: only the
 // reliable-pending ones (after FIN_RR_TRUE_HOLD)
-int launch_cg_pupdate(nq_ctx* ctx, const SpinorView& p, const SpinorView& r, CgState* st, int nr = 1, int pred = 1);
+int launch_cg_pupdate(nq_ctx* ctx, const SpinorView& p, const SpinorView& r, CgState* st, int nr = 1, int pred = 1,
+                      const SpinorView& x = SpinorView());
 // state[i].reliable = 0 for i < nr
 int launch_clear_reliable(nq_ctx* ctx, CgState* st, int nr);
 // x += alpha p ; r -= alpha s ; |r|^2 -> fin  (alpha from state, skipped when done or
 // while a reliable update is due)
 int launch_cg_update(nq_ctx* ctx, const SpinorView& x, const SpinorView& p, const SpinorView& r,
                      const SpinorView& s, CgState* st, int fin, int nr = 1);
-int launch_reliable_accum(nq_ctx* ctx, const SpinorView& x64, const SpinorView& xs, CgState* st, int nr = 1);
+int launch_reliable_accum(nq_ctx* ctx, const SpinorView& x64, const SpinorView& xs, CgState* st, int nr = 1,
+                          const SpinorView& p = SpinorView());
 int launch_convert(nq_ctx* ctx, const SpinorView& out, const SpinorView& in, int nr = 1,
                    CgState* st = nullptr, int pred = 0);
 int launch_zero(nq_ctx* ctx, const SpinorView& v, int nr = 1);

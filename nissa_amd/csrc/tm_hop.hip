@@ -157,22 +157,28 @@ __device__ __forceinline__ double hop_epilogue(const Geo& g, const HopEpi& epi, 
 #pragma unroll
     for (int k = 0; k < 12; k++) nrm += (double)o[k].re * (double)o[k].re + (double)o[k].im * (double)o[k].im;
   } else if constexpr (MODE == 2) {
-    // CG tail: s = o; x += alpha p ; r -= alpha s ; |r|^2
+    // CG tail: s = o; x += alpha p (when x is given) ; r -= alpha s ; |r|^2
     const RX alpha = (RX)st->alpha;
-    cx<RX> x[12], rv[12];
-    cx<R> p[12];
-    SpAcc<XP, RX, 12, kNT>::load(epi.x, g.stride, e, x, r);
-    SpAcc<SP, R, 12, kNT>::load(epi.p, g.stride, e, p, r);
+    if (epi.x.data) {
+      cx<RX> x[12];
+      cx<R> p[12];
+      SpAcc<XP, RX, 12, kNT>::load(epi.x, g.stride, e, x, r);
+      SpAcc<SP, R, 12, kNT>::load(epi.p, g.stride, e, p, r);
+#pragma unroll
+      for (int k = 0; k < 12; k++) {
+        x[k].re = fma(alpha, (RX)p[k].re, x[k].re);
+        x[k].im = fma(alpha, (RX)p[k].im, x[k].im);
+      }
+      SpAcc<XP, RX, 12, kNT>::store(epi.x, g.stride, e, x, r);
+    }
+    cx<RX> rv[12];
     SpAcc<XP, RX, 12, kNT>::load(epi.r, g.stride, e, rv, r);
 #pragma unroll
     for (int k = 0; k < 12; k++) {
-      x[k].re = fma(alpha, (RX)p[k].re, x[k].re);
-      x[k].im = fma(alpha, (RX)p[k].im, x[k].im);
       rv[k].re = fma(-alpha, (RX)o[k].re, rv[k].re);
       rv[k].im = fma(-alpha, (RX)o[k].im, rv[k].im);
       nrm += (double)rv[k].re * (double)rv[k].re + (double)rv[k].im * (double)rv[k].im;
     }
-    SpAcc<XP, RX, 12, kNT>::store(epi.x, g.stride, e, x, r);
     SpAcc<XP, RX, 12, kNT>::store(epi.r, g.stride, e, rv, r);
   } else {
     // residual tail: r64 = o ; rs = sloppy(o) ; |o|^2
@@ -381,7 +387,7 @@ static double lk_bytes(int lk)
 
 static int xp_of(const HopEpi& epi, int sp)
 {
-  if (epi.mode == 2) return epi.x.prec;
+  if (epi.mode == 2) return epi.r.prec;
   if (epi.mode == 3) return epi.rs.data ? epi.rs.prec : sp;
   return sp;
 }
@@ -396,7 +402,7 @@ double tm_hop_bytes(int sp, int lk, const HopEpi& epi, int ncomp)
   if (epi.mode == 0 || (epi.mode == 1 && epi.write)) b += s;
   if (epi.mode == 2) {
     const double sx = sp_site_bytes(xp_of(epi, sp), ncomp);
-    b += 4 * sx + s;  // x,r read+write, p read
+    b += epi.x.data ? 4 * sx + s : 2 * sx;  // r (and x) read+write, p read
   }
   if (epi.mode == 3) {
     b += s;
@@ -555,9 +561,10 @@ int launch_tm_hop(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& 
     return fail(NQ_EINVAL, "tm hop: output precision differs from input");
   if (epi.aux.data && epi.aux.prec != sp) return fail(NQ_EINVAL, "tm hop: aux precision mismatch");
   if (epi.aux2.data && epi.aux2.prec != sp) return fail(NQ_EINVAL, "tm hop: aux2 precision mismatch");
-  if (epi.mode == 2 && (epi.p.prec != sp || epi.x.prec != epi.r.prec)) return fail(NQ_EINVAL, "tm hop: CG tail precision mismatch");
+  if (epi.mode == 2 && (epi.p.prec != sp || (epi.x.data && epi.x.prec != epi.r.prec)))
+    return fail(NQ_EINVAL, "tm hop: CG tail precision mismatch");
   if (sp == SP_F64 && lk == LK_F64_18) {
-    if (epi.mode == 2 && epi.x.prec != SP_F64) return fail(NQ_EINVAL, "FP64 CG tail needs FP64 x,r");
+    if (epi.mode == 2 && epi.r.prec != SP_F64) return fail(NQ_EINVAL, "FP64 CG tail needs FP64 x,r");
     NQ_MODES(SP_F64, LK_F64_18, SP_F64, "f64")
     if (epi.mode == 3) {
       if (epi.r.prec != SP_F64) return fail(NQ_EINVAL, "residual tail needs FP64 r");
@@ -566,7 +573,7 @@ int launch_tm_hop(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& 
       if (epi.rs.prec == SP_F16) return go<SP_F64, LK_F64_18, 3, SP_F16>(ctx, op, out, in, U, anis, epi, "tm_hop_res_f64_s16");
     }
   } else if (sp == SP_F64 && lk == LK_F64_12) {
-    if (epi.mode == 2 && epi.x.prec != SP_F64) return fail(NQ_EINVAL, "FP64 CG tail needs FP64 x,r");
+    if (epi.mode == 2 && epi.r.prec != SP_F64) return fail(NQ_EINVAL, "FP64 CG tail needs FP64 x,r");
     NQ_MODES(SP_F64, LK_F64_12, SP_F64, "f64_l64c")
     if (epi.mode == 3) {
       if (epi.r.prec != SP_F64) return fail(NQ_EINVAL, "residual tail needs FP64 r");
@@ -575,13 +582,13 @@ int launch_tm_hop(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& 
       if (epi.rs.prec == SP_F16) return go<SP_F64, LK_F64_12, 3, SP_F16>(ctx, op, out, in, U, anis, epi, "tm_hop_res_f64_l64c_s16");
     }
   } else if (sp == SP_F32 && lk == LK_F32_18) {
-    if (epi.mode == 2 && epi.x.prec != SP_F32) return fail(NQ_EINVAL, "FP32 CG tail needs FP32 x,r");
+    if (epi.mode == 2 && epi.r.prec != SP_F32) return fail(NQ_EINVAL, "FP32 CG tail needs FP32 x,r");
     NQ_MODES(SP_F32, LK_F32_18, SP_F32, "f32")
   } else if (sp == SP_F16 && lk == LK_F32_12) {
-    if (epi.mode == 2 && epi.x.prec != SP_F32) return fail(NQ_EINVAL, "FP16 CG tail needs FP32 x,r");
+    if (epi.mode == 2 && epi.r.prec != SP_F32) return fail(NQ_EINVAL, "FP16 CG tail needs FP32 x,r");
     NQ_MODES(SP_F16, LK_F32_12, SP_F32, "f16_l32c")
   } else if (sp == SP_F16 && lk == LK_F16_12) {
-    if (epi.mode == 2 && epi.x.prec != SP_F32) return fail(NQ_EINVAL, "FP16 CG tail needs FP32 x,r");
+    if (epi.mode == 2 && epi.r.prec != SP_F32) return fail(NQ_EINVAL, "FP16 CG tail needs FP32 x,r");
     NQ_MODES(SP_F16, LK_F16_12, SP_F32, "f16_l16c")
   }
   return fail(NQ_EINVAL, "tm hop: unsupported precision combination (spinor %d, links %d, mode %d)", sp, lk, epi.mode);
