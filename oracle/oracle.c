@@ -17,6 +17,12 @@
  *                      CGM shift == single-shift CG                            pinned
  *   cg128              residual in __float128 below 1e-28                      pinned
  *   plaquette          L4T8conf fixture value 0.6350055722288719              pinned
+ *   clover term        constant abelian fields in each plane against
+ *                      -cSW/4 sum_{mu<nu} gamma_mu gamma_nu F_munu built from
+ *                      nissa's gamma matrices; hermitian blocks; gauge covariance pinned
+ *   tmclov Dee/inv/K   U=1 reduces to the tm operators; Dee^-1 Dee = 1;
+ *                      K(-mu) = K(mu)^+ identity; dense Schur complement of
+ *                      apply_tmclovQ on 2^4 x 4; eo solve solves the lx system  pinned
  */
 #include "oracle.h"
 
@@ -769,4 +775,356 @@ void or_color_rotate_lx(const or_geom* g, int nspin, double* psi, const double* 
       }
       memcpy(v, o, sizeof o);
     }
+}
+
+/* ===================================================================== twisted clover
+ * clover_term (clover_term.hpp:88-96): chromo_operator (clover_term.cpp:58-104) from the
+ * four leaves (topological_charge.hpp:139-190), times -(cSW + 1e-16)/4
+ * (chromo_operator_include_cSW, clover_term.hpp:70-83).  Per lx site: 4 su3
+ * [G, H, I, J] = [iA, B + iC, iD, E + iF] with A = P3 - P2, B = P4 + P1, C = P5 - P0,
+ * D = P3 + P2, E = P4 - P1, F = P5 + P0 and P_i = (Q_i - Q_i^+)/4, Q the sum of the four
+ * leaves of plane i (edge order 01,02,03,12,13,23, init.cpp:705-713).  Output [V][4][3][3][2].
+ */
+static void su3_dag_mul(double* c, const double* a, const double* b) /* c = a^+ b */
+{
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double re = 0, im = 0;
+      for (int k = 0; k < 3; k++) {
+        const double ar = a[(k * 3 + i) * 2], ai = -a[(k * 3 + i) * 2 + 1];
+        const double br = b[(k * 3 + j) * 2], bi = b[(k * 3 + j) * 2 + 1];
+        re += ar * br - ai * bi;
+        im += ar * bi + ai * br;
+      }
+      c[(i * 3 + j) * 2] = re;
+      c[(i * 3 + j) * 2 + 1] = im;
+    }
+}
+
+void or_clover_term_lx(const or_geom* g, double* cl, const double* conf, double csw)
+{
+  const double f = -(csw + 1e-16) / 4;
+#pragma omp parallel for schedule(static)
+  for (int64_t X = 0; X < g->V; X++) {
+    double P[6][18];
+    int iedge = 0;
+    for (int mu = 0; mu < 4; mu++) {
+      const int64_t A = g->up[X * 4 + mu], D = g->dw[X * 4 + mu];
+      for (int nu = mu + 1; nu < 4; nu++, iedge++) {
+        const int64_t B = g->up[X * 4 + nu], F = g->dw[X * 4 + nu];
+        const int64_t Cs = g->up[D * 4 + nu], E = g->dw[D * 4 + nu], G = g->dw[A * 4 + nu];
+#define LK(s, d) (conf + ((s) * 4 + (d)) * 18)
+        double t1[18], t2[18], Q[18], L[18];
+        /* leaf 1: U_mu(X) U_nu(A) U_mu(B)^+ U_nu(X)^+ */
+        su3_mul(t1, LK(X, mu), LK(A, nu));
+        su3_mul_dag(t2, t1, LK(B, mu));
+        su3_mul_dag(Q, t2, LK(X, nu));
+        /* leaf 2: U_nu(X) U_mu(C)^+ U_nu(D)^+ U_mu(D) */
+        su3_mul_dag(t1, LK(X, nu), LK(Cs, mu));
+        su3_mul_dag(t2, t1, LK(D, nu));
+        su3_mul(L, t2, LK(D, mu));
+        for (int k = 0; k < 18; k++) Q[k] += L[k];
+        /* leaf 3: U_mu(D)^+ U_nu(E)^+ U_mu(E) U_nu(F) */
+        /* U_mu(D)^+ U_nu(E)^+ = (U_nu(E) U_mu(D))^+ */
+        double t0[18];
+        su3_mul(t0, LK(E, nu), LK(D, mu));
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) {
+            t1[(i * 3 + j) * 2] = t0[(j * 3 + i) * 2];
+            t1[(i * 3 + j) * 2 + 1] = -t0[(j * 3 + i) * 2 + 1];
+          }
+        su3_mul(t2, t1, LK(E, mu));
+        su3_mul(L, t2, LK(F, nu));
+        for (int k = 0; k < 18; k++) Q[k] += L[k];
+        /* leaf 4: U_nu(F)^+ U_mu(F) U_nu(G) U_mu(X)^+ */
+        su3_dag_mul(t1, LK(F, nu), LK(F, mu));
+        su3_mul(t2, t1, LK(G, nu));
+        su3_mul_dag(L, t2, LK(X, mu));
+        for (int k = 0; k < 18; k++) Q[k] += L[k];
+#undef LK
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) {
+            P[iedge][(i * 3 + j) * 2] = (Q[(i * 3 + j) * 2] - Q[(j * 3 + i) * 2]) / 4;
+            P[iedge][(i * 3 + j) * 2 + 1] = (Q[(i * 3 + j) * 2 + 1] + Q[(j * 3 + i) * 2 + 1]) / 4;
+          }
+      }
+    }
+    double* o = cl + X * 72;
+    for (int k = 0; k < 9; k++) {
+      const int r = 2 * k, i = 2 * k + 1;
+      const double Ar = P[3][r] - P[2][r], Ai = P[3][i] - P[2][i];
+      const double Br = P[4][r] + P[1][r], Bi = P[4][i] + P[1][i];
+      const double Cr = P[5][r] - P[0][r], Ci = P[5][i] - P[0][i];
+      const double Dr = P[3][r] + P[2][r], Di = P[3][i] + P[2][i];
+      const double Er = P[4][r] - P[1][r], Ei = P[4][i] - P[1][i];
+      const double Fr = P[5][r] + P[0][r], Fi = P[5][i] + P[0][i];
+      /* G = iA, H = B + iC, I = iD, J = E + iF */
+      o[0 * 18 + r] = -Ai * f; o[0 * 18 + i] = Ar * f;
+      o[1 * 18 + r] = (Br - Ci) * f; o[1 * 18 + i] = (Bi + Cr) * f;
+      o[2 * 18 + r] = -Di * f; o[2 * 18 + i] = Dr * f;
+      o[3 * 18 + r] = (Er - Fi) * f; o[3 * 18 + i] = (Ei + Fr) * f;
+    }
+  }
+}
+
+/* 3x3 complex matrix (row-major [3][3][2]) times colour, optionally daggered, accumulated
+   with sign s into o */
+static inline void cl_mv(d_cx o[3], const double* M, int dag, double s, const d_cx v[3])
+{
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) {
+      const double mr = dag ? M[(b * 3 + a) * 2] : M[(a * 3 + b) * 2];
+      const double mi = dag ? -M[(b * 3 + a) * 2 + 1] : M[(a * 3 + b) * 2 + 1];
+      o[a].re += s * (mr * v[b].re - mi * v[b].im);
+      o[a].im += s * (mr * v[b].im + mi * v[b].re);
+    }
+}
+
+/* apply_point_twisted_clover_term_to_halfspincolor (clover_term.hpp:114-137) on the
+   spin pair (off, off+1) with the blocks c0 = Cl[off], c1 = Cl[off+1]:
+   o0 = z i0 + c0 i0 + c1^+ i1 ; o1 = z i1 + c1 i0 - c0 i1, z = 1/(2 kappa) + i mass */
+static void cl_block(double* out, const double* cl, double kappa, double mass, int off, const double* in)
+{
+  d_cx i0[3], i1[3], o0[3], o1[3];
+  col_load(i0, in + off * 6);
+  col_load(i1, in + (off + 1) * 6);
+  const double zr = 1 / (2 * kappa), zi = mass;
+  for (int c = 0; c < 3; c++) {
+    o0[c].re = zr * i0[c].re - zi * i0[c].im; o0[c].im = zr * i0[c].im + zi * i0[c].re;
+    o1[c].re = zr * i1[c].re - zi * i1[c].im; o1[c].im = zr * i1[c].im + zi * i1[c].re;
+  }
+  cl_mv(o0, cl + off * 18, 0, +1, i0);
+  cl_mv(o0, cl + (off + 1) * 18, 1, +1, i1);
+  cl_mv(o1, cl + (off + 1) * 18, 0, +1, i0);
+  cl_mv(o1, cl + off * 18, 0, -1, i1);
+  for (int c = 0; c < 3; c++) {
+    out[off * 6 + 2 * c] = o0[c].re; out[off * 6 + 2 * c + 1] = o0[c].im;
+    out[(off + 1) * 6 + 2 * c] = o1[c].re; out[(off + 1) * 6 + 2 * c + 1] = o1[c].im;
+  }
+}
+
+/* tmclovDee_or_oo_eos (dirac_operator_tmclovD_eoprec_portable.cpp:19-42): upper spins with
+   +mu, lower with -mu, mu -> -mu for the dagger; cl: [Vh][4][3][3][2] of that parity */
+void or_tmclovDee(const or_geom* g, double* out, const double* cl, double kappa, double mu, int dag,
+                  const double* in)
+{
+  if (dag) mu = -mu;
+#pragma omp parallel for schedule(static)
+  for (int64_t X = 0; X < g->Vh; X++) {
+    cl_block(out + X * 24, cl + X * 72, kappa, +mu, 0, in + X * 24);
+    cl_block(out + X * 24, cl + X * 72, kappa, -mu, 2, in + X * 24);
+  }
+}
+
+/* invert_point_twisted_clover_term (clover_term.hpp:160-247): for each chirality block
+   hl the inverse of z + [[C(2hl), C(2hl+1)^+], [C(2hl+1), -C(2hl)]], z = 1/(2 kappa) + i mass
+   (the same +mass for both blocks), laid out as inv[hl][id][ic][x_id][x_ic] (row (id,ic),
+   column (x_id,x_ic)).  The reference solves each column by CG on the squared block to a
+   relative residue of 1e-32; here Gauss-Jordan elimination with partial pivoting gives the
+   same matrix to rounding.  inv: [Vh][2][2][3][2][3][2]. */
+void or_tmclov_invert(const or_geom* g, double* inv, const double* cl, double kappa, double mass)
+{
+#pragma omp parallel for schedule(static)
+  for (int64_t X = 0; X < g->Vh; X++)
+    for (int hl = 0; hl < 2; hl++) {
+      /* build the 6x6 block column by column from unit vectors */
+      double M[6][6][2], Iv[6][6][2];
+      for (int col = 0; col < 6; col++) {
+        double e[24] = {0}, o[24];
+        e[(2 * hl + col / 3) * 6 + (col % 3) * 2] = 1;
+        cl_block(o, cl + X * 72, kappa, mass, 2 * hl, e);
+        for (int row = 0; row < 6; row++) {
+          M[row][col][0] = o[(2 * hl + row / 3) * 6 + (row % 3) * 2];
+          M[row][col][1] = o[(2 * hl + row / 3) * 6 + (row % 3) * 2 + 1];
+          Iv[row][col][0] = row == col;
+          Iv[row][col][1] = 0;
+        }
+      }
+      for (int c = 0; c < 6; c++) {
+        int p = c;
+        double best = -1;
+        for (int r = c; r < 6; r++) {
+          const double a = M[r][c][0] * M[r][c][0] + M[r][c][1] * M[r][c][1];
+          if (a > best) { best = a; p = r; }
+        }
+        if (p != c)
+          for (int k = 0; k < 6; k++)
+            for (int ri = 0; ri < 2; ri++) {
+              double t = M[c][k][ri]; M[c][k][ri] = M[p][k][ri]; M[p][k][ri] = t;
+              t = Iv[c][k][ri]; Iv[c][k][ri] = Iv[p][k][ri]; Iv[p][k][ri] = t;
+            }
+        /* scale row c by 1/M[c][c] */
+        const double dr = M[c][c][0], di = M[c][c][1], dd = dr * dr + di * di;
+        const double qr = dr / dd, qi = -di / dd;
+        for (int k = 0; k < 6; k++) {
+          double a = M[c][k][0], b = M[c][k][1];
+          M[c][k][0] = a * qr - b * qi; M[c][k][1] = a * qi + b * qr;
+          a = Iv[c][k][0]; b = Iv[c][k][1];
+          Iv[c][k][0] = a * qr - b * qi; Iv[c][k][1] = a * qi + b * qr;
+        }
+        for (int r = 0; r < 6; r++) {
+          if (r == c) continue;
+          const double fr = M[r][c][0], fi = M[r][c][1];
+          for (int k = 0; k < 6; k++) {
+            M[r][k][0] -= fr * M[c][k][0] - fi * M[c][k][1];
+            M[r][k][1] -= fr * M[c][k][1] + fi * M[c][k][0];
+            Iv[r][k][0] -= fr * Iv[c][k][0] - fi * Iv[c][k][1];
+            Iv[r][k][1] -= fr * Iv[c][k][1] + fi * Iv[c][k][0];
+          }
+        }
+      }
+      double* o = inv + (X * 2 + hl) * 72;
+      for (int row = 0; row < 6; row++)
+        for (int col = 0; col < 6; col++) {
+          o[(row * 6 + col) * 2] = Iv[row][col][0];
+          o[(row * 6 + col) * 2 + 1] = Iv[row][col][1];
+        }
+    }
+}
+
+/* inv_tmclovDee_or_oo_eos (dirac_operator_tmclovD_eoprec.hpp:22-43): upper spins times
+   inv[0], lower times inv[1]^+; for the dagger lower times inv[1], upper times inv[0]^+ */
+void or_inv_tmclovDee(const or_geom* g, double* out, const double* inv, int dag, const double* in)
+{
+#pragma omp parallel for schedule(static)
+  for (int64_t X = 0; X < g->Vh; X++)
+    for (int hl = 0; hl < 2; hl++) {
+      const int use_dag = dag ? (hl == 0) : (hl == 1);
+      const double* M = inv + (X * 2 + hl) * 72;
+      const double* v = in + X * 24 + hl * 12;
+      double* o = out + X * 24 + hl * 12;
+      for (int r = 0; r < 6; r++) {
+        double re = 0, im = 0;
+        for (int c = 0; c < 6; c++) {
+          const double mr = use_dag ? M[(c * 6 + r) * 2] : M[(r * 6 + c) * 2];
+          const double mi = use_dag ? -M[(c * 6 + r) * 2 + 1] : M[(r * 6 + c) * 2 + 1];
+          re += mr * v[2 * c] - mi * v[2 * c + 1];
+          im += mr * v[2 * c + 1] + mi * v[2 * c];
+        }
+        o[2 * r] = re;
+        o[2 * r + 1] = im;
+      }
+    }
+}
+
+/* tmclovDkern_eoprec_eos (dirac_operator_tmclovD_eoprec_portable.cpp:45-70):
+   out = g5 (Doo(dag) in - 1/4 H_oe Dee^-1(dag) H_eo in) */
+void or_tmclovDkern(const or_geom* g, double* out, double* temp, const double* ce, const double* co,
+                    double kappa, const double* anis, const double* cl_odd, const double* inv_evn, int dag,
+                    double mu, const double* in)
+{
+  d_hop(g, 0, out, ce, co, anis, in);
+  or_inv_tmclovDee(g, temp, inv_evn, dag, out);
+  d_hop(g, 1, out, ce, co, anis, temp);
+  or_tmclovDee(g, temp, cl_odd, kappa, mu, dag, in);
+  d_put_tog(g, out, temp);
+}
+
+/* tmclovDkern_eoprec_square_eos (..._portable.cpp:73-86): K(dag) then K */
+void or_tmclovDkern_square(const or_geom* g, double* out, double* t1, double* t2, const double* ce,
+                           const double* co, double kappa, const double* anis, const double* cl_odd,
+                           const double* inv_evn, double mu, const double* in)
+{
+  or_tmclovDkern(g, t1, t2, ce, co, kappa, anis, cl_odd, inv_evn, 1, mu, in);
+  or_tmclovDkern(g, out, t2, ce, co, kappa, anis, cl_odd, inv_evn, 0, mu, t1);
+}
+
+/* apply_tmclovQ (dirac_operator_tmclovQ.cpp:14-190) = apply_tmQ + g5 (chromo(Cl) in),
+   the chromo part as unsafe_apply_point_chromo_operator_to_spincolor
+   (clover_term.hpp:14-31): the twisted clover blocks without the diagonal */
+void or_apply_tmclovQ(const or_geom* g, double* out, const double* conf, double kappa, const double* anis,
+                      const double* cl_lx, double mu, const double* in)
+{
+  or_apply_tmQ(g, out, conf, kappa, anis, mu, in);
+#pragma omp parallel for schedule(static)
+  for (int64_t X = 0; X < g->V; X++) {
+    double c[24] = {0};
+    const double* v = in + X * 24;
+    for (int off = 0; off < 4; off += 2) {
+      d_cx i0[3], i1[3], o0[3] = {{0, 0}}, o1[3] = {{0, 0}};
+      col_load(i0, v + off * 6);
+      col_load(i1, v + (off + 1) * 6);
+      cl_mv(o0, cl_lx + X * 72 + off * 18, 0, +1, i0);
+      cl_mv(o0, cl_lx + X * 72 + (off + 1) * 18, 1, +1, i1);
+      cl_mv(o1, cl_lx + X * 72 + (off + 1) * 18, 0, +1, i0);
+      cl_mv(o1, cl_lx + X * 72 + off * 18, 0, -1, i1);
+      for (int k = 0; k < 3; k++) {
+        c[off * 6 + 2 * k] = o0[k].re; c[off * 6 + 2 * k + 1] = o0[k].im;
+        c[(off + 1) * 6 + 2 * k] = o1[k].re; c[(off + 1) * 6 + 2 * k + 1] = o1[k].im;
+      }
+    }
+    for (int i = 0; i < 24; i++) out[X * 24 + i] += i < 12 ? c[i] : -c[i];
+  }
+}
+
+typedef struct {
+  const or_geom* g;
+  const double *ce, *co, *anis, *cl_odd, *inv_evn;
+  double kappa, mu;
+  double *t1, *t2;
+} clov_ctx;
+
+static void op_clov_kernsq(void* v, double* out, const double* in)
+{
+  clov_ctx* c = (clov_ctx*)v;
+  or_tmclovDkern_square(c->g, out, c->t1, c->t2, c->ce, c->co, c->kappa, c->anis, c->cl_odd, c->inv_evn, c->mu, in);
+}
+
+/* inv_tmclovD_cg_eoprec_native + the residual check of inv_tmclovD_cg_eoprec
+   (cg_invert_tmclovD_eoprec.cpp:56-128,179-232) */
+int or_tmclov_solve_eoprec(const or_geom* g, double* sol_lx, const double* conf_lx, double kappa,
+                           const double* anis, double csw, double mu, int niter, double residue,
+                           const double* source_lx, or_cg_stats* st, double* check)
+{
+  const int64_t V = g->V, Vh = g->Vh, n = Vh * 24;
+  double* cl = (double*)malloc(sizeof(double) * V * 72);
+  double* cle = (double*)malloc(sizeof(double) * Vh * 72);
+  double* clo = (double*)malloc(sizeof(double) * Vh * 72);
+  double* inv = (double*)malloc(sizeof(double) * Vh * 144);
+  double* ce = (double*)malloc(sizeof(double) * Vh * 72);
+  double* co = (double*)malloc(sizeof(double) * Vh * 72);
+  double* se = (double*)malloc(sizeof(double) * n);
+  double* so = (double*)malloc(sizeof(double) * n);
+  double* xe = (double*)malloc(sizeof(double) * n);
+  double* xo = (double*)malloc(sizeof(double) * n);
+  double* et = (double*)malloc(sizeof(double) * n);
+  double* phi = (double*)malloc(sizeof(double) * n);
+  double* ot = (double*)malloc(sizeof(double) * n);
+  or_clover_term_lx(g, cl, conf_lx, csw);
+  or_split_lx(g, 72, cl, cle, clo);
+  or_tmclov_invert(g, inv, cle, kappa, mu);
+  or_split_lx(g, 72, conf_lx, ce, co);
+  or_split_lx(g, 24, source_lx, se, so);
+  /* eq. 8a */
+  or_inv_tmclovDee(g, et, inv, 0, se);
+  /* eq. 8b: phi = g5 (eta_o + 1/2 H_oe et) */
+  d_hop(g, 1, phi, ce, co, anis, et);
+  for (int64_t i = 0; i < n; i++)
+    phi[i] = ((i % 24) < 12) ? so[i] + phi[i] * 0.5 : -so[i] - phi[i] * 0.5;
+  /* eq. 9 */
+  clov_ctx c = {g, ce, co, anis, clo, inv, kappa, mu, NULL, NULL};
+  c.t1 = (double*)malloc(sizeof(double) * n);
+  c.t2 = (double*)malloc(sizeof(double) * n);
+  int rc = cg_generic(Vh, 24, ot, NULL, op_clov_kernsq, &c, niter, residue, phi, st);
+  /* eq. 10 */
+  or_tmclovDkern(g, xo, xe, ce, co, kappa, anis, clo, inv, 1, mu, ot);
+  /* eq. 11 */
+  d_hop(g, 0, et, ce, co, anis, xo);
+  for (int64_t i = 0; i < n; i++) et[i] = se[i] + et[i] * 0.5;
+  or_inv_tmclovDee(g, xe, inv, 0, et);
+  or_paste_lx(g, 24, sol_lx, xe, xo);
+  if (check) {
+    double* rv = (double*)malloc(sizeof(double) * V * 24);
+    or_apply_tmclovQ(g, rv, conf_lx, kappa, anis, cl, mu, sol_lx);
+    for (int64_t i = 0; i < V * 24; i++) {
+      const double gq = ((i % 24) < 12) ? rv[i] : -rv[i];
+      rv[i] = gq - source_lx[i];
+    }
+    *check = or_scalar_prod(V, 24, rv, rv) / or_scalar_prod(V, 24, source_lx, source_lx);
+    free(rv);
+  }
+  free(c.t1); free(c.t2);
+  free(cl); free(cle); free(clo); free(inv); free(ce); free(co); free(se); free(so); free(xe); free(xo);
+  free(et); free(phi); free(ot);
+  return rc;
 }

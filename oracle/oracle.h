@@ -136,6 +136,25 @@ void or_tmDkern_square_q(const or_geom* g, double* out, const double* conf_evn,
 /* Average plaquette of an lx conf (nissa global_plaquette_lx_conf normalisation:
    Re tr / 3 averaged over the 6 planes and all sites). */
 double or_plaquette_lx(const or_geom* g, const double* conf_lx);
+
+/* twisted clover (clover_term.hpp/.cpp, dirac_operator_tmclovD_eoprec*, dirac_operator_tmclovQ.cpp,
+   cg_invert_tmclovD_eoprec.cpp) */
+void or_clover_term_lx(const or_geom* g, double* cl, const double* conf_lx, double csw);
+void or_tmclovDee(const or_geom* g, double* out, const double* cl, double kappa, double mu, int dag,
+                  const double* in);
+void or_tmclov_invert(const or_geom* g, double* inv, const double* cl, double kappa, double mass);
+void or_inv_tmclovDee(const or_geom* g, double* out, const double* inv, int dag, const double* in);
+void or_tmclovDkern(const or_geom* g, double* out, double* temp, const double* ce, const double* co,
+                    double kappa, const double* anis, const double* cl_odd, const double* inv_evn, int dag,
+                    double mu, const double* in);
+void or_tmclovDkern_square(const or_geom* g, double* out, double* t1, double* t2, const double* ce,
+                           const double* co, double kappa, const double* anis, const double* cl_odd,
+                           const double* inv_evn, double mu, const double* in);
+void or_apply_tmclovQ(const or_geom* g, double* out, const double* conf_lx, double kappa, const double* anis,
+                      const double* cl_lx, double mu, const double* in);
+int or_tmclov_solve_eoprec(const or_geom* g, double* sol_lx, const double* conf_lx, double kappa,
+                           const double* anis, double csw, double mu, int niter, double residue,
+                           const double* source_lx, or_cg_stats* st, double* check);
 /* Random site-local gauge rotation: conf' = O(x) U_mu(x) O(x+mu)^+, psi' = O psi */
 void or_gauge_transform_lx(const or_geom* g, double* conf_lx, const double* omega /*V su3*/);
 void or_color_rotate_lx(const or_geom* g, int nspin, double* psi, const double* omega);

@@ -48,6 +48,14 @@ def lib():
             "or_tm_put_together_g5": ([_P, _DP, _DP], None),
             "or_tmDkern": ([_P, _DP, _DP, _DP, _DP, _D, _DP, _D, _DP], None),
             "or_tmDkern_square": ([_P, _DP, _DP, _DP, _DP, _DP, _D, _DP, _D, _DP], None),
+            "or_clover_term_lx": ([_P, _DP, _DP, _D], None),
+            "or_tmclovDee": ([_P, _DP, _DP, _D, _D, _I, _DP], None),
+            "or_tmclov_invert": ([_P, _DP, _DP, _D, _D], None),
+            "or_inv_tmclovDee": ([_P, _DP, _DP, _I, _DP], None),
+            "or_tmclovDkern": ([_P, _DP, _DP, _DP, _DP, _D, _DP, _DP, _DP, _I, _D, _DP], None),
+            "or_tmclovDkern_square": ([_P, _DP, _DP, _DP, _DP, _DP, _D, _DP, _DP, _DP, _D, _DP], None),
+            "or_apply_tmclovQ": ([_P, _DP, _DP, _D, _DP, _DP, _D, _DP], None),
+            "or_tmclov_solve_eoprec": ([_P, _DP, _DP, _D, _DP, _D, _D, _I, _D, _DP, C.POINTER(CgStats), _DP], _I),
             "or_apply_tmQ": ([_P, _DP, _DP, _D, _DP, _D, _DP], None),
             "or_apply_tmQ2": ([_P, _DP, _DP, _D, _DP, _D, _DP], None),
             "or_st_2Doe": ([_P, _DP, _DP, _DP, _DP], None),
@@ -162,6 +170,62 @@ class Geom:
         lib().or_tmDkern_square_q(self.h, _dp(out), _dp(ce), _dp(co), kappa, _anis(anis), mu,
                                   _dp(np.ascontiguousarray(inp)))
         return out
+
+    # -- twisted clover
+    def clover_term(self, conf_lx, csw):
+        """clover_term(Cl, cSW, conf): [V][4][3][3][2] (G, H, I, J)."""
+        cl = np.empty((self.V, 4, 3, 3, 2))
+        lib().or_clover_term_lx(self.h, _dp(cl), _dp(np.ascontiguousarray(conf_lx)), csw)
+        return cl
+
+    def tmclovDee(self, cl_par, kappa, mu, inp, dag=False):
+        out = np.empty((self.Vh, 4, 3, 2))
+        lib().or_tmclovDee(self.h, _dp(out), _dp(np.ascontiguousarray(cl_par)), kappa, mu, int(dag),
+                           _dp(np.ascontiguousarray(inp)))
+        return out
+
+    def tmclov_invert(self, cl_par, kappa, mu):
+        inv = np.empty((self.Vh, 2, 6, 6, 2))
+        lib().or_tmclov_invert(self.h, _dp(inv), _dp(np.ascontiguousarray(cl_par)), kappa, mu)
+        return inv
+
+    def inv_tmclovDee(self, inv, inp, dag=False):
+        out = np.empty((self.Vh, 4, 3, 2))
+        lib().or_inv_tmclovDee(self.h, _dp(out), _dp(np.ascontiguousarray(inv)), int(dag),
+                               _dp(np.ascontiguousarray(inp)))
+        return out
+
+    def tmclovDkern(self, ce, co, kappa, cl_odd, inv_evn, mu, inp, dag=False, anis=None):
+        out = np.empty((self.Vh, 4, 3, 2))
+        t = np.empty_like(out)
+        lib().or_tmclovDkern(self.h, _dp(out), _dp(t), _dp(ce), _dp(co), kappa, _anis(anis),
+                             _dp(np.ascontiguousarray(cl_odd)), _dp(np.ascontiguousarray(inv_evn)), int(dag), mu,
+                             _dp(np.ascontiguousarray(inp)))
+        return out
+
+    def tmclovDkern_square(self, ce, co, kappa, cl_odd, inv_evn, mu, inp, anis=None):
+        out = np.empty((self.Vh, 4, 3, 2))
+        t1 = np.empty_like(out)
+        t2 = np.empty_like(out)
+        lib().or_tmclovDkern_square(self.h, _dp(out), _dp(t1), _dp(t2), _dp(ce), _dp(co), kappa, _anis(anis),
+                                    _dp(np.ascontiguousarray(cl_odd)), _dp(np.ascontiguousarray(inv_evn)), mu,
+                                    _dp(np.ascontiguousarray(inp)))
+        return out
+
+    def apply_tmclovQ(self, conf_lx, kappa, cl_lx, mu, inp, anis=None):
+        out = np.empty((self.V, 4, 3, 2))
+        lib().or_apply_tmclovQ(self.h, _dp(out), _dp(np.ascontiguousarray(conf_lx)), kappa, _anis(anis),
+                               _dp(np.ascontiguousarray(cl_lx)), mu, _dp(np.ascontiguousarray(inp)))
+        return out
+
+    def tmclov_solve_eoprec(self, conf_lx, kappa, csw, mu, niter, residue, src_lx, anis=None):
+        sol = np.empty((self.V, 4, 3, 2))
+        st = CgStats()
+        chk = C.c_double(0)
+        rc = lib().or_tmclov_solve_eoprec(self.h, _dp(sol), _dp(np.ascontiguousarray(conf_lx)), kappa, _anis(anis),
+                                          csw, mu, niter, residue, _dp(np.ascontiguousarray(src_lx)), C.byref(st),
+                                          C.byref(chk))
+        return sol, {"rc": rc, "iters": st.iters, "rel_res_sq": st.rel_res_sq, "check": chk.value}
 
     def apply_tmQ(self, conf_lx, kappa, mu, inp, anis=None):
         out = np.empty((self.V, 4, 3, 2))
