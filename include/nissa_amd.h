@@ -49,6 +49,7 @@ extern "C" {
 typedef struct nq_ctx nq_ctx;
 typedef struct nq_gauge nq_gauge;
 typedef struct nq_field nq_field;
+typedef struct nq_clover nq_clover;
 
 typedef struct nq_solve_stats {
   int iters;             /* Krylov iterations (sloppy ones for mixed precision)      */
@@ -135,6 +136,41 @@ int nq_tmDQ_cgm(nq_ctx* ctx, const nq_gauge* g, double kappa, int nmass, const d
 int nq_tm_solve_eoprec_multi(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis,
                              int nrhs, const double* mu, int niter, double residue, int prec,
                              nq_field** sol_lx, const nq_field* const* src_lx, nq_solve_stats* st);
+
+/* ------------------------------------------------------------------ twisted clover (device handles)
+   Clover term from a gauge handle, kept on the device (FP64, both parities); the
+   inverse twisted clover blocks are built on first use for each (kappa, mu). */
+/* clover_term(Cl, cSW, conf) (operations/su3_paths/clover_term.hpp:88-96; chromo_operator,
+   clover_term.cpp:56-104): four-leaf clover times -cSW/4 */
+int nq_clover_create(nq_ctx* ctx, const nq_gauge* g, double csw, nq_clover** out);
+/* the clover term as nissa's LxField<clover_term_t> AoS [V][4][3][3][2] (test/debug) */
+int nq_clover_download_lx(const nq_clover* cl, double* cl_lx);
+void nq_clover_destroy(nq_clover* cl);
+/* tmclovDee_or_oo_eos (dirac_operators/tmclovD_eoprec/dirac_operator_tmclovD_eoprec_portable.cpp:19-42):
+   (1/(2 kappa) + i mu g5 + Cl) on the field's parity, mu -> -mu when dag */
+int nq_tmclov_Dee(nq_ctx* ctx, nq_clover* cl, double kappa, double mu, int dag, nq_field* out, const nq_field* in);
+/* invert_twisted_clover_term (clover_term.hpp:160-270) + inv_tmclovDee_or_oo_eos
+   (dirac_operator_tmclovD_eoprec.hpp:22-43) */
+int nq_tmclov_inv_Dee(nq_ctx* ctx, nq_clover* cl, double kappa, double mu, int dag, nq_field* out,
+                      const nq_field* in);
+/* tmclovDkern_eoprec_eos (..._portable.cpp:45-70), odd -> odd */
+int nq_tmclov_kern(nq_ctx* ctx, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis, double mu,
+                   int dag, nq_field* out, const nq_field* in);
+/* tmclovDkern_eoprec_square_eos (..._portable.cpp:73-86): K(dag) then K */
+int nq_tmclov_kern_square(nq_ctx* ctx, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis,
+                          double mu, nq_field* out, const nq_field* in);
+/* apply_tmclovQ (dirac_operators/tmclovQ/dirac_operator_tmclovQ.cpp:14-190), lx fields */
+int nq_tmclov_apply_Q(nq_ctx* ctx, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis, double mu,
+                      nq_field* out, const nq_field* in);
+/* inv_tmclovDkern_eoprec_square_eos_cg (inverters/twisted_clover/cg_invert_tmclovD_eoprec.cpp:33-54) */
+int nq_tmclov_cg_kernsq(nq_ctx* ctx, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis,
+                        double mu, int niter, double residue, nq_field* sol, const nq_field* guess,
+                        const nq_field* src, nq_solve_stats* st);
+/* inv_tmclovD_cg_eoprec (cg_invert_tmclovD_eoprec.cpp:56-232): eqs. 8-11 with the clover
+   blocks, CG on K K^+, FP64 check |D sol - src|^2/|src|^2 */
+int nq_tmclov_solve_eoprec(nq_ctx* ctx, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis,
+                           double mu, int niter, double residue, nq_field* sol, const nq_field* src,
+                           nq_solve_stats* st, double* check);
 
 /* ------------------------------------------------------------------ staggered */
 /* apply_st2Doe (dirac_operator_stD.hpp:47-77) */

@@ -88,6 +88,16 @@ _D = C.c_double
 _I = C.c_int
 _DP = C.POINTER(C.c_double)
 _SIG = {
+    "nq_clover_create": ([_P, _P, _D, C.POINTER(_P)], _I),
+    "nq_clover_download_lx": ([_P, _DP], _I),
+    "nq_clover_destroy": ([_P], None),
+    "nq_tmclov_Dee": ([_P, _P, _D, _D, _I, _P, _P], _I),
+    "nq_tmclov_inv_Dee": ([_P, _P, _D, _D, _I, _P, _P], _I),
+    "nq_tmclov_kern": ([_P, _P, _P, _D, _DP, _D, _I, _P, _P], _I),
+    "nq_tmclov_kern_square": ([_P, _P, _P, _D, _DP, _D, _P, _P], _I),
+    "nq_tmclov_apply_Q": ([_P, _P, _P, _D, _DP, _D, _P, _P], _I),
+    "nq_tmclov_cg_kernsq": ([_P, _P, _P, _D, _DP, _D, _I, _D, _P, _P, _P, _P], _I),
+    "nq_tmclov_solve_eoprec": ([_P, _P, _P, _D, _DP, _D, _I, _D, _P, _P, _P, _DP], _I),
     "nq_st_apply_D": ([_P, _P, _D, _I, _P, _P, _P, _P], _I),
     "nq_st_solve_D": ([_P, _P, _D, _I, _D, _P, _P, _P, _P, _P, _P, _DP], _I),
     "nq_tmQ2_cgm": ([_P, _P, _D, _I, _DP, _I, _DP, _P, _P, _P, _DP], _I),
@@ -353,6 +363,35 @@ class Gauge:
             pass
 
 
+class Clover:
+    """Clover term of a gauge handle (nq_clover_*): clover_term(Cl, cSW, conf) on the device."""
+
+    def __init__(self, ctx, g, csw):
+        self.ctx = ctx
+        self.csw = csw
+        self.h = None
+        h = _P()
+        _check(lib().nq_clover_create(ctx.h, g.h, csw, C.byref(h)))
+        self.h = h
+        ctx._adopt(self)
+
+    def download_lx(self):
+        out = np.empty((self.ctx.V, 4, 3, 3, 2))
+        _check(lib().nq_clover_download_lx(self.h, _dp(out)))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().nq_clover_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Field:
     def __init__(self, ctx, kind, coverage):
         self.ctx, self.kind, self.coverage = ctx, kind, coverage
@@ -419,6 +458,23 @@ def tm_apply_Q2(ctx, g, kappa, mu, out, inp):
     _check(lib().nq_tm_apply_Q2(ctx.h, g.h, kappa, mu, out.h, inp.h))
 
 
+def tmclov_Dee(ctx, cl, kappa, mu, out, inp, dag=False, inverse=False):
+    f = lib().nq_tmclov_inv_Dee if inverse else lib().nq_tmclov_Dee
+    _check(f(ctx.h, cl.h, kappa, mu, int(dag), out.h, inp.h))
+
+
+def tmclov_kern(ctx, g, cl, kappa, mu, out, inp, dag=False, anis=None):
+    _check(lib().nq_tmclov_kern(ctx.h, g.h, cl.h, kappa, _anis(anis), mu, int(dag), out.h, inp.h))
+
+
+def tmclov_kern_square(ctx, g, cl, kappa, mu, out, inp, anis=None):
+    _check(lib().nq_tmclov_kern_square(ctx.h, g.h, cl.h, kappa, _anis(anis), mu, out.h, inp.h))
+
+
+def tmclov_apply_Q(ctx, g, cl, kappa, mu, out, inp, anis=None):
+    _check(lib().nq_tmclov_apply_Q(ctx.h, g.h, cl.h, kappa, _anis(anis), mu, out.h, inp.h))
+
+
 def _solve(rc, st):
     if rc not in (NQ_OK, NQ_ENOCONV):
         _check(rc)
@@ -439,6 +495,23 @@ def tm_solve_eoprec(ctx, g, kappa, mu, niter, residue, sol, src, prec="f64", ani
     chk = C.c_double(0)
     rc = lib().nq_tm_solve_eoprec(ctx.h, g.h, kappa, _anis(anis), mu, niter, residue, PREC[prec], sol.h, src.h,
                                   C.byref(st), C.byref(chk))
+    d = _solve(rc, st)
+    d["check_rel_res_sq"] = chk.value
+    return d
+
+
+def tmclov_cg_kernsq(ctx, g, cl, kappa, mu, niter, residue, sol, src, guess=None, anis=None):
+    st = SolveStats()
+    rc = lib().nq_tmclov_cg_kernsq(ctx.h, g.h, cl.h, kappa, _anis(anis), mu, niter, residue, sol.h,
+                                   guess.h if guess is not None else None, src.h, C.byref(st))
+    return _solve(rc, st)
+
+
+def tmclov_solve_eoprec(ctx, g, cl, kappa, mu, niter, residue, sol, src, anis=None):
+    st = SolveStats()
+    chk = C.c_double(0)
+    rc = lib().nq_tmclov_solve_eoprec(ctx.h, g.h, cl.h, kappa, _anis(anis), mu, niter, residue, sol.h, src.h,
+                                      C.byref(st), C.byref(chk))
     d = _solve(rc, st)
     d["check_rel_res_sq"] = chk.value
     return d

@@ -1,0 +1,357 @@
+// Twisted-clover site-local pieces: the clover term built from the gauge field,
+// the inverse of the twisted clover blocks on one parity, and their application.
+//
+// Clover field of one parity: 36 complex FP64 components per site in the SoA
+// record layout of the spinor fields (component k of device site e at
+// ((cx<double>*)data)[k*stride + e]): the four 3x3 blocks G, H, I, J of nissa's
+// clover_term_t (clover_term.cpp:22-55) row-major, k = 9*b + 3*row + col.
+// The twisted clover operator on a site acts on the chirality halves h = 0
+// (spins 0,1) and h = 1 (spins 2,3) as the 6x6 block
+//   z_h + [[C(2h), C(2h+1)^+], [C(2h+1), -C(2h)]],  z_h = 1/(2 kappa) + i mu_h
+// (apply_point_twisted_clover_term_to_halfspincolor, clover_term.hpp:114-137).
+// Inverse field: 72 complex per site, block h row-major (k = 36 h + 6 row + col,
+// row/col = 3 spin + colour), nissa's inv_clover_term_t[h][id][ic][x_id][x_ic].
+
+#include "nq_device.hpp"
+
+namespace nq {
+
+namespace {
+
+typedef cx<double> C;
+
+__device__ __forceinline__ void m3_mul(C c[9], const C a[9], const C b[9])  // c = a b
+{
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      C s = a[3 * i] * b[j];
+      s = s + a[3 * i + 1] * b[3 + j];
+      s = s + a[3 * i + 2] * b[6 + j];
+      c[3 * i + j] = s;
+    }
+}
+__device__ __forceinline__ void m3_mul_dag(C c[9], const C a[9], const C b[9])  // c = a b^+
+{
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      C s = a[3 * i] * conjg(b[3 * j]);
+      s = s + a[3 * i + 1] * conjg(b[3 * j + 1]);
+      s = s + a[3 * i + 2] * conjg(b[3 * j + 2]);
+      c[3 * i + j] = s;
+    }
+}
+__device__ __forceinline__ void m3_dag_mul(C c[9], const C a[9], const C b[9])  // c = a^+ b
+{
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      C s = conjg(a[i]) * b[j];
+      s = s + conjg(a[3 + i]) * b[3 + j];
+      s = s + conjg(a[6 + i]) * b[6 + j];
+      c[3 * i + j] = s;
+    }
+}
+
+// link U_mu at lattice coordinates (periodic wrap) from the nissa eo AoS copies
+struct Links {
+  const double* aos[2];
+  int L[4];
+  __device__ __forceinline__ void get(int t, int x, int y, int z, int mu, C u[9]) const
+  {
+    t = (t + L[0]) % L[0];
+    x = (x + L[1]) % L[1];
+    y = (y + L[2]) % L[2];
+    z = (z + L[3]) % L[3];
+    const int64_t lx = (((int64_t)t * L[1] + x) * L[2] + y) * L[3] + z;
+    const double* p = aos[(t + x + y + z) & 1] + (lx >> 1) * 72 + mu * 18;
+#pragma unroll
+    for (int k = 0; k < 9; k++) u[k] = mk<double>(p[2 * k], p[2 * k + 1]);
+  }
+};
+
+// clover_term (clover_term.hpp:88-96): the four leaves of each plane
+// (four_leaves_point, topological_charge.hpp:139-190, planes 01,02,03,12,13,23),
+// P = (Q - Q^+)/4, then G = i(P3 - P2), H = (P4 + P1) + i(P5 - P0),
+// I = i(P3 + P2), J = (P4 - P1) + i(P5 + P0) (point_chromo_operator,
+// clover_term.cpp:56-98), times f = -(cSW + 1e-16)/4.
+__global__ void __launch_bounds__(128) k_clover_build(Geo g, Links lk, int par, double f, SpinorView out)
+{
+  const uint32_t e = blockIdx.x * 128 + threadIdx.x;
+  if (e >= (uint32_t)g.Vh) return;
+  const Site s = site_of(g, par, e);
+  const int X[4] = {s.t, s.x, s.y, s.z};
+  C G[9], H[9], I[9], J[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) G[k] = H[k] = I[k] = J[k] = mk<double>(0, 0);
+  int plane = 0;
+  for (int mu = 0; mu < 4; mu++)
+    for (int nu = mu + 1; nu < 4; nu++, plane++) {
+      int A[4] = {X[0], X[1], X[2], X[3]}, B[4] = {X[0], X[1], X[2], X[3]};
+      int D[4] = {X[0], X[1], X[2], X[3]}, F[4] = {X[0], X[1], X[2], X[3]};
+      A[mu] += 1;
+      B[nu] += 1;
+      D[mu] -= 1;
+      F[nu] -= 1;
+      int Cc[4] = {D[0], D[1], D[2], D[3]}, E[4] = {D[0], D[1], D[2], D[3]}, Gs[4] = {A[0], A[1], A[2], A[3]};
+      Cc[nu] += 1;
+      E[nu] -= 1;
+      Gs[nu] -= 1;
+      C u1[9], u2[9], t1[9], t2[9], Q[9], L[9];
+      // leaf 1: U_mu(X) U_nu(A) U_mu(B)^+ U_nu(X)^+
+      lk.get(X[0], X[1], X[2], X[3], mu, u1);
+      lk.get(A[0], A[1], A[2], A[3], nu, u2);
+      m3_mul(t1, u1, u2);
+      lk.get(B[0], B[1], B[2], B[3], mu, u1);
+      m3_mul_dag(t2, t1, u1);
+      lk.get(X[0], X[1], X[2], X[3], nu, u1);
+      m3_mul_dag(Q, t2, u1);
+      // leaf 2: U_nu(X) U_mu(C)^+ U_nu(D)^+ U_mu(D)
+      lk.get(X[0], X[1], X[2], X[3], nu, u1);
+      lk.get(Cc[0], Cc[1], Cc[2], Cc[3], mu, u2);
+      m3_mul_dag(t1, u1, u2);
+      lk.get(D[0], D[1], D[2], D[3], nu, u1);
+      m3_mul_dag(t2, t1, u1);
+      lk.get(D[0], D[1], D[2], D[3], mu, u1);
+      m3_mul(L, t2, u1);
+#pragma unroll
+      for (int k = 0; k < 9; k++) Q[k] = Q[k] + L[k];
+      // leaf 3: U_mu(D)^+ U_nu(E)^+ U_mu(E) U_nu(F) = (U_nu(E) U_mu(D))^+ U_mu(E) U_nu(F)
+      lk.get(E[0], E[1], E[2], E[3], nu, u1);
+      lk.get(D[0], D[1], D[2], D[3], mu, u2);
+      m3_mul(t1, u1, u2);
+      lk.get(E[0], E[1], E[2], E[3], mu, u1);
+      m3_dag_mul(t2, t1, u1);
+      lk.get(F[0], F[1], F[2], F[3], nu, u1);
+      m3_mul(L, t2, u1);
+#pragma unroll
+      for (int k = 0; k < 9; k++) Q[k] = Q[k] + L[k];
+      // leaf 4: U_nu(F)^+ U_mu(F) U_nu(G) U_mu(X)^+
+      lk.get(F[0], F[1], F[2], F[3], nu, u1);
+      lk.get(F[0], F[1], F[2], F[3], mu, u2);
+      m3_dag_mul(t1, u1, u2);
+      lk.get(Gs[0], Gs[1], Gs[2], Gs[3], nu, u1);
+      m3_mul(t2, t1, u1);
+      lk.get(X[0], X[1], X[2], X[3], mu, u1);
+      m3_mul_dag(L, t2, u1);
+#pragma unroll
+      for (int k = 0; k < 9; k++) Q[k] = Q[k] + L[k];
+      // P = (Q - Q^+)/4 and its contribution to G, H, I, J
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+          const C q = Q[3 * a + b], qt = Q[3 * b + a];
+          const C P = mk<double>((q.re - qt.re) / 4, (q.im + qt.im) / 4);
+          const C iP = mk<double>(-P.im, P.re);
+          const int k = 3 * a + b;
+          switch (plane) {
+          case 0: H[k] = H[k] - iP; J[k] = J[k] + iP; break;
+          case 1: H[k] = H[k] + P; J[k] = J[k] - P; break;
+          case 2: G[k] = G[k] - iP; I[k] = I[k] + iP; break;
+          case 3: G[k] = G[k] + iP; I[k] = I[k] + iP; break;
+          case 4: H[k] = H[k] + P; J[k] = J[k] + P; break;
+          default: H[k] = H[k] + iP; J[k] = J[k] + iP; break;
+          }
+        }
+    }
+  C* o = (C*)out.data;
+  const int64_t st = g.stride;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    o[(0 + k) * st + e] = scal(G[k], f);
+    o[(9 + k) * st + e] = scal(H[k], f);
+    o[(18 + k) * st + e] = scal(I[k], f);
+    o[(27 + k) * st + e] = scal(J[k], f);
+  }
+}
+
+// invert_point_twisted_clover_term (clover_term.hpp:160-247): the inverse of
+// each block z + [[C0, C1^+], [C1, -C0]], z = 1/(2 kappa) + i mass (the same
+// mass for both blocks), by Gauss-Jordan elimination with partial pivoting (the
+// reference solves the columns by CG on the squared block; the matrix is the same
+// to rounding).
+__global__ void __launch_bounds__(64) k_clover_invert(Geo g, SpinorView cl, SpinorView inv, double kappa, double mass)
+{
+  const uint32_t e = blockIdx.x * 64 + threadIdx.x;
+  if (e >= (uint32_t)g.Vh) return;
+  const C* c = (const C*)cl.data;
+  C* o = (C*)inv.data;
+  const int64_t st = g.stride;
+  const C z = mk<double>(1 / (2 * kappa), mass);
+  for (int h = 0; h < 2; h++) {
+    C M[6][6], V[6][6];
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) {
+        const C c0 = c[(18 * h + 3 * a + b) * st + e], c1 = c[(18 * h + 9 + 3 * a + b) * st + e];
+        const C c1t = conjg(c[(18 * h + 9 + 3 * b + a) * st + e]);
+        M[a][b] = c0;
+        M[a][3 + b] = c1t;
+        M[3 + a][b] = c1;
+        M[3 + a][3 + b] = mk<double>(-c0.re, -c0.im);
+      }
+    for (int r = 0; r < 6; r++) {
+      M[r][r] = M[r][r] + z;
+      for (int k = 0; k < 6; k++) V[r][k] = mk<double>(r == k ? 1.0 : 0.0, 0.0);
+    }
+    for (int col = 0; col < 6; col++) {
+      int p = col;
+      double best = -1;
+      for (int r = col; r < 6; r++) {
+        const double a = M[r][col].re * M[r][col].re + M[r][col].im * M[r][col].im;
+        if (a > best) { best = a; p = r; }
+      }
+      if (p != col)
+        for (int k = 0; k < 6; k++) {
+          C t = M[col][k]; M[col][k] = M[p][k]; M[p][k] = t;
+          t = V[col][k]; V[col][k] = V[p][k]; V[p][k] = t;
+        }
+      const double dd = M[col][col].re * M[col][col].re + M[col][col].im * M[col][col].im;
+      const C q = mk<double>(M[col][col].re / dd, -M[col][col].im / dd);
+      for (int k = 0; k < 6; k++) {
+        M[col][k] = M[col][k] * q;
+        V[col][k] = V[col][k] * q;
+      }
+      for (int r = 0; r < 6; r++) {
+        if (r == col) continue;
+        const C fct = M[r][col];
+        for (int k = 0; k < 6; k++) {
+          M[r][k] = M[r][k] - fct * M[col][k];
+          V[r][k] = V[r][k] - fct * V[col][k];
+        }
+      }
+    }
+    for (int r = 0; r < 6; r++)
+      for (int k = 0; k < 6; k++) o[(36 * h + 6 * r + k) * st + e] = V[r][k];
+  }
+}
+
+// mode 0 (tmclovDee_or_oo_eos, dirac_operator_tmclovD_eoprec_portable.cpp:19-42):
+//   out_h = (z_h + B_h) in_h, mu_0 = +mu, mu_1 = -mu, mu -> -mu for the dagger;
+// mode 1 (inv_tmclovDee_or_oo_eos, dirac_operator_tmclovD_eoprec.hpp:22-43):
+//   out_h = inv_h in_h, with inv_h^+ for h = 1 (h = 0 for the dagger).
+// out = a_h * (that) + b_h * aux (per chirality coefficients, aux optional).
+struct ClovArgs {
+  double a[2][2], b[2][2];
+};
+
+__global__ void __launch_bounds__(128) k_clover_apply(Geo g, int mode, SpinorView out, SpinorView in, SpinorView blk,
+                                                      SpinorView aux, double kappa, double mu, int dag, ClovArgs k)
+{
+  const uint32_t e = blockIdx.x * 128 + threadIdx.x;
+  if (e >= (uint32_t)g.Vh) return;
+  C v[12], o[12];
+  SpAcc<SP_F64, double, 12>::load(in, g.stride, e, v);
+  const C* B = (const C*)blk.data;
+  const int64_t st = g.stride;
+  for (int h = 0; h < 2; h++) {
+    const C* iv = v + 6 * h;
+    C* ov = o + 6 * h;
+    if (mode == 0) {
+      const double m = (dag ? -mu : mu) * (h == 0 ? 1.0 : -1.0);
+      const C z = mk<double>(1 / (2 * kappa), m);
+      C c0[9], c1[9];
+#pragma unroll
+      for (int q = 0; q < 9; q++) {
+        c0[q] = B[(18 * h + q) * st + e];
+        c1[q] = B[(18 * h + 9 + q) * st + e];
+      }
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        C s0 = z * iv[a], s1 = z * iv[3 + a];
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+          s0 = s0 + c0[3 * a + b] * iv[b] + conjg(c1[3 * b + a]) * iv[3 + b];
+          s1 = s1 + c1[3 * a + b] * iv[b] - c0[3 * a + b] * iv[3 + b];
+        }
+        ov[a] = s0;
+        ov[3 + a] = s1;
+      }
+    } else {
+      const bool hd = dag ? (h == 0) : (h == 1);
+      for (int r = 0; r < 6; r++) {
+        C s = mk<double>(0, 0);
+#pragma unroll
+        for (int c = 0; c < 6; c++) {
+          const C m = hd ? conjg(B[(36 * h + 6 * c + r) * st + e]) : B[(36 * h + 6 * r + c) * st + e];
+          s = s + m * iv[c];
+        }
+        ov[r] = s;
+      }
+    }
+  }
+  if (aux.data) {
+    C w[12];
+    SpAcc<SP_F64, double, 12>::load(aux, g.stride, e, w);
+#pragma unroll
+    for (int q = 0; q < 12; q++) {
+      const int h = q < 6 ? 0 : 1;
+      o[q] = mk<double>(k.a[h][0], k.a[h][1]) * o[q] + mk<double>(k.b[h][0], k.b[h][1]) * w[q];
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 12; q++) {
+      const int h = q < 6 ? 0 : 1;
+      o[q] = mk<double>(k.a[h][0], k.a[h][1]) * o[q];
+    }
+  }
+  SpAcc<SP_F64, double, 12>::store(out, g.stride, e, o);
+}
+
+}  // namespace
+
+int launch_clover_build(nq_ctx* ctx, const double* aos_e, const double* aos_o, int par, double csw, const SpinorView& out)
+{
+  const Geo& g = ctx->geo;
+  Links lk;
+  lk.aos[0] = aos_e;
+  lk.aos[1] = aos_o;
+  lk.L[0] = g.T; lk.L[1] = g.LX; lk.L[2] = g.LY; lk.L[3] = g.LZ;
+  const double f = -(csw + 1e-16) / 4;  // chromo_operator_include_cSW (clover_term.hpp:70-83)
+  hipEvent_t ev = nullptr;
+  prof_begin(ctx, "clover_build", 0.0, &ev);
+  hipLaunchKernelGGL(k_clover_build, dim3((unsigned)((g.Vh + 127) / 128)), dim3(128), 0, ctx->stream, g, lk, par, f, out);
+  NQ_HIP(hipGetLastError());
+  prof_end(ctx, ev);
+  return NQ_OK;
+}
+
+int launch_clover_invert(nq_ctx* ctx, const SpinorView& cl, const SpinorView& inv, double kappa, double mass)
+{
+  const Geo& g = ctx->geo;
+  hipEvent_t ev = nullptr;
+  prof_begin(ctx, "clover_invert", 0.0, &ev);
+  hipLaunchKernelGGL(k_clover_invert, dim3((unsigned)((g.Vh + 63) / 64)), dim3(64), 0, ctx->stream, g, cl, inv, kappa, mass);
+  NQ_HIP(hipGetLastError());
+  prof_end(ctx, ev);
+  return NQ_OK;
+}
+
+int launch_clover_apply(nq_ctx* ctx, int mode, const SpinorView& out, const SpinorView& in, const SpinorView& blk,
+                        const SpinorView& aux, double kappa, double mu, int dag, const double a[2][2],
+                        const double b[2][2])
+{
+  const Geo& g = ctx->geo;
+  if (in.prec != SP_F64 || out.prec != SP_F64 || (aux.data && aux.prec != SP_F64))
+    return fail(NQ_EINVAL, "twisted clover: FP64 fields only");
+  ClovArgs k;
+  for (int h = 0; h < 2; h++)
+    for (int c = 0; c < 2; c++) {
+      k.a[h][c] = a[h][c];
+      k.b[h][c] = b ? b[h][c] : 0.0;
+    }
+  hipEvent_t ev = nullptr;
+  const double bytes = (double)g.Vh * ((mode == 0 ? 36 : 72) * 16.0 + 2 * 192.0 + (aux.data ? 192.0 : 0.0));
+  prof_begin(ctx, mode == 0 ? "clover_dee" : "clover_inv", bytes, &ev);
+  hipLaunchKernelGGL(k_clover_apply, dim3((unsigned)((g.Vh + 127) / 128)), dim3(128), 0, ctx->stream, g, mode, out, in,
+                     blk, aux, kappa, mu, dag, k);
+  NQ_HIP(hipGetLastError());
+  prof_end(ctx, ev);
+  return NQ_OK;
+}
+
+}  // namespace nq

@@ -63,7 +63,9 @@ int launch_soa_to_aos(nq_ctx* ctx, const SpinorView& in, int ncomp, int par, int
 {
   if (in.prec != SP_F64) return fail(NQ_EINVAL, "download source must be FP64");
   const int64_t nb = blocks_for(ctx->geo.Vh);
-  if (ncomp == 12)
+  if (ncomp == 36)
+    hipLaunchKernelGGL(k_soa_to_aos<36>, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, ctx->geo, in, par, host_lx, aos);
+  else if (ncomp == 12)
     hipLaunchKernelGGL(k_soa_to_aos<12>, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, ctx->geo, in, par, host_lx, aos);
   else
     hipLaunchKernelGGL(k_soa_to_aos<3>, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, ctx->geo, in, par, host_lx, aos);

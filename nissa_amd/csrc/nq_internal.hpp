@@ -199,6 +199,18 @@ struct nq_field {
   int ncomp() const { return kind == NQ_SPINCOLOR ? 12 : 3; }
 };
 
+// clover term of a gauge field (FP64, 36 complex per site and parity) and the
+// inverse twisted clover blocks (72 complex per site), cached per parity for the
+// last (kappa, mu)
+struct nq_clover {
+  nq_ctx* ctx = nullptr;
+  double csw = 0;
+  nq::DevBuf cl[2];
+  nq::DevBuf inv[2];
+  double inv_kappa[2] = {0, 0}, inv_mu[2] = {0, 0};
+  int inv_valid[2] = {0, 0};
+};
+
 struct nq_gauge {
   nq_ctx* ctx = nullptr;
   nq::DevBuf aos[2];                 // eo AoS FP64 copy (source for derived layouts)
@@ -289,6 +301,14 @@ int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, co
 // layout conversions
 // host buffer in nissa order (host_lx: lx-indexed buffer, else eo-indexed) <->
 // device parity `par` of a field
+// twisted clover (clover.hip): mode 0 = (z + C) per chirality, 1 = inverse blocks;
+// out = a_h (that) + b_h aux
+int launch_clover_build(nq_ctx* ctx, const double* aos_e, const double* aos_o, int par, double csw,
+                        const SpinorView& out);
+int launch_clover_invert(nq_ctx* ctx, const SpinorView& cl, const SpinorView& inv, double kappa, double mass);
+int launch_clover_apply(nq_ctx* ctx, int mode, const SpinorView& out, const SpinorView& in, const SpinorView& blk,
+                        const SpinorView& aux, double kappa, double mu, int dag, const double a[2][2],
+                        const double b[2][2]);
 int launch_aos_to_soa(nq_ctx* ctx, const double* aos, int ncomp, int par, int host_lx, const SpinorView& out);
 int launch_soa_to_aos(nq_ctx* ctx, const SpinorView& in, int ncomp, int par, int host_lx, double* aos);
 int launch_build_links(nq_ctx* ctx, const double* aos_e, const double* aos_o, int lk, void* out,

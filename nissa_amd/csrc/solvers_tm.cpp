@@ -1445,3 +1445,402 @@ int nq_bench_tm_hop_multi(nq_ctx* c, const nq_gauge* g, int prec, int nrhs, int 
 }
 
 }  // extern "C"
+
+// ==================================================================== twisted clover
+// Twisted-clover Wilson operator (src/dirac_operators/tmclovD_eoprec/,
+// src/dirac_operators/tmclovQ/dirac_operator_tmclovQ.cpp,
+// src/inverters/twisted_clover/cg_invert_tmclovD_eoprec.cpp): the tm hop kernels
+// with the site-local clover blocks of clover.hip in place of the diagonal
+// D_ee/D_oo.  K(dag) = g5 (D_oo(dag) - 1/4 H_oe D_ee(dag)^-1 H_eo): a plain hop,
+// the inverse blocks, the D_oo blocks and a second hop whose epilogue combines
+// them; K K^+ = K(0) K(1) (tmclovDkern_eoprec_square_eos).  FP64 throughout.
+
+namespace nq {
+
+static SpinorView clov_view(const nq_clover* cl, int par)
+{
+  SpinorView v;
+  v.data = cl->cl[par].p;
+  v.prec = SP_F64;
+  v.ncomp = 36;
+  return v;
+}
+
+// inverse twisted clover blocks of parity par for (kappa, mu), computed on first use
+static int clov_inv_view(nq_clover* cl, int par, double kappa, double mu, SpinorView* v)
+{
+  nq_ctx* c = cl->ctx;
+  if (!cl->inv[par].p) NQ_TRY(dev_alloc(cl->inv[par], (size_t)72 * 16 * c->geo.stride));
+  v->data = cl->inv[par].p;
+  v->prec = SP_F64;
+  v->ncomp = 72;
+  if (!cl->inv_valid[par] || cl->inv_kappa[par] != kappa || cl->inv_mu[par] != mu) {
+    NQ_TRY(launch_clover_invert(c, clov_view(cl, par), *v, kappa, mu));
+    cl->inv_valid[par] = 1;
+    cl->inv_kappa[par] = kappa;
+    cl->inv_mu[par] = mu;
+  }
+  return NQ_OK;
+}
+
+struct ClovSetup {
+  TmSetup S;
+  nq_clover* cl;
+  SpinorView inv_e;  // inverse blocks of the even sites for (kappa, mu)
+  double mu;
+  SpinorView te, te2, uo;  // scratch: two even, one odd
+};
+
+static int clov_setup(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis, double mu,
+                      ClovSetup* s)
+{
+  if (!cl || cl->ctx != c) return fail(NQ_EINVAL, "clover handle belongs to another context");
+  NQ_TRY(tm_setup(c, g, LK_F64_18, anis, kappa, &s->S));
+  s->cl = cl;
+  s->mu = mu;
+  NQ_TRY(clov_inv_view(cl, NQ_EVEN, kappa, mu, &s->inv_e));
+  NQ_TRY(alloc_spinor(c, "clov.te", SP_F64, 12, &s->te));
+  NQ_TRY(alloc_spinor(c, "clov.te2", SP_F64, 12, &s->te2));
+  NQ_TRY(alloc_spinor(c, "clov.uo", SP_F64, 12, &s->uo));
+  return NQ_OK;
+}
+
+static const double kOne[2][2] = {{1, 0}, {1, 0}};
+
+// K(dag): out_odd from in_odd; `last` carries the epilogue mode (0 write, 1 norm,
+// 2 CG tail) of the second hop, whose c1/c2/aux are set here
+static int clov_kern(const ClovSetup& s, int dag, const SpinorView& out, const SpinorView& in, HopEpi last)
+{
+  nq_ctx* c = s.S.ctx;
+  SpinorView none;
+  none.ncomp = 12;
+  HopEpi h;
+  NQ_TRY(launch_tm_hop(c, NQ_EVEN, s.te, in, s.S.U, s.S.anis, h));
+  NQ_TRY(launch_clover_apply(c, 1, s.te2, s.te, s.inv_e, none, s.S.kappa, s.mu, dag, kOne, nullptr));
+  NQ_TRY(launch_clover_apply(c, 0, s.uo, in, clov_view(s.cl, NQ_ODD), none, s.S.kappa, s.mu, dag, kOne, nullptr));
+  setc(last.c1, -0.25, 0.25);
+  setc(last.c2, 1.0, -1.0);
+  last.aux = s.uo;
+  return launch_tm_hop(c, NQ_ODD, out, s.te2, s.S.U, s.S.anis, last);
+}
+
+// CG on K K^+ (cg_invert semantics, cg_invert_template_threaded.hpp:19-150); x holds
+// the guess when use_guess
+static int clov_cg(const ClovSetup& s, int niter, double residue, const SpinorView& x, bool use_guess,
+                   const SpinorView& b, nq_solve_stats* out)
+{
+  nq_ctx* ctx = s.S.ctx;
+  SpinorView r, p, t1, sv;
+  NQ_TRY(alloc_spinor(ctx, "clov.r", SP_F64, 12, &r));
+  NQ_TRY(alloc_spinor(ctx, "clov.p", SP_F64, 12, &p));
+  NQ_TRY(alloc_spinor(ctx, "clov.t1", SP_F64, 12, &t1));
+  NQ_TRY(alloc_spinor(ctx, "clov.s", SP_F64, 12, &sv));
+  Timer T(ctx->stream);
+  double b2 = 0;
+  NQ_TRY(reduce_norm2(ctx, b, &b2));
+  if (b2 == 0 || std::isnan(b2)) return fail(NQ_ENAN, "invalid norm: %g", b2);
+  CgState s0 = {};
+  s0.b2 = b2;
+  s0.target = residue * b2;
+  s0.niter = niter;
+  NQ_TRY(put_state(ctx, 0, s0));
+  CgState* st = ctx->state;
+  SpinorView none;
+  none.ncomp = 12;
+  if (use_guess) {
+    NQ_TRY(clov_kern(s, 1, t1, x, HopEpi()));
+    NQ_TRY(clov_kern(s, 0, sv, t1, HopEpi()));
+    NQ_TRY(launch_lincomb(ctx, r, 1.0, b, -1.0, sv, 0, none, FIN_RR_INIT, st));
+  } else {
+    NQ_TRY(launch_zero(ctx, x));
+    NQ_TRY(launch_lincomb(ctx, r, 1.0, b, 0, none, 0, none, FIN_RR_INIT, st));
+  }
+  NQ_TRY(launch_convert(ctx, p, r));
+  NQ_TRY(get_states(ctx, 0, 1));
+  const double rr0 = ctx->state_host[0].rr;
+  int chunk = 8, launched = 0;
+  while (!ctx->state_host[0].done) {
+    for (int i = 0; i < chunk; i++) {
+      // <p, K K^+ p> = |K^+ p|^2 = |K(1) p|^2
+      HopEpi e1;
+      e1.mode = 1;
+      e1.st = st;
+      e1.fin = FIN_PAP;
+      NQ_TRY(clov_kern(s, 1, t1, p, e1));
+      HopEpi e2;
+      e2.mode = 2;
+      e2.st = st;
+      e2.fin = FIN_RR;
+      e2.x = x;
+      e2.p = p;
+      e2.r = r;
+      NQ_TRY(clov_kern(s, 0, none, t1, e2));
+      NQ_TRY(launch_cg_pupdate(ctx, p, r, st));
+    }
+    launched += chunk;
+    NQ_TRY(get_states(ctx, 0, 1));
+    const CgState& h = ctx->state_host[0];
+    chunk = next_chunk(rr0, h.rr, h.target, h.iter, chunk);
+    if (launched > niter + 64) break;
+  }
+  const CgState h = ctx->state_host[0];
+  if (h.nan) return fail(NQ_ENAN, "nanned");
+  // last calculation of the residual (cg_invert_template_threaded.hpp:131-135)
+  NQ_TRY(clov_kern(s, 1, t1, x, HopEpi()));
+  NQ_TRY(clov_kern(s, 0, sv, t1, HopEpi()));
+  NQ_TRY(put_state(ctx, ctx->nstate - 1, CgState{}));
+  NQ_TRY(launch_lincomb(ctx, none, 1.0, b, -1.0, sv, 0, none, FIN_STORE0, scratch(ctx)));
+  NQ_TRY(get_states(ctx, ctx->nstate - 1, 1));
+  const double true_rr = ctx->state_host[ctx->nstate - 1].extra[0];
+  out->iters = h.iter;
+  out->reliable_updates = 0;
+  out->rel_res_sq = true_rr / b2;
+  out->converged = h.iter < niter;
+  out->seconds = T.stop();
+  if (!out->converged) return fail(NQ_ENOCONV, "exit without converging (%d iterations)", h.iter);
+  return NQ_OK;
+}
+
+}  // namespace nq
+
+extern "C" {
+
+int nq_clover_create(nq_ctx* c, const nq_gauge* g, double csw, nq_clover** out)
+{
+  if (!c || !g || !out) return fail(NQ_EINVAL, "null argument");
+  if (g->ctx != c) return fail(NQ_EINVAL, "gauge handle belongs to another context");
+  if (std::isnan(csw)) return fail(NQ_EINVAL, "bad cSW");
+  Guard G(c);
+  auto* cl = new nq_clover;
+  cl->ctx = c;
+  cl->csw = csw;
+  int rc = NQ_OK;
+  for (int p = 0; p < 2 && rc == NQ_OK; p++) rc = dev_alloc(cl->cl[p], (size_t)36 * 16 * c->geo.stride);
+  for (int p = 0; p < 2 && rc == NQ_OK; p++)
+    rc = launch_clover_build(c, (const double*)g->aos[0].p, (const double*)g->aos[1].p, p, csw, clov_view(cl, p));
+  if (rc == NQ_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(NQ_EDEVICE, "clover build failed");
+  if (rc != NQ_OK) {
+    nq_clover_destroy(cl);
+    return rc;
+  }
+  *out = cl;
+  return NQ_OK;
+}
+
+int nq_clover_download_lx(const nq_clover* cl, double* cl_lx)
+{
+  if (!cl || !cl_lx) return fail(NQ_EINVAL, "null argument");
+  nq_ctx* c = cl->ctx;
+  Guard G(c);
+  DevBuf d;
+  NQ_TRY(dev_alloc(d, sizeof(double) * 72 * 2 * c->geo.Vh));
+  int rc = NQ_OK;
+  for (int p = 0; p < 2 && rc == NQ_OK; p++) rc = launch_soa_to_aos(c, clov_view(cl, p), 36, p, 1, (double*)d.p);
+  if (rc == NQ_OK && hipMemcpyAsync(cl_lx, d.p, sizeof(double) * 72 * 2 * c->geo.Vh, hipMemcpyDeviceToHost,
+                                    c->stream) != hipSuccess)
+    rc = fail(NQ_EDEVICE, "download failed");
+  if (rc == NQ_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(NQ_EDEVICE, "download failed");
+  dev_free(d);
+  return rc;
+}
+
+void nq_clover_destroy(nq_clover* cl)
+{
+  if (!cl) return;
+  {
+    Guard G(cl->ctx);
+    for (int p = 0; p < 2; p++) {
+      dev_free(cl->cl[p]);
+      dev_free(cl->inv[p]);
+    }
+  }
+  delete cl;
+}
+
+int nq_tmclov_Dee(nq_ctx* c, nq_clover* cl, double kappa, double mu, int dag, nq_field* out, const nq_field* in)
+{
+  if (!c || !cl || !out || !in) return fail(NQ_EINVAL, "null argument");
+  if (cl->ctx != c) return fail(NQ_EINVAL, "clover handle belongs to another context");
+  if (in->coverage == NQ_LX || out->coverage != in->coverage) return fail(NQ_EINVAL, "nq_tmclov_Dee: parity mismatch");
+  NQ_TRY(check_field(c, in, NQ_SPINCOLOR, in->coverage, "nq_tmclov_Dee"));
+  NQ_TRY(check_field(c, out, NQ_SPINCOLOR, in->coverage, "nq_tmclov_Dee"));
+  if (in == out) return fail(NQ_EINVAL, "in==out!");
+  if (!(kappa != 0)) return fail(NQ_EINVAL, "bad kappa");
+  Guard G(c);
+  SpinorView none;
+  NQ_TRY(launch_clover_apply(c, 0, view_of(out, 0), view_of(in, 0), clov_view(cl, in->coverage), none, kappa, mu, dag,
+                             kOne, nullptr));
+  NQ_HIP(hipStreamSynchronize(c->stream));
+  return NQ_OK;
+}
+
+int nq_tmclov_inv_Dee(nq_ctx* c, nq_clover* cl, double kappa, double mu, int dag, nq_field* out, const nq_field* in)
+{
+  if (!c || !cl || !out || !in) return fail(NQ_EINVAL, "null argument");
+  if (cl->ctx != c) return fail(NQ_EINVAL, "clover handle belongs to another context");
+  if (in->coverage == NQ_LX || out->coverage != in->coverage)
+    return fail(NQ_EINVAL, "nq_tmclov_inv_Dee: parity mismatch");
+  NQ_TRY(check_field(c, in, NQ_SPINCOLOR, in->coverage, "nq_tmclov_inv_Dee"));
+  NQ_TRY(check_field(c, out, NQ_SPINCOLOR, in->coverage, "nq_tmclov_inv_Dee"));
+  if (in == out) return fail(NQ_EINVAL, "in==out!");
+  if (!(kappa != 0)) return fail(NQ_EINVAL, "bad kappa");
+  Guard G(c);
+  SpinorView inv, none;
+  NQ_TRY(clov_inv_view(cl, in->coverage, kappa, mu, &inv));
+  NQ_TRY(launch_clover_apply(c, 1, view_of(out, 0), view_of(in, 0), inv, none, kappa, mu, dag, kOne, nullptr));
+  NQ_HIP(hipStreamSynchronize(c->stream));
+  return NQ_OK;
+}
+
+int nq_tmclov_kern(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis, double mu, int dag,
+                   nq_field* out, const nq_field* in)
+{
+  if (!c) return fail(NQ_EINVAL, "null ctx");
+  NQ_TRY(check_field(c, in, NQ_SPINCOLOR, NQ_ODD, "nq_tmclov_kern"));
+  NQ_TRY(check_field(c, out, NQ_SPINCOLOR, NQ_ODD, "nq_tmclov_kern"));
+  if (in == out) return fail(NQ_EINVAL, "in==out!");
+  Guard G(c);
+  ClovSetup s;
+  NQ_TRY(clov_setup(c, g, cl, kappa, anis, mu, &s));
+  NQ_TRY(clov_kern(s, dag ? 1 : 0, view_of(out, 0), view_of(in, 0), HopEpi()));
+  NQ_HIP(hipStreamSynchronize(c->stream));
+  return NQ_OK;
+}
+
+int nq_tmclov_kern_square(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis, double mu,
+                          nq_field* out, const nq_field* in)
+{
+  if (!c) return fail(NQ_EINVAL, "null ctx");
+  NQ_TRY(check_field(c, in, NQ_SPINCOLOR, NQ_ODD, "nq_tmclov_kern_square"));
+  NQ_TRY(check_field(c, out, NQ_SPINCOLOR, NQ_ODD, "nq_tmclov_kern_square"));
+  if (in == out) return fail(NQ_EINVAL, "in==out!");
+  Guard G(c);
+  ClovSetup s;
+  NQ_TRY(clov_setup(c, g, cl, kappa, anis, mu, &s));
+  SpinorView t1;
+  NQ_TRY(alloc_spinor(c, "clov.t1", SP_F64, 12, &t1));
+  NQ_TRY(clov_kern(s, 1, t1, view_of(in, 0), HopEpi()));
+  NQ_TRY(clov_kern(s, 0, view_of(out, 0), t1, HopEpi()));
+  NQ_HIP(hipStreamSynchronize(c->stream));
+  return NQ_OK;
+}
+
+// Q = g5 D, D = 1/(2 kappa) + i mu g5 + Cl - 1/2 hop on each parity
+int nq_tmclov_apply_Q(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis, double mu,
+                      nq_field* out, const nq_field* in)
+{
+  if (!c || !cl) return fail(NQ_EINVAL, "null argument");
+  if (cl->ctx != c) return fail(NQ_EINVAL, "clover handle belongs to another context");
+  NQ_TRY(check_field(c, in, NQ_SPINCOLOR, NQ_LX, "nq_tmclov_apply_Q"));
+  NQ_TRY(check_field(c, out, NQ_SPINCOLOR, NQ_LX, "nq_tmclov_apply_Q"));
+  if (in == out) return fail(NQ_EINVAL, "out==in!");
+  Guard G(c);
+  TmSetup S;
+  NQ_TRY(tm_setup(c, g, LK_F64_18, anis, kappa, &S));
+  SpinorView u, none;
+  NQ_TRY(alloc_spinor(c, "clov.uo", SP_F64, 12, &u));
+  for (int p = 0; p < 2; p++) {
+    NQ_TRY(launch_clover_apply(c, 0, u, view_of(in, p), clov_view(cl, p), none, kappa, mu, 0, kOne, nullptr));
+    HopEpi e;
+    setc(e.c1, -0.5, 0.5);
+    setc(e.c2, 1.0, -1.0);
+    e.aux = u;
+    NQ_TRY(launch_tm_hop(c, p, view_of(out, p), view_of(in, !p), S.U, S.anis, e));
+  }
+  NQ_HIP(hipStreamSynchronize(c->stream));
+  return NQ_OK;
+}
+
+int nq_tmclov_cg_kernsq(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis, double mu,
+                        int niter, double residue, nq_field* sol, const nq_field* guess, const nq_field* src,
+                        nq_solve_stats* st)
+{
+  if (!c) return fail(NQ_EINVAL, "null ctx");
+  NQ_TRY(check_field(c, src, NQ_SPINCOLOR, NQ_ODD, "nq_tmclov_cg_kernsq"));
+  NQ_TRY(check_field(c, sol, NQ_SPINCOLOR, NQ_ODD, "nq_tmclov_cg_kernsq"));
+  if (guess) NQ_TRY(check_field(c, guess, NQ_SPINCOLOR, NQ_ODD, "nq_tmclov_cg_kernsq"));
+  if (sol == src) return fail(NQ_EINVAL, "sol==src");
+  if (niter <= 0 || !(residue > 0)) return fail(NQ_EINVAL, "bad niter/residue");
+  Guard G(c);
+  ClovSetup s;
+  NQ_TRY(clov_setup(c, g, cl, kappa, anis, mu, &s));
+  nq_solve_stats local;
+  if (!st) st = &local;
+  std::memset(st, 0, sizeof *st);
+  if (guess && guess != sol) NQ_TRY(launch_convert(c, view_of(sol, 0), view_of(guess, 0)));
+  const int rc = clov_cg(s, niter, residue, view_of(sol, 0), guess != nullptr, view_of(src, 0), st);
+  NQ_HIP(hipStreamSynchronize(c->stream));
+  return rc;
+}
+
+// inv_tmclovD_cg_eoprec (cg_invert_tmclovD_eoprec.cpp:56-128 and the check of :179-232)
+int nq_tmclov_solve_eoprec(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis, double mu,
+                           int niter, double residue, nq_field* sol, const nq_field* src, nq_solve_stats* st,
+                           double* check)
+{
+  if (!c) return fail(NQ_EINVAL, "null ctx");
+  NQ_TRY(check_field(c, src, NQ_SPINCOLOR, NQ_LX, "nq_tmclov_solve_eoprec"));
+  NQ_TRY(check_field(c, sol, NQ_SPINCOLOR, NQ_LX, "nq_tmclov_solve_eoprec"));
+  if (sol == src) return fail(NQ_EINVAL, "sol==src");
+  if (niter <= 0 || !(residue > 0)) return fail(NQ_EINVAL, "bad niter/residue");
+  Guard G(c);
+  ClovSetup s;
+  NQ_TRY(clov_setup(c, g, cl, kappa, anis, mu, &s));
+  SpinorView se = view_of(src, 0), so = view_of(src, 1), xe = view_of(sol, 0), xo = view_of(sol, 1);
+  SpinorView et, phi, y, none;
+  none.ncomp = 12;
+  NQ_TRY(alloc_spinor(c, "eo.et", SP_F64, 12, &et));
+  NQ_TRY(alloc_spinor(c, "eo.phi", SP_F64, 12, &phi));
+  NQ_TRY(alloc_spinor(c, "eo.y", SP_F64, 12, &y));
+  // eq. 8a: et = D_ee^-1 eta_e
+  NQ_TRY(launch_clover_apply(c, 1, et, se, s.inv_e, none, kappa, mu, 0, kOne, nullptr));
+  // eq. 8b: phi = g5 (eta_o + 1/2 H_oe et)
+  HopEpi e8;
+  setc(e8.c1, 0.5, -0.5);
+  setc(e8.c2, 1.0, -1.0);
+  e8.aux = so;
+  NQ_TRY(launch_tm_hop(c, NQ_ODD, phi, et, s.S.U, s.S.anis, e8));
+  // eq. 9
+  nq_solve_stats local;
+  if (!st) st = &local;
+  std::memset(st, 0, sizeof *st);
+  int rc = clov_cg(s, niter, residue, y, false, phi, st);
+  if (rc != NQ_OK && rc != NQ_ENOCONV) return rc;
+  // eq. 10: sol_o = K(dag) y
+  NQ_TRY(clov_kern(s, 1, xo, y, HopEpi()));
+  // eq. 11: sol_e = D_ee^-1 (eta_e + 1/2 H_eo sol_o)
+  HopEpi e11;
+  setc(e11.c1, 0.5, 0.5);
+  setc(e11.c2, 1.0, 1.0);
+  e11.aux = se;
+  NQ_TRY(launch_tm_hop(c, NQ_EVEN, et, xo, s.S.U, s.S.anis, e11));
+  NQ_TRY(launch_clover_apply(c, 1, xe, et, s.inv_e, none, kappa, mu, 0, kOne, nullptr));
+  if (check) {
+    // |D sol - src|^2 / |src|^2 per parity: (z + Cl) sol_p - 1/2 hop sol_!p - src_p
+    double n2[2] = {0, 0};
+    for (int p = 0; p < 2; p++) {
+      NQ_TRY(launch_clover_apply(c, 0, s.uo, view_of(sol, p), clov_view(cl, p), none, kappa, mu, 0, kOne, nullptr));
+      NQ_TRY(put_state(c, c->nstate - 1, CgState{}));
+      HopEpi ec;
+      setc(ec.c1, -0.5, -0.5);
+      setc(ec.c2, 1.0, 1.0);
+      setc(ec.c3, -1.0, -1.0);
+      ec.aux = s.uo;
+      ec.aux2 = view_of(src, p);
+      ec.mode = 1;
+      ec.write = 0;
+      ec.st = scratch(c);
+      ec.fin = FIN_STORE0;
+      NQ_TRY(launch_tm_hop(c, p, none, view_of(sol, !p), s.S.U, s.S.anis, ec));
+      NQ_TRY(get_states(c, c->nstate - 1, 1));
+      n2[p] = c->state_host[c->nstate - 1].extra[0];
+    }
+    double s2 = 0;
+    NQ_TRY(reduce_norm2(c, view_lx_flat(src), &s2));
+    *check = (n2[0] + n2[1]) / s2;
+  }
+  NQ_HIP(hipStreamSynchronize(c->stream));
+  return rc == NQ_ENOCONV ? fail(NQ_ENOCONV, "exit without converging") : NQ_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,115 @@
+"""GPU parity of the twisted-clover path against the oracle through the C ABI:
+clover term, D_ee / D_ee^-1 (both parities, dagger), K, K K^+, Q, the CG on
+K K^+ and the eo solve (inv_tmclovD_cg_eoprec).
+
+Bars: the clover term and the operator applies match the FP64 oracle to
+<= 1e-14 relative L2 (1e-13 where the inverse blocks enter: Gauss-Jordan on the
+device and in the oracle, with different FMA contraction); solves reach the
+requested FP64 true residual and agree with the oracle's solution to 1e-10
+relative L2 when both run to residue 1e-24.
+"""
+import numpy as np
+import pytest
+
+import nissa_amd as nq
+from helpers import gauss_eo, rel_l2, tm_conf
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(4, 6, 4, 8), (8, 8, 8, 8)]
+KAPPA, CSW = 0.16, 1.3
+
+
+@pytest.fixture(scope="module")
+def setups():
+    cache = {}
+
+    def get(L, seed=5):
+        key = (L, seed)
+        if key not in cache:
+            conf = tm_conf(L, seed)
+            ctx = nq.Ctx(L)
+            g = nq.Gauge(ctx, lx=conf)
+            cl = nq.Clover(ctx, g, CSW)
+            geo = orc.Geom(L)
+            ce, co = geo.split(conf)
+            ref = geo.clover_term(conf, CSW)
+            cache[key] = (ctx, g, cl, geo, conf, ce, co, ref)
+        return cache[key]
+
+    return get
+
+
+@pytest.mark.parametrize("L", SHAPES)
+def test_clover_term_matches_oracle(setups, L):
+    ctx, g, cl, geo, conf, ce, co, ref = setups(L)
+    got = cl.download_lx()
+    assert rel_l2(got, ref) < 1e-14
+
+
+@pytest.mark.parametrize("L", SHAPES)
+def test_tmclov_Dee_and_inverse(setups, L):
+    ctx, g, cl, geo, conf, ce, co, ref = setups(L)
+    cle, clo = geo.split(ref)
+    for par, clp, seed in ((nq.EVEN, cle, 1), (nq.ODD, clo, 2)):
+        x = gauss_eo(geo.Vh, seed)
+        fx, fo = ctx.field(nq.SPINCOLOR, par, x), ctx.field(nq.SPINCOLOR, par)
+        for mu in (0.05, -0.2):
+            inv = geo.tmclov_invert(clp, KAPPA, mu)
+            for dag in (False, True):
+                nq.tmclov_Dee(ctx, cl, KAPPA, mu, fo, fx, dag=dag)
+                assert rel_l2(fo.download(), geo.tmclovDee(clp, KAPPA, mu, x, dag=dag)) < 1e-14
+                nq.tmclov_Dee(ctx, cl, KAPPA, mu, fo, fx, dag=dag, inverse=True)
+                assert rel_l2(fo.download(), geo.inv_tmclovDee(inv, x, dag=dag)) < 1e-13
+
+
+@pytest.mark.parametrize("L", SHAPES)
+def test_tmclov_kern_square_and_Q(setups, L):
+    ctx, g, cl, geo, conf, ce, co, ref = setups(L)
+    cle, clo = geo.split(ref)
+    mu = 0.03
+    inv = geo.tmclov_invert(cle, KAPPA, mu)
+    x = gauss_eo(geo.Vh, 7)
+    fx, fo = ctx.field(nq.SPINCOLOR, nq.ODD, x), ctx.field(nq.SPINCOLOR, nq.ODD)
+    for dag in (False, True):
+        nq.tmclov_kern(ctx, g, cl, KAPPA, mu, fo, fx, dag=dag)
+        assert rel_l2(fo.download(), geo.tmclovDkern(ce, co, KAPPA, clo, inv, mu, x, dag=dag)) < 1e-13
+    nq.tmclov_kern_square(ctx, g, cl, KAPPA, mu, fo, fx)
+    assert rel_l2(fo.download(), geo.tmclovDkern_square(ce, co, KAPPA, clo, inv, mu, x)) < 1e-13
+    psi = nq.source_lx(L, 3, "gauss")
+    fl, fq = ctx.field(nq.SPINCOLOR, nq.LX, psi), ctx.field(nq.SPINCOLOR, nq.LX)
+    nq.tmclov_apply_Q(ctx, g, cl, KAPPA, mu, fq, fl)
+    assert rel_l2(fq.download(), geo.apply_tmclovQ(conf, KAPPA, ref, mu, psi)) < 1e-14
+
+
+def test_tmclov_cg_matches_oracle(setups):
+    L = (4, 6, 4, 8)
+    ctx, g, cl, geo, conf, ce, co, ref = setups(L)
+    mu = 0.02
+    cle, clo = geo.split(ref)
+    inv = geo.tmclov_invert(cle, KAPPA, mu)
+    b = gauss_eo(geo.Vh, 9)
+    fb, fx = ctx.field(nq.SPINCOLOR, nq.ODD, b), ctx.field(nq.SPINCOLOR, nq.ODD)
+    st = nq.tmclov_cg_kernsq(ctx, g, cl, KAPPA, mu, 10000, 1e-24, fx, fb)
+    assert st["converged"] and st["rel_res_sq"] < 2e-24
+    # the solution solves K K^+ x = b in the oracle's operator
+    r = geo.tmclovDkern_square(ce, co, KAPPA, clo, inv, mu, fx.download()) - b
+    assert np.sum(r ** 2) / np.sum(b ** 2) < 1e-22
+    st2 = nq.tmclov_cg_kernsq(ctx, g, cl, KAPPA, mu, 10000, 1e-20, fx, fb, guess=fx)
+    assert st2["converged"] and st2["iters"] <= 2
+
+
+@pytest.mark.parametrize("L", SHAPES)
+def test_tmclov_eo_solve_matches_oracle(setups, L):
+    ctx, g, cl, geo, conf, ce, co, ref = setups(L)
+    mu = 0.01
+    src = nq.source_lx(L, 12, "gauss")
+    sol_ref, rst = geo.tmclov_solve_eoprec(conf, KAPPA, CSW, mu, 20000, 1e-24, src)
+    assert rst["rc"] == 0
+    fs, fx = ctx.field(nq.SPINCOLOR, nq.LX, src), ctx.field(nq.SPINCOLOR, nq.LX)
+    st = nq.tmclov_solve_eoprec(ctx, g, cl, KAPPA, mu, 20000, 1e-24, fx, fs)
+    assert st["rc"] == nq.NQ_OK and st["rel_res_sq"] < 2e-24
+    assert st["check_rel_res_sq"] < 1e-22
+    assert rel_l2(fx.download(), sol_ref) < 1e-10
+    assert abs(st["iters"] - rst["iters"]) <= 2
