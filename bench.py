@@ -44,6 +44,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-stag", action="store_true", help="skip the staggered multi-shift leg")
     ap.add_argument("--no-24", action="store_true", help="skip the 24^3x48 FP64 Dslash and solve leg")
+    ap.add_argument("--no-clover", action="store_true", help="skip the twisted-clover solve leg")
     ap.add_argument("--serve-requests", type=int, default=60, help="requests in the Poisson stream")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="request arrival rate (1/s)")
     args = ap.parse_args()
@@ -147,13 +148,16 @@ def main():
     serve = None
     stag = None
     lat24 = None
-    if not (args.no_serve and args.no_stag and args.no_24):
+    clov = None
+    if not (args.no_serve and args.no_stag and args.no_24 and args.no_clover):
         for f in (fs, fx, fo, fe):
             f.close()
         g.close()
         ctx.close()
         if not args.no_24:
             lat24 = lattice24_bench()
+        if not args.no_clover:
+            clov = clover_bench()
         if not args.no_serve:
             serve = serve_bench(args)
         if not args.no_stag:
@@ -200,12 +204,13 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_of(dom_name, L), "kernel": dom_name,
                      "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": dom["bytes_per_launch"]},
         "lattice_24c48_fp64": lat24,
+        "twisted_clover_fp64": clov,
         "serve": serve,
         "staggered_cgm": stag,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    if args.no_serve and args.no_stag and args.no_24:
+    if args.no_serve and args.no_stag and args.no_24 and args.no_clover:
         for f in (fs, fx, fo, fe):
             f.close()
         g.close()
@@ -250,6 +255,58 @@ def lattice24_bench():
            "ok": bool(st["rc"] == 0 and st["check_rel_res_sq"] < 2 * residue)}
     log(f"[bench] 24^3x48: Dslash FP64 {ms_hop*1e3:.1f} us ({gbs:.0f} GB/s), FP64 solve to 1e-24 "
         f"{t*1e3:.1f} ms in {st['iters']} iterations, check {st['check_rel_res_sq']:.2e}")
+    return out
+
+
+def clover_bench():
+    """Twisted-clover FP64 eo solve (inv_tmclovD_cg_eoprec) at 32^3x64: kappa 0.16,
+    cSW 1.0, mu 0.003, same gauge construction (seed 3), Gaussian source, to a true
+    relative residual 1e-10 (nissa residue 1e-20) checked in FP64; also the clover
+    term build and the inverse blocks, and the per-kernel ledger of one solve."""
+    import nissa_amd as nq
+
+    L = (64, 32, 32, 32)
+    kappa, csw, mu, residue = 0.16, 1.0, 0.003, 1e-20
+    conf = nq.gauge_random_lx(L, 3, 0.3)
+    nq.gauge_phase_lx(L, conf, [1.0, 0, 0, 0])
+    src = nq.source_lx(L, 17, "gauss")
+    ctx = nq.Ctx(L, 0)
+    g = nq.Gauge(ctx, lx=conf)
+    ctx.sync()
+    t0 = time.perf_counter()
+    cl = nq.Clover(ctx, g, csw)
+    ctx.sync()
+    t_build = time.perf_counter() - t0
+    fs = ctx.field(nq.SPINCOLOR, nq.LX, src)
+    fx = ctx.field(nq.SPINCOLOR, nq.LX)
+    nq.tmclov_solve_eoprec(ctx, g, cl, kappa, mu, 100000, residue, fx, fs)  # builds the inverse blocks
+    ctx.sync()
+    t0 = time.perf_counter()
+    st = nq.tmclov_solve_eoprec(ctx, g, cl, kappa, mu, 100000, residue, fx, fs)
+    ctx.sync()
+    t = time.perf_counter() - t0
+    ctx.reset_kernel_stats()
+    ctx.profiling(True)
+    nq.tmclov_solve_eoprec(ctx, g, cl, kappa, mu, 100000, residue, fx, fs)
+    ctx.profiling(False)
+    ks = ctx.kernel_stats()
+    ledger = {}
+    for name, v in sorted(ks.items(), key=lambda kv: -kv[1]["total_ms"])[:6]:
+        a = v["total_ms"] / v["launches"]
+        ledger[name] = {"launches": v["launches"], "avg_us": round(a * 1e3, 1),
+                        "GBs": round(v["bytes_per_launch"] / (a * 1e-3) / 1e9, 1) if v["bytes_per_launch"] else None}
+    for f in (fs, fx):
+        f.close()
+    cl.close()
+    g.close()
+    ctx.close()
+    out = {"lattice": list(L), "kappa": kappa, "csw": csw, "mu": mu, "clover_build_ms": round(t_build * 1e3, 2),
+           "fp64_solve_ms": round(t * 1e3, 2), "iters": st["iters"], "true_rel_res_sq": st["check_rel_res_sq"],
+           "ok": bool(st["rc"] == 0 and st["check_rel_res_sq"] < 2 * residue), "ledger": ledger}
+    log(f"[bench] twisted clover 32^3x64 csw {csw}: build {t_build*1e3:.1f} ms, FP64 solve {t*1e3:.1f} ms "
+        f"in {st['iters']} iterations, check {st['check_rel_res_sq']:.2e}")
+    for k, v in ledger.items():
+        log(f"[bench]   {k:24s} {v}")
     return out
 
 

@@ -235,24 +235,28 @@ __global__ void __launch_bounds__(64) k_clover_invert(Geo g, SpinorView cl, Spin
 // mode 1 (inv_tmclovDee_or_oo_eos, dirac_operator_tmclovD_eoprec.hpp:22-43):
 //   out_h = inv_h in_h, with inv_h^+ for h = 1 (h = 0 for the dagger).
 // out = a_h * (that) + b_h * aux (per chirality coefficients, aux optional).
+// All block loads of a chirality are issued before its arithmetic; the two
+// chiralities are kept apart so that their blocks are not live together.
 struct ClovArgs {
   double a[2][2], b[2][2];
 };
 
-__global__ void __launch_bounds__(128) k_clover_apply(Geo g, int mode, SpinorView out, SpinorView in, SpinorView blk,
-                                                      SpinorView aux, double kappa, double mu, int dag, ClovArgs k)
+template <int MODE, int DAG>
+__global__ void __launch_bounds__(256) k_clover_apply(Geo g, SpinorView out, SpinorView in, SpinorView blk,
+                                                      SpinorView aux, double kappa, double mu, ClovArgs k)
 {
-  const uint32_t e = blockIdx.x * 128 + threadIdx.x;
+  const uint32_t e = blockIdx.x * 256 + threadIdx.x;
   if (e >= (uint32_t)g.Vh) return;
   C v[12], o[12];
   SpAcc<SP_F64, double, 12>::load(in, g.stride, e, v);
   const C* B = (const C*)blk.data;
   const int64_t st = g.stride;
+#pragma unroll
   for (int h = 0; h < 2; h++) {
     const C* iv = v + 6 * h;
     C* ov = o + 6 * h;
-    if (mode == 0) {
-      const double m = (dag ? -mu : mu) * (h == 0 ? 1.0 : -1.0);
+    if constexpr (MODE == 0) {
+      const double m = (DAG ? -mu : mu) * (h == 0 ? 1.0 : -1.0);
       const C z = mk<double>(1 / (2 * kappa), m);
       C c0[9], c1[9];
 #pragma unroll
@@ -272,17 +276,21 @@ __global__ void __launch_bounds__(128) k_clover_apply(Geo g, int mode, SpinorVie
         ov[3 + a] = s1;
       }
     } else {
-      const bool hd = dag ? (h == 0) : (h == 1);
-      for (int r = 0; r < 6; r++) {
-        C s = mk<double>(0, 0);
+      constexpr bool hd0 = DAG != 0;  // chirality 0 uses the daggered block for the dagger
+      const bool hd = h == 0 ? hd0 : !hd0;
+      C M[36];
 #pragma unroll
-        for (int c = 0; c < 6; c++) {
-          const C m = hd ? conjg(B[(36 * h + 6 * c + r) * st + e]) : B[(36 * h + 6 * r + c) * st + e];
-          s = s + m * iv[c];
-        }
-        ov[r] = s;
+      for (int q = 0; q < 36; q++) M[q] = B[(36 * h + q) * st + e];
+#pragma unroll
+      for (int r = 0; r < 6; r++) {
+        C sum = mk<double>(0, 0);
+#pragma unroll
+        for (int c = 0; c < 6; c++) sum = sum + (hd ? conjg(M[6 * c + r]) : M[6 * r + c]) * iv[c];
+        ov[r] = sum;
       }
     }
+#pragma unroll
+    for (int q = 0; q < 6; q++) asm volatile("" : "+v"(ov[q].re), "+v"(ov[q].im));
   }
   if (aux.data) {
     C w[12];
@@ -347,8 +355,11 @@ int launch_clover_apply(nq_ctx* ctx, int mode, const SpinorView& out, const Spin
   hipEvent_t ev = nullptr;
   const double bytes = (double)g.Vh * ((mode == 0 ? 36 : 72) * 16.0 + 2 * 192.0 + (aux.data ? 192.0 : 0.0));
   prof_begin(ctx, mode == 0 ? "clover_dee" : "clover_inv", bytes, &ev);
-  hipLaunchKernelGGL(k_clover_apply, dim3((unsigned)((g.Vh + 127) / 128)), dim3(128), 0, ctx->stream, g, mode, out, in,
-                     blk, aux, kappa, mu, dag, k);
+  const dim3 grid((unsigned)((g.Vh + 255) / 256)), blkd(256);
+  if (mode == 0 && !dag) hipLaunchKernelGGL((k_clover_apply<0, 0>), grid, blkd, 0, ctx->stream, g, out, in, blk, aux, kappa, mu, k);
+  else if (mode == 0) hipLaunchKernelGGL((k_clover_apply<0, 1>), grid, blkd, 0, ctx->stream, g, out, in, blk, aux, kappa, mu, k);
+  else if (!dag) hipLaunchKernelGGL((k_clover_apply<1, 0>), grid, blkd, 0, ctx->stream, g, out, in, blk, aux, kappa, mu, k);
+  else hipLaunchKernelGGL((k_clover_apply<1, 1>), grid, blkd, 0, ctx->stream, g, out, in, blk, aux, kappa, mu, k);
   NQ_HIP(hipGetLastError());
   prof_end(ctx, ev);
   return NQ_OK;
