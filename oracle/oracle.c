@@ -1128,3 +1128,82 @@ int or_tmclov_solve_eoprec(const or_geom* g, double* sol_lx, const double* conf_
   free(et); free(phi); free(ot);
   return rc;
 }
+
+/* ===================================================================== rooted staggered force
+ * summ_the_rootst_eoimpr_quark_force (hmc/fermions/rootst_eoimpr_quark_force.cpp:17-85):
+ * the links get the background field and the staggered phases (add_backfield_with_stagphases_to_conf,
+ * backfield.cpp:248-284; eta_mu(x) = prod_{nu<mu} (-1)^{x_nu}, geometry_lx.cpp:703-724), the
+ * multi-shift CG gives chi_k = (p_k - 1/4 Deo Doe)^-1 pf for the poles p_k, v_k = Doe chi_k
+ * (apply_stDoe = 1/2 apply_st2Doe), and per link
+ *   F_e[x][mu] += w_k eta u1b_e v_k(x+mu) chi_k(x)^+ ,  F_o[x][mu] -= w_k eta u1b_o chi_k(x+mu) v_k(x)^+ .
+ * ce/co: links without phases; u1b: [Vh][4][2] per parity; F: [Vh][4][3][3][2] per parity, summed into. */
+static int stag_eta(const int x[4], int mu)
+{
+  int ph = 1;
+  for (int nu = 1; nu <= mu; nu++) ph *= 1 - 2 * (x[nu - 1] % 2);
+  return ph;
+}
+
+int or_rootst_force(const or_geom* g, double* Fe, double* Fo, const double* ce, const double* co,
+                    const double* u1e, const double* u1o, const double* pf, int ndeg, const double* poles,
+                    const double* weights, int niter, double residue)
+{
+  const int64_t Vh = g->Vh;
+  const double* C[2] = {ce, co};
+  const double* U1[2] = {u1e, u1o};
+  double* P[2];
+  for (int p = 0; p < 2; p++) {
+    P[p] = (double*)malloc(sizeof(double) * Vh * 72);
+    for (int64_t i = 0; i < Vh; i++) {
+      int x[4];
+      coord_of_lx(g->L, g->lx_of_eo[p][i], x);
+      for (int mu = 0; mu < 4; mu++) {
+        const double fr = U1[p][(i * 4 + mu) * 2] * stag_eta(x, mu), fi = U1[p][(i * 4 + mu) * 2 + 1] * stag_eta(x, mu);
+        const double* u = C[p] + (i * 4 + mu) * 18;
+        double* o = P[p] + (i * 4 + mu) * 18;
+        for (int k = 0; k < 9; k++) {
+          o[2 * k] = u[2 * k] * fr - u[2 * k + 1] * fi;
+          o[2 * k + 1] = u[2 * k] * fi + u[2 * k + 1] * fr;
+        }
+      }
+    }
+  }
+  double** chi = (double**)malloc(sizeof(double*) * ndeg);
+  double* res = (double*)malloc(sizeof(double) * ndeg);
+  for (int k = 0; k < ndeg; k++) {
+    chi[k] = (double*)malloc(sizeof(double) * Vh * 6);
+    res[k] = residue;
+  }
+  or_cg_stats st;
+  int rc = or_st_cgm(g, chi, P[0], P[1], ndeg, poles, niter, res, pf, &st, NULL);
+  double* v = (double*)malloc(sizeof(double) * Vh * 6);
+  double* F[2] = {Fe, Fo};
+  for (int k = 0; k < ndeg; k++) {
+    or_st_2Doe(g, v, P[0], P[1], chi[k]);
+    for (int64_t i = 0; i < Vh * 6; i++) v[i] *= 0.5;
+    for (int p = 0; p < 2; p++)
+      for (int64_t i = 0; i < Vh; i++) {
+        int x[4];
+        coord_of_lx(g->L, g->lx_of_eo[p][i], x);
+        for (int mu = 0; mu < 4; mu++) {
+          const int64_t n = g->eoup[p][i * 4 + mu];
+          const double* a = p == 0 ? v + n * 6 : chi[k] + n * 6;
+          const double* b = p == 0 ? chi[k] + i * 6 : v + i * 6;
+          const double s = (p == 0 ? 1.0 : -1.0) * weights[k] * stag_eta(x, mu);
+          const double ur = U1[p][(i * 4 + mu) * 2], ui = U1[p][(i * 4 + mu) * 2 + 1];
+          double* f = F[p] + (i * 4 + mu) * 18;
+          for (int c1 = 0; c1 < 3; c1++)
+            for (int c2 = 0; c2 < 3; c2++) {
+              /* a_c1 conj(b_c2) u1b */
+              const double tr = a[2 * c1] * b[2 * c2] + a[2 * c1 + 1] * b[2 * c2 + 1];
+              const double ti = a[2 * c1 + 1] * b[2 * c2] - a[2 * c1] * b[2 * c2 + 1];
+              f[(c1 * 3 + c2) * 2] += s * (tr * ur - ti * ui);
+              f[(c1 * 3 + c2) * 2 + 1] += s * (tr * ui + ti * ur);
+            }
+        }
+      }
+  }
+  for (int k = 0; k < ndeg; k++) free(chi[k]);
+  free(chi); free(res); free(v); free(P[0]); free(P[1]);
+  return rc;
+}

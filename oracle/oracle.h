@@ -152,6 +152,9 @@ void or_tmclovDkern_square(const or_geom* g, double* out, double* t1, double* t2
                            const double* inv_evn, double mu, const double* in);
 void or_apply_tmclovQ(const or_geom* g, double* out, const double* conf_lx, double kappa, const double* anis,
                       const double* cl_lx, double mu, const double* in);
+int or_rootst_force(const or_geom* g, double* Fe, double* Fo, const double* ce, const double* co,
+                    const double* u1e, const double* u1o, const double* pf, int ndeg, const double* poles,
+                    const double* weights, int niter, double residue);
 int or_tmclov_solve_eoprec(const or_geom* g, double* sol_lx, const double* conf_lx, double kappa,
                            const double* anis, double csw, double mu, int niter, double residue,
                            const double* source_lx, or_cg_stats* st, double* check);

@@ -49,6 +49,7 @@ def lib():
             "or_tmDkern": ([_P, _DP, _DP, _DP, _DP, _D, _DP, _D, _DP], None),
             "or_tmDkern_square": ([_P, _DP, _DP, _DP, _DP, _DP, _D, _DP, _D, _DP], None),
             "or_clover_term_lx": ([_P, _DP, _DP, _D], None),
+            "or_rootst_force": ([_P, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _I, _DP, _DP, _I, _D], _I),
             "or_tmclovDee": ([_P, _DP, _DP, _D, _D, _I, _DP], None),
             "or_tmclov_invert": ([_P, _DP, _DP, _D, _D], None),
             "or_inv_tmclovDee": ([_P, _DP, _DP, _I, _DP], None),
@@ -226,6 +227,18 @@ class Geom:
                                           csw, mu, niter, residue, _dp(np.ascontiguousarray(src_lx)), C.byref(st),
                                           C.byref(chk))
         return sol, {"rc": rc, "iters": st.iters, "rel_res_sq": st.rel_res_sq, "check": chk.value}
+
+    # -- rooted staggered force
+    def rootst_force(self, ce, co, u1e, u1o, pf, poles, weights, niter, residue, Fe=None, Fo=None):
+        """summ_the_rootst_eoimpr_quark_force: ce/co pure links, u1e/u1o (Vh,4,2); F summed into."""
+        Fe = np.zeros((self.Vh, 4, 3, 3, 2)) if Fe is None else Fe
+        Fo = np.zeros((self.Vh, 4, 3, 3, 2)) if Fo is None else Fo
+        n = len(poles)
+        rc = lib().or_rootst_force(self.h, _dp(Fe), _dp(Fo), _dp(np.ascontiguousarray(ce)), _dp(np.ascontiguousarray(co)),
+                                   _dp(np.ascontiguousarray(u1e)), _dp(np.ascontiguousarray(u1o)),
+                                   _dp(np.ascontiguousarray(pf)), n, (C.c_double * n)(*poles),
+                                   (C.c_double * n)(*weights), niter, residue)
+        return Fe, Fo, rc
 
     def apply_tmQ(self, conf_lx, kappa, mu, inp, anis=None):
         out = np.empty((self.V, 4, 3, 2))
