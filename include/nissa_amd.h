@@ -209,6 +209,15 @@ int nq_st_cgm_rat_sum(nq_ctx* ctx, const nq_gauge* g, int nrhs, int degree, doub
                       const double* poles, const double* weights, int niter, double residue, int prec,
                       nq_field** chis, const nq_field* const* srcs, nq_solve_stats* st);
 
+/* summ_the_rootst_eoimpr_quark_force (hmc/fermions/rootst_eoimpr_quark_force.cpp:17-85): g holds the
+   links with background field and staggered phases; u1b_evn/odd the background field ([Vh][4]
+   complex, nissa eo order); the multi-shift solve runs on the poles to `residue` (prec
+   NQ_PREC_F64 or NQ_PREC_F32 with FP64 reliable updates); F_evn/F_odd (nissa EoField<quad_su3>
+   AoS) are summed into, before the TA projection and the product with U, as in the reference */
+int nq_st_rootst_force(nq_ctx* ctx, const nq_gauge* g, const double* u1b_evn, const double* u1b_odd, int ndeg,
+                       const double* poles, const double* weights, int niter, double residue, int prec,
+                       const nq_field* pf, double* F_evn, double* F_odd, nq_solve_stats* st);
+
 /* ------------------------------------------------------------------ host-buffer drop-ins */
 /* Same operations over nissa host buffers; the gauge handle stays resident. */
 int nq_tm_kern_square_host(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis,

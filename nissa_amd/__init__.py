@@ -98,6 +98,7 @@ _SIG = {
     "nq_tmclov_apply_Q": ([_P, _P, _P, _D, _DP, _D, _P, _P], _I),
     "nq_tmclov_cg_kernsq": ([_P, _P, _P, _D, _DP, _D, _I, _D, _P, _P, _P, _P], _I),
     "nq_tmclov_solve_eoprec": ([_P, _P, _P, _D, _DP, _D, _I, _D, _P, _P, _P, _DP], _I),
+    "nq_st_rootst_force": ([_P, _P, _DP, _DP, _I, _DP, _DP, _I, _D, _I, _P, _DP, _DP, _P], _I),
     "nq_st_apply_D": ([_P, _P, _D, _I, _P, _P, _P, _P], _I),
     "nq_st_solve_D": ([_P, _P, _D, _I, _D, _P, _P, _P, _P, _P, _P, _DP], _I),
     "nq_tmQ2_cgm": ([_P, _P, _D, _I, _DP, _I, _DP, _P, _P, _P, _DP], _I),
@@ -584,6 +585,20 @@ def st_solve_D(ctx, g, m, niter, residue, sol_e, sol_o, src_e, src_o, guess=None
     out = _solve(rc, st)
     out["check_rel_res_sq"] = chk.value
     return out
+
+
+def st_rootst_force(ctx, g, u1b_e, u1b_o, poles, weights, niter, residue, pf, Fe=None, Fo=None, prec="f64"):
+    """summ_the_rootst_eoimpr_quark_force: F (host, (Vh,4,3,3,2) per parity) summed into."""
+    Vh = ctx.Vh
+    Fe = np.zeros((Vh, 4, 3, 3, 2)) if Fe is None else Fe
+    Fo = np.zeros((Vh, 4, 3, 3, 2)) if Fo is None else Fo
+    n = len(poles)
+    st = SolveStats()
+    rc = lib().nq_st_rootst_force(ctx.h, g.h, _dp(np.ascontiguousarray(u1b_e)), _dp(np.ascontiguousarray(u1b_o)), n,
+                                  (C.c_double * n)(*poles), (C.c_double * n)(*weights), niter, residue, PREC[prec],
+                                  pf.h, _dp(Fe), _dp(Fo), C.byref(st))
+    out = _solve(rc, st)
+    return Fe, Fo, out
 
 
 def tmQ2_cgm(ctx, g, kappa, masses, niter, residues, sols, src, g5=False):

@@ -301,6 +301,9 @@ int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, co
 // layout conversions
 // host buffer in nissa order (host_lx: lx-indexed buffer, else eo-indexed) <->
 // device parity `par` of a field
+// rooted-staggered force of one pole on the links of parity par (st_kernels.hip)
+int launch_st_force(nq_ctx* ctx, int par, const SpinorView& a, const SpinorView& b, const double* u1b, double wsign,
+                    double* F);
 // twisted clover (clover.hip): mode 0 = (z + C) per chirality, 1 = inverse blocks;
 // out = a_h (that) + b_h aux
 int launch_clover_build(nq_ctx* ctx, const double* aos_e, const double* aos_o, int par, double csw,

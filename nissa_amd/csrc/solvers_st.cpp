@@ -655,6 +655,81 @@ int nq_st_solve_D(nq_ctx* c, const nq_gauge* g, double m, int niter, double resi
   return NQ_OK;
 }
 
+// summ_the_rootst_eoimpr_quark_force (hmc/fermions/rootst_eoimpr_quark_force.cpp:17-85).
+// g holds the links as the solver sees them (with background field and staggered
+// phases, which nissa adds at the start of the routine); u1b the background field
+// alone, whose product with eta_mu(x) is the phase of each link in the force.
+// F (host, nissa eo AoS) is summed into; the TA projection and the product with U
+// stay with the caller, as in the reference.
+int nq_st_rootst_force(nq_ctx* c, const nq_gauge* g, const double* u1b_evn, const double* u1b_odd, int ndeg,
+                       const double* poles, const double* weights, int niter, double residue, int prec,
+                       const nq_field* pf, double* F_evn, double* F_odd, nq_solve_stats* st)
+{
+  if (!c || !g || !u1b_evn || !u1b_odd || !poles || !weights || !F_evn || !F_odd) return fail(NQ_EINVAL, "null argument");
+  if (ndeg < 1 || ndeg > kMaxShift) return fail(NQ_EINVAL, "need 1..%d poles", kMaxShift);
+  NQ_TRY(check_color(c, pf, NQ_EVEN, "nq_st_rootst_force"));
+  Guard G(c);
+  const int64_t Vh = c->geo.Vh;
+  std::vector<nq_field*> chi(ndeg, nullptr);
+  DevBuf u1[2], F[2];
+  int rc = NQ_OK;
+  for (int k = 0; k < ndeg && rc == NQ_OK; k++) rc = nq_field_create(c, NQ_COLOR, NQ_EVEN, &chi[k]);
+  for (int p = 0; p < 2 && rc == NQ_OK; p++) rc = dev_alloc(u1[p], sizeof(double) * 8 * Vh);
+  for (int p = 0; p < 2 && rc == NQ_OK; p++) rc = dev_alloc(F[p], sizeof(double) * 72 * Vh);
+  auto cleanup = [&]() {
+    for (auto* f : chi) nq_field_destroy(f);
+    for (int p = 0; p < 2; p++) {
+      dev_free(u1[p]);
+      dev_free(F[p]);
+    }
+  };
+  if (rc != NQ_OK) {
+    cleanup();
+    return rc;
+  }
+  // the pseudofermion's multi-shift solve on the poles
+  std::vector<double> res(ndeg, residue), tr(ndeg);
+  nq_solve_stats local;
+  if (!st) st = &local;
+  const nq_field* srcs[1] = {pf};
+  rc = nq_st_cgm(c, g, 1, ndeg, poles, niter, res.data(), prec, chi.data(), srcs, st, tr.data());
+  if (rc != NQ_OK && rc != NQ_ENOCONV) {
+    cleanup();
+    return rc;
+  }
+  const int rc_solve = rc;
+  LinkView U;
+  SpinorView v, none;
+  none.ncomp = 3;
+  rc = link_view(g, LK_F64_18, &U);
+  if (rc == NQ_OK) rc = alloc_spinor(c, "force.v", SP_F64, 3, &v);
+  const double* U1h[2] = {u1b_evn, u1b_odd};
+  for (int p = 0; p < 2 && rc == NQ_OK; p++) {
+    if (hipMemcpyAsync(u1[p].p, U1h[p], sizeof(double) * 8 * Vh, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemsetAsync(F[p].p, 0, sizeof(double) * 72 * Vh, c->stream) != hipSuccess)
+      rc = fail(NQ_EDEVICE, "force buffers");
+  }
+  for (int k = 0; k < ndeg && rc == NQ_OK; k++) {
+    // v_k = Doe chi_k = 1/2 (2 Doe) chi_k (apply_stDoe, dirac_operator_stD.hpp:80-92)
+    rc = launch_st_hop(c, NQ_ODD, v, view_of(chi[k], 0), U, 0.5, 0.0, nullptr, none, 0, 1, 1, FIN_NONE, nullptr);
+    if (rc == NQ_OK) rc = launch_st_force(c, NQ_EVEN, v, view_of(chi[k], 0), (const double*)u1[0].p, weights[k], (double*)F[0].p);
+    if (rc == NQ_OK) rc = launch_st_force(c, NQ_ODD, view_of(chi[k], 0), v, (const double*)u1[1].p, -weights[k], (double*)F[1].p);
+  }
+  double* Fh[2] = {F_evn, F_odd};
+  std::vector<double> h(72 * Vh);
+  for (int p = 0; p < 2 && rc == NQ_OK; p++) {
+    if (hipMemcpyAsync(h.data(), F[p].p, sizeof(double) * 72 * Vh, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+      rc = fail(NQ_EDEVICE, "force download");
+      break;
+    }
+    for (int64_t i = 0; i < 72 * Vh; i++) Fh[p][i] += h[i];
+  }
+  cleanup();
+  if (rc != NQ_OK) return rc;
+  return rc_solve == NQ_ENOCONV ? fail(NQ_ENOCONV, "exit without converging") : NQ_OK;
+}
+
 int nq_st_cgm_rat_sum(nq_ctx* c, const nq_gauge* g, int nrhs, int degree, double cons, const double* poles,
                       const double* weights, int niter, double residue, int prec, nq_field** chis,
                       const nq_field* const* srcs, nq_solve_stats* st)

@@ -504,4 +504,58 @@ int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, co
   return NQ_OK;
 }
 
+// Rooted-staggered force accumulation of one pole (summ_the_rootst_eoimpr_quark_force,
+// hmc/fermions/rootst_eoimpr_quark_force.cpp:54-82) on the links of parity par:
+//   F[x][mu] += wsign * eta_mu(x) u1b[x][mu] a(x+mu) b(x)^+
+// (even: a = v = Doe chi, b = chi, wsign = +w; odd: a = chi, b = v, wsign = -w),
+// eta_mu(x) = prod_{nu<mu} (-1)^{x_nu} (getStagphaseOfLx, geometry_lx.cpp:703-724).
+// u1b and F are in nissa's eo order ([Vh][4] complex, [Vh][4][3][3] complex); every
+// (site, mu) entry belongs to one thread.
+__global__ void __launch_bounds__(256) k_st_force(Geo g, int par, SpinorView a, SpinorView b, const double* u1b,
+                                                  double wsign, double* F)
+{
+  const uint32_t e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= (uint32_t)g.Vh) return;
+  const Site s = site_of(g, par, e);
+  const int64_t ieo = nissa_lx(g, s.t, s.x, s.y, s.z) >> 1;
+  cx<double> bv[3];
+  SpAcc<SP_F64, double, 3>::load(b, g.stride, e, bv);
+  const int c[4] = {s.t, s.x, s.y, s.z};
+  int eta = 1;
+  for (int mu = 0; mu < 4; mu++) {
+    if (mu > 0) eta *= 1 - 2 * (c[mu - 1] & 1);
+    const uint32_t n = neigh_eo(g, s, mu, true);
+    cx<double> av[3];
+    SpAcc<SP_F64, double, 3>::load(a, g.stride, n, av);
+    const double* u = u1b + (ieo * 4 + mu) * 2;
+    const cx<double> ph = mk<double>(u[0] * wsign * eta, u[1] * wsign * eta);
+    double* f = F + (ieo * 4 + mu) * 18;
+#pragma unroll
+    for (int c1 = 0; c1 < 3; c1++) {
+      const cx<double> pa = ph * av[c1];
+#pragma unroll
+      for (int c2 = 0; c2 < 3; c2++) {
+        const cx<double> t = pa * conjg(bv[c2]);
+        f[(c1 * 3 + c2) * 2] += t.re;
+        f[(c1 * 3 + c2) * 2 + 1] += t.im;
+      }
+    }
+  }
+}
+
+int launch_st_force(nq_ctx* ctx, int par, const SpinorView& a, const SpinorView& b, const double* u1b, double wsign,
+                    double* F)
+{
+  const Geo& g = ctx->geo;
+  if (a.ncomp != 3 || b.ncomp != 3 || a.prec != SP_F64 || b.prec != SP_F64)
+    return fail(NQ_EINVAL, "staggered force needs FP64 colour fields");
+  hipEvent_t ev = nullptr;
+  prof_begin(ctx, "st_force", (double)g.Vh * (48.0 * 5 + 4 * (16.0 + 2 * 144.0)), &ev);
+  hipLaunchKernelGGL(k_st_force, dim3((unsigned)((g.Vh + 255) / 256)), dim3(256), 0, ctx->stream, g, par, a, b, u1b,
+                     wsign, F);
+  NQ_HIP(hipGetLastError());
+  prof_end(ctx, ev);
+  return NQ_OK;
+}
+
 }  // namespace nq
