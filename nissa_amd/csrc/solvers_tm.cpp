@@ -647,11 +647,19 @@ int TmBatch::enqueue(int nr, int iters)
     b2e.aux = t1s_;
     b2e.mode = 2;
     b2e.fin = mixed_ ? FIN_RR_SLOPPY : FIN_RR;
-    b2e.x = xs_;
     b2e.p = p_;
     b2e.r = rs_;
-    NQ_TRY(launch_tm_hop(c, NQ_ODD, none, tmps_, Ss.U, Ss.anis, b2e));
-    NQ_TRY(launch_cg_pupdate(c, p_, rs_, S_, nr));
+    if (mixed_) {
+      // sloppy: xs += alpha p moves into the p update (one read of p); a slot whose
+      // update is pending skips it and the reliable update folds alpha p in
+      NQ_TRY(launch_tm_hop(c, NQ_ODD, none, tmps_, Ss.U, Ss.anis, b2e));
+      NQ_TRY(launch_cg_pupdate(c, p_, rs_, S_, nr, 1, xs_));
+    } else {
+      // FP64: the slot that converges stops before its p update, so x stays in the tail
+      b2e.x = xs_;
+      NQ_TRY(launch_tm_hop(c, NQ_ODD, none, tmps_, Ss.U, Ss.anis, b2e));
+      NQ_TRY(launch_cg_pupdate(c, p_, rs_, S_, nr));
+    }
   }
   return NQ_OK;
 }
@@ -724,11 +732,11 @@ int TmBatch::poll(std::vector<int>* finished)
   bool any_rel = false;
   for (int i = 0; i < nr; i++) any_rel |= active_[i] && c->state_host[base + i].reliable && !c->state_host[base + i].done;
   if (any_rel) {
-    // reliable update of the flagged slots: x64 += xs, xs = 0; r64 = b - A x64 in FP64,
-    // rs = r64; p = rs + beta p; then clear the flags
+    // reliable update of the flagged slots: x64 += xs + alpha p, xs = 0; r64 = b - A x64
+    // in FP64, rs = r64; p = rs + beta p; then clear the flags
     TmSetup S64;
     NQ_TRY(tm_setup(c, g_, lk64_for(prec_, g_), has_anis_ ? &anis_ : nullptr, kappa_, &S64));
-    NQ_TRY(launch_reliable_accum(c, x64_, xs_, S_, nr));
+    NQ_TRY(launch_reliable_accum(c, x64_, xs_, S_, nr, p_));
     NQ_TRY(residual_kernsq_batched(S64, coef_, nr, r64_, rs_, x64_, b_, t64_, tmp64_, S_, FIN_RR_TRUE_HOLD, 1));
     NQ_TRY(launch_cg_pupdate(c, p_, rs_, S_, nr, 2));
     NQ_TRY(launch_clear_reliable(c, S_, nr));
