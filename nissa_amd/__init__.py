@@ -96,8 +96,8 @@ _SIG = {
     "nq_tmclov_kern": ([_P, _P, _P, _D, _DP, _D, _I, _P, _P], _I),
     "nq_tmclov_kern_square": ([_P, _P, _P, _D, _DP, _D, _P, _P], _I),
     "nq_tmclov_apply_Q": ([_P, _P, _P, _D, _DP, _D, _P, _P], _I),
-    "nq_tmclov_cg_kernsq": ([_P, _P, _P, _D, _DP, _D, _I, _D, _P, _P, _P, _P], _I),
-    "nq_tmclov_solve_eoprec": ([_P, _P, _P, _D, _DP, _D, _I, _D, _P, _P, _P, _DP], _I),
+    "nq_tmclov_cg_kernsq": ([_P, _P, _P, _D, _DP, _D, _I, _D, _I, _P, _P, _P, _P], _I),
+    "nq_tmclov_solve_eoprec": ([_P, _P, _P, _D, _DP, _D, _I, _D, _I, _P, _P, _P, _DP], _I),
     "nq_st_rootst_force": ([_P, _P, _DP, _DP, _I, _DP, _DP, _I, _D, _I, _P, _DP, _DP, _P], _I),
     "nq_st_apply_D": ([_P, _P, _D, _I, _P, _P, _P, _P], _I),
     "nq_st_solve_D": ([_P, _P, _D, _I, _D, _P, _P, _P, _P, _P, _P, _DP], _I),
@@ -501,18 +501,18 @@ def tm_solve_eoprec(ctx, g, kappa, mu, niter, residue, sol, src, prec="f64", ani
     return d
 
 
-def tmclov_cg_kernsq(ctx, g, cl, kappa, mu, niter, residue, sol, src, guess=None, anis=None):
+def tmclov_cg_kernsq(ctx, g, cl, kappa, mu, niter, residue, sol, src, guess=None, anis=None, prec="f64"):
     st = SolveStats()
-    rc = lib().nq_tmclov_cg_kernsq(ctx.h, g.h, cl.h, kappa, _anis(anis), mu, niter, residue, sol.h,
+    rc = lib().nq_tmclov_cg_kernsq(ctx.h, g.h, cl.h, kappa, _anis(anis), mu, niter, residue, PREC[prec], sol.h,
                                    guess.h if guess is not None else None, src.h, C.byref(st))
     return _solve(rc, st)
 
 
-def tmclov_solve_eoprec(ctx, g, cl, kappa, mu, niter, residue, sol, src, anis=None):
+def tmclov_solve_eoprec(ctx, g, cl, kappa, mu, niter, residue, sol, src, anis=None, prec="f64"):
     st = SolveStats()
     chk = C.c_double(0)
-    rc = lib().nq_tmclov_solve_eoprec(ctx.h, g.h, cl.h, kappa, _anis(anis), mu, niter, residue, sol.h, src.h,
-                                      C.byref(st), C.byref(chk))
+    rc = lib().nq_tmclov_solve_eoprec(ctx.h, g.h, cl.h, kappa, _anis(anis), mu, niter, residue, PREC[prec], sol.h,
+                                      src.h, C.byref(st), C.byref(chk))
     d = _solve(rc, st)
     d["check_rel_res_sq"] = chk.value
     return d

@@ -241,24 +241,26 @@ struct ClovArgs {
   double a[2][2], b[2][2];
 };
 
-template <int MODE, int DAG>
+template <int MODE, int DAG, int SP>
 __global__ void __launch_bounds__(256) k_clover_apply(Geo g, SpinorView out, SpinorView in, SpinorView blk,
                                                       SpinorView aux, double kappa, double mu, ClovArgs k)
 {
+  using R = typename std::conditional<SP == SP_F64, double, float>::type;
+  typedef cx<R> CR;
   const uint32_t e = blockIdx.x * 256 + threadIdx.x;
   if (e >= (uint32_t)g.Vh) return;
-  C v[12], o[12];
-  SpAcc<SP_F64, double, 12>::load(in, g.stride, e, v);
-  const C* B = (const C*)blk.data;
+  CR v[12], o[12];
+  SpAcc<SP, R, 12>::load(in, g.stride, e, v);
+  const CR* B = (const CR*)blk.data;  // FP64 or FP32 complex per component and site
   const int64_t st = g.stride;
 #pragma unroll
   for (int h = 0; h < 2; h++) {
-    const C* iv = v + 6 * h;
-    C* ov = o + 6 * h;
+    const CR* iv = v + 6 * h;
+    CR* ov = o + 6 * h;
     if constexpr (MODE == 0) {
       const double m = (DAG ? -mu : mu) * (h == 0 ? 1.0 : -1.0);
-      const C z = mk<double>(1 / (2 * kappa), m);
-      C c0[9], c1[9];
+      const CR z = mk<R>((R)(1 / (2 * kappa)), (R)m);
+      CR c0[9], c1[9];
 #pragma unroll
       for (int q = 0; q < 9; q++) {
         c0[q] = B[(18 * h + q) * st + e];
@@ -266,7 +268,7 @@ __global__ void __launch_bounds__(256) k_clover_apply(Geo g, SpinorView out, Spi
       }
 #pragma unroll
       for (int a = 0; a < 3; a++) {
-        C s0 = z * iv[a], s1 = z * iv[3 + a];
+        CR s0 = z * iv[a], s1 = z * iv[3 + a];
 #pragma unroll
         for (int b = 0; b < 3; b++) {
           s0 = s0 + c0[3 * a + b] * iv[b] + conjg(c1[3 * b + a]) * iv[3 + b];
@@ -278,12 +280,12 @@ __global__ void __launch_bounds__(256) k_clover_apply(Geo g, SpinorView out, Spi
     } else {
       constexpr bool hd0 = DAG != 0;  // chirality 0 uses the daggered block for the dagger
       const bool hd = h == 0 ? hd0 : !hd0;
-      C M[36];
+      CR M[36];
 #pragma unroll
       for (int q = 0; q < 36; q++) M[q] = B[(36 * h + q) * st + e];
 #pragma unroll
       for (int r = 0; r < 6; r++) {
-        C sum = mk<double>(0, 0);
+        CR sum = mk<R>(0, 0);
 #pragma unroll
         for (int c = 0; c < 6; c++) sum = sum + (hd ? conjg(M[6 * c + r]) : M[6 * r + c]) * iv[c];
         ov[r] = sum;
@@ -293,21 +295,32 @@ __global__ void __launch_bounds__(256) k_clover_apply(Geo g, SpinorView out, Spi
     for (int q = 0; q < 6; q++) asm volatile("" : "+v"(ov[q].re), "+v"(ov[q].im));
   }
   if (aux.data) {
-    C w[12];
-    SpAcc<SP_F64, double, 12>::load(aux, g.stride, e, w);
+    CR w[12];
+    SpAcc<SP, R, 12>::load(aux, g.stride, e, w);
 #pragma unroll
     for (int q = 0; q < 12; q++) {
       const int h = q < 6 ? 0 : 1;
-      o[q] = mk<double>(k.a[h][0], k.a[h][1]) * o[q] + mk<double>(k.b[h][0], k.b[h][1]) * w[q];
+      o[q] = mk<R>((R)k.a[h][0], (R)k.a[h][1]) * o[q] + mk<R>((R)k.b[h][0], (R)k.b[h][1]) * w[q];
     }
   } else {
 #pragma unroll
     for (int q = 0; q < 12; q++) {
       const int h = q < 6 ? 0 : 1;
-      o[q] = mk<double>(k.a[h][0], k.a[h][1]) * o[q];
+      o[q] = mk<R>((R)k.a[h][0], (R)k.a[h][1]) * o[q];
     }
   }
-  SpAcc<SP_F64, double, 12>::store(out, g.stride, e, o);
+  SpAcc<SP, R, 12>::store(out, g.stride, e, o);
+}
+
+// FP32 copy of an FP64 block field (sloppy solves)
+__global__ void __launch_bounds__(256) k_blocks_to_f32(Geo g, int ncomp, const C* in, cx<float>* out)
+{
+  const uint32_t e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= (uint32_t)g.Vh) return;
+  for (int k = 0; k < ncomp; k++) {
+    const C v = in[k * g.stride + e];
+    out[k * g.stride + e] = mk<float>((float)v.re, (float)v.im);
+  }
 }
 
 }  // namespace
@@ -339,13 +352,24 @@ int launch_clover_invert(nq_ctx* ctx, const SpinorView& cl, const SpinorView& in
   return NQ_OK;
 }
 
+int launch_blocks_to_f32(nq_ctx* ctx, int ncomp, const SpinorView& in, const SpinorView& out)
+{
+  const Geo& g = ctx->geo;
+  if (in.prec != SP_F64 || out.prec != SP_F32) return fail(NQ_EINVAL, "block conversion: FP64 -> FP32 only");
+  hipLaunchKernelGGL(k_blocks_to_f32, dim3((unsigned)((g.Vh + 255) / 256)), dim3(256), 0, ctx->stream, g, ncomp,
+                     (const C*)in.data, (cx<float>*)out.data);
+  NQ_HIP(hipGetLastError());
+  return NQ_OK;
+}
+
 int launch_clover_apply(nq_ctx* ctx, int mode, const SpinorView& out, const SpinorView& in, const SpinorView& blk,
                         const SpinorView& aux, double kappa, double mu, int dag, const double a[2][2],
                         const double b[2][2])
 {
   const Geo& g = ctx->geo;
-  if (in.prec != SP_F64 || out.prec != SP_F64 || (aux.data && aux.prec != SP_F64))
-    return fail(NQ_EINVAL, "twisted clover: FP64 fields only");
+  const int sp = in.prec;
+  if ((sp != SP_F64 && sp != SP_F32) || out.prec != sp || blk.prec != sp || (aux.data && aux.prec != sp))
+    return fail(NQ_EINVAL, "twisted clover: FP64 or FP32 fields and blocks of one precision");
   ClovArgs k;
   for (int h = 0; h < 2; h++)
     for (int c = 0; c < 2; c++) {
@@ -353,13 +377,20 @@ int launch_clover_apply(nq_ctx* ctx, int mode, const SpinorView& out, const Spin
       k.b[h][c] = b ? b[h][c] : 0.0;
     }
   hipEvent_t ev = nullptr;
-  const double bytes = (double)g.Vh * ((mode == 0 ? 36 : 72) * 16.0 + 2 * 192.0 + (aux.data ? 192.0 : 0.0));
-  prof_begin(ctx, mode == 0 ? "clover_dee" : "clover_inv", bytes, &ev);
+  const double cb = sp == SP_F64 ? 16.0 : 8.0;  // bytes per complex
+  const double bytes = (double)g.Vh * ((mode == 0 ? 36 : 72) * cb + (2 + (aux.data ? 1 : 0)) * 12 * cb);
+  prof_begin(ctx, sp == SP_F64 ? (mode == 0 ? "clover_dee" : "clover_inv") : (mode == 0 ? "clover_dee_f32" : "clover_inv_f32"),
+             bytes, &ev);
   const dim3 grid((unsigned)((g.Vh + 255) / 256)), blkd(256);
-  if (mode == 0 && !dag) hipLaunchKernelGGL((k_clover_apply<0, 0>), grid, blkd, 0, ctx->stream, g, out, in, blk, aux, kappa, mu, k);
-  else if (mode == 0) hipLaunchKernelGGL((k_clover_apply<0, 1>), grid, blkd, 0, ctx->stream, g, out, in, blk, aux, kappa, mu, k);
-  else if (!dag) hipLaunchKernelGGL((k_clover_apply<1, 0>), grid, blkd, 0, ctx->stream, g, out, in, blk, aux, kappa, mu, k);
-  else hipLaunchKernelGGL((k_clover_apply<1, 1>), grid, blkd, 0, ctx->stream, g, out, in, blk, aux, kappa, mu, k);
+#define NQ_CLOV(M, D, S) hipLaunchKernelGGL((k_clover_apply<M, D, S>), grid, blkd, 0, ctx->stream, g, out, in, blk, aux, kappa, mu, k)
+  if (sp == SP_F64) {
+    if (mode == 0) { if (dag) NQ_CLOV(0, 1, SP_F64); else NQ_CLOV(0, 0, SP_F64); }
+    else { if (dag) NQ_CLOV(1, 1, SP_F64); else NQ_CLOV(1, 0, SP_F64); }
+  } else {
+    if (mode == 0) { if (dag) NQ_CLOV(0, 1, SP_F32); else NQ_CLOV(0, 0, SP_F32); }
+    else { if (dag) NQ_CLOV(1, 1, SP_F32); else NQ_CLOV(1, 0, SP_F32); }
+  }
+#undef NQ_CLOV
   NQ_HIP(hipGetLastError());
   prof_end(ctx, ev);
   return NQ_OK;

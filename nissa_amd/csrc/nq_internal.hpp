@@ -209,6 +209,9 @@ struct nq_clover {
   nq::DevBuf inv[2];
   double inv_kappa[2] = {0, 0}, inv_mu[2] = {0, 0};
   int inv_valid[2] = {0, 0};
+  // FP32 copies for sloppy solves (inv32 follows inv)
+  nq::DevBuf cl32[2], inv32[2];
+  int cl32_valid[2] = {0, 0}, inv32_valid[2] = {0, 0};
 };
 
 struct nq_gauge {
@@ -309,6 +312,7 @@ int launch_st_force(nq_ctx* ctx, int par, const SpinorView& a, const SpinorView&
 int launch_clover_build(nq_ctx* ctx, const double* aos_e, const double* aos_o, int par, double csw,
                         const SpinorView& out);
 int launch_clover_invert(nq_ctx* ctx, const SpinorView& cl, const SpinorView& inv, double kappa, double mass);
+int launch_blocks_to_f32(nq_ctx* ctx, int ncomp, const SpinorView& in, const SpinorView& out);
 int launch_clover_apply(nq_ctx* ctx, int mode, const SpinorView& out, const SpinorView& in, const SpinorView& blk,
                         const SpinorView& aux, double kappa, double mu, int dag, const double a[2][2],
                         const double b[2][2]);

@@ -1487,6 +1487,37 @@ static int clov_inv_view(nq_clover* cl, int par, double kappa, double mu, Spinor
     cl->inv_valid[par] = 1;
     cl->inv_kappa[par] = kappa;
     cl->inv_mu[par] = mu;
+    cl->inv32_valid[par] = 0;
+  }
+  return NQ_OK;
+}
+
+// FP32 copies of the clover blocks and of the inverse blocks (for (kappa, mu))
+static int clov_view32(nq_clover* cl, int par, SpinorView* v)
+{
+  nq_ctx* c = cl->ctx;
+  if (!cl->cl32[par].p) NQ_TRY(dev_alloc(cl->cl32[par], (size_t)36 * 8 * c->geo.stride));
+  v->data = cl->cl32[par].p;
+  v->prec = SP_F32;
+  v->ncomp = 36;
+  if (!cl->cl32_valid[par]) {
+    NQ_TRY(launch_blocks_to_f32(c, 36, clov_view(cl, par), *v));
+    cl->cl32_valid[par] = 1;
+  }
+  return NQ_OK;
+}
+static int clov_inv_view32(nq_clover* cl, int par, double kappa, double mu, SpinorView* v)
+{
+  nq_ctx* c = cl->ctx;
+  SpinorView v64;
+  NQ_TRY(clov_inv_view(cl, par, kappa, mu, &v64));
+  if (!cl->inv32[par].p) NQ_TRY(dev_alloc(cl->inv32[par], (size_t)72 * 8 * c->geo.stride));
+  v->data = cl->inv32[par].p;
+  v->prec = SP_F32;
+  v->ncomp = 72;
+  if (!cl->inv32_valid[par]) {
+    NQ_TRY(launch_blocks_to_f32(c, 72, v64, *v));
+    cl->inv32_valid[par] = 1;
   }
   return NQ_OK;
 }
@@ -1495,6 +1526,7 @@ struct ClovSetup {
   TmSetup S;
   nq_clover* cl;
   SpinorView inv_e;  // inverse blocks of the even sites for (kappa, mu)
+  SpinorView cl_o;   // clover blocks of the odd sites
   double mu;
   SpinorView te, te2, uo;  // scratch: two even, one odd
 };
@@ -1507,9 +1539,25 @@ static int clov_setup(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa,
   s->cl = cl;
   s->mu = mu;
   NQ_TRY(clov_inv_view(cl, NQ_EVEN, kappa, mu, &s->inv_e));
+  s->cl_o = clov_view(cl, NQ_ODD);
   NQ_TRY(alloc_spinor(c, "clov.te", SP_F64, 12, &s->te));
   NQ_TRY(alloc_spinor(c, "clov.te2", SP_F64, 12, &s->te2));
   NQ_TRY(alloc_spinor(c, "clov.uo", SP_F64, 12, &s->uo));
+  return NQ_OK;
+}
+
+// the same operator in FP32 (FP32 18-real links, FP32 blocks and spinors)
+static int clov_setup32(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis, double mu,
+                        ClovSetup* s)
+{
+  NQ_TRY(tm_setup(c, g, LK_F32_18, anis, kappa, &s->S));
+  s->cl = cl;
+  s->mu = mu;
+  NQ_TRY(clov_inv_view32(cl, NQ_EVEN, kappa, mu, &s->inv_e));
+  NQ_TRY(clov_view32(cl, NQ_ODD, &s->cl_o));
+  NQ_TRY(alloc_spinor(c, "clov.te32", SP_F32, 12, &s->te));
+  NQ_TRY(alloc_spinor(c, "clov.te2_32", SP_F32, 12, &s->te2));
+  NQ_TRY(alloc_spinor(c, "clov.uo32", SP_F32, 12, &s->uo));
   return NQ_OK;
 }
 
@@ -1525,7 +1573,7 @@ static int clov_kern(const ClovSetup& s, int dag, const SpinorView& out, const S
   HopEpi h;
   NQ_TRY(launch_tm_hop(c, NQ_EVEN, s.te, in, s.S.U, s.S.anis, h));
   NQ_TRY(launch_clover_apply(c, 1, s.te2, s.te, s.inv_e, none, s.S.kappa, s.mu, dag, kOne, nullptr));
-  NQ_TRY(launch_clover_apply(c, 0, s.uo, in, clov_view(s.cl, NQ_ODD), none, s.S.kappa, s.mu, dag, kOne, nullptr));
+  NQ_TRY(launch_clover_apply(c, 0, s.uo, in, s.cl_o, none, s.S.kappa, s.mu, dag, kOne, nullptr));
   setc(last.c1, -0.25, 0.25);
   setc(last.c2, 1.0, -1.0);
   last.aux = s.uo;
@@ -1609,6 +1657,111 @@ static int clov_cg(const ClovSetup& s, int niter, double residue, const SpinorVi
   return NQ_OK;
 }
 
+// Mixed precision: the CG iterates on the FP32 operator (s32) with the FP32 x/r of the
+// tm mixed solver and FP64 reliable updates every delta = 1e-2 drop of |r| (x64 += xs +
+// alpha p; r64 = b - K K^+ x64 on the FP64 operator s64; rs = r64; p = rs + beta p).
+static int clov_cg_mixed(const ClovSetup& s64, const ClovSetup& s32, int niter, double residue, const SpinorView& x64,
+                         bool use_guess, const SpinorView& b, nq_solve_stats* out)
+{
+  nq_ctx* ctx = s64.S.ctx;
+  SpinorView r64, t1, sv, xs, rs, p, t1s;
+  NQ_TRY(alloc_spinor(ctx, "clov.r", SP_F64, 12, &r64));
+  NQ_TRY(alloc_spinor(ctx, "clov.t1", SP_F64, 12, &t1));
+  NQ_TRY(alloc_spinor(ctx, "clov.s", SP_F64, 12, &sv));
+  NQ_TRY(alloc_spinor(ctx, "clov.xs32", SP_F32, 12, &xs));
+  NQ_TRY(alloc_spinor(ctx, "clov.rs32", SP_F32, 12, &rs));
+  NQ_TRY(alloc_spinor(ctx, "clov.p32", SP_F32, 12, &p));
+  NQ_TRY(alloc_spinor(ctx, "clov.t1s32", SP_F32, 12, &t1s));
+  Timer T(ctx->stream);
+  double b2 = 0;
+  NQ_TRY(reduce_norm2(ctx, b, &b2));
+  if (b2 == 0 || std::isnan(b2)) return fail(NQ_ENAN, "invalid norm: %g", b2);
+  CgState s0 = {};
+  s0.b2 = b2;
+  s0.target = residue * b2;
+  s0.niter = niter;
+  s0.mixed = 1;
+  s0.delta = 1e-2;
+  NQ_TRY(put_state(ctx, 0, s0));
+  CgState* st = ctx->state;
+  SpinorView none;
+  none.ncomp = 12;
+  auto residual = [&](int fin) -> int {
+    NQ_TRY(clov_kern(s64, 1, t1, x64, HopEpi()));
+    NQ_TRY(clov_kern(s64, 0, sv, t1, HopEpi()));
+    return launch_lincomb(ctx, r64, 1.0, b, -1.0, sv, 0, none, fin, st);
+  };
+  if (use_guess) {
+    NQ_TRY(residual(FIN_RR_INIT));
+  } else {
+    NQ_TRY(launch_zero(ctx, x64));
+    NQ_TRY(launch_lincomb(ctx, r64, 1.0, b, 0, none, 0, none, FIN_RR_INIT, st));
+  }
+  NQ_TRY(launch_convert(ctx, rs, r64));
+  NQ_TRY(launch_convert(ctx, p, rs));
+  NQ_TRY(launch_zero(ctx, xs));
+  NQ_TRY(get_states(ctx, 0, 1));
+  const double rr_init = ctx->state_host[0].rr;
+  int chunk = 8, launched = 0;
+  while (!ctx->state_host[0].done) {
+    for (int i = 0; i < chunk; i++) {
+      HopEpi e1;
+      e1.mode = 1;
+      e1.st = st;
+      e1.fin = FIN_PAP;
+      e1.pred_reliable = 2;
+      NQ_TRY(clov_kern(s32, 1, t1s, p, e1));
+      HopEpi e2;
+      e2.mode = 2;
+      e2.st = st;
+      e2.fin = FIN_RR_SLOPPY;
+      e2.pred_reliable = 2;
+      e2.p = p;
+      e2.r = rs;
+      NQ_TRY(clov_kern(s32, 0, none, t1s, e2));
+      NQ_TRY(launch_cg_pupdate(ctx, p, rs, st, 1, 1, xs));
+    }
+    launched += chunk;
+    NQ_TRY(get_states(ctx, 0, 1));
+    if (ctx->state_host[0].reliable && !ctx->state_host[0].done) {
+      NQ_TRY(launch_reliable_accum(ctx, x64, xs, st, 1, p));
+      NQ_TRY(residual(FIN_RR_TRUE));
+      NQ_TRY(launch_convert(ctx, rs, r64));
+      NQ_TRY(launch_cg_pupdate(ctx, p, rs, st));
+      NQ_TRY(get_states(ctx, 0, 1));
+    }
+    const CgState& h = ctx->state_host[0];
+    if (h.nan || h.done) break;
+    const double goal = std::max(h.delta * h.delta * h.r0 * h.r0, h.target);
+    const double slope = h.iter > 0 && h.rr < rr_init ? std::log(h.rr / rr_init) / h.iter : -0.3;
+    const int need = (int)std::ceil(std::log(goal / h.rr) / slope);
+    chunk = std::max(1, std::min(32, need));
+    if (launched > niter + 64) break;
+  }
+  const CgState h = ctx->state_host[0];
+  if (h.nan) return fail(NQ_ENAN, "nanned");
+  out->iters = h.iter;
+  out->reliable_updates = h.nreliable;
+  out->rel_res_sq = h.true_rr / b2;
+  out->converged = h.true_rr < h.target;
+  out->seconds = T.stop();
+  if (!out->converged) return fail(NQ_ENOCONV, "exit without converging (%d iterations, rel res %g)", h.iter, h.true_rr / b2);
+  return NQ_OK;
+}
+
+// CG on K K^+ in the requested precision (NQ_PREC_F64, or NQ_PREC_F32 with FP64
+// reliable updates)
+static int clov_cg_any(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis, double mu,
+                       int prec, const ClovSetup& s64, int niter, double residue, const SpinorView& x, bool use_guess,
+                       const SpinorView& b, nq_solve_stats* st)
+{
+  if (prec == NQ_PREC_F64) return clov_cg(s64, niter, residue, x, use_guess, b, st);
+  if (prec != NQ_PREC_F32) return fail(NQ_EINVAL, "twisted clover solves take NQ_PREC_F64 or NQ_PREC_F32");
+  ClovSetup s32;
+  NQ_TRY(clov_setup32(c, g, cl, kappa, anis, mu, &s32));
+  return clov_cg_mixed(s64, s32, niter, residue, x, use_guess, b, st);
+}
+
 }  // namespace nq
 
 extern "C" {
@@ -1660,6 +1813,8 @@ void nq_clover_destroy(nq_clover* cl)
     for (int p = 0; p < 2; p++) {
       dev_free(cl->cl[p]);
       dev_free(cl->inv[p]);
+      dev_free(cl->cl32[p]);
+      dev_free(cl->inv32[p]);
     }
   }
   delete cl;
@@ -1760,7 +1915,7 @@ int nq_tmclov_apply_Q(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa,
 }
 
 int nq_tmclov_cg_kernsq(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis, double mu,
-                        int niter, double residue, nq_field* sol, const nq_field* guess, const nq_field* src,
+                        int niter, double residue, int prec, nq_field* sol, const nq_field* guess, const nq_field* src,
                         nq_solve_stats* st)
 {
   if (!c) return fail(NQ_EINVAL, "null ctx");
@@ -1776,14 +1931,15 @@ int nq_tmclov_cg_kernsq(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kapp
   if (!st) st = &local;
   std::memset(st, 0, sizeof *st);
   if (guess && guess != sol) NQ_TRY(launch_convert(c, view_of(sol, 0), view_of(guess, 0)));
-  const int rc = clov_cg(s, niter, residue, view_of(sol, 0), guess != nullptr, view_of(src, 0), st);
+  const int rc = clov_cg_any(c, g, cl, kappa, anis, mu, prec, s, niter, residue, view_of(sol, 0), guess != nullptr,
+                             view_of(src, 0), st);
   NQ_HIP(hipStreamSynchronize(c->stream));
   return rc;
 }
 
 // inv_tmclovD_cg_eoprec (cg_invert_tmclovD_eoprec.cpp:56-128 and the check of :179-232)
 int nq_tmclov_solve_eoprec(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis, double mu,
-                           int niter, double residue, nq_field* sol, const nq_field* src, nq_solve_stats* st,
+                           int niter, double residue, int prec, nq_field* sol, const nq_field* src, nq_solve_stats* st,
                            double* check)
 {
   if (!c) return fail(NQ_EINVAL, "null ctx");
@@ -1812,7 +1968,7 @@ int nq_tmclov_solve_eoprec(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double k
   nq_solve_stats local;
   if (!st) st = &local;
   std::memset(st, 0, sizeof *st);
-  int rc = clov_cg(s, niter, residue, y, false, phi, st);
+  int rc = clov_cg_any(c, g, cl, kappa, anis, mu, prec, s, niter, residue, y, false, phi, st);
   if (rc != NQ_OK && rc != NQ_ENOCONV) return rc;
   // eq. 10: sol_o = K(dag) y
   NQ_TRY(clov_kern(s, 1, xo, y, HopEpi()));
