@@ -83,6 +83,24 @@ def test_tmclov_kern_square_and_Q(setups, L):
     assert rel_l2(fq.download(), geo.apply_tmclovQ(conf, KAPPA, ref, mu, psi)) < 1e-14
 
 
+def test_tmclov_with_anisotropy(setups):
+    """The anisotropy factor on the spatial hops (optional<double> anis of nissa's
+    tmclovDkern_eoprec_eos and apply_tmclovQ)."""
+    L = (4, 6, 4, 8)
+    ctx, g, cl, geo, conf, ce, co, ref = setups(L)
+    cle, clo = geo.split(ref)
+    mu, anis = 0.04, 0.83
+    inv = geo.tmclov_invert(cle, KAPPA, mu)
+    x = gauss_eo(geo.Vh, 17)
+    fx, fo = ctx.field(nq.SPINCOLOR, nq.ODD, x), ctx.field(nq.SPINCOLOR, nq.ODD)
+    nq.tmclov_kern(ctx, g, cl, KAPPA, mu, fo, fx, anis=anis)
+    assert rel_l2(fo.download(), geo.tmclovDkern(ce, co, KAPPA, clo, inv, mu, x, anis=anis)) < 1e-13
+    psi = nq.source_lx(L, 4, "gauss")
+    fl, fq = ctx.field(nq.SPINCOLOR, nq.LX, psi), ctx.field(nq.SPINCOLOR, nq.LX)
+    nq.tmclov_apply_Q(ctx, g, cl, KAPPA, mu, fq, fl, anis=anis)
+    assert rel_l2(fq.download(), geo.apply_tmclovQ(conf, KAPPA, ref, mu, psi, anis=anis)) < 1e-14
+
+
 def test_tmclov_cg_matches_oracle(setups):
     L = (4, 6, 4, 8)
     ctx, g, cl, geo, conf, ce, co, ref = setups(L)
@@ -122,3 +140,24 @@ def test_tmclov_eo_solve_matches_oracle(setups, L, prec):
         assert abs(st["iters"] - rst["iters"]) <= 2
     else:
         assert st["reliable_updates"] >= 1
+
+
+def test_tmclov_errors_are_reported(setups):
+    """Non-convergence comes back as NQ_ENOCONV with the stats filled, a NaN source as
+    NQ_ENAN, a field of the wrong parity as NQ_EINVAL (nissa CRASHes in all three)."""
+    L = (4, 6, 4, 8)
+    ctx, g, cl, geo, conf, ce, co, ref = setups(L)
+    fb = ctx.field(nq.SPINCOLOR, nq.ODD, gauss_eo(geo.Vh, 21))
+    fx = ctx.field(nq.SPINCOLOR, nq.ODD)
+    st = nq.tmclov_cg_kernsq(ctx, g, cl, KAPPA, 0.01, 5, 1e-24, fx, fb)
+    assert st["rc"] == nq.NQ_ENOCONV and st["iters"] == 5 and not st["converged"]
+    bad = gauss_eo(geo.Vh, 22)
+    bad[3, 1, 2, 0] = np.nan
+    fn = ctx.field(nq.SPINCOLOR, nq.ODD, bad)
+    with pytest.raises(nq.NqError) as e:
+        nq.tmclov_cg_kernsq(ctx, g, cl, KAPPA, 0.01, 100, 1e-24, fx, fn)
+    assert e.value.code == nq.NQ_ENAN
+    fe = ctx.field(nq.SPINCOLOR, nq.EVEN)
+    with pytest.raises(nq.NqError) as e:
+        nq.tmclov_kern(ctx, g, cl, KAPPA, 0.01, fe, fb)
+    assert e.value.code == nq.NQ_EINVAL
