@@ -285,12 +285,14 @@ def clover_bench():
     st = nq.tmclov_solve_eoprec(ctx, g, cl, kappa, mu, 100000, residue, fx, fs)
     ctx.sync()
     t = time.perf_counter() - t0
-    nq.tmclov_solve_eoprec(ctx, g, cl, kappa, mu, 100000, residue, fx, fs, prec="f32")
-    ctx.sync()
-    t0 = time.perf_counter()
-    st32 = nq.tmclov_solve_eoprec(ctx, g, cl, kappa, mu, 100000, residue, fx, fs, prec="f32")
-    ctx.sync()
-    t32 = time.perf_counter() - t0
+    sloppy = {}
+    for pv in ("f32", "f16"):
+        nq.tmclov_solve_eoprec(ctx, g, cl, kappa, mu, 100000, residue, fx, fs, prec=pv)
+        ctx.sync()
+        t0 = time.perf_counter()
+        sv = nq.tmclov_solve_eoprec(ctx, g, cl, kappa, mu, 100000, residue, fx, fs, prec=pv)
+        ctx.sync()
+        sloppy[pv] = (time.perf_counter() - t0, sv)
     ctx.reset_kernel_stats()
     ctx.profiling(True)
     nq.tmclov_solve_eoprec(ctx, g, cl, kappa, mu, 100000, residue, fx, fs)
@@ -308,13 +310,16 @@ def clover_bench():
     ctx.close()
     out = {"lattice": list(L), "kappa": kappa, "csw": csw, "mu": mu, "clover_build_ms": round(t_build * 1e3, 2),
            "fp64_solve_ms": round(t * 1e3, 2), "iters": st["iters"], "true_rel_res_sq": st["check_rel_res_sq"],
-           "f32_inner_solve_ms": round(t32 * 1e3, 2), "f32_inner_iters": st32["iters"],
-           "f32_inner_reliable_updates": st32["reliable_updates"], "f32_inner_true_rel_res_sq": st32["check_rel_res_sq"],
-           "ok": bool(st["rc"] == 0 and st["check_rel_res_sq"] < 2 * residue and st32["rc"] == 0
-                      and st32["check_rel_res_sq"] < 2 * residue), "ledger": ledger}
+           "inner": {pv: {"solve_ms": round(tv * 1e3, 2), "iters": sv["iters"],
+                          "reliable_updates": sv["reliable_updates"], "true_rel_res_sq": sv["check_rel_res_sq"],
+                          "speedup_vs_fp64": round(t / tv, 3)} for pv, (tv, sv) in sloppy.items()},
+           "ok": bool(st["rc"] == 0 and st["check_rel_res_sq"] < 2 * residue
+                      and all(sv["rc"] == 0 and sv["check_rel_res_sq"] < 2 * residue for _, sv in sloppy.values())),
+           "ledger": ledger}
     log(f"[bench] twisted clover 32^3x64 csw {csw}: build {t_build*1e3:.1f} ms, FP64 solve {t*1e3:.1f} ms "
-        f"in {st['iters']} iterations, check {st['check_rel_res_sq']:.2e}; FP32 inner {t32*1e3:.1f} ms "
-        f"({st32['iters']} iterations, {st32['reliable_updates']} reliable updates)")
+        f"in {st['iters']} iterations, check {st['check_rel_res_sq']:.2e}; "
+        + "; ".join(f"{pv} inner {tv*1e3:.1f} ms ({sv['iters']} it, {sv['reliable_updates']} reliable)"
+                    for pv, (tv, sv) in sloppy.items()))
     for k, v in ledger.items():
         log(f"[bench]   {k:24s} {v}")
     return out

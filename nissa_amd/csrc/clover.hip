@@ -246,7 +246,7 @@ __global__ void __launch_bounds__(256) k_clover_apply(Geo g, SpinorView out, Spi
                                                       SpinorView aux, double kappa, double mu, ClovArgs k)
 {
   using R = typename std::conditional<SP == SP_F64, double, float>::type;
-  typedef cx<R> CR;
+  typedef cx<R> CR;  // FP16 spinors come with FP32 blocks
   const uint32_t e = blockIdx.x * 256 + threadIdx.x;
   if (e >= (uint32_t)g.Vh) return;
   CR v[12], o[12];
@@ -368,8 +368,9 @@ int launch_clover_apply(nq_ctx* ctx, int mode, const SpinorView& out, const Spin
 {
   const Geo& g = ctx->geo;
   const int sp = in.prec;
-  if ((sp != SP_F64 && sp != SP_F32) || out.prec != sp || blk.prec != sp || (aux.data && aux.prec != sp))
-    return fail(NQ_EINVAL, "twisted clover: FP64 or FP32 fields and blocks of one precision");
+  const int bp = sp == SP_F16 ? SP_F32 : sp;  // block precision
+  if (out.prec != sp || blk.prec != bp || (aux.data && aux.prec != sp))
+    return fail(NQ_EINVAL, "twisted clover: FP64/FP32/FP16 fields with FP64/FP32/FP32 blocks");
   ClovArgs k;
   for (int h = 0; h < 2; h++)
     for (int c = 0; c < 2; c++) {
@@ -377,18 +378,25 @@ int launch_clover_apply(nq_ctx* ctx, int mode, const SpinorView& out, const Spin
       k.b[h][c] = b ? b[h][c] : 0.0;
     }
   hipEvent_t ev = nullptr;
-  const double cb = sp == SP_F64 ? 16.0 : 8.0;  // bytes per complex
-  const double bytes = (double)g.Vh * ((mode == 0 ? 36 : 72) * cb + (2 + (aux.data ? 1 : 0)) * 12 * cb);
-  prof_begin(ctx, sp == SP_F64 ? (mode == 0 ? "clover_dee" : "clover_inv") : (mode == 0 ? "clover_dee_f32" : "clover_inv_f32"),
-             bytes, &ev);
+  const double cb = bp == SP_F64 ? 16.0 : 8.0;                        // bytes per block complex
+  const double sb = sp == SP_F64 ? 192.0 : (sp == SP_F32 ? 96.0 : 52.0);  // bytes per spinor site
+  const double bytes = (double)g.Vh * ((mode == 0 ? 36 : 72) * cb + (2 + (aux.data ? 1 : 0)) * sb);
+  const char* names[3][2] = {{"clover_dee", "clover_inv"}, {"clover_dee_f32", "clover_inv_f32"},
+                             {"clover_dee_f16", "clover_inv_f16"}};
+  prof_begin(ctx, names[sp == SP_F64 ? 0 : (sp == SP_F32 ? 1 : 2)][mode == 0 ? 0 : 1], bytes, &ev);
   const dim3 grid((unsigned)((g.Vh + 255) / 256)), blkd(256);
 #define NQ_CLOV(M, D, S) hipLaunchKernelGGL((k_clover_apply<M, D, S>), grid, blkd, 0, ctx->stream, g, out, in, blk, aux, kappa, mu, k)
   if (sp == SP_F64) {
     if (mode == 0) { if (dag) NQ_CLOV(0, 1, SP_F64); else NQ_CLOV(0, 0, SP_F64); }
     else { if (dag) NQ_CLOV(1, 1, SP_F64); else NQ_CLOV(1, 0, SP_F64); }
-  } else {
+  } else if (sp == SP_F32) {
     if (mode == 0) { if (dag) NQ_CLOV(0, 1, SP_F32); else NQ_CLOV(0, 0, SP_F32); }
     else { if (dag) NQ_CLOV(1, 1, SP_F32); else NQ_CLOV(1, 0, SP_F32); }
+  } else if (sp == SP_F16) {
+    if (mode == 0) { if (dag) NQ_CLOV(0, 1, SP_F16); else NQ_CLOV(0, 0, SP_F16); }
+    else { if (dag) NQ_CLOV(1, 1, SP_F16); else NQ_CLOV(1, 0, SP_F16); }
+  } else {
+    return fail(NQ_EINVAL, "twisted clover: unknown spinor precision");
   }
 #undef NQ_CLOV
   NQ_HIP(hipGetLastError());

@@ -1546,18 +1546,20 @@ static int clov_setup(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa,
   return NQ_OK;
 }
 
-// the same operator in FP32 (FP32 18-real links, FP32 blocks and spinors)
-static int clov_setup32(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis, double mu,
-                        ClovSetup* s)
+// the same operator in a sloppy precision: FP32 (FP32 18-real links and FP32 spinors)
+// or FP16 (FP16 spinors with per-site norms and FP32 12-real links), FP32 blocks
+static int clov_setup_sloppy(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis,
+                             double mu, int sp, ClovSetup* s)
 {
-  NQ_TRY(tm_setup(c, g, LK_F32_18, anis, kappa, &s->S));
+  NQ_TRY(tm_setup(c, g, sp == SP_F16 ? LK_F32_12 : LK_F32_18, anis, kappa, &s->S));
   s->cl = cl;
   s->mu = mu;
   NQ_TRY(clov_inv_view32(cl, NQ_EVEN, kappa, mu, &s->inv_e));
   NQ_TRY(clov_view32(cl, NQ_ODD, &s->cl_o));
-  NQ_TRY(alloc_spinor(c, "clov.te32", SP_F32, 12, &s->te));
-  NQ_TRY(alloc_spinor(c, "clov.te2_32", SP_F32, 12, &s->te2));
-  NQ_TRY(alloc_spinor(c, "clov.uo32", SP_F32, 12, &s->uo));
+  const bool h = sp == SP_F16;
+  NQ_TRY(alloc_spinor(c, h ? "clov.te16" : "clov.te32", sp, 12, &s->te));
+  NQ_TRY(alloc_spinor(c, h ? "clov.te2_16" : "clov.te2_32", sp, 12, &s->te2));
+  NQ_TRY(alloc_spinor(c, h ? "clov.uo16" : "clov.uo32", sp, 12, &s->uo));
   return NQ_OK;
 }
 
@@ -1660,8 +1662,8 @@ static int clov_cg(const ClovSetup& s, int niter, double residue, const SpinorVi
 // Mixed precision: the CG iterates on the FP32 operator (s32) with the FP32 x/r of the
 // tm mixed solver and FP64 reliable updates every delta = 1e-2 drop of |r| (x64 += xs +
 // alpha p; r64 = b - K K^+ x64 on the FP64 operator s64; rs = r64; p = rs + beta p).
-static int clov_cg_mixed(const ClovSetup& s64, const ClovSetup& s32, int niter, double residue, const SpinorView& x64,
-                         bool use_guess, const SpinorView& b, nq_solve_stats* out)
+static int clov_cg_mixed(const ClovSetup& s64, const ClovSetup& s32, int sp, int niter, double residue,
+                         const SpinorView& x64, bool use_guess, const SpinorView& b, nq_solve_stats* out)
 {
   nq_ctx* ctx = s64.S.ctx;
   SpinorView r64, t1, sv, xs, rs, p, t1s;
@@ -1670,8 +1672,8 @@ static int clov_cg_mixed(const ClovSetup& s64, const ClovSetup& s32, int niter, 
   NQ_TRY(alloc_spinor(ctx, "clov.s", SP_F64, 12, &sv));
   NQ_TRY(alloc_spinor(ctx, "clov.xs32", SP_F32, 12, &xs));
   NQ_TRY(alloc_spinor(ctx, "clov.rs32", SP_F32, 12, &rs));
-  NQ_TRY(alloc_spinor(ctx, "clov.p32", SP_F32, 12, &p));
-  NQ_TRY(alloc_spinor(ctx, "clov.t1s32", SP_F32, 12, &t1s));
+  NQ_TRY(alloc_spinor(ctx, sp == SP_F16 ? "clov.p16" : "clov.p32", sp, 12, &p));
+  NQ_TRY(alloc_spinor(ctx, sp == SP_F16 ? "clov.t1s16" : "clov.t1s32", sp, 12, &t1s));
   Timer T(ctx->stream);
   double b2 = 0;
   NQ_TRY(reduce_norm2(ctx, b, &b2));
@@ -1749,17 +1751,19 @@ static int clov_cg_mixed(const ClovSetup& s64, const ClovSetup& s32, int niter, 
   return NQ_OK;
 }
 
-// CG on K K^+ in the requested precision (NQ_PREC_F64, or NQ_PREC_F32 with FP64
-// reliable updates)
+// CG on K K^+ in the requested precision (NQ_PREC_F64, or NQ_PREC_F32 / NQ_PREC_F16
+// inner iterations with FP64 reliable updates)
 static int clov_cg_any(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis, double mu,
                        int prec, const ClovSetup& s64, int niter, double residue, const SpinorView& x, bool use_guess,
                        const SpinorView& b, nq_solve_stats* st)
 {
   if (prec == NQ_PREC_F64) return clov_cg(s64, niter, residue, x, use_guess, b, st);
-  if (prec != NQ_PREC_F32) return fail(NQ_EINVAL, "twisted clover solves take NQ_PREC_F64 or NQ_PREC_F32");
-  ClovSetup s32;
-  NQ_TRY(clov_setup32(c, g, cl, kappa, anis, mu, &s32));
-  return clov_cg_mixed(s64, s32, niter, residue, x, use_guess, b, st);
+  if (prec != NQ_PREC_F32 && prec != NQ_PREC_F16)
+    return fail(NQ_EINVAL, "twisted clover solves take NQ_PREC_F64, NQ_PREC_F32 or NQ_PREC_F16");
+  const int sp = prec == NQ_PREC_F16 ? SP_F16 : SP_F32;
+  ClovSetup ss;
+  NQ_TRY(clov_setup_sloppy(c, g, cl, kappa, anis, mu, sp, &ss));
+  return clov_cg_mixed(s64, ss, sp, niter, residue, x, use_guess, b, st);
 }
 
 }  // namespace nq

@@ -122,10 +122,10 @@ def test_tmclov_cg_matches_oracle(setups):
     assert rel_l2(fy.download(), fx.download()) < 1e-10
 
 
-@pytest.mark.parametrize("L,prec", [(SHAPES[0], "f64"), (SHAPES[1], "f64"), (SHAPES[1], "f32")])
+@pytest.mark.parametrize("L,prec", [(SHAPES[0], "f64"), (SHAPES[1], "f64"), (SHAPES[1], "f32"), (SHAPES[1], "f16")])
 def test_tmclov_eo_solve_matches_oracle(setups, L, prec):
-    """FP64, and FP32 inner iterations with FP64 reliable updates (same FP64 residue and
-    the same solution to 1e-10)."""
+    """FP64, and FP32 or FP16 inner iterations with FP64 reliable updates (same FP64
+    residue and the same solution to 1e-10)."""
     ctx, g, cl, geo, conf, ce, co, ref = setups(L)
     mu = 0.01
     src = nq.source_lx(L, 12, "gauss")
