@@ -192,14 +192,18 @@ struct Timer {
 };
 
 // CG on K K^+ (FP64 throughout), cg_invert semantics (cg_invert_template_threaded.hpp:19-150)
+// kx (optional, FP64 odd): receives K(-mu) x of the final residual, the eo solve's
+// eq. 10, so that the caller need not apply K(-mu) again
 static int cg_kernsq_f64(const TmSetup& S, double mu, int niter, double residue, const SpinorView& x,
-                         const SpinorView* guess, const SpinorView& b, nq_solve_stats* out)
+                         const SpinorView* guess, const SpinorView& b, nq_solve_stats* out,
+                         const SpinorView* kx = nullptr)
 {
   nq_ctx* ctx = S.ctx;
   SpinorView r, p, t1, tmp;
   NQ_TRY(alloc_spinor(ctx, "cg.r", SP_F64, 12, &r));
   NQ_TRY(alloc_spinor(ctx, "cg.p", SP_F64, 12, &p));
-  NQ_TRY(alloc_spinor(ctx, "cg.t1", SP_F64, 12, &t1));
+  if (kx) t1 = *kx;
+  else NQ_TRY(alloc_spinor(ctx, "cg.t1", SP_F64, 12, &t1));
   NQ_TRY(alloc_spinor(ctx, "cg.tmp", SP_F64, 12, &tmp));
   Timer T(ctx->stream);
   double b2 = 0;
@@ -274,14 +278,17 @@ static int cg_kernsq_f64(const TmSetup& S, double mu, int niter, double residue,
 // spinors, FP32/FP16 links), FP64 solution accumulation and true-residual
 // recomputation whenever |r| drops by delta (or the sloppy residual reaches the
 // target); convergence is declared only on the FP64 residual.
+// kx as in cg_kernsq_f64: every FP64 residual computes K(-mu) x64 there, so after
+// convergence it holds that of the final x64
 static int cg_kernsq_mixed(const TmSetup& S64, const TmSetup& Ss, int sp, double mu, int niter, double residue,
                            double delta, const SpinorView& x64, const SpinorView* guess, const SpinorView& b,
-                           nq_solve_stats* out)
+                           nq_solve_stats* out, const SpinorView* kx = nullptr)
 {
   nq_ctx* ctx = S64.ctx;
   SpinorView r64, t64, tmp64, xs, rs, p, t1s, tmps;
   NQ_TRY(alloc_spinor(ctx, "cg.r", SP_F64, 12, &r64));
-  NQ_TRY(alloc_spinor(ctx, "cg.t1", SP_F64, 12, &t64));
+  if (kx) t64 = *kx;
+  else NQ_TRY(alloc_spinor(ctx, "cg.t1", SP_F64, 12, &t64));
   NQ_TRY(alloc_spinor(ctx, "cg.tmp", SP_F64, 12, &tmp64));
   // sloppy solution increment and residual stay FP32 (FP16 storage of either,
   // with per-site norms, measured 64 -> 70 iterations at 32^3x64 and a slower solve)
@@ -379,7 +386,7 @@ static int cg_kernsq_mixed(const TmSetup& S64, const TmSetup& Ss, int sp, double
 
 static int cg_kernsq_any(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis, double mu, int niter,
                          double residue, int prec, const SpinorView& x, const SpinorView* guess, const SpinorView& b,
-                         nq_solve_stats* st)
+                         nq_solve_stats* st, const SpinorView* kx = nullptr)
 {
   int sp, lk;
   NQ_TRY(lk_of_prec(prec, &sp, &lk));
@@ -390,10 +397,10 @@ static int cg_kernsq_any(nq_ctx* ctx, const nq_gauge* g, double kappa, const dou
   nq_solve_stats dummy;
   if (!st) st = &dummy;
   std::memset(st, 0, sizeof *st);
-  if (prec == NQ_PREC_F64) return cg_kernsq_f64(S64, mu, niter, residue, x, guess, b, st);
+  if (prec == NQ_PREC_F64) return cg_kernsq_f64(S64, mu, niter, residue, x, guess, b, st, kx);
   TmSetup Ss;
   NQ_TRY(tm_setup(ctx, g, lk, anis, kappa, &Ss));
-  return cg_kernsq_mixed(S64, Ss, sp, mu, niter, residue, 1e-2, x, guess, b, st);
+  return cg_kernsq_mixed(S64, Ss, sp, mu, niter, residue, 1e-2, x, guess, b, st, kx);
 }
 
 static int apply_Q_views(const TmSetup& S, double mu, const nq_field* out, const nq_field* in)
@@ -1213,10 +1220,11 @@ int nq_tm_solve_eoprec(nq_ctx* c, const nq_gauge* g, double kappa, const double*
   // eq. 9
   nq_solve_stats local;
   if (!st) st = &local;
-  int rc = cg_kernsq_any(c, g, kappa, anis, mu, niter, residue, prec, y, nullptr, phi, st);
+  // the CG's FP64 residuals compute K(-mu) y into sol_o: after convergence that is eq. 10
+  int rc = cg_kernsq_any(c, g, kappa, anis, mu, niter, residue, prec, y, nullptr, phi, st, &xo);
   if (rc != NQ_OK && rc != NQ_ENOCONV) return rc;
-  // eq. 10: sol_o = K(-mu) y  (sol_e used as scratch)
-  NQ_TRY(apply_kern(S, -mu, xo, y, xe));
+  // eq. 10: sol_o = K(-mu) y  (sol_e used as scratch), unless the CG left it there
+  if (rc != NQ_OK) NQ_TRY(apply_kern(S, -mu, xo, y, xe));
   // eq. 11: sol_e = D_ee^-1 (eta_e + 1/2 H_eo sol_o)
   HopEpi e11;
   setc(e11.c1, 0.5 * zi_u, 0.5 * zi_l);
