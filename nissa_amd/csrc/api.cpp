@@ -209,6 +209,7 @@ int nq_ctx_create(int T, int LX, int LY, int LZ, int device, nq_ctx** out)
   NQ_HIP(hipSetDevice(device));
   auto* c = new nq_ctx;
   c->device = device;
+  if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) c->ncu = 0;
   Geo& g = c->geo;
   g.T = T; g.LX = LX; g.LY = LY; g.LZ = LZ;
   g.Vh = (int64_t)T * LX * LY * LZ / 2;

@@ -168,6 +168,7 @@ struct KernelStat {
 
 struct nq_ctx {
   int device = 0;
+  int ncu = 0;  // compute units of the device (persistent launches)
   nq::Geo geo{};
   hipStream_t stream = nullptr;
   std::recursive_mutex mu;

@@ -281,6 +281,76 @@ __global__ void __launch_bounds__(256, (SP != SP_F64 ? 2 : 1)) k_tm_hop_lds(Geo 
   if constexpr (MODE != 0) store_partial(nrm, partial, nchunks, m.r, m.chunk);
 }
 
+// Persistent form of k_tm_hop_lds for the FP64 hop, which runs one wavefront per
+// SIMD: one resident workgroup per slot loops over 256-site blocks, so no SIMD
+// idles between a finished workgroup and the dispatch of the next.  The site
+// body is the one of k_tm_hop_lds, written out in the same form (the compiler's
+// FMA contraction follows the code's shape, and the kernels must agree bit for
+// bit with each other and with k_tm_hop).  The
+// workgroups of XCD x (b % 8 == x, the hardware's round-robin) walk one
+// contiguous eighth of the blocks together, block bi, bi + nbx, ... of it, as the
+// one-shot kernel's dispatch order does.  Partials stay per 64-site chunk.
+template <int SP, int LK, int MODE, int XP>
+__global__ void __launch_bounds__(256, 1) k_tm_hop_lds_p(Geo g, int op, SpinorView out, SpinorView in, LinkView U,
+                                                         double anis_d, int use_anis, HopEpi epi, int64_t nblk,
+                                                         int64_t nchunks, double* partial)
+{
+  using R = RealOf<SP>;
+  __shared__ __attribute__((aligned(16))) char lds[Stage<SP, 12>::BYTES];
+  CgState* st = epi.st;
+  if (st && rhs_idle(st, epi.pred_reliable)) return;  // uniform: one right-hand side per launch
+  const uint32_t b = blockIdx.x, nb = gridDim.x;
+  const uint32_t xcd = b & 7u, bi = b >> 3;
+  const int64_t nbx = (int64_t)(nb >> 3) + (xcd < (nb & 7u) ? 1 : 0);  // workgroups on this XCD
+  const int64_t q8 = nblk >> 3, r8 = nblk & 7;
+  const int64_t lo = (int64_t)xcd * q8 + ((int64_t)xcd < r8 ? (int64_t)xcd : r8);
+  const int64_t hi = lo + q8 + ((int64_t)xcd < r8 ? 1 : 0);
+  const int t = (int)threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const void* ub = U.data[op];
+  const double* det = &U.det[op][0][0];
+  const R anis = (R)anis_d;
+  const bool ua = use_anis != 0;
+  for (int64_t cb = lo + bi; cb < hi; cb += nbx) {  // uniform over the workgroup
+    const uint32_t e0 = (uint32_t)(cb * kStageSites);
+    __syncthreads();  // the previous block's LDS reads are done
+    Stage<SP, 12>::put(lds, in, g.stride, e0 + t, e0 + t < (uint64_t)g.Vh, t, 0);
+    __syncthreads();
+    const uint32_t e = e0 + (uint32_t)t;
+    double nrm = 0;
+    if (e < (uint64_t)g.Vh) {
+      const Site s = site_of(g, op, e);
+      cx<R> o[12];
+#pragma unroll
+      for (int k = 0; k < 12; k++) o[k] = mk<R>(0, 0);
+      hop_slot<SP, LK, R, 0, true, false>(g, s, e, in, 0, ub, det, anis, ua, o, lds, e0);
+      hop_slot<SP, LK, R, 0, false, false>(g, s, e, in, 0, ub, det, anis, ua, o, lds, e0);
+      hop_slot<SP, LK, R, 1, true, false>(g, s, e, in, 0, ub, det, anis, ua, o, lds, e0);
+      hop_slot<SP, LK, R, 1, false, true>(g, s, e, in, 0, ub, det, anis, ua, o, lds, e0);
+      hop_slot<SP, LK, R, 2, true, false>(g, s, e, in, 0, ub, det, anis, ua, o, lds, e0);
+      hop_slot<SP, LK, R, 2, false, false>(g, s, e, in, 0, ub, det, anis, ua, o, lds, e0);
+      hop_slot<SP, LK, R, 3, true, false>(g, s, e, in, 0, ub, det, anis, ua, o, lds, e0);
+      hop_slot<SP, LK, R, 3, false, true>(g, s, e, in, 0, ub, det, anis, ua, o, lds, e0);
+      nrm = hop_epilogue<SP, MODE, XP, R>(g, epi, out, e, 0, st, o);
+    }
+    if constexpr (MODE != 0) store_partial(nrm, partial, nchunks, 0, cb * 4 + wave);
+  }
+}
+
+// resident workgroups per CU of the persistent kernel (queried once per instance)
+template <int SP, int LK, int MODE, int XP>
+static int persist_per_cu()
+{
+  static const int occ = [] {
+    int o = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, (const void*)k_tm_hop_lds_p<SP, LK, MODE, XP>, 256, 0) !=
+            hipSuccess || o < 1)
+      o = 0;
+    return o;
+  }();
+  return occ;
+}
+
 // ---------------------------------------------------------------- register-blocked RHS
 // One slot of the hop for RB right-hand sides r0..r0+nrb-1 of the same site: the
 // link is loaded once and applied to each right-hand side's projected neighbour.
@@ -503,6 +573,28 @@ static int tuned_rb(nq_ctx* ctx, int op, const SpinorView& in, const LinkView& U
   return NQ_OK;
 }
 
+// Launches the persistent staged hop and returns true when it applies: FP64
+// with 18-real links and a lattice that fills the resident slots at least twice
+// (measured 2.8 % faster at 32^3x64).  The 12-real FP64 hop runs two wavefronts
+// per SIMD in the one-shot kernel and measured slower in the persistent one.
+template <int SP, int LK, int MODE, int XP>
+static bool launch_persistent(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, const LinkView& U,
+                              double anis, const HopEpi& epi, const Launch& L)
+{
+  if constexpr (SP == SP_F64 && LK == LK_F64_18) {
+    const char* penv = getenv("NQ_PERSIST");
+    if (penv && penv[0] == '0') return false;
+    const int64_t nblk = (L.nchunks + 3) / 4, slots = (int64_t)ctx->ncu * persist_per_cu<SP, LK, MODE, XP>();
+    if (slots < 1 || nblk < 2 * slots) return false;
+    hipLaunchKernelGGL((k_tm_hop_lds_p<SP, LK, MODE, XP>), dim3((unsigned)slots), dim3(256), 0, ctx->stream,
+                       ctx->geo, op, out, in, U, anis, anis != 1.0 ? 1 : 0, epi, nblk, L.nchunks, ctx->partial);
+    return true;
+  } else {
+    (void)ctx; (void)op; (void)out; (void)in; (void)U; (void)anis; (void)epi; (void)L;
+    return false;
+  }
+}
+
 template <int SP, int LK, int MODE, int XP>
 static int go(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, const LinkView& U, double anis,
               const HopEpi& epi, const char* name)
@@ -531,9 +623,11 @@ static int go(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, 
   // FP64: neighbours staged in LDS (measured 4 % faster at 32^3x64; the sloppy
   // kernels measured slower with staging and read their neighbours from memory)
   const char* lenv = getenv("NQ_LDS");
-  if (SP == SP_F64 && nr == 1 && L.s.W == 4 && !(lenv && lenv[0] == '0'))
-    hipLaunchKernelGGL((k_tm_hop_lds<SP, LK, MODE, XP>), dim3((unsigned)L.blocks), dim3(L.threads), 0, ctx->stream, g, op,
-                       out, in, U, anis, anis != 1.0 ? 1 : 0, epi, L.s, L.nchunks, ctx->partial);
+  if (SP == SP_F64 && nr == 1 && L.s.W == 4 && !(lenv && lenv[0] == '0')) {
+    if (!launch_persistent<SP, LK, MODE, XP>(ctx, op, out, in, U, anis, epi, L))
+      hipLaunchKernelGGL((k_tm_hop_lds<SP, LK, MODE, XP>), dim3((unsigned)L.blocks), dim3(L.threads), 0, ctx->stream, g,
+                         op, out, in, U, anis, anis != 1.0 ? 1 : 0, epi, L.s, L.nchunks, ctx->partial);
+  }
   else
     hipLaunchKernelGGL((k_tm_hop<SP, LK, MODE, XP>), dim3((unsigned)L.blocks), dim3(L.threads), 0, ctx->stream, g, op,
                        out, in, U, anis, anis != 1.0 ? 1 : 0, epi, L.s, L.nchunks, ctx->partial);

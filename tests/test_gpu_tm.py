@@ -60,6 +60,42 @@ def test_hop_24c48(setups):
     assert rel_l2(fo.download(), geo.tm_hop(1, ce, co, x)) < 1e-14
 
 
+def test_persistent_hop_matches_one_shot(setups, monkeypatch):
+    """The FP64 hop runs as a persistent kernel once the lattice fills the resident
+    workgroup slots twice (64x16^3: 512 blocks of 256 sites).  Its hop matches the
+    oracle to <= 1e-14; the hop and an FP64 eo solve (every epilogue mode) give the
+    same bits as the one-shot kernel (NQ_PERSIST=0); and a source solved alone
+    (persistent kernel) gives the same bits as inside a batch of two (multi-RHS
+    kernel)."""
+    L = (64, 16, 16, 16)
+    ctx, g, geo, conf, ce, co = setups(L, 3)
+    x = gauss_eo(geo.Vh, 9)
+    fi = ctx.field(nq.SPINCOLOR, nq.EVEN, x)
+    fo = ctx.field(nq.SPINCOLOR, nq.ODD)
+    nq.tm_hop(ctx, g, fo, fi)
+    hop_p = fo.download()
+    assert rel_l2(hop_p, geo.tm_hop(1, ce, co, x)) < 1e-14
+    fs = ctx.field(nq.SPINCOLOR, nq.LX, nq.source_lx(L, 11))
+    fx = ctx.field(nq.SPINCOLOR, nq.LX)
+    st_p = nq.tm_solve_eoprec(ctx, g, 0.16, 0.01, 10000, 1e-20, fx, fs)
+    assert st_p["rc"] == 0 and st_p["converged"] and st_p["check_rel_res_sq"] < 1e-19, st_p
+    sol_p = fx.download()
+    monkeypatch.setenv("NQ_PERSIST", "0")
+    nq.tm_hop(ctx, g, fo, fi)
+    assert np.array_equal(fo.download(), hop_p)
+    st_1 = nq.tm_solve_eoprec(ctx, g, 0.16, 0.01, 10000, 1e-20, fx, fs)
+    assert st_1["iters"] == st_p["iters"]
+    assert np.array_equal(fx.download(), sol_p)
+    monkeypatch.delenv("NQ_PERSIST")
+    fs2 = ctx.field(nq.SPINCOLOR, nq.LX, nq.source_lx(L, 12))
+    pair = [ctx.field(nq.SPINCOLOR, nq.LX) for _ in range(2)]
+    res = nq.tm_solve_eoprec_multi(ctx, g, 0.16, [0.01, 0.02], 10000, 1e-20, pair, [fs, fs2], prec="f64")
+    assert all(r["rc"] == 0 and r["converged"] for r in res), res
+    alone = [ctx.field(nq.SPINCOLOR, nq.LX)]
+    nq.tm_solve_eoprec_multi(ctx, g, 0.16, [0.01], 10000, 1e-20, alone, [fs], prec="f64")
+    assert np.array_equal(alone[0].download(), pair[0].download())
+
+
 @pytest.mark.parametrize("L", SHAPES[:3])
 @pytest.mark.parametrize("mu", [0.01, -0.2])
 def test_kern_and_square(setups, L, mu):
