@@ -25,127 +25,180 @@ namespace nq {
 template <int SP>
 using RealOfS = typename std::conditional<SP == SP_F64, double, float>::type;
 
-template <int SP, int LK, class R, int MU, bool FWD, bool PIN>
-__device__ __forceinline__ void st_slot(const Geo& g, const Site& s, uint32_t e, const SpinorView& in, int r,
-                                        const void* ub, cx<R> o[3])
+// One link slot of the hop applied to the RB right-hand sides r0.. of one site:
+// the link is loaded once and multiplies every busy right-hand side's neighbour.
+// Each right-hand side runs the same unrolled code for every RB, so a result
+// does not depend on how many right-hand sides share the wavefront.
+template <int SP, int LK, class R, int MU, bool FWD, bool PIN, int RB>
+__device__ __forceinline__ void st_slot(const Geo& g, const Site& s, uint32_t e, const SpinorView& in, int r0,
+                                        unsigned busy, const void* ub, cx<R> o[RB][3])
 {
   const uint32_t n = neigh_eo(g, s, MU, FWD);
-  cx<R> v[3];
-  SpAcc<SP, R, 3>::load(in, g.stride, n, v, r);
   cx<R> u[9];
   constexpr int slot = 2 * MU + (FWD ? 0 : 1);
   const double det[2] = {1, 0};
   LkAcc<LK, R>::load(ub, g.stride, slot, e, det, u);
-  cx<R> w[3];
-  mat_vec(u, v, w);
 #pragma unroll
-  for (int c = 0; c < 3; c++) o[c] = FWD ? o[c] + w[c] : o[c] - w[c];
-  if constexpr (PIN) {
+  for (int b = 0; b < RB; b++) {
+    if (!((busy >> b) & 1u)) continue;  // uniform over the wavefront
+    cx<R> v[3];
+    SpAcc<SP, R, 3>::load(in, g.stride, n, v, r0 + b);
+    cx<R> w[3];
+    mat_vec(u, v, w);
 #pragma unroll
-    for (int c = 0; c < 3; c++) asm volatile("" : "+v"(o[c].re), "+v"(o[c].im));
-    asm volatile("" ::: "memory");
+    for (int c = 0; c < 3; c++) o[b][c] = FWD ? o[b][c] + w[c] : o[b][c] - w[c];
+    if constexpr (PIN) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) asm volatile("" : "+v"(o[b][c].re), "+v"(o[b][c].im));
+    }
   }
+  if constexpr (PIN) asm volatile("" ::: "memory");
 }
 
-template <int SP, int LK, class R>
-__device__ __forceinline__ void st_hop_site(const Geo& g, int op, uint32_t e, const SpinorView& in, int r,
-                                            const LinkView& U, cx<R> o[3])
+template <int SP, int LK, class R, int RB>
+__device__ __forceinline__ void st_hop_site(const Geo& g, int op, uint32_t e, const SpinorView& in, int r0,
+                                            unsigned busy, const LinkView& U, cx<R> o[RB][3])
 {
   const Site s = site_of(g, op, e);
   const void* ub = U.data[op];
 #pragma unroll
-  for (int c = 0; c < 3; c++) o[c] = mk<R>(0, 0);
-  st_slot<SP, LK, R, 0, true, false>(g, s, e, in, r, ub, o);
-  st_slot<SP, LK, R, 0, false, false>(g, s, e, in, r, ub, o);
-  st_slot<SP, LK, R, 1, true, false>(g, s, e, in, r, ub, o);
-  st_slot<SP, LK, R, 1, false, true>(g, s, e, in, r, ub, o);
-  st_slot<SP, LK, R, 2, true, false>(g, s, e, in, r, ub, o);
-  st_slot<SP, LK, R, 2, false, false>(g, s, e, in, r, ub, o);
-  st_slot<SP, LK, R, 3, true, false>(g, s, e, in, r, ub, o);
-  st_slot<SP, LK, R, 3, false, true>(g, s, e, in, r, ub, o);
+  for (int b = 0; b < RB; b++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) o[b][c] = mk<R>(0, 0);
+  st_slot<SP, LK, R, 0, true, false, RB>(g, s, e, in, r0, busy, ub, o);
+  st_slot<SP, LK, R, 0, false, false, RB>(g, s, e, in, r0, busy, ub, o);
+  st_slot<SP, LK, R, 1, true, false, RB>(g, s, e, in, r0, busy, ub, o);
+  st_slot<SP, LK, R, 1, false, true, RB>(g, s, e, in, r0, busy, ub, o);
+  st_slot<SP, LK, R, 2, true, false, RB>(g, s, e, in, r0, busy, ub, o);
+  st_slot<SP, LK, R, 2, false, false, RB>(g, s, e, in, r0, busy, ub, o);
+  st_slot<SP, LK, R, 3, true, false, RB>(g, s, e, in, r0, busy, ub, o);
+  st_slot<SP, LK, R, 3, false, true, RB>(g, s, e, in, r0, busy, ub, o);
+}
+
+// right-hand side r is skipped when skip_flags[r*skip_stride] (or, with skip_n > 1,
+// the next flag) is set
+__device__ __forceinline__ bool st_skipped(const int* skip_flags, int skip_stride, int skip_n, int r)
+{
+  return skip_flags && (skip_flags[r * skip_stride] || (skip_n > 1 && skip_flags[r * skip_stride + 1]));
 }
 
 // out = c1 hop + c2 aux (real coefficients, per rhs from coef[2r], coef[2r+1] when
 // tabulated). MODE 0: write; 1: write (if write) + |out|^2; 4: write (if write) + Re<aux,out>.
-template <int SP, int LK, int MODE>
+// Wavefront w of a workgroup takes chunk 4*cb + w and the RB right-hand sides of
+// group m.r (Shape W = 4, NRB = 1, NR = NRG = number of RB groups).
+template <int SP, int LK, int MODE, int RB>
 __global__ void __launch_bounds__(256) k_st_hop(Geo g, int op, SpinorView out, SpinorView in, LinkView U, double c1,
                                                  double c2, const double* coef, SpinorView aux, int write, Shape S,
-                                                 int64_t nchunks, double* partial, const int* skip_flags,
+                                                 int nr, int64_t nchunks, double* partial, const int* skip_flags,
                                                  int skip_stride, int skip_n)
 {
   using R = RealOfS<SP>;
   const Map m = map_thread(S, g.Vh);
-  const bool skip = skip_flags && (skip_flags[m.r * skip_stride] || (skip_n > 1 && skip_flags[m.r * skip_stride + 1]));
-  double red = 0;
-  if (m.active && !skip) {
-    cx<R> o[3];
-    st_hop_site<SP, LK, R>(g, op, m.e, in, m.r, U, o);
-    const double a = coef ? coef[2 * m.r] : c1, b = coef ? coef[2 * m.r + 1] : c2;
+  const int r0 = m.r * RB, nrb = min(RB, nr - r0);
+  unsigned busy = 0;
 #pragma unroll
-    for (int c = 0; c < 3; c++) o[c] = scal(o[c], (R)a);
-    if (aux.data) {
-      cx<R> x[3];
-      SpAcc<SP, R, 3, kNT>::load(aux, g.stride, m.e, x, m.r);
+  for (int b = 0; b < RB; b++)
+    if (b < nrb && !st_skipped(skip_flags, skip_stride, skip_n, r0 + b)) busy |= 1u << b;
+  if (!busy || !m.active) {
+    if constexpr (MODE != 0) {
 #pragma unroll
-      for (int c = 0; c < 3; c++) {
-        o[c].re = fma((R)b, x[c].re, o[c].re);
-        o[c].im = fma((R)b, x[c].im, o[c].im);
-        if constexpr (MODE == 4) red += (double)x[c].re * o[c].re + (double)x[c].im * o[c].im;
+      for (int b = 0; b < RB; b++)
+        if (b < nrb) store_partial(0.0, partial, nchunks, r0 + b, m.chunk);
+    }
+    return;
+  }
+  cx<R> o[RB][3];
+  st_hop_site<SP, LK, R, RB>(g, op, m.e, in, r0, busy, U, o);
+#pragma unroll
+  for (int b = 0; b < RB; b++) {
+    if (b >= nrb) continue;
+    const int r = r0 + b;
+    double red = 0;
+    if ((busy >> b) & 1u) {
+      const double a = coef ? coef[2 * r] : c1, bb = coef ? coef[2 * r + 1] : c2;
+#pragma unroll
+      for (int c = 0; c < 3; c++) o[b][c] = scal(o[b][c], (R)a);
+      if (aux.data) {
+        cx<R> x[3];
+        SpAcc<SP, R, 3, kNT>::load(aux, g.stride, m.e, x, r);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          o[b][c].re = fma((R)bb, x[c].re, o[b][c].re);
+          o[b][c].im = fma((R)bb, x[c].im, o[b][c].im);
+          if constexpr (MODE == 4) red += (double)x[c].re * o[b][c].re + (double)x[c].im * o[b][c].im;
+        }
+      }
+      if (MODE == 0 || write) SpAcc<SP, R, 3, kNT>::store(out, g.stride, m.e, o[b], r);
+      if constexpr (MODE == 1) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) red += (double)o[b][c].re * o[b][c].re + (double)o[b][c].im * o[b][c].im;
       }
     }
-    if (MODE == 0 || write) SpAcc<SP, R, 3, kNT>::store(out, g.stride, m.e, o, m.r);
-    if constexpr (MODE == 1) {
-#pragma unroll
-      for (int c = 0; c < 3; c++) red += (double)o[c].re * o[c].re + (double)o[c].im * o[c].im;
-    }
+    if constexpr (MODE != 0) store_partial(red, partial, nchunks, r, m.chunk);
   }
-  if constexpr (MODE != 0) store_partial(red, partial, nchunks, m.r, m.chunk);
 }
 
 // ------------------------------------------------------------------ CGM
 // C2: s = -1/4 Deo t (even output, never stored) ; sol_k -= betas_k ps_k ; r += betaa s ; |r|^2
 // sol/ps of shift k of rhs r: base + r*rstride + k*sstride (elements)
-template <int SP, int LK>
+// (RB right-hand sides per wavefront as in k_st_hop)
+template <int SP, int LK, int RB>
 __global__ void __launch_bounds__(256) k_cgm_hop2(Geo g, SpinorView t, LinkView U, SpinorView rv, SpinorView sol,
-                                                   SpinorView ps, int64_t sstride, CgmState* st, Shape SH,
+                                                   SpinorView ps, int64_t sstride, CgmState* st, Shape SH, int nr,
                                                    int64_t nchunks, double* partial)
 {
   using R = RealOfS<SP>;
   const Map m = map_thread(SH, g.Vh);
-  const CgmState* S = st + m.r;
-  double red = 0;
-  if (m.active && !S->done && !S->reliable) {
-    cx<R> o[3];
-    st_hop_site<SP, LK, R>(g, 0, m.e, t, m.r, U, o);
-    const R betaa = (R)S->betaa;
-    for (int k = 0; k < S->nshift; k++) {
-      if (!S->run[k]) continue;
-      SpinorView sk = sol, pk = ps;
-      sk.data = (char*)sol.data + k * sstride * (SP == SP_F64 ? 16 : 8);
-      pk.data = (char*)ps.data + k * sstride * (SP == SP_F64 ? 16 : 8);
-      cx<R> x[3], q[3];
-      SpAcc<SP, R, 3, kNT>::load(sk, g.stride, m.e, x, m.r);
-      SpAcc<SP, R, 3, kNT>::load(pk, g.stride, m.e, q, m.r);
-      const R b = (R)S->betas[k];
+  const int r0 = m.r * RB, nrb = min(RB, nr - r0);
+  unsigned busy = 0;
+#pragma unroll
+  for (int b = 0; b < RB; b++)
+    if (b < nrb && !st[r0 + b].done && !st[r0 + b].reliable) busy |= 1u << b;
+  if (!busy || !m.active) {
+#pragma unroll
+    for (int b = 0; b < RB; b++)
+      if (b < nrb) store_partial(0.0, partial, nchunks, r0 + b, m.chunk);
+    return;
+  }
+  cx<R> o[RB][3];
+  st_hop_site<SP, LK, R, RB>(g, 0, m.e, t, r0, busy, U, o);
+#pragma unroll
+  for (int b = 0; b < RB; b++) {
+    if (b >= nrb) continue;
+    const int r = r0 + b;
+    const CgmState* S = st + r;
+    double red = 0;
+    if ((busy >> b) & 1u) {
+      const R betaa = (R)S->betaa;
+      for (int k = 0; k < S->nshift; k++) {
+        if (!S->run[k]) continue;
+        SpinorView sk = sol, pk = ps;
+        sk.data = (char*)sol.data + k * sstride * (SP == SP_F64 ? 16 : 8);
+        pk.data = (char*)ps.data + k * sstride * (SP == SP_F64 ? 16 : 8);
+        cx<R> x[3], q[3];
+        SpAcc<SP, R, 3, kNT>::load(sk, g.stride, m.e, x, r);
+        SpAcc<SP, R, 3, kNT>::load(pk, g.stride, m.e, q, r);
+        const R bk = (R)S->betas[k];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          x[c].re = fma(-bk, q[c].re, x[c].re);
+          x[c].im = fma(-bk, q[c].im, x[c].im);
+        }
+        SpAcc<SP, R, 3, kNT>::store(sk, g.stride, m.e, x, r);
+      }
+      cx<R> rr[3];
+      SpAcc<SP, R, 3, kNT>::load(rv, g.stride, m.e, rr, r);
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        x[c].re = fma(-b, q[c].re, x[c].re);
-        x[c].im = fma(-b, q[c].im, x[c].im);
+        const cx<R> sv = scal(o[b][c], (R)-0.25);
+        rr[c].re = fma(betaa, sv.re, rr[c].re);
+        rr[c].im = fma(betaa, sv.im, rr[c].im);
+        red += (double)rr[c].re * rr[c].re + (double)rr[c].im * rr[c].im;
       }
-      SpAcc<SP, R, 3, kNT>::store(sk, g.stride, m.e, x, m.r);
+      SpAcc<SP, R, 3, kNT>::store(rv, g.stride, m.e, rr, r);
     }
-    cx<R> rr[3];
-    SpAcc<SP, R, 3, kNT>::load(rv, g.stride, m.e, rr, m.r);
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      const cx<R> sv = scal(o[c], (R)-0.25);
-      rr[c].re = fma(betaa, sv.re, rr[c].re);
-      rr[c].im = fma(betaa, sv.im, rr[c].im);
-      red += (double)rr[c].re * rr[c].re + (double)rr[c].im * rr[c].im;
-    }
-    SpAcc<SP, R, 3, kNT>::store(rv, g.stride, m.e, rr, m.r);
+    store_partial(red, partial, nchunks, r, m.chunk);
   }
-  store_partial(red, partial, nchunks, m.r, m.chunk);
 }
 
 // C3: p = r + alpha p ; ps_k = zfs_k r + alphas_k ps_k for running shifts
@@ -255,6 +308,44 @@ __global__ void __launch_bounds__(1024) k_cgm_finalize(const double* partial, in
 static double csite(int sp) { return sp == SP_F64 ? 48.0 : 24.0; }
 static double lbytes(int lk) { return lk == LK_F64_18 ? 144.0 : 72.0; }
 
+// Right-hand sides per wavefront of the staggered hops: one for a single right-hand
+// side, else 4 (NQ_ST_RB = 1 or 2 overrides); every choice gives the same bits.
+// The launch shape for RB: W = 4 chunks per workgroup, one RB group per wavefront.
+static int st_rb(int nr)
+{
+  if (nr <= 1) return 1;
+  const char* e = getenv("NQ_ST_RB");
+  const int v = e ? atoi(e) : 4;
+  return v == 1 || v == 2 ? v : 4;
+}
+static Shape st_shape(int nr, int RB)
+{
+  Shape sh;
+  sh.W = 4;
+  sh.NRB = 1;
+  sh.NRG = sh.NR = (nr + RB - 1) / RB;
+  return sh;
+}
+
+template <int SP, int LK, int MODE>
+static void st_hop_go(nq_ctx* ctx, int RB, int op, const SpinorView& out, const SpinorView& in, const LinkView& U,
+                      double c1, double c2, const double* coef, const SpinorView& aux, int write, int nr,
+                      const int* skip_flags, int skip_stride, int skip_n)
+{
+  const Geo& g = ctx->geo;
+  const Shape sh = st_shape(nr, RB);
+  const int64_t nchunks = (g.Vh + 63) / 64;
+  const dim3 grid((unsigned)((nchunks + 3) / 4 * sh.NRG)), blk(256);
+  switch (RB) {
+  case 1: hipLaunchKernelGGL((k_st_hop<SP, LK, MODE, 1>), grid, blk, 0, ctx->stream, g, op, out, in, U, c1, c2, coef, aux,
+                             write, sh, nr, nchunks, ctx->partial, skip_flags, skip_stride, skip_n); break;
+  case 2: hipLaunchKernelGGL((k_st_hop<SP, LK, MODE, 2>), grid, blk, 0, ctx->stream, g, op, out, in, U, c1, c2, coef, aux,
+                             write, sh, nr, nchunks, ctx->partial, skip_flags, skip_stride, skip_n); break;
+  default: hipLaunchKernelGGL((k_st_hop<SP, LK, MODE, 4>), grid, blk, 0, ctx->stream, g, op, out, in, U, c1, c2, coef,
+                              aux, write, sh, nr, nchunks, ctx->partial, skip_flags, skip_stride, skip_n); break;
+  }
+}
+
 int launch_st_hop(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, const LinkView& U, double c1,
                   double c2, const double* coef_dev, const SpinorView& aux, int mode, int write, int nr, int fin,
                   CgState* st, const int* skip_flags, int skip_stride, int skip_n)
@@ -262,22 +353,23 @@ int launch_st_hop(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& 
   const Geo& g = ctx->geo;
   if (in.ncomp != 3) return fail(NQ_EINVAL, "staggered hop needs colour fields");
   if (nr < 1 || nr > 16) return fail(NQ_EINVAL, "right-hand sides per launch must be 1..16");
-  const Launch L = launch_shape(g, nr);
+  const int64_t nchunks = (g.Vh + 63) / 64;
+  if (nchunks * nr > ctx->max_partials) return fail(NQ_EINVAL, "st hop: reduction scratch too small");
   const int sp = in.prec, lk = U.prec;
   if ((mode == 0 || write) && out.prec != sp) return fail(NQ_EINVAL, "st hop: output precision mismatch");
   if (aux.data && aux.prec != sp) return fail(NQ_EINVAL, "st hop: aux precision mismatch");
   hipEvent_t ev = nullptr;
   const double by = (8 * lbytes(lk) + csite(sp) * (1 + (aux.data ? 1 : 0) + ((mode == 0 || write) ? 1 : 0))) * nr;
   prof_begin(ctx, mode == 0 ? "st_hop" : "st_hop_red", by * (double)g.Vh, &ev);
-  const dim3 grid((unsigned)L.blocks), blk(L.threads);
-#define NQ_ST(SPV, LKV)                                                                                            \
-  switch (mode) {                                                                                                  \
-  case 0: hipLaunchKernelGGL((k_st_hop<SPV, LKV, 0>), grid, blk, 0, ctx->stream, g, op, out, in, U, c1, c2, coef_dev, \
-                             aux, write, L.s, L.nchunks, ctx->partial, skip_flags, skip_stride, skip_n); break;         \
-  case 1: hipLaunchKernelGGL((k_st_hop<SPV, LKV, 1>), grid, blk, 0, ctx->stream, g, op, out, in, U, c1, c2, coef_dev, \
-                             aux, write, L.s, L.nchunks, ctx->partial, skip_flags, skip_stride, skip_n); break;         \
-  default: hipLaunchKernelGGL((k_st_hop<SPV, LKV, 4>), grid, blk, 0, ctx->stream, g, op, out, in, U, c1, c2, coef_dev, \
-                              aux, write, L.s, L.nchunks, ctx->partial, skip_flags, skip_stride, skip_n); break;        \
+  const int RB = st_rb(nr);
+#define NQ_ST(SPV, LKV)                                                                                              \
+  switch (mode) {                                                                                                    \
+  case 0: st_hop_go<SPV, LKV, 0>(ctx, RB, op, out, in, U, c1, c2, coef_dev, aux, write, nr, skip_flags, skip_stride, \
+                                 skip_n); break;                                                                     \
+  case 1: st_hop_go<SPV, LKV, 1>(ctx, RB, op, out, in, U, c1, c2, coef_dev, aux, write, nr, skip_flags, skip_stride, \
+                                 skip_n); break;                                                                     \
+  default: st_hop_go<SPV, LKV, 4>(ctx, RB, op, out, in, U, c1, c2, coef_dev, aux, write, nr, skip_flags,             \
+                                  skip_stride, skip_n); break;                                                       \
   }
   if (sp == SP_F64 && lk == LK_F64_18) { NQ_ST(SP_F64, LK_F64_18) }
   else if (sp == SP_F32 && lk == LK_F32_18) { NQ_ST(SP_F32, LK_F32_18) }
@@ -467,14 +559,13 @@ int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, co
   none.ncomp = 3;
   hipEvent_t ev = nullptr;
   prof_begin(ctx, "cgm_hop1", (8 * lbytes(lk) + 2 * csite(sp)) * nr * (double)g.Vh, &ev);
+  const int RB = st_rb(nr);
+  const int* skip = (const int*)&st->done;
+  const int sstr = (int)(sizeof(CgmState) / sizeof(int));
   if (sp == SP_F64)
-    hipLaunchKernelGGL((k_st_hop<SP_F64, LK_F64_18, 1>), grid, blk, 0, ctx->stream, g, 1, t, p, U, 1.0, 0.0, nullptr,
-                       none, 1, L.s, L.nchunks, ctx->partial, (const int*)&st->done,
-                       (int)(sizeof(CgmState) / sizeof(int)), 2);
+    st_hop_go<SP_F64, LK_F64_18, 1>(ctx, RB, 1, t, p, U, 1.0, 0.0, nullptr, none, 1, nr, skip, sstr, 2);
   else
-    hipLaunchKernelGGL((k_st_hop<SP_F32, LK_F32_18, 1>), grid, blk, 0, ctx->stream, g, 1, t, p, U, 1.0, 0.0, nullptr,
-                       none, 1, L.s, L.nchunks, ctx->partial, (const int*)&st->done,
-                       (int)(sizeof(CgmState) / sizeof(int)), 2);
+    st_hop_go<SP_F32, LK_F32_18, 1>(ctx, RB, 1, t, p, U, 1.0, 0.0, nullptr, none, 1, nr, skip, sstr, 2);
   NQ_HIP(hipGetLastError());
   prof_end(ctx, ev);
   hipLaunchKernelGGL(k_cgm_finalize, dim3(nr), dim3(1024), 0, ctx->stream, ctx->partial, L.nchunks, st,
@@ -482,12 +573,22 @@ int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, co
   NQ_HIP(hipGetLastError());
   // C2
   prof_begin(ctx, "cgm_hop2_update", (8 * lbytes(lk) + 3 * csite(sp)) * nr * (double)g.Vh, &ev);
-  if (sp == SP_F64)
-    hipLaunchKernelGGL((k_cgm_hop2<SP_F64, LK_F64_18>), grid, blk, 0, ctx->stream, g, t, U, r, sol, ps, sstride, st,
-                       L.s, L.nchunks, ctx->partial);
-  else
-    hipLaunchKernelGGL((k_cgm_hop2<SP_F32, LK_F32_18>), grid, blk, 0, ctx->stream, g, t, U, r, sol, ps, sstride, st,
-                       L.s, L.nchunks, ctx->partial);
+  {
+    const Shape sh = st_shape(nr, RB);
+    const dim3 grid2((unsigned)((L.nchunks + 3) / 4 * sh.NRG)), blk2(256);
+#define NQ_C2(SPV, LKV)                                                                                           \
+  switch (RB) {                                                                                                   \
+  case 1: hipLaunchKernelGGL((k_cgm_hop2<SPV, LKV, 1>), grid2, blk2, 0, ctx->stream, g, t, U, r, sol, ps, sstride, st, \
+                             sh, nr, L.nchunks, ctx->partial); break;                                             \
+  case 2: hipLaunchKernelGGL((k_cgm_hop2<SPV, LKV, 2>), grid2, blk2, 0, ctx->stream, g, t, U, r, sol, ps, sstride, st, \
+                             sh, nr, L.nchunks, ctx->partial); break;                                             \
+  default: hipLaunchKernelGGL((k_cgm_hop2<SPV, LKV, 4>), grid2, blk2, 0, ctx->stream, g, t, U, r, sol, ps, sstride,  \
+                              st, sh, nr, L.nchunks, ctx->partial); break;                                        \
+  }
+    if (sp == SP_F64) { NQ_C2(SP_F64, LK_F64_18) }
+    else { NQ_C2(SP_F32, LK_F32_18) }
+#undef NQ_C2
+  }
   NQ_HIP(hipGetLastError());
   prof_end(ctx, ev);
   hipLaunchKernelGGL(k_cgm_finalize, dim3(nr), dim3(1024), 0, ctx->stream, ctx->partial, L.nchunks, st,
