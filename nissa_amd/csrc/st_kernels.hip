@@ -25,26 +25,35 @@ namespace nq {
 template <int SP>
 using RealOfS = typename std::conditional<SP == SP_F64, double, float>::type;
 
-// One link slot of the hop applied to the RB right-hand sides r0.. of one site:
-// the link is loaded once and multiplies every busy right-hand side's neighbour.
-// Each right-hand side runs the same unrolled code for every RB, so a result
-// does not depend on how many right-hand sides share the wavefront.
+// One link slot of the hop applied to the nrb <= RB right-hand sides r0.. of one
+// site: the link is loaded once and multiplies every right-hand side's
+// neighbour.  With a full group the neighbour loads are issued together, with no
+// branch between them; right-hand sides that idle are computed and dropped by
+// the caller.  Each right-hand side runs the same unrolled code for every RB, so
+// a result does not depend on how many right-hand sides share the wavefront.
 template <int SP, int LK, class R, int MU, bool FWD, bool PIN, int RB>
 __device__ __forceinline__ void st_slot(const Geo& g, const Site& s, uint32_t e, const SpinorView& in, int r0,
-                                        unsigned busy, const void* ub, cx<R> o[RB][3])
+                                        int nrb, const void* ub, cx<R> o[RB][3])
 {
   const uint32_t n = neigh_eo(g, s, MU, FWD);
   cx<R> u[9];
   constexpr int slot = 2 * MU + (FWD ? 0 : 1);
   const double det[2] = {1, 0};
   LkAcc<LK, R>::load(ub, g.stride, slot, e, det, u);
+  cx<R> v[RB][3];
+  if (RB == 1 || nrb == RB) {
+#pragma unroll
+    for (int b = 0; b < RB; b++) SpAcc<SP, R, 3>::load(in, g.stride, n, v[b], r0 + b);
+  } else {
+#pragma unroll
+    for (int b = 0; b < RB; b++)
+      if (b < nrb) SpAcc<SP, R, 3>::load(in, g.stride, n, v[b], r0 + b);  // uniform over the wavefront
+  }
 #pragma unroll
   for (int b = 0; b < RB; b++) {
-    if (!((busy >> b) & 1u)) continue;  // uniform over the wavefront
-    cx<R> v[3];
-    SpAcc<SP, R, 3>::load(in, g.stride, n, v, r0 + b);
+    if (RB > 1 && b >= nrb) break;
     cx<R> w[3];
-    mat_vec(u, v, w);
+    mat_vec(u, v[b], w);
 #pragma unroll
     for (int c = 0; c < 3; c++) o[b][c] = FWD ? o[b][c] + w[c] : o[b][c] - w[c];
     if constexpr (PIN) {
@@ -56,8 +65,8 @@ __device__ __forceinline__ void st_slot(const Geo& g, const Site& s, uint32_t e,
 }
 
 template <int SP, int LK, class R, int RB>
-__device__ __forceinline__ void st_hop_site(const Geo& g, int op, uint32_t e, const SpinorView& in, int r0,
-                                            unsigned busy, const LinkView& U, cx<R> o[RB][3])
+__device__ __forceinline__ void st_hop_site(const Geo& g, int op, uint32_t e, const SpinorView& in, int r0, int nrb,
+                                            const LinkView& U, cx<R> o[RB][3])
 {
   const Site s = site_of(g, op, e);
   const void* ub = U.data[op];
@@ -65,14 +74,14 @@ __device__ __forceinline__ void st_hop_site(const Geo& g, int op, uint32_t e, co
   for (int b = 0; b < RB; b++)
 #pragma unroll
     for (int c = 0; c < 3; c++) o[b][c] = mk<R>(0, 0);
-  st_slot<SP, LK, R, 0, true, false, RB>(g, s, e, in, r0, busy, ub, o);
-  st_slot<SP, LK, R, 0, false, false, RB>(g, s, e, in, r0, busy, ub, o);
-  st_slot<SP, LK, R, 1, true, false, RB>(g, s, e, in, r0, busy, ub, o);
-  st_slot<SP, LK, R, 1, false, true, RB>(g, s, e, in, r0, busy, ub, o);
-  st_slot<SP, LK, R, 2, true, false, RB>(g, s, e, in, r0, busy, ub, o);
-  st_slot<SP, LK, R, 2, false, false, RB>(g, s, e, in, r0, busy, ub, o);
-  st_slot<SP, LK, R, 3, true, false, RB>(g, s, e, in, r0, busy, ub, o);
-  st_slot<SP, LK, R, 3, false, true, RB>(g, s, e, in, r0, busy, ub, o);
+  st_slot<SP, LK, R, 0, true, false, RB>(g, s, e, in, r0, nrb, ub, o);
+  st_slot<SP, LK, R, 0, false, false, RB>(g, s, e, in, r0, nrb, ub, o);
+  st_slot<SP, LK, R, 1, true, false, RB>(g, s, e, in, r0, nrb, ub, o);
+  st_slot<SP, LK, R, 1, false, true, RB>(g, s, e, in, r0, nrb, ub, o);
+  st_slot<SP, LK, R, 2, true, false, RB>(g, s, e, in, r0, nrb, ub, o);
+  st_slot<SP, LK, R, 2, false, false, RB>(g, s, e, in, r0, nrb, ub, o);
+  st_slot<SP, LK, R, 3, true, false, RB>(g, s, e, in, r0, nrb, ub, o);
+  st_slot<SP, LK, R, 3, false, true, RB>(g, s, e, in, r0, nrb, ub, o);
 }
 
 // right-hand side r is skipped when skip_flags[r*skip_stride] (or, with skip_n > 1,
@@ -108,7 +117,7 @@ __global__ void __launch_bounds__(256) k_st_hop(Geo g, int op, SpinorView out, S
     return;
   }
   cx<R> o[RB][3];
-  st_hop_site<SP, LK, R, RB>(g, op, m.e, in, r0, busy, U, o);
+  st_hop_site<SP, LK, R, RB>(g, op, m.e, in, r0, nrb, U, o);
 #pragma unroll
   for (int b = 0; b < RB; b++) {
     if (b >= nrb) continue;
@@ -161,7 +170,7 @@ __global__ void __launch_bounds__(256) k_cgm_hop2(Geo g, SpinorView t, LinkView 
     return;
   }
   cx<R> o[RB][3];
-  st_hop_site<SP, LK, R, RB>(g, 0, m.e, t, r0, busy, U, o);
+  st_hop_site<SP, LK, R, RB>(g, 0, m.e, t, r0, nrb, U, o);
 #pragma unroll
   for (int b = 0; b < RB; b++) {
     if (b >= nrb) continue;
