@@ -65,8 +65,8 @@ def test_persistent_hop_matches_one_shot(setups, monkeypatch):
     workgroup slots twice (64x16^3: 512 blocks of 256 sites).  Its hop matches the
     oracle to <= 1e-14; the hop and an FP64 eo solve (every epilogue mode) give the
     same bits as the one-shot kernel (NQ_PERSIST=0); and a source solved alone
-    (persistent kernel) gives the same bits as inside a batch of two (multi-RHS
-    kernel)."""
+    through the batched solver gives the same bits as inside a batch of two (the
+    register-blocked FP64 kernel with one and with two right-hand sides)."""
     L = (64, 16, 16, 16)
     ctx, g, geo, conf, ce, co = setups(L, 3)
     x = gauss_eo(geo.Vh, 9)
