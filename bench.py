@@ -12,8 +12,12 @@ before the timed region.
 
 Also reported: the plain FP64 solve (same workload), the FP64 single-parity hop
 ("Dslash") bandwidth over 1000 applies, the roofline of the dominant kernel of
-the timed workload (HIP-event ledger kept by the library), and the CPU oracle
-timed on the host cores on a bounded sample of the FP64 solve.
+the timed workload (HIP-event ledger kept by the library), and the CPU oracle's
+full FP64 solve of the same workload timed on the host cores.  That oracle
+solve is also the reference for the error of each precision's GPU solution;
+the 8^4 case (kappa 0.16, mu 0.01, Z2 source on the even sites, residue 1e-24)
+adds the error against the oracle's __float128 solve (--quad does the same at
+the full lattice).
 """
 import argparse
 import json
@@ -40,7 +44,7 @@ def main():
     ap.add_argument("--lattice", default="64,32,32,32", help="T,LX,LY,LZ")
     ap.add_argument("--prec", default="f16", choices=["f64", "f32", "f16", "f16h"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-iters", type=int, default=6, help="oracle CG iterations timed for the CPU baseline")
+    ap.add_argument("--quad", action="store_true", help="also the __float128 oracle solve at the full lattice")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-stag", action="store_true", help="skip the staggered multi-shift leg")
     ap.add_argument("--no-24", action="store_true", help="skip the 24^3x48 FP64 Dslash and solve leg")
@@ -89,6 +93,7 @@ def main():
     ctx.sync()
     elapsed = time.perf_counter() - t0
     solves_per_s = args.steps / elapsed
+    gpu_sols = {args.prec: fx.download()}  # each precision's solution, for the oracle's error check
     log(f"[bench] timed {args.steps} {args.prec} solves: {elapsed*1e3/args.steps:.2f} ms/solve "
         f"({solves_per_s:.2f} solves/s), iters={stats[-1]['iters']}")
 
@@ -100,6 +105,7 @@ def main():
     st64 = [solve("f64") for _ in range(n64)]
     ctx.sync()
     t64 = (time.perf_counter() - t0) / n64
+    gpu_sols["f64"] = fx.download()
     log(f"[bench] FP64 solve: {t64*1e3:.2f} ms, iters={st64[-1]['iters']}")
 
     # the other inner-precision variants of the same workload
@@ -113,6 +119,7 @@ def main():
         sv = [solve(pv) for _ in range(3)]
         ctx.sync()
         tv = (time.perf_counter() - t0) / 3
+        gpu_sols[pv] = fx.download()
         variants[pv] = {"ms_per_solve": round(tv * 1e3, 3), "iters": sv[-1]["iters"],
                         "reliable_updates": sv[-1]["reliable_updates"],
                         "true_rel_res_sq": sv[-1]["check_rel_res_sq"]}
@@ -145,6 +152,8 @@ def main():
     log(f"[bench] Dslash FP64: {ms_hop*1e3:.1f} us/apply, {dslash_gbs:.0f} GB/s; "
         f"{args.prec}: {ms_hop16*1e3:.1f} us/apply, {bytes_hop16/(ms_hop16*1e-3)/1e9:.0f} GB/s")
 
+    small = None if args.no_cpu else small_solves()
+
     serve = None
     stag = None
     lat24 = None
@@ -163,10 +172,12 @@ def main():
         if not args.no_stag:
             stag = stag_bench(args)
 
-    # CPU baseline: the oracle's FP64 CG on the same inputs, bounded sample
+    # CPU baseline: the oracle's full FP64 solve of the same workload, which also
+    # checks every precision's GPU solution (and the 8^4 solves, with __float128)
     cpu = None
+    accuracy = None
     if not args.no_cpu:
-        cpu = cpu_baseline(L, conf, src, kappa, mu, args.cpu_iters, st64[-1]["iters"])
+        cpu, accuracy = cpu_baseline(L, conf, src, kappa, mu, residue, gpu_sols, small, args.quad)
 
     out = {
         "metric": METRIC,
@@ -208,6 +219,7 @@ def main():
         "serve": serve,
         "staggered_cgm": stag,
         "cpu_baseline": cpu,
+        "accuracy": accuracy,
     }
     print(json.dumps(out), flush=True)
     if args.no_serve and args.no_stag and args.no_24 and args.no_clover:
@@ -490,32 +502,89 @@ def traffic_of(kernel, L):
     return None
 
 
-def cpu_baseline(L, conf, src, kappa, mu, iters, fp64_iters):
-    """Time `iters` iterations of the oracle's FP64 CG on K K^+ (the reference's
-    unfused pass sequence) at the full lattice and scale to a complete solve of
-    fp64_iters iterations plus the eo preparation/reconstruction (2 K applies)."""
+def small_solves():
+    """GPU eo solves of the 8^4 case (kappa 0.16, mu 0.01, antiperiodic t, seeded
+    SU(3) links U=exp(i 0.3 H), one Z2 source on the even sites, nissa residue
+    1e-24) in every inner precision; the oracle checks them in the CPU leg."""
+    import nissa_amd as nq
+
+    L = (8, 8, 8, 8)
+    kappa, mu, residue = 0.16, 0.01, 1e-24
+    conf = nq.gauge_random_lx(L, 1, 0.3)
+    nq.gauge_phase_lx(L, conf, [1.0, 0, 0, 0])
+    src = nq.source_lx(L, 7, "z2", 12, 1)
+    out = {"L": L, "kappa": kappa, "mu": mu, "residue": residue, "conf": conf, "src": src, "sols": {}, "iters": {}}
+    with nq.Ctx(L, 0) as ctx:
+        g = nq.Gauge(ctx, lx=conf)
+        fs = ctx.field(nq.SPINCOLOR, nq.LX, src)
+        fx = ctx.field(nq.SPINCOLOR, nq.LX)
+        for prec in ("f64", "f32", "f16", "f16h"):
+            st = nq.tm_solve_eoprec(ctx, g, kappa, mu, 100000, residue, fx, fs, prec=prec)
+            out["sols"][prec] = fx.download()
+            out["iters"][prec] = st["iters"]
+        for f in (fs, fx):
+            f.close()
+        g.close()
+    return out
+
+
+def _rel_l2(a, b):
     import numpy as np
 
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def eo_solve_quad(geo, conf, kappa, mu, src):
+    """The oracle's eo solve of D psi = eta (cg_invert_tmD_eoprec.cpp:102-150) with the
+    K K^+ system (eq. 9) solved by its __float128 defect correction to
+    |r|^2/|b|^2 < 1e-30; eqs. 8 and 10-11 in FP64."""
+    ce, co = geo.split(conf)
+    se, so = geo.split(src)
+    e = geo.tmDee(kappa, mu, se, inverse=True)
+    phi = so + 0.5 * geo.tm_hop(1, ce, co, e)
+    phi[:, 2:] *= -1  # gamma5 = diag(1, 1, -1, -1)
+    y, st = geo.tm_cg128_kernsq(ce, co, kappa, mu, 1e-30, phi)
+    xo = geo.tmDkern(ce, co, kappa, -mu, y)
+    xe = geo.tmDee(kappa, mu, se + 0.5 * geo.tm_hop(0, ce, co, xo), inverse=True)
+    return geo.paste(xe, xo), st
+
+
+def cpu_baseline(L, conf, src, kappa, mu, residue, gpu_sols, small, quad):
+    """The CPU leg.  Times the oracle's full FP64 eo solve of the workload on the
+    host cores (OpenMP, the reference's unfused pass sequence) as the baseline, and
+    uses the oracle's solutions to check the GPU's: the relative L2 error of each
+    precision's solution against the FP64 oracle solve, and for the 8^4 case also
+    against the __float128 solve (at the full lattice too with `quad`)."""
     from oracle import oracle as orc
 
     cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
     geo = orc.Geom(L)
-    ce, co = geo.split(conf)
-    se, so = geo.split(src)
-    phi = so.copy()  # a representative odd source for the CG sample
     t0 = time.perf_counter()
-    geo.tmDkern_square(ce, co, kappa, mu, phi)
-    t_apply = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    _, st = geo.tm_cg_kernsq(ce, co, kappa, mu, iters, 1e-20, phi)
-    t_cg = time.perf_counter() - t0
-    t_iter = t_cg / max(1, st["iters"] + 2)  # the sample also does the initial and final residual matvecs
-    t_solve = t_iter * fp64_iters + 2 * t_apply
-    log(f"[bench] CPU oracle: K K^+ apply {t_apply:.2f}s, {st['iters']} CG iterations in {t_cg:.2f}s "
-        f"-> {t_solve:.1f}s per {fp64_iters}-iteration solve on {cores} threads")
-    return {"value": round(1.0 / t_solve, 6), "unit": "solves/s", "cores": cores, "kind": "port",
-            "sample": f"oracle FP64 CG on K K^+ at {L[1]}^3x{L[0]}: {st['iters']} iterations timed "
-                      f"({t_cg:.1f}s) plus one apply, scaled to the {fp64_iters}-iteration FP64 solve"}
+    x_cpu, st = geo.tm_solve_eoprec(conf, kappa, mu, 100000, residue, src)
+    t_solve = time.perf_counter() - t0
+    log(f"[bench] CPU oracle: FP64 eo solve {t_solve:.1f}s, {st['iters']} iterations on {cores} threads")
+    cpu = {"value": round(1.0 / t_solve, 6), "unit": "solves/s", "cores": cores, "kind": "port",
+           "sample": f"one full oracle FP64 eo solve of the workload at {L[1]}^3x{L[0]} "
+                     f"({st['iters']} CG iterations, {t_solve:.1f}s)"}
+    acc = {f"{L[1]}^3x{L[0]}": {"vs_fp64_oracle": {p: _rel_l2(x, x_cpu) for p, x in gpu_sols.items()},
+                                  "oracle_iters": st["iters"]}}
+    if quad:
+        t0 = time.perf_counter()
+        xq, stq = eo_solve_quad(geo, conf, kappa, mu, src)
+        log(f"[bench] CPU oracle: __float128 eo solve {time.perf_counter() - t0:.1f}s")
+        acc[f"{L[1]}^3x{L[0]}"]["vs_quad_oracle"] = {p: _rel_l2(x, xq) for p, x in gpu_sols.items()}
+        acc[f"{L[1]}^3x{L[0]}"]["fp64_oracle_vs_quad"] = _rel_l2(x_cpu, xq)
+    if small is not None:
+        gs = orc.Geom(small["L"])
+        x64, s64 = gs.tm_solve_eoprec(small["conf"], small["kappa"], small["mu"], 100000, small["residue"], small["src"])
+        xq, stq = eo_solve_quad(gs, small["conf"], small["kappa"], small["mu"], small["src"])
+        acc["8^4"] = {"kappa": small["kappa"], "mu": small["mu"], "residue": small["residue"],
+                      "gpu_iters": small["iters"], "oracle_iters": s64["iters"],
+                      "vs_fp64_oracle": {p: _rel_l2(x, x64) for p, x in small["sols"].items()},
+                      "vs_quad_oracle": {p: _rel_l2(x, xq) for p, x in small["sols"].items()},
+                      "fp64_oracle_vs_quad": _rel_l2(x64, xq)}
+    log(f"[bench] accuracy: {json.dumps(acc)}")
+    return cpu, acc
 
 
 if __name__ == "__main__":
