@@ -166,6 +166,30 @@ def test_cg_kernsq_f64_matches_oracle(setups, L):
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32", "f16", "f16h"])
+def test_cg_kernsq_vs_quad_oracle(setups, prec):
+    """The K K^+ solve in every inner precision against the oracle's __float128
+    defect-correction solve (|r|^2/|b|^2 < 1e-30) and its FP64 CG, on 8^4 to
+    residue 1e-24: the GPU solution is as close to the quad solution as the FP64
+    oracle's is (both stop at |r|/|b| = 1e-12), and within 1e-11 of the FP64
+    oracle's."""
+    L = (8, 8, 8, 8)
+    ctx, g, geo, conf, ce, co = setups(L)
+    kappa, mu = 0.16, 0.01
+    b = gauss_eo(geo.Vh, 21)
+    xq, stq = geo.tm_cg128_kernsq(ce, co, kappa, mu, 1e-30, b)
+    assert stq["rc"] == 0 and stq["rel_res_sq"] < 1e-30
+    x64, _ = geo.tm_cg_kernsq(ce, co, kappa, mu, 10000, 1e-24, b)
+    fb = ctx.field(nq.SPINCOLOR, nq.ODD, b)
+    fx = ctx.field(nq.SPINCOLOR, nq.ODD)
+    st = nq.tm_cg_kernsq(ctx, g, kappa, mu, 10000, 1e-24, fx, fb, prec=prec)
+    assert st["rc"] == 0 and st["converged"] and st["rel_res_sq"] < 2e-24, st
+    x = fx.download()
+    e_q, e_64 = rel_l2(x, xq), rel_l2(x, x64)
+    assert e_q < 2 * max(rel_l2(x64, xq), 1e-13), (e_q, rel_l2(x64, xq))
+    assert e_64 < 1e-11, e_64
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32", "f16", "f16h"])
 def test_eo_solve_all_precisions(setups, prec):
     """inv_tmD_cg_eoprec on 8^4 (config 0 shape): Z2 source on even sites, kappa 0.16,
     mu 0.01, residue 1e-24; solution within 1e-10 of the oracle's FP64 solve."""
