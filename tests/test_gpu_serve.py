@@ -71,6 +71,30 @@ def test_server_point_sources_and_mix_equality():
     srv.close()
 
 
+def test_server_nan_source_fails_alone():
+    """A request whose source holds a NaN fails with NQ_ENAN; the requests batched
+    with it in the same group still converge, with the same bits as alone."""
+    L = (4, 6, 4, 8)
+    conf = tm_conf(L, 16)
+    good = [nq.source_lx(L, 90 + i, "gauss") for i in range(2)]
+    bad = [nq.source_lx(L, 95 + i, "gauss") for i in range(2)]
+    bad[1][17, 2, 1, 0] = np.nan
+    srv = nq.Server(max_groups=1)
+    vol = srv.add_volume(L, conf)
+    alone = point_sols(L, 2)
+    r = srv.wait(srv.submit(vol, KAPPA, 0.01, 1e-20, prec="f32", srcs=good, sols=alone))
+    assert r["status"] == nq.NQ_OK
+    mixed = point_sols(L, 2)
+    ids = [srv.submit(vol, KAPPA, 0.02, 1e-20, prec="f32", srcs=bad),
+           srv.submit(vol, KAPPA, 0.01, 1e-20, prec="f32", srcs=good, sols=mixed)]
+    rb, rg = srv.wait(ids[0]), srv.wait(ids[1])
+    assert rb["status"] == nq.NQ_ENAN, rb
+    assert rg["status"] == nq.NQ_OK and rg["max_rel_res_sq"] < 1e-20, rg
+    for a, m in zip(alone, mixed):
+        assert np.array_equal(a, m)
+    srv.close()
+
+
 def test_server_admission_by_footprint():
     """With a budget that fits one batch group (64 MiB of headroom plus this tiny
     lattice's fields), 42 sources stream through its 16 slots; with a budget below
