@@ -127,6 +127,24 @@ def test_st_cgm_batch_equals_alone(st_setups):
         assert np.array_equal(alone[0][k].download(), batch[1][k].download())
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_st_cgm_nan_source_is_rejected(st_setups, prec):
+    """A NaN in one source of a batched multi-shift solve fails the call with
+    NQ_ENAN (nissa CRASHes on a NaN norm); the context then solves normally."""
+    L = (4, 6, 4, 8)
+    ctx, g, geo, conf, ce, co = st_setups(L)
+    shifts = [0.001, 0.05]
+    srcs = [gauss_eo(geo.Vh, 70 + i, ncomp=3) for i in range(2)]
+    srcs[1][5, 2, 1] = np.nan
+    fsrc = [ctx.field(nq.COLOR, nq.EVEN, s) for s in srcs]
+    sols = [[ctx.field(nq.COLOR, nq.EVEN) for _ in shifts] for _ in srcs]
+    with pytest.raises(nq.NqError) as e:
+        nq.st_cgm(ctx, g, shifts, 20000, [1e-20] * 2, sols, fsrc, prec=prec)
+    assert e.value.code == nq.NQ_ENAN
+    res = nq.st_cgm(ctx, g, shifts, 20000, [1e-20] * 2, sols[:1], fsrc[:1], prec=prec)
+    assert res[0]["rc"] == nq.NQ_OK and max(res[0]["true_res_sq"]) < 2e-20
+
+
 def test_tmQ2_cg_matches_oracle():
     L = (4, 6, 4, 8)
     conf = tm_conf(L, 3)
