@@ -318,13 +318,17 @@ static double csite(int sp) { return sp == SP_F64 ? 48.0 : 24.0; }
 static double lbytes(int lk) { return lk == LK_F64_18 ? 144.0 : 72.0; }
 
 // Right-hand sides per wavefront of the staggered hops: one for a single right-hand
-// side, else 4 (NQ_ST_RB = 1 or 2 overrides); every choice gives the same bits.
-// The launch shape for RB: W = 4 chunks per workgroup, one RB group per wavefront.
-static int st_rb(int nr)
+// side; 8 for FP32 batches of a multiple of 8 (the 16-system refinement groups:
+// 538 vs 601 us per hop at 32^3x64; 12 sources split 8 + 4 measured slower than
+// 4 + 4 + 4), else 4.  NQ_ST_RB = 1, 2, 4 or 8 overrides; every choice gives the
+// same bits.  The launch shape for RB: W = 4 chunks per workgroup, one RB group
+// per wavefront.
+static int st_rb(int nr, int sp)
 {
   if (nr <= 1) return 1;
   const char* e = getenv("NQ_ST_RB");
-  const int v = e ? atoi(e) : 4;
+  const int v = e ? atoi(e) : (sp == SP_F32 && nr % 8 == 0 ? 8 : 4);
+  if (v == 8) return sp == SP_F32 ? 8 : 4;
   return v == 1 || v == 2 ? v : 4;
 }
 static Shape st_shape(int nr, int RB)
@@ -350,6 +354,13 @@ static void st_hop_go(nq_ctx* ctx, int RB, int op, const SpinorView& out, const 
                              write, sh, nr, nchunks, ctx->partial, skip_flags, skip_stride, skip_n); break;
   case 2: hipLaunchKernelGGL((k_st_hop<SP, LK, MODE, 2>), grid, blk, 0, ctx->stream, g, op, out, in, U, c1, c2, coef, aux,
                              write, sh, nr, nchunks, ctx->partial, skip_flags, skip_stride, skip_n); break;
+  case 8:
+    if constexpr (SP == SP_F32) {
+      hipLaunchKernelGGL((k_st_hop<SP, LK, MODE, 8>), grid, blk, 0, ctx->stream, g, op, out, in, U, c1, c2, coef, aux,
+                         write, sh, nr, nchunks, ctx->partial, skip_flags, skip_stride, skip_n);
+      break;
+    }
+    [[fallthrough]];
   default: hipLaunchKernelGGL((k_st_hop<SP, LK, MODE, 4>), grid, blk, 0, ctx->stream, g, op, out, in, U, c1, c2, coef,
                               aux, write, sh, nr, nchunks, ctx->partial, skip_flags, skip_stride, skip_n); break;
   }
@@ -370,7 +381,7 @@ int launch_st_hop(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& 
   hipEvent_t ev = nullptr;
   const double by = (8 * lbytes(lk) + csite(sp) * (1 + (aux.data ? 1 : 0) + ((mode == 0 || write) ? 1 : 0))) * nr;
   prof_begin(ctx, mode == 0 ? "st_hop" : "st_hop_red", by * (double)g.Vh, &ev);
-  const int RB = st_rb(nr);
+  const int RB = st_rb(nr, sp);
 #define NQ_ST(SPV, LKV)                                                                                              \
   switch (mode) {                                                                                                    \
   case 0: st_hop_go<SPV, LKV, 0>(ctx, RB, op, out, in, U, c1, c2, coef_dev, aux, write, nr, skip_flags, skip_stride, \
@@ -568,7 +579,7 @@ int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, co
   none.ncomp = 3;
   hipEvent_t ev = nullptr;
   prof_begin(ctx, "cgm_hop1", (8 * lbytes(lk) + 2 * csite(sp)) * nr * (double)g.Vh, &ev);
-  const int RB = st_rb(nr);
+  const int RB = st_rb(nr, sp);
   const int* skip = (const int*)&st->done;
   const int sstr = (int)(sizeof(CgmState) / sizeof(int));
   if (sp == SP_F64)
@@ -595,6 +606,9 @@ int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, co
                               st, sh, nr, L.nchunks, ctx->partial); break;                                        \
   }
     if (sp == SP_F64) { NQ_C2(SP_F64, LK_F64_18) }
+    else if (RB == 8)
+      hipLaunchKernelGGL((k_cgm_hop2<SP_F32, LK_F32_18, 8>), grid2, blk2, 0, ctx->stream, g, t, U, r, sol, ps, sstride, st,
+                         sh, nr, L.nchunks, ctx->partial);
     else { NQ_C2(SP_F32, LK_F32_18) }
 #undef NQ_C2
   }

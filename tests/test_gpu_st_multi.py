@@ -112,17 +112,19 @@ def test_st_cgm_matches_oracle(st_setups, prec):
             assert rel_l2(fsol[i][k].download(), ref[k]) < 1e-10, (i, k)
 
 
-def test_st_cgm_batch_equals_alone(st_setups):
-    """A source solved alone and inside a batch of three gives identical bits."""
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_st_cgm_batch_equals_alone(st_setups, prec):
+    """A source solved alone and inside a batch of three gives identical bits (in
+    mixed precision also through the refinement of the systems that need it)."""
     L = (4, 6, 4, 8)
     ctx, g, geo, conf, ce, co = st_setups(L)
     shifts = [0.001, 0.05]
     srcs = [gauss_eo(geo.Vh, 60 + i, ncomp=3) for i in range(3)]
     fsrc = [ctx.field(nq.COLOR, nq.EVEN, s) for s in srcs]
     batch = [[ctx.field(nq.COLOR, nq.EVEN) for _ in shifts] for _ in srcs]
-    nq.st_cgm(ctx, g, shifts, 20000, [1e-22, 1e-22], batch, fsrc)
+    nq.st_cgm(ctx, g, shifts, 20000, [1e-22, 1e-22], batch, fsrc, prec=prec)
     alone = [[ctx.field(nq.COLOR, nq.EVEN) for _ in shifts]]
-    nq.st_cgm(ctx, g, shifts, 20000, [1e-22, 1e-22], alone, fsrc[1:2])
+    nq.st_cgm(ctx, g, shifts, 20000, [1e-22, 1e-22], alone, fsrc[1:2], prec=prec)
     for k in range(len(shifts)):
         assert np.array_equal(alone[0][k].download(), batch[1][k].download())
 
