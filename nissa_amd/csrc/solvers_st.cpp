@@ -72,6 +72,10 @@ static int apply_DDee(nq_ctx* ctx, const LinkView& U, double mass2, const double
 // prec NQ_PREC_F64: plain CG (cg_invert semantics); NQ_PREC_F32: FP32 inner iteration
 // with FP64 reliable updates.  x64 holds the guess on entry (use_guess) and the
 // solution on exit; residues[i] is the target of |r_i|^2/|b_i|^2.  States: ctx->state[0..nr).
+// Smallest zeta of the reference shift for which the mixed multi-shift CG replaces
+// its Krylov residual by the FP64 residual of that shift divided by zeta.
+constexpr double kMinZetaReplace = 1e-2;
+
 static int st_cg_batched(nq_ctx* ctx, const nq_gauge* g, int nr, const double* shifts, int niter,
                          const double* residues, int prec, const SpinorView& x64, const SpinorView& b, bool use_guess,
                          nq_solve_stats* st)
@@ -286,9 +290,24 @@ static int st_cgm_batched(nq_ctx* ctx, const LinkView& U, int nr, int nshift, co
       // residual by the FP64 residual of the hardest running shift kref,
       // r = (b - A(sigma_kref) x_kref) / zeta_kref, which it equals in exact arithmetic
       NQ_TRY(launch_cgm_flush(ctx, *x64, sol, csz, nshift, dst, nr, 1));
+      std::vector<int> rep;
       for (int i : upd) {
         const int kr = hst[i].kref;
         const double zeta = hst[i].zas[kr];
+        if (zeta < kMinZetaReplace) {
+          // The Krylov residual is that of the unshifted base system.  When it lags
+          // this far behind the hardest running shift (a near-zero mode of
+          // -1/4 Deo Doe), the replacement would amplify the FP32 error of x_kref
+          // by 1/zeta and the FP32 recursion breaks down (measured: |r|^2 growing
+          // to overflow on 8x4x2x2).  The source leaves the multi-shift iteration
+          // with its flushed FP64 solutions; its unconverged shifts go to the
+          // per-shift refinement below.
+          hst[i].done = 1;
+          hst[i].reliable = 0;
+          hst[i].nreliable += 1;
+          continue;
+        }
+        rep.push_back(i);
         const SpinorView xk = shifted(*x64, (int64_t)i * x64->rstride + kr * csz);
         NQ_TRY(launch_st_hop(ctx, NQ_ODD, t64, xk, *U64, 1.0, 0.0, nullptr, none, 0, 1, 1, FIN_NONE, nullptr));
         NQ_TRY(launch_st_hop(ctx, NQ_EVEN, s64, t64, *U64, -0.25, shifts[kr], nullptr, xk, 0, 1, 1, FIN_NONE, nullptr));
@@ -298,7 +317,7 @@ static int st_cgm_batched(nq_ctx* ctx, const LinkView& U, int nr, int nshift, co
                               -1.0 / zeta, s64, 0, none, FIN_STORE0, ctx->state + i, 1));
       }
       NQ_TRY(get_states(ctx, 0, nr));
-      for (int i : upd) {
+      for (int i : rep) {
         CgmState& h = hst[i];
         h.rr = ctx->state_host[i].extra[0];
         for (int k = 0; k < nshift; k++)

@@ -25,6 +25,51 @@ namespace nq {
 template <int SP>
 using RealOfS = typename std::conditional<SP == SP_F64, double, float>::type;
 
+// LDS staging of colour fields (three complex per site): the records of the 256
+// sites of a workgroup for each of its right-hand sides, [record][site].  FP64:
+// three 16-byte records; FP32: one 16-byte record (components 0, 1) and the
+// 8-byte leftover (component 2).  The decoded values are those of a memory load.
+template <int SP>
+struct StStage {
+  static constexpr int BYTES = SP == SP_F64 ? 3 * kStageSites * 16 : kStageSites * 24;
+  __device__ __forceinline__ static void put(char* lds, const SpinorView& v, int64_t stride, uint32_t e, bool valid,
+                                             int t, int r)
+  {
+    constexpr int EB = SP == SP_F64 ? 16 : 8;
+    const char* base = (const char*)v.data + (int64_t)r * v.rstride * EB;
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc(base, (int64_t)3 * stride * EB);
+    if constexpr (SP == SP_F64) {
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        nq_u4 u = {0u, 0u, 0u, 0u};
+        if (valid) u = __builtin_amdgcn_raw_buffer_load_b128(rs, e * 16u, (uint32_t)(q * stride * 16), 0);
+        *(nq_u4*)(lds + (q * kStageSites + t) * 16) = u;
+      }
+    } else {
+      nq_u4 u = {0u, 0u, 0u, 0u};
+      nq_fv2 f = {0.f, 0.f};
+      if (valid) {
+        u = __builtin_amdgcn_raw_buffer_load_b128(rs, e * 16u, 0u, 0);
+        f = __builtin_bit_cast(nq_fv2, __builtin_amdgcn_raw_buffer_load_b64(rs, e * 8u, (uint32_t)(stride * 16), 0));
+      }
+      *(nq_u4*)(lds + t * 16) = u;
+      *(nq_fv2*)(lds + kStageSites * 16 + t * 8) = f;
+    }
+  }
+  template <class R>
+  __device__ __forceinline__ static void get(const char* lds, int d, cx<R> s[3])
+  {
+    if constexpr (SP == SP_F64) {
+#pragma unroll
+      for (int q = 0; q < 3; q++) decode_record<16, R>(*(const nq_u4*)(lds + (q * kStageSites + d) * 16), s + q);
+    } else {
+      decode_record<8, R>(*(const nq_u4*)(lds + d * 16), s);
+      const nq_fv2 f = *(const nq_fv2*)(lds + kStageSites * 16 + d * 8);
+      s[2] = mk<R>((R)f.x, (R)f.y);
+    }
+  }
+};
+
 // One link slot of the hop applied to the nrb <= RB right-hand sides r0.. of one
 // site: the link is loaded once and multiplies every right-hand side's
 // neighbour.  With a full group the neighbour loads are issued together, with no
@@ -33,7 +78,7 @@ using RealOfS = typename std::conditional<SP == SP_F64, double, float>::type;
 // a result does not depend on how many right-hand sides share the wavefront.
 template <int SP, int LK, class R, int MU, bool FWD, bool PIN, int RB>
 __device__ __forceinline__ void st_slot(const Geo& g, const Site& s, uint32_t e, const SpinorView& in, int r0,
-                                        int nrb, const void* ub, cx<R> o[RB][3])
+                                        int nrb, const void* ub, cx<R> o[RB][3], const char* lds, uint32_t lds_e0)
 {
   const uint32_t n = neigh_eo(g, s, MU, FWD);
   cx<R> u[9];
@@ -41,7 +86,12 @@ __device__ __forceinline__ void st_slot(const Geo& g, const Site& s, uint32_t e,
   const double det[2] = {1, 0};
   LkAcc<LK, R>::load(ub, g.stride, slot, e, det, u);
   cx<R> v[RB][3];
-  if (RB == 1 || nrb == RB) {
+  const uint32_t d = n - lds_e0;
+  if (d < (uint32_t)kStageSites) {  // neighbour staged in LDS
+#pragma unroll
+    for (int b = 0; b < RB; b++)
+      if (RB == 1 || b < nrb) StStage<SP>::get(lds + b * StStage<SP>::BYTES, (int)d, v[b]);
+  } else if (RB == 1 || nrb == RB) {
 #pragma unroll
     for (int b = 0; b < RB; b++) SpAcc<SP, R, 3>::load(in, g.stride, n, v[b], r0 + b);
   } else {
@@ -66,7 +116,7 @@ __device__ __forceinline__ void st_slot(const Geo& g, const Site& s, uint32_t e,
 
 template <int SP, int LK, class R, int RB>
 __device__ __forceinline__ void st_hop_site(const Geo& g, int op, uint32_t e, const SpinorView& in, int r0, int nrb,
-                                            const LinkView& U, cx<R> o[RB][3])
+                                            const LinkView& U, cx<R> o[RB][3], const char* lds, uint32_t lds_e0)
 {
   const Site s = site_of(g, op, e);
   const void* ub = U.data[op];
@@ -74,14 +124,14 @@ __device__ __forceinline__ void st_hop_site(const Geo& g, int op, uint32_t e, co
   for (int b = 0; b < RB; b++)
 #pragma unroll
     for (int c = 0; c < 3; c++) o[b][c] = mk<R>(0, 0);
-  st_slot<SP, LK, R, 0, true, false, RB>(g, s, e, in, r0, nrb, ub, o);
-  st_slot<SP, LK, R, 0, false, false, RB>(g, s, e, in, r0, nrb, ub, o);
-  st_slot<SP, LK, R, 1, true, false, RB>(g, s, e, in, r0, nrb, ub, o);
-  st_slot<SP, LK, R, 1, false, true, RB>(g, s, e, in, r0, nrb, ub, o);
-  st_slot<SP, LK, R, 2, true, false, RB>(g, s, e, in, r0, nrb, ub, o);
-  st_slot<SP, LK, R, 2, false, false, RB>(g, s, e, in, r0, nrb, ub, o);
-  st_slot<SP, LK, R, 3, true, false, RB>(g, s, e, in, r0, nrb, ub, o);
-  st_slot<SP, LK, R, 3, false, true, RB>(g, s, e, in, r0, nrb, ub, o);
+  st_slot<SP, LK, R, 0, true, false, RB>(g, s, e, in, r0, nrb, ub, o, lds, lds_e0);
+  st_slot<SP, LK, R, 0, false, false, RB>(g, s, e, in, r0, nrb, ub, o, lds, lds_e0);
+  st_slot<SP, LK, R, 1, true, false, RB>(g, s, e, in, r0, nrb, ub, o, lds, lds_e0);
+  st_slot<SP, LK, R, 1, false, true, RB>(g, s, e, in, r0, nrb, ub, o, lds, lds_e0);
+  st_slot<SP, LK, R, 2, true, false, RB>(g, s, e, in, r0, nrb, ub, o, lds, lds_e0);
+  st_slot<SP, LK, R, 2, false, false, RB>(g, s, e, in, r0, nrb, ub, o, lds, lds_e0);
+  st_slot<SP, LK, R, 3, true, false, RB>(g, s, e, in, r0, nrb, ub, o, lds, lds_e0);
+  st_slot<SP, LK, R, 3, false, true, RB>(g, s, e, in, r0, nrb, ub, o, lds, lds_e0);
 }
 
 // right-hand side r is skipped when skip_flags[r*skip_stride] (or, with skip_n > 1,
@@ -89,6 +139,21 @@ __device__ __forceinline__ void st_hop_site(const Geo& g, int op, uint32_t e, co
 __device__ __forceinline__ bool st_skipped(const int* skip_flags, int skip_stride, int skip_n, int r)
 {
   return skip_flags && (skip_flags[r * skip_stride] || (skip_n > 1 && skip_flags[r * skip_stride + 1]));
+}
+
+// Stage the records of the workgroup's 256 sites of the nrb right-hand sides r0..
+// of `in` (every thread of the workgroup takes part); returns the first site.
+template <int SP, int RB>
+__device__ __forceinline__ uint32_t stage_sources(char* lds, const Geo& g, const SpinorView& in, const Map& m, int r0,
+                                                  int nrb)
+{
+  const int t = (int)threadIdx.x;
+  const uint32_t e0 = (uint32_t)((m.chunk - (t >> 6)) * 64);
+#pragma unroll
+  for (int b = 0; b < RB; b++)
+    if (b < nrb) StStage<SP>::put(lds + b * StStage<SP>::BYTES, in, g.stride, e0 + t, e0 + t < (uint64_t)g.Vh, t, r0 + b);
+  __syncthreads();
+  return e0;
 }
 
 // out = c1 hop + c2 aux (real coefficients, per rhs from coef[2r], coef[2r+1] when
@@ -108,7 +173,7 @@ __global__ void __launch_bounds__(256) k_st_hop(Geo g, int op, SpinorView out, S
 #pragma unroll
   for (int b = 0; b < RB; b++)
     if (b < nrb && !st_skipped(skip_flags, skip_stride, skip_n, r0 + b)) busy |= 1u << b;
-  if (!busy || !m.active) {
+  if (!busy) {  // uniform over the workgroup (one group of right-hand sides)
     if constexpr (MODE != 0) {
 #pragma unroll
       for (int b = 0; b < RB; b++)
@@ -116,14 +181,18 @@ __global__ void __launch_bounds__(256) k_st_hop(Geo g, int op, SpinorView out, S
     }
     return;
   }
+  __shared__ __attribute__((aligned(16))) char lds[RB * StStage<SP>::BYTES];
+  const uint32_t e0 = stage_sources<SP, RB>(lds, g, in, m, r0, nrb);
+  // sites past the end (a partial last chunk) compute nothing but take part in
+  // the wavefront reductions with zero
   cx<R> o[RB][3];
-  st_hop_site<SP, LK, R, RB>(g, op, m.e, in, r0, nrb, U, o);
+  if (m.active) st_hop_site<SP, LK, R, RB>(g, op, m.e, in, r0, nrb, U, o, lds, e0);
 #pragma unroll
   for (int b = 0; b < RB; b++) {
     if (b >= nrb) continue;
     const int r = r0 + b;
     double red = 0;
-    if ((busy >> b) & 1u) {
+    if (m.active && ((busy >> b) & 1u)) {
       const double a = coef ? coef[2 * r] : c1, bb = coef ? coef[2 * r + 1] : c2;
 #pragma unroll
       for (int c = 0; c < 3; c++) o[b][c] = scal(o[b][c], (R)a);
@@ -163,21 +232,23 @@ __global__ void __launch_bounds__(256) k_cgm_hop2(Geo g, SpinorView t, LinkView 
 #pragma unroll
   for (int b = 0; b < RB; b++)
     if (b < nrb && !st[r0 + b].done && !st[r0 + b].reliable) busy |= 1u << b;
-  if (!busy || !m.active) {
+  if (!busy) {  // uniform over the workgroup
 #pragma unroll
     for (int b = 0; b < RB; b++)
       if (b < nrb) store_partial(0.0, partial, nchunks, r0 + b, m.chunk);
     return;
   }
+  __shared__ __attribute__((aligned(16))) char lds[RB * StStage<SP>::BYTES];
+  const uint32_t e0 = stage_sources<SP, RB>(lds, g, t, m, r0, nrb);
   cx<R> o[RB][3];
-  st_hop_site<SP, LK, R, RB>(g, 0, m.e, t, r0, nrb, U, o);
+  if (m.active) st_hop_site<SP, LK, R, RB>(g, 0, m.e, t, r0, nrb, U, o, lds, e0);
 #pragma unroll
   for (int b = 0; b < RB; b++) {
     if (b >= nrb) continue;
     const int r = r0 + b;
     const CgmState* S = st + r;
     double red = 0;
-    if ((busy >> b) & 1u) {
+    if (m.active && ((busy >> b) & 1u)) {
       const R betaa = (R)S->betaa;
       for (int k = 0; k < S->nshift; k++) {
         if (!S->run[k]) continue;

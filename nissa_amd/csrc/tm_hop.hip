@@ -426,39 +426,35 @@ __global__ void __launch_bounds__(256) k_tm_hop_rb(Geo g, int op, SpinorView out
                            e0 + threadIdx.x < (uint64_t)g.Vh, (int)threadIdx.x, r0 + b);
     __syncthreads();
   }
-  if (!m.active) {
-    if constexpr (MODE != 0) {
-#pragma unroll
-      for (int b = 0; b < RB; b++)
-        if (b < nrb) store_partial(0.0, partial, nchunks, r0 + b, m.chunk);
-    }
-    return;
-  }
   const char* lp = STG ? lds : nullptr;
   const uint32_t e = m.e;
-  const Site s = site_of(g, op, e);
-  const void* ub = U.data[op];
-  const double* det = &U.det[op][0][0];
-  const R anis = (R)anis_d;
-  const bool ua = use_anis != 0;
   cx<R> o[RB][12];
 #pragma unroll
   for (int b = 0; b < RB; b++)
 #pragma unroll
     for (int k = 0; k < 12; k++) o[b][k] = mk<R>(0, 0);
-  hop_slot_rb<SP, LK, R, 0, true, RB>(g, s, e, in, r0, nrb, busy, ub, det, anis, ua, o, lp, e0);
-  hop_slot_rb<SP, LK, R, 0, false, RB>(g, s, e, in, r0, nrb, busy, ub, det, anis, ua, o, lp, e0);
-  hop_slot_rb<SP, LK, R, 1, true, RB>(g, s, e, in, r0, nrb, busy, ub, det, anis, ua, o, lp, e0);
-  hop_slot_rb<SP, LK, R, 1, false, RB>(g, s, e, in, r0, nrb, busy, ub, det, anis, ua, o, lp, e0);
-  hop_slot_rb<SP, LK, R, 2, true, RB>(g, s, e, in, r0, nrb, busy, ub, det, anis, ua, o, lp, e0);
-  hop_slot_rb<SP, LK, R, 2, false, RB>(g, s, e, in, r0, nrb, busy, ub, det, anis, ua, o, lp, e0);
-  hop_slot_rb<SP, LK, R, 3, true, RB>(g, s, e, in, r0, nrb, busy, ub, det, anis, ua, o, lp, e0);
-  hop_slot_rb<SP, LK, R, 3, false, RB>(g, s, e, in, r0, nrb, busy, ub, det, anis, ua, o, lp, e0);
+  // sites past the end (a partial last chunk) compute nothing but take part in the
+  // wavefront reductions with zero
+  if (m.active) {
+    const Site s = site_of(g, op, e);
+    const void* ub = U.data[op];
+    const double* det = &U.det[op][0][0];
+    const R anis = (R)anis_d;
+    const bool ua = use_anis != 0;
+    hop_slot_rb<SP, LK, R, 0, true, RB>(g, s, e, in, r0, nrb, busy, ub, det, anis, ua, o, lp, e0);
+    hop_slot_rb<SP, LK, R, 0, false, RB>(g, s, e, in, r0, nrb, busy, ub, det, anis, ua, o, lp, e0);
+    hop_slot_rb<SP, LK, R, 1, true, RB>(g, s, e, in, r0, nrb, busy, ub, det, anis, ua, o, lp, e0);
+    hop_slot_rb<SP, LK, R, 1, false, RB>(g, s, e, in, r0, nrb, busy, ub, det, anis, ua, o, lp, e0);
+    hop_slot_rb<SP, LK, R, 2, true, RB>(g, s, e, in, r0, nrb, busy, ub, det, anis, ua, o, lp, e0);
+    hop_slot_rb<SP, LK, R, 2, false, RB>(g, s, e, in, r0, nrb, busy, ub, det, anis, ua, o, lp, e0);
+    hop_slot_rb<SP, LK, R, 3, true, RB>(g, s, e, in, r0, nrb, busy, ub, det, anis, ua, o, lp, e0);
+    hop_slot_rb<SP, LK, R, 3, false, RB>(g, s, e, in, r0, nrb, busy, ub, det, anis, ua, o, lp, e0);
+  }
 #pragma unroll
   for (int b = 0; b < RB; b++) {
     if (b >= nrb) continue;
     double nrm = 0;
-    if ((busy >> b) & 1u) nrm = hop_epilogue<SP, MODE, XP, R>(g, epi, out, e, r0 + b, epi.st ? epi.st + r0 + b : nullptr, o[b]);
+    if (m.active && ((busy >> b) & 1u)) nrm = hop_epilogue<SP, MODE, XP, R>(g, epi, out, e, r0 + b, epi.st ? epi.st + r0 + b : nullptr, o[b]);
     if constexpr (MODE != 0) store_partial(nrm, partial, nchunks, r0 + b, m.chunk);
   }
 }
