@@ -162,6 +162,14 @@ int nq_tmclov_kern_square(nq_ctx* ctx, const nq_gauge* g, nq_clover* cl, double 
 /* apply_tmclovQ (dirac_operators/tmclovQ/dirac_operator_tmclovQ.cpp:14-190), lx fields */
 int nq_tmclov_apply_Q(nq_ctx* ctx, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis, double mu,
                       nq_field* out, const nq_field* in);
+/* apply_tmclovQ2 (dirac_operators/tmclovQ2/dirac_operator_tmclovQ2.hpp:9-20): Q(-mu) Q(+mu), lx */
+int nq_tmclov_apply_Q2(nq_ctx* ctx, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis, double mu,
+                       nq_field* out, const nq_field* in);
+/* inv_tmclovQ2_cg (inverters/twisted_clover/cg_invert_tmclovQ2.hpp:9-24, cg_64_invert_tmclovQ2_portable.hpp:14-40):
+   cg_invert on Q(-mu) Q(+mu) over lx fields, FP64; guess may be null */
+int nq_tmclovQ2_cg(nq_ctx* ctx, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis, double mu,
+                   int niter, double residue, nq_field* sol, const nq_field* guess, const nq_field* src,
+                   nq_solve_stats* st);
 /* inv_tmclovDkern_eoprec_square_eos_cg (inverters/twisted_clover/cg_invert_tmclovD_eoprec.cpp:33-54) */
 int nq_tmclov_cg_kernsq(nq_ctx* ctx, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis,
                         double mu, int niter, double residue, int prec, nq_field* sol, const nq_field* guess,

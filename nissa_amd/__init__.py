@@ -96,6 +96,8 @@ _SIG = {
     "nq_tmclov_kern": ([_P, _P, _P, _D, _DP, _D, _I, _P, _P], _I),
     "nq_tmclov_kern_square": ([_P, _P, _P, _D, _DP, _D, _P, _P], _I),
     "nq_tmclov_apply_Q": ([_P, _P, _P, _D, _DP, _D, _P, _P], _I),
+    "nq_tmclov_apply_Q2": ([_P, _P, _P, _D, _DP, _D, _P, _P], _I),
+    "nq_tmclovQ2_cg": ([_P, _P, _P, _D, _DP, _D, _I, _D, _P, _P, _P, C.POINTER(SolveStats)], _I),
     "nq_tmclov_cg_kernsq": ([_P, _P, _P, _D, _DP, _D, _I, _D, _I, _P, _P, _P, _P], _I),
     "nq_tmclov_solve_eoprec": ([_P, _P, _P, _D, _DP, _D, _I, _D, _I, _P, _P, _P, _DP], _I),
     "nq_st_rootst_force": ([_P, _P, _DP, _DP, _I, _DP, _DP, _I, _D, _I, _P, _DP, _DP, _P], _I),
@@ -474,6 +476,17 @@ def tmclov_kern_square(ctx, g, cl, kappa, mu, out, inp, anis=None):
 
 def tmclov_apply_Q(ctx, g, cl, kappa, mu, out, inp, anis=None):
     _check(lib().nq_tmclov_apply_Q(ctx.h, g.h, cl.h, kappa, _anis(anis), mu, out.h, inp.h))
+
+
+def tmclov_apply_Q2(ctx, g, cl, kappa, mu, out, inp, anis=None):
+    _check(lib().nq_tmclov_apply_Q2(ctx.h, g.h, cl.h, kappa, _anis(anis), mu, out.h, inp.h))
+
+
+def tmclovQ2_cg(ctx, g, cl, kappa, mu, niter, residue, sol, src, guess=None, anis=None):
+    st = SolveStats()
+    rc = lib().nq_tmclovQ2_cg(ctx.h, g.h, cl.h, kappa, _anis(anis), mu, niter, residue, sol.h,
+                              guess.h if guess is not None else None, src.h, C.byref(st))
+    return _solve(rc, st)
 
 
 def _solve(rc, st):

@@ -20,6 +20,7 @@
 #include <complex>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 
 #include "nq_internal.hpp"
 #include "tm_batch.hpp"
@@ -838,10 +839,14 @@ int TmBatch::finish(int slot, const SpinorView& src_e, const SpinorView& src_o, 
 }
 
 // CG on Q(-mu) Q(+mu) over lx (inv_tmQ2_cg), FP64, cg_invert semantics
-static int tmQ2_cg_f64(const TmSetup& S, double mu, int niter, double residue, const nq_field* sol,
-                       const nq_field* guess, const nq_field* src, nq_solve_stats* out)
+// Q(mu) on lx fields: writes out = Q(mu) in
+typedef std::function<int(double mu, const nq_field* out, const nq_field* in)> ApplyQ;
+
+// cg_invert on Q(-mu)Q(+mu) over lx fields (inv_tmQ2_cg / inv_tmclovQ2_cg): the matvec
+// is two Q applies and <p, Q(-mu)Q(+mu) p> = |Q(+mu) p|^2 (Q(mu)^+ = Q(-mu)).  FP64.
+static int q2_cg_f64(nq_ctx* ctx, const ApplyQ& Q, double mu, int niter, double residue, const nq_field* sol,
+                     const nq_field* guess, const nq_field* src, nq_solve_stats* out)
 {
-  nq_ctx* ctx = S.ctx;
   Timer T(ctx->stream);
   // lx temporaries as fields (even half then odd half) so parity views apply
   SpinorView rv, pv, tv, sv;
@@ -881,8 +886,8 @@ static int tmQ2_cg_f64(const TmSetup& S, double mu, int niter, double residue, c
   s0.niter = niter;
   NQ_TRY(put_state(ctx, 0, s0));
   auto q2 = [&](const nq_field* out_f, const nq_field* in_f) -> int {
-    NQ_TRY(apply_Q_views(S, +mu, &ft, in_f));
-    return apply_Q_views(S, -mu, out_f, &ft);
+    NQ_TRY(Q(+mu, &ft, in_f));
+    return Q(-mu, out_f, &ft);
   };
   if (guess) {
     if (guess != sol) NQ_TRY(launch_convert(ctx, x, view_lx_flat(guess)));
@@ -899,9 +904,9 @@ static int tmQ2_cg_f64(const TmSetup& S, double mu, int niter, double residue, c
   while (!ctx->state_host[0].done) {
     for (int i = 0; i < chunk; i++) {
       // s = Q(-mu) Q(+mu) p ; <p, s> = |Q(+mu) p|^2
-      NQ_TRY(apply_Q_views(S, +mu, &ft, &fp));
+      NQ_TRY(Q(+mu, &ft, &fp));
       NQ_TRY(launch_lincomb(ctx, none, 1.0, view_lx_flat(&ft), 0, none, 0, none, FIN_PAP, st, 1, 1));
-      NQ_TRY(apply_Q_views(S, -mu, &fs, &ft));
+      NQ_TRY(Q(-mu, &fs, &ft));
       NQ_TRY(launch_cg_update(ctx, x, p, r, s, st, FIN_RR));
       NQ_TRY(launch_cg_pupdate(ctx, p, r, st));
     }
@@ -1277,7 +1282,8 @@ int nq_tmQ2_cg(nq_ctx* c, const nq_gauge* g, double kappa, double mu, int niter,
   nq_solve_stats local;
   if (!st) st = &local;
   std::memset(st, 0, sizeof *st);
-  return tmQ2_cg_f64(S, mu, niter, residue, sol, guess, src, st);
+  const ApplyQ Q = [&](double m, const nq_field* o, const nq_field* i) { return apply_Q_views(S, m, o, i); };
+  return q2_cg_f64(c, Q, mu, niter, residue, sol, guess, src, st);
 }
 
 // inv_tmD_cg_eoprec for several sources at once: eqs. 8 and 10-11 per source, the
@@ -1901,6 +1907,23 @@ int nq_tmclov_kern_square(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double ka
 }
 
 // Q = g5 D, D = 1/(2 kappa) + i mu g5 + Cl - 1/2 hop on each parity
+// apply_tmclovQ on lx views: per parity, out = gamma5 (D_pp-block in_p - 1/2 hop in_!p)
+static int clov_apply_Q_views(const TmSetup& S, nq_clover* cl, double mu, const nq_field* out, const nq_field* in)
+{
+  nq_ctx* c = S.ctx;
+  SpinorView u, none;
+  NQ_TRY(alloc_spinor(c, "clov.uo", SP_F64, 12, &u));
+  for (int p = 0; p < 2; p++) {
+    NQ_TRY(launch_clover_apply(c, 0, u, view_of(in, p), clov_view(cl, p), none, S.kappa, mu, 0, kOne, nullptr));
+    HopEpi e;
+    setc(e.c1, -0.5, 0.5);
+    setc(e.c2, 1.0, -1.0);
+    e.aux = u;
+    NQ_TRY(launch_tm_hop(c, p, view_of(out, p), view_of(in, !p), S.U, S.anis, e));
+  }
+  return NQ_OK;
+}
+
 int nq_tmclov_apply_Q(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis, double mu,
                       nq_field* out, const nq_field* in)
 {
@@ -1912,18 +1935,62 @@ int nq_tmclov_apply_Q(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa,
   Guard G(c);
   TmSetup S;
   NQ_TRY(tm_setup(c, g, LK_F64_18, anis, kappa, &S));
-  SpinorView u, none;
-  NQ_TRY(alloc_spinor(c, "clov.uo", SP_F64, 12, &u));
-  for (int p = 0; p < 2; p++) {
-    NQ_TRY(launch_clover_apply(c, 0, u, view_of(in, p), clov_view(cl, p), none, kappa, mu, 0, kOne, nullptr));
-    HopEpi e;
-    setc(e.c1, -0.5, 0.5);
-    setc(e.c2, 1.0, -1.0);
-    e.aux = u;
-    NQ_TRY(launch_tm_hop(c, p, view_of(out, p), view_of(in, !p), S.U, S.anis, e));
-  }
+  NQ_TRY(clov_apply_Q_views(S, cl, mu, out, in));
   NQ_HIP(hipStreamSynchronize(c->stream));
   return NQ_OK;
+}
+
+// apply_tmclovQ2 (dirac_operators/tmclovQ2/dirac_operator_tmclovQ2.hpp:9-20):
+// Q(+mu) into a temporary, then Q(-mu)
+int nq_tmclov_apply_Q2(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis, double mu,
+                       nq_field* out, const nq_field* in)
+{
+  if (!c || !cl) return fail(NQ_EINVAL, "null argument");
+  if (cl->ctx != c) return fail(NQ_EINVAL, "clover handle belongs to another context");
+  NQ_TRY(check_field(c, in, NQ_SPINCOLOR, NQ_LX, "nq_tmclov_apply_Q2"));
+  NQ_TRY(check_field(c, out, NQ_SPINCOLOR, NQ_LX, "nq_tmclov_apply_Q2"));
+  if (in == out) return fail(NQ_EINVAL, "out==in!");
+  Guard G(c);
+  TmSetup S;
+  NQ_TRY(tm_setup(c, g, LK_F64_18, anis, kappa, &S));
+  SpinorView tv;
+  NQ_TRY(alloc_spinor_nr(c, "clov.q2t", SP_F64, 12, 2, &tv));
+  nq_field ft;
+  ft.ctx = c;
+  ft.kind = NQ_SPINCOLOR;
+  ft.coverage = NQ_LX;
+  ft.buf.p = tv.data;
+  struct Release {
+    nq_field& f;
+    ~Release() { f.buf.p = nullptr; }
+  } rel{ft};
+  NQ_TRY(clov_apply_Q_views(S, cl, +mu, &ft, in));
+  NQ_TRY(clov_apply_Q_views(S, cl, -mu, out, &ft));
+  NQ_HIP(hipStreamSynchronize(c->stream));
+  return NQ_OK;
+}
+
+// inv_tmclovQ2_cg (inverters/twisted_clover/cg_invert_tmclovQ2.hpp:9-24 ->
+// cg_64_invert_tmclovQ2_portable.hpp:14-40): cg_invert on apply_tmclovQ2 over lx fields
+int nq_tmclovQ2_cg(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis, double mu,
+                   int niter, double residue, nq_field* sol, const nq_field* guess, const nq_field* src,
+                   nq_solve_stats* st)
+{
+  if (!c || !cl) return fail(NQ_EINVAL, "null argument");
+  if (cl->ctx != c) return fail(NQ_EINVAL, "clover handle belongs to another context");
+  NQ_TRY(check_field(c, src, NQ_SPINCOLOR, NQ_LX, "nq_tmclovQ2_cg"));
+  NQ_TRY(check_field(c, sol, NQ_SPINCOLOR, NQ_LX, "nq_tmclovQ2_cg"));
+  if (guess) NQ_TRY(check_field(c, guess, NQ_SPINCOLOR, NQ_LX, "nq_tmclovQ2_cg"));
+  if (sol == src) return fail(NQ_EINVAL, "sol==src");
+  if (niter <= 0 || !(residue > 0)) return fail(NQ_EINVAL, "bad niter/residue");
+  Guard G(c);
+  TmSetup S;
+  NQ_TRY(tm_setup(c, g, LK_F64_18, anis, kappa, &S));
+  nq_solve_stats local;
+  if (!st) st = &local;
+  std::memset(st, 0, sizeof *st);
+  const ApplyQ Q = [&](double m, const nq_field* o, const nq_field* i) { return clov_apply_Q_views(S, cl, m, o, i); };
+  return q2_cg_f64(c, Q, mu, niter, residue, sol, guess, src, st);
 }
 
 int nq_tmclov_cg_kernsq(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis, double mu,

@@ -1,6 +1,6 @@
 """GPU parity of the twisted-clover path against the oracle through the C ABI:
-clover term, D_ee / D_ee^-1 (both parities, dagger), K, K K^+, Q, the CG on
-K K^+ and the eo solve (inv_tmclovD_cg_eoprec).
+clover term, D_ee / D_ee^-1 (both parities, dagger), K, K K^+, Q, Q^2, the CG on
+K K^+, the lx CG on Q^2 (inv_tmclovQ2_cg) and the eo solve (inv_tmclovD_cg_eoprec).
 
 Bars: the clover term and the operator applies match the FP64 oracle to
 <= 1e-14 relative L2 (1e-13 where the inverse blocks enter: Gauss-Jordan on the
@@ -81,6 +81,46 @@ def test_tmclov_kern_square_and_Q(setups, L):
     fl, fq = ctx.field(nq.SPINCOLOR, nq.LX, psi), ctx.field(nq.SPINCOLOR, nq.LX)
     nq.tmclov_apply_Q(ctx, g, cl, KAPPA, mu, fq, fl)
     assert rel_l2(fq.download(), geo.apply_tmclovQ(conf, KAPPA, ref, mu, psi)) < 1e-14
+
+
+def g5(x):
+    y = x.copy()
+    y[:, 2:] *= -1
+    return y
+
+
+def test_tmclov_apply_Q2(setups):
+    """apply_tmclovQ2 = Q(-mu) Q(+mu) on lx fields, against the oracle's Q applied twice."""
+    L = (4, 6, 4, 8)
+    ctx, g, cl, geo, conf, ce, co, ref = setups(L)
+    mu = 0.05
+    psi = nq.source_lx(L, 8, "gauss")
+    fl, fq = ctx.field(nq.SPINCOLOR, nq.LX, psi), ctx.field(nq.SPINCOLOR, nq.LX)
+    nq.tmclov_apply_Q2(ctx, g, cl, KAPPA, mu, fq, fl)
+    expect = geo.apply_tmclovQ(conf, KAPPA, ref, -mu, geo.apply_tmclovQ(conf, KAPPA, ref, mu, psi))
+    assert rel_l2(fq.download(), expect) < 1e-14
+
+
+def test_tmclovQ2_cg_matches_two_eo_solves(setups):
+    """inv_tmclovQ2_cg on lx fields: the FP64 true residual of the oracle's operator
+    reaches the residue, and the solution equals x = D(+mu)^-1 g5 D(-mu)^-1 g5 b built
+    from two oracle eo solves (Q(mu) = g5 D(mu)), a different algorithm."""
+    L = (4, 6, 4, 8)
+    ctx, g, cl, geo, conf, ce, co, ref = setups(L)
+    mu = 0.05
+    b = nq.source_lx(L, 9, "gauss")
+    fb, fx = ctx.field(nq.SPINCOLOR, nq.LX, b), ctx.field(nq.SPINCOLOR, nq.LX)
+    st = nq.tmclovQ2_cg(ctx, g, cl, KAPPA, mu, 20000, 1e-24, fx, fb)
+    assert st["rc"] == 0 and st["converged"] and st["rel_res_sq"] < 2e-24, st
+    x = fx.download()
+    r = b - geo.apply_tmclovQ(conf, KAPPA, ref, -mu, geo.apply_tmclovQ(conf, KAPPA, ref, mu, x))
+    assert np.vdot(r, r).real / np.vdot(b, b).real < 2e-24
+    y, _ = geo.tmclov_solve_eoprec(conf, KAPPA, CSW, -mu, 20000, 1e-26, g5(b))
+    xr, _ = geo.tmclov_solve_eoprec(conf, KAPPA, CSW, mu, 20000, 1e-26, g5(y))
+    assert rel_l2(x, xr) < 1e-8  # cond(Q^2) ~ 1e4 times |r|/|b| = 1e-12
+    # restart from the solution
+    st2 = nq.tmclovQ2_cg(ctx, g, cl, KAPPA, mu, 20000, 1e-20, fx, fb, guess=fx)
+    assert st2["rc"] == 0 and st2["iters"] <= 2
 
 
 def test_tmclov_with_anisotropy(setups):
