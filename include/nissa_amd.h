@@ -170,6 +170,12 @@ int nq_tmclov_apply_Q2(nq_ctx* ctx, const nq_gauge* g, nq_clover* cl, double kap
 int nq_tmclovQ2_cg(nq_ctx* ctx, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis, double mu,
                    int niter, double residue, nq_field* sol, const nq_field* guess, const nq_field* src,
                    nq_solve_stats* st);
+/* inv_tmclovQ2_cgm (inverters/twisted_clover/cgm_invert_tmclovQ2.cpp:8-28): multi-shift CG on
+   Q(0)^2 + m_k^2 over lx fields, FP64; sols[k] per mass, true_res[k] (optional) the FP64
+   |b - (Q(0)^2 + m_k^2) x_k|^2 / |b|^2 */
+int nq_tmclovQ2_cgm(nq_ctx* ctx, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis, int nmass,
+                    const double* masses, int niter, const double* residues, nq_field** sols, const nq_field* src,
+                    nq_solve_stats* st, double* true_res);
 /* inv_tmclovDkern_eoprec_square_eos_cg (inverters/twisted_clover/cg_invert_tmclovD_eoprec.cpp:33-54) */
 int nq_tmclov_cg_kernsq(nq_ctx* ctx, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis,
                         double mu, int niter, double residue, int prec, nq_field* sol, const nq_field* guess,

@@ -98,6 +98,7 @@ _SIG = {
     "nq_tmclov_apply_Q": ([_P, _P, _P, _D, _DP, _D, _P, _P], _I),
     "nq_tmclov_apply_Q2": ([_P, _P, _P, _D, _DP, _D, _P, _P], _I),
     "nq_tmclovQ2_cg": ([_P, _P, _P, _D, _DP, _D, _I, _D, _P, _P, _P, C.POINTER(SolveStats)], _I),
+    "nq_tmclovQ2_cgm": ([_P, _P, _P, _D, _DP, _I, _DP, _I, _DP, _P, _P, _P, _DP], _I),
     "nq_tmclov_cg_kernsq": ([_P, _P, _P, _D, _DP, _D, _I, _D, _I, _P, _P, _P, _P], _I),
     "nq_tmclov_solve_eoprec": ([_P, _P, _P, _D, _DP, _D, _I, _D, _I, _P, _P, _P, _DP], _I),
     "nq_st_rootst_force": ([_P, _P, _DP, _DP, _I, _DP, _DP, _I, _D, _I, _P, _DP, _DP, _P], _I),
@@ -487,6 +488,20 @@ def tmclovQ2_cg(ctx, g, cl, kappa, mu, niter, residue, sol, src, guess=None, ani
     rc = lib().nq_tmclovQ2_cg(ctx.h, g.h, cl.h, kappa, _anis(anis), mu, niter, residue, sol.h,
                               guess.h if guess is not None else None, src.h, C.byref(st))
     return _solve(rc, st)
+
+
+def tmclovQ2_cgm(ctx, g, cl, kappa, masses, niter, residues, sols, src, anis=None):
+    """inv_tmclovQ2_cgm: sols[k] per mass; returns stats and the FP64 true residual per mass."""
+    n = len(masses)
+    ma = (C.c_double * n)(*masses)
+    rs = (C.c_double * n)(*residues)
+    so = (_P * n)(*[x.h for x in sols])
+    st = SolveStats()
+    tr = (C.c_double * n)()
+    rc = lib().nq_tmclovQ2_cgm(ctx.h, g.h, cl.h, kappa, _anis(anis), n, ma, niter, rs, so, src.h, C.byref(st), tr)
+    if rc not in (NQ_OK, NQ_ENOCONV):
+        _check(rc)
+    return dict(st.as_dict(), rc=rc, true_res_sq=list(tr))
 
 
 def _solve(rc, st):

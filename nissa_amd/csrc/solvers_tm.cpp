@@ -934,12 +934,11 @@ static int q2_cg_f64(nq_ctx* ctx, const ApplyQ& Q, double mu, int niter, double 
 // Multi-mass CG on Q(-m_k)Q(+m_k) = Q(0)^2 + m_k^2 over lx (inv_tmQ2_cgm,
 // cgm_invert_tmQ2.cpp:46-53 with cgm_invert's recurrences): the matvec is Q(0)^2 (four
 // hop launches), <p, Q(0)^2 p> = |Q(0) p|^2, the shifts are m_k^2.  FP64.
-static int tmQ2_cgm_f64(nq_ctx* ctx, const nq_gauge* g, double kappa, int nmass, const double* masses, int niter,
-                        const double* residues, nq_field** sols, const nq_field* src, nq_solve_stats* st,
-                        double* true_res)
+// The Q apply comes as a callback (twisted mass or twisted clover).
+static int q2_cgm_f64(nq_ctx* ctx, const ApplyQ& Q, int nmass, const double* masses, int niter,
+                      const double* residues, nq_field** sols, const nq_field* src, nq_solve_stats* st,
+                      double* true_res)
 {
-  TmSetup S;
-  NQ_TRY(tm_setup(ctx, g, LK_F64_18, nullptr, kappa, &S));
   const int64_t fsz = 24 * ctx->geo.stride;  // elements of one lx field
   SpinorView rv, pv, tv, sv, ps, xs;
   NQ_TRY(alloc_spinor_nr(ctx, "q2.r", SP_F64, 12, 2, &rv));
@@ -1009,10 +1008,10 @@ static int tmQ2_cgm_f64(nq_ctx* ctx, const nq_gauge* g, double kappa, int nmass,
   int chunk = 8, launched = 0;
   for (;;) {
     for (int it = 0; it < chunk; it++) {
-      NQ_TRY(apply_Q_views(S, 0.0, &ft, &fp));  // t = Q(0) p
+      NQ_TRY(Q(0.0, &ft, &fp));  // t = Q(0) p
       NQ_TRY(launch_lincomb(ctx, none, 1.0, t, 0, none, 0, none, FIN_PARTIAL, nullptr));
       NQ_TRY(launch_cgm_finalize(ctx, dst, 1, 1, 1.0));  // pap = |t|^2
-      NQ_TRY(apply_Q_views(S, 0.0, &fs, &ft));  // s = Q(0)^2 p
+      NQ_TRY(Q(0.0, &fs, &ft));  // s = Q(0)^2 p
       NQ_TRY(launch_cgm_vec2(ctx, s, r, xs, ps, fsz, dst));
       NQ_TRY(launch_cgm_vec3(ctx, r, p, ps, fsz, dst));
     }
@@ -1030,8 +1029,8 @@ static int tmQ2_cgm_f64(nq_ctx* ctx, const nq_gauge* g, double kappa, int nmass,
     SpinorView xk = xs;
     xk.data = (char*)xs.data + k * fsz * 16;
     NQ_TRY(launch_convert(ctx, view_lx_flat(sols[k]), xk));
-    NQ_TRY(apply_Q_views(S, +masses[k], &ft, sols[k]));
-    NQ_TRY(apply_Q_views(S, -masses[k], &fs, &ft));
+    NQ_TRY(Q(+masses[k], &ft, sols[k]));
+    NQ_TRY(Q(-masses[k], &fs, &ft));
     NQ_TRY(put_state(ctx, 0, CgState{}));
     NQ_TRY(launch_lincomb(ctx, none, 1.0, s, -1.0, b, 0, none, FIN_STORE0, ctx->state));
     NQ_TRY(get_states(ctx, 0, 1));
@@ -1350,7 +1349,10 @@ int nq_tmQ2_cgm(nq_ctx* c, const nq_gauge* g, double kappa, int nmass, const dou
     if (sols[k] == src) return fail(NQ_EINVAL, "sol==src");
   }
   Guard G(c);
-  return tmQ2_cgm_f64(c, g, kappa, nmass, masses, niter, residues, sols, src, st, true_res);
+  TmSetup S;
+  NQ_TRY(tm_setup(c, g, LK_F64_18, nullptr, kappa, &S));
+  const ApplyQ Q = [&](double m, const nq_field* o, const nq_field* i) { return apply_Q_views(S, m, o, i); };
+  return q2_cgm_f64(c, Q, nmass, masses, niter, residues, sols, src, st, true_res);
 }
 
 // inv_tmDQ_cgm (cgm_invert_tmQ2.cpp:55-75): the same on g5 src
@@ -1991,6 +1993,28 @@ int nq_tmclovQ2_cg(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa, co
   std::memset(st, 0, sizeof *st);
   const ApplyQ Q = [&](double m, const nq_field* o, const nq_field* i) { return clov_apply_Q_views(S, cl, m, o, i); };
   return q2_cg_f64(c, Q, mu, niter, residue, sol, guess, src, st);
+}
+
+// inv_tmclovQ2_cgm (inverters/twisted_clover/cgm_invert_tmclovQ2.cpp:8-28): cgm_invert
+// with shifts m_k^2 on apply_tmclovQ2_m2 (Q(0)^2 + m_k^2), lx fields, FP64
+int nq_tmclovQ2_cgm(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis, int nmass,
+                    const double* masses, int niter, const double* residues, nq_field** sols, const nq_field* src,
+                    nq_solve_stats* st, double* true_res)
+{
+  if (!c || !cl || !masses || !residues || !sols) return fail(NQ_EINVAL, "null argument");
+  if (cl->ctx != c) return fail(NQ_EINVAL, "clover handle belongs to another context");
+  if (nmass < 1 || nmass > kMaxShift) return fail(NQ_EINVAL, "need 1..%d masses", kMaxShift);
+  if (niter <= 0) return fail(NQ_EINVAL, "bad niter");
+  NQ_TRY(check_field(c, src, NQ_SPINCOLOR, NQ_LX, "nq_tmclovQ2_cgm"));
+  for (int k = 0; k < nmass; k++) {
+    NQ_TRY(check_field(c, sols[k], NQ_SPINCOLOR, NQ_LX, "nq_tmclovQ2_cgm"));
+    if (sols[k] == src) return fail(NQ_EINVAL, "sol==src");
+  }
+  Guard G(c);
+  TmSetup S;
+  NQ_TRY(tm_setup(c, g, LK_F64_18, anis, kappa, &S));
+  const ApplyQ Q = [&](double m, const nq_field* o, const nq_field* i) { return clov_apply_Q_views(S, cl, m, o, i); };
+  return q2_cgm_f64(c, Q, nmass, masses, niter, residues, sols, src, st, true_res);
 }
 
 int nq_tmclov_cg_kernsq(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis, double mu,

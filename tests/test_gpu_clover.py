@@ -1,6 +1,7 @@
 """GPU parity of the twisted-clover path against the oracle through the C ABI:
 clover term, D_ee / D_ee^-1 (both parities, dagger), K, K K^+, Q, Q^2, the CG on
-K K^+, the lx CG on Q^2 (inv_tmclovQ2_cg) and the eo solve (inv_tmclovD_cg_eoprec).
+K K^+, the lx CG and multi-mass CG on Q^2 (inv_tmclovQ2_cg, inv_tmclovQ2_cgm) and
+the eo solve (inv_tmclovD_cg_eoprec).
 
 Bars: the clover term and the operator applies match the FP64 oracle to
 <= 1e-14 relative L2 (1e-13 where the inverse blocks enter: Gauss-Jordan on the
@@ -121,6 +122,29 @@ def test_tmclovQ2_cg_matches_two_eo_solves(setups):
     # restart from the solution
     st2 = nq.tmclovQ2_cg(ctx, g, cl, KAPPA, mu, 20000, 1e-20, fx, fb, guess=fx)
     assert st2["rc"] == 0 and st2["iters"] <= 2
+
+
+def test_tmclovQ2_cgm_matches_single_mass_cg(setups):
+    """inv_tmclovQ2_cgm: every mass's solution reaches its residue in FP64 (the
+    oracle's Q applied twice gives the true residual) and equals the single-mass
+    lx CG on Q(-m)Q(+m) to the solves' accuracy."""
+    L = (4, 6, 4, 8)
+    ctx, g, cl, geo, conf, ce, co, ref = setups(L)
+    masses = [0.04, 0.1, 0.3]
+    b = nq.source_lx(L, 10, "gauss")
+    fb = ctx.field(nq.SPINCOLOR, nq.LX, b)
+    sols = [ctx.field(nq.SPINCOLOR, nq.LX) for _ in masses]
+    res = nq.tmclovQ2_cgm(ctx, g, cl, KAPPA, masses, 20000, [1e-24] * 3, sols, fb)
+    assert res["rc"] == nq.NQ_OK, res
+    fx = ctx.field(nq.SPINCOLOR, nq.LX)
+    for k, m in enumerate(masses):
+        x = sols[k].download()
+        r = b - geo.apply_tmclovQ(conf, KAPPA, ref, -m, geo.apply_tmclovQ(conf, KAPPA, ref, m, x))
+        assert np.vdot(r, r).real / np.vdot(b, b).real < 2e-24, k
+        assert res["true_res_sq"][k] < 2e-24
+        st = nq.tmclovQ2_cg(ctx, g, cl, KAPPA, m, 20000, 1e-24, fx, fb)
+        assert st["rc"] == 0
+        assert rel_l2(x, fx.download()) < 1e-8, k  # two solves at |r|/|b| = 1e-12
 
 
 def test_tmclov_with_anisotropy(setups):
