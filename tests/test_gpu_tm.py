@@ -189,6 +189,28 @@ def test_cg_kernsq_vs_quad_oracle(setups, prec):
     assert e_64 < 1e-11, e_64
 
 
+@pytest.mark.parametrize("prec", ["f64", "f16"])
+def test_eo_solve_with_anisotropy(setups, prec):
+    """inv_tmD_cg_eoprec with the anisotropy factor on the spatial hops, alone and in
+    the batched solver (sloppy links and the FP64 reliable updates carry it too)."""
+    L = (4, 6, 4, 8)
+    ctx, g, geo, conf, ce, co = setups(L)
+    kappa, anis = 0.16, 1.21
+    mus = [0.01, 0.03]
+    srcs = [nq.source_lx(L, 60 + i) for i in range(2)]
+    refs = [geo.tm_solve_eoprec(conf, kappa, m, 20000, 1e-24, s, anis=anis)[0] for m, s in zip(mus, srcs)]
+    fs = [ctx.field(nq.SPINCOLOR, nq.LX, s) for s in srcs]
+    fx = ctx.field(nq.SPINCOLOR, nq.LX)
+    st = nq.tm_solve_eoprec(ctx, g, kappa, mus[0], 20000, 1e-24, fx, fs[0], prec=prec, anis=anis)
+    assert st["rc"] == 0 and st["check_rel_res_sq"] < 2e-24, st
+    assert rel_l2(fx.download(), refs[0]) < 1e-10
+    sols = [ctx.field(nq.SPINCOLOR, nq.LX) for _ in srcs]
+    res = nq.tm_solve_eoprec_multi(ctx, g, kappa, mus, 20000, 1e-24, sols, fs, prec=prec, anis=anis)
+    for i in range(2):
+        assert res[i]["rc"] == 0 and res[i]["converged"], res[i]
+        assert rel_l2(sols[i].download(), refs[i]) < 1e-10, i
+
+
 @pytest.mark.parametrize("prec", ["f64", "f32", "f16", "f16h"])
 def test_eo_solve_all_precisions(setups, prec):
     """inv_tmD_cg_eoprec on 8^4 (config 0 shape): Z2 source on even sites, kappa 0.16,
