@@ -53,8 +53,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0, help="request arrival rate (1/s)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}): this build has no multi-GPU path; "
-              "the solve would shard by request/RHS across GPUs (see DESIGN.md).", file=sys.stderr)
+        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}): there is no multi-GPU path; "
+              "requests and right-hand sides would shard across GPUs with the gauge field replicated, "
+              "and a single large solve along T with halo exchange.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

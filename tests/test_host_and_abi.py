@@ -52,7 +52,8 @@ def test_gauge_generator_is_su3_and_matches_expm():
 
 
 def test_gauge_generator_plaquette_matches_reference_construction():
-    # the reference's U = exp(i 0.3 H) construction gives plaquette ~0.612 (survey probe)
+    # the reference's U = exp(i 0.3 H) construction gives plaquette ~0.612 (a run of
+    # nissa's own generator)
     L = (8, 8, 8, 8)
     g = orc.Geom(L)
     p = g.plaquette(nq.gauge_random_lx(L, 1, 0.3))

@@ -316,7 +316,8 @@ def test_eo_solve_solves_full_system():
 
 def test_eo_solve_iteration_count_matches_reference_probe():
     """8^4, kappa 0.16, mu 0.01, Z2 source on even sites, residue 1e-24: the reference
-    took 70 CG iterations on a gauge field of the same construction (survey probe)."""
+    took 70 CG iterations on a gauge field of the same construction (a run of nissa's
+    own inv_tmD_cg_eoprec)."""
     L = (8, 8, 8, 8)
     g = orc.Geom(L)
     conf = tm_conf(L, 1)
