@@ -79,3 +79,37 @@ def test_partial_chunk_batched_cgm(prec):
         nq.st_cgm(ctx, g, shifts, 20000, [1e-24] * 3, alone, fsrc[2:3], prec=prec)
         for k in range(3):
             assert np.array_equal(alone[0][k].download(), sols[2][k].download())
+
+
+@pytest.mark.parametrize("Ls", [(2, 2, 2, 2), (2, 4, 2, 6), (10, 6, 2, 2), (14, 4, 4, 2), (20, 2, 2, 2)])
+def test_small_and_odd_lattices(Ls):
+    """Extents down to 2 and t tiles that are not powers of two (the device site
+    order takes the largest even divisor of T up to 64): the eo solve in FP64 and
+    FP16 inner, the batched eo solve and the mixed multi-shift solve against the
+    oracle."""
+    geo = orc.Geom(Ls)
+    conf = tm_conf(Ls, 3)
+    src = nq.source_lx(Ls, 4)
+    ref, _ = geo.tm_solve_eoprec(conf, 0.16, 0.02, 20000, 1e-24, src)
+    sconf = st_conf(Ls, 5)
+    ce, co = geo.split(sconf)
+    shifts = [0.01, 0.1, 1.0]
+    ss = [gauss_eo(geo.Vh, 6 + i, ncomp=3) for i in range(2)]
+    sref, _ = geo.st_cgm(ce, co, shifts, 20000, [1e-24] * 3, ss[0])
+    with nq.Ctx(Ls) as ctx:
+        g = nq.Gauge(ctx, lx=conf)
+        fs = ctx.field(nq.SPINCOLOR, nq.LX, src)
+        fx = ctx.field(nq.SPINCOLOR, nq.LX)
+        for prec in ("f64", "f16"):
+            st = nq.tm_solve_eoprec(ctx, g, 0.16, 0.02, 20000, 1e-24, fx, fs, prec=prec)
+            assert st["rc"] == 0 and rel_l2(fx.download(), ref) < 1e-10, prec
+        fxs = [ctx.field(nq.SPINCOLOR, nq.LX) for _ in range(2)]
+        res = nq.tm_solve_eoprec_multi(ctx, g, 0.16, [0.02, 0.05], 20000, 1e-24, fxs, [fs, fs], prec="f16")
+        assert all(r["rc"] == 0 for r in res) and rel_l2(fxs[0].download(), ref) < 1e-10
+        gs = nq.Gauge(ctx, lx=sconf)
+        fsrc = [ctx.field(nq.COLOR, nq.EVEN, s) for s in ss]
+        sols = [[ctx.field(nq.COLOR, nq.EVEN) for _ in shifts] for _ in ss]
+        r = nq.st_cgm(ctx, gs, shifts, 20000, [1e-24] * 3, sols, fsrc, prec="f32")
+        assert all(x["rc"] == 0 for x in r)
+        for k in range(3):
+            assert rel_l2(sols[0][k].download(), sref[k]) < 1e-10, k
