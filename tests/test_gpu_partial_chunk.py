@@ -113,3 +113,34 @@ def test_small_and_odd_lattices(Ls):
         assert all(x["rc"] == 0 for x in r)
         for k in range(3):
             assert rel_l2(sols[0][k].download(), sref[k]) < 1e-10, k
+
+
+@pytest.mark.parametrize("Ls", [(2, 2, 2, 2), (10, 6, 2, 2)])
+def test_small_lattice_clover_and_force(Ls):
+    """The twisted-clover eo solve (FP64 and FP16 inner) and the rooted-staggered
+    force (mixed multi-shift solve) on small and partial-chunk lattices."""
+    import test_gpu_force as tf
+    from helpers import real
+
+    geo = orc.Geom(Ls)
+    conf = tm_conf(Ls, 7)
+    src = nq.source_lx(Ls, 8)
+    ref, _ = geo.tmclov_solve_eoprec(conf, 0.16, 1.3, 0.02, 20000, 1e-24, src)
+    gg, conf2, u1, phased, pf = tf._inputs(Ls, 3)
+    ce, co = gg.split(conf2)
+    u1e, u1o = gg.split(real(u1))
+    poles, weights = [0.003, 0.02, 0.25], [0.04, 0.15, 0.5]
+    Fe_ref, Fo_ref, rc = gg.rootst_force(ce, co, u1e, u1o, pf, poles, weights, 20000, 1e-28)
+    assert rc == 0
+    with nq.Ctx(Ls) as ctx:
+        g = nq.Gauge(ctx, lx=conf)
+        cl = nq.Clover(ctx, g, 1.3)
+        fs = ctx.field(nq.SPINCOLOR, nq.LX, src)
+        fx = ctx.field(nq.SPINCOLOR, nq.LX)
+        for prec in ("f64", "f16"):
+            st = nq.tmclov_solve_eoprec(ctx, g, cl, 0.16, 0.02, 20000, 1e-24, fx, fs, prec=prec)
+            assert st["rc"] == 0 and rel_l2(fx.download(), ref) < 1e-10, prec
+        gh = nq.Gauge(ctx, lx=phased)
+        fpf = ctx.field(nq.COLOR, nq.EVEN, pf)
+        Fe, Fo, st = nq.st_rootst_force(ctx, gh, u1e, u1o, poles, weights, 20000, 1e-26, fpf, prec="f32")
+        assert st["rc"] == 0 and rel_l2(Fe, Fe_ref) < 1e-10 and rel_l2(Fo, Fo_ref) < 1e-10
