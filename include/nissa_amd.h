@@ -82,6 +82,18 @@ int nq_gauge_create_lx(nq_ctx* ctx, const double* conf_lx, nq_gauge** out);
 void nq_gauge_destroy(nq_gauge* g);
 /* device bytes held by the gauge handle (all precisions built so far) */
 int64_t nq_gauge_device_bytes(const nq_gauge* g);
+/* global_plaquette_lx_conf (operations/gauge/plaquette.hpp:52-84): the average of
+   Re tr(U_mu U_nu U_mu^+ U_nu^+)/3 over the temporal planes and over the spatial
+   planes, then the mean of the two; computed on the device from the handle's links */
+int nq_gauge_plaquette(nq_ctx* ctx, const nq_gauge* g, double* plaq);
+/* read_ildg_gauge_conf (io/reader.hpp:75-98, io/ILDG_File.cpp): the "ildg-binary-data"
+   record of a LIME file (big-endian, 64- or 32-bit as its "ildg-format" record says;
+   sites t slowest ... x fastest, directions x,y,z,t) into an lx conf in nissa's
+   order (mu = 0 is t; quad_su3_ildg_to_nissa_reord, su3_op.hpp:217-225).
+   checks (may be NULL) receives the "scidac-checksum" record's suma, sumb (0, 0
+   when absent) and the ones computed from the data (ildgChecksum, io/checksum.hpp).
+   nissa only warns when they differ, and so does this: compare them. */
+int nq_read_ildg_gauge_conf(const char* path, int T, int LX, int LY, int LZ, double* conf_lx, uint32_t checks[4]);
 
 /* ------------------------------------------------------------------ fields */
 int nq_field_create(nq_ctx* ctx, int kind, int coverage, nq_field** out);

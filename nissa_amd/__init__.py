@@ -128,6 +128,8 @@ _SIG = {
     "nq_gauge_create_lx": ([_P, _DP, C.POINTER(_P)], _I),
     "nq_gauge_destroy": ([_P], None),
     "nq_gauge_device_bytes": ([_P], C.c_int64),
+    "nq_gauge_plaquette": ([_P, _P, _DP], _I),
+    "nq_read_ildg_gauge_conf": ([C.c_char_p, _I, _I, _I, _I, _DP, C.POINTER(C.c_uint32)], _I),
     "nq_field_create": ([_P, _I, _I, C.POINTER(_P)], _I),
     "nq_field_destroy": ([_P], None),
     "nq_field_upload": ([_P, _DP], _I),
@@ -237,6 +239,21 @@ def gauge_phase_lx(L, conf, theta_in_pi):
     th = np.ascontiguousarray(theta_in_pi, dtype=np.float64)
     _check(lib().nq_host_gauge_phase_lx(*dims(L), _dp(th), _dp(conf)))
     return conf
+
+
+def read_ildg_gauge_conf(path, L):
+    """nq_read_ildg_gauge_conf: (lx conf (V,4,3,3,2) in nissa's order, checksums
+    (suma, sumb) read from the file and (suma, sumb) computed from its data)."""
+    conf = np.empty((int(np.prod(L)), 4, 3, 3, 2))
+    chk = (C.c_uint32 * 4)()
+    _check(lib().nq_read_ildg_gauge_conf(str(path).encode(), *dims(L), _dp(conf), chk))
+    return conf, (chk[0], chk[1]), (chk[2], chk[3])
+
+
+def gauge_plaquette(ctx, g):
+    d = C.c_double(0)
+    _check(lib().nq_gauge_plaquette(ctx.h, g.h, C.byref(d)))
+    return d.value
 
 
 def gauge_stagphases_lx(L, conf):

@@ -505,6 +505,18 @@ void nq_gauge_destroy(nq_gauge* g)
 
 int64_t nq_gauge_device_bytes(const nq_gauge* g) { return g ? g->bytes() : 0; }
 
+int nq_gauge_plaquette(nq_ctx* c, const nq_gauge* g, double* plaq)
+{
+  if (!c || !g || !plaq) return fail(NQ_EINVAL, "null argument");
+  if (g->ctx != c) return fail(NQ_EINVAL, "gauge handle belongs to another context");
+  Guard G(c);
+  double ts[2];
+  NQ_TRY(launch_plaquette(c, (const double*)g->aos[0].p, (const double*)g->aos[1].p, ts));
+  const double norm = 2.0 * (double)c->geo.Vh * 9.0;  // V sites x 3 planes x trace of 3
+  *plaq = (ts[0] / norm + ts[1] / norm) / 2;
+  return NQ_OK;
+}
+
 }  // extern "C"
 
 namespace nq {

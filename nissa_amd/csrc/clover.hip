@@ -374,7 +374,63 @@ __global__ void __launch_bounds__(256) k_blocks_to_f32(Geo g, int ncomp, const C
   }
 }
 
+// Re tr of the six plaquettes U_mu(X) U_nu(X+mu) U_mu(X+nu)^+ U_nu(X)^+ of each
+// site, summed over the temporal planes (mu = 0) and the spatial ones
+// (global_plaquette_lx_conf, operations/gauge/plaquette.hpp:52-84); one partial
+// per (parity, temporal/spatial, 64-site chunk)
+__global__ void __launch_bounds__(64) k_plaquette(Geo g, Links lk, int par, int64_t nchunks, double* partial)
+{
+  const uint32_t e = blockIdx.x * 64 + threadIdx.x;
+  double ts[2] = {0, 0};
+  if (e < (uint32_t)g.Vh) {
+    const Site s = site_of(g, par, e);
+    const int X[4] = {s.t, s.x, s.y, s.z};
+    for (int mu = 0; mu < 4; mu++)
+      for (int nu = mu + 1; nu < 4; nu++) {
+        int A[4] = {X[0], X[1], X[2], X[3]}, B[4] = {X[0], X[1], X[2], X[3]};
+        A[mu] += 1;
+        B[nu] += 1;
+        C u1[9], u2[9], t1[9], t2[9];
+        lk.get(X[0], X[1], X[2], X[3], mu, u1);
+        lk.get(A[0], A[1], A[2], A[3], nu, u2);
+        m3_mul(t1, u1, u2);
+        lk.get(X[0], X[1], X[2], X[3], nu, u1);
+        lk.get(B[0], B[1], B[2], B[3], mu, u2);
+        m3_mul(t2, u1, u2);
+        double re = 0;  // Re tr(t1 t2^+) = sum_ij Re(t1_ij conj(t2_ij))
+#pragma unroll
+        for (int k = 0; k < 9; k++) re += t1[k].re * t2[k].re + t1[k].im * t2[k].im;
+        ts[mu != 0] += re;
+      }
+  }
+  store_partial(ts[0], partial, nchunks, 2 * par, blockIdx.x);
+  store_partial(ts[1], partial, nchunks, 2 * par + 1, blockIdx.x);
+}
+
 }  // namespace
+
+int launch_plaquette(nq_ctx* ctx, const double* aos_e, const double* aos_o, double ts[2])
+{
+  const Geo& g = ctx->geo;
+  const int64_t nchunks = (g.Vh + 63) / 64;
+  if (4 * nchunks > ctx->max_partials) return fail(NQ_EINVAL, "plaquette: reduction scratch too small");
+  Links lk;
+  lk.aos[0] = aos_e;
+  lk.aos[1] = aos_o;
+  lk.L[0] = g.T; lk.L[1] = g.LX; lk.L[2] = g.LY; lk.L[3] = g.LZ;
+  for (int par = 0; par < 2; par++)
+    hipLaunchKernelGGL(k_plaquette, dim3((unsigned)nchunks), dim3(64), 0, ctx->stream, g, lk, par, nchunks, ctx->partial);
+  NQ_HIP(hipGetLastError());
+  std::vector<double> h(4 * nchunks);
+  NQ_HIP(hipMemcpyAsync(h.data(), ctx->partial, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+  NQ_HIP(hipStreamSynchronize(ctx->stream));
+  // fixed order: parity, then chunk
+  ts[0] = ts[1] = 0;
+  for (int par = 0; par < 2; par++)
+    for (int k = 0; k < 2; k++)
+      for (int64_t c = 0; c < nchunks; c++) ts[k] += h[(2 * par + k) * nchunks + c];
+  return NQ_OK;
+}
 
 int launch_clover_build(nq_ctx* ctx, const double* aos_e, const double* aos_o, int par, double csw, const SpinorView& out)
 {

@@ -1,0 +1,133 @@
+// ILDG / LIME gauge-configuration reader (read_ildg_gauge_conf, io/reader.hpp:75-98;
+// record search in io/ILDG_File.cpp; checksum in io/checksum.hpp).
+//
+// A LIME file is a sequence of records, each a 144-byte big-endian header (magic
+// 0x456789ab u32, version u16, flags u16, data length u64, type string of 128
+// bytes) followed by the data, padded to a multiple of 8 bytes.  The gauge field is
+// the "ildg-binary-data" record: per site (t slowest, then z, y, x fastest) four
+// su3 in direction order x, y, z, t, big-endian reals of the precision that the
+// "ildg-format" record names.  The SciDAC checksum XORs, over the file sites i,
+// the CRC-32 of the site's bytes rotated left by i % 29 (suma) and i % 31 (sumb).
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "nq_internal.hpp"
+
+namespace {
+
+using nq::fail;
+
+uint64_t be(const unsigned char* p, int n)
+{
+  uint64_t v = 0;
+  for (int i = 0; i < n; i++) v = (v << 8) | p[i];
+  return v;
+}
+
+// CRC-32 (reflected polynomial 0xedb88320, initial value and final XOR 0xffffffff)
+uint32_t crc32(const unsigned char* p, size_t n)
+{
+  static uint32_t table[256];
+  static bool init = false;
+  if (!init) {
+    for (uint32_t i = 0; i < 256; i++) {
+      uint32_t c = i;
+      for (int j = 0; j < 8; j++) c = (c & 1) ? 0xedb88320u ^ (c >> 1) : (c >> 1);
+      table[i] = c;
+    }
+    init = true;
+  }
+  uint32_t c = 0xffffffffu;
+  for (size_t i = 0; i < n; i++) c = table[(c ^ p[i]) & 0xff] ^ (c >> 8);
+  return c ^ 0xffffffffu;
+}
+
+uint32_t rotl(uint32_t v, uint32_t r) { return r ? (v << r) | (v >> (32 - r)) : v; }
+
+// integer between <tag> and </tag>, or -1
+long long xml_int(const std::string& s, const char* tag, int base = 10)
+{
+  const std::string open = std::string("<") + tag + ">";
+  const size_t a = s.find(open);
+  if (a == std::string::npos) return -1;
+  return std::strtoll(s.c_str() + a + open.size(), nullptr, base);
+}
+
+}  // namespace
+
+extern "C" int nq_read_ildg_gauge_conf(const char* path, int T, int LX, int LY, int LZ, double* conf_lx,
+                                       uint32_t checks[4])
+{
+  if (!path || !conf_lx || T < 1 || LX < 1 || LY < 1 || LZ < 1) return fail(NQ_EINVAL, "nq_read_ildg_gauge_conf: bad argument");
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return fail(NQ_EINVAL, "cannot open %s", path);
+  const std::vector<unsigned char> d((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+  const unsigned char* data = nullptr;
+  uint64_t ndata = 0;
+  std::string format, checksum;
+  size_t pos = 0;
+  while (pos + 144 <= d.size()) {
+    if (be(&d[pos], 4) != 0x456789abu) return fail(NQ_EINVAL, "%s: bad LIME magic at byte %zu", path, pos);
+    const uint64_t len = be(&d[pos + 8], 8);
+    const std::string type(reinterpret_cast<const char*>(&d[pos + 16]), strnlen(reinterpret_cast<const char*>(&d[pos + 16]), 128));
+    const size_t start = pos + 144;
+    if (len > d.size() - start) return fail(NQ_EINVAL, "%s: truncated record %s", path, type.c_str());
+    if (type == "ildg-binary-data") {
+      data = &d[start];
+      ndata = len;
+    } else if (type == "ildg-format") {
+      format.assign(reinterpret_cast<const char*>(&d[start]), len);
+    } else if (type == "scidac-checksum") {
+      checksum.assign(reinterpret_cast<const char*>(&d[start]), len);
+    }
+    pos = start + ((len + 7) / 8) * 8;
+  }
+  if (!data) return fail(NQ_EINVAL, "%s: record ildg-binary-data not found", path);
+  const int64_t V = (int64_t)T * LX * LY * LZ;
+  int prec = (int)xml_int(format, "precision");
+  if (prec < 0) prec = ndata == (uint64_t)V * 72 * 4 ? 32 : 64;
+  if (prec != 32 && prec != 64) return fail(NQ_EINVAL, "%s: precision %d not supported", path, prec);
+  const long long fl[4] = {xml_int(format, "lt"), xml_int(format, "lx"), xml_int(format, "ly"), xml_int(format, "lz")};
+  const int want[4] = {T, LX, LY, LZ};
+  for (int k = 0; k < 4; k++)
+    if (fl[k] >= 0 && fl[k] != want[k])
+      return fail(NQ_EINVAL, "%s: lattice %lldx%lldx%lldx%lld in the file, %dx%dx%dx%d requested", path, fl[0], fl[1],
+                  fl[2], fl[3], T, LX, LY, LZ);
+  const int wb = prec / 8;
+  const size_t site_bytes = (size_t)72 * wb;
+  if (ndata != (uint64_t)V * site_bytes)
+    return fail(NQ_EINVAL, "%s: %llu data bytes, %lld expected", path, (unsigned long long)ndata, (long long)(V * site_bytes));
+  uint32_t suma = 0, sumb = 0;
+  for (int64_t i = 0; i < V; i++) {
+    const unsigned char* sp = data + i * site_bytes;
+    const int64_t x = i % LX, y = (i / LX) % LY, z = (i / ((int64_t)LX * LY)) % LZ, t = i / ((int64_t)LX * LY * LZ);
+    const int64_t lx = ((t * LX + x) * LY + y) * LZ + z;
+    for (int mf = 0; mf < 4; mf++) {
+      double* u = conf_lx + (lx * 4 + (mf + 1) % 4) * 18;  // file x,y,z,t -> nissa t,x,y,z
+      for (int k = 0; k < 18; k++) {
+        const uint64_t w = be(sp + (mf * 18 + k) * wb, wb);
+        if (wb == 8) {
+          std::memcpy(&u[k], &w, 8);
+        } else {
+          const uint32_t w32 = (uint32_t)w;
+          float v;
+          std::memcpy(&v, &w32, 4);
+          u[k] = v;
+        }
+      }
+    }
+    const uint32_t c = crc32(sp, site_bytes);
+    suma ^= rotl(c, (uint32_t)(i % 29));
+    sumb ^= rotl(c, (uint32_t)(i % 31));
+  }
+  if (checks) {
+    checks[0] = (uint32_t)(checksum.empty() ? 0 : xml_int(checksum, "suma", 16));
+    checks[1] = (uint32_t)(checksum.empty() ? 0 : xml_int(checksum, "sumb", 16));
+    checks[2] = suma;
+    checks[3] = sumb;
+  }
+  return NQ_OK;
+}

@@ -313,6 +313,8 @@ int launch_st_force(nq_ctx* ctx, int par, const SpinorView& a, const SpinorView&
 int launch_clover_build(nq_ctx* ctx, const double* aos_e, const double* aos_o, int par, double csw,
                         const SpinorView& out);
 int launch_clover_invert(nq_ctx* ctx, const SpinorView& cl, const SpinorView& inv, double kappa, double mass);
+// Re tr sums of the temporal (ts[0]) and spatial (ts[1]) plaquettes over the lattice
+int launch_plaquette(nq_ctx* ctx, const double* aos_e, const double* aos_o, double ts[2]);
 int launch_blocks_to_f32(nq_ctx* ctx, int ncomp, const SpinorView& in, const SpinorView& out);
 int launch_clover_apply(nq_ctx* ctx, int mode, const SpinorView& out, const SpinorView& in, const SpinorView& blk,
                         const SpinorView& aux, double kappa, double mu, int dag, const double a[2][2],
