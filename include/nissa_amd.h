@@ -94,6 +94,12 @@ int nq_gauge_plaquette(nq_ctx* ctx, const nq_gauge* g, double* plaq);
    when absent) and the ones computed from the data (ildgChecksum, io/checksum.hpp).
    nissa only warns when they differ, and so does this: compare them. */
 int nq_read_ildg_gauge_conf(const char* path, int T, int LX, int LY, int LZ, double* conf_lx, uint32_t checks[4]);
+/* read_real_vector(out, path, record_name) (io/reader.hpp:52-71): the named binary
+   record (e.g. "scidac-binary-data" of a spincolor source or propagator file) as
+   nreals_per_site reals per site in nissa's degree order, same site order and
+   checksums as above; the precision (64 or 32 bit) follows from the record length */
+int nq_read_lime_real_vector(const char* path, const char* record, int nreals_per_site, int T, int LX, int LY, int LZ,
+                             double* out_lx, uint32_t checks[4]);
 
 /* ------------------------------------------------------------------ fields */
 int nq_field_create(nq_ctx* ctx, int kind, int coverage, nq_field** out);

@@ -130,6 +130,7 @@ _SIG = {
     "nq_gauge_device_bytes": ([_P], C.c_int64),
     "nq_gauge_plaquette": ([_P, _P, _DP], _I),
     "nq_read_ildg_gauge_conf": ([C.c_char_p, _I, _I, _I, _I, _DP, C.POINTER(C.c_uint32)], _I),
+    "nq_read_lime_real_vector": ([C.c_char_p, C.c_char_p, _I, _I, _I, _I, _I, _DP, C.POINTER(C.c_uint32)], _I),
     "nq_field_create": ([_P, _I, _I, C.POINTER(_P)], _I),
     "nq_field_destroy": ([_P], None),
     "nq_field_upload": ([_P, _DP], _I),
@@ -248,6 +249,14 @@ def read_ildg_gauge_conf(path, L):
     chk = (C.c_uint32 * 4)()
     _check(lib().nq_read_ildg_gauge_conf(str(path).encode(), *dims(L), _dp(conf), chk))
     return conf, (chk[0], chk[1]), (chk[2], chk[3])
+
+
+def read_lime_real_vector(path, L, record="scidac-binary-data", nreals=24):
+    """nq_read_lime_real_vector: (lx array (V, nreals), checksums read, checksums computed)."""
+    out = np.empty((int(np.prod(L)), nreals))
+    chk = (C.c_uint32 * 4)()
+    _check(lib().nq_read_lime_real_vector(str(path).encode(), record.encode(), nreals, *dims(L), _dp(out), chk))
+    return out, (chk[0], chk[1]), (chk[2], chk[3])
 
 
 def gauge_plaquette(ctx, g):

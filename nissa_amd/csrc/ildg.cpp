@@ -58,10 +58,15 @@ long long xml_int(const std::string& s, const char* tag, int base = 10)
 
 }  // namespace
 
-extern "C" int nq_read_ildg_gauge_conf(const char* path, int T, int LX, int LY, int LZ, double* conf_lx,
-                                       uint32_t checks[4])
+// The sites of one binary record into an lx buffer of nreals doubles per site.
+// gauge: the record is "ildg-binary-data", its precision and extents come from
+// "ildg-format", and the four directions move from x,y,z,t to nissa's t,x,y,z.
+// Otherwise the named record holds nreals reals per site in nissa's order, and
+// the precision follows from its length.
+static int read_sites(const char* path, const char* record, bool gauge, int nreals, int T, int LX, int LY, int LZ,
+                      double* out_lx, uint32_t checks[4])
 {
-  if (!path || !conf_lx || T < 1 || LX < 1 || LY < 1 || LZ < 1) return fail(NQ_EINVAL, "nq_read_ildg_gauge_conf: bad argument");
+  if (!path || !out_lx || T < 1 || LX < 1 || LY < 1 || LZ < 1) return fail(NQ_EINVAL, "LIME reader: bad argument");
   std::ifstream f(path, std::ios::binary);
   if (!f) return fail(NQ_EINVAL, "cannot open %s", path);
   const std::vector<unsigned char> d((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
@@ -75,20 +80,20 @@ extern "C" int nq_read_ildg_gauge_conf(const char* path, int T, int LX, int LY, 
     const std::string type(reinterpret_cast<const char*>(&d[pos + 16]), strnlen(reinterpret_cast<const char*>(&d[pos + 16]), 128));
     const size_t start = pos + 144;
     if (len > d.size() - start) return fail(NQ_EINVAL, "%s: truncated record %s", path, type.c_str());
-    if (type == "ildg-binary-data") {
+    if (type == record && !data) {
       data = &d[start];
       ndata = len;
-    } else if (type == "ildg-format") {
+    } else if (gauge && type == "ildg-format") {
       format.assign(reinterpret_cast<const char*>(&d[start]), len);
-    } else if (type == "scidac-checksum") {
-      checksum.assign(reinterpret_cast<const char*>(&d[start]), len);
+    } else if (type == "scidac-checksum" && data && checksum.empty()) {
+      checksum.assign(reinterpret_cast<const char*>(&d[start]), len);  // the one after the data record
     }
     pos = start + ((len + 7) / 8) * 8;
   }
-  if (!data) return fail(NQ_EINVAL, "%s: record ildg-binary-data not found", path);
+  if (!data) return fail(NQ_EINVAL, "%s: record %s not found", path, record);
   const int64_t V = (int64_t)T * LX * LY * LZ;
   int prec = (int)xml_int(format, "precision");
-  if (prec < 0) prec = ndata == (uint64_t)V * 72 * 4 ? 32 : 64;
+  if (prec < 0) prec = ndata == (uint64_t)V * nreals * 4 ? 32 : 64;
   if (prec != 32 && prec != 64) return fail(NQ_EINVAL, "%s: precision %d not supported", path, prec);
   const long long fl[4] = {xml_int(format, "lt"), xml_int(format, "lx"), xml_int(format, "ly"), xml_int(format, "lz")};
   const int want[4] = {T, LX, LY, LZ};
@@ -97,18 +102,20 @@ extern "C" int nq_read_ildg_gauge_conf(const char* path, int T, int LX, int LY, 
       return fail(NQ_EINVAL, "%s: lattice %lldx%lldx%lldx%lld in the file, %dx%dx%dx%d requested", path, fl[0], fl[1],
                   fl[2], fl[3], T, LX, LY, LZ);
   const int wb = prec / 8;
-  const size_t site_bytes = (size_t)72 * wb;
+  const size_t site_bytes = (size_t)nreals * wb;
   if (ndata != (uint64_t)V * site_bytes)
     return fail(NQ_EINVAL, "%s: %llu data bytes, %lld expected", path, (unsigned long long)ndata, (long long)(V * site_bytes));
+  const int per_block = gauge ? 18 : nreals;
   uint32_t suma = 0, sumb = 0;
   for (int64_t i = 0; i < V; i++) {
     const unsigned char* sp = data + i * site_bytes;
     const int64_t x = i % LX, y = (i / LX) % LY, z = (i / ((int64_t)LX * LY)) % LZ, t = i / ((int64_t)LX * LY * LZ);
     const int64_t lx = ((t * LX + x) * LY + y) * LZ + z;
-    for (int mf = 0; mf < 4; mf++) {
-      double* u = conf_lx + (lx * 4 + (mf + 1) % 4) * 18;  // file x,y,z,t -> nissa t,x,y,z
-      for (int k = 0; k < 18; k++) {
-        const uint64_t w = be(sp + (mf * 18 + k) * wb, wb);
+    for (int b = 0; b < nreals / per_block; b++) {
+      // gauge: file direction b (x,y,z,t) -> nissa (b + 1) % 4 (t,x,y,z)
+      double* u = out_lx + lx * nreals + (gauge ? (b + 1) % 4 : b) * per_block;
+      for (int k = 0; k < per_block; k++) {
+        const uint64_t w = be(sp + (b * per_block + k) * wb, wb);
         if (wb == 8) {
           std::memcpy(&u[k], &w, 8);
         } else {
@@ -130,4 +137,17 @@ extern "C" int nq_read_ildg_gauge_conf(const char* path, int T, int LX, int LY, 
     checks[3] = sumb;
   }
   return NQ_OK;
+}
+
+extern "C" int nq_read_ildg_gauge_conf(const char* path, int T, int LX, int LY, int LZ, double* conf_lx,
+                                       uint32_t checks[4])
+{
+  return read_sites(path, "ildg-binary-data", true, 72, T, LX, LY, LZ, conf_lx, checks);
+}
+
+extern "C" int nq_read_lime_real_vector(const char* path, const char* record, int nreals_per_site, int T, int LX,
+                                        int LY, int LZ, double* out_lx, uint32_t checks[4])
+{
+  if (!record || nreals_per_site < 1) return fail(NQ_EINVAL, "nq_read_lime_real_vector: bad argument");
+  return read_sites(path, record, false, nreals_per_site, T, LX, LY, LZ, out_lx, checks);
 }

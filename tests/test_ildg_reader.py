@@ -60,3 +60,13 @@ def test_device_plaquette_matches_oracle(Ls):
         p = nq.gauge_plaquette(ctx, g)
         g.close()
     assert abs(p - ref) < 1e-14
+
+
+@pytest.mark.parametrize("fname", ["source_1.0003.00.inverted.00", "source_2.0003.00.inverted.04"])
+def test_real_vector_reader_on_propagator_files(golden_dir, fname):
+    """read_real_vector on the tmLQCD propagators of the reference's meson_2pts
+    test: the same reals as the tests' reader and the file's own checksum."""
+    v, read, comp = nq.read_lime_real_vector(f"{golden_dir}/{fname}", L)
+    ref = lime.read_scidac_spincolor(f"{golden_dir}/{fname}", L)
+    assert np.array_equal(v, ref.reshape(len(ref), 24))
+    assert read != (0, 0) and read == comp
