@@ -13,7 +13,6 @@ from __future__ import annotations
 
 import ctypes as C
 import os
-import weakref
 
 import numpy as np
 
@@ -302,19 +301,28 @@ class Ctx:
         self.V = volume(L)
         self.Vh = self.V // 2
         self.h = None
-        self._children = weakref.WeakSet()
+        # native handles of the device objects made in this context, with their
+        # destroy functions.  Only handles are kept, never the objects: when the
+        # cyclic garbage collector finalises a context together with its objects it
+        # clears weak references first and runs the finalisers in any order, so the
+        # context must be able to destroy whatever is still open by itself, and an
+        # object finalised after its context must not touch the freed context.
+        self._open = {}
         h = _P()
         _check(lib().nq_ctx_create(*self.L, device, C.byref(h)))
         self.h = h
 
-    def _adopt(self, obj):
-        self._children.add(obj)
+    def _adopt(self, obj, destroy):
+        key = id(obj)
+        self._open[key] = (destroy, obj.h)
+        obj._release = lambda: self._open.pop(key, None) is not None
 
     def close(self):
         if self.h:
             # device objects of this context must go before the context itself
-            for ch in list(self._children):
-                ch.close()
+            for destroy, h in reversed(list(self._open.values())):
+                destroy(h)
+            self._open.clear()
             lib().nq_ctx_destroy(self.h)
             self.h = None
 
@@ -376,14 +384,15 @@ class Gauge:
             _check(lib().nq_gauge_create_eo(ctx.h, _dp(np.ascontiguousarray(evn)), _dp(np.ascontiguousarray(odd)),
                                             C.byref(h)))
         self.h = h
-        ctx._adopt(self)
+        ctx._adopt(self, lib().nq_gauge_destroy)
 
     def device_bytes(self):
         return lib().nq_gauge_device_bytes(self.h)
 
     def close(self):
         if self.h:
-            lib().nq_gauge_destroy(self.h)
+            if self._release():  # still registered: the context has not destroyed it
+                lib().nq_gauge_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -403,7 +412,7 @@ class Clover:
         h = _P()
         _check(lib().nq_clover_create(ctx.h, g.h, csw, C.byref(h)))
         self.h = h
-        ctx._adopt(self)
+        ctx._adopt(self, lib().nq_clover_destroy)
 
     def download_lx(self):
         out = np.empty((self.ctx.V, 4, 3, 3, 2))
@@ -412,7 +421,8 @@ class Clover:
 
     def close(self):
         if self.h:
-            lib().nq_clover_destroy(self.h)
+            if self._release():  # still registered: the context has not destroyed it
+                lib().nq_clover_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -429,7 +439,7 @@ class Field:
         h = _P()
         _check(lib().nq_field_create(ctx.h, kind, coverage, C.byref(h)))
         self.h = h
-        ctx._adopt(self)
+        ctx._adopt(self, lib().nq_field_destroy)
 
     @property
     def shape(self):
@@ -453,7 +463,8 @@ class Field:
 
     def close(self):
         if self.h:
-            lib().nq_field_destroy(self.h)
+            if self._release():  # still registered: the context has not destroyed it
+                lib().nq_field_destroy(self.h)
             self.h = None
 
     def __del__(self):

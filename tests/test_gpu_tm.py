@@ -292,3 +292,28 @@ def test_sloppy_hop_storage_formats(setups, prec, tol):
     fo = ctx.field(nq.SPINCOLOR, nq.EVEN)
     nq.bench_tm_hop(ctx, g, 2, fo, fi, prec=prec)
     assert rel_l2(fo.download(), geo.tm_hop(0, ce, co, x)) < tol
+
+
+def test_context_collected_in_a_cycle_with_its_objects(setups):
+    """A context finalised by the cyclic garbage collector together with its
+    gauge and fields (weak references cleared first, finalisers in any order)
+    destroys them itself, and nothing touches the freed context afterwards: the
+    next context's launches see no stale HIP error."""
+    import gc
+    L = (4, 4, 4, 4)
+    conf = tm_conf(L, 1)
+    for _ in range(4):
+        ctx = nq.Ctx(L)
+        g = nq.Gauge(ctx, lx=conf)
+        f = ctx.field(nq.SPINCOLOR, nq.ODD)
+        cyc = {"ctx": ctx, "g": g, "f": f}
+        cyc["self"] = cyc
+        del ctx, g, f, cyc
+        gc.collect()
+    ctx, g, geo, conf8, ce, co = setups((8, 8, 8, 8))
+    x = gauss_eo(geo.Vh, 3)
+    fi = ctx.field(nq.SPINCOLOR, nq.ODD, x)
+    fo = ctx.field(nq.SPINCOLOR, nq.EVEN)
+    nq.tm_hop(ctx, g, fo, fi)
+    ref = geo.tm_hop(0, ce, co, x)
+    assert rel_l2(fo.download(), ref) < 1e-14
