@@ -98,6 +98,11 @@ int nq_read_ildg_gauge_conf(const char* path, int T, int LX, int LY, int LZ, dou
    record (e.g. "scidac-binary-data" of a spincolor source or propagator file) as
    nreals_per_site reals per site in nissa's degree order, same site order and
    checksums as above; the precision (64 or 32 bit) follows from the record length */
+/* unitarity_check_lx_conf (operations/gaugeconf.hpp:204-245), which nissa runs after
+   reading a conf: per link |U U^+ - 1| (Frobenius, su3_get_non_unitariness,
+   new_types/su3_op.hpp:3037-3055); out = {average, maximum}, nbroken (may be NULL)
+   counts the links above 1e-13 */
+int nq_host_unitarity_check_lx(int T, int LX, int LY, int LZ, const double* conf_lx, double out[2], int64_t* nbroken);
 int nq_read_lime_real_vector(const char* path, const char* record, int nreals_per_site, int T, int LX, int LY, int LZ,
                              double* out_lx, uint32_t checks[4]);
 

@@ -129,6 +129,7 @@ _SIG = {
     "nq_gauge_device_bytes": ([_P], C.c_int64),
     "nq_gauge_plaquette": ([_P, _P, _DP], _I),
     "nq_read_ildg_gauge_conf": ([C.c_char_p, _I, _I, _I, _I, _DP, C.POINTER(C.c_uint32)], _I),
+    "nq_host_unitarity_check_lx": ([_I, _I, _I, _I, _DP, _DP, C.POINTER(C.c_int64)], _I),
     "nq_read_lime_real_vector": ([C.c_char_p, C.c_char_p, _I, _I, _I, _I, _I, _DP, C.POINTER(C.c_uint32)], _I),
     "nq_field_create": ([_P, _I, _I, C.POINTER(_P)], _I),
     "nq_field_destroy": ([_P], None),
@@ -256,6 +257,15 @@ def read_lime_real_vector(path, L, record="scidac-binary-data", nreals=24):
     chk = (C.c_uint32 * 4)()
     _check(lib().nq_read_lime_real_vector(str(path).encode(), record.encode(), nreals, *dims(L), _dp(out), chk))
     return out, (chk[0], chk[1]), (chk[2], chk[3])
+
+
+def unitarity_check_lx(L, conf):
+    """nq_host_unitarity_check_lx: (average, maximum, number above 1e-13) of |U U^+ - 1|."""
+    out = (C.c_double * 2)()
+    nb = C.c_int64(0)
+    _check(lib().nq_host_unitarity_check_lx(*dims(L), _dp(np.ascontiguousarray(conf, dtype=np.float64)),
+                                            C.cast(out, _DP), C.byref(nb)))
+    return out[0], out[1], nb.value
 
 
 def gauge_plaquette(ctx, g):

@@ -8,6 +8,7 @@
 // su3 in direction order x, y, z, t, big-endian reals of the precision that the
 // "ildg-format" record names.  The SciDAC checksum XORs, over the file sites i,
 // the CRC-32 of the site's bytes rotated left by i % 29 (suma) and i % 31 (sumb).
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -150,4 +151,44 @@ extern "C" int nq_read_lime_real_vector(const char* path, const char* record, in
 {
   if (!record || nreals_per_site < 1) return fail(NQ_EINVAL, "nq_read_lime_real_vector: bad argument");
   return read_sites(path, record, false, nreals_per_site, T, LX, LY, LZ, out_lx, checks);
+}
+
+// unitarity_check_lx_conf (operations/gaugeconf.hpp:204-245) with
+// su3_get_non_unitariness (new_types/su3_op.hpp:3037-3055): per link the Frobenius
+// norm of U U^+ - 1, i.e. sqrt(sum_i (|row_i|^2 - 1)^2 + 2 sum_{i<j} |<row_i, row_j>|^2);
+// out[0] = average over the 4 V links, out[1] = maximum; nbroken counts links above 1e-13
+extern "C" int nq_host_unitarity_check_lx(int T, int LX, int LY, int LZ, const double* conf_lx, double out[2],
+                                          int64_t* nbroken)
+{
+  if (!conf_lx || !out || T < 1 || LX < 1 || LY < 1 || LZ < 1) return fail(NQ_EINVAL, "nq_host_unitarity_check_lx: bad argument");
+  const int64_t nlinks = (int64_t)T * LX * LY * LZ * 4;
+  double sum = 0, mx = 0;
+  int64_t nb = 0;
+  for (int64_t l = 0; l < nlinks; l++) {
+    const double* u = conf_lx + l * 18;
+    double res = 0;
+    for (int i = 0; i < 3; i++) {
+      double n2 = 0;
+      for (int k = 0; k < 6; k++) n2 += u[i * 6 + k] * u[i * 6 + k];
+      res += (n2 - 1.0) * (n2 - 1.0);
+      for (int j = i + 1; j < 3; j++) {
+        double re = 0, im = 0;  // sum_c conj(u_ic) u_jc
+        for (int c = 0; c < 3; c++) {
+          const double ar = u[i * 6 + 2 * c], ai = u[i * 6 + 2 * c + 1];
+          const double br = u[j * 6 + 2 * c], bi = u[j * 6 + 2 * c + 1];
+          re += ar * br + ai * bi;
+          im += ar * bi - ai * br;
+        }
+        res += 2 * (re * re + im * im);
+      }
+    }
+    const double err = std::sqrt(res);
+    sum += err;
+    mx = err > mx ? err : mx;
+    nb += err > 1e-13;
+  }
+  out[0] = sum / (double)nlinks;
+  out[1] = mx;
+  if (nbroken) *nbroken = nb;
+  return NQ_OK;
 }

@@ -70,3 +70,19 @@ def test_real_vector_reader_on_propagator_files(golden_dir, fname):
     ref = lime.read_scidac_spincolor(f"{golden_dir}/{fname}", L)
     assert np.array_equal(v, ref.reshape(len(ref), 24))
     assert read != (0, 0) and read == comp
+
+
+def test_unitarity_check(golden_dir):
+    """nissa's post-read unitarity check: the reference conf is unitary to rounding,
+    and a perturbed link gives the Frobenius norm of U U^+ - 1 that numpy computes."""
+    conf, _, _ = nq.read_ildg_gauge_conf(f"{golden_dir}/L4T8conf", L)
+    avg, mx, nb = nq.unitarity_check_lx(L, conf)
+    assert mx < 1e-13 and avg <= mx and nb == 0
+    U = conf[5, 2, ..., 0] + 1j * conf[5, 2, ..., 1]
+    W = U @ (np.eye(3) + 1e-4 * np.array([[0, 1, 0], [0.5j, 0, 0], [0, 0, 2]]))
+    conf[5, 2, ..., 0], conf[5, 2, ..., 1] = W.real, W.imag
+    want = np.linalg.norm(W @ W.conj().T - np.eye(3))
+    avg2, mx2, nb2 = nq.unitarity_check_lx(L, conf)
+    assert abs(mx2 - want) < 1e-12 * want and nb2 == 1
+    n = conf.shape[0] * 4  # links; the replaced link's own deviation was below 1e-13
+    assert abs(avg2 - (avg * n + want) / n) < 1e-13
