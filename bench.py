@@ -16,8 +16,8 @@ the timed workload (HIP-event ledger kept by the library), and the CPU oracle's
 full FP64 solve of the same workload timed on the host cores.  That oracle
 solve is also the reference for the error of each precision's GPU solution;
 the 8^4 case (kappa 0.16, mu 0.01, Z2 source on the even sites, residue 1e-24)
-adds the error against the oracle's __float128 solve (--quad does the same at
-the full lattice).
+adds the error against the oracle's __float128 solve, and so does the full
+lattice (--no-quad skips that one).
 """
 import argparse
 import json
@@ -44,7 +44,8 @@ def main():
     ap.add_argument("--lattice", default="64,32,32,32", help="T,LX,LY,LZ")
     ap.add_argument("--prec", default="f16", choices=["f64", "f32", "f16", "f16h"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--quad", action="store_true", help="also the __float128 oracle solve at the full lattice")
+    ap.add_argument("--no-quad", action="store_true",
+                    help="skip the __float128 oracle solve at the full lattice (about 40 s of host time)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-stag", action="store_true", help="skip the staggered multi-shift leg")
     ap.add_argument("--no-24", action="store_true", help="skip the 24^3x48 FP64 Dslash and solve leg")
@@ -178,7 +179,7 @@ def main():
     cpu = None
     accuracy = None
     if not args.no_cpu:
-        cpu, accuracy = cpu_baseline(L, conf, src, kappa, mu, residue, gpu_sols, small, args.quad)
+        cpu, accuracy = cpu_baseline(L, conf, src, kappa, mu, residue, gpu_sols, small, not args.no_quad)
 
     out = {
         "metric": METRIC,
@@ -556,15 +557,27 @@ def eo_solve_quad(geo, conf, kappa, mu, src):
     return geo.paste(xe, xo), st
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(L, conf, src, kappa, mu, residue, gpu_sols, small, quad):
     """The CPU leg.  Times the oracle's full FP64 eo solve of the workload on the
     host cores (OpenMP, the reference's unfused pass sequence) as the baseline, and
     uses the oracle's solutions to check the GPU's: the relative L2 error of each
     precision's solution against the FP64 oracle solve, and for the 8^4 case also
-    against the __float128 solve (at the full lattice too with `quad`)."""
+    against the __float128 solve (at the full lattice too with `quad`).  `cores` is
+    the oracle's OpenMP thread count as the oracle reports it."""
     from oracle import oracle as orc
 
-    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    cores = orc.omp_max_threads()  # the threads the oracle's loops actually run on
     geo = orc.Geom(L)
     t0 = time.perf_counter()
     x_cpu, st = geo.tm_solve_eoprec(conf, kappa, mu, 100000, residue, src)
@@ -572,7 +585,8 @@ def cpu_baseline(L, conf, src, kappa, mu, residue, gpu_sols, small, quad):
     log(f"[bench] CPU oracle: FP64 eo solve {t_solve:.1f}s, {st['iters']} iterations on {cores} threads")
     cpu = {"value": round(1.0 / t_solve, 6), "unit": "solves/s", "cores": cores, "kind": "port",
            "sample": f"one full oracle FP64 eo solve of the workload at {L[1]}^3x{L[0]} "
-                     f"({st['iters']} CG iterations, {t_solve:.1f}s)"}
+                     f"({st['iters']} CG iterations, {t_solve:.1f}s)",
+           "cpu_model": cpu_model(), "omp_threads": cores, "host_cpus": os.cpu_count()}
     acc = {f"{L[1]}^3x{L[0]}": {"vs_fp64_oracle": {p: _rel_l2(x, x_cpu) for p, x in gpu_sols.items()},
                                   "oracle_iters": st["iters"]}}
     if quad:

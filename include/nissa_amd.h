@@ -80,6 +80,19 @@ int nq_gauge_create_eo(nq_ctx* ctx, const double* conf_evn, const double* conf_o
 /* LxField<quad_su3>: split on the device like split_lx_vector_into_eo_parts */
 int nq_gauge_create_lx(nq_ctx* ctx, const double* conf_lx, nq_gauge** out);
 void nq_gauge_destroy(nq_gauge* g);
+/* No counterpart in the reference: for callers that cache gauge handles by host buffer
+   (the shims' gauge cache).  nissa changes confs in place (put_boundaries_conditions,
+   adapt_theta: operations/gaugeconf.cpp:125-160; add_or_rem_stagphases_to_conf:
+   hmc/backfield.cpp:230-245; a new conf read into the same buffer), so a cached
+   handle is valid only while the host conf is unchanged.  nq_host_conf_fingerprint
+   hashes a sample of the buffer (the first and last site and ~65536 doubles spread
+   over every real of a site); nq_gauge_matches_* set *same = 1 when the host conf's
+   fingerprint equals the one taken when the handle was made from it.  A change
+   confined to a few links can escape the sample: callers that make such changes
+   destroy the handle themselves. */
+int nq_host_conf_fingerprint(const double* data, int64_t ndoubles, uint64_t* out);
+int nq_gauge_matches_lx(const nq_gauge* g, const double* conf_lx, int* same);
+int nq_gauge_matches_eo(const nq_gauge* g, const double* conf_evn, const double* conf_odd, int* same);
 /* device bytes held by the gauge handle (all precisions built so far) */
 int64_t nq_gauge_device_bytes(const nq_gauge* g);
 /* global_plaquette_lx_conf (operations/gauge/plaquette.hpp:52-84): the average of
@@ -137,9 +150,14 @@ int nq_tm_cg_kernsq(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* 
                     int niter, double residue, int prec, nq_field* sol_odd,
                     const nq_field* guess_odd, const nq_field* src_odd, nq_solve_stats* st);
 /* inv_tmD_cg_eoprec (cg_invert_tmD_eoprec.cpp:152-213): full eo-preconditioned solve of
-   D sol = src on lx, plus the FP64 check |g5 Q sol - src|^2/|src|^2 */
+   D sol = src on lx, plus the FP64 check |g5 Q sol - src|^2/|src|^2.
+   guess_Koo (odd field or NULL) is nissa's std::optional<OddField<spincolor>> guess_Koo
+   (:102-103,152-153): the start of the K K^+ CG of eq. 9; on return it holds that CG's
+   solution (the reference's "if a guess was passed, return new one", :137; nissa takes
+   the optional by value, so its caller never sees the update: a shim that keeps nissa's
+   semantics passes a copy). */
 int nq_tm_solve_eoprec(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis, double mu,
-                       int niter, double residue, int prec, nq_field* sol_lx,
+                       int niter, double residue, int prec, nq_field* sol_lx, nq_field* guess_Koo,
                        const nq_field* src_lx, nq_solve_stats* st, double* check_rel_res_sq);
 /* inv_tmQ2_cg (cg_invert_tmQ2.cpp:12-19): CG on Q(-mu)Q(+mu) over lx */
 int nq_tmQ2_cg(nq_ctx* ctx, const nq_gauge* g, double kappa, double mu, int niter, double residue,
@@ -155,10 +173,12 @@ int nq_tmDQ_cgm(nq_ctx* ctx, const nq_gauge* g, double kappa, int nmass, const d
                 const double* residues, nq_field** sols_lx, const nq_field* src_lx, nq_solve_stats* st,
                 double* true_res_sq);
 /* multi-RHS eo solve: nrhs sources share the gauge handle, kappa and the iteration;
-   mu[r] per right-hand side (per-RHS diagonal). Residuals are per RHS. */
+   mu[r] per right-hand side (per-RHS diagonal). Residuals are per RHS.  guess_Koo (NULL,
+   or nrhs odd fields of which any may be NULL) as in nq_tm_solve_eoprec, per source. */
 int nq_tm_solve_eoprec_multi(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis,
                              int nrhs, const double* mu, int niter, double residue, int prec,
-                             nq_field** sol_lx, const nq_field* const* src_lx, nq_solve_stats* st);
+                             nq_field** sol_lx, nq_field** guess_Koo, const nq_field* const* src_lx,
+                             nq_solve_stats* st);
 
 /* ------------------------------------------------------------------ twisted clover (device handles)
    Clover term from a gauge handle, kept on the device (FP64, both parities); the
@@ -258,11 +278,16 @@ int nq_st_rootst_force(nq_ctx* ctx, const nq_gauge* g, const double* u1b_evn, co
 
 /* ------------------------------------------------------------------ host-buffer drop-ins */
 /* Same operations over nissa host buffers; the gauge handle stays resident. */
+/* tmDkern_eoprec_square_eos (..._portable.cpp:54-65) on OddField<spincolor> host buffers
+   (Vh x 24 doubles, nissa eo order) */
 int nq_tm_kern_square_host(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis,
                            double mu, double* out_odd, const double* in_odd);
+/* inv_tmD_cg_eoprec (cg_invert_tmD_eoprec.cpp:152-213) on LxField<spincolor> host buffers;
+   guess_Koo (NULL or Vh x 24 doubles, nissa eo order) as in nq_tm_solve_eoprec */
 int nq_tm_solve_eoprec_host(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis,
                             double mu, int niter, double residue, int prec, double* sol_lx,
-                            const double* src_lx, nq_solve_stats* st, double* check_rel_res_sq);
+                            double* guess_Koo, const double* src_lx, nq_solve_stats* st,
+                            double* check_rel_res_sq);
 
 /* ------------------------------------------------------------------ host-side harness helpers */
 /* Seeded synthetic gauge field U = exp(i eps H) per link, H traceless Hermitian with

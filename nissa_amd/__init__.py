@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -127,6 +128,9 @@ _SIG = {
     "nq_gauge_create_lx": ([_P, _DP, C.POINTER(_P)], _I),
     "nq_gauge_destroy": ([_P], None),
     "nq_gauge_device_bytes": ([_P], C.c_int64),
+    "nq_host_conf_fingerprint": ([_DP, C.c_int64, C.POINTER(C.c_uint64)], _I),
+    "nq_gauge_matches_lx": ([_P, _DP, C.POINTER(_I)], _I),
+    "nq_gauge_matches_eo": ([_P, _DP, _DP, C.POINTER(_I)], _I),
     "nq_gauge_plaquette": ([_P, _P, _DP], _I),
     "nq_read_ildg_gauge_conf": ([C.c_char_p, _I, _I, _I, _I, _DP, C.POINTER(C.c_uint32)], _I),
     "nq_host_unitarity_check_lx": ([_I, _I, _I, _I, _DP, _DP, C.POINTER(C.c_int64)], _I),
@@ -144,17 +148,18 @@ _SIG = {
     "nq_tm_apply_Q": ([_P, _P, _D, _DP, _D, _P, _P], _I),
     "nq_tm_apply_Q2": ([_P, _P, _D, _D, _P, _P], _I),
     "nq_tm_cg_kernsq": ([_P, _P, _D, _DP, _D, _I, _D, _I, _P, _P, _P, C.POINTER(SolveStats)], _I),
-    "nq_tm_solve_eoprec": ([_P, _P, _D, _DP, _D, _I, _D, _I, _P, _P, C.POINTER(SolveStats), _DP], _I),
+    "nq_tm_solve_eoprec": ([_P, _P, _D, _DP, _D, _I, _D, _I, _P, _P, _P, C.POINTER(SolveStats), _DP], _I),
     "nq_tmQ2_cg": ([_P, _P, _D, _D, _I, _D, _P, _P, _P, C.POINTER(SolveStats)], _I),
     "nq_tm_solve_eoprec_multi": ([_P, _P, _D, _DP, _I, _DP, _I, _D, _I, C.POINTER(_P), C.POINTER(_P),
-                                  C.POINTER(SolveStats)], _I),
+                                  C.POINTER(_P), C.POINTER(SolveStats)], _I),
     "nq_st_2Doe": ([_P, _P, _P, _P], _I),
     "nq_st_Deo_half": ([_P, _P, _P, _P], _I),
     "nq_st_DDee": ([_P, _P, _D, _P, _P], _I),
     "nq_st_cg": ([_P, _P, _D, _I, _D, _P, _P, _P, C.POINTER(SolveStats)], _I),
     "nq_st_cgm": ([_P, _P, _I, _I, _DP, _I, _DP, _I, C.POINTER(_P), C.POINTER(_P), C.POINTER(SolveStats), _DP], _I),
     "nq_tm_kern_square_host": ([_P, _P, _D, _DP, _D, _DP, _DP], _I),
-    "nq_tm_solve_eoprec_host": ([_P, _P, _D, _DP, _D, _I, _D, _I, _DP, _DP, C.POINTER(SolveStats), _DP], _I),
+    "nq_tm_solve_eoprec_host": ([_P, _P, _D, _DP, _D, _I, _D, _I, _DP, _DP, _DP, C.POINTER(SolveStats), _DP],
+                                _I),
     "nq_host_gauge_random_lx": ([_I, _I, _I, _I, C.c_uint64, _D, _DP], _I),
     "nq_host_link_algebra": ([C.c_uint64, C.c_int64, _I, _D, _DP, _DP], _I),
     "nq_host_gauge_phase_lx": ([_I, _I, _I, _I, _DP, _DP], _I),
@@ -261,11 +266,29 @@ def read_lime_real_vector(path, L, record="scidac-binary-data", nreals=24):
 
 def unitarity_check_lx(L, conf):
     """nq_host_unitarity_check_lx: (average, maximum, number above 1e-13) of |U U^+ - 1|."""
+    conf = np.ascontiguousarray(conf, dtype=np.float64)
+    if conf.size != volume(L) * 72:
+        raise NqError(NQ_EINVAL, f"conf of {conf.size} doubles, {volume(L) * 72} expected for {dims(L)}")
     out = (C.c_double * 2)()
     nb = C.c_int64(0)
-    _check(lib().nq_host_unitarity_check_lx(*dims(L), _dp(np.ascontiguousarray(conf, dtype=np.float64)),
+    _check(lib().nq_host_unitarity_check_lx(*dims(L), _dp(conf),
                                             C.cast(out, _DP), C.byref(nb)))
     return out[0], out[1], nb.value
+
+
+def conf_fingerprint(a):
+    """nq_host_conf_fingerprint of a float64 host array."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    out = C.c_uint64(0)
+    _check(lib().nq_host_conf_fingerprint(_dp(a), a.size, C.byref(out)))
+    return out.value
+
+
+def gauge_matches_lx(g, conf_lx):
+    """True when the host lx conf still matches the one the handle was made from."""
+    same = C.c_int(0)
+    _check(lib().nq_gauge_matches_lx(g.h, _dp(np.ascontiguousarray(conf_lx, dtype=np.float64)), C.byref(same)))
+    return bool(same.value)
 
 
 def gauge_plaquette(ctx, g):
@@ -324,14 +347,21 @@ class Ctx:
 
     def _adopt(self, obj, destroy):
         key = id(obj)
-        self._open[key] = (destroy, obj.h)
+        # the weak reference only serves to mark a live object closed; when it is
+        # already cleared (the object is being collected) the handle alone suffices
+        self._open[key] = (destroy, obj.h, weakref.ref(obj))
         obj._release = lambda: self._open.pop(key, None) is not None
 
     def close(self):
         if self.h:
-            # device objects of this context must go before the context itself
-            for destroy, h in reversed(list(self._open.values())):
+            # device objects of this context must go before the context itself; a
+            # live object is marked closed, so that using it afterwards passes a null
+            # handle (NQ_EINVAL) and never a freed one
+            for destroy, h, ref in reversed(list(self._open.values())):
                 destroy(h)
+                o = ref()
+                if o is not None:
+                    o.h = None
             self._open.clear()
             lib().nq_ctx_destroy(self.h)
             self.h = None
@@ -389,10 +419,16 @@ class Gauge:
         self.h = None
         h = _P()
         if lx is not None:
-            _check(lib().nq_gauge_create_lx(ctx.h, _dp(np.ascontiguousarray(lx)), C.byref(h)))
+            lx = np.ascontiguousarray(lx, dtype=np.float64)
+            if lx.size != ctx.V * 72:
+                raise NqError(NQ_EINVAL, f"lx conf of {lx.size} doubles, {ctx.V * 72} expected")
+            _check(lib().nq_gauge_create_lx(ctx.h, _dp(lx), C.byref(h)))
         else:
-            _check(lib().nq_gauge_create_eo(ctx.h, _dp(np.ascontiguousarray(evn)), _dp(np.ascontiguousarray(odd)),
-                                            C.byref(h)))
+            evn = np.ascontiguousarray(evn, dtype=np.float64)
+            odd = np.ascontiguousarray(odd, dtype=np.float64)
+            if evn.size != ctx.Vh * 72 or odd.size != ctx.Vh * 72:
+                raise NqError(NQ_EINVAL, f"eo conf of {evn.size}+{odd.size} doubles, 2x{ctx.Vh * 72} expected")
+            _check(lib().nq_gauge_create_eo(ctx.h, _dp(evn), _dp(odd), C.byref(h)))
         self.h = h
         ctx._adopt(self, lib().nq_gauge_destroy)
 
@@ -566,14 +602,43 @@ def tm_cg_kernsq(ctx, g, kappa, mu, niter, residue, sol, src, prec="f64", guess=
     return _solve(rc, st)
 
 
-def tm_solve_eoprec(ctx, g, kappa, mu, niter, residue, sol, src, prec="f64", anis=None):
+def tm_solve_eoprec(ctx, g, kappa, mu, niter, residue, sol, src, prec="f64", anis=None, guess=None):
+    """inv_tmD_cg_eoprec; guess (odd field, optional) starts the eq. 9 CG and receives its solution."""
     st = SolveStats()
     chk = C.c_double(0)
-    rc = lib().nq_tm_solve_eoprec(ctx.h, g.h, kappa, _anis(anis), mu, niter, residue, PREC[prec], sol.h, src.h,
-                                  C.byref(st), C.byref(chk))
+    rc = lib().nq_tm_solve_eoprec(ctx.h, g.h, kappa, _anis(anis), mu, niter, residue, PREC[prec], sol.h,
+                                  guess.h if guess is not None else None, src.h, C.byref(st), C.byref(chk))
     d = _solve(rc, st)
     d["check_rel_res_sq"] = chk.value
     return d
+
+
+def tm_solve_eoprec_host(ctx, g, kappa, mu, niter, residue, src_lx, prec="f64", anis=None, guess=None):
+    """nq_tm_solve_eoprec_host on nissa host buffers: returns (solution lx, stats); guess
+    (odd host array (Vh,4,3,2), optional) is updated in place with the eq. 9 solution."""
+    src_lx = np.ascontiguousarray(src_lx, dtype=np.float64)
+    if src_lx.size != ctx.V * 24:
+        raise NqError(NQ_EINVAL, f"source of {src_lx.size} doubles, {ctx.V * 24} expected")
+    if guess is not None and (guess.size != ctx.Vh * 24 or guess.dtype != np.float64 or not guess.flags.c_contiguous):
+        raise NqError(NQ_EINVAL, "guess must be a contiguous float64 odd field")
+    sol = np.empty((ctx.V, 4, 3, 2))
+    st = SolveStats()
+    chk = C.c_double(0)
+    rc = lib().nq_tm_solve_eoprec_host(ctx.h, g.h, kappa, _anis(anis), mu, niter, residue, PREC[prec], _dp(sol),
+                                       _dp(guess), _dp(src_lx), C.byref(st), C.byref(chk))
+    d = _solve(rc, st)
+    d["check_rel_res_sq"] = chk.value
+    return sol, d
+
+
+def tm_kern_square_host(ctx, g, kappa, mu, inp_odd, anis=None):
+    """nq_tm_kern_square_host: K K^+ on an odd host array (Vh,4,3,2)."""
+    inp_odd = np.ascontiguousarray(inp_odd, dtype=np.float64)
+    if inp_odd.size != ctx.Vh * 24:
+        raise NqError(NQ_EINVAL, f"input of {inp_odd.size} doubles, {ctx.Vh * 24} expected")
+    out = np.empty((ctx.Vh, 4, 3, 2))
+    _check(lib().nq_tm_kern_square_host(ctx.h, g.h, kappa, _anis(anis), mu, _dp(out), _dp(inp_odd)))
+    return out
 
 
 def tmclov_cg_kernsq(ctx, g, cl, kappa, mu, niter, residue, sol, src, guess=None, anis=None, prec="f64"):
@@ -600,14 +665,19 @@ def tmQ2_cg(ctx, g, kappa, mu, niter, residue, sol, src, guess=None):
     return _solve(rc, st)
 
 
-def tm_solve_eoprec_multi(ctx, g, kappa, mus, niter, residue, sols, srcs, prec="f16", anis=None):
+def tm_solve_eoprec_multi(ctx, g, kappa, mus, niter, residue, sols, srcs, prec="f16", anis=None, guesses=None):
     n = len(srcs)
     assert len(sols) == n and len(mus) == n
     mu = (C.c_double * n)(*mus)
     so = (_P * n)(*[s.h for s in sols])
     sr = (_P * n)(*[s.h for s in srcs])
+    gs = None
+    if guesses is not None:
+        assert len(guesses) == n
+        gs = (_P * n)(*[x.h if x is not None else None for x in guesses])
     st = (SolveStats * n)()
-    rc = lib().nq_tm_solve_eoprec_multi(ctx.h, g.h, kappa, _anis(anis), n, mu, niter, residue, PREC[prec], so, sr, st)
+    rc = lib().nq_tm_solve_eoprec_multi(ctx.h, g.h, kappa, _anis(anis), n, mu, niter, residue, PREC[prec], so, gs, sr,
+                                        st)
     if rc not in (NQ_OK, NQ_ENOCONV):
         _check(rc)
     return [dict(st[i].as_dict(), rc=rc) for i in range(n)]

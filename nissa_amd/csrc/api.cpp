@@ -3,6 +3,7 @@
 // (src/base/field.hpp:659-1805, src/base/memory_manager.hpp:221-434,
 // src/threads/threads.hpp:86-265) with explicit device handles.
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdarg>
@@ -54,6 +55,48 @@ int ctx_temp(nq_ctx* ctx, const std::string& role, size_t bytes, void** out)
   NQ_TRY(dev_alloc(b, bytes));
   *out = b.p;
   return NQ_OK;
+}
+
+// free the temporaries whose role starts with `prefix` (after the stream has drained:
+// kernels already enqueued may still read them)
+int ctx_temp_release(nq_ctx* ctx, const std::string& prefix)
+{
+  NQ_HIP(hipStreamSynchronize(ctx->stream));
+  for (auto it = ctx->temps.begin(); it != ctx->temps.end();) {
+    if (it->first.compare(0, prefix.size(), prefix) == 0) {
+      dev_free(it->second);
+      it = ctx->temps.erase(it);
+    } else {
+      ++it;
+    }
+  }
+  return NQ_OK;
+}
+
+// Fingerprint of a host conf: 64-bit words of a sample mixed with the splitmix64
+// finaliser.  The sample is the first and last 72 doubles (one quad_su3 site) and about
+// 65536 doubles spread with a stride coprime to 72, so that every one of the 72 reals
+// of a site (4 directions x 3 x 3 complex) is sampled about 900 times: an in-place
+// change of every link of one direction (boundary phases, staggered phases, a new
+// conf read into the same buffer) changes it.  A change confined to a few links can
+// escape the sample.
+uint64_t conf_fingerprint(const double* p, int64_t n)
+{
+  uint64_t h = 0x9e3779b97f4a7c15ull ^ (uint64_t)n;
+  auto mix = [&h](double d) {
+    uint64_t w;
+    std::memcpy(&w, &d, 8);
+    uint64_t z = h ^ w;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    h = z ^ (z >> 31);
+  };
+  int64_t stride = std::max<int64_t>(1, n / 65536) | 1;
+  while (stride % 3 == 0) stride += 2;
+  for (int64_t i = 0; i < std::min<int64_t>(n, 72); i++) mix(p[i]);
+  for (int64_t i = 0; i < n; i += stride) mix(p[i]);
+  for (int64_t i = std::max<int64_t>(0, n - 72); i < n; i++) mix(p[i]);
+  return h;
 }
 
 int spinor_bytes(int prec, int ncomp, const Geo& g, size_t* data_bytes, size_t* norm_bytes)
@@ -473,6 +516,8 @@ int nq_gauge_create_eo(nq_ctx* c, const double* ce, const double* co, nq_gauge**
     nq_gauge_destroy(g);
     return rc;
   }
+  g->fp_eo[0] = conf_fingerprint(ce, 72 * Vh);
+  g->fp_eo[1] = conf_fingerprint(co, 72 * Vh);
   *out = g;
   return NQ_OK;
 }
@@ -483,7 +528,33 @@ int nq_gauge_create_lx(nq_ctx* c, const double* lx, nq_gauge** out)
   const Geo& g = c->geo;
   std::vector<double> e(72 * g.Vh), o(72 * g.Vh);
   NQ_TRY(nq_host_split_lx(g.T, g.LX, g.LY, g.LZ, 72, lx, e.data(), o.data()));
-  return nq_gauge_create_eo(c, e.data(), o.data(), out);
+  NQ_TRY(nq_gauge_create_eo(c, e.data(), o.data(), out));
+  (*out)->fp_lx = conf_fingerprint(lx, 144 * g.Vh);
+  (*out)->from_lx = true;
+  return NQ_OK;
+}
+
+int nq_host_conf_fingerprint(const double* data, int64_t ndoubles, uint64_t* out)
+{
+  if (!data || !out || ndoubles < 0) return fail(NQ_EINVAL, "bad argument");
+  *out = conf_fingerprint(data, ndoubles);
+  return NQ_OK;
+}
+
+int nq_gauge_matches_lx(const nq_gauge* g, const double* lx, int* same)
+{
+  if (!g || !lx || !same) return fail(NQ_EINVAL, "null argument");
+  if (!g->from_lx) return fail(NQ_EINVAL, "gauge handle was made from eo buffers");
+  *same = conf_fingerprint(lx, 144 * g->ctx->geo.Vh) == g->fp_lx;
+  return NQ_OK;
+}
+
+int nq_gauge_matches_eo(const nq_gauge* g, const double* ce, const double* co, int* same)
+{
+  if (!g || !ce || !co || !same) return fail(NQ_EINVAL, "null argument");
+  const int64_t n = 72 * g->ctx->geo.Vh;
+  *same = conf_fingerprint(ce, n) == g->fp_eo[0] && conf_fingerprint(co, n) == g->fp_eo[1];
+  return NQ_OK;
 }
 
 void nq_gauge_destroy(nq_gauge* g)

@@ -70,24 +70,41 @@ static int read_sites(const char* path, const char* record, bool gauge, int nrea
   if (!path || !out_lx || T < 1 || LX < 1 || LY < 1 || LZ < 1) return fail(NQ_EINVAL, "LIME reader: bad argument");
   std::ifstream f(path, std::ios::binary);
   if (!f) return fail(NQ_EINVAL, "cannot open %s", path);
-  const std::vector<unsigned char> d((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+  f.seekg(0, std::ios::end);
+  const uint64_t fsize = (uint64_t)f.tellg();
+  f.seekg(0, std::ios::beg);
+  // walk the record headers and read only the payloads needed: the data record
+  // (one allocation of its exact size), the format and the checksum after it
+  std::vector<unsigned char> payload;
   const unsigned char* data = nullptr;
   uint64_t ndata = 0;
   std::string format, checksum;
-  size_t pos = 0;
-  while (pos + 144 <= d.size()) {
-    if (be(&d[pos], 4) != 0x456789abu) return fail(NQ_EINVAL, "%s: bad LIME magic at byte %zu", path, pos);
-    const uint64_t len = be(&d[pos + 8], 8);
-    const std::string type(reinterpret_cast<const char*>(&d[pos + 16]), strnlen(reinterpret_cast<const char*>(&d[pos + 16]), 128));
-    const size_t start = pos + 144;
-    if (len > d.size() - start) return fail(NQ_EINVAL, "%s: truncated record %s", path, type.c_str());
-    if (type == record && !data) {
-      data = &d[start];
-      ndata = len;
-    } else if (gauge && type == "ildg-format") {
-      format.assign(reinterpret_cast<const char*>(&d[start]), len);
-    } else if (type == "scidac-checksum" && data && checksum.empty()) {
-      checksum.assign(reinterpret_cast<const char*>(&d[start]), len);  // the one after the data record
+  uint64_t pos = 0;
+  unsigned char hdr[144];
+  while (pos + 144 <= fsize) {
+    f.seekg((std::streamoff)pos);
+    if (!f.read(reinterpret_cast<char*>(hdr), 144)) return fail(NQ_EINVAL, "%s: read error at byte %llu", path, (unsigned long long)pos);
+    if (be(hdr, 4) != 0x456789abu) return fail(NQ_EINVAL, "%s: bad LIME magic at byte %llu", path, (unsigned long long)pos);
+    const uint64_t len = be(hdr + 8, 8);
+    const std::string type(reinterpret_cast<const char*>(hdr + 16), strnlen(reinterpret_cast<const char*>(hdr + 16), 128));
+    const uint64_t start = pos + 144;
+    if (len > fsize - start) return fail(NQ_EINVAL, "%s: truncated record %s", path, type.c_str());
+    const bool want_data = type == record && !data;
+    const bool want_fmt = !want_data && gauge && type == "ildg-format";
+    const bool want_chk = !want_data && type == "scidac-checksum" && data && checksum.empty();  // the one after the data
+    if (want_data || want_fmt || want_chk) {
+      std::vector<unsigned char> buf(len);
+      if (len && !f.read(reinterpret_cast<char*>(buf.data()), (std::streamsize)len))
+        return fail(NQ_EINVAL, "%s: read error in record %s", path, type.c_str());
+      if (want_data) {
+        payload.swap(buf);
+        data = payload.data();
+        ndata = len;
+      } else if (want_fmt) {
+        format.assign(buf.begin(), buf.end());
+      } else {
+        checksum.assign(buf.begin(), buf.end());
+      }
     }
     pos = start + ((len + 7) / 8) * 8;
   }

@@ -225,6 +225,10 @@ struct nq_gauge {
   nq::DevBuf l64c;                   // LK_F64_12
   bool compressible = false;         // per-direction constant det
   double det[4][2] = {};             // det(U_mu) when compressible
+  // sampled fingerprints of the host conf the handle was made from (nq_gauge_matches_*)
+  uint64_t fp_eo[2] = {0, 0};
+  uint64_t fp_lx = 0;
+  bool from_lx = false;
   int64_t bytes() const;
 };
 
@@ -234,6 +238,9 @@ namespace nq {
 int dev_alloc(DevBuf& b, size_t bytes);
 void dev_free(DevBuf& b);
 int ctx_temp(nq_ctx* ctx, const std::string& role, size_t bytes, void** out);
+int ctx_temp_release(nq_ctx* ctx, const std::string& prefix);
+// sampled fingerprint of a host buffer of doubles (nq_host_conf_fingerprint)
+uint64_t conf_fingerprint(const double* p, int64_t n);
 int ensure_state(nq_ctx* ctx, int n);
 
 // views

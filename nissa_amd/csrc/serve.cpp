@@ -63,6 +63,8 @@ struct Group {
   nq_field* sol[TmBatch::kMaxSlots] = {};
   int64_t bytes = 0;
   bool ready = false;
+  bool draining = false;  // takes no new sources: it is to be replaced by a group of
+                          // another kappa / precision once its running sources finish
   ~Group()
   {
     for (int i = 0; i < TmBatch::kMaxSlots; i++) {
@@ -82,6 +84,13 @@ struct Group {
       if (slot[i].req) return true;
     return false;
   }
+  int nbusy() const
+  {
+    int n = 0;
+    for (int i = 0; i < TmBatch::kMaxSlots; i++) n += slot[i].req != nullptr;
+    return n;
+  }
+  bool fits(const Req& q) const { return !draining && kappa == q.r.kappa && prec == q.r.prec && has_free(); }
 };
 
 struct Engine {
@@ -90,7 +99,10 @@ struct Engine {
   nq_ctx* ctx = nullptr;
   nq_gauge* g = nullptr;
   std::deque<RhsRef> queue;
-  std::vector<std::unique_ptr<Group>> groups;
+  // batch groups by index: group i owns the solver states [16 i, 16 i + 16) of the
+  // context and the temporaries named "tb<i>.", so a group is replaced in place
+  std::unique_ptr<Group> groups[TmBatch::kMaxGroups];
+  int64_t gauge_bytes = 0;
   std::thread th;
 };
 
@@ -106,7 +118,8 @@ struct nq_server {
   std::vector<std::unique_ptr<std::condition_variable>> cv_work;
   std::map<int64_t, std::shared_ptr<Req>> reqs;
   int64_t next_id = 1;
-  int64_t used = 0;
+  int64_t used = 0;         // resident gauges + admitted groups
+  int64_t gauges = 0;       // resident gauges alone
   bool stop = false;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
@@ -153,73 +166,124 @@ void finish_source(nq_server* s, Req& q, int k, int rc, const nq_solve_stats& st
   }
 }
 
+// Admission of the queued sources of one engine (called under s->mu).  Every queued
+// source is tried in order; one that cannot start now stays queued and the scan goes
+// on past it.  A source goes into a free slot of a group with its kappa and
+// precision; otherwise into a new group at a free index when the group's footprint
+// fits the budget; otherwise, when every index is taken, one group is marked
+// draining: it takes no new sources, and once its running sources have finished it
+// is released and replaced by a group for the waiting source, so a request of a new
+// kappa or precision cannot wait forever behind a stream of compatible ones.  A
+// source whose group can never fit (resident gauges plus one group above the
+// budget) is refused with NQ_EBUSY.
+struct Admission {
+  std::vector<std::pair<Group*, int>> starts;
+  std::vector<int> fresh;                        // group indices to set up
+  std::vector<std::unique_ptr<Group>> released;  // idle groups to free (outside the lock)
+};
+
+void admit(nq_server* s, Engine* E, Admission* A)
+{
+  nq_ctx* c = E->ctx;
+  const int ng = std::min(s->max_groups, TmBatch::kMaxGroups);
+  bool drain_marked = false;
+  for (auto it = E->queue.begin(); it != E->queue.end();) {
+    const Req& q = *it->req;
+    Group* sel = nullptr;
+    for (int gi = 0; gi < ng && !sel; gi++)
+      if (E->groups[gi] && E->groups[gi]->fits(q)) sel = E->groups[gi].get();
+    const int64_t need = group_bytes(c, q.r.prec);
+    if (!sel && s->gauges + need > s->budget) {
+      // no group of this precision fits even with every other group released
+      RhsRef ref = *it;
+      it = E->queue.erase(it);
+      if (ref.req->t_admit < 0) ref.req->t_admit = s->now();
+      finish_source(s, *ref.req, ref.k, NQ_EBUSY, nq_solve_stats{});
+      if (ref.req->done) s->cv_done.notify_all();
+      continue;
+    }
+    if (!sel) {
+      // a group of this kappa and precision that is merely full: wait for its slots
+      // (or take a free index), never displace another group for it
+      bool compat = false;
+      for (int i = 0; i < ng; i++)
+        compat |= E->groups[i] && !E->groups[i]->draining && E->groups[i]->kappa == q.r.kappa &&
+                  E->groups[i]->prec == q.r.prec;
+      int gi = -1;
+      for (int i = 0; i < ng && gi < 0; i++)
+        if (!E->groups[i]) gi = i;
+      // otherwise an idle group (draining or not) gives way
+      for (int i = 0; i < ng && gi < 0 && !compat; i++)
+        if (E->groups[i] && !E->groups[i]->busy()) gi = i;
+      const int64_t freed = gi >= 0 && E->groups[gi] ? E->groups[gi]->bytes : 0;
+      if (gi >= 0 && s->used - freed + need <= s->budget) {
+        if (E->groups[gi]) {
+          s->used -= freed;
+          A->released.push_back(std::move(E->groups[gi]));
+        }
+        auto gp = std::make_unique<Group>();
+        gp->kappa = q.r.kappa;
+        gp->prec = q.r.prec;
+        gp->bytes = need;
+        s->used += need;
+        sel = gp.get();
+        E->groups[gi] = std::move(gp);
+        A->fresh.push_back(gi);
+      } else if (gi < 0 && !compat && !drain_marked) {
+        // every index holds a busy group of other parameters: drain the one with the
+        // fewest sources (unless one drains already) so that it can be replaced
+        bool any = false;
+        for (int i = 0; i < ng; i++) any |= E->groups[i] && E->groups[i]->draining;
+        if (!any) {
+          int best = -1;
+          for (int i = 0; i < ng; i++)
+            if (E->groups[i] && (best < 0 || E->groups[i]->nbusy() < E->groups[best]->nbusy())) best = i;
+          if (best >= 0) E->groups[best]->draining = true;
+        }
+        drain_marked = true;
+      }
+    }
+    if (!sel) {
+      ++it;  // waits for a slot, for a group to drain or for memory
+      continue;
+    }
+    int i = 0;
+    while (sel->slot[i].req) i++;
+    sel->slot[i] = *it;
+    sel->pending[i] = true;
+    if (it->req->t_admit < 0) it->req->t_admit = s->now();
+    A->starts.push_back({sel, i});
+    it = E->queue.erase(it);
+  }
+}
+
 void engine_loop(nq_server* s, Engine* E)
 {
   (void)hipSetDevice(s->device);
   nq_ctx* c = E->ctx;
   std::condition_variable& cv = *s->cv_work[E->id];
   for (;;) {
-    std::vector<std::pair<Group*, int>> starts;
-    std::vector<Group*> fresh;
+    Admission A;
     {
       std::unique_lock<std::mutex> lk(s->mu);
       if (s->stop) return;
-      // admission: queued sources into free slots of compatible groups
-      while (!E->queue.empty()) {
-        RhsRef& q = E->queue.front();
-        Group* sel = nullptr;
-        for (auto& gp : E->groups)
-          if (gp->kappa == q.req->r.kappa && gp->prec == q.req->r.prec && gp->has_free()) {
-            sel = gp.get();
-            break;
-          }
-        if (!sel && (int)E->groups.size() < std::min(s->max_groups, TmBatch::kMaxGroups)) {
-          const int64_t need = group_bytes(c, q.req->r.prec);
-          if (s->used + need <= s->budget) {
-            auto gp = std::make_unique<Group>();
-            gp->kappa = q.req->r.kappa;
-            gp->prec = q.req->r.prec;
-            gp->bytes = need;
-            s->used += need;
-            sel = gp.get();
-            fresh.push_back(sel);
-            E->groups.push_back(std::move(gp));
-          }
-        }
-        if (!sel) break;  // wait for a slot or for memory
-        int i = 0;
-        while (sel->slot[i].req) i++;
-        sel->slot[i] = q;
-        sel->pending[i] = true;
-        if (q.req->t_admit < 0) q.req->t_admit = s->now();
-        starts.push_back({sel, i});
-        E->queue.pop_front();
-      }
-      bool active = !starts.empty();
-      for (auto& gp : E->groups) active |= gp->busy();
+      admit(s, E, &A);
+      bool active = !A.starts.empty() || !A.released.empty();
+      for (auto& gp : E->groups) active |= gp && gp->busy();
       if (!active) {
-        if (E->queue.empty()) {
-          cv.wait(lk, [&] { return s->stop || !E->queue.empty(); });
-        } else if (E->groups.empty()) {
-          // not even one batch group fits the HBM budget: refuse the queued sources
-          while (!E->queue.empty()) {
-            RhsRef q = E->queue.front();
-            E->queue.pop_front();
-            if (q.req->t_admit < 0) q.req->t_admit = s->now();
-            finish_source(s, *q.req, q.k, NQ_EBUSY, nq_solve_stats{});
-            if (q.req->done) s->cv_done.notify_all();
-          }
-        } else {
-          // incompatible work waits for a group of its own kappa/precision
-          cv.wait_for(lk, std::chrono::milliseconds(5));
-        }
+        if (E->queue.empty()) cv.wait(lk, [&] { return s->stop || !E->queue.empty(); });
+        else cv.wait_for(lk, std::chrono::milliseconds(5));  // memory held by other volumes
         continue;
       }
     }
-    // new groups: device set-up (group index = position in the engine)
-    for (Group* gp : fresh) {
-      int gi = 0;
-      while (E->groups[gi].get() != gp) gi++;
+    // released groups go first (batches, graphs, fields); the temporaries of an index
+    // that gets a new group are freed too, since a group of another precision names
+    // some of them differently and would leave the old ones allocated
+    A.released.clear();
+    for (int gi : A.fresh) (void)ctx_temp_release(c, "tb" + std::to_string(gi) + ".");
+    // new groups: device set-up (group index = its slot in the engine)
+    for (int gi : A.fresh) {
+      Group* gp = E->groups[gi].get();
       int rc = gp->b.setup(c, E->g, gp->kappa, nullptr, gp->prec, gi);
       for (int i = 0; rc == NQ_OK && i < TmBatch::kMaxSlots; i++) {
         rc = nq_field_create(c, NQ_SPINCOLOR, NQ_LX, &gp->src[i]);
@@ -228,7 +292,7 @@ void engine_loop(nq_server* s, Engine* E)
       gp->ready = rc == NQ_OK;
     }
     // start the admitted sources
-    for (auto& [gp, i] : starts) {
+    for (auto& [gp, i] : A.starts) {
       RhsRef ref = gp->slot[i];
       Req& q = *ref.req;
       int rc = gp->ready ? NQ_OK : NQ_EOOM;
@@ -246,13 +310,14 @@ void engine_loop(nq_server* s, Engine* E)
     }
     // iterate every group that has work, then poll and retire finished sources
     for (auto& gp : E->groups)
-      if (gp->ready && gp->b.nactive() > 0) (void)gp->b.run(std::min(gp->b.suggested_chunk(), 32));
+      if (gp && gp->ready && gp->b.nactive() > 0) (void)gp->b.run(std::min(gp->b.suggested_chunk(), 32));
     for (auto& gp : E->groups) {
-      if (!gp->ready || gp->b.nactive() == 0) continue;
+      if (!gp || !gp->ready || gp->b.nactive() == 0) continue;
       std::vector<int> fin;
       int prc = gp->b.poll(&fin);
       if (prc != NQ_OK) {
         // device error: fail every source of the group
+        fin.clear();
         for (int i = 0; i < TmBatch::kMaxSlots; i++)
           if (gp->slot[i].req && gp->b.active(i)) fin.push_back(i);
       }
@@ -313,7 +378,9 @@ int nq_server_add_volume(nq_server* s, int T, int LX, int LY, int LZ, const doub
     return rc;
   }
   std::lock_guard<std::mutex> lk(s->mu);
-  s->used += nq_gauge_device_bytes(E->g);
+  E->gauge_bytes = nq_gauge_device_bytes(E->g);
+  s->used += E->gauge_bytes;
+  s->gauges += E->gauge_bytes;
   E->id = (int)s->engines.size();
   *volume_id = E->id;
   s->cv_work.push_back(std::make_unique<std::condition_variable>());
@@ -344,10 +411,13 @@ int nq_server_submit(nq_server* s, const nq_serve_request* r, int64_t* id)
   q->remaining = r->nsrc;
   std::lock_guard<std::mutex> lk(s->mu);
   if (r->volume < 0 || r->volume >= (int)s->engines.size()) return fail(NQ_EINVAL, "unknown volume %d", r->volume);
+  Engine* E = s->engines[r->volume].get();
+  const int64_t V = (int64_t)E->dims[0] * E->dims[1] * E->dims[2] * E->dims[3];
+  if (r->src_kind == 1 && (r->site0 < 0 || r->site0 >= V))
+    return fail(NQ_EINVAL, "point source site %lld outside the volume (%lld sites)", (long long)r->site0, (long long)V);
   q->id = s->next_id++;
   q->t_submit = s->now();
   s->reqs[q->id] = q;
-  Engine* E = s->engines[r->volume].get();
   for (int k = 0; k < r->nsrc; k++) E->queue.push_back(RhsRef{q, k});
   s->cv_work[r->volume]->notify_one();
   if (id) *id = q->id;
@@ -391,7 +461,7 @@ void nq_server_destroy(nq_server* s)
   for (auto& E : s->engines)
     if (E->th.joinable()) E->th.join();
   for (auto& E : s->engines) {
-    E->groups.clear();
+    for (auto& gp : E->groups) gp.reset();
     nq_gauge_destroy(E->g);
     nq_ctx_destroy(E->ctx);
   }

@@ -590,7 +590,8 @@ static SpinorView slot_view(const SpinorView& v, int slot)
   return o;
 }
 
-int TmBatch::start(int slot, double mu, double residue, int niter, const SpinorView& src_e, const SpinorView& src_o)
+int TmBatch::start(int slot, double mu, double residue, int niter, const SpinorView& src_e, const SpinorView& src_o,
+                   const SpinorView* guess)
 {
   if (slot < 0 || slot >= kMaxSlots) return fail(NQ_EINVAL, "slot %d out of range", slot);
   if (active_[slot]) return fail(NQ_EINVAL, "slot %d is busy", slot);
@@ -623,7 +624,14 @@ int TmBatch::start(int slot, double mu, double residue, int niter, const SpinorV
   NQ_HIP(hipMemcpyAsync(S_ + slot, c->state_host + 16 * group_ + slot, sizeof(CgState), hipMemcpyHostToDevice,
                         c->stream));
   NQ_TRY(launch_lincomb(c, slot_view(r64_, slot), 1.0, bs, 0, none, 0, none, FIN_INIT_B2, S_ + slot, 1));
-  NQ_TRY(launch_zero(c, slot_view(x64_, slot)));
+  if (guess) {
+    // x = guess, r = b - K K^+ x (|r|^2 and the stop test of the start, FIN_RR_INIT)
+    NQ_TRY(launch_convert(c, slot_view(x64_, slot), *guess));
+    NQ_TRY(residual_kernsq(S, mu, slot_view(r64_, slot), none, slot_view(x64_, slot), bs, slot_view(t64_, slot), et_,
+                           S_ + slot, FIN_RR_INIT, 0));
+  } else {
+    NQ_TRY(launch_zero(c, slot_view(x64_, slot)));
+  }
   if (mixed_) {
     NQ_TRY(launch_convert(c, slot_view(rs_, slot), slot_view(r64_, slot)));
     NQ_TRY(launch_zero(c, slot_view(xs_, slot)));
@@ -768,7 +776,7 @@ int TmBatch::poll(std::vector<int>* finished)
 }
 
 int TmBatch::finish(int slot, const SpinorView& src_e, const SpinorView& src_o, const SpinorView& sol_e,
-                    const SpinorView& sol_o, nq_solve_stats* st, double* check)
+                    const SpinorView& sol_o, nq_solve_stats* st, double* check, const SpinorView* guess_out)
 {
   if (slot < 0 || slot >= kMaxSlots || !active_[slot]) return fail(NQ_EINVAL, "slot %d is not active", slot);
   nq_ctx* c = ctx_;
@@ -792,6 +800,7 @@ int TmBatch::finish(int slot, const SpinorView& src_e, const SpinorView& src_o, 
   NQ_TRY(tm_setup(c, g_, lk64_for(prec_, g_), has_anis_ ? &anis_ : nullptr, kappa_, &S));
   const double a = 1 / (2 * kappa_), n = a * a + mu * mu;
   const cplx zi_u(a / n, -mu / n), zi_l(a / n, mu / n);
+  if (guess_out) NQ_TRY(launch_convert(c, *guess_out, slot_view(x64_, slot)));
   // eq. 10: sol_o = K(-mu) y ; eq. 11: sol_e = D_ee^-1 (eta_e + 1/2 H_eo sol_o)
   NQ_TRY(apply_kern(S, -mu, sol_o, slot_view(x64_, slot), sol_e));
   HopEpi e11;
@@ -1190,14 +1199,17 @@ int nq_tm_cg_kernsq(nq_ctx* c, const nq_gauge* g, double kappa, const double* an
                        view_of(src, 0), st);
 }
 
-// inv_tmD_cg_eoprec_native + check (cg_invert_tmD_eoprec.cpp:102-150,192-212)
+// inv_tmD_cg_eoprec_native + check (cg_invert_tmD_eoprec.cpp:102-150,192-212).
+// guess_Koo (odd, optional): the start of the eq. 9 CG, overwritten with its solution
+// ("if a guess was passed, return new one", :137)
 int nq_tm_solve_eoprec(nq_ctx* c, const nq_gauge* g, double kappa, const double* anis, double mu, int niter,
-                       double residue, int prec, nq_field* sol, const nq_field* src, nq_solve_stats* st,
-                       double* check)
+                       double residue, int prec, nq_field* sol, nq_field* guess_Koo, const nq_field* src,
+                       nq_solve_stats* st, double* check)
 {
   if (!c) return fail(NQ_EINVAL, "null ctx");
   NQ_TRY(check_field(c, src, NQ_SPINCOLOR, NQ_LX, "nq_tm_solve_eoprec"));
   NQ_TRY(check_field(c, sol, NQ_SPINCOLOR, NQ_LX, "nq_tm_solve_eoprec"));
+  if (guess_Koo) NQ_TRY(check_field(c, guess_Koo, NQ_SPINCOLOR, NQ_ODD, "nq_tm_solve_eoprec (guess_Koo)"));
   if (sol == src) return fail(NQ_EINVAL, "sol==src");
   Guard G(c);
   TmSetup S;
@@ -1225,8 +1237,11 @@ int nq_tm_solve_eoprec(nq_ctx* c, const nq_gauge* g, double kappa, const double*
   nq_solve_stats local;
   if (!st) st = &local;
   // the CG's FP64 residuals compute K(-mu) y into sol_o: after convergence that is eq. 10
-  int rc = cg_kernsq_any(c, g, kappa, anis, mu, niter, residue, prec, y, nullptr, phi, st, &xo);
+  SpinorView gv;
+  if (guess_Koo) gv = view_of(guess_Koo, 0);
+  int rc = cg_kernsq_any(c, g, kappa, anis, mu, niter, residue, prec, y, guess_Koo ? &gv : nullptr, phi, st, &xo);
   if (rc != NQ_OK && rc != NQ_ENOCONV) return rc;
+  if (guess_Koo) NQ_TRY(launch_convert(c, gv, y));
   // eq. 10: sol_o = K(-mu) y  (sol_e used as scratch), unless the CG left it there
   if (rc != NQ_OK) NQ_TRY(apply_kern(S, -mu, xo, y, xe));
   // eq. 11: sol_e = D_ee^-1 (eta_e + 1/2 H_eo sol_o)
@@ -1289,7 +1304,7 @@ int nq_tmQ2_cg(nq_ctx* c, const nq_gauge* g, double kappa, double mu, int niter,
 // K K^+ CG batched over up to 16 sources per group
 int nq_tm_solve_eoprec_multi(nq_ctx* c, const nq_gauge* g, double kappa, const double* anis, int nrhs,
                              const double* mus, int niter, double residue, int prec, nq_field** sols,
-                             const nq_field* const* srcs, nq_solve_stats* st)
+                             nq_field** guesses, const nq_field* const* srcs, nq_solve_stats* st)
 {
   if (!c || !mus || !sols || !srcs) return fail(NQ_EINVAL, "null argument");
   if (nrhs < 1) return fail(NQ_EINVAL, "need at least one source");
@@ -1297,6 +1312,8 @@ int nq_tm_solve_eoprec_multi(nq_ctx* c, const nq_gauge* g, double kappa, const d
   for (int i = 0; i < nrhs; i++) {
     NQ_TRY(check_field(c, srcs[i], NQ_SPINCOLOR, NQ_LX, "nq_tm_solve_eoprec_multi"));
     NQ_TRY(check_field(c, sols[i], NQ_SPINCOLOR, NQ_LX, "nq_tm_solve_eoprec_multi"));
+    if (guesses && guesses[i])
+      NQ_TRY(check_field(c, guesses[i], NQ_SPINCOLOR, NQ_ODD, "nq_tm_solve_eoprec_multi (guess_Koo)"));
     for (int j = 0; j < nrhs; j++)
       if (sols[i] == srcs[j]) return fail(NQ_EINVAL, "solution aliases a source");
   }
@@ -1310,7 +1327,11 @@ int nq_tm_solve_eoprec_multi(nq_ctx* c, const nq_gauge* g, double kappa, const d
   auto fill = [&]() -> int {
     for (int s = 0; s < TmBatch::kMaxSlots && next < nrhs; s++)
       if (!B.active(s)) {
-        NQ_TRY(B.start(s, mus[next], residue, niter, view_of(srcs[next], 0), view_of(srcs[next], 1)));
+        SpinorView gv;
+        const bool hg = guesses && guesses[next];
+        if (hg) gv = view_of(guesses[next], 0);
+        NQ_TRY(B.start(s, mus[next], residue, niter, view_of(srcs[next], 0), view_of(srcs[next], 1),
+                       hg ? &gv : nullptr));
         owner[s] = next++;
       }
     return NQ_OK;
@@ -1322,8 +1343,11 @@ int nq_tm_solve_eoprec_multi(nq_ctx* c, const nq_gauge* g, double kappa, const d
     NQ_TRY(B.poll(&fin));
     for (int s : fin) {
       const int i = owner[s];
+      SpinorView gv;
+      const bool hg = guesses && guesses[i];
+      if (hg) gv = view_of(guesses[i], 0);
       const int r = B.finish(s, view_of(srcs[i], 0), view_of(srcs[i], 1), view_of(sols[i], 0), view_of(sols[i], 1),
-                             &local[i], nullptr);
+                             &local[i], nullptr, hg ? &gv : nullptr);
       if (r == NQ_ENOCONV) rc = r;
       else if (r != NQ_OK) return r;
       owner[s] = -1;
@@ -1396,19 +1420,25 @@ int nq_tm_kern_square_host(nq_ctx* c, const nq_gauge* g, double kappa, const dou
 }
 
 int nq_tm_solve_eoprec_host(nq_ctx* c, const nq_gauge* g, double kappa, const double* anis, double mu, int niter,
-                            double residue, int prec, double* sol, const double* src, nq_solve_stats* st,
-                            double* check)
+                            double residue, int prec, double* sol, double* guess_Koo, const double* src,
+                            nq_solve_stats* st, double* check)
 {
   if (!c || !sol || !src) return fail(NQ_EINVAL, "null argument");
   Guard G(c);
-  nq_field *fs = nullptr, *fx = nullptr;
+  nq_field *fs = nullptr, *fx = nullptr, *fg = nullptr;
   NQ_TRY(nq_field_create(c, NQ_SPINCOLOR, NQ_LX, &fs));
   int rc = nq_field_create(c, NQ_SPINCOLOR, NQ_LX, &fx);
+  if (rc == NQ_OK && guess_Koo) rc = nq_field_create(c, NQ_SPINCOLOR, NQ_ODD, &fg);
   if (rc == NQ_OK) rc = nq_field_upload(fs, src);
-  int rs = rc == NQ_OK ? nq_tm_solve_eoprec(c, g, kappa, anis, mu, niter, residue, prec, fx, fs, st, check) : rc;
-  if (rs == NQ_OK || rs == NQ_ENOCONV) rc = nq_field_download(fx, sol);
+  if (rc == NQ_OK && fg) rc = nq_field_upload(fg, guess_Koo);
+  int rs = rc == NQ_OK ? nq_tm_solve_eoprec(c, g, kappa, anis, mu, niter, residue, prec, fx, fg, fs, st, check) : rc;
+  if (rs == NQ_OK || rs == NQ_ENOCONV) {
+    rc = nq_field_download(fx, sol);
+    if (rc == NQ_OK && fg) rc = nq_field_download(fg, guess_Koo);
+  }
   nq_field_destroy(fs);
   nq_field_destroy(fx);
+  nq_field_destroy(fg);
   return rc == NQ_OK ? rs : rc;
 }
 

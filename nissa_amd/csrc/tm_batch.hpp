@@ -32,17 +32,20 @@ class TmBatch {
   // prec: NQ_PREC_*; group selects the state range and the names of the temporaries
   int setup(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis, int prec, int group);
   // eq. 8 for the source (src_e, src_o FP64 views of one lx field) into `slot`, then
-  // the start of its CG (x = 0, r = b); `residue` is nissa's squared relative residue
-  int start(int slot, double mu, double residue, int niter, const SpinorView& src_e, const SpinorView& src_o);
+  // the start of its CG (x = guess or 0, r = b - K K^+ x); `residue` is nissa's squared
+  // relative residue
+  int start(int slot, double mu, double residue, int niter, const SpinorView& src_e, const SpinorView& src_o,
+            const SpinorView* guess = nullptr);
   // enqueue `iters` iterations over the active slots (no host synchronisation)
   int run(int iters);
   // wait for the stream, run the FP64 reliable updates that are due, and append the
   // slots that have finished (converged, out of iterations or NaN) to `finished`
   int poll(std::vector<int>* finished);
   // eqs. 10-11 into (sol_e, sol_o) and the FP64 check |D sol - src|^2/|src|^2 with the
-  // 18-real links; returns NQ_ENOCONV / NQ_ENAN for a slot that did not converge
+  // 18-real links (guess_out, optional: receives the eq. 9 solution); returns
+  // NQ_ENOCONV / NQ_ENAN for a slot that did not converge
   int finish(int slot, const SpinorView& src_e, const SpinorView& src_o, const SpinorView& sol_e,
-             const SpinorView& sol_o, nq_solve_stats* st, double* check);
+             const SpinorView& sol_o, nq_solve_stats* st, double* check, const SpinorView* guess_out = nullptr);
   int suggested_chunk() const { return chunk_; }
   bool active(int slot) const { return active_[slot]; }
   int nactive() const;

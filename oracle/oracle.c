@@ -31,6 +31,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+int or_omp_max_threads(void) { return omp_get_max_threads(); }
+
 struct or_geom {
   int L[4];
   int64_t V, Vh;

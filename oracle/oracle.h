@@ -33,6 +33,8 @@ extern "C" {
 typedef struct or_geom or_geom;
 
 /* geometry (geometry_lx.cpp:51-60,403-424; geometry_eo.cpp:41-113) */
+/* OpenMP threads the oracle's parallel loops use (omp_get_max_threads) */
+int or_omp_max_threads(void);
 or_geom* or_geom_create(int T, int LX, int LY, int LZ);
 void     or_geom_destroy(or_geom* g);
 int64_t  or_geom_vol(const or_geom* g);

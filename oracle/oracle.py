@@ -33,6 +33,7 @@ def lib():
             subprocess.run(["make", "-C", HERE], check=True, capture_output=True)
         L = C.CDLL(LIB_PATH)
         sig = {
+            "or_omp_max_threads": ([], _I),
             "or_geom_create": ([_I, _I, _I, _I], _P),
             "or_geom_destroy": ([_P], None),
             "or_geom_vol": ([_P], C.c_int64),
@@ -331,6 +332,16 @@ class Geom:
                              (C.c_double * n)(*residues), _dp(np.ascontiguousarray(src)), C.byref(st), tr)
         return sols, {"rc": rc, "iters": st.iters, "true_res_sq": list(tr)}
 
+    def st_rat_summ(self, ce, co, cons, poles, weights, niter, residue, src):
+        """chi = cons src + sum_k weights[k] (A + poles[k])^-1 src, A = -1/4 Deo Doe (massless)."""
+        n = len(poles)
+        chi = np.empty((self.Vh, 3, 2))
+        st = CgStats()
+        rc = lib().or_st_rat_summ(self.h, _dp(chi), _dp(ce), _dp(co), n, cons, (C.c_double * n)(*poles),
+                                  (C.c_double * n)(*weights), niter, residue, _dp(np.ascontiguousarray(src)),
+                                  C.byref(st))
+        return chi, {"rc": rc, "iters": st.iters}
+
     def plaquette(self, conf_lx):
         return lib().or_plaquette_lx(self.h, _dp(np.ascontiguousarray(conf_lx)))
 
@@ -343,6 +354,11 @@ class Geom:
         p = np.ascontiguousarray(psi_lx).copy()
         lib().or_color_rotate_lx(self.h, nspin, _dp(p), _dp(np.ascontiguousarray(omega)))
         return p
+
+
+def omp_max_threads():
+    """Threads the oracle's OpenMP loops run on."""
+    return lib().or_omp_max_threads()
 
 
 def scalar_prod(a, b, nd):

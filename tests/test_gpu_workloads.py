@@ -1,0 +1,172 @@
+"""GPU parity at the benchmark workloads' full sizes, through the C ABI.
+
+* 24^3x48, kappa 0.16, mu 0.005 (gauge seed 2): the FP64 eo solve of a Gaussian
+  source to nissa residue 1e-24 against the oracle's solve of the same system.
+* 32^3x64, kappa 0.16, mu 0.003 (gauge seed 3, Gaussian source 17: the bench's
+  workload): FP32-inner and FP16-inner (per-site norms, 12-real FP32 links) solves
+  with FP64 reliable updates to nissa residue 1e-20 (|r|/|b| = 1e-10), the FP64 true
+  residual recomputed on the host with the oracle's operator, and the solution
+  against the oracle's FP64 solve.
+* 32^3x64 staggered, m = 0.0025, 12 shifts m^2 + 10^(-4+5k/11), 12 Gaussian even
+  sources (gauge seed 4): the batched FP32-inner CGM with FP64 reliable updates; every
+  (source, shift) true residual <= 1e-20 in FP64 (the per-shift check of
+  cgm_invert_template_threaded.hpp:266-279), source 0's residuals recomputed on the
+  host with the oracle's operator, and two of its shifts against the oracle's CG.
+* Serving: a seeded Poisson stream over 16^3x32, 24^3x48 and 32^3x64 (50/30/20 mix,
+  masses 0.003-0.02, 12 point sources per request, FP16 inner, residue 1e-20); every
+  request's FP64 check, one request's solution against the oracle and bit-identical
+  to the same request served alone.
+"""
+import numpy as np
+import pytest
+
+import nissa_amd as nq
+from helpers import rel_l2, tm_conf, st_conf, G5
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+KAPPA = 0.16
+
+
+def g5_residual_sq(geo, conf, kappa, mu, sol, src):
+    """|D sol - src|^2 / |src|^2 with D = g5 Q, on the host (oracle operator)."""
+    q = geo.apply_tmQ(conf, kappa, mu, sol)
+    q[:, 2:] *= -1  # g5 = diag(1, 1, -1, -1) in nissa's basis
+    d = q - src
+    return float(np.vdot(d.ravel(), d.ravel()) / np.vdot(src.ravel(), src.ravel()))
+
+
+def test_fp64_solve_24c48():
+    L = (48, 24, 24, 24)
+    mu, residue = 0.005, 1e-24
+    conf = tm_conf(L, 2)
+    src = nq.source_lx(L, 18, "gauss")
+    ctx = nq.Ctx(L)
+    g = nq.Gauge(ctx, lx=conf)
+    fs = ctx.field(nq.SPINCOLOR, nq.LX, src)
+    fx = ctx.field(nq.SPINCOLOR, nq.LX)
+    st = nq.tm_solve_eoprec(ctx, g, KAPPA, mu, 100000, residue, fx, fs, prec="f64")
+    assert st["rc"] == nq.NQ_OK and st["check_rel_res_sq"] < 2 * residue, st
+    x = fx.download()
+    ctx.close()
+    geo = orc.Geom(L)
+    ref, rst = geo.tm_solve_eoprec(conf, KAPPA, mu, 100000, residue, src)
+    assert rst["rc"] == 0
+    assert abs(st["iters"] - rst["iters"]) <= 1, (st["iters"], rst["iters"])
+    assert rel_l2(x, ref) < 1e-10
+
+
+@pytest.fixture(scope="module")
+def lat32():
+    L = (64, 32, 32, 32)
+    conf = tm_conf(L, 3)
+    src = nq.source_lx(L, 17, "gauss")
+    return L, conf, src
+
+
+@pytest.fixture(scope="module")
+def oracle32(lat32):
+    L, conf, src = lat32
+    geo = orc.Geom(L)
+    ref, rst = geo.tm_solve_eoprec(conf, KAPPA, 0.003, 100000, 1e-20, src)
+    assert rst["rc"] == 0
+    return geo, ref
+
+
+@pytest.mark.parametrize("prec", ["f32", "f16"])
+def test_mixed_solve_32c64(lat32, oracle32, prec):
+    L, conf, src = lat32
+    geo, ref = oracle32
+    residue = 1e-20
+    ctx = nq.Ctx(L)
+    g = nq.Gauge(ctx, lx=conf)
+    fs = ctx.field(nq.SPINCOLOR, nq.LX, src)
+    fx = ctx.field(nq.SPINCOLOR, nq.LX)
+    st = nq.tm_solve_eoprec(ctx, g, KAPPA, 0.003, 100000, residue, fx, fs, prec=prec)
+    assert st["rc"] == nq.NQ_OK and st["converged"], st
+    assert st["check_rel_res_sq"] < residue and st["reliable_updates"] >= 4, st
+    x = fx.download()
+    ctx.close()
+    assert g5_residual_sq(geo, conf, KAPPA, 0.003, x, src) < residue
+    assert rel_l2(x, ref) < 1e-9
+
+
+def test_batched_staggered_cgm_32c64():
+    L = (64, 32, 32, 32)
+    conf = st_conf(L, 4)
+    m = 0.0025
+    shifts = [m * m + 10 ** (-4 + 5 * k / 11) for k in range(12)]
+    residue = 1e-20
+    ctx = nq.Ctx(L)
+    g = nq.Gauge(ctx, lx=conf)
+    Vh = ctx.Vh
+    rng = np.random.default_rng(44)
+    host = [rng.normal(size=(Vh, 3, 2)) * np.sqrt(0.5) for _ in range(12)]
+    srcs = [ctx.field(nq.COLOR, nq.EVEN, h) for h in host]
+    sols = [[ctx.field(nq.COLOR, nq.EVEN) for _ in shifts] for _ in srcs]
+    res = nq.st_cgm(ctx, g, shifts, 100000, [residue] * 12, sols, srcs, prec="f32")
+    for r in res:
+        assert r["rc"] == nq.NQ_OK, r
+        assert max(r["true_res_sq"]) <= residue, r["true_res_sq"]
+    x0 = [f.download() for f in sols[0]]
+    ctx.close()
+    geo = orc.Geom(L)
+    ce, co = geo.split(conf)
+    b2 = float(np.vdot(host[0].ravel(), host[0].ravel()))
+    for k, s in enumerate(shifts):
+        d = geo.stD2ee_m2(ce, co, s, x0[k]) - host[0]
+        assert float(np.vdot(d.ravel(), d.ravel())) / b2 < residue, k
+    for k in (8, 11):
+        ref, rst = geo.st_cg(ce, co, shifts[k], 100000, 1e-24, host[0])
+        assert rst["rc"] == 0
+        assert rel_l2(x0[k], ref) < 1e-8, k
+
+
+def test_serving_stream_three_volumes():
+    vols = [(32, 16, 16, 16), (48, 24, 24, 24), (64, 32, 32, 32)]
+    mix = [0.5, 0.3, 0.2]
+    mus = [0.003, 0.005, 0.01, 0.02]
+    residue = 1e-20
+    confs = [tm_conf(L, 5 + i) for i, L in enumerate(vols)]
+    srv = nq.Server(max_groups=1)
+    vid = [srv.add_volume(L, c) for L, c in zip(vols, confs)]
+    rng = np.random.default_rng(7)
+    n = 8
+    arrivals = np.cumsum(rng.exponential(1.0 / 20.0, size=n))
+    which = list(rng.choice(3, size=n, p=mix))
+    which[0] = 0  # the request checked below is on the smallest volume, and every volume is in the mix
+    which[1], which[2] = 1, 2
+    mu_i = rng.integers(0, 4, size=n)
+    V0 = int(np.prod(vols[0]))
+    site = int(rng.integers(0, V0))
+    mixed = [np.zeros((V0, 4, 3, 2)) for _ in range(12)]
+    import time
+
+    t0 = time.perf_counter()
+    ids = []
+    for i in range(n):
+        dt = arrivals[i] - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        w = which[i]
+        s = site if i == 0 else int(rng.integers(0, int(np.prod(vols[w]))))
+        ids.append(srv.submit(vid[w], KAPPA, mus[mu_i[i]], residue, prec="f16", point_site=s,
+                              sols=mixed if i == 0 else None))
+    for i in ids:
+        r = srv.wait(i, timeout=300.0)
+        assert r["status"] == nq.NQ_OK and r["max_rel_res_sq"] < residue, r
+        assert r["t_submit"] <= r["t_admit"] <= r["t_done"]
+    alone = [np.zeros((V0, 4, 3, 2)) for _ in range(12)]
+    r = srv.wait(srv.submit(vid[0], KAPPA, mus[mu_i[0]], residue, prec="f16", point_site=site, sols=alone))
+    assert r["status"] == nq.NQ_OK
+    srv.close()
+    for k in range(12):
+        assert np.array_equal(alone[k], mixed[k]), k
+    geo = orc.Geom(vols[0])
+    for sc in (0, 7):
+        src = nq.source_lx(vols[0], 0, "point", site0=site, sc=sc)
+        assert g5_residual_sq(geo, confs[0], KAPPA, mus[mu_i[0]], mixed[sc], src) < residue
+    src = nq.source_lx(vols[0], 0, "point", site0=site, sc=7)
+    ref, _ = geo.tm_solve_eoprec(confs[0], KAPPA, mus[mu_i[0]], 100000, 1e-24, src)
+    assert rel_l2(mixed[7], ref) < 1e-8

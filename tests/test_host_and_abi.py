@@ -126,3 +126,94 @@ def test_abi_structs_match_the_header(tmp_path):
         assert got[(cname, "size")] == C.sizeof(py), cname
         for f, _ in py._fields_:
             assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+
+
+def test_closing_a_context_closes_its_objects(monkeypatch):
+    """A context destroys its still-open gauge and field handles when it closes and marks
+    the Python objects closed: using one afterwards passes a null handle, which the
+    library refuses with NQ_EINVAL, and never the freed one.  The library is replaced by
+    a recorder of the calls (no device here)."""
+    import ctypes as C
+
+    calls, live = [], set()
+
+    class FakeLib:
+        def __init__(self):
+            self.next = 100
+
+        def _new(self, out):
+            self.next += 1
+            out._obj.value = self.next
+            live.add(self.next)
+            return nq.NQ_OK
+
+        def nq_ctx_create(self, T, X, Y, Z, dev, out):
+            return self._new(out)
+
+        def nq_field_create(self, ctx, kind, cov, out):
+            return self._new(out)
+
+        def nq_gauge_create_lx(self, ctx, conf, out):
+            return self._new(out)
+
+        def _destroy(self, name, h):
+            v = h.value if isinstance(h, C.c_void_p) else h
+            calls.append((name, v))
+            live.discard(v)
+
+        def nq_field_destroy(self, h):
+            self._destroy("field", h)
+
+        def nq_gauge_destroy(self, h):
+            self._destroy("gauge", h)
+
+        def nq_ctx_destroy(self, h):
+            self._destroy("ctx", h)
+
+        def nq_field_download(self, h, buf):
+            v = h.value if isinstance(h, C.c_void_p) else h
+            if not v:
+                return nq.NQ_EINVAL
+            assert v in live, "a freed handle reached the library"
+            return nq.NQ_OK
+
+        def nq_last_error(self):
+            return b"null argument"
+
+    monkeypatch.setattr(nq, "_lib", FakeLib())
+    ctx = nq.Ctx((4, 4, 4, 4))
+    g = nq.Gauge(ctx, lx=np.zeros((256, 4, 3, 3, 2)))
+    f = ctx.field(nq.SPINCOLOR, nq.ODD)
+    f.download()
+    ctx.close()
+    assert [c[0] for c in calls] == ["field", "gauge", "ctx"]
+    assert f.h is None and g.h is None
+    with pytest.raises(nq.NqError) as e:
+        f.download()
+    assert e.value.code == nq.NQ_EINVAL
+    # closing the objects afterwards destroys nothing twice
+    f.close()
+    g.close()
+    assert len(calls) == 3
+    with pytest.raises(nq.NqError):
+        nq.Gauge(ctx, lx=np.zeros((10, 4, 3, 3, 2)))
+
+
+def test_conf_fingerprint_sees_in_place_changes():
+    """The gauge cache's fingerprint (nq_host_conf_fingerprint): equal for equal
+    content at another address, different after each of nissa's in-place changes of
+    a conf (a boundary phase on any one direction, staggered phases, another conf
+    read into the same buffer)."""
+    L = (8, 4, 4, 6)
+    conf = nq.gauge_random_lx(L, 9, 0.3)
+    f0 = nq.conf_fingerprint(conf)
+    assert nq.conf_fingerprint(conf.copy()) == f0
+    for mu in range(4):
+        th = [0.0] * 4
+        th[mu] = 1.0
+        assert nq.conf_fingerprint(nq.gauge_phase_lx(L, conf.copy(), th)) != f0, mu
+    assert nq.conf_fingerprint(nq.gauge_stagphases_lx(L, conf.copy())) != f0
+    assert nq.conf_fingerprint(nq.gauge_random_lx(L, 10, 0.3)) != f0
+    # eo halves, as nq_gauge_create_eo takes them
+    e, o = nq.split_lx(L, conf)
+    assert nq.conf_fingerprint(e) != nq.conf_fingerprint(o)

@@ -364,6 +364,31 @@ def test_staggered_cgm_matches_single_shift_cg():
         assert tr < 2e-22
 
 
+def test_staggered_rational_sum_against_dense_partial_fractions():
+    """summ_src_and_all_inv_stD2ee_m2_cgm (cgm_invert_stD2ee_m2_portable.hpp:31-55):
+    chi = c0 eta + sum_k w_k (A + p_k)^-1 eta with A the massless -1/4 Deo Doe, here
+    against the same sum formed with dense solves on 4x4x2x2.  Distinct weights of both
+    signs and a non-zero c0, so a dropped pole, a flipped weight, a shift of the wrong
+    sign or c0 on the wrong vector each change the result; and the partial fractions
+    of 1/((x+a)(x+b)) give (A+b)(A+a) chi = eta."""
+    L = (4, 4, 2, 2)
+    g = orc.Geom(L)
+    ce, co = g.split(st_conf(L, 63))
+    se, _ = g.split(nq.source_lx(L, 12, "gauss", 3, 1))
+    A = _dense(lambda v: g.stD2ee_m2(ce, co, 0.0, v), g.Vh * 3, (g.Vh, 3, 2))
+    eta = cplx(se).ravel()
+    c0, poles, weights = 0.35, [0.004, 0.09, 1.7], [0.8, -0.25, 2.1]
+    chi, st = g.st_rat_summ(ce, co, c0, poles, weights, 20000, 1e-28, se)
+    assert st["rc"] == 0
+    ref = c0 * eta + sum(w * np.linalg.solve(A + p * np.eye(len(eta)), eta) for p, w in zip(poles, weights))
+    assert rel_l2(cplx(chi).ravel(), ref) < 1e-11
+    a, b = 0.01, 0.2
+    chi2, st2 = g.st_rat_summ(ce, co, 0.0, [a, b], [1 / (b - a), -1 / (b - a)], 20000, 1e-28, se)
+    assert st2["rc"] == 0
+    z = g.stD2ee_m2(ce, co, b, g.stD2ee_m2(ce, co, a, chi2))
+    assert rel_l2(z, se) < 1e-11
+
+
 def test_quad_precision_defect_correction():
     L = (4, 4, 4, 4)
     g = orc.Geom(L)
