@@ -63,10 +63,18 @@ typedef struct nq_solve_stats {
 /* ------------------------------------------------------------------ context */
 const char* nq_last_error(void);
 int nq_device_count(int* n);
-/* replaces nissa's initGrid(T,L) geometry set-up on one rank (src/base/init.cpp:563) */
+/* replaces nissa's initGrid(T,L) geometry set-up on one rank (src/base/init.cpp:563).
+   From 2^18 sites per parity up, the device site order (t tile, x/y patch) is chosen by
+   timing the hops on synthetic data at creation, or read from the tile cache that run
+   wrote (NQ_TUNE_CACHE, else $HOME/.cache/nissa_amd/site_tiles.txt), as nissa's kernel
+   tuner keeps its table on disk (src/threads/benchmarks.hpp:118-200,207-328).
+   NQ_TTILE / NQ_XYTILE fix the tiles, NQ_TUNE_TILES=0 keeps the defaults.  The order
+   changes global sums at the rounding level only. */
 int nq_ctx_create(int T, int LX, int LY, int LZ, int device, nq_ctx** out);
 void nq_ctx_destroy(nq_ctx* ctx);
 int nq_ctx_sync(nq_ctx* ctx);
+/* the context's site-order tiles: t tile, x patch, y patch */
+int nq_ctx_site_tiles(const nq_ctx* ctx, int* tb, int* tx, int* ty);
 /* per-kernel timing ledger (HIP events around every launch when enabled) */
 int nq_ctx_set_profiling(nq_ctx* ctx, int on);
 int nq_ctx_kernel_count(nq_ctx* ctx, int* n);

@@ -120,6 +120,7 @@ _SIG = {
     "nq_ctx_create": ([_I, _I, _I, _I, _I, C.POINTER(_P)], _I),
     "nq_ctx_destroy": ([_P], None),
     "nq_ctx_sync": ([_P], _I),
+    "nq_ctx_site_tiles": ([_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)], _I),
     "nq_ctx_set_profiling": ([_P, _I], _I),
     "nq_ctx_kernel_count": ([_P, C.POINTER(C.c_int)], _I),
     "nq_ctx_kernel_stat": ([_P, _I, C.c_char_p, _I, C.POINTER(C.c_int64), _DP, _DP], _I),
@@ -380,6 +381,12 @@ class Ctx:
 
     def sync(self):
         _check(lib().nq_ctx_sync(self.h))
+
+    def site_tiles(self):
+        """(t tile, x patch, y patch) of the device site order."""
+        tb, tx, ty = C.c_int(0), C.c_int(0), C.c_int(0)
+        _check(lib().nq_ctx_site_tiles(self.h, C.byref(tb), C.byref(tx), C.byref(ty)))
+        return tb.value, tx.value, ty.value
 
     def profiling(self, on=True):
         _check(lib().nq_ctx_set_profiling(self.h, 1 if on else 0))

@@ -200,6 +200,37 @@ void prof_flush(nq_ctx* ctx)
   ctx->pending.clear();
 }
 
+// Device site order of a T x LX x LY x LZ lattice: t tiles of TB slices (the largest
+// even divisor of T up to tbmax), x/y patches of TX x TY sites (powers of two up to
+// 16, halved until they divide LX and LY).
+Geo make_geo(int T, int LX, int LY, int LZ, int tbmax, int tx, int ty)
+{
+  Geo g{};
+  g.T = T; g.LX = LX; g.LY = LY; g.LZ = LZ;
+  g.Vh = (int64_t)T * LX * LY * LZ / 2;
+  g.stride = (g.Vh + 63) / 64 * 64;
+  int tb = 2;
+  for (int d = 2; d <= std::min(T, tbmax); d += 2)
+    if (T % d == 0) tb = d;
+  g.TB = tb;
+  g.NTH = T / g.TB;
+  auto pow2 = [](int v) { return v >= 1 && v <= 16 && (v & (v - 1)) == 0; };
+  if (!pow2(tx)) tx = 1;
+  if (!pow2(ty)) ty = 1;
+  while (LX % tx) tx /= 2;
+  while (LY % ty) ty /= 2;
+  auto lg = [](int v) { int l = 0; while ((1 << l) < v) l++; return l; };
+  g.TX = tx; g.logTX = lg(tx); g.NXT = LX / tx;
+  g.TY = ty; g.logTY = lg(ty); g.NYT = LY / ty;
+  g.Sz = g.TB;
+  g.Syi = (int64_t)g.TB * LZ;
+  g.Sxi = g.Syi * g.TY;
+  g.Syt = g.Sxi * g.TX;
+  g.Sxt = g.Syt * g.NYT;
+  g.Sth = g.Sxt * g.NXT;
+  return g;
+}
+
 }  // namespace nq
 
 using namespace nq;
@@ -253,43 +284,24 @@ int nq_ctx_create(int T, int LX, int LY, int LZ, int device, nq_ctx** out)
   auto* c = new nq_ctx;
   c->device = device;
   if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) c->ncu = 0;
-  Geo& g = c->geo;
-  g.T = T; g.LX = LX; g.LY = LY; g.LZ = LZ;
-  g.Vh = (int64_t)T * LX * LY * LZ / 2;
-  g.stride = (g.Vh + 63) / 64 * 64;
   // t tile of the device site order: the largest even divisor of T up to 64, or
   // up to NQ_TTILE when set.  Whole t rows in one tile keep every t neighbour in
   // the lines the wavefront already reads: a t tile boundary sends a quarter of
   // the lanes' t-neighbour loads to lines of a far tile (of another XCD), which
   // costs ~10% of the hop at 32^3x64 with tiles of 8.
   int tbmax = 64;
-  if (const char* e = getenv("NQ_TTILE")) tbmax = atoi(e);
+  const char* ett = getenv("NQ_TTILE");
+  if (ett) tbmax = atoi(ett);
   if (tbmax < 2) tbmax = 64;
-  int tb = 2;
-  for (int d = 2; d <= std::min(T, tbmax); d += 2)
-    if (T % d == 0) tb = d;
-  g.TB = tb;
-  g.NTH = T / g.TB;
-  // x/y patch of the device site order (tuning knob NQ_XYTILE "TXxTY", powers of two;
-  // default 4x2: the x and y neighbours inside a patch are a few thousand sites
-  // apart; measured at 32^3x64 with the persistent FP64 hop, 4x2 269 us, 4x1 271 us,
-  // 8x1 287 us, 2x1 280 us)
+  // x/y patch of the device site order (NQ_XYTILE "TXxTY", powers of two; default
+  // 4x2: the x and y neighbours inside a patch are a few thousand sites apart;
+  // measured at 32^3x64 with the persistent FP64 hop, 4x2 269 us, 4x1 271 us, 8x1
+  // 287 us, 2x1 280 us)
   int tx = 4, ty = 2;
-  if (const char* e = getenv("NQ_XYTILE")) sscanf(e, "%dx%d", &tx, &ty);
-  auto pow2 = [](int v) { return v >= 1 && v <= 16 && (v & (v - 1)) == 0; };
-  if (!pow2(tx)) tx = 1;
-  if (!pow2(ty)) ty = 1;
-  while (LX % tx) tx /= 2;
-  while (LY % ty) ty /= 2;
-  auto lg = [](int v) { int l = 0; while ((1 << l) < v) l++; return l; };
-  g.TX = tx; g.logTX = lg(tx); g.NXT = LX / tx;
-  g.TY = ty; g.logTY = lg(ty); g.NYT = LY / ty;
-  g.Sz = g.TB;
-  g.Syi = (int64_t)g.TB * LZ;
-  g.Sxi = g.Syi * g.TY;
-  g.Syt = g.Sxi * g.TX;
-  g.Sxt = g.Syt * g.NYT;
-  g.Sth = g.Sxt * g.NXT;
+  const char* exy = getenv("NQ_XYTILE");
+  if (exy) sscanf(exy, "%dx%d", &tx, &ty);
+  c->geo = make_geo(T, LX, LY, LZ, tbmax, tx, ty);
+  const Geo& g = c->geo;
   int rc = NQ_OK;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = fail(NQ_EDEVICE, "stream creation failed");
   const int64_t nchunks = (g.Vh + 63) / 64;
@@ -301,11 +313,23 @@ int nq_ctx_create(int T, int LX, int LY, int LZ, int device, nq_ctx** out)
     rc = fail(NQ_EOOM, "pinned state");
   if (rc == NQ_OK && hipMemset(c->state, 0, sizeof(CgState) * c->nstate) != hipSuccess) rc = fail(NQ_EDEVICE, "memset");
   if (rc == NQ_OK && hipDeviceSynchronize() != hipSuccess) rc = fail(NQ_EDEVICE, "sync");
+  // site-order tiles chosen by the autotuner (cached per device and lattice), unless
+  // the caller fixed them or the lattice is small enough for the defaults
+  if (rc == NQ_OK && !ett && !exy) rc = tune_site_tiles(c);
   if (rc != NQ_OK) {
     nq_ctx_destroy(c);
     return rc;
   }
   *out = c;
+  return NQ_OK;
+}
+
+int nq_ctx_site_tiles(const nq_ctx* c, int* tb, int* tx, int* ty)
+{
+  if (!c || !tb || !tx || !ty) return fail(NQ_EINVAL, "null argument");
+  *tb = c->geo.TB;
+  *tx = c->geo.TX;
+  *ty = c->geo.TY;
   return NQ_OK;
 }
 

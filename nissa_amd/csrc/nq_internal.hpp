@@ -328,6 +328,11 @@ int launch_clover_apply(nq_ctx* ctx, int mode, const SpinorView& out, const Spin
                         const double b[2][2]);
 int launch_aos_to_soa(nq_ctx* ctx, const double* aos, int ncomp, int par, int host_lx, const SpinorView& out);
 int launch_soa_to_aos(nq_ctx* ctx, const SpinorView& in, int ncomp, int par, int host_lx, double* aos);
+// device site order of a lattice (t tile up to tbmax, x/y patch tx x ty)
+Geo make_geo(int T, int LX, int LY, int LZ, int tbmax, int tx, int ty);
+// choose the site-order tiles of a fresh context (no fields yet) by timing the hops
+// on synthetic data, or take them from the tile cache; sets ctx->geo
+int tune_site_tiles(nq_ctx* ctx);
 int launch_build_links(nq_ctx* ctx, const double* aos_e, const double* aos_o, int lk, void* out,
                        const double det[4][2]);
 int launch_lx_split_gauge(nq_ctx* ctx, const double* lx, double* evn, double* odd);
