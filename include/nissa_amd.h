@@ -146,6 +146,15 @@ int nq_tm_kern(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis,
 /* tmDkern_eoprec_square_eos (..._portable.cpp:54-65) */
 int nq_tm_kern_square(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis, double mu,
                       nq_field* out_odd, const nq_field* in_odd);
+/* No counterpart in the reference: nq_tm_kern_square for callers that cannot allocate
+   (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+   Its two FP64 temporaries live in a static 128 MiB device array of the library, so it
+   calls no hipMalloc, hipFree or other allocating routine; lattices of more than 349504
+   sites per parity (24^3x48 fits, 32^3x64 does not) are refused with NQ_EINVAL.  Every
+   call on one GPU shares that array: calls that overlap on one GPU race.  Same kernels
+   and results as nq_tm_kern_square; the launch ledger does not record it. */
+int nq_tm_kern_square_noalloc(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis, double mu,
+                              nq_field* out_odd, const nq_field* in_odd);
 /* apply_tmQ on lx (dirac_operator_tmQ_portable.cpp:29-176) */
 int nq_tm_apply_Q(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis, double mu,
                   nq_field* out_lx, const nq_field* in_lx);

@@ -146,6 +146,7 @@ _SIG = {
     "nq_tm_Dee": ([_P, _D, _D, _I, _P, _P], _I),
     "nq_tm_kern": ([_P, _P, _D, _DP, _D, _P, _P], _I),
     "nq_tm_kern_square": ([_P, _P, _D, _DP, _D, _P, _P], _I),
+    "nq_tm_kern_square_noalloc": ([_P, _P, _D, _DP, _D, _P, _P], _I),
     "nq_tm_apply_Q": ([_P, _P, _D, _DP, _D, _P, _P], _I),
     "nq_tm_apply_Q2": ([_P, _P, _D, _D, _P, _P], _I),
     "nq_tm_cg_kernsq": ([_P, _P, _D, _DP, _D, _I, _D, _I, _P, _P, _P, C.POINTER(SolveStats)], _I),
@@ -542,6 +543,11 @@ def tm_kern(ctx, g, kappa, mu, out, inp, anis=None):
 
 def tm_kern_square(ctx, g, kappa, mu, out, inp, anis=None):
     _check(lib().nq_tm_kern_square(ctx.h, g.h, kappa, _anis(anis), mu, out.h, inp.h))
+
+
+def tm_kern_square_noalloc(ctx, g, kappa, mu, out, inp, anis=None):
+    """nq_tm_kern_square_noalloc: K K^+ with the library's static device scratch (no allocation)."""
+    _check(lib().nq_tm_kern_square_noalloc(ctx.h, g.h, kappa, _anis(anis), mu, out.h, inp.h))
 
 
 def tm_apply_Q(ctx, g, kappa, mu, out, inp, anis=None):

@@ -140,7 +140,7 @@ static int lk_of_prec(int prec, int* sp, int* lk)
   }
 }
 
-static CgState* scratch(nq_ctx* ctx) { return ctx->state + (ctx->nstate - 1); }
+static CgState* scratch_state(nq_ctx* ctx) { return ctx->state + (ctx->nstate - 1); }
 
 // Upload a host-initialised state to device slot i (async on the ctx stream).
 static int put_state(nq_ctx* ctx, int i, const CgState& s)
@@ -263,7 +263,7 @@ static int cg_kernsq_f64(const TmSetup& S, double mu, int niter, double residue,
   // last calculation of the residual (cg_invert_template_threaded.hpp:131-135)
   CgState sc = {};
   NQ_TRY(put_state(ctx, ctx->nstate - 1, sc));
-  NQ_TRY(residual_kernsq(S, mu, r, none, x, b, t1, tmp, scratch(ctx), FIN_STORE0, 0));
+  NQ_TRY(residual_kernsq(S, mu, r, none, x, b, t1, tmp, scratch_state(ctx), FIN_STORE0, 0));
   NQ_TRY(get_states(ctx, ctx->nstate - 1, 1));
   const double true_rr = ctx->state_host[ctx->nstate - 1].extra[0];
   out->iters = h.iter;
@@ -1076,6 +1076,40 @@ int check_field(const nq_ctx* c, const nq_field* f, int kind, int cov, const cha
 }
 }  // namespace
 
+// ---------------------------------------------------------------- allocation-free K K^+
+// Scratch of nq_tm_kern_square_noalloc: the even intermediate and the odd K(-mu) in of
+// one FP64 K K^+ apply (two spincolor fields of `stride` sites, 384 B per site), in a
+// device array that each GPU holds from module load to process end.  The host never
+// passes the array's name to a kernel (in host code it is a stand-in, not a device
+// address): the device address comes from hipGetSymbolAddress, once per device with
+// that device current, and is kept per device.  Every call on one GPU shares its copy,
+// so calls that overlap on one GPU race; nothing in it has to start at zero (both
+// fields are written in full before they are read).
+static constexpr size_t kNoAllocScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kMaxDevices] = {};
+
+// device address of this device's scratch (the caller has made `dev` current)
+int noalloc_scratch(int dev, char** out)
+{
+  if (dev < 0 || dev >= kMaxDevices) return fail(NQ_EINVAL, "device %d out of range for the scratch table", dev);
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(NQ_EDEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_noalloc_ptr[dev] = p;
+  }
+  *out = (char*)g_noalloc_ptr[dev];
+  return NQ_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int nq_tm_hop(nq_ctx* c, const nq_gauge* g, const double* anis, nq_field* out, const nq_field* in)
@@ -1141,6 +1175,41 @@ int nq_tm_kern_square(nq_ctx* c, const nq_gauge* g, double kappa, const double* 
   NQ_TRY(alloc_spinor(c, "op.t1", SP_F64, 12, &t1));
   NQ_TRY(apply_kern(S, -mu, t1, view_of(in, 0), tmp));
   NQ_TRY(apply_kern(S, +mu, view_of(out, 0), t1, tmp));
+  NQ_HIP(hipStreamSynchronize(c->stream));
+  return NQ_OK;
+}
+
+// nq_tm_kern_square with its two temporaries in the library's static device scratch:
+// no device or host-pinned allocation, and the launch ledger stays off for it.  The
+// gauge handle's FP64 18-real links are built when the handle is created.
+int nq_tm_kern_square_noalloc(nq_ctx* c, const nq_gauge* g, double kappa, const double* anis, double mu,
+                              nq_field* out, const nq_field* in)
+{
+  if (!c || !g) return fail(NQ_EINVAL, "null argument");
+  NQ_TRY(check_field(c, in, NQ_SPINCOLOR, NQ_ODD, "nq_tm_kern_square_noalloc"));
+  NQ_TRY(check_field(c, out, NQ_SPINCOLOR, NQ_ODD, "nq_tm_kern_square_noalloc"));
+  const size_t field = (size_t)12 * 16 * (size_t)c->geo.stride;
+  if (2 * field > kNoAllocScratchBytes)
+    return fail(NQ_EINVAL, "nq_tm_kern_square_noalloc: %lld sites per parity need %zu bytes of scratch, the limit is %zu "
+                           "(%lld sites per parity)",
+                (long long)c->geo.Vh, 2 * field, kNoAllocScratchBytes,
+                (long long)(kNoAllocScratchBytes / (2 * 12 * 16) / 64 * 64));
+  if (g->ctx != c) return fail(NQ_EINVAL, "nq_tm_kern_square_noalloc: gauge belongs to another context");
+  if (!g->l64.p) return fail(NQ_EINVAL, "nq_tm_kern_square_noalloc: gauge handle without FP64 links");
+  Guard G(c);
+  char* base = nullptr;
+  NQ_TRY(noalloc_scratch(c->device, &base));
+  TmSetup S;
+  NQ_TRY(tm_setup(c, g, LK_F64_18, anis, kappa, &S));
+  SpinorView tmp, t1;
+  tmp.data = base;
+  t1.data = base + field;
+  const bool prof = c->profiling;
+  c->profiling = false;
+  int rc = apply_kern(S, -mu, t1, view_of(in, 0), tmp);
+  if (rc == NQ_OK) rc = apply_kern(S, +mu, view_of(out, 0), t1, tmp);
+  c->profiling = prof;
+  NQ_TRY(rc);
   NQ_HIP(hipStreamSynchronize(c->stream));
   return NQ_OK;
 }
@@ -1267,7 +1336,7 @@ int nq_tm_solve_eoprec(nq_ctx* c, const nq_gauge* g, double kappa, const double*
       ec.aux2 = view_of(src, p);
       ec.mode = 1;
       ec.write = 0;
-      ec.st = scratch(c);
+      ec.st = scratch_state(c);
       ec.fin = FIN_STORE0;
       NQ_TRY(launch_tm_hop(c, p, none, view_of(sol, !p), Sc.U, Sc.anis, ec));
       NQ_TRY(get_states(c, c->nstate - 1, 1));
@@ -1693,7 +1762,7 @@ static int clov_cg(const ClovSetup& s, int niter, double residue, const SpinorVi
   NQ_TRY(clov_kern(s, 1, t1, x, HopEpi()));
   NQ_TRY(clov_kern(s, 0, sv, t1, HopEpi()));
   NQ_TRY(put_state(ctx, ctx->nstate - 1, CgState{}));
-  NQ_TRY(launch_lincomb(ctx, none, 1.0, b, -1.0, sv, 0, none, FIN_STORE0, scratch(ctx)));
+  NQ_TRY(launch_lincomb(ctx, none, 1.0, b, -1.0, sv, 0, none, FIN_STORE0, scratch_state(ctx)));
   NQ_TRY(get_states(ctx, ctx->nstate - 1, 1));
   const double true_rr = ctx->state_host[ctx->nstate - 1].extra[0];
   out->iters = h.iter;
@@ -2126,7 +2195,7 @@ int nq_tmclov_solve_eoprec(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double k
       ec.aux2 = view_of(src, p);
       ec.mode = 1;
       ec.write = 0;
-      ec.st = scratch(c);
+      ec.st = scratch_state(c);
       ec.fin = FIN_STORE0;
       NQ_TRY(launch_tm_hop(c, p, none, view_of(sol, !p), s.S.U, s.S.anis, ec));
       NQ_TRY(get_states(c, c->nstate - 1, 1));
