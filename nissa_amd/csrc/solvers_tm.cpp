@@ -142,6 +142,21 @@ static int lk_of_prec(int prec, int* sp, int* lk)
 
 static CgState* scratch_state(nq_ctx* ctx) { return ctx->state + (ctx->nstate - 1); }
 
+// Drop of |r| between two FP64 reliable updates of a mixed solve: the fewest updates
+// for which no drop goes below the smallest one the inner storage keeps reliable
+// (FP16 spinors: 3e-3, FP32: 3e-4), spread evenly over the requested reduction
+// sqrt(residue) of |r|/|b|.  Measured at 32^3x64 to residue 1e-20: FP16 storage with
+// drops of 1e-2 (5 updates) and 3.2e-3 (4) both take 64 iterations, 1e-3 takes 69 and
+// 3e-4 99; FP32 takes 64 iterations with drops down to 3e-4 (3 updates).
+static double reliable_delta(int sp, double residue)
+{
+  const double dmin = sp == SP_F16 ? 3e-3 : 3e-4;
+  const double red = std::sqrt(std::max(residue, 1e-300));
+  if (!(red < dmin)) return dmin;
+  const int n = (int)std::ceil(std::log(red) / std::log(dmin) - 1e-9);
+  return std::pow(red, 1.0 / n);
+}
+
 // Upload a host-initialised state to device slot i (async on the ctx stream).
 static int put_state(nq_ctx* ctx, int i, const CgState& s)
 {
@@ -401,7 +416,7 @@ static int cg_kernsq_any(nq_ctx* ctx, const nq_gauge* g, double kappa, const dou
   if (prec == NQ_PREC_F64) return cg_kernsq_f64(S64, mu, niter, residue, x, guess, b, st, kx);
   TmSetup Ss;
   NQ_TRY(tm_setup(ctx, g, lk, anis, kappa, &Ss));
-  return cg_kernsq_mixed(S64, Ss, sp, mu, niter, residue, 1e-2, x, guess, b, st, kx);
+  return cg_kernsq_mixed(S64, Ss, sp, mu, niter, residue, reliable_delta(sp, residue), x, guess, b, st, kx);
 }
 
 static int apply_Q_views(const TmSetup& S, double mu, const nq_field* out, const nq_field* in)
@@ -619,7 +634,7 @@ int TmBatch::start(int slot, double mu, double residue, int niter, const SpinorV
   s0.target = residue;
   s0.niter = niter;
   s0.mixed = mixed_;
-  s0.delta = 1e-2;
+  s0.delta = reliable_delta(sp_, residue);
   c->state_host[16 * group_ + slot] = s0;
   NQ_HIP(hipMemcpyAsync(S_ + slot, c->state_host + 16 * group_ + slot, sizeof(CgState), hipMemcpyHostToDevice,
                         c->stream));
@@ -1775,7 +1790,7 @@ static int clov_cg(const ClovSetup& s, int niter, double residue, const SpinorVi
 }
 
 // Mixed precision: the CG iterates on the FP32 operator (s32) with the FP32 x/r of the
-// tm mixed solver and FP64 reliable updates every delta = 1e-2 drop of |r| (x64 += xs +
+// tm mixed solver and FP64 reliable updates every reliable_delta drop of |r| (x64 += xs +
 // alpha p; r64 = b - K K^+ x64 on the FP64 operator s64; rs = r64; p = rs + beta p).
 static int clov_cg_mixed(const ClovSetup& s64, const ClovSetup& s32, int sp, int niter, double residue,
                          const SpinorView& x64, bool use_guess, const SpinorView& b, nq_solve_stats* out)
@@ -1798,7 +1813,7 @@ static int clov_cg_mixed(const ClovSetup& s64, const ClovSetup& s32, int sp, int
   s0.target = residue * b2;
   s0.niter = niter;
   s0.mixed = 1;
-  s0.delta = 1e-2;
+  s0.delta = reliable_delta(sp, residue);
   NQ_TRY(put_state(ctx, 0, s0));
   CgState* st = ctx->state;
   SpinorView none;

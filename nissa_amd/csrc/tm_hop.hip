@@ -108,6 +108,25 @@ __device__ __forceinline__ void hop_slot(const Geo& g, const Site& s, uint32_t e
   }
 }
 
+// The eight slots of one output site, the accumulators pinned after every PK slots
+// (PK = 4: two groups of four slots; the persistent FP64 kernel takes PK = 2).  The
+// pins order the schedule only; the arithmetic, and so every bit of the result, is
+// the same for every PK.
+template <int SP, int LK, class R, int PK>
+__device__ __forceinline__ void hop_site(const Geo& g, const Site& s, uint32_t e, const SpinorView& in, int r,
+                                         const void* ub, const double* det, R anis, bool ua, cx<R> o[12],
+                                         const char* lds = nullptr, uint32_t e0 = 0)
+{
+  hop_slot<SP, LK, R, 0, true, (1 % PK) == 0>(g, s, e, in, r, ub, det, anis, ua, o, lds, e0);
+  hop_slot<SP, LK, R, 0, false, (2 % PK) == 0>(g, s, e, in, r, ub, det, anis, ua, o, lds, e0);
+  hop_slot<SP, LK, R, 1, true, (3 % PK) == 0>(g, s, e, in, r, ub, det, anis, ua, o, lds, e0);
+  hop_slot<SP, LK, R, 1, false, (4 % PK) == 0>(g, s, e, in, r, ub, det, anis, ua, o, lds, e0);
+  hop_slot<SP, LK, R, 2, true, (5 % PK) == 0>(g, s, e, in, r, ub, det, anis, ua, o, lds, e0);
+  hop_slot<SP, LK, R, 2, false, (6 % PK) == 0>(g, s, e, in, r, ub, det, anis, ua, o, lds, e0);
+  hop_slot<SP, LK, R, 3, true, (7 % PK) == 0>(g, s, e, in, r, ub, det, anis, ua, o, lds, e0);
+  hop_slot<SP, LK, R, 3, false, true>(g, s, e, in, r, ub, det, anis, ua, o, lds, e0);
+}
+
 template <int SP>
 using RealOf = typename std::conditional<SP == SP_F64, double, float>::type;
 
@@ -225,14 +244,7 @@ __global__ void __launch_bounds__(256, (SP != SP_F64 ? 2 : 1)) k_tm_hop(Geo g, i
     cx<R> o[12];
 #pragma unroll
     for (int k = 0; k < 12; k++) o[k] = mk<R>(0, 0);
-    hop_slot<SP, LK, R, 0, true, false>(g, s, e, in, r, ub, det, anis, ua, o);
-    hop_slot<SP, LK, R, 0, false, false>(g, s, e, in, r, ub, det, anis, ua, o);
-    hop_slot<SP, LK, R, 1, true, false>(g, s, e, in, r, ub, det, anis, ua, o);
-    hop_slot<SP, LK, R, 1, false, true>(g, s, e, in, r, ub, det, anis, ua, o);
-    hop_slot<SP, LK, R, 2, true, false>(g, s, e, in, r, ub, det, anis, ua, o);
-    hop_slot<SP, LK, R, 2, false, false>(g, s, e, in, r, ub, det, anis, ua, o);
-    hop_slot<SP, LK, R, 3, true, false>(g, s, e, in, r, ub, det, anis, ua, o);
-    hop_slot<SP, LK, R, 3, false, true>(g, s, e, in, r, ub, det, anis, ua, o);
+    hop_site<SP, LK, R, 4>(g, s, e, in, r, ub, det, anis, ua, o);
     nrm = hop_epilogue<SP, MODE, XP, R>(g, epi, out, e, r, st, o);
   }
   if constexpr (MODE != 0) store_partial(nrm, partial, nchunks, m.r, m.chunk);
@@ -268,22 +280,19 @@ __global__ void __launch_bounds__(256, (SP != SP_F64 ? 2 : 1)) k_tm_hop_lds(Geo 
     cx<R> o[12];
 #pragma unroll
     for (int k = 0; k < 12; k++) o[k] = mk<R>(0, 0);
-    hop_slot<SP, LK, R, 0, true, false>(g, s, e, in, r, ub, det, anis, ua, o, lds, e0);
-    hop_slot<SP, LK, R, 0, false, false>(g, s, e, in, r, ub, det, anis, ua, o, lds, e0);
-    hop_slot<SP, LK, R, 1, true, false>(g, s, e, in, r, ub, det, anis, ua, o, lds, e0);
-    hop_slot<SP, LK, R, 1, false, true>(g, s, e, in, r, ub, det, anis, ua, o, lds, e0);
-    hop_slot<SP, LK, R, 2, true, false>(g, s, e, in, r, ub, det, anis, ua, o, lds, e0);
-    hop_slot<SP, LK, R, 2, false, false>(g, s, e, in, r, ub, det, anis, ua, o, lds, e0);
-    hop_slot<SP, LK, R, 3, true, false>(g, s, e, in, r, ub, det, anis, ua, o, lds, e0);
-    hop_slot<SP, LK, R, 3, false, true>(g, s, e, in, r, ub, det, anis, ua, o, lds, e0);
+    hop_site<SP, LK, R, 4>(g, s, e, in, r, ub, det, anis, ua, o, lds, e0);
     nrm = hop_epilogue<SP, MODE, XP, R>(g, epi, out, e, r, st, o);
   }
   if constexpr (MODE != 0) store_partial(nrm, partial, nchunks, m.r, m.chunk);
 }
 
-// Persistent form of k_tm_hop_lds for the FP64 hop, which runs one wavefront per
-// SIMD: one resident workgroup per slot loops over 256-site blocks, so no SIMD
-// idles between a finished workgroup and the dispatch of the next.  The site
+// Persistent form of k_tm_hop_lds for the FP64 hop: one resident workgroup per slot
+// loops over 256-site blocks, so no SIMD idles between a finished workgroup and the
+// dispatch of the next.  Its accumulators are pinned after every two slots (PK = 2:
+// 176-195 VGPRs, two wavefronts per SIMD; with PK = 4 the loads of four slots in
+// flight take 256 VGPRs + 32 AGPRs and one wavefront per SIMD).  Measured, hop
+// alone: 24^3x48 102.2 us (PK = 2) against 106.1 (PK = 4), 32^3x64 276.0 against
+// 276.5; PK = 1 (136-162 VGPRs, three wavefronts) 106 and 296 us.  The site
 // body is the one of k_tm_hop_lds, written out in the same form (the compiler's
 // FMA contraction follows the code's shape, and the kernels must agree bit for
 // bit with each other and with k_tm_hop).  The
@@ -323,14 +332,7 @@ __global__ void __launch_bounds__(256, 1) k_tm_hop_lds_p(Geo g, int op, SpinorVi
       cx<R> o[12];
 #pragma unroll
       for (int k = 0; k < 12; k++) o[k] = mk<R>(0, 0);
-      hop_slot<SP, LK, R, 0, true, false>(g, s, e, in, 0, ub, det, anis, ua, o, lds, e0);
-      hop_slot<SP, LK, R, 0, false, false>(g, s, e, in, 0, ub, det, anis, ua, o, lds, e0);
-      hop_slot<SP, LK, R, 1, true, false>(g, s, e, in, 0, ub, det, anis, ua, o, lds, e0);
-      hop_slot<SP, LK, R, 1, false, true>(g, s, e, in, 0, ub, det, anis, ua, o, lds, e0);
-      hop_slot<SP, LK, R, 2, true, false>(g, s, e, in, 0, ub, det, anis, ua, o, lds, e0);
-      hop_slot<SP, LK, R, 2, false, false>(g, s, e, in, 0, ub, det, anis, ua, o, lds, e0);
-      hop_slot<SP, LK, R, 3, true, false>(g, s, e, in, 0, ub, det, anis, ua, o, lds, e0);
-      hop_slot<SP, LK, R, 3, false, true>(g, s, e, in, 0, ub, det, anis, ua, o, lds, e0);
+      hop_site<SP, LK, R, 2>(g, s, e, in, 0, ub, det, anis, ua, o, lds, e0);
       nrm = hop_epilogue<SP, MODE, XP, R>(g, epi, out, e, 0, st, o);
     }
     if constexpr (MODE != 0) store_partial(nrm, partial, nchunks, 0, cb * 4 + wave);
@@ -598,7 +600,7 @@ static int tuned_rb(nq_ctx* ctx, int op, const SpinorView& in, const LinkView& U
 
 // Launches the persistent staged hop and returns true when it applies: FP64
 // with 18-real links and a lattice that fills the resident slots at least twice
-// (measured 2.8 % faster at 32^3x64).  The 12-real FP64 hop runs two wavefronts
+// (measured 2.8 % faster at 32^3x64 than the one-shot kernel).  The 12-real FP64 hop runs two wavefronts
 // per SIMD in the one-shot kernel and measured slower in the persistent one.
 template <int SP, int LK, int MODE, int XP>
 static bool launch_persistent(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, const LinkView& U,

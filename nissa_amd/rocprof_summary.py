@@ -25,7 +25,9 @@ def _db(path):
 
 
 def short(name: str) -> str:
-    n = name.split("(")[0]
+    # kernels in an unnamed namespace carry "(anonymous namespace)::" before their
+    # name; drop it before cutting the parameter list at the first "("
+    n = name.replace("(anonymous namespace)::", "").split("(")[0]
     return n.replace("void ", "").replace("nq::", "")
 
 
