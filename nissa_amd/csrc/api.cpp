@@ -192,7 +192,7 @@ void prof_flush(nq_ctx* ctx)
       KernelStat& s = ctx->stats[p.name];
       s.launches++;
       s.total_ms += ms;
-      s.bytes = p.bytes;
+      s.total_bytes += p.bytes;
     }
     ctx->event_pool.push_back(p.a);
     ctx->event_pool.push_back(p.b);
@@ -395,7 +395,7 @@ int nq_ctx_kernel_stat(nq_ctx* c, int i, char* name, int name_len, int64_t* laun
   if (name && name_len > 0) snprintf(name, name_len, "%s", it->first.c_str());
   if (launches) *launches = it->second.launches;
   if (total_ms) *total_ms = it->second.total_ms;
-  if (bytes_per_launch) *bytes_per_launch = it->second.bytes;
+  if (bytes_per_launch) *bytes_per_launch = it->second.launches ? it->second.total_bytes / it->second.launches : 0.0;
   return NQ_OK;
 }
 

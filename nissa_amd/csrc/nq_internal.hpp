@@ -150,6 +150,14 @@ struct HopEpi {
   SpinorView aux, aux2;
   SpinorView x, p, r, rs;
   CgState* st = nullptr;
+  // twisted clover blocks in the epilogue (clover_block.hpp): blk_mode 1 applies the
+  // inverse blocks blk (72 complex per site) to the hop before c1; blk_mode 2 applies
+  // the blocks z + clover term (blk: 36 complex per site, z = 1/(2 blk_kappa) +- i
+  // blk_mu) to aux before c2; blk_dag selects the dagger
+  SpinorView blk;
+  int blk_mode = 0;
+  int blk_dag = 0;
+  double blk_kappa = 0, blk_mu = 0;
 };
 
 // ---------------------------------------------------------------- objects
@@ -161,7 +169,7 @@ struct DevBuf {
 struct KernelStat {
   int64_t launches = 0;
   double total_ms = 0;
-  double bytes = 0;  // algorithmic bytes per launch (last value)
+  double total_bytes = 0;  // algorithmic bytes of all launches (the mean per launch is reported)
 };
 
 }  // namespace nq
@@ -304,10 +312,12 @@ int launch_cgm_vec2(nq_ctx* ctx, const SpinorView& s, const SpinorView& r, const
 int launch_cgm_vec3(nq_ctx* ctx, const SpinorView& r, const SpinorView& p, const SpinorView& ps, int64_t sstride,
                     void* states);
 // one multi-shift CG iteration for nr sources (states: device CgmState[nr]);
-// sol/ps hold the shifts of each source at element offset k*sstride
+// sol/ps hold the shifts of each source at element offset k*sstride; nbusy and nrun
+// (sources still iterating and their running shifts, as the host last saw them) only
+// size the ledger's byte count
 int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, const SpinorView& t,
                          const SpinorView& r, const SpinorView& sol, const SpinorView& ps, int64_t sstride,
-                         void* states, int nr);
+                         void* states, int nr, int nbusy, int nrun);
 
 // layout conversions
 // host buffer in nissa order (host_lx: lx-indexed buffer, else eo-indexed) <->

@@ -278,7 +278,14 @@ static int st_cgm_batched(nq_ctx* ctx, const LinkView& U, int nr, int nshift, co
   }
   int chunk = 16, launched = 0;
   for (;;) {
-    for (int it = 0; it < chunk; it++) NQ_TRY(launch_cgm_iteration(ctx, U, p, t, r, sol, ps, csz, dst, nr));
+    int nbusy = 0, nrun = 0;  // as of the last poll (the ledger's byte count)
+    for (int i = 0; i < nr; i++)
+      if (!hst[i].done) {
+        nbusy++;
+        for (int k = 0; k < nshift; k++) nrun += hst[i].run[k] != 0;
+      }
+    for (int it = 0; it < chunk; it++)
+      NQ_TRY(launch_cgm_iteration(ctx, U, p, t, r, sol, ps, csz, dst, nr, nbusy, nrun));
     launched += chunk;
     NQ_HIP(hipMemcpyAsync(hst, dst, sizeof(CgmState) * nr, hipMemcpyDeviceToHost, ctx->stream));
     NQ_HIP(hipStreamSynchronize(ctx->stream));

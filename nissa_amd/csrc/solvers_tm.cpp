@@ -1589,9 +1589,9 @@ int nq_bench_tm_hop_multi(nq_ctx* c, const nq_gauge* g, int prec, int nrhs, int 
 // src/dirac_operators/tmclovQ/dirac_operator_tmclovQ.cpp,
 // src/inverters/twisted_clover/cg_invert_tmclovD_eoprec.cpp): the tm hop kernels
 // with the site-local clover blocks of clover.hip in place of the diagonal
-// D_ee/D_oo.  K(dag) = g5 (D_oo(dag) - 1/4 H_oe D_ee(dag)^-1 H_eo): a plain hop,
-// the inverse blocks, the D_oo blocks and a second hop whose epilogue combines
-// them; K K^+ = K(0) K(1) (tmclovDkern_eoprec_square_eos).  FP64 throughout.
+// D_ee/D_oo.  K(dag) = g5 (D_oo(dag) - 1/4 H_oe D_ee(dag)^-1 H_eo): a hop whose
+// epilogue applies the inverse blocks and a second hop whose epilogue applies the
+// D_oo blocks and combines; K K^+ = K(0) K(1) (tmclovDkern_eoprec_square_eos).
 
 namespace nq {
 
@@ -1658,7 +1658,7 @@ struct ClovSetup {
   SpinorView inv_e;  // inverse blocks of the even sites for (kappa, mu)
   SpinorView cl_o;   // clover blocks of the odd sites
   double mu;
-  SpinorView te, te2, uo;  // scratch: two even, one odd
+  SpinorView te2, uo;  // scratch: the even intermediate of K; odd (the FP64 check)
 };
 
 static int clov_setup(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis, double mu,
@@ -1670,7 +1670,6 @@ static int clov_setup(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa,
   s->mu = mu;
   NQ_TRY(clov_inv_view(cl, NQ_EVEN, kappa, mu, &s->inv_e));
   s->cl_o = clov_view(cl, NQ_ODD);
-  NQ_TRY(alloc_spinor(c, "clov.te", SP_F64, 12, &s->te));
   NQ_TRY(alloc_spinor(c, "clov.te2", SP_F64, 12, &s->te2));
   NQ_TRY(alloc_spinor(c, "clov.uo", SP_F64, 12, &s->uo));
   return NQ_OK;
@@ -1687,9 +1686,7 @@ static int clov_setup_sloppy(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double
   NQ_TRY(clov_inv_view32(cl, NQ_EVEN, kappa, mu, &s->inv_e));
   NQ_TRY(clov_view32(cl, NQ_ODD, &s->cl_o));
   const bool h = sp == SP_F16;
-  NQ_TRY(alloc_spinor(c, h ? "clov.te16" : "clov.te32", sp, 12, &s->te));
   NQ_TRY(alloc_spinor(c, h ? "clov.te2_16" : "clov.te2_32", sp, 12, &s->te2));
-  NQ_TRY(alloc_spinor(c, h ? "clov.uo16" : "clov.uo32", sp, 12, &s->uo));
   return NQ_OK;
 }
 
@@ -1697,18 +1694,28 @@ static const double kOne[2][2] = {{1, 0}, {1, 0}};
 
 // K(dag): out_odd from in_odd; `last` carries the epilogue mode (0 write, 1 norm,
 // 2 CG tail) of the second hop, whose c1/c2/aux are set here
+// Two hop launches: the first applies the inverse blocks of the even sites to the
+// hop in its epilogue (te2 = D_ee(dag)^-1 H in), the second the D_oo(dag) blocks to
+// `in` read at its own site (out = g5 (D_oo in - 1/4 H te2)), so neither the hop nor
+// the D_oo product goes through memory.
 static int clov_kern(const ClovSetup& s, int dag, const SpinorView& out, const SpinorView& in, HopEpi last)
 {
   nq_ctx* c = s.S.ctx;
-  SpinorView none;
-  none.ncomp = 12;
   HopEpi h;
-  NQ_TRY(launch_tm_hop(c, NQ_EVEN, s.te, in, s.S.U, s.S.anis, h));
-  NQ_TRY(launch_clover_apply(c, 1, s.te2, s.te, s.inv_e, none, s.S.kappa, s.mu, dag, kOne, nullptr));
-  NQ_TRY(launch_clover_apply(c, 0, s.uo, in, s.cl_o, none, s.S.kappa, s.mu, dag, kOne, nullptr));
+  h.blk = s.inv_e;
+  h.blk_mode = 1;
+  h.blk_dag = dag;
+  h.blk_kappa = s.S.kappa;
+  h.blk_mu = s.mu;
+  NQ_TRY(launch_tm_hop(c, NQ_EVEN, s.te2, in, s.S.U, s.S.anis, h));
   setc(last.c1, -0.25, 0.25);
   setc(last.c2, 1.0, -1.0);
-  last.aux = s.uo;
+  last.aux = in;
+  last.blk = s.cl_o;
+  last.blk_mode = 2;
+  last.blk_dag = dag;
+  last.blk_kappa = s.S.kappa;
+  last.blk_mu = s.mu;
   return launch_tm_hop(c, NQ_ODD, out, s.te2, s.S.U, s.S.anis, last);
 }
 

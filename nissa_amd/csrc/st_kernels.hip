@@ -450,7 +450,8 @@ int launch_st_hop(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& 
   if ((mode == 0 || write) && out.prec != sp) return fail(NQ_EINVAL, "st hop: output precision mismatch");
   if (aux.data && aux.prec != sp) return fail(NQ_EINVAL, "st hop: aux precision mismatch");
   hipEvent_t ev = nullptr;
-  const double by = (8 * lbytes(lk) + csite(sp) * (1 + (aux.data ? 1 : 0) + ((mode == 0 || write) ? 1 : 0))) * nr;
+  // links once for all right-hand sides, the fields per right-hand side
+  const double by = 8 * lbytes(lk) + csite(sp) * (1 + (aux.data ? 1 : 0) + ((mode == 0 || write) ? 1 : 0)) * nr;
   prof_begin(ctx, mode == 0 ? "st_hop" : "st_hop_red", by * (double)g.Vh, &ev);
   const int RB = st_rb(nr, sp);
 #define NQ_ST(SPV, LKV)                                                                                              \
@@ -638,7 +639,7 @@ int cgm_state_bytes() { return (int)sizeof(CgmState); }
 
 int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, const SpinorView& t,
                          const SpinorView& r, const SpinorView& sol, const SpinorView& ps, int64_t sstride,
-                         void* states, int nr)
+                         void* states, int nr, int nbusy, int nrun)
 {
   const Geo& g = ctx->geo;
   const Launch L = launch_shape(g, nr);
@@ -649,7 +650,12 @@ int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, co
   SpinorView none;
   none.ncomp = 3;
   hipEvent_t ev = nullptr;
-  prof_begin(ctx, "cgm_hop1", (8 * lbytes(lk) + 2 * csite(sp)) * nr * (double)g.Vh, &ev);
+  // ledger bytes (algorithmic, per site): the links once for all sources; per busy
+  // source p gathered and t written (C1); t gathered, r read and written, and per
+  // running shift sol read and written and ps read (C2); r read, p read and written,
+  // and per running shift ps read and written (C3)
+  const double cs = csite(sp);
+  prof_begin(ctx, "cgm_hop1", (8 * lbytes(lk) + 2 * cs * nbusy) * (double)g.Vh, &ev);
   const int RB = st_rb(nr, sp);
   const int* skip = (const int*)&st->done;
   const int sstr = (int)(sizeof(CgmState) / sizeof(int));
@@ -663,7 +669,7 @@ int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, co
                      (int)FIN_CGM_PAP, 0.25);
   NQ_HIP(hipGetLastError());
   // C2
-  prof_begin(ctx, "cgm_hop2_update", (8 * lbytes(lk) + 3 * csite(sp)) * nr * (double)g.Vh, &ev);
+  prof_begin(ctx, "cgm_hop2_update", (8 * lbytes(lk) + 3 * cs * nbusy + 3 * cs * nrun) * (double)g.Vh, &ev);
   {
     const Shape sh = st_shape(nr, RB);
     const dim3 grid2((unsigned)((L.nchunks + 3) / 4 * sh.NRG)), blk2(256);
@@ -689,7 +695,7 @@ int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, co
                      (int)FIN_CGM_RFRF, 0.25);
   NQ_HIP(hipGetLastError());
   // C3
-  prof_begin(ctx, "cgm_update", 4 * csite(sp) * nr * (double)g.Vh, &ev);
+  prof_begin(ctx, "cgm_update", (3 * cs * nbusy + 2 * cs * nrun) * (double)g.Vh, &ev);
   if (sp == SP_F64)
     hipLaunchKernelGGL(k_cgm_update<SP_F64>, grid, blk, 0, ctx->stream, g, r, p, ps, sstride, st, L.s);
   else

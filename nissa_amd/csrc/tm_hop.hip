@@ -21,7 +21,7 @@
 #include <cstdlib>
 #include <string>
 
-#include "nq_device.hpp"
+#include "clover_block.hpp"
 
 namespace nq {
 
@@ -130,11 +130,28 @@ __device__ __forceinline__ void hop_site(const Geo& g, const Site& s, uint32_t e
 template <int SP>
 using RealOf = typename std::conditional<SP == SP_F64, double, float>::type;
 
+// Twisted clover block (MODE 0: z + clover term, 1: inverse) of each chirality
+// applied in place to the twelve components v of site e (the epilogue's block stage)
+template <int BMODE, class R>
+__device__ __forceinline__ void clover_site(const HopEpi& epi, const Geo& g, uint32_t e, cx<R> v[12])
+{
+  const cx<R>* B = (const cx<R>*)epi.blk.data;
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    cx<R> w[6];
+    if (epi.blk_dag) clover_half<BMODE, 1, R>(B, g.stride, e, h, epi.blk_kappa, epi.blk_mu, v + 6 * h, w);
+    else clover_half<BMODE, 0, R>(B, g.stride, e, h, epi.blk_kappa, epi.blk_mu, v + 6 * h, w);
+#pragma unroll
+    for (int q = 0; q < 6; q++) v[6 * h + q] = w[q];
+  }
+}
+
 // Site-local epilogue of one right-hand side r at output site e (shared by the
 // one-RHS and the register-blocked kernels, so both give the same bits):
 //   out_h = c1_h hop + c2_h aux + c3_h aux2, then the mode's tail; returns the
-// site's contribution to the reduced norm (modes 1-3).
-template <int SP, int MODE, int XP, class R>
+// site's contribution to the reduced norm (modes 1-3).  CLV 1: the inverse clover
+// blocks act on c1 hop; CLV 2: the clover blocks act on aux (the twisted clover K).
+template <int SP, int MODE, int XP, class R, int CLV = 0>
 __device__ __forceinline__ double hop_epilogue(const Geo& g, const HopEpi& epi, const SpinorView& out, uint32_t e,
                                                int r, CgState* st, cx<R> o[12])
 {
@@ -155,9 +172,11 @@ __device__ __forceinline__ double hop_epilogue(const Geo& g, const HopEpi& epi, 
   const cx<R> c1u = mk<R>((R)cf[0], (R)cf[1]), c1l = mk<R>((R)cf[2], (R)cf[3]);
 #pragma unroll
   for (int k = 0; k < 12; k++) o[k] = (k < 6 ? c1u : c1l) * o[k];
+  if constexpr (CLV == 1) clover_site<1, R>(epi, g, e, o);
   if (epi.aux.data) {
     cx<R> a[12];
     SpAcc<SP, R, 12, kNT>::load(epi.aux, g.stride, e, a, r);
+    if constexpr (CLV == 2) clover_site<0, R>(epi, g, e, a);
     const cx<R> cu = mk<R>((R)cf[4], (R)cf[5]), cl = mk<R>((R)cf[6], (R)cf[7]);
 #pragma unroll
     for (int k = 0; k < 12; k++) fma_to(o[k], k < 6 ? cu : cl, a[k]);
@@ -221,7 +240,7 @@ __device__ __forceinline__ bool rhs_idle(const CgState* st, int pred)
 
 // workgroups of at most 256 threads; FP64 builds get the full 256-VGPR budget
 // (one wavefront per SIMD per workgroup), the sloppy builds two workgroups per CU
-template <int SP, int LK, int MODE, int XP>
+template <int SP, int LK, int MODE, int XP, int CLV = 0>
 __global__ void __launch_bounds__(256, (SP != SP_F64 ? 2 : 1)) k_tm_hop(Geo g, int op, SpinorView out, SpinorView in, LinkView U, double anis_d,
                                                int use_anis, HopEpi epi, Shape S, int64_t nchunks,
                                                double* partial)
@@ -245,14 +264,14 @@ __global__ void __launch_bounds__(256, (SP != SP_F64 ? 2 : 1)) k_tm_hop(Geo g, i
 #pragma unroll
     for (int k = 0; k < 12; k++) o[k] = mk<R>(0, 0);
     hop_site<SP, LK, R, 4>(g, s, e, in, r, ub, det, anis, ua, o);
-    nrm = hop_epilogue<SP, MODE, XP, R>(g, epi, out, e, r, st, o);
+    nrm = hop_epilogue<SP, MODE, XP, R, CLV>(g, epi, out, e, r, st, o);
   }
   if constexpr (MODE != 0) store_partial(nrm, partial, nchunks, m.r, m.chunk);
 }
 
 // One right-hand side, neighbours staged in LDS (Stage): the workgroup's four
 // wavefronts cover 256 consecutive output sites (Shape W = 4, NRB = 1).
-template <int SP, int LK, int MODE, int XP>
+template <int SP, int LK, int MODE, int XP, int CLV = 0>
 __global__ void __launch_bounds__(256, (SP != SP_F64 ? 2 : 1)) k_tm_hop_lds(Geo g, int op, SpinorView out,
                                                                         SpinorView in, LinkView U, double anis_d,
                                                                         int use_anis, HopEpi epi, Shape S,
@@ -281,7 +300,7 @@ __global__ void __launch_bounds__(256, (SP != SP_F64 ? 2 : 1)) k_tm_hop_lds(Geo 
 #pragma unroll
     for (int k = 0; k < 12; k++) o[k] = mk<R>(0, 0);
     hop_site<SP, LK, R, 4>(g, s, e, in, r, ub, det, anis, ua, o, lds, e0);
-    nrm = hop_epilogue<SP, MODE, XP, R>(g, epi, out, e, r, st, o);
+    nrm = hop_epilogue<SP, MODE, XP, R, CLV>(g, epi, out, e, r, st, o);
   }
   if constexpr (MODE != 0) store_partial(nrm, partial, nchunks, m.r, m.chunk);
 }
@@ -299,7 +318,7 @@ __global__ void __launch_bounds__(256, (SP != SP_F64 ? 2 : 1)) k_tm_hop_lds(Geo 
 // workgroups of XCD x (b % 8 == x, the hardware's round-robin) walk one
 // contiguous eighth of the blocks together, block bi, bi + nbx, ... of it, as the
 // one-shot kernel's dispatch order does.  Partials stay per 64-site chunk.
-template <int SP, int LK, int MODE, int XP>
+template <int SP, int LK, int MODE, int XP, int CLV = 0>
 __global__ void __launch_bounds__(256, 1) k_tm_hop_lds_p(Geo g, int op, SpinorView out, SpinorView in, LinkView U,
                                                          double anis_d, int use_anis, HopEpi epi, int64_t nblk,
                                                          int64_t nchunks, double* partial)
@@ -333,19 +352,19 @@ __global__ void __launch_bounds__(256, 1) k_tm_hop_lds_p(Geo g, int op, SpinorVi
 #pragma unroll
       for (int k = 0; k < 12; k++) o[k] = mk<R>(0, 0);
       hop_site<SP, LK, R, 2>(g, s, e, in, 0, ub, det, anis, ua, o, lds, e0);
-      nrm = hop_epilogue<SP, MODE, XP, R>(g, epi, out, e, 0, st, o);
+      nrm = hop_epilogue<SP, MODE, XP, R, CLV>(g, epi, out, e, 0, st, o);
     }
     if constexpr (MODE != 0) store_partial(nrm, partial, nchunks, 0, cb * 4 + wave);
   }
 }
 
 // resident workgroups per CU of the persistent kernel (queried once per instance)
-template <int SP, int LK, int MODE, int XP>
+template <int SP, int LK, int MODE, int XP, int CLV>
 static int persist_per_cu()
 {
   static const int occ = [] {
     int o = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, (const void*)k_tm_hop_lds_p<SP, LK, MODE, XP>, 256, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, (const void*)k_tm_hop_lds_p<SP, LK, MODE, XP, CLV>, 256, 0) !=
             hipSuccess || o < 1)
       o = 0;
     return o;
@@ -503,7 +522,10 @@ double tm_hop_bytes(int sp, int lk, const HopEpi& epi, int ncomp)
     b += s;
     if (epi.rs.data) b += sp_site_bytes(epi.rs.prec, ncomp);
   }
-  return 8 * lk_bytes(lk) + b * epi.nrhs;
+  // clover blocks of the epilogue: 72 (inverse) or 36 complex per site, FP64 with FP64
+  // spinors and FP32 otherwise, read once for all right-hand sides
+  const double blk = epi.blk_mode ? (epi.blk_mode == 1 ? 72.0 : 36.0) * (sp == SP_F64 ? 16 : 8) : 0.0;
+  return 8 * lk_bytes(lk) + blk + b * epi.nrhs;
 }
 
 int64_t blocks_for(int64_t n) { return (n + kBlock - 1) / kBlock; }
@@ -602,16 +624,16 @@ static int tuned_rb(nq_ctx* ctx, int op, const SpinorView& in, const LinkView& U
 // with 18-real links and a lattice that fills the resident slots at least twice
 // (measured 2.8 % faster at 32^3x64 than the one-shot kernel).  The 12-real FP64 hop runs two wavefronts
 // per SIMD in the one-shot kernel and measured slower in the persistent one.
-template <int SP, int LK, int MODE, int XP>
+template <int SP, int LK, int MODE, int XP, int CLV>
 static bool launch_persistent(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, const LinkView& U,
                               double anis, const HopEpi& epi, const Launch& L)
 {
   if constexpr (SP == SP_F64 && LK == LK_F64_18) {
     const char* penv = getenv("NQ_PERSIST");
     if (penv && penv[0] == '0') return false;
-    const int64_t nblk = (L.nchunks + 3) / 4, slots = (int64_t)ctx->ncu * persist_per_cu<SP, LK, MODE, XP>();
+    const int64_t nblk = (L.nchunks + 3) / 4, slots = (int64_t)ctx->ncu * persist_per_cu<SP, LK, MODE, XP, CLV>();
     if (slots < 1 || nblk < 2 * slots) return false;
-    hipLaunchKernelGGL((k_tm_hop_lds_p<SP, LK, MODE, XP>), dim3((unsigned)slots), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL((k_tm_hop_lds_p<SP, LK, MODE, XP, CLV>), dim3((unsigned)slots), dim3(256), 0, ctx->stream,
                        ctx->geo, op, out, in, U, anis, anis != 1.0 ? 1 : 0, epi, nblk, L.nchunks, ctx->partial);
     return true;
   } else {
@@ -620,7 +642,7 @@ static bool launch_persistent(nq_ctx* ctx, int op, const SpinorView& out, const 
   }
 }
 
-template <int SP, int LK, int MODE, int XP>
+template <int SP, int LK, int MODE, int XP, int CLV = 0>
 static int go(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, const LinkView& U, double anis,
               const HopEpi& epi, const char* name)
 {
@@ -630,7 +652,7 @@ static int go(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, 
   const Launch L = launch_shape(g, nr);
   if (L.nchunks * nr > ctx->max_partials) return fail(NQ_EINVAL, "tm hop: reduction scratch too small");
   hipEvent_t ev = nullptr;
-  if constexpr (MODE != 3) {
+  if constexpr (MODE != 3 && CLV == 0) {
     if (epi.batched || nr > 1) {
       // register-blocked right-hand sides (always for batched solves, so that a
       // right-hand side runs the same code alone and in a batch); RB right-hand
@@ -651,12 +673,12 @@ static int go(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, 
   // kernels measured slower with staging and read their neighbours from memory)
   const char* lenv = getenv("NQ_LDS");
   if (SP == SP_F64 && nr == 1 && L.s.W == 4 && !(lenv && lenv[0] == '0')) {
-    if (!launch_persistent<SP, LK, MODE, XP>(ctx, op, out, in, U, anis, epi, L))
-      hipLaunchKernelGGL((k_tm_hop_lds<SP, LK, MODE, XP>), dim3((unsigned)L.blocks), dim3(L.threads), 0, ctx->stream, g,
+    if (!launch_persistent<SP, LK, MODE, XP, CLV>(ctx, op, out, in, U, anis, epi, L))
+      hipLaunchKernelGGL((k_tm_hop_lds<SP, LK, MODE, XP, CLV>), dim3((unsigned)L.blocks), dim3(L.threads), 0, ctx->stream, g,
                          op, out, in, U, anis, anis != 1.0 ? 1 : 0, epi, L.s, L.nchunks, ctx->partial);
   }
   else
-    hipLaunchKernelGGL((k_tm_hop<SP, LK, MODE, XP>), dim3((unsigned)L.blocks), dim3(L.threads), 0, ctx->stream, g, op,
+    hipLaunchKernelGGL((k_tm_hop<SP, LK, MODE, XP, CLV>), dim3((unsigned)L.blocks), dim3(L.threads), 0, ctx->stream, g, op,
                        out, in, U, anis, anis != 1.0 ? 1 : 0, epi, L.s, L.nchunks, ctx->partial);
   NQ_HIP(hipGetLastError());
   prof_end(ctx, ev);
@@ -672,6 +694,44 @@ static int go(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, 
   default: break;                                                                           \
   }
 
+// Twisted clover K: the first hop with the inverse blocks (blk_mode 1, mode 0 only)
+// and the second with the clover blocks on aux (blk_mode 2, modes 0-2), for the
+// operator precisions of the clover solves: FP64 spinors with FP64 18-real links and
+// FP64 blocks; FP32 spinors with FP32 18-real links, or FP16 spinors with FP32
+// 12-real links, with FP32 blocks.
+template <int SP, int LK, int XP>
+static int clover_modes(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, const LinkView& U,
+                        double anis, const HopEpi& epi, const char* tag)
+{
+  const std::string n1 = std::string("tm_hop_clinv_") + tag, n0 = std::string("tm_hop_cldee_") + tag,
+                    nn = std::string("tm_hop_norm_cldee_") + tag, nt = std::string("tm_hop_cgtail_cldee_") + tag;
+  if (epi.blk_mode == 1 && epi.mode == 0) return go<SP, LK, 0, SP, 1>(ctx, op, out, in, U, anis, epi, n1.c_str());
+  if (epi.blk_mode == 2) {
+    if (!epi.aux.data) return fail(NQ_EINVAL, "tm hop: clover blocks on aux without aux");
+    switch (epi.mode) {
+    case 0: return go<SP, LK, 0, SP, 2>(ctx, op, out, in, U, anis, epi, n0.c_str());
+    case 1: return go<SP, LK, 1, SP, 2>(ctx, op, out, in, U, anis, epi, nn.c_str());
+    case 2: return go<SP, LK, 2, XP, 2>(ctx, op, out, in, U, anis, epi, nt.c_str());
+    default: break;
+    }
+  }
+  return fail(NQ_EINVAL, "tm hop: clover epilogue %d with mode %d", epi.blk_mode, epi.mode);
+}
+
+static int launch_tm_hop_clover(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, const LinkView& U,
+                                double anis, const HopEpi& epi)
+{
+  const int sp = in.prec, lk = U.prec;
+  const int bp = sp == SP_F64 ? SP_F64 : SP_F32;
+  if (!epi.blk.data || epi.blk.prec != bp || epi.blk.ncomp != (epi.blk_mode == 1 ? 72 : 36))
+    return fail(NQ_EINVAL, "tm hop: clover blocks of the wrong kind or precision");
+  if (sp == SP_F64 && lk == LK_F64_18) return clover_modes<SP_F64, LK_F64_18, SP_F64>(ctx, op, out, in, U, anis, epi, "f64");
+  if (sp == SP_F32 && lk == LK_F32_18) return clover_modes<SP_F32, LK_F32_18, SP_F32>(ctx, op, out, in, U, anis, epi, "f32");
+  if (sp == SP_F16 && lk == LK_F32_12)
+    return clover_modes<SP_F16, LK_F32_12, SP_F32>(ctx, op, out, in, U, anis, epi, "f16_l32c");
+  return fail(NQ_EINVAL, "tm hop: no clover epilogue for spinor precision %d with links %d", sp, lk);
+}
+
 int launch_tm_hop(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, const LinkView& U, double anis,
                   const HopEpi& epi)
 {
@@ -684,6 +744,7 @@ int launch_tm_hop(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& 
   if (epi.aux2.data && epi.aux2.prec != sp) return fail(NQ_EINVAL, "tm hop: aux2 precision mismatch");
   if (epi.mode == 2 && (epi.p.prec != sp || (epi.x.data && epi.x.prec != epi.r.prec)))
     return fail(NQ_EINVAL, "tm hop: CG tail precision mismatch");
+  if (epi.blk_mode) return launch_tm_hop_clover(ctx, op, out, in, U, anis, epi);
   if (sp == SP_F64 && lk == LK_F64_18) {
     if (epi.mode == 2 && epi.r.prec != SP_F64) return fail(NQ_EINVAL, "FP64 CG tail needs FP64 x,r");
     NQ_MODES(SP_F64, LK_F64_18, SP_F64, "f64")

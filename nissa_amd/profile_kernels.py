@@ -76,7 +76,10 @@ def main():
         print(f"stagcgm x{a.nrhs} {a.prec}: {dt:.3f} s, iters {res[0]['iters']}, "
               f"worst {max(max(r['true_res_sq']) for r in res):.2e}")
         for name, v in sorted(ctx.kernel_stats().items(), key=lambda kv: -kv[1]["total_ms"])[:10]:
-            print(f"   {name:28s} {v['launches']:7d} x {v['total_ms']/v['launches']*1e3:8.1f} us = {v['total_ms']:9.1f} ms")
+            avg = v["total_ms"] / v["launches"]
+            gbs = v["bytes_per_launch"] / (avg * 1e-3) / 1e9 if v["bytes_per_launch"] else 0.0
+            print(f"   {name:28s} {v['launches']:7d} x {avg*1e3:8.1f} us = {v['total_ms']:9.1f} ms"
+                  f"  {v['bytes_per_launch']/1e6:9.1f} MB/launch  {gbs:6.0f} GB/s")
     elif a.what == "copy":
         fo2 = ctx.field(nq.SPINCOLOR, nq.ODD)
         for _ in range(a.n):
