@@ -471,6 +471,8 @@ def serve_bench(args):
         log(f"[bench] serve: {len(bad)} failed requests, statuses {sorted(set(r['status'] for r in bad))}, "
             f"worst check {max(r['max_rel_res_sq'] for r in bad):.2e}")
     lat = np.array([r["t_done"] - r["t_submit"] for r in res])
+    wait = np.array([r["t_admit"] - r["t_submit"] for r in res])  # queued until the first source took a slot
+    service = np.array([r["t_done"] - r["t_admit"] for r in res])
     span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
     nsol = 12 * n
     fp64_one_at_a_time = nsol / float(sum(12 * t64[w] for w in which))
@@ -479,6 +481,11 @@ def serve_bench(args):
         "volumes": ["%dx%d^3" % (L[0], L[1]) for L in vols], "mix": mix, "masses": mus,
         "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
         "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+        "p50_queue_wait_s": round(float(np.percentile(wait, 50)), 4),
+        "p99_queue_wait_s": round(float(np.percentile(wait, 99)), 4),
+        "p50_service_s": round(float(np.percentile(service, 50)), 4),
+        "p99_service_s": round(float(np.percentile(service, 99)), 4),
+        "offered_solves_per_s": round(12 * args.serve_rate, 1),
         "solves_per_s": round(nsol / span, 2),
         "fp64_one_at_a_time_solves_per_s": round(fp64_one_at_a_time, 2),
         "fp64_solve_s_per_volume": [round(t, 4) for t in t64],

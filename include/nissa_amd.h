@@ -246,6 +246,14 @@ int nq_tmclov_cg_kernsq(nq_ctx* ctx, const nq_gauge* g, nq_clover* cl, double ka
 int nq_tmclov_solve_eoprec(nq_ctx* ctx, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis,
                            double mu, int niter, double residue, int prec, nq_field* sol, const nq_field* src,
                            nq_solve_stats* st, double* check);
+/* inv_tmclovD_cg_eoprec for nrhs sources at one (kappa, mu) and the handle's cSW, as the
+   propagator driver calls it for its 12 spin-colour sources (projects/ib/prop.cpp:93-96):
+   the K K^+ CG of up to 16 sources runs in the same launches, with the clover blocks of a
+   site read once for all of them; sources stream through the slots; each result as
+   nq_tmclov_solve_eoprec's (FP64 check in st[i].rel_res_sq) */
+int nq_tmclov_solve_eoprec_multi(nq_ctx* ctx, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis,
+                                 double mu, int nrhs, int niter, double residue, int prec, nq_field** sols,
+                                 const nq_field* const* srcs, nq_solve_stats* st);
 
 /* ------------------------------------------------------------------ staggered */
 /* apply_st2Doe (dirac_operator_stD.hpp:47-77) */
@@ -346,6 +354,8 @@ typedef struct nq_serve_request {
   int64_t site0;
   const double* const* src; /* src_kind 0: nsrc host buffers, read when the source starts */
   double* const* sol;       /* nsrc host lx buffers written when each source finishes, or NULL */
+  double csw;               /* 0: twisted mass; > 0: twisted clover with this cSW
+                               (inv_tmclovD_cg_eoprec; its sources batch per (kappa, mu, cSW)) */
 } nq_serve_request;
 typedef struct nq_serve_result {
   int64_t id;

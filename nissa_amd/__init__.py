@@ -61,6 +61,7 @@ class ServeRequest(C.Structure):
         ("site0", C.c_int64),
         ("src", C.c_void_p),
         ("sol", C.c_void_p),
+        ("csw", C.c_double),
     ]
 
 
@@ -101,6 +102,7 @@ _SIG = {
     "nq_tmclovQ2_cgm": ([_P, _P, _P, _D, _DP, _I, _DP, _I, _DP, _P, _P, _P, _DP], _I),
     "nq_tmclov_cg_kernsq": ([_P, _P, _P, _D, _DP, _D, _I, _D, _I, _P, _P, _P, _P], _I),
     "nq_tmclov_solve_eoprec": ([_P, _P, _P, _D, _DP, _D, _I, _D, _I, _P, _P, _P, _DP], _I),
+    "nq_tmclov_solve_eoprec_multi": ([_P, _P, _P, _D, _DP, _D, _I, _I, _D, _I, _P, _P, _P], _I),
     "nq_st_rootst_force": ([_P, _P, _DP, _DP, _I, _DP, _DP, _I, _D, _I, _P, _DP, _DP, _P], _I),
     "nq_st_apply_D": ([_P, _P, _D, _I, _P, _P, _P, _P], _I),
     "nq_st_solve_D": ([_P, _P, _D, _I, _D, _P, _P, _P, _P, _P, _P, _DP], _I),
@@ -671,6 +673,20 @@ def tmclov_solve_eoprec(ctx, g, cl, kappa, mu, niter, residue, sol, src, anis=No
     return d
 
 
+def tmclov_solve_eoprec_multi(ctx, g, cl, kappa, mu, niter, residue, sols, srcs, prec="f16", anis=None):
+    """nq_tmclov_solve_eoprec_multi: the twisted clover eo solve of several sources at one mu."""
+    n = len(srcs)
+    assert len(sols) == n
+    so = (_P * n)(*[s.h for s in sols])
+    sr = (_P * n)(*[s.h for s in srcs])
+    st = (SolveStats * n)()
+    rc = lib().nq_tmclov_solve_eoprec_multi(ctx.h, g.h, cl.h, kappa, _anis(anis), mu, n, niter, residue, PREC[prec],
+                                            so, sr, st)
+    if rc not in (NQ_OK, NQ_ENOCONV):
+        _check(rc)
+    return [dict(st[i].as_dict(), rc=rc) for i in range(n)]
+
+
 def tmQ2_cg(ctx, g, kappa, mu, niter, residue, sol, src, guess=None):
     st = SolveStats()
     rc = lib().nq_tmQ2_cg(ctx.h, g.h, kappa, mu, niter, residue, sol.h, guess.h if guess is not None else None,
@@ -821,11 +837,13 @@ class Server:
         return vid.value
 
     def submit(self, volume, kappa, mu, residue, prec="f16", niter=100000, point_site=None, nsrc=12,
-               srcs=None, sols=None):
+               srcs=None, sols=None, csw=0.0):
         """Point sources (spin-colour 0..nsrc-1 at lx site point_site) or host sources
-        `srcs` (list of lx arrays); `sols` (list of lx arrays) receive the solutions."""
+        `srcs` (list of lx arrays); `sols` (list of lx arrays) receive the solutions;
+        csw > 0 makes it a twisted clover request."""
         r = ServeRequest()
         r.volume, r.prec, r.kappa, r.mu, r.residue, r.niter = volume, PREC[prec], kappa, mu, residue, niter
+        r.csw = csw
         keep = []
         if srcs is not None:
             srcs = [np.ascontiguousarray(x, dtype=np.float64) for x in srcs]

@@ -319,6 +319,66 @@ __global__ void __launch_bounds__(256) k_clover_apply_h(Geo g, SpinorView out, S
   SpAcc<SP, R, 6>::store(oh, g.stride, e, o);
 }
 
+// Batched forms for nr right-hand sides (views with rstride), out_r = block in_r, for
+// the batched twisted clover solve.  A right-hand side whose solver state says idle is
+// skipped (CgState done; with pred 1 no reliable update due, with pred 2 one due).  FP64 and
+// FP32 spinors: a wavefront per chirality keeps the block in registers and applies
+// it to every right-hand side; FP16: one thread per site, the block read per
+// right-hand side through the cache (the per-site norm spans both chiralities).
+__device__ __forceinline__ bool clov_idle(const CgState* st, int r, int pred)
+{
+  return st && (st[r].done || (pred == 1 && !st[r].reliable) || (pred == 2 && st[r].reliable));
+}
+
+template <int MODE, int DAG, int SP>
+__global__ void __launch_bounds__(256) k_clover_apply_hb(Geo g, SpinorView out, SpinorView in, SpinorView blk,
+                                                         double kappa, double mu, int nr, const CgState* st, int pred)
+{
+  static_assert(SP == SP_F64 || SP == SP_F32, "chirality split needs whole records per half");
+  using R = typename std::conditional<SP == SP_F64, double, float>::type;
+  typedef cx<R> CR;
+  constexpr int EB = SP == SP_F64 ? 16 : 8;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int h = wave & 1;
+  const uint32_t e = (blockIdx.x * 2u + (uint32_t)(wave >> 1)) * 64u + (threadIdx.x & 63u);
+  if (e >= (uint32_t)g.Vh) return;
+  const int64_t hoff = (int64_t)h * 6 * g.stride * EB;
+  SpinorView ih = in, oh = out;
+  ih.data = (char*)in.data + hoff;
+  oh.data = (char*)out.data + hoff;
+  ClovBlk<MODE, R> k;
+  clover_load<MODE, R>((const CR*)blk.data, g.stride, e, h, k);
+  for (int r = 0; r < nr; r++) {
+    if (clov_idle(st, r, pred)) continue;  // uniform over the grid
+    CR v[6], o[6];
+    SpAcc<SP, R, 6>::load(ih, g.stride, e, v, r);
+    clover_apply_blk<MODE, DAG, R>(k, h, kappa, mu, v, o);
+    SpAcc<SP, R, 6>::store(oh, g.stride, e, o, r);
+  }
+}
+
+template <int MODE, int DAG>
+__global__ void __launch_bounds__(256) k_clover_apply_b16(Geo g, SpinorView out, SpinorView in, SpinorView blk,
+                                                          double kappa, double mu, int nr, const CgState* st, int pred)
+{
+  typedef cx<float> CR;
+  const uint32_t e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= (uint32_t)g.Vh) return;
+  const CR* B = (const CR*)blk.data;
+  for (int r = 0; r < nr; r++) {
+    if (clov_idle(st, r, pred)) continue;
+    CR v[12], o[12];
+    SpAcc<SP_F16, float, 12>::load(in, g.stride, e, v, r);
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      clover_half<MODE, DAG, float>(B, g.stride, e, h, kappa, mu, v + 6 * h, o + 6 * h);
+#pragma unroll
+      for (int q = 0; q < 6; q++) asm volatile("" : "+v"(o[6 * h + q].re), "+v"(o[6 * h + q].im));
+    }
+    SpAcc<SP_F16, float, 12>::store(out, g.stride, e, o, r);
+  }
+}
+
 // FP32 copy of an FP64 block field (sloppy solves)
 __global__ void __launch_bounds__(256) k_blocks_to_f32(Geo g, int ncomp, const C* in, cx<float>* out)
 {
@@ -469,6 +529,40 @@ int launch_clover_apply(nq_ctx* ctx, int mode, const SpinorView& out, const Spin
     return fail(NQ_EINVAL, "twisted clover: unknown spinor precision");
   }
 #undef NQ_CLOV
+  NQ_HIP(hipGetLastError());
+  prof_end(ctx, ev);
+  return NQ_OK;
+}
+
+int launch_clover_apply_batched(nq_ctx* ctx, int mode, int dag, const SpinorView& out, const SpinorView& in,
+                                const SpinorView& blk, double kappa, double mu, int nr, const CgState* st, int pred)
+{
+  const Geo& g = ctx->geo;
+  const int sp = in.prec;
+  const int bp = sp == SP_F16 ? SP_F32 : sp;
+  if (out.prec != sp || blk.prec != bp || nr < 1 || nr > 16)
+    return fail(NQ_EINVAL, "batched twisted clover: FP64/FP32/FP16 fields with FP64/FP32/FP32 blocks, 1..16 RHS");
+  hipEvent_t ev = nullptr;
+  const double cb = bp == SP_F64 ? 16.0 : 8.0;
+  const double sb = sp == SP_F64 ? 192.0 : (sp == SP_F32 ? 96.0 : 52.0);
+  prof_begin(ctx, mode == 0 ? "clover_dee_batch" : "clover_inv_batch",
+             (double)g.Vh * ((mode == 0 ? 36 : 72) * cb + 2 * sb * nr), &ev);
+  const dim3 grid((unsigned)((g.Vh + 255) / 256)), grid_h((unsigned)((g.Vh + 127) / 128)), b256(256);
+#define NQ_CLB(M, D)                                                                                             \
+  do {                                                                                                           \
+    if (sp == SP_F64)                                                                                            \
+      hipLaunchKernelGGL((k_clover_apply_hb<M, D, SP_F64>), grid_h, b256, 0, ctx->stream, g, out, in, blk, kappa, mu, \
+                         nr, st, pred);                                                                          \
+    else if (sp == SP_F32)                                                                                       \
+      hipLaunchKernelGGL((k_clover_apply_hb<M, D, SP_F32>), grid_h, b256, 0, ctx->stream, g, out, in, blk, kappa, mu, \
+                         nr, st, pred);                                                                          \
+    else                                                                                                         \
+      hipLaunchKernelGGL((k_clover_apply_b16<M, D>), grid, b256, 0, ctx->stream, g, out, in, blk, kappa, mu, nr, st, \
+                         pred);                                                                                  \
+  } while (0)
+  if (mode == 0) { if (dag) NQ_CLB(0, 1); else NQ_CLB(0, 0); }
+  else { if (dag) NQ_CLB(1, 1); else NQ_CLB(1, 0); }
+#undef NQ_CLB
   NQ_HIP(hipGetLastError());
   prof_end(ctx, ev);
   return NQ_OK;

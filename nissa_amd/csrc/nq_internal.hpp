@@ -333,6 +333,11 @@ int launch_clover_invert(nq_ctx* ctx, const SpinorView& cl, const SpinorView& in
 // Re tr sums of the temporal (ts[0]) and spatial (ts[1]) plaquettes over the lattice
 int launch_plaquette(nq_ctx* ctx, const double* aos_e, const double* aos_o, double ts[2]);
 int launch_blocks_to_f32(nq_ctx* ctx, int ncomp, const SpinorView& in, const SpinorView& out);
+// the block (mode 0: z + clover term, 1: inverse) of every site applied to nr right-hand
+// sides (views with rstride), skipping those whose state st[r] is idle (pred as for the
+// hop epilogues; st may be null)
+int launch_clover_apply_batched(nq_ctx* ctx, int mode, int dag, const SpinorView& out, const SpinorView& in,
+                                const SpinorView& blk, double kappa, double mu, int nr, const CgState* st, int pred);
 int launch_clover_apply(nq_ctx* ctx, int mode, const SpinorView& out, const SpinorView& in, const SpinorView& blk,
                         const SpinorView& aux, double kappa, double mu, int dag, const double a[2][2],
                         const double b[2][2]);

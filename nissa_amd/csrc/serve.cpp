@@ -15,7 +15,10 @@
 //  * engines of different volumes run on their own streams and threads, so their
 //    kernels overlap on the device.
 // A source's arithmetic is independent of the other slots and of the slot index
-// (TmBatch), so its solution is bit-identical alone and in any mix.
+// (TmBatch), so its solution is bit-identical alone and in any mix.  Requests with a
+// cSW are twisted clover solves (inv_tmclovD_cg_eoprec, as the driver calls it when
+// its input sets cSW, projects/ib/prop.cpp:93-96): the engine builds the clover term
+// of its gauge field once per cSW, and such sources batch per (kappa, mu, cSW).
 
 #include <algorithm>
 #include <chrono>
@@ -57,6 +60,8 @@ struct Group {
   TmBatch b;
   double kappa = 0;
   int prec = 0;
+  double csw = 0;  // > 0: a twisted clover group, whose sources share one mu
+  double mu = 0;
   RhsRef slot[TmBatch::kMaxSlots];
   bool pending[TmBatch::kMaxSlots] = {};  // assigned, not started yet
   nq_field* src[TmBatch::kMaxSlots] = {};
@@ -90,7 +95,13 @@ struct Group {
     for (int i = 0; i < TmBatch::kMaxSlots; i++) n += slot[i].req != nullptr;
     return n;
   }
-  bool fits(const Req& q) const { return !draining && kappa == q.r.kappa && prec == q.r.prec && has_free(); }
+  // the parameters a source must share with the group's (kappa, precision; cSW and,
+  // for twisted clover, mu)
+  bool same(const Req& q) const
+  {
+    return kappa == q.r.kappa && prec == q.r.prec && csw == q.r.csw && (csw == 0 || mu == q.r.mu);
+  }
+  bool fits(const Req& q) const { return !draining && same(q) && has_free(); }
 };
 
 struct Engine {
@@ -102,6 +113,7 @@ struct Engine {
   // batch groups by index: group i owns the solver states [16 i, 16 i + 16) of the
   // context and the temporaries named "tb<i>.", so a group is replaced in place
   std::unique_ptr<Group> groups[TmBatch::kMaxGroups];
+  std::map<double, nq_clover*> clovers;  // clover term of the resident gauge per cSW (made on first use)
   int64_t gauge_bytes = 0;
   std::thread th;
 };
@@ -127,10 +139,15 @@ struct nq_server {
 
 namespace {
 
-int64_t group_bytes(const nq_ctx* c, int prec)
+int64_t group_bytes(const nq_ctx* c, int prec, bool clover)
 {
-  const int64_t lx = 2 * 12 * 16 * c->geo.stride;  // one FP64 lx spincolor field
-  return TmBatch::kMaxSlots * (TmBatch::bytes_per_slot(c->geo, prec) + 2 * lx) + (64 << 20);
+  const int64_t s = c->geo.stride;
+  const int64_t lx = 2 * 12 * 16 * s;  // one FP64 lx spincolor field
+  int64_t b = TmBatch::kMaxSlots * (TmBatch::bytes_per_slot(c->geo, prec, clover) + 2 * lx) + (64 << 20);
+  // clover: the group's inverse blocks (FP64, FP32 when mixed) and, counted with every
+  // group, the clover term of both parities with its FP32 odd copy
+  if (clover) b += 72 * 16 * s + (prec != NQ_PREC_F64 ? 72 * 8 * s : 0) + 2 * 36 * 16 * s + 36 * 8 * s + 12 * 16 * s;
+  return b;
 }
 
 // point source: 1 at lx site `site` and spin-colour component sc (id*3+ic), 0 elsewhere
@@ -192,7 +209,7 @@ void admit(nq_server* s, Engine* E, Admission* A)
     Group* sel = nullptr;
     for (int gi = 0; gi < ng && !sel; gi++)
       if (E->groups[gi] && E->groups[gi]->fits(q)) sel = E->groups[gi].get();
-    const int64_t need = group_bytes(c, q.r.prec);
+    const int64_t need = group_bytes(c, q.r.prec, q.r.csw > 0);
     if (!sel && s->gauges + need > s->budget) {
       // no group of this precision fits even with every other group released
       RhsRef ref = *it;
@@ -206,9 +223,7 @@ void admit(nq_server* s, Engine* E, Admission* A)
       // a group of this kappa and precision that is merely full: wait for its slots
       // (or take a free index), never displace another group for it
       bool compat = false;
-      for (int i = 0; i < ng; i++)
-        compat |= E->groups[i] && !E->groups[i]->draining && E->groups[i]->kappa == q.r.kappa &&
-                  E->groups[i]->prec == q.r.prec;
+      for (int i = 0; i < ng; i++) compat |= E->groups[i] && !E->groups[i]->draining && E->groups[i]->same(q);
       int gi = -1;
       for (int i = 0; i < ng && gi < 0; i++)
         if (!E->groups[i]) gi = i;
@@ -224,6 +239,8 @@ void admit(nq_server* s, Engine* E, Admission* A)
         auto gp = std::make_unique<Group>();
         gp->kappa = q.r.kappa;
         gp->prec = q.r.prec;
+        gp->csw = q.r.csw;
+        gp->mu = q.r.mu;
         gp->bytes = need;
         s->used += need;
         sel = gp.get();
@@ -284,7 +301,18 @@ void engine_loop(nq_server* s, Engine* E)
     // new groups: device set-up (group index = its slot in the engine)
     for (int gi : A.fresh) {
       Group* gp = E->groups[gi].get();
-      int rc = gp->b.setup(c, E->g, gp->kappa, nullptr, gp->prec, gi);
+      nq_clover* cl = nullptr;
+      int rc = NQ_OK;
+      if (gp->csw > 0) {
+        auto it = E->clovers.find(gp->csw);
+        if (it == E->clovers.end()) {
+          rc = nq_clover_create(c, E->g, gp->csw, &cl);
+          if (rc == NQ_OK) E->clovers[gp->csw] = cl;
+        } else {
+          cl = it->second;
+        }
+      }
+      if (rc == NQ_OK) rc = gp->b.setup(c, E->g, gp->kappa, nullptr, gp->prec, gi, cl, gp->mu);
       for (int i = 0; rc == NQ_OK && i < TmBatch::kMaxSlots; i++) {
         rc = nq_field_create(c, NQ_SPINCOLOR, NQ_LX, &gp->src[i]);
         if (rc == NQ_OK) rc = nq_field_create(c, NQ_SPINCOLOR, NQ_LX, &gp->sol[i]);
@@ -399,6 +427,8 @@ int nq_server_submit(nq_server* s, const nq_serve_request* r, int64_t* id)
   if (r->src_kind != 0 && r->src_kind != 1) return fail(NQ_EINVAL, "unknown source kind %d", r->src_kind);
   if (r->prec < NQ_PREC_F64 || r->prec > NQ_PREC_F16H) return fail(NQ_EINVAL, "unknown precision %d", r->prec);
   if (!(r->residue > 0) || r->niter <= 0 || !(r->kappa != 0)) return fail(NQ_EINVAL, "bad kappa/residue/niter");
+  if (!(r->csw >= 0)) return fail(NQ_EINVAL, "bad cSW %g", r->csw);
+  if (r->csw > 0 && r->prec == NQ_PREC_F16H) return fail(NQ_EINVAL, "twisted clover: inner precision FP64, FP32 or FP16");
   auto q = std::make_shared<Req>();
   q->r = *r;
   for (int k = 0; k < r->nsrc; k++) {
@@ -462,6 +492,7 @@ void nq_server_destroy(nq_server* s)
     if (E->th.joinable()) E->th.join();
   for (auto& E : s->engines) {
     for (auto& gp : E->groups) gp.reset();
+    for (auto& kv : E->clovers) nq_clover_destroy(kv.second);
     nq_gauge_destroy(E->g);
     nq_ctx_destroy(E->ctx);
   }

@@ -14,6 +14,7 @@
 // FP64 true residual of each shifted system.
 
 #include <algorithm>
+#include <map>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -276,7 +277,20 @@ static int st_cgm_batched(nq_ctx* ctx, const LinkView& U, int nr, int nshift, co
     NQ_TRY(alloc_spinor(ctx, "cgm.t64", SP_F64, 3, &t64));
     NQ_TRY(alloc_spinor(ctx, "cgm.s64", SP_F64, 3, &s64));
   }
-  int chunk = 16, launched = 0;
+  // Chunks of 2^k iterations are captured once into HIP graphs and replayed: every
+  // launch of an iteration has fixed arguments (this solve's vectors and states; the
+  // scalars live on the device).  The first chunk runs eagerly, and so does every chunk
+  // while the launch ledger is on; NQ_GRAPHS=0 switches the graphs off.
+  struct Graphs {
+    std::map<int, hipGraphExec_t> ex;
+    ~Graphs()
+    {
+      for (auto& kv : ex) (void)hipGraphExecDestroy(kv.second);
+    }
+  } graphs;
+  const char* genv = getenv("NQ_GRAPHS");
+  const bool use_graphs = !(genv && genv[0] == '0');
+  int chunk = ctx->profiling ? 2 : 16, launched = 0;
   for (;;) {
     int nbusy = 0, nrun = 0;  // as of the last poll (the ledger's byte count)
     for (int i = 0; i < nr; i++)
@@ -284,8 +298,38 @@ static int st_cgm_batched(nq_ctx* ctx, const LinkView& U, int nr, int nshift, co
         nbusy++;
         for (int k = 0; k < nshift; k++) nrun += hst[i].run[k] != 0;
       }
-    for (int it = 0; it < chunk; it++)
-      NQ_TRY(launch_cgm_iteration(ctx, U, p, t, r, sol, ps, csz, dst, nr, nbusy, nrun));
+    if (use_graphs && launched > 0 && !ctx->profiling) {
+      for (int k = 7, left = chunk; k >= 0; k--) {
+        const int n = 1 << k;
+        for (; left >= n; left -= n) {
+          auto it = graphs.ex.find(n);
+          if (it == graphs.ex.end()) {
+            hipGraph_t gr = nullptr;
+            NQ_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+            ctx->capturing = true;
+            int rc = NQ_OK;
+            for (int i = 0; i < n && rc == NQ_OK; i++)
+              rc = launch_cgm_iteration(ctx, U, p, t, r, sol, ps, csz, dst, nr, nbusy, nrun);
+            ctx->capturing = false;
+            const hipError_t e = hipStreamEndCapture(ctx->stream, &gr);
+            if (rc != NQ_OK) {
+              if (gr) (void)hipGraphDestroy(gr);
+              return rc;
+            }
+            if (e != hipSuccess) return fail(NQ_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
+            hipGraphExec_t ex = nullptr;
+            const hipError_t ei = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            if (ei != hipSuccess) return fail(NQ_EDEVICE, "graph instantiation failed: %s", hipGetErrorString(ei));
+            it = graphs.ex.emplace(n, ex).first;
+          }
+          NQ_HIP(hipGraphLaunch(it->second, ctx->stream));
+        }
+      }
+    } else {
+      for (int it = 0; it < chunk; it++)
+        NQ_TRY(launch_cgm_iteration(ctx, U, p, t, r, sol, ps, csz, dst, nr, nbusy, nrun));
+    }
     launched += chunk;
     NQ_HIP(hipMemcpyAsync(hst, dst, sizeof(CgmState) * nr, hipMemcpyDeviceToHost, ctx->stream));
     NQ_HIP(hipStreamSynchronize(ctx->stream));
@@ -349,7 +393,9 @@ static int st_cgm_batched(nq_ctx* ctx, const LinkView& U, int nr, int nshift, co
       if (slope < 0) need = std::max(need, (int)std::ceil(-worst / slope));
     }
     if (all || launched > niter + 64) break;
-    chunk = std::max(4, std::min(128, need));
+    // with the launch ledger on, short chunks keep its count of running shifts (taken
+    // at each poll) close to the shifts that actually run
+    chunk = ctx->profiling ? 2 : std::max(4, std::min(128, need));
   }
   if (mixed) NQ_TRY(launch_cgm_flush(ctx, *x64, sol, csz, nshift, dst, nr, 0));
   if (iters) {

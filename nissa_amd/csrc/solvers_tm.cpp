@@ -496,21 +496,150 @@ static int residual_kernsq_batched(const TmSetup& S, const double* coef, int nr,
   return launch_tm_hop(ctx, NQ_ODD, none, tmp, S.U, S.anis, e);
 }
 
+// ---------------------------------------------------------------- twisted clover operator
+// (the solvers follow in the twisted clover section below)
+static SpinorView clov_view(const nq_clover* cl, int par)
+{
+  SpinorView v;
+  v.data = cl->cl[par].p;
+  v.prec = SP_F64;
+  v.ncomp = 36;
+  return v;
+}
+
+// inverse twisted clover blocks of parity par for (kappa, mu), computed on first use
+static int clov_inv_view(nq_clover* cl, int par, double kappa, double mu, SpinorView* v)
+{
+  nq_ctx* c = cl->ctx;
+  if (!cl->inv[par].p) NQ_TRY(dev_alloc(cl->inv[par], (size_t)72 * 16 * c->geo.stride));
+  v->data = cl->inv[par].p;
+  v->prec = SP_F64;
+  v->ncomp = 72;
+  if (!cl->inv_valid[par] || cl->inv_kappa[par] != kappa || cl->inv_mu[par] != mu) {
+    NQ_TRY(launch_clover_invert(c, clov_view(cl, par), *v, kappa, mu));
+    cl->inv_valid[par] = 1;
+    cl->inv_kappa[par] = kappa;
+    cl->inv_mu[par] = mu;
+    cl->inv32_valid[par] = 0;
+  }
+  return NQ_OK;
+}
+
+// FP32 copies of the clover blocks and of the inverse blocks (for (kappa, mu))
+static int clov_view32(nq_clover* cl, int par, SpinorView* v)
+{
+  nq_ctx* c = cl->ctx;
+  if (!cl->cl32[par].p) NQ_TRY(dev_alloc(cl->cl32[par], (size_t)36 * 8 * c->geo.stride));
+  v->data = cl->cl32[par].p;
+  v->prec = SP_F32;
+  v->ncomp = 36;
+  if (!cl->cl32_valid[par]) {
+    NQ_TRY(launch_blocks_to_f32(c, 36, clov_view(cl, par), *v));
+    cl->cl32_valid[par] = 1;
+  }
+  return NQ_OK;
+}
+static int clov_inv_view32(nq_clover* cl, int par, double kappa, double mu, SpinorView* v)
+{
+  nq_ctx* c = cl->ctx;
+  SpinorView v64;
+  NQ_TRY(clov_inv_view(cl, par, kappa, mu, &v64));
+  if (!cl->inv32[par].p) NQ_TRY(dev_alloc(cl->inv32[par], (size_t)72 * 8 * c->geo.stride));
+  v->data = cl->inv32[par].p;
+  v->prec = SP_F32;
+  v->ncomp = 72;
+  if (!cl->inv32_valid[par]) {
+    NQ_TRY(launch_blocks_to_f32(c, 72, v64, *v));
+    cl->inv32_valid[par] = 1;
+  }
+  return NQ_OK;
+}
+
+struct ClovSetup {
+  TmSetup S;
+  nq_clover* cl;
+  SpinorView inv_e;  // inverse blocks of the even sites for (kappa, mu)
+  SpinorView cl_o;   // clover blocks of the odd sites
+  double mu;
+  SpinorView te2, uo;  // scratch: the even intermediate of K; odd (the FP64 check)
+};
+
+static int clov_setup(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis, double mu,
+                      ClovSetup* s)
+{
+  if (!cl || cl->ctx != c) return fail(NQ_EINVAL, "clover handle belongs to another context");
+  NQ_TRY(tm_setup(c, g, LK_F64_18, anis, kappa, &s->S));
+  s->cl = cl;
+  s->mu = mu;
+  NQ_TRY(clov_inv_view(cl, NQ_EVEN, kappa, mu, &s->inv_e));
+  s->cl_o = clov_view(cl, NQ_ODD);
+  NQ_TRY(alloc_spinor(c, "clov.te2", SP_F64, 12, &s->te2));
+  NQ_TRY(alloc_spinor(c, "clov.uo", SP_F64, 12, &s->uo));
+  return NQ_OK;
+}
+
+// the same operator in a sloppy precision: FP32 (FP32 18-real links and FP32 spinors)
+// or FP16 (FP16 spinors with per-site norms and FP32 12-real links), FP32 blocks
+static int clov_setup_sloppy(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis,
+                             double mu, int sp, ClovSetup* s)
+{
+  NQ_TRY(tm_setup(c, g, sp == SP_F16 ? LK_F32_12 : LK_F32_18, anis, kappa, &s->S));
+  s->cl = cl;
+  s->mu = mu;
+  NQ_TRY(clov_inv_view32(cl, NQ_EVEN, kappa, mu, &s->inv_e));
+  NQ_TRY(clov_view32(cl, NQ_ODD, &s->cl_o));
+  const bool h = sp == SP_F16;
+  NQ_TRY(alloc_spinor(c, h ? "clov.te2_16" : "clov.te2_32", sp, 12, &s->te2));
+  return NQ_OK;
+}
+
+static const double kOne[2][2] = {{1, 0}, {1, 0}};
+
+// K(dag): out_odd from in_odd; `last` carries the epilogue mode (0 write, 1 norm,
+// 2 CG tail) of the second hop, whose c1/c2/aux are set here
+// Two hop launches: the first applies the inverse blocks of the even sites to the
+// hop in its epilogue (te2 = D_ee(dag)^-1 H in), the second the D_oo(dag) blocks to
+// `in` read at its own site (out = g5 (D_oo in - 1/4 H te2)), so neither the hop nor
+// the D_oo product goes through memory.
+static int clov_kern(const ClovSetup& s, int dag, const SpinorView& out, const SpinorView& in, HopEpi last)
+{
+  nq_ctx* c = s.S.ctx;
+  HopEpi h;
+  h.blk = s.inv_e;
+  h.blk_mode = 1;
+  h.blk_dag = dag;
+  h.blk_kappa = s.S.kappa;
+  h.blk_mu = s.mu;
+  NQ_TRY(launch_tm_hop(c, NQ_EVEN, s.te2, in, s.S.U, s.S.anis, h));
+  setc(last.c1, -0.25, 0.25);
+  setc(last.c2, 1.0, -1.0);
+  last.aux = in;
+  last.blk = s.cl_o;
+  last.blk_mode = 2;
+  last.blk_dag = dag;
+  last.blk_kappa = s.S.kappa;
+  last.blk_mu = s.mu;
+  return launch_tm_hop(c, NQ_ODD, out, s.te2, s.S.U, s.S.anis, last);
+}
+
 // ---------------------------------------------------------------- TmBatch
-int64_t TmBatch::bytes_per_slot(const Geo& g, int prec)
+int64_t TmBatch::bytes_per_slot(const Geo& g, int prec, bool clover)
 {
   const int64_t s = g.stride;
+  const int64_t sloppy = prec == NQ_PREC_F32 ? 12 * 8 * s : 12 * 4 * s + 4 * s;
   int64_t b = 5 * 12 * 16 * s;  // b, x64, r64, t64, tmp64
   if (prec != NQ_PREC_F64) {
-    b += 2 * 12 * 8 * s;                                                       // xs, rs
-    b += 3 * (prec == NQ_PREC_F32 ? 12 * 8 * s : 12 * 4 * s + 4 * s);          // p, t1, tmp
+    b += 2 * 12 * 8 * s;  // xs, rs
+    b += 3 * sloppy;      // p, t1, tmp
   } else {
     b += 12 * 16 * s;  // p
   }
+  if (clover) b += 2 * 12 * 16 * s + (prec != NQ_PREC_F64 ? 2 * sloppy : 0);  // te2, uo (FP64 and sloppy)
   return b;
 }
 
-int TmBatch::setup(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis, int prec, int group)
+int TmBatch::setup(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis, int prec, int group,
+                   nq_clover* cl, double clover_mu)
 {
   if (group < 0 || group >= kMaxGroups) return fail(NQ_EINVAL, "batch group %d out of range", group);
   NQ_TRY(lk_of_prec(prec, &sp_, &lk_));
@@ -524,7 +653,7 @@ int TmBatch::setup(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* a
   mixed_ = prec != NQ_PREC_F64;
   S_ = ctx->state + 16 * group;
   TmSetup chk;
-  NQ_TRY(tm_setup(ctx, g, lk64_for(prec, g), anis, kappa, &chk));  // validates kappa / builds links
+  NQ_TRY(tm_setup(ctx, g, cl ? LK_F64_18 : lk64_for(prec, g), anis, kappa, &chk));  // validates kappa / builds links
   if (mixed_) NQ_TRY(tm_setup(ctx, g, lk_, anis, kappa, &chk));
   const std::string t = "tb" + std::to_string(group) + ".";
   const int n = kMaxSlots;
@@ -548,6 +677,38 @@ int TmBatch::setup(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* a
     rs_ = r64_;
     t1s_ = t64_;
     tmps_ = tmp64_;
+  }
+  cl_ = cl;
+  clmu_ = clover_mu;
+  if (cl) {
+    // the group's inverse blocks of the even sites for (kappa, mu) (the clover handle
+    // caches one pair only, and groups of other masses may share the handle)
+    if (cl->ctx != ctx) return fail(NQ_EINVAL, "clover handle belongs to another context");
+    void* p = nullptr;
+    NQ_TRY(ctx_temp(ctx, t + "clinv", (size_t)72 * 16 * ctx->geo.stride, &p));
+    clinv64_.data = p;
+    clinv64_.prec = SP_F64;
+    clinv64_.ncomp = 72;
+    NQ_TRY(launch_clover_invert(ctx, clov_view(cl, NQ_EVEN), clinv64_, kappa, clover_mu));
+    clo64_ = clov_view(cl, NQ_ODD);
+    NQ_TRY(alloc_spinor_nr(ctx, t + "cte2", SP_F64, 12, n, &te2_64_));
+    NQ_TRY(alloc_spinor_nr(ctx, t + "cuo", SP_F64, 12, n, &uo64_));
+    NQ_TRY(alloc_spinor(ctx, t + "cte1", SP_F64, 12, &cte2_));
+    if (mixed_) {
+      NQ_TRY(ctx_temp(ctx, t + "clinv32", (size_t)72 * 8 * ctx->geo.stride, &p));
+      clinv_s_.data = p;
+      clinv_s_.prec = SP_F32;
+      clinv_s_.ncomp = 72;
+      NQ_TRY(launch_blocks_to_f32(ctx, 72, clinv64_, clinv_s_));
+      NQ_TRY(clov_view32(cl, NQ_ODD, &clo_s_));
+      NQ_TRY(alloc_spinor_nr(ctx, t + "cte2_" + std::to_string(sp_), sp_, 12, n, &te2s_));
+      NQ_TRY(alloc_spinor_nr(ctx, t + "cuo_" + std::to_string(sp_), sp_, 12, n, &uos_));
+    } else {
+      clinv_s_ = clinv64_;
+      clo_s_ = clo64_;
+      te2s_ = te2_64_;
+      uos_ = uo64_;
+    }
   }
   // every slot starts idle
   for (int i = 0; i < n; i++) {
@@ -581,15 +742,31 @@ int TmBatch::upload_row(int slot, double mu)
 {
   SpinorView none;
   double* h = hcoef_.data();
-  put_coef(&h[(TB_INVDEE_M * 16 + slot) * 12], epi_inv_dee(kappa_, -mu));
-  put_coef(&h[(TB_KOUT_M * 16 + slot) * 12], epi_kout(kappa_, -mu, none));
-  put_coef(&h[(TB_INVDEE_P * 16 + slot) * 12], epi_inv_dee(kappa_, mu));
-  put_coef(&h[(TB_KOUT_P * 16 + slot) * 12], epi_kout(kappa_, mu, none));
-  HopEpi e = epi_kout(kappa_, mu, none);
-  for (int a = 0; a < 2; a++)
-    for (int k = 0; k < 2; k++) { e.c1[a][k] = -e.c1[a][k]; e.c2[a][k] = -e.c2[a][k]; }
-  setc(e.c3, 1.0, 1.0);
-  put_coef(&h[(TB_RES * 16 + slot) * 12], e);
+  if (cl_) {
+    // twisted clover: the mass sits in the blocks; the hop of D_ee^-1 H is plain
+    // (c1 = 1), the K out hop forms g5 (uo - 1/4 hop), the residual b - g5 (uo - 1/4 hop)
+    HopEpi plain, kout, res;
+    setc(kout.c1, -0.25, 0.25);
+    setc(kout.c2, 1.0, -1.0);
+    setc(res.c1, 0.25, -0.25);
+    setc(res.c2, -1.0, 1.0);
+    setc(res.c3, 1.0, 1.0);
+    put_coef(&h[(TB_INVDEE_M * 16 + slot) * 12], plain);
+    put_coef(&h[(TB_KOUT_M * 16 + slot) * 12], kout);
+    put_coef(&h[(TB_INVDEE_P * 16 + slot) * 12], plain);
+    put_coef(&h[(TB_KOUT_P * 16 + slot) * 12], kout);
+    put_coef(&h[(TB_RES * 16 + slot) * 12], res);
+  } else {
+    put_coef(&h[(TB_INVDEE_M * 16 + slot) * 12], epi_inv_dee(kappa_, -mu));
+    put_coef(&h[(TB_KOUT_M * 16 + slot) * 12], epi_kout(kappa_, -mu, none));
+    put_coef(&h[(TB_INVDEE_P * 16 + slot) * 12], epi_inv_dee(kappa_, mu));
+    put_coef(&h[(TB_KOUT_P * 16 + slot) * 12], epi_kout(kappa_, mu, none));
+    HopEpi e = epi_kout(kappa_, mu, none);
+    for (int a = 0; a < 2; a++)
+      for (int k = 0; k < 2; k++) { e.c1[a][k] = -e.c1[a][k]; e.c2[a][k] = -e.c2[a][k]; }
+    setc(e.c3, 1.0, 1.0);
+    put_coef(&h[(TB_RES * 16 + slot) * 12], e);
+  }
   for (int t = 0; t < TB_N; t++)
     NQ_HIP(hipMemcpyAsync(coef_ + (t * 16 + slot) * 12, &h[(t * 16 + slot) * 12], sizeof(double) * 12,
                           hipMemcpyHostToDevice, ctx_->stream));
@@ -611,18 +788,24 @@ int TmBatch::start(int slot, double mu, double residue, int niter, const SpinorV
   if (slot < 0 || slot >= kMaxSlots) return fail(NQ_EINVAL, "slot %d out of range", slot);
   if (active_[slot]) return fail(NQ_EINVAL, "slot %d is busy", slot);
   if (niter <= 0 || !(residue > 0)) return fail(NQ_EINVAL, "bad niter/residue");
+  if (cl_ && mu != clmu_) return fail(NQ_EINVAL, "twisted clover batch: mu %g differs from the group's %g", mu, clmu_);
+  if (cl_ && guess) return fail(NQ_EINVAL, "twisted clover batch: no guess");
   nq_ctx* c = ctx_;
   TmSetup S;
-  NQ_TRY(tm_setup(c, g_, lk64_for(prec_, g_), has_anis_ ? &anis_ : nullptr, kappa_, &S));
+  NQ_TRY(tm_setup(c, g_, lk64(), has_anis_ ? &anis_ : nullptr, kappa_, &S));
   NQ_TRY(upload_row(slot, mu));
   mu_[slot] = mu;
   const double a = 1 / (2 * kappa_), n = a * a + mu * mu;
   SpinorView none;
   none.ncomp = 12;
   // eq. 8a/8b: b = g5 (eta_o + 1/2 H_oe D_ee^-1 eta_e)
-  double ca[2][2], cb[2][2] = {{0, 0}, {0, 0}};
-  setc(ca, cplx(a / n, -mu / n), cplx(a / n, mu / n));
-  NQ_TRY(launch_spin_diag(c, et_, ca, src_e, cb, none));
+  if (cl_) {
+    NQ_TRY(launch_clover_apply(c, 1, et_, src_e, clinv64_, none, kappa_, mu, 0, kOne, nullptr));
+  } else {
+    double ca[2][2], cb[2][2] = {{0, 0}, {0, 0}};
+    setc(ca, cplx(a / n, -mu / n), cplx(a / n, mu / n));
+    NQ_TRY(launch_spin_diag(c, et_, ca, src_e, cb, none));
+  }
   HopEpi e8;
   setc(e8.c1, 0.5, -0.5);
   setc(e8.c2, 1.0, -1.0);
@@ -658,6 +841,23 @@ int TmBatch::start(int slot, double mu, double residue, int niter, const SpinorV
   return NQ_OK;
 }
 
+int TmBatch::lk64() const { return cl_ ? LK_F64_18 : lk64_for(prec_, g_); }
+
+// K(dag) in = g5 (D_oo(dag) in - 1/4 H D_ee(dag)^-1 H in) on nr slots: a plain hop, the
+// batched inverse blocks, the batched D_oo blocks of `in`, and the K out hop whose
+// epilogue (last: its coefficient rows, mode and tail) combines them
+int TmBatch::clov_kern_batched(const LinkView& U, double anis, int dag, const SpinorView& out, const SpinorView& in,
+                               const SpinorView& tmp, const SpinorView& te2, const SpinorView& uo,
+                               const SpinorView& inv, const SpinorView& clo, HopEpi last, int nr, int pred)
+{
+  nq_ctx* c = ctx_;
+  NQ_TRY(launch_tm_hop(c, NQ_EVEN, tmp, in, U, anis, bepi(coef_, dag ? TB_INVDEE_M : TB_INVDEE_P, nr, last.st, pred)));
+  NQ_TRY(launch_clover_apply_batched(c, 1, dag, te2, tmp, inv, kappa_, clmu_, nr, last.st, pred));
+  NQ_TRY(launch_clover_apply_batched(c, 0, dag, uo, in, clo, kappa_, clmu_, nr, last.st, pred));
+  last.aux = uo;
+  return launch_tm_hop(c, NQ_ODD, out, te2, U, anis, last);
+}
+
 int TmBatch::enqueue(int nr, int iters)
 {
   nq_ctx* c = ctx_;
@@ -666,6 +866,25 @@ int TmBatch::enqueue(int nr, int iters)
   const int pred = mixed_ ? 2 : 0;
   SpinorView none;
   none.ncomp = 12;
+  if (cl_) {
+    // twisted clover: K(1) p (|.|^2 = <p, K K^+ p>), then K(0) with the CG tail
+    for (int it = 0; it < iters; it++) {
+      HopEpi b1 = bepi(coef_, TB_KOUT_M, nr, S_, pred);
+      b1.mode = 1;
+      b1.fin = FIN_PAP;
+      NQ_TRY(clov_kern_batched(Ss.U, Ss.anis, 1, t1s_, p_, tmps_, te2s_, uos_, clinv_s_, clo_s_, b1, nr, pred));
+      HopEpi b2e = bepi(coef_, TB_KOUT_P, nr, S_, pred);
+      b2e.mode = 2;
+      b2e.fin = mixed_ ? FIN_RR_SLOPPY : FIN_RR;
+      b2e.p = p_;
+      b2e.r = rs_;
+      if (!mixed_) b2e.x = xs_;
+      NQ_TRY(clov_kern_batched(Ss.U, Ss.anis, 0, none, t1s_, tmps_, te2s_, uos_, clinv_s_, clo_s_, b2e, nr, pred));
+      if (mixed_) NQ_TRY(launch_cg_pupdate(c, p_, rs_, S_, nr, 1, xs_));
+      else NQ_TRY(launch_cg_pupdate(c, p_, rs_, S_, nr));
+    }
+    return NQ_OK;
+  }
   for (int it = 0; it < iters; it++) {
     NQ_TRY(launch_tm_hop(c, NQ_EVEN, tmps_, p_, Ss.U, Ss.anis, bepi(coef_, TB_INVDEE_M, nr, S_, pred)));
     HopEpi b1 = bepi(coef_, TB_KOUT_M, nr, S_, pred);
@@ -766,9 +985,24 @@ int TmBatch::poll(std::vector<int>* finished)
     // reliable update of the flagged slots: x64 += xs + alpha p, xs = 0; r64 = b - A x64
     // in FP64, rs = r64; p = rs + beta p; then clear the flags
     TmSetup S64;
-    NQ_TRY(tm_setup(c, g_, lk64_for(prec_, g_), has_anis_ ? &anis_ : nullptr, kappa_, &S64));
+    NQ_TRY(tm_setup(c, g_, lk64(), has_anis_ ? &anis_ : nullptr, kappa_, &S64));
     NQ_TRY(launch_reliable_accum(c, x64_, xs_, S_, nr, p_));
-    NQ_TRY(residual_kernsq_batched(S64, coef_, nr, r64_, rs_, x64_, b_, t64_, tmp64_, S_, FIN_RR_TRUE_HOLD, 1));
+    if (cl_) {
+      // r64 = b - K(0) K(1) x64 (FP64 blocks and links), rs = r64
+      NQ_TRY(clov_kern_batched(S64.U, S64.anis, 1, t64_, x64_, tmp64_, te2_64_, uo64_, clinv64_, clo64_,
+                               bepi(coef_, TB_KOUT_M, nr, S_, 1), nr, 1));
+      HopEpi e = bepi(coef_, TB_RES, nr, S_, 1);
+      e.aux2 = b_;
+      e.mode = 3;
+      e.r = r64_;
+      e.rs = rs_;
+      e.fin = FIN_RR_TRUE_HOLD;
+      SpinorView none;
+      none.ncomp = 12;
+      NQ_TRY(clov_kern_batched(S64.U, S64.anis, 0, none, t64_, tmp64_, te2_64_, uo64_, clinv64_, clo64_, e, nr, 1));
+    } else {
+      NQ_TRY(residual_kernsq_batched(S64, coef_, nr, r64_, rs_, x64_, b_, t64_, tmp64_, S_, FIN_RR_TRUE_HOLD, 1));
+    }
     NQ_TRY(launch_cg_pupdate(c, p_, rs_, S_, nr, 2));
     NQ_TRY(launch_clear_reliable(c, S_, nr));
     NQ_TRY(get_states(c, base, nr));
@@ -812,17 +1046,38 @@ int TmBatch::finish(int slot, const SpinorView& src_e, const SpinorView& src_o, 
   }
   const double mu = mu_[slot];
   TmSetup S;
-  NQ_TRY(tm_setup(c, g_, lk64_for(prec_, g_), has_anis_ ? &anis_ : nullptr, kappa_, &S));
+  NQ_TRY(tm_setup(c, g_, lk64(), has_anis_ ? &anis_ : nullptr, kappa_, &S));
   const double a = 1 / (2 * kappa_), n = a * a + mu * mu;
   const cplx zi_u(a / n, -mu / n), zi_l(a / n, mu / n);
   if (guess_out) NQ_TRY(launch_convert(c, *guess_out, slot_view(x64_, slot)));
-  // eq. 10: sol_o = K(-mu) y ; eq. 11: sol_e = D_ee^-1 (eta_e + 1/2 H_eo sol_o)
-  NQ_TRY(apply_kern(S, -mu, sol_o, slot_view(x64_, slot), sol_e));
-  HopEpi e11;
-  setc(e11.c1, 0.5 * zi_u, 0.5 * zi_l);
-  setc(e11.c2, zi_u, zi_l);
-  e11.aux = src_e;
-  NQ_TRY(launch_tm_hop(c, NQ_EVEN, sol_e, sol_o, S.U, S.anis, e11));
+  SpinorView none;
+  none.ncomp = 12;
+  if (cl_) {
+    // eq. 10: sol_o = K(1) y ; eq. 11: sol_e = D_ee^-1 (eta_e + 1/2 H_eo sol_o), with the
+    // group's inverse blocks (cg_invert_tmclovD_eoprec.cpp:200-225)
+    ClovSetup cs;
+    cs.S = S;
+    cs.cl = cl_;
+    cs.inv_e = clinv64_;
+    cs.cl_o = clo64_;
+    cs.mu = clmu_;
+    cs.te2 = cte2_;
+    NQ_TRY(clov_kern(cs, 1, sol_o, slot_view(x64_, slot), HopEpi()));
+    HopEpi e11;
+    setc(e11.c1, 0.5, 0.5);
+    setc(e11.c2, 1.0, 1.0);
+    e11.aux = src_e;
+    NQ_TRY(launch_tm_hop(c, NQ_EVEN, cte2_, sol_o, S.U, S.anis, e11));
+    NQ_TRY(launch_clover_apply(c, 1, sol_e, cte2_, clinv64_, none, kappa_, mu, 0, kOne, nullptr));
+  } else {
+    // eq. 10: sol_o = K(-mu) y ; eq. 11: sol_e = D_ee^-1 (eta_e + 1/2 H_eo sol_o)
+    NQ_TRY(apply_kern(S, -mu, sol_o, slot_view(x64_, slot), sol_e));
+    HopEpi e11;
+    setc(e11.c1, 0.5 * zi_u, 0.5 * zi_l);
+    setc(e11.c2, zi_u, zi_l);
+    e11.aux = src_e;
+    NQ_TRY(launch_tm_hop(c, NQ_EVEN, sol_e, sol_o, S.U, S.anis, e11));
+  }
   // FP64 check with the 18-real links: |D sol - src|^2 / |src|^2
   TmSetup Sc;
   NQ_TRY(tm_setup(c, g_, LK_F64_18, has_anis_ ? &anis_ : nullptr, kappa_, &Sc));
@@ -831,14 +1086,19 @@ int TmBatch::finish(int slot, const SpinorView& src_e, const SpinorView& src_o, 
   c->state_host[49 + 2 * group_] = CgState{};
   NQ_HIP(hipMemcpyAsync(sc, c->state_host + 48 + 2 * group_, 2 * sizeof(CgState), hipMemcpyHostToDevice, c->stream));
   const SpinorView sol_v[2] = {sol_e, sol_o}, src_v[2] = {src_e, src_o};
-  SpinorView none;
-  none.ncomp = 12;
   for (int p = 0; p < 2; p++) {
     HopEpi ec;
     setc(ec.c1, -0.5, -0.5);
-    setc(ec.c2, cplx(a, mu), cplx(a, -mu));
     setc(ec.c3, -1.0, -1.0);
-    ec.aux = sol_v[p];
+    if (cl_) {
+      // (z + Cl) sol_p - 1/2 hop sol_!p - src_p
+      NQ_TRY(launch_clover_apply(c, 0, cte2_, sol_v[p], clov_view(cl_, p), none, kappa_, mu, 0, kOne, nullptr));
+      setc(ec.c2, 1.0, 1.0);
+      ec.aux = cte2_;
+    } else {
+      setc(ec.c2, cplx(a, mu), cplx(a, -mu));
+      ec.aux = sol_v[p];
+    }
     ec.aux2 = src_v[p];
     ec.mode = 1;
     ec.write = 0;
@@ -1088,6 +1348,67 @@ int check_field(const nq_ctx* c, const nq_field* f, int kind, int cov, const cha
   if (f->ctx != c) return fail(NQ_EINVAL, "%s: field belongs to another context", what);
   if (f->kind != kind || f->coverage != cov) return fail(NQ_EINVAL, "%s: wrong field kind/coverage", what);
   return NQ_OK;
+}
+
+int check_multi(const nq_ctx* c, int nrhs, int niter, double residue, nq_field** sols, nq_field** guesses,
+                const nq_field* const* srcs, const char* what)
+{
+  if (nrhs < 1) return fail(NQ_EINVAL, "need at least one source");
+  if (niter <= 0 || !(residue > 0)) return fail(NQ_EINVAL, "bad niter/residue");
+  for (int i = 0; i < nrhs; i++) {
+    NQ_TRY(check_field(c, srcs[i], NQ_SPINCOLOR, NQ_LX, what));
+    NQ_TRY(check_field(c, sols[i], NQ_SPINCOLOR, NQ_LX, what));
+    if (guesses && guesses[i]) NQ_TRY(check_field(c, guesses[i], NQ_SPINCOLOR, NQ_ODD, what));
+    for (int j = 0; j < nrhs; j++)
+      if (sols[i] == srcs[j]) return fail(NQ_EINVAL, "solution aliases a source");
+  }
+  return NQ_OK;
+}
+
+// Streams nrhs sources through one batch: sources enter free slots as earlier ones
+// finish (eq. 8 and the CG start), the batch iterates, finished slots run eqs. 10-11
+// and the FP64 check.
+int run_batch_solves(nq_ctx* c, TmBatch& B, int nrhs, const double* mus, int niter, double residue,
+                     nq_field** sols, nq_field** guesses, const nq_field* const* srcs, nq_solve_stats* st)
+{
+  std::vector<nq_solve_stats> local(nrhs);
+  std::vector<int> owner(TmBatch::kMaxSlots, -1);
+  int rc = NQ_OK, next = 0, ndone = 0;
+  auto fill = [&]() -> int {
+    for (int s = 0; s < TmBatch::kMaxSlots && next < nrhs; s++)
+      if (!B.active(s)) {
+        SpinorView gv;
+        const bool hg = guesses && guesses[next];
+        if (hg) gv = view_of(guesses[next], 0);
+        NQ_TRY(B.start(s, mus[next], residue, niter, view_of(srcs[next], 0), view_of(srcs[next], 1),
+                       hg ? &gv : nullptr));
+        owner[s] = next++;
+      }
+    return NQ_OK;
+  };
+  NQ_TRY(fill());
+  while (ndone < nrhs) {
+    NQ_TRY(B.run(B.suggested_chunk()));
+    std::vector<int> fin;
+    NQ_TRY(B.poll(&fin));
+    for (int s : fin) {
+      const int i = owner[s];
+      SpinorView gv;
+      const bool hg = guesses && guesses[i];
+      if (hg) gv = view_of(guesses[i], 0);
+      const int r = B.finish(s, view_of(srcs[i], 0), view_of(srcs[i], 1), view_of(sols[i], 0), view_of(sols[i], 1),
+                             &local[i], nullptr, hg ? &gv : nullptr);
+      if (r == NQ_ENOCONV) rc = r;
+      else if (r != NQ_OK) return r;
+      owner[s] = -1;
+      ndone++;
+    }
+    NQ_TRY(fill());
+  }
+  NQ_HIP(hipStreamSynchronize(c->stream));
+  if (st)
+    for (int i = 0; i < nrhs; i++) st[i] = local[i];
+  return rc == NQ_ENOCONV ? fail(NQ_ENOCONV, "exit without converging") : NQ_OK;
 }
 }  // namespace
 
@@ -1391,58 +1712,11 @@ int nq_tm_solve_eoprec_multi(nq_ctx* c, const nq_gauge* g, double kappa, const d
                              nq_field** guesses, const nq_field* const* srcs, nq_solve_stats* st)
 {
   if (!c || !mus || !sols || !srcs) return fail(NQ_EINVAL, "null argument");
-  if (nrhs < 1) return fail(NQ_EINVAL, "need at least one source");
-  if (niter <= 0 || !(residue > 0)) return fail(NQ_EINVAL, "bad niter/residue");
-  for (int i = 0; i < nrhs; i++) {
-    NQ_TRY(check_field(c, srcs[i], NQ_SPINCOLOR, NQ_LX, "nq_tm_solve_eoprec_multi"));
-    NQ_TRY(check_field(c, sols[i], NQ_SPINCOLOR, NQ_LX, "nq_tm_solve_eoprec_multi"));
-    if (guesses && guesses[i])
-      NQ_TRY(check_field(c, guesses[i], NQ_SPINCOLOR, NQ_ODD, "nq_tm_solve_eoprec_multi (guess_Koo)"));
-    for (int j = 0; j < nrhs; j++)
-      if (sols[i] == srcs[j]) return fail(NQ_EINVAL, "solution aliases a source");
-  }
+  NQ_TRY(check_multi(c, nrhs, niter, residue, sols, guesses, srcs, "nq_tm_solve_eoprec_multi"));
   Guard G(c);
   TmBatch B;
   NQ_TRY(B.setup(c, g, kappa, anis, prec, 0));
-  std::vector<nq_solve_stats> local(nrhs);
-  std::vector<int> owner(TmBatch::kMaxSlots, -1);
-  int rc = NQ_OK, next = 0, ndone = 0;
-  // sources enter free slots as earlier ones finish
-  auto fill = [&]() -> int {
-    for (int s = 0; s < TmBatch::kMaxSlots && next < nrhs; s++)
-      if (!B.active(s)) {
-        SpinorView gv;
-        const bool hg = guesses && guesses[next];
-        if (hg) gv = view_of(guesses[next], 0);
-        NQ_TRY(B.start(s, mus[next], residue, niter, view_of(srcs[next], 0), view_of(srcs[next], 1),
-                       hg ? &gv : nullptr));
-        owner[s] = next++;
-      }
-    return NQ_OK;
-  };
-  NQ_TRY(fill());
-  while (ndone < nrhs) {
-    NQ_TRY(B.run(B.suggested_chunk()));
-    std::vector<int> fin;
-    NQ_TRY(B.poll(&fin));
-    for (int s : fin) {
-      const int i = owner[s];
-      SpinorView gv;
-      const bool hg = guesses && guesses[i];
-      if (hg) gv = view_of(guesses[i], 0);
-      const int r = B.finish(s, view_of(srcs[i], 0), view_of(srcs[i], 1), view_of(sols[i], 0), view_of(sols[i], 1),
-                             &local[i], nullptr, hg ? &gv : nullptr);
-      if (r == NQ_ENOCONV) rc = r;
-      else if (r != NQ_OK) return r;
-      owner[s] = -1;
-      ndone++;
-    }
-    NQ_TRY(fill());
-  }
-  NQ_HIP(hipStreamSynchronize(c->stream));
-  if (st)
-    for (int i = 0; i < nrhs; i++) st[i] = local[i];
-  return rc == NQ_ENOCONV ? fail(NQ_ENOCONV, "exit without converging") : NQ_OK;
+  return run_batch_solves(c, B, nrhs, mus, niter, residue, sols, guesses, srcs, st);
 }
 
 int nq_tmQ2_cgm(nq_ctx* c, const nq_gauge* g, double kappa, int nmass, const double* masses, int niter,
@@ -1594,130 +1868,6 @@ int nq_bench_tm_hop_multi(nq_ctx* c, const nq_gauge* g, int prec, int nrhs, int 
 // D_oo blocks and combines; K K^+ = K(0) K(1) (tmclovDkern_eoprec_square_eos).
 
 namespace nq {
-
-static SpinorView clov_view(const nq_clover* cl, int par)
-{
-  SpinorView v;
-  v.data = cl->cl[par].p;
-  v.prec = SP_F64;
-  v.ncomp = 36;
-  return v;
-}
-
-// inverse twisted clover blocks of parity par for (kappa, mu), computed on first use
-static int clov_inv_view(nq_clover* cl, int par, double kappa, double mu, SpinorView* v)
-{
-  nq_ctx* c = cl->ctx;
-  if (!cl->inv[par].p) NQ_TRY(dev_alloc(cl->inv[par], (size_t)72 * 16 * c->geo.stride));
-  v->data = cl->inv[par].p;
-  v->prec = SP_F64;
-  v->ncomp = 72;
-  if (!cl->inv_valid[par] || cl->inv_kappa[par] != kappa || cl->inv_mu[par] != mu) {
-    NQ_TRY(launch_clover_invert(c, clov_view(cl, par), *v, kappa, mu));
-    cl->inv_valid[par] = 1;
-    cl->inv_kappa[par] = kappa;
-    cl->inv_mu[par] = mu;
-    cl->inv32_valid[par] = 0;
-  }
-  return NQ_OK;
-}
-
-// FP32 copies of the clover blocks and of the inverse blocks (for (kappa, mu))
-static int clov_view32(nq_clover* cl, int par, SpinorView* v)
-{
-  nq_ctx* c = cl->ctx;
-  if (!cl->cl32[par].p) NQ_TRY(dev_alloc(cl->cl32[par], (size_t)36 * 8 * c->geo.stride));
-  v->data = cl->cl32[par].p;
-  v->prec = SP_F32;
-  v->ncomp = 36;
-  if (!cl->cl32_valid[par]) {
-    NQ_TRY(launch_blocks_to_f32(c, 36, clov_view(cl, par), *v));
-    cl->cl32_valid[par] = 1;
-  }
-  return NQ_OK;
-}
-static int clov_inv_view32(nq_clover* cl, int par, double kappa, double mu, SpinorView* v)
-{
-  nq_ctx* c = cl->ctx;
-  SpinorView v64;
-  NQ_TRY(clov_inv_view(cl, par, kappa, mu, &v64));
-  if (!cl->inv32[par].p) NQ_TRY(dev_alloc(cl->inv32[par], (size_t)72 * 8 * c->geo.stride));
-  v->data = cl->inv32[par].p;
-  v->prec = SP_F32;
-  v->ncomp = 72;
-  if (!cl->inv32_valid[par]) {
-    NQ_TRY(launch_blocks_to_f32(c, 72, v64, *v));
-    cl->inv32_valid[par] = 1;
-  }
-  return NQ_OK;
-}
-
-struct ClovSetup {
-  TmSetup S;
-  nq_clover* cl;
-  SpinorView inv_e;  // inverse blocks of the even sites for (kappa, mu)
-  SpinorView cl_o;   // clover blocks of the odd sites
-  double mu;
-  SpinorView te2, uo;  // scratch: the even intermediate of K; odd (the FP64 check)
-};
-
-static int clov_setup(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis, double mu,
-                      ClovSetup* s)
-{
-  if (!cl || cl->ctx != c) return fail(NQ_EINVAL, "clover handle belongs to another context");
-  NQ_TRY(tm_setup(c, g, LK_F64_18, anis, kappa, &s->S));
-  s->cl = cl;
-  s->mu = mu;
-  NQ_TRY(clov_inv_view(cl, NQ_EVEN, kappa, mu, &s->inv_e));
-  s->cl_o = clov_view(cl, NQ_ODD);
-  NQ_TRY(alloc_spinor(c, "clov.te2", SP_F64, 12, &s->te2));
-  NQ_TRY(alloc_spinor(c, "clov.uo", SP_F64, 12, &s->uo));
-  return NQ_OK;
-}
-
-// the same operator in a sloppy precision: FP32 (FP32 18-real links and FP32 spinors)
-// or FP16 (FP16 spinors with per-site norms and FP32 12-real links), FP32 blocks
-static int clov_setup_sloppy(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis,
-                             double mu, int sp, ClovSetup* s)
-{
-  NQ_TRY(tm_setup(c, g, sp == SP_F16 ? LK_F32_12 : LK_F32_18, anis, kappa, &s->S));
-  s->cl = cl;
-  s->mu = mu;
-  NQ_TRY(clov_inv_view32(cl, NQ_EVEN, kappa, mu, &s->inv_e));
-  NQ_TRY(clov_view32(cl, NQ_ODD, &s->cl_o));
-  const bool h = sp == SP_F16;
-  NQ_TRY(alloc_spinor(c, h ? "clov.te2_16" : "clov.te2_32", sp, 12, &s->te2));
-  return NQ_OK;
-}
-
-static const double kOne[2][2] = {{1, 0}, {1, 0}};
-
-// K(dag): out_odd from in_odd; `last` carries the epilogue mode (0 write, 1 norm,
-// 2 CG tail) of the second hop, whose c1/c2/aux are set here
-// Two hop launches: the first applies the inverse blocks of the even sites to the
-// hop in its epilogue (te2 = D_ee(dag)^-1 H in), the second the D_oo(dag) blocks to
-// `in` read at its own site (out = g5 (D_oo in - 1/4 H te2)), so neither the hop nor
-// the D_oo product goes through memory.
-static int clov_kern(const ClovSetup& s, int dag, const SpinorView& out, const SpinorView& in, HopEpi last)
-{
-  nq_ctx* c = s.S.ctx;
-  HopEpi h;
-  h.blk = s.inv_e;
-  h.blk_mode = 1;
-  h.blk_dag = dag;
-  h.blk_kappa = s.S.kappa;
-  h.blk_mu = s.mu;
-  NQ_TRY(launch_tm_hop(c, NQ_EVEN, s.te2, in, s.S.U, s.S.anis, h));
-  setc(last.c1, -0.25, 0.25);
-  setc(last.c2, 1.0, -1.0);
-  last.aux = in;
-  last.blk = s.cl_o;
-  last.blk_mode = 2;
-  last.blk_dag = dag;
-  last.blk_kappa = s.S.kappa;
-  last.blk_mu = s.mu;
-  return launch_tm_hop(c, NQ_ODD, out, s.te2, s.S.U, s.S.anis, last);
-}
 
 // CG on K K^+ (cg_invert semantics, cg_invert_template_threaded.hpp:19-150); x holds
 // the guess when use_guess
@@ -2229,6 +2379,22 @@ int nq_tmclov_solve_eoprec(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double k
   }
   NQ_HIP(hipStreamSynchronize(c->stream));
   return rc == NQ_ENOCONV ? fail(NQ_ENOCONV, "exit without converging") : NQ_OK;
+}
+
+// inv_tmclovD_cg_eoprec for several sources at one (kappa, mu, cSW) (the propagator
+// driver's 12 spin-colour sources, projects/ib/prop.cpp:93-96): the K K^+ CG batched
+// over up to 16 sources per group, sources streaming through the slots
+int nq_tmclov_solve_eoprec_multi(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis,
+                                 double mu, int nrhs, int niter, double residue, int prec, nq_field** sols,
+                                 const nq_field* const* srcs, nq_solve_stats* st)
+{
+  if (!c || !cl || !sols || !srcs) return fail(NQ_EINVAL, "null argument");
+  NQ_TRY(check_multi(c, nrhs, niter, residue, sols, nullptr, srcs, "nq_tmclov_solve_eoprec_multi"));
+  Guard G(c);
+  TmBatch B;
+  NQ_TRY(B.setup(c, g, kappa, anis, prec, 0, cl, mu));
+  std::vector<double> mus(nrhs, mu);
+  return run_batch_solves(c, B, nrhs, mus.data(), niter, residue, sols, nullptr, srcs, st);
 }
 
 }  // extern "C"

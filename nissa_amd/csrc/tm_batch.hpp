@@ -9,6 +9,12 @@
 // boundary (continuous batching).  The arithmetic of a slot does not depend on the
 // other slots or on its slot index, so a source gives the same bits alone and in
 // any mix.
+//
+// With a clover handle the batch solves the twisted clover eo system instead
+// (inv_tmclovD_cg_eoprec, cg_invert_tmclovD_eoprec.cpp:146-232, for every slot) at one
+// twisted mass for the whole group: K = g5 (D_oo - 1/4 H D_ee^-1 H) with the clover
+// blocks applied to every slot by batched block kernels that read each site's block
+// once, and the inverse blocks of the group's (kappa, mu) kept by the group.
 #pragma once
 
 #include <map>
@@ -29,8 +35,10 @@ class TmBatch {
   static constexpr int kMaxSlots = 16;
   static constexpr int kMaxGroups = 3;  // state ranges [16*group, 16*group+16) of the ctx
 
-  // prec: NQ_PREC_*; group selects the state range and the names of the temporaries
-  int setup(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis, int prec, int group);
+  // prec: NQ_PREC_*; group selects the state range and the names of the temporaries;
+  // cl (optional): twisted clover at twisted mass clover_mu for every slot
+  int setup(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* anis, int prec, int group,
+            nq_clover* cl = nullptr, double clover_mu = 0);
   // eq. 8 for the source (src_e, src_o FP64 views of one lx field) into `slot`, then
   // the start of its CG (x = guess or 0, r = b - K K^+ x); `residue` is nissa's squared
   // relative residue
@@ -51,14 +59,21 @@ class TmBatch {
   int nactive() const;
   int prec() const { return prec_; }
   double kappa() const { return kappa_; }
-  // device bytes of the per-slot work vectors
-  static int64_t bytes_per_slot(const Geo& g, int prec);
+  bool clover() const { return cl_ != nullptr; }
+  // device bytes of the per-slot work vectors (clover: two more per precision)
+  static int64_t bytes_per_slot(const Geo& g, int prec, bool clover = false);
 
  private:
   int hi() const;
+  int lk64() const;  // links of the FP64 work: the tm solves' (12-real when mixed), FP64 18-real for clover
   int upload_row(int slot, double mu);
   int enqueue(int nr, int iters);
   int run_graphs(int nr, int iters);
+  // K(dag) on nr slots of the batch vectors: out = K in (clover), with the last hop's
+  // epilogue from `last` (the coefficient table of the K out hop)
+  int clov_kern_batched(const LinkView& U, double anis, int dag, const SpinorView& out, const SpinorView& in,
+                        const SpinorView& tmp, const SpinorView& te2, const SpinorView& uo, const SpinorView& inv,
+                        const SpinorView& clo, HopEpi last, int nr, int pred);
 
   nq_ctx* ctx_ = nullptr;
   const nq_gauge* g_ = nullptr;
@@ -75,6 +90,11 @@ class TmBatch {
   int niter_[kMaxSlots] = {};
   int launched_[kMaxSlots] = {};  // iterations enqueued since the slot started
   int chunk_ = 8;
+  nq_clover* cl_ = nullptr;
+  double clmu_ = 0;
+  SpinorView clinv64_, clinv_s_, clo64_, clo_s_;  // even inverse blocks (group's), odd clover blocks
+  SpinorView te2_64_, uo64_, te2s_, uos_;         // batched even / odd block products
+  SpinorView cte2_;                               // one even FP64 field (single-slot K in finish)
   std::map<int, hipGraphExec_t> graphs_;  // (slots, 2^k iterations) -> captured chunk
   std::set<int> eager_done_;              // slot counts that ran eagerly once
 };

@@ -106,6 +106,42 @@ namespace nissa::nq_shims
     x.down(solution_lx);
     return true;
   }
+
+  // the propagator driver's clover branch (projects/ib/prop.cpp:93-94) for several
+  // sources at one (kappa, cSW, mass): one clover term, the K K^+ CG of up to 16
+  // sources in the same launches, FP32 inner iterations
+  inline void solve_tmclovD_eoprec_multi(std::vector<LxField<spincolor>*>& solutions_lx,
+					 const LxField<quad_su3>& conf_lx,
+					 const double& kappa,
+					 const std::optional<double>& anis,
+					 const double& cSW,
+					 const double& mass,
+					 const int& nitermax,
+					 const double& residue,
+					 const std::vector<const LxField<spincolor>*>& sources_lx)
+  {
+    const int n=(int)sources_lx.size();
+    std::vector<Dev> s,x;
+    s.reserve(n);
+    x.reserve(n);
+    std::vector<nq_field*> sf(n),xf(n);
+    for(int i=0;i<n;i++)
+      {
+	s.emplace_back(NQ_SPINCOLOR,NQ_LX);
+	x.emplace_back(NQ_SPINCOLOR,NQ_LX);
+	s.back().up(*sources_lx[i]);
+	sf[i]=s.back().f;
+	xf[i]=x.back().f;
+      }
+    nq_clover* cl=nullptr;
+    check(nq_clover_create(ctx(),gauge(conf_lx),cSW,&cl),"clover_term");
+    std::vector<nq_solve_stats> st(n);
+    const int rc=nq_tmclov_solve_eoprec_multi(ctx(),gauge(conf_lx),cl,kappa,anis_ptr(anis),mass,n,nitermax,residue,
+					      NQ_PREC_F32,xf.data(),sf.data(),st.data());
+    nq_clover_destroy(cl);
+    check(rc,"inv_tmclovD_cg_eoprec (multi)");
+    for(int i=0;i<n;i++) x[i].down(*solutions_lx[i]);
+  }
 }
 
 namespace nissa

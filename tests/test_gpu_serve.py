@@ -129,3 +129,35 @@ def test_point_source_fp16_far_field():
     st = nq.tm_solve_eoprec(ctx, g, KAPPA, 0.003, 100000, 1e-20, fx, fs, prec="f16")
     assert st["rc"] == nq.NQ_OK and st["check_rel_res_sq"] < 1e-20
     ctx.close()
+
+
+def test_server_twisted_clover_requests():
+    """Requests with a cSW are twisted clover solves: they match the oracle's
+    inv_tmclovD_cg_eoprec, give the same bits alone and next to twisted-mass requests
+    and clover requests of another mass on the same volume (separate groups)."""
+    L = (8, 8, 8, 8)
+    conf = tm_conf(L, 14)
+    csw = 1.1
+    srv = nq.Server(max_groups=3)
+    vol = srv.add_volume(L, conf)
+    srcs = [nq.source_lx(L, 120 + i, "gauss") for i in range(3)]
+    alone = point_sols(L, 3)
+    r0 = srv.wait(srv.submit(vol, KAPPA, 0.01, 1e-22, prec="f16", srcs=srcs, sols=alone, csw=csw))
+    assert r0["status"] == nq.NQ_OK and r0["max_rel_res_sq"] < 1e-22, r0
+    geo = orc.Geom(L)
+    for s, x in zip(srcs, alone):
+        ref, rst = geo.tmclov_solve_eoprec(conf, KAPPA, csw, 0.01, 20000, 1e-24, s)
+        assert rst["rc"] == 0
+        assert rel_l2(x, ref) < 1e-9
+    mixed = point_sols(L, 3)
+    ids = [
+        srv.submit(vol, KAPPA, 0.02, 1e-22, prec="f16", point_site=5, csw=csw),
+        srv.submit(vol, KAPPA, 0.01, 1e-22, prec="f16", point_site=9),
+        srv.submit(vol, KAPPA, 0.01, 1e-22, prec="f16", srcs=srcs, sols=mixed, csw=csw),
+    ]
+    for i in ids:
+        r = srv.wait(i)
+        assert r["status"] == nq.NQ_OK and r["max_rel_res_sq"] < 1e-22, r
+    srv.close()
+    for a, b in zip(alone, mixed):
+        assert np.array_equal(a, b)

@@ -214,6 +214,27 @@ def test_multi_rhs_graph_replays_match_eager(monkeypatch):
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_cgm_graph_replays_match_eager(st_setups, monkeypatch, prec):
+    """The multi-shift CG replays captured chunks of iterations (HIP graphs) between
+    its eager polls and reliable updates; every (source, shift) solution must match
+    the eager launch sequence bit for bit, and the replays run between host work."""
+    L = (8, 8, 8, 8)
+    ctx, g, geo, conf, ce, co = st_setups(L)
+    shifts = [0.001, 0.01, 0.1]
+    srcs = [ctx.field(nq.COLOR, nq.EVEN, gauss_eo(geo.Vh, 140 + i, ncomp=3)) for i in range(3)]
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("NQ_GRAPHS", mode)
+        sols = [[ctx.field(nq.COLOR, nq.EVEN) for _ in shifts] for _ in srcs]
+        res = nq.st_cgm(ctx, g, shifts, 20000, [1e-22] * 3, sols, srcs, prec=prec)
+        assert all(r["rc"] == nq.NQ_OK and max(r["true_res_sq"]) < 1e-22 for r in res)
+        out[mode] = [[f.download() for f in row] for row in sols]
+    for ra, rb in zip(out["0"], out["1"]):
+        for a, b in zip(ra, rb):
+            assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_st_rational_sum_closed_form(st_setups, prec):
     """summ_src_and_all_inv_stD2ee_m2_cgm: with the partial fractions of
     1/((x+a)(x+b)) = (1/(b-a)) (1/(x+a) - 1/(x+b)) (cons 0, poles a, b, weights
