@@ -228,6 +228,15 @@ Geo make_geo(int T, int LX, int LY, int LZ, int tbmax, int tx, int ty)
   g.Syt = g.Sxi * g.TX;
   g.Sxt = g.Syt * g.NYT;
   g.Sth = g.Sxt * g.NXT;
+  // z-slab visit order of the batched kernels when every column is a whole number
+  // (at least two) of 256-site blocks; NQ_RB_ORDER=0 keeps the storage order
+  const int64_t col = (int64_t)g.TB * LZ / 2;
+  const char* ro = getenv("NQ_RB_ORDER");
+  const bool slabs = !(ro && ro[0] == '0');
+  if (slabs && g.Vh % 256 == 0 && col % 256 == 0 && col >= 512) {
+    g.rb_bpc = col / 256;
+    g.rb_ncol = g.Vh / col;
+  }
   return g;
 }
 

@@ -56,6 +56,10 @@ struct Geo {
   int64_t Vh;          // sites per parity
   int64_t stride;      // per-component stride of SoA fields (Vh rounded up to 64)
   int64_t Sz, Syi, Sxi, Syt, Sxt, Sth;  // device-order strides (tl stride is 1)
+  // visit order of the register-blocked (multi-RHS) kernels: rb_bpc 256-site blocks
+  // per (th, x, y) column and rb_ncol columns, visited z slab by z slab
+  // (map_thread_rb); rb_bpc = 0 visits the blocks in storage order
+  int64_t rb_bpc, rb_ncol;
 };
 
 // rank of lattice site (t, x, y, z) in the device order (both parities)

@@ -84,7 +84,7 @@ __device__ __forceinline__ void st_slot(const Geo& g, const Site& s, uint32_t e,
   cx<R> u[9];
   constexpr int slot = 2 * MU + (FWD ? 0 : 1);
   const double det[2] = {1, 0};
-  LkAcc<LK, R>::load(ub, g.stride, slot, e, det, u);
+  LkAcc<LK, R, (RB > 1 ? kRbLinkCP : kLinkCP)>::load(ub, g.stride, slot, e, det, u);
   cx<R> v[RB][3];
   const uint32_t d = n - lds_e0;
   if (d < (uint32_t)kStageSites) {  // neighbour staged in LDS

@@ -339,6 +339,43 @@ __global__ void __launch_bounds__(256, 1) k_tm_hop_lds_p(Geo g, int op, SpinorVi
   const double* det = &U.det[op][0][0];
   const R anis = (R)anis_d;
   const bool ua = use_anis != 0;
+  if constexpr (MODE == 0) {
+    // No reduction, so no 64-site chunk to respect: the XCD's sites are walked in
+    // rounds of 256 per workgroup, and the sites left after the last whole round
+    // are split evenly over all of the XCD's workgroups and, inside each, over its
+    // four wavefronts.  With whole blocks only, 24^3x48 (162 blocks per XCD on 64
+    // workgroups) ends with a third round in which 30 of 64 workgroups idle.
+    const int64_t s_lo = (int64_t)g.Vh * xcd / 8, n = (int64_t)g.Vh * (xcd + 1) / 8 - s_lo;
+    const int64_t per_round = nbx * kStageSites, full = n / per_round;
+    const int lane = t & 63;
+    for (int64_t k = 0; k <= full; k++) {  // uniform over the workgroup
+      int64_t e0, len;
+      if (k < full) {
+        e0 = s_lo + (k * nbx + bi) * kStageSites;
+        len = kStageSites;
+      } else {
+        const int64_t rem = n - full * per_round, q = (rem + nbx - 1) / nbx;
+        e0 = s_lo + full * per_round + (int64_t)bi * q;
+        len = s_lo + n - e0 < q ? s_lo + n - e0 : q;
+        if (len <= 0) break;
+      }
+      __syncthreads();  // the previous window's LDS reads are done
+      Stage<SP, 12>::put(lds, in, g.stride, (uint32_t)e0 + t, e0 + t < g.Vh, t, 0);
+      __syncthreads();
+      const int wl = (int)((len + 3) / 4);  // sites per wavefront (64 in a whole round)
+      const int64_t eo = (int64_t)wave * wl + lane;
+      if (lane < wl && eo < len) {
+        const uint32_t e = (uint32_t)(e0 + eo);
+        const Site s = site_of(g, op, e);
+        cx<R> o[12];
+#pragma unroll
+        for (int k2 = 0; k2 < 12; k2++) o[k2] = mk<R>(0, 0);
+        hop_site<SP, LK, R, 2>(g, s, e, in, 0, ub, det, anis, ua, o, lds, (uint32_t)e0);
+        hop_epilogue<SP, MODE, XP, R, CLV>(g, epi, out, e, 0, st, o);
+      }
+    }
+    return;
+  }
   for (int64_t cb = lo + bi; cb < hi; cb += nbx) {  // uniform over the workgroup
     const uint32_t e0 = (uint32_t)(cb * kStageSites);
     __syncthreads();  // the previous block's LDS reads are done
@@ -384,7 +421,7 @@ __device__ __forceinline__ void hop_slot_rb(const Geo& g, const Site& s, uint32_
   const uint32_t n = neigh_eo(g, s, MU, FWD);
   cx<R> u[9];
   constexpr int slot = 2 * MU + (FWD ? 0 : 1);
-  LkAcc<LK, R>::load(ub, g.stride, slot, e, det + 2 * slot, u);
+  LkAcc<LK, R, kRbLinkCP>::load(ub, g.stride, slot, e, det + 2 * slot, u);
 #pragma unroll
   for (int b = 0; b < RB; b++) {
     if (b >= nrb || !((busy >> b) & 1u)) continue;  // uniform over the wavefront
@@ -417,7 +454,7 @@ __global__ void __launch_bounds__(256) k_tm_hop_rb(Geo g, int op, SpinorView out
                                                    int64_t nchunks, double* partial)
 {
   using R = RealOf<SP>;
-  const Map m = map_thread(S, g.Vh);
+  const Map m = map_thread_rb(S, g, nr);
   const int r0 = m.r * RB, nrb = min(RB, nr - r0);
   unsigned busy = 0;
 #pragma unroll
