@@ -117,7 +117,11 @@ struct CgmState {
   double betas[kMaxShift], alphas[kMaxShift], res[kMaxShift];
   int run[kMaxShift];
   int done, reliable;  // adjacent: the sloppy kernels skip a source when either is set
-  int nshift, nrun, iter, niter, nan, mixed, kref, nreliable, pad1[2];
+  int nshift, nrun, iter, niter, nan, mixed, kref, nreliable;
+  // pend: the ps_k update of the iteration just finalised is left to the next
+  // iteration's hop2 kernel (set by the rfrf finalisation when the source goes on
+  // without a reliable update; the update kernel then updates p only)
+  int pend, pad1;
 };
 
 enum Fin {

@@ -14,6 +14,10 @@
 //   C1: t = Doe p,                |t|^2 -> pap = |t|^2/4   (= <p, A(0) p>, Doe^+ = -Deo)
 //   C2: s = -1/4 Deo t; sol_k -= beta_k ps_k; r += betaa s;  |r|^2
 //   C3: p = r + alpha p; ps_k = zeta_k r + alpha_k ps_k
+// The ps_k update of C3 is carried out by the next iteration's C2, which reads ps_k
+// and the unchanged r anyway: each running shift then costs one ps_k read and write
+// fewer per iteration (C2 applies the pending update first, with the same operands
+// and operations, so every bit is the one C3 would write).
 // The scalar recurrences of the reference (zeta/beta/alpha per shift, the
 // per-shift residual rr*zeta_k^2/|b|^2 < res_k stop) run in the finalisation
 // kernels on the device.
@@ -250,6 +254,9 @@ __global__ void __launch_bounds__(256) k_cgm_hop2(Geo g, SpinorView t, LinkView 
     double red = 0;
     if (m.active && ((busy >> b) & 1u)) {
       const R betaa = (R)S->betaa;
+      cx<R> rr[3];
+      SpAcc<SP, R, 3, kNT>::load(rv, g.stride, m.e, rr, r);
+      const bool pend = S->pend != 0;
       for (int k = 0; k < S->nshift; k++) {
         if (!S->run[k]) continue;
         SpinorView sk = sol, pk = ps;
@@ -258,6 +265,17 @@ __global__ void __launch_bounds__(256) k_cgm_hop2(Geo g, SpinorView t, LinkView 
         cx<R> x[3], q[3];
         SpAcc<SP, R, 3, kNT>::load(sk, g.stride, m.e, x, r);
         SpAcc<SP, R, 3, kNT>::load(pk, g.stride, m.e, q, r);
+        if (pend) {
+          // the last iteration's C3 update ps_k = zeta_k r + alpha_k ps_k (r not yet
+          // updated; zas_k holds the zfs_k that C3 would use, alphas_k is unchanged)
+          const R z = (R)S->zas[k], al = (R)S->alphas[k];
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            q[c].re = fma(al, q[c].re, z * rr[c].re);
+            q[c].im = fma(al, q[c].im, z * rr[c].im);
+          }
+          SpAcc<SP, R, 3, kNT>::store(pk, g.stride, m.e, q, r);
+        }
         const R bk = (R)S->betas[k];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -266,8 +284,6 @@ __global__ void __launch_bounds__(256) k_cgm_hop2(Geo g, SpinorView t, LinkView 
         }
         SpAcc<SP, R, 3, kNT>::store(sk, g.stride, m.e, x, r);
       }
-      cx<R> rr[3];
-      SpAcc<SP, R, 3, kNT>::load(rv, g.stride, m.e, rr, r);
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         const cx<R> sv = scal(o[b][c], (R)-0.25);
@@ -281,7 +297,9 @@ __global__ void __launch_bounds__(256) k_cgm_hop2(Geo g, SpinorView t, LinkView 
   }
 }
 
-// C3: p = r + alpha p ; ps_k = zfs_k r + alphas_k ps_k for running shifts
+// C3: p = r + alpha p ; ps_k = zfs_k r + alphas_k ps_k for running shifts (the ps_k
+// part only when no update is pending for C2: before a reliable update, which
+// replaces r, and never otherwise)
 template <int SP>
 __global__ void __launch_bounds__(256) k_cgm_update(Geo g, SpinorView rv, SpinorView p, SpinorView ps,
                                                      int64_t sstride, CgmState* st, Shape SH)
@@ -302,6 +320,7 @@ __global__ void __launch_bounds__(256) k_cgm_update(Geo g, SpinorView rv, Spinor
     pp[c].im = fma(a, pp[c].im, rr[c].im);
   }
   SpAcc<SP, R, 3>::store(p, g.stride, m.e, pp, m.r);
+  if (S->pend) return;  // the ps_k update runs in the next iteration's C2
   for (int k = 0; k < S->nshift; k++) {
     if (!S->run[k]) continue;
     SpinorView pk = ps;
@@ -365,6 +384,9 @@ __device__ inline void cgm_finalize(CgmState* s, int fin, double total, double p
       s->kref = kr;
       if (kr >= 0 && s->res[kr] < s->delta2 * s->rel_ref) s->reliable = 1;
     }
+    // C3 leaves the ps_k update to the next C2 unless the source stops here or a
+    // reliable update (which replaces r) comes first
+    s->pend = !s->done && !s->reliable;
   }
 }
 
@@ -652,8 +674,9 @@ int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, co
   hipEvent_t ev = nullptr;
   // ledger bytes (algorithmic, per site): the links once for all sources; per busy
   // source p gathered and t written (C1); t gathered, r read and written, and per
-  // running shift sol read and written and ps read (C2); r read, p read and written,
-  // and per running shift ps read and written (C3)
+  // running shift sol read and written and ps read and written (C2, which carries
+  // the ps update); r read, p read and written (C3; its ps update runs only before
+  // a reliable update)
   const double cs = csite(sp);
   prof_begin(ctx, "cgm_hop1", (8 * lbytes(lk) + 2 * cs * nbusy) * (double)g.Vh, &ev);
   const int RB = st_rb(nr, sp);
@@ -669,7 +692,7 @@ int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, co
                      (int)FIN_CGM_PAP, 0.25);
   NQ_HIP(hipGetLastError());
   // C2
-  prof_begin(ctx, "cgm_hop2_update", (8 * lbytes(lk) + 3 * cs * nbusy + 3 * cs * nrun) * (double)g.Vh, &ev);
+  prof_begin(ctx, "cgm_hop2_update", (8 * lbytes(lk) + 3 * cs * nbusy + 4 * cs * nrun) * (double)g.Vh, &ev);
   {
     const Shape sh = st_shape(nr, RB);
     const dim3 grid2((unsigned)((L.nchunks + 3) / 4 * sh.NRG)), blk2(256);
@@ -695,7 +718,7 @@ int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, co
                      (int)FIN_CGM_RFRF, 0.25);
   NQ_HIP(hipGetLastError());
   // C3
-  prof_begin(ctx, "cgm_update", (3 * cs * nbusy + 2 * cs * nrun) * (double)g.Vh, &ev);
+  prof_begin(ctx, "cgm_update", 3 * cs * nbusy * (double)g.Vh, &ev);
   if (sp == SP_F64)
     hipLaunchKernelGGL(k_cgm_update<SP_F64>, grid, blk, 0, ctx->stream, g, r, p, ps, sstride, st, L.s);
   else
