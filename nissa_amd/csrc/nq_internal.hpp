@@ -115,13 +115,20 @@ struct CgmState {
   double b2, rr, betaa, betap, alpha, rfrf, pap, rel_ref, delta2, pad0;
   double shift[kMaxShift], target[kMaxShift], zps[kMaxShift], zas[kMaxShift], zfs[kMaxShift];
   double betas[kMaxShift], alphas[kMaxShift], res[kMaxShift];
+  double betas_prev[kMaxShift];  // betas_k of the previous iteration (paired sol_k updates)
   int run[kMaxShift];
+  // owe_k: the sol_k term betas_k ps_k of the last hop2 is still to be applied,
+  // 1 by the host's settle kernel (the shift stopped or the source ended), 2 by
+  // the update kernel (a reliable update follows)
+  int owe[kMaxShift];
   int done, reliable;  // adjacent: the sloppy kernels skip a source when either is set
   int nshift, nrun, iter, niter, nan, mixed, kref, nreliable;
   // pend: the ps_k update of the iteration just finalised is left to the next
   // iteration's hop2 kernel (set by the rfrf finalisation when the source goes on
   // without a reliable update; the update kernel then updates p only)
-  int pend, pad1;
+  // lag: the last hop2 left its sol_k terms to the next one, which applies both
+  // iterations' terms in order (so sol_k is read and written every other iteration)
+  int pend, lag;
 };
 
 enum Fin {
@@ -323,6 +330,8 @@ int launch_cgm_vec3(nq_ctx* ctx, const SpinorView& r, const SpinorView& p, const
 // sol/ps hold the shifts of each source at element offset k*sstride; nbusy and nrun
 // (sources still iterating and their running shifts, as the host last saw them) only
 // size the ledger's byte count
+// sol_k -= betas_k ps_k for the (source, shift) pairs whose CgmState owe_k is 1
+int launch_cgm_settle(nq_ctx* ctx, const SpinorView& sol, const SpinorView& ps, int64_t sstride, void* states, int nr);
 int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, const SpinorView& t,
                          const SpinorView& r, const SpinorView& sol, const SpinorView& ps, int64_t sstride,
                          void* states, int nr, int nbusy, int nrun);

@@ -333,6 +333,22 @@ static int st_cgm_batched(nq_ctx* ctx, const LinkView& U, int nr, int nshift, co
     launched += chunk;
     NQ_HIP(hipMemcpyAsync(hst, dst, sizeof(CgmState) * nr, hipMemcpyDeviceToHost, ctx->stream));
     NQ_HIP(hipStreamSynchronize(ctx->stream));
+    // Shifts that stopped, and sources that ended, right after a C2 that left its
+    // sol_k terms to the next one: apply those terms before anything reads sol.
+    bool owes = false, flags = false;
+    for (int i = 0; i < nr; i++)
+      for (int k = 0; k < nshift; k++) {
+        owes = owes || hst[i].owe[k] == 1;
+        flags = flags || hst[i].owe[k] != 0;
+      }
+    if (owes) NQ_TRY(launch_cgm_settle(ctx, sol, ps, csz, dst, nr));
+    if (flags) {
+      for (int i = 0; i < nr; i++)
+        for (int k = 0; k < nshift; k++) hst[i].owe[k] = 0;
+      NQ_HIP(hipMemcpyAsync(dst, hst, sizeof(CgmState) * nr, hipMemcpyHostToDevice, ctx->stream));
+      // the host edits hst below (reliable updates): the copy must have read it first
+      NQ_HIP(hipStreamSynchronize(ctx->stream));
+    }
     std::vector<int> upd;
     for (int i = 0; i < nr; i++)
       if (mixed && hst[i].reliable && !hst[i].done) upd.push_back(i);

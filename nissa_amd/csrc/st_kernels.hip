@@ -257,14 +257,18 @@ __global__ void __launch_bounds__(256) k_cgm_hop2(Geo g, SpinorView t, LinkView 
       cx<R> rr[3];
       SpAcc<SP, R, 3, kNT>::load(rv, g.stride, m.e, rr, r);
       const bool pend = S->pend != 0;
+      // lag: the last hop2 left its terms betas_prev_k ps_k(previous) to this one
+      // (then pend is set too, so the stored ps_k is still the previous iteration's)
+      const bool lag = S->lag != 0;
       for (int k = 0; k < S->nshift; k++) {
         if (!S->run[k]) continue;
         SpinorView sk = sol, pk = ps;
         sk.data = (char*)sol.data + k * sstride * (SP == SP_F64 ? 16 : 8);
         pk.data = (char*)ps.data + k * sstride * (SP == SP_F64 ? 16 : 8);
-        cx<R> x[3], q[3];
-        SpAcc<SP, R, 3, kNT>::load(sk, g.stride, m.e, x, r);
+        cx<R> q[3], qo[3];
         SpAcc<SP, R, 3, kNT>::load(pk, g.stride, m.e, q, r);
+#pragma unroll
+        for (int c = 0; c < 3; c++) qo[c] = q[c];
         if (pend) {
           // the last iteration's C3 update ps_k = zeta_k r + alpha_k ps_k (r not yet
           // updated; zas_k holds the zfs_k that C3 would use, alphas_k is unchanged)
@@ -276,7 +280,17 @@ __global__ void __launch_bounds__(256) k_cgm_hop2(Geo g, SpinorView t, LinkView 
           }
           SpAcc<SP, R, 3, kNT>::store(pk, g.stride, m.e, q, r);
         }
-        const R bk = (R)S->betas[k];
+        if (!lag) continue;  // this iteration's term waits for the next hop2
+        // sol_k -= betas_prev_k ps_k(previous) ; sol_k -= betas_k ps_k, in that order:
+        // the bits of two separate updates
+        cx<R> x[3];
+        SpAcc<SP, R, 3, kNT>::load(sk, g.stride, m.e, x, r);
+        const R bp = (R)S->betas_prev[k], bk = (R)S->betas[k];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          x[c].re = fma(-bp, qo[c].re, x[c].re);
+          x[c].im = fma(-bp, qo[c].im, x[c].im);
+        }
 #pragma unroll
         for (int c = 0; c < 3; c++) {
           x[c].re = fma(-bk, q[c].re, x[c].re);
@@ -299,10 +313,10 @@ __global__ void __launch_bounds__(256) k_cgm_hop2(Geo g, SpinorView t, LinkView 
 
 // C3: p = r + alpha p ; ps_k = zfs_k r + alphas_k ps_k for running shifts (the ps_k
 // part only when no update is pending for C2: before a reliable update, which
-// replaces r, and never otherwise)
+// replaces r, and never otherwise; it then applies the sol_k term that C2 left)
 template <int SP>
 __global__ void __launch_bounds__(256) k_cgm_update(Geo g, SpinorView rv, SpinorView p, SpinorView ps,
-                                                     int64_t sstride, CgmState* st, Shape SH)
+                                                     SpinorView sol, int64_t sstride, CgmState* st, Shape SH)
 {
   using R = RealOfS<SP>;
   const Map m = map_thread(SH, g.Vh);
@@ -327,6 +341,20 @@ __global__ void __launch_bounds__(256) k_cgm_update(Geo g, SpinorView rv, Spinor
     pk.data = (char*)ps.data + k * sstride * (SP == SP_F64 ? 16 : 8);
     cx<R> q[3];
     SpAcc<SP, R, 3, kNT>::load(pk, g.stride, m.e, q, m.r);
+    if (S->owe[k] == 2) {
+      // a reliable update follows: the sol_k term of this iteration's C2 first
+      SpinorView sk = sol;
+      sk.data = (char*)sol.data + k * sstride * (SP == SP_F64 ? 16 : 8);
+      cx<R> x[3];
+      SpAcc<SP, R, 3, kNT>::load(sk, g.stride, m.e, x, m.r);
+      const R bk = (R)S->betas[k];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        x[c].re = fma(-bk, q[c].re, x[c].re);
+        x[c].im = fma(-bk, q[c].im, x[c].im);
+      }
+      SpAcc<SP, R, 3, kNT>::store(sk, g.stride, m.e, x, m.r);
+    }
     const R z = (R)S->zfs[k], al = (R)S->alphas[k];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
@@ -353,6 +381,7 @@ __device__ inline void cgm_finalize(CgmState* s, int fin, double total, double p
                                        s->betap * (1 - s->shift[k] * s->betaa));
       if (isnan(ratio)) { s->nan = 1; s->done = 1; }
       s->zfs[k] = s->zas[k] * ratio;
+      s->betas_prev[k] = s->betas[k];
       s->betas[k] = s->betaa * ratio;
     }
   } else {
@@ -367,16 +396,24 @@ __device__ inline void cgm_finalize(CgmState* s, int fin, double total, double p
     }
     s->rr = rfrf;
     s->iter += 1;
+    // the C2 just run left its sol_k terms when lag was clear (the next C2 applies
+    // them); a shift that stops, or a source that ends, owes them to the settle kernel
+    const bool owed = s->lag == 0;
     for (int k = 0; k < s->nshift; k++) {
       if (!s->run[k]) continue;
+      s->owe[k] = 0;
       s->res[k] = s->rr * s->zfs[k] * s->zfs[k] / s->b2;
       if (s->res[k] < s->target[k]) {
         s->run[k] = 0;
         s->nrun -= 1;
+        if (owed) s->owe[k] = 1;
       }
     }
     if (isnan(rfrf)) { s->nan = 1; s->done = 1; }
     if (s->nrun <= 0 || s->iter >= s->niter) s->done = 1;
+    if (s->done && owed)
+      for (int k = 0; k < s->nshift; k++)
+        if (s->run[k]) s->owe[k] = 1;
     if (s->mixed && !s->done) {
       int kr = -1;
       for (int k = 0; k < s->nshift; k++)
@@ -384,9 +421,14 @@ __device__ inline void cgm_finalize(CgmState* s, int fin, double total, double p
       s->kref = kr;
       if (kr >= 0 && s->res[kr] < s->delta2 * s->rel_ref) s->reliable = 1;
     }
+    // before a reliable update C3 applies the running shifts' owed terms
+    if (s->reliable && owed)
+      for (int k = 0; k < s->nshift; k++)
+        if (s->run[k]) s->owe[k] = 2;
     // C3 leaves the ps_k update to the next C2 unless the source stops here or a
     // reliable update (which replaces r) comes first
     s->pend = !s->done && !s->reliable;
+    s->lag = owed && s->pend;
   }
 }
 
@@ -404,6 +446,52 @@ __global__ void __launch_bounds__(1024) k_cgm_finalize(const double* partial, in
   }
   const double tot = block_sum_partials(partial + (int64_t)blockIdx.x * nchunks, nchunks, vl);
   if (threadIdx.x == 0) cgm_finalize(s, fin, tot, pap_scale);
+}
+
+// sol_k -= betas_k ps_k for the shifts whose term the last C2 left and that no
+// later C2 applies (owe_k == 1: the shift stopped or the source ended); the host
+// launches it after a poll that finds such a shift and then clears the flags
+template <int SP>
+__global__ void __launch_bounds__(256) k_cgm_settle(Geo g, SpinorView sol, SpinorView ps, int64_t sstride,
+                                                    const CgmState* st, Shape SH)
+{
+  using R = RealOfS<SP>;
+  const Map m = map_thread(SH, g.Vh);
+  const CgmState* S = st + m.r;
+  if (!m.active) return;
+  for (int k = 0; k < S->nshift; k++) {
+    if (S->owe[k] != 1) continue;
+    SpinorView sk = sol, pk = ps;
+    sk.data = (char*)sol.data + k * sstride * (SP == SP_F64 ? 16 : 8);
+    pk.data = (char*)ps.data + k * sstride * (SP == SP_F64 ? 16 : 8);
+    cx<R> x[3], q[3];
+    SpAcc<SP, R, 3>::load(sk, g.stride, m.e, x, m.r);
+    SpAcc<SP, R, 3>::load(pk, g.stride, m.e, q, m.r);
+    const R bk = (R)S->betas[k];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      x[c].re = fma(-bk, q[c].re, x[c].re);
+      x[c].im = fma(-bk, q[c].im, x[c].im);
+    }
+    SpAcc<SP, R, 3>::store(sk, g.stride, m.e, x, m.r);
+  }
+}
+
+int launch_cgm_settle(nq_ctx* ctx, const SpinorView& sol, const SpinorView& ps, int64_t sstride, void* states, int nr)
+{
+  const Geo& g = ctx->geo;
+  const Launch L = launch_shape(g, nr);
+  hipEvent_t ev = nullptr;
+  prof_begin(ctx, "cgm_settle", 0.0, &ev);
+  if (sol.prec == SP_F64)
+    hipLaunchKernelGGL(k_cgm_settle<SP_F64>, dim3((unsigned)L.blocks), dim3(L.threads), 0, ctx->stream, g, sol, ps,
+                       sstride, (const CgmState*)states, L.s);
+  else
+    hipLaunchKernelGGL(k_cgm_settle<SP_F32>, dim3((unsigned)L.blocks), dim3(L.threads), 0, ctx->stream, g, sol, ps,
+                       sstride, (const CgmState*)states, L.s);
+  NQ_HIP(hipGetLastError());
+  prof_end(ctx, ev);
+  return NQ_OK;
 }
 
 // ------------------------------------------------------------------ launchers
@@ -674,9 +762,10 @@ int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, co
   hipEvent_t ev = nullptr;
   // ledger bytes (algorithmic, per site): the links once for all sources; per busy
   // source p gathered and t written (C1); t gathered, r read and written, and per
-  // running shift sol read and written and ps read and written (C2, which carries
-  // the ps update); r read, p read and written (C3; its ps update runs only before
-  // a reliable update)
+  // running shift ps read and written and, every other iteration, sol read and
+  // written (C2, which carries the ps update and pairs the sol updates: 3 fields
+  // per running shift on average); r read, p read and written (C3; its ps update
+  // runs only before a reliable update)
   const double cs = csite(sp);
   prof_begin(ctx, "cgm_hop1", (8 * lbytes(lk) + 2 * cs * nbusy) * (double)g.Vh, &ev);
   const int RB = st_rb(nr, sp);
@@ -692,7 +781,7 @@ int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, co
                      (int)FIN_CGM_PAP, 0.25);
   NQ_HIP(hipGetLastError());
   // C2
-  prof_begin(ctx, "cgm_hop2_update", (8 * lbytes(lk) + 3 * cs * nbusy + 4 * cs * nrun) * (double)g.Vh, &ev);
+  prof_begin(ctx, "cgm_hop2_update", (8 * lbytes(lk) + 3 * cs * nbusy + 3 * cs * nrun) * (double)g.Vh, &ev);
   {
     const Shape sh = st_shape(nr, RB);
     const dim3 grid2((unsigned)((L.nchunks + 3) / 4 * sh.NRG)), blk2(256);
@@ -720,9 +809,9 @@ int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, co
   // C3
   prof_begin(ctx, "cgm_update", 3 * cs * nbusy * (double)g.Vh, &ev);
   if (sp == SP_F64)
-    hipLaunchKernelGGL(k_cgm_update<SP_F64>, grid, blk, 0, ctx->stream, g, r, p, ps, sstride, st, L.s);
+    hipLaunchKernelGGL(k_cgm_update<SP_F64>, grid, blk, 0, ctx->stream, g, r, p, ps, sol, sstride, st, L.s);
   else
-    hipLaunchKernelGGL(k_cgm_update<SP_F32>, grid, blk, 0, ctx->stream, g, r, p, ps, sstride, st, L.s);
+    hipLaunchKernelGGL(k_cgm_update<SP_F32>, grid, blk, 0, ctx->stream, g, r, p, ps, sol, sstride, st, L.s);
   NQ_HIP(hipGetLastError());
   prof_end(ctx, ev);
   return NQ_OK;
