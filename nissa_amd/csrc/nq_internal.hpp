@@ -306,7 +306,8 @@ int launch_tm_hop(nq_ctx* ctx, int out_par, const SpinorView& out, const SpinorV
 double tm_hop_bytes(int sp, int lk, const HopEpi& epi, int ncomp = 12);
 
 // staggered hop of output parity op: out = c1*hop + c2*aux (per-rhs (c1,c2) from
-// coef_dev when given). mode 0 write; 1 |out|^2; 4 Re<aux,out> (reduced into st with fin).
+// coef_dev when given). mode 0 write; 1 |out|^2; 4 Re<aux,out> (reduced into st with fin);
+// 6 out = c1*hop and sigma_r |in|^2 + |out|^2/4 with sigma_r = coef_dev[2r+1].
 // skip_flags (optional): rhs r is skipped when any of skip_flags[r*skip_stride + i],
 // i < skip_n, is set (skip_n = 2 on &CgState::done also skips while a reliable update
 // is due, and the finalisation then leaves such states alone).
@@ -330,6 +331,11 @@ int launch_cgm_vec3(nq_ctx* ctx, const SpinorView& r, const SpinorView& p, const
 // sol/ps hold the shifts of each source at element offset k*sstride; nbusy and nrun
 // (sources still iterating and their running shifts, as the host last saw them) only
 // size the ledger's byte count
+// batched staggered CG: s = coef[2r] hop(t) + coef[2r+1] p at the even sites fused with
+// x += alpha p, r -= alpha s and |r|^2 (finalised with fin; done or reliable-pending
+// systems idle)
+int launch_st_cgtail(nq_ctx* ctx, const SpinorView& t, const LinkView& U, const double* coef_dev, const SpinorView& p,
+                     const SpinorView& x, const SpinorView& r, CgState* st, int fin, int nr);
 // sol_k -= betas_k ps_k for the (source, shift) pairs whose CgmState owe_k is 1
 int launch_cgm_settle(nq_ctx* ctx, const SpinorView& sol, const SpinorView& ps, int64_t sstride, void* states, int nr);
 int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, const SpinorView& t,

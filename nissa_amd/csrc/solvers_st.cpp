@@ -90,7 +90,7 @@ static int st_cg_batched(nq_ctx* ctx, const nq_gauge* g, int nr, const double* s
   std::vector<double> hc(2 * nr);
   for (int i = 0; i < nr; i++) { hc[2 * i] = -0.25; hc[2 * i + 1] = shifts[i]; }
   NQ_HIP(hipMemcpyAsync(coef, hc.data(), sizeof(double) * 2 * nr, hipMemcpyHostToDevice, ctx->stream));
-  SpinorView r64, t64, s64, xs, rs, p, t, s;
+  SpinorView r64, t64, s64, xs, rs, p, t;
   NQ_TRY(alloc_spinor_nr(ctx, "stcg.r", SP_F64, 3, nr, &r64));
   NQ_TRY(alloc_spinor_nr(ctx, "stcg.t", SP_F64, 3, nr, &t64));
   NQ_TRY(alloc_spinor_nr(ctx, "stcg.s", SP_F64, 3, nr, &s64));
@@ -99,11 +99,9 @@ static int st_cg_batched(nq_ctx* ctx, const nq_gauge* g, int nr, const double* s
     NQ_TRY(alloc_spinor_nr(ctx, "stcg.rs", SP_F32, 3, nr, &rs));
     NQ_TRY(alloc_spinor_nr(ctx, "stcg.p32", SP_F32, 3, nr, &p));
     NQ_TRY(alloc_spinor_nr(ctx, "stcg.t32", SP_F32, 3, nr, &t));
-    NQ_TRY(alloc_spinor_nr(ctx, "stcg.s32", SP_F32, 3, nr, &s));
   } else {
     NQ_TRY(alloc_spinor_nr(ctx, "stcg.p", SP_F64, 3, nr, &p));
     t = t64;
-    s = s64;
     xs = x64;
     rs = r64;
   }
@@ -152,11 +150,12 @@ static int st_cg_batched(nq_ctx* ctx, const nq_gauge* g, int nr, const double* s
   };
   while (!all_done()) {
     for (int it = 0; it < chunk; it++) {
-      // t = Doe p ; s = sigma p - 1/4 Deo t with <p, s> -> pAp (skip idle systems)
-      NQ_TRY(launch_st_hop(ctx, NQ_ODD, t, p, Ui, 1.0, 0.0, nullptr, none, 0, 1, nr, FIN_NONE, nullptr, &S->done,
-                           kStateInts, 2));
-      NQ_TRY(launch_st_hop(ctx, NQ_EVEN, s, t, Ui, -0.25, 0.0, coef, p, 4, 1, nr, FIN_PAP, S, &S->done, kStateInts, 2));
-      NQ_TRY(launch_cg_update(ctx, xs, p, rs, s, S, mixed ? FIN_RR_SLOPPY : FIN_RR, nr));
+      // t = Doe p with <p, A p> = sigma |p|^2 + |t|^2/4 (Deo^+ = -Doe) -> alpha;
+      // s = sigma p - 1/4 Deo t fused with x += alpha p, r -= alpha s, |r|^2 -> beta;
+      // p = r + beta p (idle systems skipped throughout)
+      NQ_TRY(launch_st_hop(ctx, NQ_ODD, t, p, Ui, 1.0, 0.0, coef, none, 6, 1, nr, FIN_PAP, S, &S->done, kStateInts,
+                           2));
+      NQ_TRY(launch_st_cgtail(ctx, t, Ui, coef, p, xs, rs, S, mixed ? FIN_RR_SLOPPY : FIN_RR, nr));
       NQ_TRY(launch_cg_pupdate(ctx, p, rs, S, nr));
     }
     launched += chunk;

@@ -161,7 +161,11 @@ __device__ __forceinline__ uint32_t stage_sources(char* lds, const Geo& g, const
 }
 
 // out = c1 hop + c2 aux (real coefficients, per rhs from coef[2r], coef[2r+1] when
-// tabulated). MODE 0: write; 1: write (if write) + |out|^2; 4: write (if write) + Re<aux,out>.
+// tabulated). MODE 0: write; 1: write (if write) + |out|^2; 4: write (if write) + Re<aux,out>;
+// 6 (first half of the batched CG matvec, out = c1 hop): write (if write) +
+// sigma_r |in|^2 + 1/4 |out|^2 = <in, A(sigma_r) in> for A = sigma - 1/4 Deo Doe with
+// sigma_r = coef[2r+1], |in|^2 taken from the staged input sites of the lanes' own
+// indices (each input site is staged by exactly one workgroup).
 // Wavefront w of a workgroup takes chunk 4*cb + w and the RB right-hand sides of
 // group m.r (Shape W = 4, NRB = 1, NR = NRG = number of RB groups).
 template <int SP, int LK, int MODE, int RB>
@@ -197,7 +201,8 @@ __global__ void __launch_bounds__(256) k_st_hop(Geo g, int op, SpinorView out, S
     const int r = r0 + b;
     double red = 0;
     if (m.active && ((busy >> b) & 1u)) {
-      const double a = coef ? coef[2 * r] : c1, bb = coef ? coef[2 * r + 1] : c2;
+      const bool tab = coef && MODE != 6;
+      const double a = tab ? coef[2 * r] : c1, bb = tab ? coef[2 * r + 1] : c2;
 #pragma unroll
       for (int c = 0; c < 3; c++) o[b][c] = scal(o[b][c], (R)a);
       if (aux.data) {
@@ -215,8 +220,82 @@ __global__ void __launch_bounds__(256) k_st_hop(Geo g, int op, SpinorView out, S
 #pragma unroll
         for (int c = 0; c < 3; c++) red += (double)o[b][c].re * o[b][c].re + (double)o[b][c].im * o[b][c].im;
       }
+      if constexpr (MODE == 6) {
+        cx<R> pv[3];
+        StStage<SP>::get(lds + b * StStage<SP>::BYTES, (int)(m.e - e0), pv);
+        double p2 = 0, t2 = 0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          p2 += (double)pv[c].re * pv[c].re + (double)pv[c].im * pv[c].im;
+          t2 += (double)o[b][c].re * o[b][c].re + (double)o[b][c].im * o[b][c].im;
+        }
+        red = coef[2 * r + 1] * p2 + 0.25 * t2;
+      }
     }
     if constexpr (MODE != 0) store_partial(red, partial, nchunks, r, m.chunk);
+  }
+}
+
+// Second half of the batched CG matvec fused with the CG update: at even site e of
+// system r, s = coef[2r] hop(t) + coef[2r+1] p (-1/4 Deo t + sigma_r p), then
+// x += alpha_r p ; r -= alpha_r s ; |r|^2 partials, the update in FP64 on the
+// loaded values as the BLAS update kernel does.  Systems that are done or wait for
+// a reliable update idle.
+template <int SP, int LK, int RB>
+__global__ void __launch_bounds__(256) k_st_cgtail(Geo g, SpinorView t, LinkView U, const double* coef, SpinorView p,
+                                                    SpinorView x, SpinorView rv, const CgState* st, Shape S, int nr,
+                                                    int64_t nchunks, double* partial)
+{
+  using R = RealOfS<SP>;
+  const Map m = map_thread(S, g.Vh);
+  const int r0 = m.r * RB, nrb = min(RB, nr - r0);
+  unsigned busy = 0;
+#pragma unroll
+  for (int b = 0; b < RB; b++)
+    if (b < nrb && !st[r0 + b].done && !st[r0 + b].reliable) busy |= 1u << b;
+  if (!busy) {  // uniform over the workgroup
+#pragma unroll
+    for (int b = 0; b < RB; b++)
+      if (b < nrb) store_partial(0.0, partial, nchunks, r0 + b, m.chunk);
+    return;
+  }
+  __shared__ __attribute__((aligned(16))) char lds[RB * StStage<SP>::BYTES];
+  const uint32_t e0 = stage_sources<SP, RB>(lds, g, t, m, r0, nrb);
+  cx<R> o[RB][3];
+  if (m.active) st_hop_site<SP, LK, R, RB>(g, 0, m.e, t, r0, nrb, U, o, lds, e0);
+#pragma unroll
+  for (int b = 0; b < RB; b++) {
+    if (b >= nrb) continue;
+    const int r = r0 + b;
+    double red = 0;
+    if (m.active && ((busy >> b) & 1u)) {
+      const double a = coef[2 * r], bb = coef[2 * r + 1];
+      cx<R> pv[3];
+      SpAcc<SP, R, 3, kNT>::load(p, g.stride, m.e, pv, r);
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        o[b][c] = scal(o[b][c], (R)a);
+        o[b][c].re = fma((R)bb, pv[c].re, o[b][c].re);
+        o[b][c].im = fma((R)bb, pv[c].im, o[b][c].im);
+      }
+      const double alpha = st[r].alpha;
+      cx<R> xv[3], rr[3];
+      SpAcc<SP, R, 3, kNT>::load(x, g.stride, m.e, xv, r);
+      SpAcc<SP, R, 3, kNT>::load(rv, g.stride, m.e, rr, r);
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const double xr = fma(alpha, (double)pv[c].re, (double)xv[c].re);
+        const double xi = fma(alpha, (double)pv[c].im, (double)xv[c].im);
+        const double rre = fma(-alpha, (double)o[b][c].re, (double)rr[c].re);
+        const double rim = fma(-alpha, (double)o[b][c].im, (double)rr[c].im);
+        red += rre * rre + rim * rim;
+        xv[c] = mk<R>((R)xr, (R)xi);
+        rr[c] = mk<R>((R)rre, (R)rim);
+      }
+      SpAcc<SP, R, 3, kNT>::store(x, g.stride, m.e, xv, r);
+      SpAcc<SP, R, 3, kNT>::store(rv, g.stride, m.e, rr, r);
+    }
+    store_partial(red, partial, nchunks, r, m.chunk);
   }
 }
 
@@ -570,6 +649,8 @@ int launch_st_hop(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& 
                                  skip_n); break;                                                                     \
   case 1: st_hop_go<SPV, LKV, 1>(ctx, RB, op, out, in, U, c1, c2, coef_dev, aux, write, nr, skip_flags, skip_stride, \
                                  skip_n); break;                                                                     \
+  case 6: st_hop_go<SPV, LKV, 6>(ctx, RB, op, out, in, U, c1, c2, coef_dev, aux, write, nr, skip_flags, skip_stride, \
+                                 skip_n); break;                                                                     \
   default: st_hop_go<SPV, LKV, 4>(ctx, RB, op, out, in, U, c1, c2, coef_dev, aux, write, nr, skip_flags,             \
                                   skip_stride, skip_n); break;                                                       \
   }
@@ -581,6 +662,41 @@ int launch_st_hop(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& 
   prof_end(ctx, ev);
   if (mode != 0 && fin != FIN_NONE) NQ_TRY(launch_finalize(ctx, nr, st, fin, skip_n > 1 ? 2 : 0));
   return NQ_OK;
+}
+
+int launch_st_cgtail(nq_ctx* ctx, const SpinorView& t, const LinkView& U, const double* coef_dev, const SpinorView& p,
+                     const SpinorView& x, const SpinorView& r, CgState* st, int fin, int nr)
+{
+  const Geo& g = ctx->geo;
+  if (t.ncomp != 3 || p.ncomp != 3) return fail(NQ_EINVAL, "staggered CG tail needs colour fields");
+  if (nr < 1 || nr > 16) return fail(NQ_EINVAL, "right-hand sides per launch must be 1..16");
+  const int sp = t.prec, lk = U.prec;
+  if (p.prec != sp || x.prec != sp || r.prec != sp) return fail(NQ_EINVAL, "st CG tail: precision mismatch");
+  const int64_t nchunks = (g.Vh + 63) / 64;
+  if (nchunks * nr > ctx->max_partials) return fail(NQ_EINVAL, "st CG tail: reduction scratch too small");
+  hipEvent_t ev = nullptr;
+  // links once; per system t gathered, p read, x and r read and written
+  prof_begin(ctx, "st_cgtail", (8 * lbytes(lk) + 6 * csite(sp) * nr) * (double)g.Vh, &ev);
+  const int RB = st_rb(nr, sp);
+  const Shape sh = st_shape(nr, RB);
+  const dim3 grid((unsigned)((nchunks + 3) / 4 * sh.NRG)), blk(256);
+#define NQ_TL(SPV, LKV, RBV)                                                                                        \
+  hipLaunchKernelGGL((k_st_cgtail<SPV, LKV, RBV>), grid, blk, 0, ctx->stream, g, t, U, coef_dev, p, x, r, st, sh, nr, \
+                     nchunks, ctx->partial)
+  if (sp == SP_F64 && lk == LK_F64_18) {
+    if (RB == 1) NQ_TL(SP_F64, LK_F64_18, 1);
+    else if (RB == 2) NQ_TL(SP_F64, LK_F64_18, 2);
+    else NQ_TL(SP_F64, LK_F64_18, 4);
+  } else if (sp == SP_F32 && lk == LK_F32_18) {
+    if (RB == 1) NQ_TL(SP_F32, LK_F32_18, 1);
+    else if (RB == 2) NQ_TL(SP_F32, LK_F32_18, 2);
+    else if (RB == 8) NQ_TL(SP_F32, LK_F32_18, 8);
+    else NQ_TL(SP_F32, LK_F32_18, 4);
+  } else return fail(NQ_EINVAL, "st CG tail: unsupported precision combination");
+#undef NQ_TL
+  NQ_HIP(hipGetLastError());
+  prof_end(ctx, ev);
+  return launch_finalize(ctx, nr, st, fin, 2);
 }
 
 // sol64_k += sol_k ; sol_k = 0 for all shifts k of source r
