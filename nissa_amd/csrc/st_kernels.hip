@@ -339,43 +339,73 @@ __global__ void __launch_bounds__(256) k_cgm_hop2(Geo g, SpinorView t, LinkView 
       // lag: the last hop2 left its terms betas_prev_k ps_k(previous) to this one
       // (then pend is set too, so the stored ps_k is still the previous iteration's)
       const bool lag = S->lag != 0;
-      for (int k = 0; k < S->nshift; k++) {
-        if (!S->run[k]) continue;
-        SpinorView sk = sol, pk = ps;
-        sk.data = (char*)sol.data + k * sstride * (SP == SP_F64 ? 16 : 8);
-        pk.data = (char*)ps.data + k * sstride * (SP == SP_F64 ? 16 : 8);
-        cx<R> q[3], qo[3];
-        SpAcc<SP, R, 3, kNT>::load(pk, g.stride, m.e, q, r);
+      // shifts in groups of G, every load of a group issued before its first store
+      // (one memory wait per group instead of one per shift)
+      constexpr int G = SP == SP_F64 ? 2 : 4;
+      const int64_t kb = sstride * (SP == SP_F64 ? 16 : 8);  // bytes between shifts
+      for (int k0 = 0; k0 < S->nshift; k0 += G) {
+        bool on[G];
+        bool any = false;
 #pragma unroll
-        for (int c = 0; c < 3; c++) qo[c] = q[c];
-        if (pend) {
-          // the last iteration's C3 update ps_k = zeta_k r + alpha_k ps_k (r not yet
-          // updated; zas_k holds the zfs_k that C3 would use, alphas_k is unchanged)
-          const R z = (R)S->zas[k], al = (R)S->alphas[k];
+        for (int j = 0; j < G; j++) {
+          on[j] = k0 + j < S->nshift && S->run[k0 + j];
+          any = any || on[j];
+        }
+        if (!any) continue;  // uniform over the wavefront
+        cx<R> q[G][3], qo[G][3], x[G][3];
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+          if (!on[j]) continue;
+          SpinorView pk = ps;
+          pk.data = (char*)ps.data + (k0 + j) * kb;
+          SpAcc<SP, R, 3, kNT>::load(pk, g.stride, m.e, q[j], r);
+        }
+        if (lag) {
+#pragma unroll
+          for (int j = 0; j < G; j++) {
+            if (!on[j]) continue;
+            SpinorView sk = sol;
+            sk.data = (char*)sol.data + (k0 + j) * kb;
+            SpAcc<SP, R, 3, kNT>::load(sk, g.stride, m.e, x[j], r);
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+          if (!on[j]) continue;
+          const int k = k0 + j;
+#pragma unroll
+          for (int c = 0; c < 3; c++) qo[j][c] = q[j][c];
+          if (pend) {
+            // the last iteration's C3 update ps_k = zeta_k r + alpha_k ps_k (r not yet
+            // updated; zas_k holds the zfs_k that C3 would use, alphas_k is unchanged)
+            const R z = (R)S->zas[k], al = (R)S->alphas[k];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+              q[j][c].re = fma(al, q[j][c].re, z * rr[c].re);
+              q[j][c].im = fma(al, q[j][c].im, z * rr[c].im);
+            }
+            SpinorView pk = ps;
+            pk.data = (char*)ps.data + k * kb;
+            SpAcc<SP, R, 3, kNT>::store(pk, g.stride, m.e, q[j], r);
+          }
+          if (!lag) continue;  // this iteration's term waits for the next hop2
+          // sol_k -= betas_prev_k ps_k(previous) ; sol_k -= betas_k ps_k, in that
+          // order: the bits of two separate updates
+          const R bp = (R)S->betas_prev[k], bk = (R)S->betas[k];
 #pragma unroll
           for (int c = 0; c < 3; c++) {
-            q[c].re = fma(al, q[c].re, z * rr[c].re);
-            q[c].im = fma(al, q[c].im, z * rr[c].im);
+            x[j][c].re = fma(-bp, qo[j][c].re, x[j][c].re);
+            x[j][c].im = fma(-bp, qo[j][c].im, x[j][c].im);
           }
-          SpAcc<SP, R, 3, kNT>::store(pk, g.stride, m.e, q, r);
-        }
-        if (!lag) continue;  // this iteration's term waits for the next hop2
-        // sol_k -= betas_prev_k ps_k(previous) ; sol_k -= betas_k ps_k, in that order:
-        // the bits of two separate updates
-        cx<R> x[3];
-        SpAcc<SP, R, 3, kNT>::load(sk, g.stride, m.e, x, r);
-        const R bp = (R)S->betas_prev[k], bk = (R)S->betas[k];
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-          x[c].re = fma(-bp, qo[c].re, x[c].re);
-          x[c].im = fma(-bp, qo[c].im, x[c].im);
+          for (int c = 0; c < 3; c++) {
+            x[j][c].re = fma(-bk, q[j][c].re, x[j][c].re);
+            x[j][c].im = fma(-bk, q[j][c].im, x[j][c].im);
+          }
+          SpinorView sk = sol;
+          sk.data = (char*)sol.data + k * kb;
+          SpAcc<SP, R, 3, kNT>::store(sk, g.stride, m.e, x[j], r);
         }
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-          x[c].re = fma(-bk, q[c].re, x[c].re);
-          x[c].im = fma(-bk, q[c].im, x[c].im);
-        }
-        SpAcc<SP, R, 3, kNT>::store(sk, g.stride, m.e, x, r);
       }
 #pragma unroll
       for (int c = 0; c < 3; c++) {
