@@ -26,6 +26,7 @@ def main():
     ap.add_argument("--prec", default="f64", choices=["f64", "f32", "f16", "f16h"])
     ap.add_argument("--n", type=int, default=20)
     ap.add_argument("--lattice", default="64,32,32,32")
+    ap.add_argument("--ledger", action="store_true", help="print the library's launch ledger (solve, multisolve)")
     a = ap.parse_args()
     import numpy as np
 
@@ -40,6 +41,13 @@ def main():
     x = np.random.default_rng(1).normal(size=(Vh, 4, 3, 2))
     fo = ctx.field(nq.SPINCOLOR, nq.ODD, x)
     fe = ctx.field(nq.SPINCOLOR, nq.EVEN)
+    def print_ledger(top=12):
+        for name, v in sorted(ctx.kernel_stats().items(), key=lambda kv: -kv[1]["total_ms"])[:top]:
+            avg = v["total_ms"] / v["launches"]
+            gbs = v["bytes_per_launch"] / (avg * 1e-3) / 1e9 if v["bytes_per_launch"] else 0.0
+            print(f"   {name:28s} {v['launches']:7d} x {avg*1e3:8.1f} us = {v['total_ms']:9.1f} ms"
+                  f"  {v['bytes_per_launch']/1e6:9.1f} MB/launch  {gbs:6.0f} GB/s")
+
     if a.what == "hop":
         ms, by = nq.bench_tm_hop(ctx, g, a.n, fe, fo, prec=a.prec)
         print(f"hop {a.prec}: {ms*1e3:.1f} us/apply, {by/1e6:.1f} MB algorithmic, {by/(ms*1e-3)/1e9:.0f} GB/s")
@@ -51,13 +59,17 @@ def main():
         srcs = [ctx.field(nq.SPINCOLOR, nq.LX, nq.source_lx(L, 30 + i, "gauss")) for i in range(a.nrhs)]
         sols = [ctx.field(nq.SPINCOLOR, nq.LX) for _ in range(a.nrhs)]
         mus = [[0.003, 0.005, 0.01, 0.02][i % 4] for i in range(a.nrhs)]
-        for _ in range(a.n):
+        for i in range(a.n):
             import time as _t
+            ctx.profiling(a.ledger and i == a.n - 1)
             t0 = _t.perf_counter()
             res = nq.tm_solve_eoprec_multi(ctx, g, 0.16, mus, 100000, 1e-20, sols, srcs, prec=a.prec)
             dt = _t.perf_counter() - t0
             print(f"multisolve x{a.nrhs} {a.prec}: {dt*1e3:.1f} ms ({dt*1e3/a.nrhs:.2f} ms per source), "
                   f"iters {[r['iters'] for r in res]}, worst check {max(r['rel_res_sq'] for r in res):.2e}")
+        if a.ledger:
+            ctx.profiling(False)
+            print_ledger()
     elif a.what == "stagcgm":
         import time as _t
         sconf = nq.gauge_random_lx(L, 4, 0.3)
@@ -75,11 +87,8 @@ def main():
         ctx.profiling(False)
         print(f"stagcgm x{a.nrhs} {a.prec}: {dt:.3f} s, iters {res[0]['iters']}, "
               f"worst {max(max(r['true_res_sq']) for r in res):.2e}")
-        for name, v in sorted(ctx.kernel_stats().items(), key=lambda kv: -kv[1]["total_ms"])[:10]:
-            avg = v["total_ms"] / v["launches"]
-            gbs = v["bytes_per_launch"] / (avg * 1e-3) / 1e9 if v["bytes_per_launch"] else 0.0
-            print(f"   {name:28s} {v['launches']:7d} x {avg*1e3:8.1f} us = {v['total_ms']:9.1f} ms"
-                  f"  {v['bytes_per_launch']/1e6:9.1f} MB/launch  {gbs:6.0f} GB/s")
+        if a.n > 0:
+            print_ledger(10)
     elif a.what == "copy":
         fo2 = ctx.field(nq.SPINCOLOR, nq.ODD)
         for _ in range(a.n):
@@ -90,9 +99,13 @@ def main():
         src = nq.source_lx(L, 17, "gauss")
         fs = ctx.field(nq.SPINCOLOR, nq.LX, src)
         fx = ctx.field(nq.SPINCOLOR, nq.LX)
-        for _ in range(a.n):
+        for i in range(a.n):
+            ctx.profiling(a.ledger and i == a.n - 1)
             st = nq.tm_solve_eoprec(ctx, g, 0.16, 0.003, 100000, 1e-20, fx, fs, prec=a.prec)
+        ctx.profiling(False)
         print(f"solve {a.prec}: {st}")
+        if a.ledger:
+            print_ledger()
     ctx.close()
 
 
