@@ -175,6 +175,30 @@ void prof_begin(nq_ctx* ctx, const char* name, double bytes, hipEvent_t* a)
   *a = e1;
 }
 
+void prof_begin_ext(nq_ctx* ctx, const char* name, double bytes, hipEvent_t* a, hipEvent_t* b)
+{
+  *a = *b = nullptr;
+  if (!ctx->profiling) return;
+  hipEvent_t e0, e1;
+  if (ctx->event_pool.size() >= 2) {
+    e0 = ctx->event_pool.back();
+    ctx->event_pool.pop_back();
+    e1 = ctx->event_pool.back();
+    ctx->event_pool.pop_back();
+  } else {
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return;
+  }
+  ctx->pending.push_back({name, e0, e1, bytes});
+  *a = e0;
+  *b = e1;
+}
+
+void prof_end_ext(nq_ctx* ctx)
+{
+  ctx->hop_ev[0] = ctx->hop_ev[1] = nullptr;
+  if (ctx->pending.size() > 8192) prof_flush(ctx);
+}
+
 void prof_end(nq_ctx* ctx, hipEvent_t b)
 {
   if (!b) return;

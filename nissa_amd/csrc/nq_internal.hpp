@@ -207,6 +207,7 @@ struct nq_ctx {
   struct Pending { std::string name; hipEvent_t a, b; double bytes; };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> event_pool;
+  hipEvent_t hop_ev[2] = {nullptr, nullptr};  // events of the hop launch in flight (ledger on)
   // cached temporaries keyed by role
   std::map<std::string, nq::DevBuf> temps;
   // autotuned launch choices (key: kernel family and shape), e.g. right-hand sides
@@ -280,6 +281,11 @@ int link_view(const nq_gauge* g, int lk, LinkView* v);
 // ---------------------------------------------------------------- profiling
 void prof_begin(nq_ctx* ctx, const char* name, double bytes, hipEvent_t* a);
 void prof_end(nq_ctx* ctx, hipEvent_t a);
+// ledger entry whose two events the caller hands to hipExtLaunchKernelGGL, which
+// records the kernel's own start and end (as a profiler's kernel trace sees them);
+// both stay null when the ledger is off.  prof_end_ext closes the entry.
+void prof_begin_ext(nq_ctx* ctx, const char* name, double bytes, hipEvent_t* a, hipEvent_t* b);
+void prof_end_ext(nq_ctx* ctx);
 void prof_flush(nq_ctx* ctx);
 
 // ---------------------------------------------------------------- launchers (device code)

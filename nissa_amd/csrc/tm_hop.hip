@@ -18,12 +18,26 @@
 // x += alpha p, r -= alpha s and |r|^2; mode 3 writes a residual b - A x in FP64
 // plus its sloppy copy and |r|^2.
 
+#include <hip/hip_ext.h>
+
 #include <cstdlib>
 #include <string>
 
 #include "clover_block.hpp"
 
 namespace nq {
+
+// Hop launches: with the ledger on (ctx->hop_ev set by prof_begin_ext) the launch
+// records the kernel's own start and end, so that the ledger's time per launch is
+// the kernel's, as in a profiler's kernel trace, without the dispatch gap that two
+// stream events around the launch would add.
+#define NQ_HOP_LAUNCH(KERNEL, GRID, BLOCK, ...)                                                          \
+  do {                                                                                                  \
+    if (ctx->hop_ev[0])                                                                                 \
+      hipExtLaunchKernelGGL(KERNEL, GRID, BLOCK, 0, ctx->stream, ctx->hop_ev[0], ctx->hop_ev[1], 0, __VA_ARGS__); \
+    else                                                                                                \
+      hipLaunchKernelGGL(KERNEL, GRID, BLOCK, 0, ctx->stream, __VA_ARGS__);                             \
+  } while (0)
 
 template <class R>
 struct Half {
@@ -578,7 +592,7 @@ static int launch_rb(nq_ctx* ctx, int RB, int op, const SpinorView& out, const S
   if (RB <= 0) {
     // one right-hand side per wavefront (the kernel of the unbatched operators)
     const Launch L = launch_shape(g, nr);
-    hipLaunchKernelGGL((k_tm_hop<SP, LK, MODE, XP>), dim3((unsigned)L.blocks), dim3(L.threads), 0, ctx->stream, g, op,
+    NQ_HOP_LAUNCH((k_tm_hop<SP, LK, MODE, XP>), dim3((unsigned)L.blocks), dim3(L.threads), g, op,
                        out, in, U, anis, ua, epi, L.s, L.nchunks, ctx->partial);
     NQ_HIP(hipGetLastError());
     return NQ_OK;
@@ -590,9 +604,9 @@ static int launch_rb(nq_ctx* ctx, int RB, int op, const SpinorView& out, const S
   const int64_t nchunks = (g.Vh + 63) / 64;
   const dim3 grid((unsigned)((nchunks + 3) / 4 * sh.NRG)), blk(256);
   switch (RB) {
-  case 1: hipLaunchKernelGGL((k_tm_hop_rb<SP, LK, MODE, XP, 1>), grid, blk, 0, ctx->stream, g, op, out, in, U, anis, ua, epi, sh, nr, nchunks, ctx->partial); break;
-  case 2: hipLaunchKernelGGL((k_tm_hop_rb<SP, LK, MODE, XP, 2>), grid, blk, 0, ctx->stream, g, op, out, in, U, anis, ua, epi, sh, nr, nchunks, ctx->partial); break;
-  default: hipLaunchKernelGGL((k_tm_hop_rb<SP, LK, MODE, XP, 4>), grid, blk, 0, ctx->stream, g, op, out, in, U, anis, ua, epi, sh, nr, nchunks, ctx->partial); break;
+  case 1: NQ_HOP_LAUNCH((k_tm_hop_rb<SP, LK, MODE, XP, 1>), grid, blk, g, op, out, in, U, anis, ua, epi, sh, nr, nchunks, ctx->partial); break;
+  case 2: NQ_HOP_LAUNCH((k_tm_hop_rb<SP, LK, MODE, XP, 2>), grid, blk, g, op, out, in, U, anis, ua, epi, sh, nr, nchunks, ctx->partial); break;
+  default: NQ_HOP_LAUNCH((k_tm_hop_rb<SP, LK, MODE, XP, 4>), grid, blk, g, op, out, in, U, anis, ua, epi, sh, nr, nchunks, ctx->partial); break;
   }
   NQ_HIP(hipGetLastError());
   return NQ_OK;
@@ -670,7 +684,7 @@ static bool launch_persistent(nq_ctx* ctx, int op, const SpinorView& out, const 
     if (penv && penv[0] == '0') return false;
     const int64_t nblk = (L.nchunks + 3) / 4, slots = (int64_t)ctx->ncu * persist_per_cu<SP, LK, MODE, XP, CLV>();
     if (slots < 1 || nblk < 2 * slots) return false;
-    hipLaunchKernelGGL((k_tm_hop_lds_p<SP, LK, MODE, XP, CLV>), dim3((unsigned)slots), dim3(256), 0, ctx->stream,
+    NQ_HOP_LAUNCH((k_tm_hop_lds_p<SP, LK, MODE, XP, CLV>), dim3((unsigned)slots), dim3(256),
                        ctx->geo, op, out, in, U, anis, anis != 1.0 ? 1 : 0, epi, nblk, L.nchunks, ctx->partial);
     return true;
   } else {
@@ -688,7 +702,6 @@ static int go(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, 
   if (nr < 1 || nr > 16) return fail(NQ_EINVAL, "tm hop: right-hand sides per launch must be 1..16");
   const Launch L = launch_shape(g, nr);
   if (L.nchunks * nr > ctx->max_partials) return fail(NQ_EINVAL, "tm hop: reduction scratch too small");
-  hipEvent_t ev = nullptr;
   if constexpr (MODE != 3 && CLV == 0) {
     if (epi.batched || nr > 1) {
       // register-blocked right-hand sides (always for batched solves, so that a
@@ -698,27 +711,28 @@ static int go(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, 
       // 953 -> 879 us for 12 sources, the batched FP64 solve 95 -> 85 ms per source)
       int RB = 0;
       NQ_TRY((tuned_rb<SP, LK, XP>(ctx, op, in, U, anis, epi, nr, &RB)));
-      prof_begin(ctx, name, tm_hop_bytes(SP, LK, epi) * (double)g.Vh, &ev);
-      NQ_TRY((launch_rb<SP, LK, MODE, XP>(ctx, RB, op, out, in, U, anis, epi, nr)));
-      prof_end(ctx, ev);
+      prof_begin_ext(ctx, name, tm_hop_bytes(SP, LK, epi) * (double)g.Vh, &ctx->hop_ev[0], &ctx->hop_ev[1]);
+      const int rc = launch_rb<SP, LK, MODE, XP>(ctx, RB, op, out, in, U, anis, epi, nr);
+      prof_end_ext(ctx);
+      NQ_TRY(rc);
       if (MODE != 0) NQ_TRY(launch_finalize(ctx, nr, epi.st, epi.fin, epi.pred_reliable));
       return NQ_OK;
     }
   }
-  prof_begin(ctx, name, tm_hop_bytes(SP, LK, epi) * (double)g.Vh, &ev);
+  prof_begin_ext(ctx, name, tm_hop_bytes(SP, LK, epi) * (double)g.Vh, &ctx->hop_ev[0], &ctx->hop_ev[1]);
   // FP64: neighbours staged in LDS (measured 4 % faster at 32^3x64; the sloppy
   // kernels measured slower with staging and read their neighbours from memory)
   const char* lenv = getenv("NQ_LDS");
   if (SP == SP_F64 && nr == 1 && L.s.W == 4 && !(lenv && lenv[0] == '0')) {
     if (!launch_persistent<SP, LK, MODE, XP, CLV>(ctx, op, out, in, U, anis, epi, L))
-      hipLaunchKernelGGL((k_tm_hop_lds<SP, LK, MODE, XP, CLV>), dim3((unsigned)L.blocks), dim3(L.threads), 0, ctx->stream, g,
+      NQ_HOP_LAUNCH((k_tm_hop_lds<SP, LK, MODE, XP, CLV>), dim3((unsigned)L.blocks), dim3(L.threads), g,
                          op, out, in, U, anis, anis != 1.0 ? 1 : 0, epi, L.s, L.nchunks, ctx->partial);
   }
   else
-    hipLaunchKernelGGL((k_tm_hop<SP, LK, MODE, XP, CLV>), dim3((unsigned)L.blocks), dim3(L.threads), 0, ctx->stream, g, op,
+    NQ_HOP_LAUNCH((k_tm_hop<SP, LK, MODE, XP, CLV>), dim3((unsigned)L.blocks), dim3(L.threads), g, op,
                        out, in, U, anis, anis != 1.0 ? 1 : 0, epi, L.s, L.nchunks, ctx->partial);
+  prof_end_ext(ctx);
   NQ_HIP(hipGetLastError());
-  prof_end(ctx, ev);
   if (MODE != 0) NQ_TRY(launch_finalize(ctx, nr, epi.st, epi.fin, epi.pred_reliable));
   return NQ_OK;
 }
