@@ -156,6 +156,42 @@ __global__ void __launch_bounds__(256) k_pupdate(BlasArgs A, SpinorView p, Spino
   store_any<NC>(p, A.g.stride, m.e, pv, m.r);
 }
 
+// p_out = r + beta p_in for one right-hand side (p_in, p_out distinct buffers: the
+// one-source mixed CG alternates between two).  With x, every other iteration (xstep)
+// x += alpha_prev p_prev + alpha p_in, p_prev being the previous iteration's p still in
+// p_out: x is read and written half as often, and the x terms of an iteration that
+// does not apply them wait in p_out until the next one.
+template <int NC>
+__global__ void __launch_bounds__(256) k_pupdate_pair(BlasArgs A, SpinorView pin, SpinorView pout, SpinorView r,
+                                                      SpinorView x)
+{
+  const Map m = map_thread(A.s, A.g.Vh);
+  const CgState* s = A.st + m.r;
+  if (s->done || !m.active || s->reliable) return;
+  const double beta = s->beta;
+  cx<double> pv[NC], rv[NC];
+  load_any<NC>(pin, A.g.stride, m.e, pv, m.r);
+  load_any<NC>(r, A.g.stride, m.e, rv, m.r);
+  if (x.data && s->xstep) {
+    const double a0 = s->alpha_prev, a1 = s->alpha;
+    cx<double> xv[NC], qv[NC];
+    load_any<NC>(x, A.g.stride, m.e, xv, m.r);
+    load_any<NC>(pout, A.g.stride, m.e, qv, m.r);
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      xv[k].re = fma(a1, pv[k].re, fma(a0, qv[k].re, xv[k].re));
+      xv[k].im = fma(a1, pv[k].im, fma(a0, qv[k].im, xv[k].im));
+    }
+    store_any<NC>(x, A.g.stride, m.e, xv, m.r);
+  }
+#pragma unroll
+  for (int k = 0; k < NC; k++) {
+    pv[k].re = fma(beta, pv[k].re, rv[k].re);
+    pv[k].im = fma(beta, pv[k].im, rv[k].im);
+  }
+  store_any<NC>(pout, A.g.stride, m.e, pv, m.r);
+}
+
 // x += alpha p ; r -= alpha s ; |r|^2 (plain CG update, cg_invert_template_threaded.hpp:82-102)
 template <int NC>
 __global__ void __launch_bounds__(256) k_cg_update(BlasArgs A, SpinorView x, SpinorView p, SpinorView r, SpinorView s)
@@ -186,7 +222,8 @@ __global__ void __launch_bounds__(256) k_cg_update(BlasArgs A, SpinorView x, Spi
 
 // x64 += xs ; xs = 0 (reliable update), only for right-hand sides flagged reliable
 template <int NC>
-__global__ void __launch_bounds__(256) k_reliable_accum(BlasArgs A, SpinorView x64, SpinorView xs, SpinorView p)
+__global__ void __launch_bounds__(256) k_reliable_accum(BlasArgs A, SpinorView x64, SpinorView xs, SpinorView p,
+                                                         SpinorView p_prev)
 {
   const Map m = map_thread(A.s, A.g.Vh);
   const CgState* s = A.st + m.r;
@@ -194,6 +231,17 @@ __global__ void __launch_bounds__(256) k_reliable_accum(BlasArgs A, SpinorView x
   cx<double> a[NC], b[NC];
   load_any<NC>(x64, A.g.stride, m.e, a, m.r);
   load_any<NC>(xs, A.g.stride, m.e, b, m.r);
+  if (p_prev.data && s->xlag) {
+    // paired x updates: the previous iteration's alpha p, left to the skipped update
+    const double alpha = s->alpha_prev;
+    cx<double> pv[NC];
+    load_any<NC>(p_prev, A.g.stride, m.e, pv, m.r);
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      b[k].re = fma(alpha, pv[k].re, b[k].re);
+      b[k].im = fma(alpha, pv[k].im, b[k].im);
+    }
+  }
   if (p.data) {
     // the last iteration's alpha p, not yet folded into xs
     const double alpha = s->alpha;
@@ -433,8 +481,29 @@ int launch_clear_reliable(nq_ctx* ctx, CgState* st, int nr)
   return NQ_OK;
 }
 
+int launch_cg_pupdate_pair(nq_ctx* ctx, const SpinorView& p_in, const SpinorView& p_out, const SpinorView& r,
+                           CgState* st, const SpinorView& x)
+{
+  BlasArgs A;
+  Launch L;
+  NQ_TRY(make_args(ctx, 1, st, FIN_NONE, 1, &A, &L));
+  if (p_in.data == p_out.data) return fail(NQ_EINVAL, "pupdate pair: p_in and p_out must differ");
+  if (p_in.ncomp != p_out.ncomp || (x.data && x.ncomp != p_in.ncomp)) return fail(NQ_EINVAL, "pupdate pair: component mismatch");
+  hipEvent_t ev = nullptr;
+  // x and the previous p every other iteration: half their bytes on average
+  const double xb = x.data ? 0.5 * (2.0 * site_bytes(x) + site_bytes(p_out)) : 0.0;
+  prof_begin(ctx, x.data ? "blas_xpupdate" : "blas_pupdate",
+             (site_bytes(p_in) + site_bytes(p_out) + site_bytes(r) + xb) * (double)ctx->geo.Vh, &ev);
+  const dim3 grid((unsigned)L.blocks), blk(L.threads);
+  if (p_in.ncomp == 12) hipLaunchKernelGGL(k_pupdate_pair<12>, grid, blk, 0, ctx->stream, A, p_in, p_out, r, x);
+  else return fail(NQ_EINVAL, "pupdate pair: spincolor fields expected");
+  NQ_HIP(hipGetLastError());
+  prof_end(ctx, ev);
+  return NQ_OK;
+}
+
 int launch_reliable_accum(nq_ctx* ctx, const SpinorView& x64, const SpinorView& xs, CgState* st, int nr,
-                          const SpinorView& p)
+                          const SpinorView& p, const SpinorView& p_prev)
 {
   BlasArgs A;
   Launch L;
@@ -443,8 +512,8 @@ int launch_reliable_accum(nq_ctx* ctx, const SpinorView& x64, const SpinorView& 
   prof_begin(ctx, "blas_reliable_accum",
              nr * (2 * (site_bytes(x64) + site_bytes(xs)) + (p.data ? site_bytes(p) : 0)) * (double)ctx->geo.Vh, &ev);
   const dim3 grid((unsigned)L.blocks), blk(L.threads);
-  if (x64.ncomp == 12) hipLaunchKernelGGL(k_reliable_accum<12>, grid, blk, 0, ctx->stream, A, x64, xs, p);
-  else hipLaunchKernelGGL(k_reliable_accum<3>, grid, blk, 0, ctx->stream, A, x64, xs, p);
+  if (x64.ncomp == 12) hipLaunchKernelGGL(k_reliable_accum<12>, grid, blk, 0, ctx->stream, A, x64, xs, p, p_prev);
+  else hipLaunchKernelGGL(k_reliable_accum<3>, grid, blk, 0, ctx->stream, A, x64, xs, p, p_prev);
   NQ_HIP(hipGetLastError());
   prof_end(ctx, ev);
   return NQ_OK;

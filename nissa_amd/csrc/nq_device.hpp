@@ -431,6 +431,7 @@ __device__ inline void finalize(CgState* st, int fin, double total)
   case FIN_STORE1: st->extra[1] = total; break;
   case FIN_PAP:
     st->pAp = total;
+    st->alpha_prev = st->alpha;
     st->alpha = st->rr / total;
     if (!(total > 0.0) || isnan(total)) { st->nan = 1; st->done = 1; }
     break;
@@ -457,6 +458,9 @@ __device__ inline void finalize(CgState* st, int fin, double total)
     } else {
       st->beta = rr_new / st->rr;
       st->rr = rr_new;
+      // paired x updates: this p update applies two terms when the last one left its own
+      st->xstep = st->xlag;
+      st->xlag = !st->xlag;
     }
     break;
   }
@@ -468,6 +472,7 @@ __device__ inline void finalize(CgState* st, int fin, double total)
     st->beta = rr_true / st->rr;
     st->rr = rr_true;
     st->r0 = sqrt(rr_true);
+    st->xlag = 0;  // the reliable update has taken every x term
     if (fin == FIN_RR_TRUE) st->reliable = 0;
     if (isnan(rr_true)) { st->nan = 1; st->done = 1; }
     if (rr_true < st->target || st->iter >= st->niter) st->done = 1;

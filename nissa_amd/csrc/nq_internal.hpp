@@ -99,7 +99,11 @@ struct CgState {
   double r0, delta;     // reliable-update reference |r| and threshold factor
   double true_rr;       // last FP64 residual
   double extra[6];
+  double alpha_prev;    // alpha of the previous iteration (paired x updates)
   int iter, niter, done, reliable, nreliable, nan, mixed, flags;
+  // paired x updates of the one-source mixed CG: xlag = the last p update left its
+  // alpha p term to the next one; xstep = the coming p update applies both terms
+  int xlag, xstep;
 };
 
 constexpr int kMaxShift = 16;
@@ -403,7 +407,12 @@ int launch_clear_reliable(nq_ctx* ctx, CgState* st, int nr);
 int launch_cg_update(nq_ctx* ctx, const SpinorView& x, const SpinorView& p, const SpinorView& r,
                      const SpinorView& s, CgState* st, int fin, int nr = 1);
 int launch_reliable_accum(nq_ctx* ctx, const SpinorView& x64, const SpinorView& xs, CgState* st, int nr = 1,
-                          const SpinorView& p = SpinorView());
+                          const SpinorView& p = SpinorView(), const SpinorView& p_prev = SpinorView());
+// p_out = r + beta p_in (one right-hand side, p_in and p_out distinct; skipped when done or
+// reliable-pending); with x, when the state's xstep is set, first
+// x += alpha_prev p_prev + alpha p_in, p_prev being p_out's old content
+int launch_cg_pupdate_pair(nq_ctx* ctx, const SpinorView& p_in, const SpinorView& p_out, const SpinorView& r,
+                           CgState* st, const SpinorView& x = SpinorView());
 int launch_convert(nq_ctx* ctx, const SpinorView& out, const SpinorView& in, int nr = 1,
                    CgState* st = nullptr, int pred = 0);
 int launch_zero(nq_ctx* ctx, const SpinorView& v, int nr = 1);

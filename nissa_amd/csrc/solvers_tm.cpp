@@ -301,7 +301,7 @@ static int cg_kernsq_mixed(const TmSetup& S64, const TmSetup& Ss, int sp, double
                            nq_solve_stats* out, const SpinorView* kx = nullptr)
 {
   nq_ctx* ctx = S64.ctx;
-  SpinorView r64, t64, tmp64, xs, rs, p, t1s, tmps;
+  SpinorView r64, t64, tmp64, xs, rs, p, pb, t1s, tmps;
   NQ_TRY(alloc_spinor(ctx, "cg.r", SP_F64, 12, &r64));
   if (kx) t64 = *kx;
   else NQ_TRY(alloc_spinor(ctx, "cg.t1", SP_F64, 12, &t64));
@@ -311,6 +311,10 @@ static int cg_kernsq_mixed(const TmSetup& S64, const TmSetup& Ss, int sp, double
   NQ_TRY(alloc_spinor(ctx, "mcg.xs", SP_F32, 12, &xs));
   NQ_TRY(alloc_spinor(ctx, "mcg.rs", SP_F32, 12, &rs));
   NQ_TRY(alloc_spinor(ctx, sp == SP_F16 ? "mcg.p16" : "mcg.p32", sp, 12, &p));
+  // the search direction alternates between two buffers, so that the p update can
+  // apply two iterations' x terms at once (iteration j reads p[j % 2], writes p[(j+1) % 2])
+  NQ_TRY(alloc_spinor(ctx, sp == SP_F16 ? "mcg.p16b" : "mcg.p32b", sp, 12, &pb));
+  const SpinorView pv[2] = {p, pb};
   NQ_TRY(alloc_spinor(ctx, sp == SP_F16 ? "mcg.t16" : "mcg.t32", sp, 12, &t1s));
   NQ_TRY(alloc_spinor(ctx, sp == SP_F16 ? "mcg.e16" : "mcg.e32", sp, 12, &tmps));
   Timer T(ctx->stream);
@@ -344,7 +348,11 @@ static int cg_kernsq_mixed(const TmSetup& S64, const TmSetup& Ss, int sp, double
   // next update is expected, and then runs the FP64 update unconditionally.
   int chunk = 8, launched = 0;
   while (!ctx->state_host[0].done) {
+    // iteration j of the solve uses p[j % 2]; the chunk starts at the device's count
+    const int par = ctx->state_host[0].iter & 1;
     for (int i = 0; i < chunk; i++) {
+      const SpinorView& p = pv[(par + i) & 1];
+      const SpinorView& pn = pv[(par + i + 1) & 1];
       HopEpi a = epi_inv_dee(Ss.kappa, -mu);
       a.st = st;
       a.pred_reliable = 2;
@@ -367,17 +375,20 @@ static int cg_kernsq_mixed(const TmSetup& S64, const TmSetup& Ss, int sp, double
       b2.p = p;
       b2.r = rs;
       NQ_TRY(launch_tm_hop(ctx, NQ_ODD, none, tmps, Ss.U, Ss.anis, b2));
-      // xs += alpha p ; p = rs + beta p (one read of p)
-      NQ_TRY(launch_cg_pupdate(ctx, p, rs, st, 1, 1, xs));
+      // p_next = rs + beta p and, every other iteration, xs += the two iterations'
+      // alpha p terms
+      NQ_TRY(launch_cg_pupdate_pair(ctx, p, pn, rs, st, xs));
     }
     launched += chunk;
     NQ_TRY(get_states(ctx, 0, 1));
     if (ctx->state_host[0].reliable && !ctx->state_host[0].done) {
-      // reliable update (the sloppy update of this iteration was skipped):
-      // x64 += xs + alpha p, xs = 0; r64 = b - A x64 in FP64; rs = r64; p = rs + beta p
-      NQ_TRY(launch_reliable_accum(ctx, x64, xs, st, 1, p));
+      // reliable update after iteration j = iter - 1, whose p update was skipped:
+      // x64 += xs + (the previous iteration's alpha p when still owed) + alpha p, xs = 0;
+      // r64 = b - A x64 in FP64; rs = r64; p[(j+1) % 2] = rs + beta p[j % 2]
+      const int j = ctx->state_host[0].iter - 1;
+      NQ_TRY(launch_reliable_accum(ctx, x64, xs, st, 1, pv[j & 1], pv[(j + 1) & 1]));
       NQ_TRY(residual_kernsq(S64, mu, r64, rs, x64, b, t64, tmp64, st, FIN_RR_TRUE, 1));
-      NQ_TRY(launch_cg_pupdate(ctx, p, rs, st));
+      NQ_TRY(launch_cg_pupdate_pair(ctx, pv[j & 1], pv[(j + 1) & 1], rs, st));
       NQ_TRY(get_states(ctx, 0, 1));
     }
     const CgState& h = ctx->state_host[0];
