@@ -613,4 +613,32 @@ __device__ __forceinline__ double block_sum_partials(const double* pr, int64_t n
   return vt < 64 ? wave_sum(vl[vt]) : 0.0;
 }
 
+// block_sum_partials with 256 threads: each thread takes the chains of the virtual
+// lanes t, t+256, t+512, t+768 and the LDS tree the same pairs, so the sum has the
+// bits of the 1024-thread version (vl: 1024 doubles)
+__device__ __forceinline__ double block_sum_partials_256(const double* pr, int64_t nchunks, double* vl)
+{
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int vt = t + 256 * q;
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    int64_t c = vt;
+    for (; c + 3072 < nchunks; c += 4096) {
+      a0 += pr[c];
+      a1 += pr[c + 1024];
+      a2 += pr[c + 2048];
+      a3 += pr[c + 3072];
+    }
+    for (; c < nchunks; c += 1024) a0 += pr[c];
+    vl[vt] = (a0 + a1) + (a2 + a3);
+  }
+  __syncthreads();
+  for (int h = 512; h >= 64; h >>= 1) {
+    for (int vt = t; vt < h; vt += 256) vl[vt] += vl[vt + h];
+    __syncthreads();
+  }
+  return t < 64 ? wave_sum(vl[t]) : 0.0;
+}
+
 }  // namespace nq

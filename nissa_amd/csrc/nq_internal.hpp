@@ -169,6 +169,11 @@ struct HopEpi {
   SpinorView aux, aux2;
   SpinorView x, p, r, rs;
   CgState* st = nullptr;
+  // finalisation of the previous launch's reduction (partials left with FIN_PARTIAL)
+  // carried out by one extra workgroup of this launch (plain one-RHS hop, mode 0;
+  // other kernels launch it separately): fold_fin on st with predicate fold_pred
+  int fold_fin = FIN_NONE;
+  int fold_pred = 0;
   // twisted clover blocks in the epilogue (clover_block.hpp): blk_mode 1 applies the
   // inverse blocks blk (72 complex per site) to the hop before c1; blk_mode 2 applies
   // the blocks z + clover term (blk: 36 complex per site, z = 1/(2 blk_kappa) +- i

@@ -357,15 +357,19 @@ static int cg_kernsq_mixed(const TmSetup& S64, const TmSetup& Ss, int sp, double
       a.st = st;
       a.pred_reliable = 2;
       NQ_TRY(launch_tm_hop(ctx, NQ_EVEN, tmps, p, Ss.U, Ss.anis, a));
+      // <p, K K^+ p> = |K(-mu) p|^2 reduced here, finalised (alpha) by an extra
+      // workgroup of the next hop, which does not need alpha
       HopEpi b1 = epi_kout(Ss.kappa, -mu, p);
       b1.mode = 1;
       b1.st = st;
-      b1.fin = FIN_PAP;
+      b1.fin = FIN_PARTIAL;
       b1.pred_reliable = 2;
       NQ_TRY(launch_tm_hop(ctx, NQ_ODD, t1s, tmps, Ss.U, Ss.anis, b1));
       HopEpi a2 = epi_inv_dee(Ss.kappa, mu);
       a2.st = st;
       a2.pred_reliable = 2;
+      a2.fold_fin = FIN_PAP;
+      a2.fold_pred = 2;
       NQ_TRY(launch_tm_hop(ctx, NQ_EVEN, tmps, t1s, Ss.U, Ss.anis, a2));
       HopEpi b2 = epi_kout(Ss.kappa, mu, t1s);
       b2.mode = 2;

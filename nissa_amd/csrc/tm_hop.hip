@@ -260,6 +260,19 @@ __global__ void __launch_bounds__(256, (SP != SP_F64 ? 2 : 1)) k_tm_hop(Geo g, i
                                                double* partial)
 {
   using R = RealOf<SP>;
+  if constexpr (MODE == 0) {
+    // the workgroup past the last block finalises the previous launch's reduction
+    // (one right-hand side; the hop itself never reads what it writes)
+    if (epi.fold_fin != FIN_NONE && xcd_block() == (int64_t)gridDim.x - 1) {
+      __shared__ double vl[1024];
+      CgState* fs = epi.st;
+      if (!(fs->done || (epi.fold_pred == 1 && !fs->reliable) || (epi.fold_pred == 2 && fs->reliable))) {
+        const double tot = block_sum_partials_256(partial, nchunks, vl);
+        if (threadIdx.x == 0) finalize(fs, epi.fold_fin, tot);
+      }
+      return;
+    }
+  }
   const Map m = map_thread(S, g.Vh);
   CgState* st = epi.st ? epi.st + m.r : nullptr;
   const bool skip = st && rhs_idle(st, epi.pred_reliable);
@@ -715,11 +728,13 @@ static int go(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, 
       const int rc = launch_rb<SP, LK, MODE, XP>(ctx, RB, op, out, in, U, anis, epi, nr);
       prof_end_ext(ctx);
       NQ_TRY(rc);
-      if (MODE != 0) NQ_TRY(launch_finalize(ctx, nr, epi.st, epi.fin, epi.pred_reliable));
+      if (epi.fold_fin != FIN_NONE) NQ_TRY(launch_finalize(ctx, nr, epi.st, epi.fold_fin, epi.fold_pred));
+      if (MODE != 0 && epi.fin != FIN_PARTIAL) NQ_TRY(launch_finalize(ctx, nr, epi.st, epi.fin, epi.pred_reliable));
       return NQ_OK;
     }
   }
   prof_begin_ext(ctx, name, tm_hop_bytes(SP, LK, epi) * (double)g.Vh, &ctx->hop_ev[0], &ctx->hop_ev[1]);
+  bool fold_here = false;
   // FP64: neighbours staged in LDS (measured 4 % faster at 32^3x64; the sloppy
   // kernels measured slower with staging and read their neighbours from memory)
   const char* lenv = getenv("NQ_LDS");
@@ -728,12 +743,16 @@ static int go(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, 
       NQ_HOP_LAUNCH((k_tm_hop_lds<SP, LK, MODE, XP, CLV>), dim3((unsigned)L.blocks), dim3(L.threads), g,
                          op, out, in, U, anis, anis != 1.0 ? 1 : 0, epi, L.s, L.nchunks, ctx->partial);
   }
-  else
-    NQ_HOP_LAUNCH((k_tm_hop<SP, LK, MODE, XP, CLV>), dim3((unsigned)L.blocks), dim3(L.threads), g, op,
-                       out, in, U, anis, anis != 1.0 ? 1 : 0, epi, L.s, L.nchunks, ctx->partial);
+  else {
+    // one extra workgroup when it carries the finalisation of the previous reduction
+    fold_here = MODE == 0 && epi.fold_fin != FIN_NONE && nr == 1 && epi.st;
+    NQ_HOP_LAUNCH((k_tm_hop<SP, LK, MODE, XP, CLV>), dim3((unsigned)(L.blocks + (fold_here ? 1 : 0))),
+                  dim3(L.threads), g, op, out, in, U, anis, anis != 1.0 ? 1 : 0, epi, L.s, L.nchunks, ctx->partial);
+  }
   prof_end_ext(ctx);
   NQ_HIP(hipGetLastError());
-  if (MODE != 0) NQ_TRY(launch_finalize(ctx, nr, epi.st, epi.fin, epi.pred_reliable));
+  if (epi.fold_fin != FIN_NONE && !fold_here) NQ_TRY(launch_finalize(ctx, 1, epi.st, epi.fold_fin, epi.fold_pred));
+  if (MODE != 0 && epi.fin != FIN_PARTIAL) NQ_TRY(launch_finalize(ctx, nr, epi.st, epi.fin, epi.pred_reliable));
   return NQ_OK;
 }
 
