@@ -616,10 +616,17 @@ static const double kOne[2][2] = {{1, 0}, {1, 0}};
 // hop in its epilogue (te2 = D_ee(dag)^-1 H in), the second the D_oo(dag) blocks to
 // `in` read at its own site (out = g5 (D_oo in - 1/4 H te2)), so neither the hop nor
 // the D_oo product goes through memory.
-static int clov_kern(const ClovSetup& s, int dag, const SpinorView& out, const SpinorView& in, HopEpi last)
+// The first hop idles when the last one does (same state and predicate); fold_fin:
+// the finalisation that the first hop carries for the reduction launched before
+static int clov_kern(const ClovSetup& s, int dag, const SpinorView& out, const SpinorView& in, HopEpi last,
+                     int fold_fin = FIN_NONE)
 {
   nq_ctx* c = s.S.ctx;
   HopEpi h;
+  h.st = last.st;
+  h.pred_reliable = last.pred_reliable;
+  h.fold_fin = fold_fin;
+  h.fold_pred = last.pred_reliable;
   h.blk = s.inv_e;
   h.blk_mode = 1;
   h.blk_dag = dag;
@@ -1975,6 +1982,10 @@ static int clov_cg_mixed(const ClovSetup& s64, const ClovSetup& s32, int sp, int
   NQ_TRY(alloc_spinor(ctx, "clov.xs32", SP_F32, 12, &xs));
   NQ_TRY(alloc_spinor(ctx, "clov.rs32", SP_F32, 12, &rs));
   NQ_TRY(alloc_spinor(ctx, sp == SP_F16 ? "clov.p16" : "clov.p32", sp, 12, &p));
+  // two search-direction buffers for the paired x updates (as in the twisted-mass CG)
+  SpinorView pb;
+  NQ_TRY(alloc_spinor(ctx, sp == SP_F16 ? "clov.p16b" : "clov.p32b", sp, 12, &pb));
+  const SpinorView pv[2] = {p, pb};
   NQ_TRY(alloc_spinor(ctx, sp == SP_F16 ? "clov.t1s16" : "clov.t1s32", sp, 12, &t1s));
   Timer T(ctx->stream);
   double b2 = 0;
@@ -2008,11 +2019,16 @@ static int clov_cg_mixed(const ClovSetup& s64, const ClovSetup& s32, int sp, int
   const double rr_init = ctx->state_host[0].rr;
   int chunk = 8, launched = 0;
   while (!ctx->state_host[0].done) {
+    const int par = ctx->state_host[0].iter & 1;  // iteration j reads p[j % 2]
     for (int i = 0; i < chunk; i++) {
+      const SpinorView& p = pv[(par + i) & 1];
+      const SpinorView& pn = pv[(par + i + 1) & 1];
+      // <p, K K^+ p> reduced by the second hop, finalised by an extra workgroup of the
+      // third (which does not need alpha)
       HopEpi e1;
       e1.mode = 1;
       e1.st = st;
-      e1.fin = FIN_PAP;
+      e1.fin = FIN_PARTIAL;
       e1.pred_reliable = 2;
       NQ_TRY(clov_kern(s32, 1, t1s, p, e1));
       HopEpi e2;
@@ -2022,16 +2038,17 @@ static int clov_cg_mixed(const ClovSetup& s64, const ClovSetup& s32, int sp, int
       e2.pred_reliable = 2;
       e2.p = p;
       e2.r = rs;
-      NQ_TRY(clov_kern(s32, 0, none, t1s, e2));
-      NQ_TRY(launch_cg_pupdate(ctx, p, rs, st, 1, 1, xs));
+      NQ_TRY(clov_kern(s32, 0, none, t1s, e2, FIN_PAP));
+      NQ_TRY(launch_cg_pupdate_pair(ctx, p, pn, rs, st, xs));
     }
     launched += chunk;
     NQ_TRY(get_states(ctx, 0, 1));
     if (ctx->state_host[0].reliable && !ctx->state_host[0].done) {
-      NQ_TRY(launch_reliable_accum(ctx, x64, xs, st, 1, p));
+      const int j = ctx->state_host[0].iter - 1;  // the flagged iteration, its p update skipped
+      NQ_TRY(launch_reliable_accum(ctx, x64, xs, st, 1, pv[j & 1], pv[(j + 1) & 1]));
       NQ_TRY(residual(FIN_RR_TRUE));
       NQ_TRY(launch_convert(ctx, rs, r64));
-      NQ_TRY(launch_cg_pupdate(ctx, p, rs, st));
+      NQ_TRY(launch_cg_pupdate_pair(ctx, pv[j & 1], pv[(j + 1) & 1], rs, st));
       NQ_TRY(get_states(ctx, 0, 1));
     }
     const CgState& h = ctx->state_host[0];
