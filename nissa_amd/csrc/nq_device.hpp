@@ -517,13 +517,14 @@ __device__ __forceinline__ int64_t xcd_block()
   return (int64_t)xcd * q8 + (xcd < r8 ? xcd : r8) + i;
 }
 
-__device__ __forceinline__ Map map_thread(const Shape& S, int64_t Vh)
+// lb: the logical block (xcd_block() for a one-shot launch; a persistent kernel
+// passes the block it is working on)
+__device__ __forceinline__ Map map_thread_lb(const Shape& S, int64_t Vh, int64_t lb)
 {
   Map m;
   // the wave index is uniform across the wavefront: keep r and chunk in SGPRs so
   // that buffer descriptors built from them stay scalar
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int64_t lb = xcd_block();
   const int rg = (int)(lb % S.NRG);
   m.r = rg * S.NRB + wave % S.NRB;
   m.chunk = (lb / S.NRG) * S.W + wave / S.NRB;
@@ -540,6 +541,8 @@ __device__ __forceinline__ Map map_thread(const Shape& S, int64_t Vh)
   m.e = (uint32_t)e;
   return m;
 }
+
+__device__ __forceinline__ Map map_thread(const Shape& S, int64_t Vh) { return map_thread_lb(S, Vh, xcd_block()); }
 
 // Thread mapping of the register-blocked (multi-RHS) kernels (Shape W = 4, NRB = 1):
 // with g.rb_bpc > 0 the 256-site blocks are visited z slab by z slab, logical block

@@ -351,6 +351,14 @@ int launch_cgm_vec3(nq_ctx* ctx, const SpinorView& r, const SpinorView& p, const
 // systems idle)
 int launch_st_cgtail(nq_ctx* ctx, const SpinorView& t, const LinkView& U, const double* coef_dev, const SpinorView& p,
                      const SpinorView& x, const SpinorView& r, CgState* st, int fin, int nr);
+// niters multi-shift iterations of nr sources in one cooperative launch (the same
+// block code and bits as launch_cgm_iteration); bar: device scratch of
+// cgm_grid_bar_bytes() whose `abort` word (third int) is nonzero after the launch when
+// a grid barrier waited more than spin_limit polls
+int launch_cgm_persistent(nq_ctx* ctx, const LinkView& U, const SpinorView& p, const SpinorView& t,
+                          const SpinorView& r, const SpinorView& sol, const SpinorView& ps, int64_t sstride,
+                          void* states, int nr, int niters, void* bar, unsigned spin_limit);
+int cgm_grid_bar_bytes();
 // sol_k -= betas_k ps_k for the (source, shift) pairs whose CgmState owe_k is 1
 int launch_cgm_settle(nq_ctx* ctx, const SpinorView& sol, const SpinorView& ps, int64_t sstride, void* states, int nr);
 int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, const SpinorView& t,

@@ -289,6 +289,26 @@ static int st_cgm_batched(nq_ctx* ctx, const LinkView& U, int nr, int nshift, co
   } graphs;
   const char* genv = getenv("NQ_GRAPHS");
   const bool use_graphs = !(genv && genv[0] == '0');
+  // NQ_CGM_PERSIST=1: each chunk of iterations runs as one cooperative launch
+  // (launch_cgm_persistent; same bits); NQ_CGM_SPIN bounds its barrier waits
+  const char* penv = getenv("NQ_CGM_PERSIST");
+  const bool persist = penv && penv[0] == '1';
+  unsigned spin_limit = 1u << 22;
+  if (const char* e = getenv("NQ_CGM_SPIN")) spin_limit = (unsigned)strtoul(e, nullptr, 10);
+  void* bar = nullptr;
+  int* abort_host = nullptr;
+  if (persist) {
+    NQ_TRY(ctx_temp(ctx, "cgm.bar", (size_t)cgm_grid_bar_bytes(), &bar));
+    NQ_HIP(hipHostMalloc((void**)&abort_host, sizeof(int), hipHostMallocDefault));
+    *abort_host = 0;
+  }
+  struct HostFreeInt {
+    int* p;
+    ~HostFreeInt()
+    {
+      if (p) (void)hipHostFree(p);
+    }
+  } hfi{abort_host};
   int chunk = ctx->profiling ? 2 : 16, launched = 0;
   for (;;) {
     int nbusy = 0, nrun = 0;  // as of the last poll (the ledger's byte count)
@@ -297,7 +317,11 @@ static int st_cgm_batched(nq_ctx* ctx, const LinkView& U, int nr, int nshift, co
         nbusy++;
         for (int k = 0; k < nshift; k++) nrun += hst[i].run[k] != 0;
       }
-    if (use_graphs && launched > 0 && !ctx->profiling) {
+    if (persist) {
+      NQ_TRY(launch_cgm_persistent(ctx, U, p, t, r, sol, ps, csz, dst, nr, chunk, bar, spin_limit));
+      NQ_HIP(hipMemcpyAsync(abort_host, (char*)bar + 2 * sizeof(unsigned), sizeof(int), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    } else if (use_graphs && launched > 0 && !ctx->profiling) {
       for (int k = 7, left = chunk; k >= 0; k--) {
         const int n = 1 << k;
         for (; left >= n; left -= n) {
@@ -332,6 +356,7 @@ static int st_cgm_batched(nq_ctx* ctx, const LinkView& U, int nr, int nshift, co
     launched += chunk;
     NQ_HIP(hipMemcpyAsync(hst, dst, sizeof(CgmState) * nr, hipMemcpyDeviceToHost, ctx->stream));
     NQ_HIP(hipStreamSynchronize(ctx->stream));
+    if (persist && *abort_host) return fail(NQ_EDEVICE, "persistent CGM: a grid barrier waited past its limit");
     // Shifts that stopped, and sources that ended, right after a C2 that left its
     // sol_k terms to the next one: apply those terms before anything reads sol.
     bool owes = false, flags = false;

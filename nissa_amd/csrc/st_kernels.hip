@@ -168,14 +168,16 @@ __device__ __forceinline__ uint32_t stage_sources(char* lds, const Geo& g, const
 // indices (each input site is staged by exactly one workgroup).
 // Wavefront w of a workgroup takes chunk 4*cb + w and the RB right-hand sides of
 // group m.r (Shape W = 4, NRB = 1, NR = NRG = number of RB groups).
+// (one logical block lb; lds: RB staging areas; shared by k_st_hop and the persistent CGM)
 template <int SP, int LK, int MODE, int RB>
-__global__ void __launch_bounds__(256) k_st_hop(Geo g, int op, SpinorView out, SpinorView in, LinkView U, double c1,
-                                                 double c2, const double* coef, SpinorView aux, int write, Shape S,
-                                                 int nr, int64_t nchunks, double* partial, const int* skip_flags,
-                                                 int skip_stride, int skip_n)
+__device__ __forceinline__ void st_hop_block(int64_t lb, char* lds, const Geo& g, int op, const SpinorView& out,
+                                             const SpinorView& in, const LinkView& U, double c1, double c2,
+                                             const double* coef, const SpinorView& aux, int write, const Shape& S,
+                                             int nr, int64_t nchunks, double* partial, const int* skip_flags,
+                                             int skip_stride, int skip_n)
 {
   using R = RealOfS<SP>;
-  const Map m = map_thread(S, g.Vh);
+  const Map m = map_thread_lb(S, g.Vh, lb);
   const int r0 = m.r * RB, nrb = min(RB, nr - r0);
   unsigned busy = 0;
 #pragma unroll
@@ -189,7 +191,6 @@ __global__ void __launch_bounds__(256) k_st_hop(Geo g, int op, SpinorView out, S
     }
     return;
   }
-  __shared__ __attribute__((aligned(16))) char lds[RB * StStage<SP>::BYTES];
   const uint32_t e0 = stage_sources<SP, RB>(lds, g, in, m, r0, nrb);
   // sites past the end (a partial last chunk) compute nothing but take part in
   // the wavefront reductions with zero
@@ -234,6 +235,17 @@ __global__ void __launch_bounds__(256) k_st_hop(Geo g, int op, SpinorView out, S
     }
     if constexpr (MODE != 0) store_partial(red, partial, nchunks, r, m.chunk);
   }
+}
+
+template <int SP, int LK, int MODE, int RB>
+__global__ void __launch_bounds__(256) k_st_hop(Geo g, int op, SpinorView out, SpinorView in, LinkView U, double c1,
+                                                 double c2, const double* coef, SpinorView aux, int write, Shape S,
+                                                 int nr, int64_t nchunks, double* partial, const int* skip_flags,
+                                                 int skip_stride, int skip_n)
+{
+  __shared__ __attribute__((aligned(16))) char lds[RB * StStage<SP>::BYTES];
+  st_hop_block<SP, LK, MODE, RB>(xcd_block(), lds, g, op, out, in, U, c1, c2, coef, aux, write, S, nr, nchunks,
+                                 partial, skip_flags, skip_stride, skip_n);
 }
 
 // Second half of the batched CG matvec fused with the CG update: at even site e of
@@ -304,12 +316,13 @@ __global__ void __launch_bounds__(256) k_st_cgtail(Geo g, SpinorView t, LinkView
 // sol/ps of shift k of rhs r: base + r*rstride + k*sstride (elements)
 // (RB right-hand sides per wavefront as in k_st_hop)
 template <int SP, int LK, int RB>
-__global__ void __launch_bounds__(256) k_cgm_hop2(Geo g, SpinorView t, LinkView U, SpinorView rv, SpinorView sol,
-                                                   SpinorView ps, int64_t sstride, CgmState* st, Shape SH, int nr,
-                                                   int64_t nchunks, double* partial)
+__device__ __forceinline__ void cgm_hop2_block(int64_t lb, char* lds, const Geo& g, const SpinorView& t,
+                                               const LinkView& U, const SpinorView& rv, const SpinorView& sol,
+                                               const SpinorView& ps, int64_t sstride, const CgmState* st,
+                                               const Shape& SH, int nr, int64_t nchunks, double* partial)
 {
   using R = RealOfS<SP>;
-  const Map m = map_thread(SH, g.Vh);
+  const Map m = map_thread_lb(SH, g.Vh, lb);
   const int r0 = m.r * RB, nrb = min(RB, nr - r0);
   unsigned busy = 0;
 #pragma unroll
@@ -321,7 +334,6 @@ __global__ void __launch_bounds__(256) k_cgm_hop2(Geo g, SpinorView t, LinkView 
       if (b < nrb) store_partial(0.0, partial, nchunks, r0 + b, m.chunk);
     return;
   }
-  __shared__ __attribute__((aligned(16))) char lds[RB * StStage<SP>::BYTES];
   const uint32_t e0 = stage_sources<SP, RB>(lds, g, t, m, r0, nrb);
   cx<R> o[RB][3];
   if (m.active) st_hop_site<SP, LK, R, RB>(g, 0, m.e, t, r0, nrb, U, o, lds, e0);
@@ -420,15 +432,25 @@ __global__ void __launch_bounds__(256) k_cgm_hop2(Geo g, SpinorView t, LinkView 
   }
 }
 
+template <int SP, int LK, int RB>
+__global__ void __launch_bounds__(256) k_cgm_hop2(Geo g, SpinorView t, LinkView U, SpinorView rv, SpinorView sol,
+                                                   SpinorView ps, int64_t sstride, CgmState* st, Shape SH, int nr,
+                                                   int64_t nchunks, double* partial)
+{
+  __shared__ __attribute__((aligned(16))) char lds[RB * StStage<SP>::BYTES];
+  cgm_hop2_block<SP, LK, RB>(xcd_block(), lds, g, t, U, rv, sol, ps, sstride, st, SH, nr, nchunks, partial);
+}
+
 // C3: p = r + alpha p ; ps_k = zfs_k r + alphas_k ps_k for running shifts (the ps_k
 // part only when no update is pending for C2: before a reliable update, which
 // replaces r, and never otherwise; it then applies the sol_k term that C2 left)
 template <int SP>
-__global__ void __launch_bounds__(256) k_cgm_update(Geo g, SpinorView rv, SpinorView p, SpinorView ps,
-                                                     SpinorView sol, int64_t sstride, CgmState* st, Shape SH)
+__device__ __forceinline__ void cgm_update_block(int64_t lb, const Geo& g, const SpinorView& rv, const SpinorView& p,
+                                                 const SpinorView& ps, const SpinorView& sol, int64_t sstride,
+                                                 const CgmState* st, const Shape& SH)
 {
   using R = RealOfS<SP>;
-  const Map m = map_thread(SH, g.Vh);
+  const Map m = map_thread_lb(SH, g.Vh, lb);
   const CgmState* S = st + m.r;
   // reliable == 1: the update was flagged in this iteration, whose p/ps update
   // still runs; 2: waiting for the host's reliable update
@@ -472,6 +494,13 @@ __global__ void __launch_bounds__(256) k_cgm_update(Geo g, SpinorView rv, Spinor
     }
     SpAcc<SP, R, 3, kNT>::store(pk, g.stride, m.e, q, m.r);
   }
+}
+
+template <int SP>
+__global__ void __launch_bounds__(256) k_cgm_update(Geo g, SpinorView rv, SpinorView p, SpinorView ps,
+                                                     SpinorView sol, int64_t sstride, CgmState* st, Shape SH)
+{
+  cgm_update_block<SP>(xcd_block(), g, rv, p, ps, sol, sstride, st, SH);
 }
 
 enum { FIN_CGM_PAP = 1, FIN_CGM_RFRF = 2 };
@@ -555,6 +584,135 @@ __global__ void __launch_bounds__(1024) k_cgm_finalize(const double* partial, in
   }
   const double tot = block_sum_partials(partial + (int64_t)blockIdx.x * nchunks, nchunks, vl);
   if (threadIdx.x == 0) cgm_finalize(s, fin, tot, pap_scale);
+}
+
+// ------------------------------------------------------------------ persistent CGM
+// A chunk of multi-shift iterations in one cooperative launch: the five phases of an
+// iteration (C1, the pap finalisation, C2, the rfrf finalisation, C3) run the same
+// block functions as the separate kernels over the same logical blocks, each
+// workgroup walking a contiguous share of its XCD's blocks, with a grid barrier
+// between phases.  The finalisations sum the chunk partials with 256 threads in the
+// 1024-lane order, so every result has the bits of the separate launches.
+//
+// The barrier waits a bounded time: a workgroup that spins past spin_limit raises
+// `abort`, every workgroup leaves at its next barrier, and the host reports the
+// launch as failed; no wait outlasts the limit whatever happens to the grid.
+struct GridBar {
+  unsigned count, gen;
+  int abort, pad;
+};
+
+struct CgmPArgs {
+  Geo g;
+  LinkView U;
+  SpinorView p, t, r, sol, ps, none;
+  int64_t sstride, nchunks, blocks_hop, blocks_upd;
+  CgmState* st;
+  double* partial;
+  Shape sh_hop, sh_upd;
+  int nr, niters, upd_threads;
+  unsigned spin_limit;
+  GridBar* bar;
+};
+
+// the logical blocks [lo, hi) of this workgroup's XCD and the step between the
+// blocks of one workgroup (workgroup b runs on XCD b % 8, the hardware's order)
+__device__ __forceinline__ void persist_range(int64_t nblk, int64_t* first, int64_t* hi, int64_t* step)
+{
+  const uint32_t b = blockIdx.x, nb = gridDim.x;
+  const uint32_t xcd = b & 7u, bi = b >> 3;
+  const int64_t nbx = (int64_t)(nb >> 3) + (xcd < (nb & 7u) ? 1 : 0);
+  const int64_t q8 = nblk >> 3, r8 = nblk & 7;
+  const int64_t lo = (int64_t)xcd * q8 + ((int64_t)xcd < r8 ? (int64_t)xcd : r8);
+  *hi = lo + q8 + ((int64_t)xcd < r8 ? 1 : 0);
+  *first = lo + bi;
+  *step = nbx;
+}
+
+// grid barrier; returns false (uniformly over the workgroup) once the grid aborted
+__device__ __forceinline__ bool grid_sync(GridBar* bar, unsigned* gen, unsigned spin_limit, int* s_ok)
+{
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();  // this workgroup's writes, device-wide
+    const unsigned g0 = *gen;
+    const unsigned arrived = atomicAdd(&bar->count, 1u) + 1u;
+    if (arrived == gridDim.x) {
+      atomicExch(&bar->count, 0u);
+      __threadfence();
+      atomicAdd(&bar->gen, 1u);
+    } else {
+      unsigned spins = 0;
+      while (__hip_atomic_load(&bar->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g0) {
+        if (__hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        if (++spins > spin_limit) {
+          atomicExch(&bar->abort, 1);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    *gen = g0 + 1;
+    __threadfence();  // the other workgroups' writes, here
+    *s_ok = __hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+  }
+  __syncthreads();
+  return *s_ok != 0;
+}
+
+// one source's finalisation as k_cgm_finalize does it, with 256 threads
+__device__ __forceinline__ void cgm_finalize_block(CgmState* s, const double* pr, int64_t nchunks, int fin,
+                                                   double pap_scale, double* vl)
+{
+  if (s->done) return;  // uniform over the workgroup
+  if (s->reliable) {
+    if (threadIdx.x == 0) s->reliable = 2;
+    return;
+  }
+  const double tot = block_sum_partials_256(pr, nchunks, vl);
+  if (threadIdx.x == 0) cgm_finalize(s, fin, tot, pap_scale);
+}
+
+template <int SP, int LK, int RB>
+__global__ void __launch_bounds__(256) k_cgm_persistent(CgmPArgs A)
+{
+  constexpr int kStage = RB * StStage<SP>::BYTES;
+  __shared__ __attribute__((aligned(16))) char lds[kStage > 8192 ? kStage : 8192];
+  __shared__ int s_ok;
+  unsigned gen = 0;  // barrier generation (thread 0)
+  const int* skip = (const int*)&A.st->done;
+  const int sstr = (int)(sizeof(CgmState) / sizeof(int));
+  int64_t f, hi, step;
+  for (int it = 0; it < A.niters; it++) {
+    // C1: t = Doe p, |t|^2 partials
+    persist_range(A.blocks_hop, &f, &hi, &step);
+    for (int64_t lb = f; lb < hi; lb += step) {
+      __syncthreads();  // the last block's LDS reads are done
+      st_hop_block<SP, LK, 1, RB>(lb, lds, A.g, 1, A.t, A.p, A.U, 1.0, 0.0, nullptr, A.none, 1, A.sh_hop, A.nr,
+                                  A.nchunks, A.partial, skip, sstr, 2);
+    }
+    if (!grid_sync(A.bar, &gen, A.spin_limit, &s_ok)) return;
+    for (int s = blockIdx.x; s < A.nr; s += gridDim.x)
+      cgm_finalize_block(A.st + s, A.partial + (int64_t)s * A.nchunks, A.nchunks, FIN_CGM_PAP, 0.25, (double*)lds);
+    if (!grid_sync(A.bar, &gen, A.spin_limit, &s_ok)) return;
+    // C2: hop + shift and r updates, |r|^2 partials
+    for (int64_t lb = f; lb < hi; lb += step) {
+      __syncthreads();
+      cgm_hop2_block<SP, LK, RB>(lb, lds, A.g, A.t, A.U, A.r, A.sol, A.ps, A.sstride, A.st, A.sh_hop, A.nr,
+                                 A.nchunks, A.partial);
+    }
+    if (!grid_sync(A.bar, &gen, A.spin_limit, &s_ok)) return;
+    for (int s = blockIdx.x; s < A.nr; s += gridDim.x)
+      cgm_finalize_block(A.st + s, A.partial + (int64_t)s * A.nchunks, A.nchunks, FIN_CGM_RFRF, 0.25, (double*)lds);
+    if (!grid_sync(A.bar, &gen, A.spin_limit, &s_ok)) return;
+    // C3: p update (and ps before a reliable update)
+    int64_t uf, uhi, ustep;
+    persist_range(A.blocks_upd, &uf, &uhi, &ustep);
+    if ((int)threadIdx.x < A.upd_threads)
+      for (int64_t lb = uf; lb < uhi; lb += ustep)
+        cgm_update_block<SP>(lb, A.g, A.r, A.p, A.ps, A.sol, A.sstride, A.st, A.sh_upd);
+    if (!grid_sync(A.bar, &gen, A.spin_limit, &s_ok)) return;
+  }
 }
 
 // sol_k -= betas_k ps_k for the shifts whose term the last C2 left and that no
@@ -892,6 +1050,77 @@ int launch_cgm_vec3(nq_ctx* ctx, const SpinorView& r, const SpinorView& p, const
 }
 
 int cgm_state_bytes() { return (int)sizeof(CgmState); }
+
+template <int SP, int LK, int RB>
+static int persistent_go(nq_ctx* ctx, CgmPArgs& A)
+{
+  const void* fn = (const void*)k_cgm_persistent<SP, LK, RB>;
+  static const int occ = [fn] {
+    int o = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, fn, 256, 0) != hipSuccess) o = 0;
+    return o;
+  }();
+  if (occ < 1 || ctx->ncu < 1) return fail(NQ_EDEVICE, "persistent CGM: no resident workgroup");
+  const int grid = occ * ctx->ncu;
+  void* args[] = {&A};
+  NQ_HIP(hipLaunchCooperativeKernel(fn, dim3((unsigned)grid), dim3(256), args, 0, ctx->stream));
+  return NQ_OK;
+}
+
+int launch_cgm_persistent(nq_ctx* ctx, const LinkView& U, const SpinorView& p, const SpinorView& t,
+                          const SpinorView& r, const SpinorView& sol, const SpinorView& ps, int64_t sstride,
+                          void* states, int nr, int niters, void* bar, unsigned spin_limit)
+{
+  const Geo& g = ctx->geo;
+  const int sp = p.prec, lk = U.prec;
+  if (niters < 1) return NQ_OK;
+  if (nr < 1 || nr > 16) return fail(NQ_EINVAL, "persistent CGM: 1..16 sources");
+  CgmPArgs A;
+  A.g = g;
+  A.U = U;
+  A.p = p;
+  A.t = t;
+  A.r = r;
+  A.sol = sol;
+  A.ps = ps;
+  A.none.ncomp = 3;
+  A.sstride = sstride;
+  A.nchunks = (g.Vh + 63) / 64;
+  const int RB = st_rb(nr, sp);
+  A.sh_hop = st_shape(nr, RB);
+  A.blocks_hop = (A.nchunks + 3) / 4 * A.sh_hop.NRG;
+  const Launch L = launch_shape(g, nr);
+  A.sh_upd = L.s;
+  A.blocks_upd = L.blocks;
+  A.upd_threads = L.threads;
+  A.st = (CgmState*)states;
+  A.partial = ctx->partial;
+  A.nr = nr;
+  A.niters = niters;
+  A.spin_limit = spin_limit;
+  A.bar = (GridBar*)bar;
+  if (A.nchunks * nr > ctx->max_partials) return fail(NQ_EINVAL, "persistent CGM: reduction scratch too small");
+  NQ_HIP(hipMemsetAsync(bar, 0, sizeof(GridBar), ctx->stream));
+  hipEvent_t ev = nullptr;
+  prof_begin(ctx, "cgm_persistent", 0.0, &ev);
+  int rc = NQ_OK;
+  if (sp == SP_F64 && lk == LK_F64_18) {
+    if (RB == 1) rc = persistent_go<SP_F64, LK_F64_18, 1>(ctx, A);
+    else if (RB == 2) rc = persistent_go<SP_F64, LK_F64_18, 2>(ctx, A);
+    else rc = persistent_go<SP_F64, LK_F64_18, 4>(ctx, A);
+  } else if (sp == SP_F32 && lk == LK_F32_18) {
+    if (RB == 1) rc = persistent_go<SP_F32, LK_F32_18, 1>(ctx, A);
+    else if (RB == 2) rc = persistent_go<SP_F32, LK_F32_18, 2>(ctx, A);
+    else if (RB == 8) rc = persistent_go<SP_F32, LK_F32_18, 8>(ctx, A);
+    else rc = persistent_go<SP_F32, LK_F32_18, 4>(ctx, A);
+  } else {
+    rc = fail(NQ_EINVAL, "persistent CGM: unsupported precision combination");
+  }
+  prof_end(ctx, ev);
+  return rc;
+}
+
+int cgm_grid_bar_bytes() { return (int)sizeof(GridBar); }
 
 int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, const SpinorView& t,
                          const SpinorView& r, const SpinorView& sol, const SpinorView& ps, int64_t sstride,
