@@ -27,6 +27,11 @@ COMMON = [
     "-fPIC",
     f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # multiply-adds fuse within one source expression only (hipcc's default fuses
+    # across statements as the optimiser sees fit, which can round one template
+    # instantiation differently from another: the register-blocked multi-source
+    # kernels must give the bits of the one-source ones)
+    "-ffp-contract=on",
     "-Wall", "-Wno-unused-function", "-Wno-unused-result",
     "-I" + os.path.join(ROOT, "include"),
 ]
