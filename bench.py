@@ -52,6 +52,8 @@ def main():
     ap.add_argument("--no-clover", action="store_true", help="skip the twisted-clover solve leg")
     ap.add_argument("--serve-requests", type=int, default=60, help="requests in the Poisson stream")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="request arrival rate (1/s)")
+    ap.add_argument("--serve-rate-low", type=float, default=6.0,
+                    help="arrival rate of a second stream below the capacity (1/s; 0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}): there is no multi-GPU path; "
@@ -446,55 +448,66 @@ def serve_bench(args):
         ts = [nq.tm_solve_eoprec(ctx, g, kappa, mu, 100000, residue, fx, fs, prec="f64")["seconds"] for mu in mus]
         t64.append(float(np.mean(ts)))
         fs.close(), fx.close(), g.close(), ctx.close()
-    srv = nq.Server(max_groups=1)
-    vid = [srv.add_volume(L, c) for L, c in zip(vols, confs)]
-    # warm-up: one request per volume builds the batch groups and link copies
-    for v in vid:
-        srv.wait(srv.submit(v, kappa, mus[0], residue, prec="f16", point_site=0))
-    rng = np.random.default_rng(2026)
+    def stream(n, rate, seed):
+        srv = nq.Server(max_groups=1)
+        vid = [srv.add_volume(L, c) for L, c in zip(vols, confs)]
+        # warm-up: one request per volume builds the batch groups and link copies
+        for v in vid:
+            srv.wait(srv.submit(v, kappa, mus[0], residue, prec="f16", point_site=0))
+        rng = np.random.default_rng(seed)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n))
+        which = rng.choice(len(vols), size=n, p=mix)
+        mu_i = rng.integers(0, len(mus), size=n)
+        sites = [int(rng.integers(0, int(np.prod(vols[w])))) for w in which]
+        t0 = time.perf_counter()
+        ids = []
+        for i in range(n):
+            dt = arrivals[i] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            ids.append(srv.submit(vid[which[i]], kappa, mus[mu_i[i]], residue, prec="f16", point_site=sites[i]))
+        res = [srv.wait(i) for i in ids]
+        srv.close()
+        bad = [r for r in res if r["status"] != 0 or r["max_rel_res_sq"] > 2 * residue]
+        if bad:
+            log(f"[bench] serve: {len(bad)} failed requests, statuses {sorted(set(r['status'] for r in bad))}, "
+                f"worst check {max(r['max_rel_res_sq'] for r in bad):.2e}")
+        lat = np.array([r["t_done"] - r["t_submit"] for r in res])
+        wait = np.array([r["t_admit"] - r["t_submit"] for r in res])  # queued until the first source took a slot
+        service = np.array([r["t_done"] - r["t_admit"] for r in res])
+        span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
+        pct = lambda a, q: round(float(np.percentile(a, q)), 4)
+        return {
+            "requests": n, "rate_per_s": rate,
+            "p50_latency_s": pct(lat, 50), "p99_latency_s": pct(lat, 99),
+            "p50_queue_wait_s": pct(wait, 50), "p99_queue_wait_s": pct(wait, 99),
+            "p50_service_s": pct(service, 50), "p99_service_s": pct(service, 99),
+            "offered_solves_per_s": round(12 * rate, 1), "solves_per_s": round(12 * n / span, 2),
+            "failed": len(bad), "worst_rel_res_sq": float(max(r["max_rel_res_sq"] for r in res)),
+        }, which
+
     n = args.serve_requests
-    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, size=n))
-    which = rng.choice(len(vols), size=n, p=mix)
-    mu_i = rng.integers(0, len(mus), size=n)
-    sites = [int(rng.integers(0, int(np.prod(vols[w])))) for w in which]
-    t0 = time.perf_counter()
-    ids = []
-    for i in range(n):
-        dt = arrivals[i] - (time.perf_counter() - t0)
-        if dt > 0:
-            time.sleep(dt)
-        ids.append(srv.submit(vid[which[i]], kappa, mus[mu_i[i]], residue, prec="f16", point_site=sites[i]))
-    res = [srv.wait(i) for i in ids]
-    srv.close()
-    bad = [r for r in res if r["status"] != 0 or r["max_rel_res_sq"] > 2 * residue]
-    if bad:
-        log(f"[bench] serve: {len(bad)} failed requests, statuses {sorted(set(r['status'] for r in bad))}, "
-            f"worst check {max(r['max_rel_res_sq'] for r in bad):.2e}")
-    lat = np.array([r["t_done"] - r["t_submit"] for r in res])
-    wait = np.array([r["t_admit"] - r["t_submit"] for r in res])  # queued until the first source took a slot
-    service = np.array([r["t_done"] - r["t_admit"] for r in res])
-    span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
+    head, which = stream(n, args.serve_rate, 2026)
     nsol = 12 * n
     fp64_one_at_a_time = nsol / float(sum(12 * t64[w] for w in which))
-    out = {
-        "requests": n, "rate_per_s": args.serve_rate, "sources_per_request": 12,
+    out = dict(head)
+    out.update({
+        "sources_per_request": 12,
         "volumes": ["%dx%d^3" % (L[0], L[1]) for L in vols], "mix": mix, "masses": mus,
-        "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
-        "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
-        "p50_queue_wait_s": round(float(np.percentile(wait, 50)), 4),
-        "p99_queue_wait_s": round(float(np.percentile(wait, 99)), 4),
-        "p50_service_s": round(float(np.percentile(service, 50)), 4),
-        "p99_service_s": round(float(np.percentile(service, 99)), 4),
-        "offered_solves_per_s": round(12 * args.serve_rate, 1),
-        "solves_per_s": round(nsol / span, 2),
         "fp64_one_at_a_time_solves_per_s": round(fp64_one_at_a_time, 2),
         "fp64_solve_s_per_volume": [round(t, 4) for t in t64],
-        "failed": len(bad), "worst_rel_res_sq": float(max(r["max_rel_res_sq"] for r in res)),
         "inner": "f16 spinors + FP32 12-real links, FP64 reliable updates",
-    }
+    })
+    # the same stream at an arrival rate below the capacity (the rate above offers
+    # about 2.5 times what one GPU serves, so its latencies are those of a growing queue)
+    if args.serve_rate_low > 0:
+        out["below_capacity"], _ = stream(n, args.serve_rate_low, 2027)
+        b = out["below_capacity"]
+        log(f"[bench] serve below capacity: {n} requests at {args.serve_rate_low}/s: p50 {b['p50_latency_s']}s "
+            f"p99 {b['p99_latency_s']}s, {b['solves_per_s']} solves/s, failed {b['failed']}")
     log(f"[bench] serve: {n} requests at {args.serve_rate}/s: p50 {out['p50_latency_s']}s p99 "
         f"{out['p99_latency_s']}s, {out['solves_per_s']} solves/s (FP64 one at a time: "
-        f"{out['fp64_one_at_a_time_solves_per_s']}), failed {len(bad)}")
+        f"{out['fp64_one_at_a_time_solves_per_s']}), failed {out['failed']}")
     return out
 
 
