@@ -147,6 +147,26 @@ def test_st_cgm_nan_source_is_rejected(st_setups, prec):
     assert res[0]["rc"] == nq.NQ_OK and max(res[0]["true_res_sq"]) < 2e-20
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_st_zero_source_is_rejected(st_setups, prec):
+    """A source of norm zero in a multi-shift solve, and in the staggered CG, is
+    refused with NQ_ENAN (nissa CRASHes with "invalid norm")."""
+    L = (4, 6, 4, 8)
+    ctx, g, geo, conf, ce, co = st_setups(L)
+    shifts = [0.001, 0.05]
+    srcs = [gauss_eo(geo.Vh, 80, ncomp=3), np.zeros((geo.Vh, 3, 2))]
+    fsrc = [ctx.field(nq.COLOR, nq.EVEN, s) for s in srcs]
+    sols = [[ctx.field(nq.COLOR, nq.EVEN) for _ in shifts] for _ in srcs]
+    with pytest.raises(nq.NqError) as e:
+        nq.st_cgm(ctx, g, shifts, 20000, [1e-20] * 2, sols, fsrc, prec=prec)
+    assert e.value.code == nq.NQ_ENAN
+    if prec == "f64":
+        x = ctx.field(nq.COLOR, nq.EVEN)
+        with pytest.raises(nq.NqError) as e:
+            nq.st_cg(ctx, g, 0.01, 1000, 1e-20, x, fsrc[1])
+        assert e.value.code == nq.NQ_ENAN
+
+
 def test_tmQ2_cg_matches_oracle():
     L = (4, 6, 4, 8)
     conf = tm_conf(L, 3)

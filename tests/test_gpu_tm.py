@@ -272,6 +272,33 @@ def test_nan_source_is_rejected(setups):
     assert e.value.code == nq.NQ_ENAN
 
 
+@pytest.mark.parametrize("prec", ["f64", "f16"])
+def test_zero_source_is_rejected(setups, prec):
+    """A source of norm zero is refused with NQ_ENAN, as nissa's CG CRASHes with
+    "invalid norm" (cg_invert: source_norm == 0); the eo solve and the batched
+    solve refuse it too, and the context then solves normally."""
+    L = (4, 6, 4, 8)
+    ctx, g, geo, conf, ce, co = setups(L)
+    fz = ctx.field(nq.SPINCOLOR, nq.ODD, np.zeros((geo.Vh, 4, 3, 2)))
+    fx = ctx.field(nq.SPINCOLOR, nq.ODD)
+    with pytest.raises(nq.NqError) as e:
+        nq.tm_cg_kernsq(ctx, g, 0.16, 0.01, 100, 1e-20, fx, fz, prec=prec)
+    assert e.value.code == nq.NQ_ENAN
+    zl = ctx.field(nq.SPINCOLOR, nq.LX, np.zeros((geo.V, 4, 3, 2)))
+    xl = ctx.field(nq.SPINCOLOR, nq.LX)
+    with pytest.raises(nq.NqError) as e:
+        nq.tm_solve_eoprec(ctx, g, 0.16, 0.01, 100, 1e-20, xl, zl, prec=prec)
+    assert e.value.code == nq.NQ_ENAN
+    good = ctx.field(nq.SPINCOLOR, nq.LX, nq.source_lx(L, 3, "gauss"))
+    xs = [ctx.field(nq.SPINCOLOR, nq.LX) for _ in range(2)]
+    with pytest.raises(nq.NqError) as e:
+        nq.tm_solve_eoprec_multi(ctx, g, 0.16, [0.01, 0.01], 100, 1e-20, xs, [good, zl], prec=prec)
+    assert e.value.code == nq.NQ_ENAN
+    fb = ctx.field(nq.SPINCOLOR, nq.ODD, gauss_eo(geo.Vh, 4))
+    st = nq.tm_cg_kernsq(ctx, g, 0.16, 0.01, 1000, 1e-20, fx, fb, prec=prec)
+    assert st["rc"] == nq.NQ_OK and st["converged"]
+
+
 def test_non_convergence_is_reported(setups):
     L = (4, 6, 4, 8)
     ctx, g, geo, conf, ce, co = setups(L)
