@@ -16,6 +16,9 @@
  *   CG / CGM           dense direct solve (4^4), true residual recomputation,
  *                      CGM shift == single-shift CG                            pinned
  *   cg128              residual in __float128 below 1e-28                      pinned
+ *   or_st_rat_summ     the same sum from dense solves on 4x4x2x2; partial
+ *                      fractions of 1/((x+a)(x+b))                             pinned
+ *   or_rootst_force    finite-difference gradient of the pseudofermion action  pinned
  *   plaquette          L4T8conf fixture value 0.6350055722288719              pinned
  *   clover term        constant abelian fields in each plane against
  *                      -cSW/4 sum_{mu<nu} gamma_mu gamma_nu F_munu built from
