@@ -241,7 +241,7 @@ struct ClovArgs {
   double a[2][2], b[2][2];
 };
 
-// FP16 spinors (FP32 blocks): one thread per site, both chiralities (the per-site
+// FP16 spinors (FP16 blocks): one thread per site, both chiralities (the per-site
 // norm of the output covers all twelve components)
 template <int MODE, int DAG, int SP>
 __global__ void __launch_bounds__(256) k_clover_apply(Geo g, SpinorView out, SpinorView in, SpinorView blk,
@@ -253,10 +253,10 @@ __global__ void __launch_bounds__(256) k_clover_apply(Geo g, SpinorView out, Spi
   if (e >= (uint32_t)g.Vh) return;
   CR v[12], o[12];
   SpAcc<SP, R, 12>::load(in, g.stride, e, v);
-  const CR* B = (const CR*)blk.data;  // FP32 complex per component and site
+  constexpr int BEB = SP == SP_F64 ? 16 : (SP == SP_F32 ? 8 : 4);  // block element bytes
 #pragma unroll
   for (int h = 0; h < 2; h++) {
-    clover_half<MODE, DAG, R>(B, g.stride, e, h, kappa, mu, v + 6 * h, o + 6 * h);
+    clover_half<MODE, DAG, R, BEB>(blk.data, g.stride, e, h, kappa, mu, v + 6 * h, o + 6 * h);
 #pragma unroll
     for (int q = 0; q < 6; q++) asm volatile("" : "+v"(o[6 * h + q].re), "+v"(o[6 * h + q].im));
   }
@@ -302,7 +302,7 @@ __global__ void __launch_bounds__(256) k_clover_apply_h(Geo g, SpinorView out, S
   oh.data = (char*)out.data + hoff;
   CR v[6], o[6];
   SpAcc<SP, R, 6>::load(ih, g.stride, e, v);
-  clover_half<MODE, DAG, R>((const CR*)blk.data, g.stride, e, h, kappa, mu, v, o);
+  clover_half<MODE, DAG, R, EB>(blk.data, g.stride, e, h, kappa, mu, v, o);
   const CR ka = mk<R>((R)k.a[h][0], (R)k.a[h][1]);
   if (aux.data) {
     SpinorView ah = aux;
@@ -347,7 +347,7 @@ __global__ void __launch_bounds__(256) k_clover_apply_hb(Geo g, SpinorView out, 
   ih.data = (char*)in.data + hoff;
   oh.data = (char*)out.data + hoff;
   ClovBlk<MODE, R> k;
-  clover_load<MODE, R>((const CR*)blk.data, g.stride, e, h, k);
+  clover_load<MODE, R, EB>(blk.data, g.stride, e, h, k);
   for (int r = 0; r < nr; r++) {
     if (clov_idle(st, r, pred)) continue;  // uniform over the grid
     CR v[6], o[6];
@@ -364,14 +364,13 @@ __global__ void __launch_bounds__(256) k_clover_apply_b16(Geo g, SpinorView out,
   typedef cx<float> CR;
   const uint32_t e = blockIdx.x * 256 + threadIdx.x;
   if (e >= (uint32_t)g.Vh) return;
-  const CR* B = (const CR*)blk.data;
   for (int r = 0; r < nr; r++) {
     if (clov_idle(st, r, pred)) continue;
     CR v[12], o[12];
     SpAcc<SP_F16, float, 12>::load(in, g.stride, e, v, r);
 #pragma unroll
     for (int h = 0; h < 2; h++) {
-      clover_half<MODE, DAG, float>(B, g.stride, e, h, kappa, mu, v + 6 * h, o + 6 * h);
+      clover_half<MODE, DAG, float, 4>(blk.data, g.stride, e, h, kappa, mu, v + 6 * h, o + 6 * h);
 #pragma unroll
       for (int q = 0; q < 6; q++) asm volatile("" : "+v"(o[6 * h + q].re), "+v"(o[6 * h + q].im));
     }
@@ -379,15 +378,15 @@ __global__ void __launch_bounds__(256) k_clover_apply_b16(Geo g, SpinorView out,
   }
 }
 
-// FP32 copy of an FP64 block field (sloppy solves)
-__global__ void __launch_bounds__(256) k_blocks_to_f32(Geo g, int ncomp, const C* in, cx<float>* out)
+// FP32 or FP16 packed copy of an FP64 block field (sloppy solves): MODE 0 the clover
+// blocks (36 complex per site), 1 the inverse blocks (72)
+template <int MODE, int EB>
+__global__ void __launch_bounds__(256) k_blocks_pack(Geo g, const C* in, void* out)
 {
   const uint32_t e = blockIdx.x * 256 + threadIdx.x;
   if (e >= (uint32_t)g.Vh) return;
-  for (int k = 0; k < ncomp; k++) {
-    const C v = in[k * g.stride + e];
-    out[k * g.stride + e] = mk<float>((float)v.re, (float)v.im);
-  }
+  clover_pack<MODE, EB>(in, out, g.stride, e, 0);
+  clover_pack<MODE, EB>(in, out, g.stride, e, 1);
 }
 
 // Re tr of the six plaquettes U_mu(X) U_nu(X+mu) U_mu(X+nu)^+ U_nu(X)^+ of each
@@ -475,12 +474,20 @@ int launch_clover_invert(nq_ctx* ctx, const SpinorView& cl, const SpinorView& in
   return NQ_OK;
 }
 
-int launch_blocks_to_f32(nq_ctx* ctx, int ncomp, const SpinorView& in, const SpinorView& out)
+int launch_blocks_sloppy(nq_ctx* ctx, int ncomp, const SpinorView& in, const SpinorView& out)
 {
   const Geo& g = ctx->geo;
-  if (in.prec != SP_F64 || out.prec != SP_F32) return fail(NQ_EINVAL, "block conversion: FP64 -> FP32 only");
-  hipLaunchKernelGGL(k_blocks_to_f32, dim3((unsigned)((g.Vh + 255) / 256)), dim3(256), 0, ctx->stream, g, ncomp,
-                     (const C*)in.data, (cx<float>*)out.data);
+  if (in.prec != SP_F64 || (out.prec != SP_F32 && out.prec != SP_F16) || (ncomp != 36 && ncomp != 72))
+    return fail(NQ_EINVAL, "block conversion: FP64 -> FP32 or FP16, 36 or 72 components");
+  const dim3 grid((unsigned)((g.Vh + 255) / 256)), b256(256);
+  const C* src = (const C*)in.data;
+  if (out.prec == SP_F32) {
+    if (ncomp == 36) hipLaunchKernelGGL((k_blocks_pack<0, 8>), grid, b256, 0, ctx->stream, g, src, out.data);
+    else hipLaunchKernelGGL((k_blocks_pack<1, 8>), grid, b256, 0, ctx->stream, g, src, out.data);
+  } else {
+    if (ncomp == 36) hipLaunchKernelGGL((k_blocks_pack<0, 4>), grid, b256, 0, ctx->stream, g, src, out.data);
+    else hipLaunchKernelGGL((k_blocks_pack<1, 4>), grid, b256, 0, ctx->stream, g, src, out.data);
+  }
   NQ_HIP(hipGetLastError());
   return NQ_OK;
 }
@@ -491,9 +498,8 @@ int launch_clover_apply(nq_ctx* ctx, int mode, const SpinorView& out, const Spin
 {
   const Geo& g = ctx->geo;
   const int sp = in.prec;
-  const int bp = sp == SP_F16 ? SP_F32 : sp;  // block precision
-  if (out.prec != sp || blk.prec != bp || (aux.data && aux.prec != sp))
-    return fail(NQ_EINVAL, "twisted clover: FP64/FP32/FP16 fields with FP64/FP32/FP32 blocks");
+  if (out.prec != sp || blk.prec != sp || (aux.data && aux.prec != sp))
+    return fail(NQ_EINVAL, "twisted clover: fields and blocks of one precision (FP64, FP32 or FP16)");
   ClovArgs k;
   for (int h = 0; h < 2; h++)
     for (int c = 0; c < 2; c++) {
@@ -501,7 +507,7 @@ int launch_clover_apply(nq_ctx* ctx, int mode, const SpinorView& out, const Spin
       k.b[h][c] = b ? b[h][c] : 0.0;
     }
   hipEvent_t ev = nullptr;
-  const double cb = bp == SP_F64 ? 16.0 : 8.0;                        // bytes per block complex
+  const double cb = sp == SP_F64 ? 16.0 : (sp == SP_F32 ? 8.0 : 4.0);    // bytes per block complex
   const double sb = sp == SP_F64 ? 192.0 : (sp == SP_F32 ? 96.0 : 52.0);  // bytes per spinor site
   const double bytes = (double)g.Vh * ((mode == 0 ? 36 : 72) * cb + (2 + (aux.data ? 1 : 0)) * sb);
   const char* names[3][2] = {{"clover_dee", "clover_inv"}, {"clover_dee_f32", "clover_inv_f32"},
@@ -539,11 +545,10 @@ int launch_clover_apply_batched(nq_ctx* ctx, int mode, int dag, const SpinorView
 {
   const Geo& g = ctx->geo;
   const int sp = in.prec;
-  const int bp = sp == SP_F16 ? SP_F32 : sp;
-  if (out.prec != sp || blk.prec != bp || nr < 1 || nr > 16)
-    return fail(NQ_EINVAL, "batched twisted clover: FP64/FP32/FP16 fields with FP64/FP32/FP32 blocks, 1..16 RHS");
+  if (out.prec != sp || blk.prec != sp || nr < 1 || nr > 16)
+    return fail(NQ_EINVAL, "batched twisted clover: fields and blocks of one precision, 1..16 RHS");
   hipEvent_t ev = nullptr;
-  const double cb = bp == SP_F64 ? 16.0 : 8.0;
+  const double cb = sp == SP_F64 ? 16.0 : (sp == SP_F32 ? 8.0 : 4.0);
   const double sb = sp == SP_F64 ? 192.0 : (sp == SP_F32 ? 96.0 : 52.0);
   prof_begin(ctx, mode == 0 ? "clover_dee_batch" : "clover_inv_batch",
              (double)g.Vh * ((mode == 0 ? 36 : 72) * cb + 2 * sb * nr), &ev);

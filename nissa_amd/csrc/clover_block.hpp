@@ -5,8 +5,11 @@
 // Clover field of one parity: 36 complex components per site (G, H, I, J 3x3 blocks
 // of nissa's clover_term_t, clover_term.cpp:22-55, k = 9*b + 3*row + col); inverse
 // field: 72 complex per site (k = 36 h + 6 row + col, row/col = 3 spin + colour,
-// inv_clover_term_t[h][id][ic][x_id][x_ic]); component k of device site e at
-// B[k*stride + e] in FP64 or FP32 complex.
+// inv_clover_term_t[h][id][ic][x_id][x_ic]).  Storage: chirality h (N = 18 or 36
+// components) is a packed array of 16-byte records, [record][site], of 16/EB
+// components each (Pack<EB, N>; EB = 16 FP64, 8 FP32, 4 FP16) starting at byte
+// h N EB stride, so that a site's block takes N EB / 16 record loads.  For FP64
+// that is component k of device site e at byte (k stride + e) 16.
 #pragma once
 
 #include "nq_device.hpp"
@@ -23,19 +26,35 @@ struct ClovBlk {
   cx<R> m[MODE == 0 ? 18 : 36];  // MODE 0: C0 (9) then C1 (9); MODE 1: the 6x6 inverse
 };
 
-template <int MODE, class R>
-__device__ __forceinline__ void clover_load(const cx<R>* B, int64_t st, uint32_t e, int h, ClovBlk<MODE, R>& k)
+// the N components of chirality h of site e (64-bit addressing: a block field
+// can exceed the 4 GB reach of a buffer offset)
+template <int MODE, class R, int EB>
+__device__ __forceinline__ void clover_load(const void* B, int64_t st, uint32_t e, int h, ClovBlk<MODE, R>& k)
 {
-  if constexpr (MODE == 0) {
+  constexpr int N = MODE == 0 ? 18 : 36;
+  using P = Pack<EB, N>;
+  static_assert(P::NL == 0 || (EB == 4 && P::NL == 2), "block layout");
+  const char* base = (const char*)B + (int64_t)h * N * EB * st;
 #pragma unroll
-    for (int q = 0; q < 9; q++) {
-      k.m[q] = B[(18 * h + q) * st + e];
-      k.m[9 + q] = B[(18 * h + 9 + q) * st + e];
-    }
-  } else {
-#pragma unroll
-    for (int q = 0; q < 36; q++) k.m[q] = B[(36 * h + q) * st + e];
+  for (int q = 0; q < P::NG; q++)
+    decode_record<EB, R>(*(const nq_u4*)(base + ((int64_t)q * st + e) * 16), k.m + q * P::G);
+  if constexpr (P::NL == 2) {
+    const nq_hv4 t = *(const nq_hv4*)(base + (int64_t)P::NG * st * 16 + (int64_t)e * 8);
+    k.m[N - 2] = mk<R>((R)(float)t[0], (R)(float)t[1]);
+    k.m[N - 1] = mk<R>((R)(float)t[2], (R)(float)t[3]);
   }
+}
+
+// chirality h of an FP64 block field ([k][e] complex, N per chirality) stored in the
+// packed layout above with element bytes EB (the sloppy copies)
+template <int MODE, int EB>
+__device__ __forceinline__ void clover_pack(const cx<double>* in, void* out, int64_t st, uint32_t e, int h)
+{
+  constexpr int N = MODE == 0 ? 18 : 36;
+  cx<double> v[N];
+#pragma unroll
+  for (int q = 0; q < N; q++) v[q] = in[(int64_t)(N * h + q) * st + e];
+  store_packed<EB, N, 0, double>((char*)out + (int64_t)h * N * EB * st, st, e, v);
 }
 
 template <int MODE, int DAG, class R>
@@ -73,12 +92,12 @@ __device__ __forceinline__ void clover_apply_blk(const ClovBlk<MODE, R>& k, int 
   }
 }
 
-template <int MODE, int DAG, class R>
-__device__ __forceinline__ void clover_half(const cx<R>* B, int64_t st, uint32_t e, int h, double kappa, double mu,
+template <int MODE, int DAG, class R, int EB>
+__device__ __forceinline__ void clover_half(const void* B, int64_t st, uint32_t e, int h, double kappa, double mu,
                                             const cx<R> iv[6], cx<R> ov[6])
 {
   ClovBlk<MODE, R> k;
-  clover_load<MODE, R>(B, st, e, h, k);
+  clover_load<MODE, R, EB>(B, st, e, h, k);
   clover_apply_blk<MODE, DAG, R>(k, h, kappa, mu, iv, ov);
 }
 

@@ -243,9 +243,10 @@ struct nq_clover {
   nq::DevBuf inv[2];
   double inv_kappa[2] = {0, 0}, inv_mu[2] = {0, 0};
   int inv_valid[2] = {0, 0};
-  // FP32 copies for sloppy solves (inv32 follows inv)
-  nq::DevBuf cl32[2], inv32[2];
-  int cl32_valid[2] = {0, 0}, inv32_valid[2] = {0, 0};
+  // packed FP32 ([0]) and FP16 ([1]) copies for sloppy solves (the inverse copies
+  // follow inv)
+  nq::DevBuf cls[2][2], invs[2][2];
+  int cls_valid[2][2] = {{0, 0}, {0, 0}}, invs_valid[2][2] = {{0, 0}, {0, 0}};
 };
 
 struct nq_gauge {
@@ -378,7 +379,7 @@ int launch_clover_build(nq_ctx* ctx, const double* aos_e, const double* aos_o, i
 int launch_clover_invert(nq_ctx* ctx, const SpinorView& cl, const SpinorView& inv, double kappa, double mass);
 // Re tr sums of the temporal (ts[0]) and spatial (ts[1]) plaquettes over the lattice
 int launch_plaquette(nq_ctx* ctx, const double* aos_e, const double* aos_o, double ts[2]);
-int launch_blocks_to_f32(nq_ctx* ctx, int ncomp, const SpinorView& in, const SpinorView& out);
+int launch_blocks_sloppy(nq_ctx* ctx, int ncomp, const SpinorView& in, const SpinorView& out);
 // the block (mode 0: z + clover term, 1: inverse) of every site applied to nr right-hand
 // sides (views with rstride), skipping those whose state st[r] is idle (pred as for the
 // hop epilogues; st may be null)

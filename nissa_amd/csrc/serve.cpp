@@ -144,9 +144,11 @@ int64_t group_bytes(const nq_ctx* c, int prec, bool clover)
   const int64_t s = c->geo.stride;
   const int64_t lx = 2 * 12 * 16 * s;  // one FP64 lx spincolor field
   int64_t b = TmBatch::kMaxSlots * (TmBatch::bytes_per_slot(c->geo, prec, clover) + 2 * lx) + (64 << 20);
-  // clover: the group's inverse blocks (FP64, FP32 when mixed) and, counted with every
-  // group, the clover term of both parities with its FP32 odd copy
-  if (clover) b += 72 * 16 * s + (prec != NQ_PREC_F64 ? 72 * 8 * s : 0) + 2 * 36 * 16 * s + 36 * 8 * s + 12 * 16 * s;
+  // clover: the group's inverse blocks (FP64, and a sloppy copy when mixed: 8 bytes per
+  // complex in FP32, 4 in FP16) and, counted with every group, the clover term of both
+  // parities with its sloppy odd copy
+  const int64_t cs = prec == NQ_PREC_F64 ? 0 : (prec == NQ_PREC_F32 ? 8 : 4);
+  if (clover) b += 72 * 16 * s + 72 * cs * s + 2 * 36 * 16 * s + 36 * cs * s + 12 * 16 * s;
   return b;
 }
 

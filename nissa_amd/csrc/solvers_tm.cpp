@@ -535,37 +535,40 @@ static int clov_inv_view(nq_clover* cl, int par, double kappa, double mu, Spinor
     cl->inv_valid[par] = 1;
     cl->inv_kappa[par] = kappa;
     cl->inv_mu[par] = mu;
-    cl->inv32_valid[par] = 0;
+    cl->invs_valid[0][par] = cl->invs_valid[1][par] = 0;
   }
   return NQ_OK;
 }
 
-// FP32 copies of the clover blocks and of the inverse blocks (for (kappa, mu))
-static int clov_view32(nq_clover* cl, int par, SpinorView* v)
+// packed copies in the sloppy spinor precision sp (FP32 or FP16) of the clover blocks
+// and of the inverse blocks (for (kappa, mu))
+static int clov_view_s(nq_clover* cl, int par, int sp, SpinorView* v)
 {
   nq_ctx* c = cl->ctx;
-  if (!cl->cl32[par].p) NQ_TRY(dev_alloc(cl->cl32[par], (size_t)36 * 8 * c->geo.stride));
-  v->data = cl->cl32[par].p;
-  v->prec = SP_F32;
+  const int i = sp == SP_F16 ? 1 : 0;
+  if (!cl->cls[i][par].p) NQ_TRY(dev_alloc(cl->cls[i][par], (size_t)36 * (i ? 4 : 8) * c->geo.stride));
+  v->data = cl->cls[i][par].p;
+  v->prec = i ? SP_F16 : SP_F32;
   v->ncomp = 36;
-  if (!cl->cl32_valid[par]) {
-    NQ_TRY(launch_blocks_to_f32(c, 36, clov_view(cl, par), *v));
-    cl->cl32_valid[par] = 1;
+  if (!cl->cls_valid[i][par]) {
+    NQ_TRY(launch_blocks_sloppy(c, 36, clov_view(cl, par), *v));
+    cl->cls_valid[i][par] = 1;
   }
   return NQ_OK;
 }
-static int clov_inv_view32(nq_clover* cl, int par, double kappa, double mu, SpinorView* v)
+static int clov_inv_view_s(nq_clover* cl, int par, double kappa, double mu, int sp, SpinorView* v)
 {
   nq_ctx* c = cl->ctx;
+  const int i = sp == SP_F16 ? 1 : 0;
   SpinorView v64;
   NQ_TRY(clov_inv_view(cl, par, kappa, mu, &v64));
-  if (!cl->inv32[par].p) NQ_TRY(dev_alloc(cl->inv32[par], (size_t)72 * 8 * c->geo.stride));
-  v->data = cl->inv32[par].p;
-  v->prec = SP_F32;
+  if (!cl->invs[i][par].p) NQ_TRY(dev_alloc(cl->invs[i][par], (size_t)72 * (i ? 4 : 8) * c->geo.stride));
+  v->data = cl->invs[i][par].p;
+  v->prec = i ? SP_F16 : SP_F32;
   v->ncomp = 72;
-  if (!cl->inv32_valid[par]) {
-    NQ_TRY(launch_blocks_to_f32(c, 72, v64, *v));
-    cl->inv32_valid[par] = 1;
+  if (!cl->invs_valid[i][par]) {
+    NQ_TRY(launch_blocks_sloppy(c, 72, v64, *v));
+    cl->invs_valid[i][par] = 1;
   }
   return NQ_OK;
 }
@@ -593,16 +596,16 @@ static int clov_setup(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa,
   return NQ_OK;
 }
 
-// the same operator in a sloppy precision: FP32 (FP32 18-real links and FP32 spinors)
-// or FP16 (FP16 spinors with per-site norms and FP32 12-real links), FP32 blocks
+// the same operator in a sloppy precision: FP32 (FP32 18-real links, FP32 spinors and
+// blocks) or FP16 (FP16 spinors with per-site norms, FP32 12-real links, FP16 blocks)
 static int clov_setup_sloppy(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis,
                              double mu, int sp, ClovSetup* s)
 {
   NQ_TRY(tm_setup(c, g, sp == SP_F16 ? LK_F32_12 : LK_F32_18, anis, kappa, &s->S));
   s->cl = cl;
   s->mu = mu;
-  NQ_TRY(clov_inv_view32(cl, NQ_EVEN, kappa, mu, &s->inv_e));
-  NQ_TRY(clov_view32(cl, NQ_ODD, &s->cl_o));
+  NQ_TRY(clov_inv_view_s(cl, NQ_EVEN, kappa, mu, sp, &s->inv_e));
+  NQ_TRY(clov_view_s(cl, NQ_ODD, sp, &s->cl_o));
   const bool h = sp == SP_F16;
   NQ_TRY(alloc_spinor(c, h ? "clov.te2_16" : "clov.te2_32", sp, 12, &s->te2));
   return NQ_OK;
@@ -717,12 +720,14 @@ int TmBatch::setup(nq_ctx* ctx, const nq_gauge* g, double kappa, const double* a
     NQ_TRY(alloc_spinor_nr(ctx, t + "cuo", SP_F64, 12, n, &uo64_));
     NQ_TRY(alloc_spinor(ctx, t + "cte1", SP_F64, 12, &cte2_));
     if (mixed_) {
-      NQ_TRY(ctx_temp(ctx, t + "clinv32", (size_t)72 * 8 * ctx->geo.stride, &p));
+      // blocks in the sloppy spinor precision (packed FP32 or FP16)
+      const bool h16 = sp_ == SP_F16;
+      NQ_TRY(ctx_temp(ctx, t + (h16 ? "clinv16" : "clinv32"), (size_t)72 * (h16 ? 4 : 8) * ctx->geo.stride, &p));
       clinv_s_.data = p;
-      clinv_s_.prec = SP_F32;
+      clinv_s_.prec = h16 ? SP_F16 : SP_F32;
       clinv_s_.ncomp = 72;
-      NQ_TRY(launch_blocks_to_f32(ctx, 72, clinv64_, clinv_s_));
-      NQ_TRY(clov_view32(cl, NQ_ODD, &clo_s_));
+      NQ_TRY(launch_blocks_sloppy(ctx, 72, clinv64_, clinv_s_));
+      NQ_TRY(clov_view_s(cl, NQ_ODD, sp_, &clo_s_));
       NQ_TRY(alloc_spinor_nr(ctx, t + "cte2_" + std::to_string(sp_), sp_, 12, n, &te2s_));
       NQ_TRY(alloc_spinor_nr(ctx, t + "cuo_" + std::to_string(sp_), sp_, 12, n, &uos_));
     } else {
@@ -2136,8 +2141,10 @@ void nq_clover_destroy(nq_clover* cl)
     for (int p = 0; p < 2; p++) {
       dev_free(cl->cl[p]);
       dev_free(cl->inv[p]);
-      dev_free(cl->cl32[p]);
-      dev_free(cl->inv32[p]);
+      for (int i = 0; i < 2; i++) {
+        dev_free(cl->cls[i][p]);
+        dev_free(cl->invs[i][p]);
+      }
     }
   }
   delete cl;

@@ -145,16 +145,18 @@ template <int SP>
 using RealOf = typename std::conditional<SP == SP_F64, double, float>::type;
 
 // Twisted clover block (MODE 0: z + clover term, 1: inverse) of each chirality
-// applied in place to the twelve components v of site e (the epilogue's block stage)
-template <int BMODE, class R>
+// applied in place to the twelve components v of site e (the epilogue's block stage);
+// the blocks are stored in the spinor precision SP
+template <int BMODE, int SP, class R>
 __device__ __forceinline__ void clover_site(const HopEpi& epi, const Geo& g, uint32_t e, cx<R> v[12])
 {
-  const cx<R>* B = (const cx<R>*)epi.blk.data;
+  constexpr int EB = SP == SP_F64 ? 16 : (SP == SP_F32 ? 8 : 4);
+  const void* B = epi.blk.data;
 #pragma unroll
   for (int h = 0; h < 2; h++) {
     cx<R> w[6];
-    if (epi.blk_dag) clover_half<BMODE, 1, R>(B, g.stride, e, h, epi.blk_kappa, epi.blk_mu, v + 6 * h, w);
-    else clover_half<BMODE, 0, R>(B, g.stride, e, h, epi.blk_kappa, epi.blk_mu, v + 6 * h, w);
+    if (epi.blk_dag) clover_half<BMODE, 1, R, EB>(B, g.stride, e, h, epi.blk_kappa, epi.blk_mu, v + 6 * h, w);
+    else clover_half<BMODE, 0, R, EB>(B, g.stride, e, h, epi.blk_kappa, epi.blk_mu, v + 6 * h, w);
 #pragma unroll
     for (int q = 0; q < 6; q++) v[6 * h + q] = w[q];
   }
@@ -186,11 +188,11 @@ __device__ __forceinline__ double hop_epilogue(const Geo& g, const HopEpi& epi, 
   const cx<R> c1u = mk<R>((R)cf[0], (R)cf[1]), c1l = mk<R>((R)cf[2], (R)cf[3]);
 #pragma unroll
   for (int k = 0; k < 12; k++) o[k] = (k < 6 ? c1u : c1l) * o[k];
-  if constexpr (CLV == 1) clover_site<1, R>(epi, g, e, o);
+  if constexpr (CLV == 1) clover_site<1, SP, R>(epi, g, e, o);
   if (epi.aux.data) {
     cx<R> a[12];
     SpAcc<SP, R, 12, kNT>::load(epi.aux, g.stride, e, a, r);
-    if constexpr (CLV == 2) clover_site<0, R>(epi, g, e, a);
+    if constexpr (CLV == 2) clover_site<0, SP, R>(epi, g, e, a);
     const cx<R> cu = mk<R>((R)cf[4], (R)cf[5]), cl = mk<R>((R)cf[6], (R)cf[7]);
 #pragma unroll
     for (int k = 0; k < 12; k++) fma_to(o[k], k < 6 ? cu : cl, a[k]);
@@ -586,9 +588,9 @@ double tm_hop_bytes(int sp, int lk, const HopEpi& epi, int ncomp)
     b += s;
     if (epi.rs.data) b += sp_site_bytes(epi.rs.prec, ncomp);
   }
-  // clover blocks of the epilogue: 72 (inverse) or 36 complex per site, FP64 with FP64
-  // spinors and FP32 otherwise, read once for all right-hand sides
-  const double blk = epi.blk_mode ? (epi.blk_mode == 1 ? 72.0 : 36.0) * (sp == SP_F64 ? 16 : 8) : 0.0;
+  // clover blocks of the epilogue: 72 (inverse) or 36 complex per site in the spinor
+  // precision, read once for all right-hand sides
+  const double blk = epi.blk_mode ? (epi.blk_mode == 1 ? 72.0 : 36.0) * (sp == SP_F64 ? 16 : (sp == SP_F32 ? 8 : 4)) : 0.0;
   return 8 * lk_bytes(lk) + blk + b * epi.nrhs;
 }
 
@@ -767,8 +769,8 @@ static int go(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, 
 // Twisted clover K: the first hop with the inverse blocks (blk_mode 1, mode 0 only)
 // and the second with the clover blocks on aux (blk_mode 2, modes 0-2), for the
 // operator precisions of the clover solves: FP64 spinors with FP64 18-real links and
-// FP64 blocks; FP32 spinors with FP32 18-real links, or FP16 spinors with FP32
-// 12-real links, with FP32 blocks.
+// FP64 blocks; FP32 spinors with FP32 18-real links and FP32 blocks; FP16 spinors
+// with FP32 12-real links and FP16 blocks.
 template <int SP, int LK, int XP>
 static int clover_modes(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, const LinkView& U,
                         double anis, const HopEpi& epi, const char* tag)
@@ -792,8 +794,7 @@ static int launch_tm_hop_clover(nq_ctx* ctx, int op, const SpinorView& out, cons
                                 double anis, const HopEpi& epi)
 {
   const int sp = in.prec, lk = U.prec;
-  const int bp = sp == SP_F64 ? SP_F64 : SP_F32;
-  if (!epi.blk.data || epi.blk.prec != bp || epi.blk.ncomp != (epi.blk_mode == 1 ? 72 : 36))
+  if (!epi.blk.data || epi.blk.prec != sp || epi.blk.ncomp != (epi.blk_mode == 1 ? 72 : 36))
     return fail(NQ_EINVAL, "tm hop: clover blocks of the wrong kind or precision");
   if (sp == SP_F64 && lk == LK_F64_18) return clover_modes<SP_F64, LK_F64_18, SP_F64>(ctx, op, out, in, U, anis, epi, "f64");
   if (sp == SP_F32 && lk == LK_F32_18) return clover_modes<SP_F32, LK_F32_18, SP_F32>(ctx, op, out, in, U, anis, epi, "f32");
