@@ -302,7 +302,7 @@ def clover_bench():
     ctx.sync()
     t = time.perf_counter() - t0
     sloppy = {}
-    for pv in ("f32", "f16"):
+    for pv in ("f32", "f16", "f16h"):
         nq.tmclov_solve_eoprec(ctx, g, cl, kappa, mu, 100000, residue, fx, fs, prec=pv)
         ctx.sync()
         t0 = time.perf_counter()

@@ -241,8 +241,9 @@ int nq_tmclov_cg_kernsq(nq_ctx* ctx, const nq_gauge* g, nq_clover* cl, double ka
                         double mu, int niter, double residue, int prec, nq_field* sol, const nq_field* guess,
                         const nq_field* src, nq_solve_stats* st);
 /* inv_tmclovD_cg_eoprec (cg_invert_tmclovD_eoprec.cpp:56-232): eqs. 8-11 with the clover
-   blocks, CG on K K^+ (prec NQ_PREC_F64, or NQ_PREC_F32 / NQ_PREC_F16 inner iterations with
-   FP64 reliable updates), FP64 check |D sol - src|^2/|src|^2 */
+   blocks, CG on K K^+ (prec NQ_PREC_F64, or NQ_PREC_F32 / NQ_PREC_F16 / NQ_PREC_F16H inner
+   iterations with FP64 reliable updates; FP16 inner iterations use FP16 copies of the blocks),
+   FP64 check |D sol - src|^2/|src|^2 */
 int nq_tmclov_solve_eoprec(nq_ctx* ctx, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis,
                            double mu, int niter, double residue, int prec, nq_field* sol, const nq_field* src,
                            nq_solve_stats* st, double* check);

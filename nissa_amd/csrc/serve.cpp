@@ -430,7 +430,6 @@ int nq_server_submit(nq_server* s, const nq_serve_request* r, int64_t* id)
   if (r->prec < NQ_PREC_F64 || r->prec > NQ_PREC_F16H) return fail(NQ_EINVAL, "unknown precision %d", r->prec);
   if (!(r->residue > 0) || r->niter <= 0 || !(r->kappa != 0)) return fail(NQ_EINVAL, "bad kappa/residue/niter");
   if (!(r->csw >= 0)) return fail(NQ_EINVAL, "bad cSW %g", r->csw);
-  if (r->csw > 0 && r->prec == NQ_PREC_F16H) return fail(NQ_EINVAL, "twisted clover: inner precision FP64, FP32 or FP16");
   auto q = std::make_shared<Req>();
   q->r = *r;
   for (int k = 0; k < r->nsrc; k++) {

@@ -597,11 +597,12 @@ static int clov_setup(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa,
 }
 
 // the same operator in a sloppy precision: FP32 (FP32 18-real links, FP32 spinors and
-// blocks) or FP16 (FP16 spinors with per-site norms, FP32 12-real links, FP16 blocks)
+// blocks) or FP16 (FP16 spinors with per-site norms, FP32 or, for NQ_PREC_F16H, FP16
+// 12-real links, FP16 blocks)
 static int clov_setup_sloppy(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa, const double* anis,
-                             double mu, int sp, ClovSetup* s)
+                             double mu, int sp, int lk, ClovSetup* s)
 {
-  NQ_TRY(tm_setup(c, g, sp == SP_F16 ? LK_F32_12 : LK_F32_18, anis, kappa, &s->S));
+  NQ_TRY(tm_setup(c, g, lk, anis, kappa, &s->S));
   s->cl = cl;
   s->mu = mu;
   NQ_TRY(clov_inv_view_s(cl, NQ_EVEN, kappa, mu, sp, &s->inv_e));
@@ -2082,11 +2083,10 @@ static int clov_cg_any(nq_ctx* c, const nq_gauge* g, nq_clover* cl, double kappa
                        const SpinorView& b, nq_solve_stats* st)
 {
   if (prec == NQ_PREC_F64) return clov_cg(s64, niter, residue, x, use_guess, b, st);
-  if (prec != NQ_PREC_F32 && prec != NQ_PREC_F16)
-    return fail(NQ_EINVAL, "twisted clover solves take NQ_PREC_F64, NQ_PREC_F32 or NQ_PREC_F16");
-  const int sp = prec == NQ_PREC_F16 ? SP_F16 : SP_F32;
+  int sp = 0, lk = 0;
+  NQ_TRY(lk_of_prec(prec, &sp, &lk));
   ClovSetup ss;
-  NQ_TRY(clov_setup_sloppy(c, g, cl, kappa, anis, mu, sp, &ss));
+  NQ_TRY(clov_setup_sloppy(c, g, cl, kappa, anis, mu, sp, lk, &ss));
   return clov_cg_mixed(s64, ss, sp, niter, residue, x, use_guess, b, st);
 }
 

@@ -770,7 +770,7 @@ static int go(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, 
 // and the second with the clover blocks on aux (blk_mode 2, modes 0-2), for the
 // operator precisions of the clover solves: FP64 spinors with FP64 18-real links and
 // FP64 blocks; FP32 spinors with FP32 18-real links and FP32 blocks; FP16 spinors
-// with FP32 12-real links and FP16 blocks.
+// with FP32 or FP16 12-real links and FP16 blocks.
 template <int SP, int LK, int XP>
 static int clover_modes(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, const LinkView& U,
                         double anis, const HopEpi& epi, const char* tag)
@@ -800,6 +800,8 @@ static int launch_tm_hop_clover(nq_ctx* ctx, int op, const SpinorView& out, cons
   if (sp == SP_F32 && lk == LK_F32_18) return clover_modes<SP_F32, LK_F32_18, SP_F32>(ctx, op, out, in, U, anis, epi, "f32");
   if (sp == SP_F16 && lk == LK_F32_12)
     return clover_modes<SP_F16, LK_F32_12, SP_F32>(ctx, op, out, in, U, anis, epi, "f16_l32c");
+  if (sp == SP_F16 && lk == LK_F16_12)
+    return clover_modes<SP_F16, LK_F16_12, SP_F32>(ctx, op, out, in, U, anis, epi, "f16_l16c");
   return fail(NQ_EINVAL, "tm hop: no clover epilogue for spinor precision %d with links %d", sp, lk);
 }
 

@@ -186,10 +186,11 @@ def test_tmclov_cg_matches_oracle(setups):
     assert rel_l2(fy.download(), fx.download()) < 1e-10
 
 
-@pytest.mark.parametrize("L,prec", [(SHAPES[0], "f64"), (SHAPES[1], "f64"), (SHAPES[1], "f32"), (SHAPES[1], "f16")])
+@pytest.mark.parametrize("L,prec", [(SHAPES[0], "f64"), (SHAPES[1], "f64"), (SHAPES[1], "f32"), (SHAPES[1], "f16"),
+                                    (SHAPES[1], "f16h")])
 def test_tmclov_eo_solve_matches_oracle(setups, L, prec):
-    """FP64, and FP32 or FP16 inner iterations with FP64 reliable updates (same FP64
-    residue and the same solution to 1e-10)."""
+    """FP64, and FP32 or FP16 (FP32 or FP16 links) inner iterations with FP64 reliable
+    updates (same FP64 residue and the same solution to 1e-10)."""
     ctx, g, cl, geo, conf, ce, co, ref = setups(L)
     mu = 0.01
     src = nq.source_lx(L, 12, "gauss")
@@ -206,7 +207,7 @@ def test_tmclov_eo_solve_matches_oracle(setups, L, prec):
         assert st["reliable_updates"] >= 1
 
 
-@pytest.mark.parametrize("prec", ["f64", "f32", "f16"])
+@pytest.mark.parametrize("prec", ["f64", "f32", "f16", "f16h"])
 def test_tmclov_batched_solve(setups, prec):
     """nq_tmclov_solve_eoprec_multi: five sources at one mu through the batch (the
     clover blocks applied to every slot by the batched block kernels) reach the
