@@ -79,8 +79,9 @@ namespace nissa::nq_shims
   }
   
   // inv_tmclovD_cg_eoprec (src/inverters/twisted_clover/cg_invert_tmclovD_eoprec.cpp:146-232),
-  // FP32 inner iterations; the device solve takes no guess, so a call with one is left
-  // to the native path (returns false)
+  // FP16 inner iterations (FP16 spinors and clover blocks, FP32 12-real links) with FP64
+  // reliable updates; the device solve takes no guess, so a call with one is left to the
+  // native path (returns false)
   inline bool solve_tmclovD_eoprec(LxField<spincolor>& solution_lx,
 				   const std::optional<OddField<spincolor>>& guess_Koo,
 				   const LxField<quad_su3>& conf_lx,
@@ -100,7 +101,7 @@ namespace nissa::nq_shims
     nq_solve_stats st;
     double chk;
     const int rc=nq_tmclov_solve_eoprec(ctx(),gauge(conf_lx),cl,kappa,anis_ptr(anis),mass,nitermax,targResidue,
-					NQ_PREC_F32,x.f,s.f,&st,&chk);
+					NQ_PREC_F16,x.f,s.f,&st,&chk);
     nq_clover_destroy(cl);
     if(rc!=NQ_ENOCONV) check(rc,"inv_tmclovD_cg_eoprec");
     x.down(solution_lx);
@@ -109,7 +110,7 @@ namespace nissa::nq_shims
 
   // the propagator driver's clover branch (projects/ib/prop.cpp:93-94) for several
   // sources at one (kappa, cSW, mass): one clover term, the K K^+ CG of up to 16
-  // sources in the same launches, FP32 inner iterations
+  // sources in the same launches, FP16 inner iterations
   inline void solve_tmclovD_eoprec_multi(std::vector<LxField<spincolor>*>& solutions_lx,
 					 const LxField<quad_su3>& conf_lx,
 					 const double& kappa,
@@ -137,7 +138,7 @@ namespace nissa::nq_shims
     check(nq_clover_create(ctx(),gauge(conf_lx),cSW,&cl),"clover_term");
     std::vector<nq_solve_stats> st(n);
     const int rc=nq_tmclov_solve_eoprec_multi(ctx(),gauge(conf_lx),cl,kappa,anis_ptr(anis),mass,n,nitermax,residue,
-					      NQ_PREC_F32,xf.data(),sf.data(),st.data());
+					      NQ_PREC_F16,xf.data(),sf.data(),st.data());
     nq_clover_destroy(cl);
     check(rc,"inv_tmclovD_cg_eoprec (multi)");
     for(int i=0;i<n;i++) x[i].down(*solutions_lx[i]);
