@@ -353,3 +353,41 @@ def test_st_solve_D_matches_oracle(st_setups, L):
     ye = ctx.field(nq.COLOR, nq.EVEN)
     st2 = nq.st_solve_D(ctx, g, m, 10000, 1e-24, ye, None, fe, fo)
     assert st2["converged"] and np.array_equal(ye.download(), xe.download())
+
+
+def test_limits_are_refused_and_wide_batches_stream(st_setups):
+    """The documented limits: lattice extents must be even and >= 2 (NQ_EINVAL, as
+    nissa's geometry CRASHes on an odd extent), at most 16 shifts per multi-shift
+    solve and 16 masses per multi-mass CG (NQ_EINVAL); and more sources than one
+    launch takes (17 here, against 16) stream through in groups, each source with
+    the bits it has alone."""
+    for bad in [(3, 4, 4, 4), (4, 4, 4, 1), (4, 0, 4, 4)]:
+        with pytest.raises(nq.NqError) as e:
+            nq.Ctx(bad)
+        assert e.value.code == nq.NQ_EINVAL, bad
+    L = (4, 6, 4, 8)
+    ctx, g, geo, conf, ce, co = st_setups(L)
+    src = ctx.field(nq.COLOR, nq.EVEN, gauss_eo(geo.Vh, 90, ncomp=3))
+    shifts = [0.01 * (k + 1) for k in range(17)]
+    sols = [[ctx.field(nq.COLOR, nq.EVEN) for _ in shifts]]
+    with pytest.raises(nq.NqError) as e:
+        nq.st_cgm(ctx, g, shifts, 1000, [1e-20] * 17, sols, [src])
+    assert e.value.code == nq.NQ_EINVAL
+    tconf = tm_conf(L, 91)
+    gt = nq.Gauge(ctx, lx=tconf)
+    fs = ctx.field(nq.SPINCOLOR, nq.LX, nq.source_lx(L, 92, "gauss"))
+    masses = [0.01 * (k + 1) for k in range(17)]
+    xs = [ctx.field(nq.SPINCOLOR, nq.LX) for _ in masses]
+    with pytest.raises(nq.NqError) as e:
+        nq.tmQ2_cgm(ctx, gt, 0.16, masses, 1000, [1e-20] * 17, xs, fs)
+    assert e.value.code == nq.NQ_EINVAL
+    srcs = [ctx.field(nq.COLOR, nq.EVEN, gauss_eo(geo.Vh, 100 + i, ncomp=3)) for i in range(17)]
+    sh = [0.001, 0.05]
+    wide = [[ctx.field(nq.COLOR, nq.EVEN) for _ in sh] for _ in srcs]
+    res = nq.st_cgm(ctx, g, sh, 20000, [1e-22] * 2, wide, srcs)
+    assert all(r["rc"] == nq.NQ_OK and max(r["true_res_sq"]) < 2e-22 for r in res)
+    for i in (0, 16):
+        alone = [[ctx.field(nq.COLOR, nq.EVEN) for _ in sh]]
+        nq.st_cgm(ctx, g, sh, 20000, [1e-22] * 2, alone, srcs[i:i + 1])
+        for k in range(2):
+            assert np.array_equal(alone[0][k].download(), wide[i][k].download()), (i, k)
