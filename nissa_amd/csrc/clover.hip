@@ -381,12 +381,13 @@ __global__ void __launch_bounds__(256) k_clover_apply_b16(Geo g, SpinorView out,
 // FP32 or FP16 packed copy of an FP64 block field (sloppy solves): MODE 0 the clover
 // blocks (36 complex per site), 1 the inverse blocks (72)
 template <int MODE, int EB>
-__global__ void __launch_bounds__(256) k_blocks_pack(Geo g, const C* in, void* out)
+__global__ void __launch_bounds__(256) k_blocks_pack(Geo g, const C* in, void* out, int* out_of_range)
 {
   const uint32_t e = blockIdx.x * 256 + threadIdx.x;
   if (e >= (uint32_t)g.Vh) return;
-  clover_pack<MODE, EB>(in, out, g.stride, e, 0);
-  clover_pack<MODE, EB>(in, out, g.stride, e, 1);
+  const bool ok0 = clover_pack<MODE, EB>(in, out, g.stride, e, 0);
+  const bool ok1 = clover_pack<MODE, EB>(in, out, g.stride, e, 1);
+  if (!(ok0 && ok1)) *out_of_range = 1;  // every writer stores the same value
 }
 
 // Re tr of the six plaquettes U_mu(X) U_nu(X+mu) U_mu(X+nu)^+ U_nu(X)^+ of each
@@ -479,16 +480,28 @@ int launch_blocks_sloppy(nq_ctx* ctx, int ncomp, const SpinorView& in, const Spi
   const Geo& g = ctx->geo;
   if (in.prec != SP_F64 || (out.prec != SP_F32 && out.prec != SP_F16) || (ncomp != 36 && ncomp != 72))
     return fail(NQ_EINVAL, "block conversion: FP64 -> FP32 or FP16, 36 or 72 components");
+  if (ctx->capturing) return fail(NQ_EINVAL, "block conversion inside a graph capture");
   const dim3 grid((unsigned)((g.Vh + 255) / 256)), b256(256);
   const C* src = (const C*)in.data;
+  int* flag = nullptr;
+  NQ_TRY(ctx_temp(ctx, "blk.range", sizeof(int), (void**)&flag));
+  NQ_HIP(hipMemsetAsync(flag, 0, sizeof(int), ctx->stream));
   if (out.prec == SP_F32) {
-    if (ncomp == 36) hipLaunchKernelGGL((k_blocks_pack<0, 8>), grid, b256, 0, ctx->stream, g, src, out.data);
-    else hipLaunchKernelGGL((k_blocks_pack<1, 8>), grid, b256, 0, ctx->stream, g, src, out.data);
+    if (ncomp == 36) hipLaunchKernelGGL((k_blocks_pack<0, 8>), grid, b256, 0, ctx->stream, g, src, out.data, flag);
+    else hipLaunchKernelGGL((k_blocks_pack<1, 8>), grid, b256, 0, ctx->stream, g, src, out.data, flag);
   } else {
-    if (ncomp == 36) hipLaunchKernelGGL((k_blocks_pack<0, 4>), grid, b256, 0, ctx->stream, g, src, out.data);
-    else hipLaunchKernelGGL((k_blocks_pack<1, 4>), grid, b256, 0, ctx->stream, g, src, out.data);
+    if (ncomp == 36) hipLaunchKernelGGL((k_blocks_pack<0, 4>), grid, b256, 0, ctx->stream, g, src, out.data, flag);
+    else hipLaunchKernelGGL((k_blocks_pack<1, 4>), grid, b256, 0, ctx->stream, g, src, out.data, flag);
   }
   NQ_HIP(hipGetLastError());
+  if (out.prec == SP_F16) {
+    // an FP16 copy of blocks beyond its range (near-singular D_ee at a tiny twisted
+    // mass) would turn the sloppy operator into infinities: refuse it instead
+    int bad = 0;
+    NQ_HIP(hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    NQ_HIP(hipStreamSynchronize(ctx->stream));
+    if (bad) return fail(NQ_EINVAL, "clover blocks beyond the FP16 range (|entry| > 65000): use FP32 inner iterations");
+  }
   return NQ_OK;
 }
 

@@ -46,15 +46,21 @@ __device__ __forceinline__ void clover_load(const void* B, int64_t st, uint32_t 
 }
 
 // chirality h of an FP64 block field ([k][e] complex, N per chirality) stored in the
-// packed layout above with element bytes EB (the sloppy copies)
+// packed layout above with element bytes EB (the sloppy copies); false when an FP16
+// copy would overflow (a real part or imaginary part above 65000 in magnitude)
 template <int MODE, int EB>
-__device__ __forceinline__ void clover_pack(const cx<double>* in, void* out, int64_t st, uint32_t e, int h)
+__device__ __forceinline__ bool clover_pack(const cx<double>* in, void* out, int64_t st, uint32_t e, int h)
 {
   constexpr int N = MODE == 0 ? 18 : 36;
   cx<double> v[N];
+  double big = 0;
 #pragma unroll
-  for (int q = 0; q < N; q++) v[q] = in[(int64_t)(N * h + q) * st + e];
+  for (int q = 0; q < N; q++) {
+    v[q] = in[(int64_t)(N * h + q) * st + e];
+    big = fmax(big, fmax(fabs(v[q].re), fabs(v[q].im)));
+  }
   store_packed<EB, N, 0, double>((char*)out + (int64_t)h * N * EB * st, st, e, v);
+  return EB != 4 || big <= 65000.0;
 }
 
 template <int MODE, int DAG, class R>

@@ -250,3 +250,27 @@ def test_tmclov_errors_are_reported(setups):
     with pytest.raises(nq.NqError) as e:
         nq.tmclov_kern(ctx, g, cl, KAPPA, 0.01, fe, fb)
     assert e.value.code == nq.NQ_EINVAL
+
+
+def test_tmclov_fp16_blocks_out_of_range_are_refused(setups):
+    """FP16 inner iterations keep FP16 copies of the clover blocks; blocks beyond the
+    FP16 range (here a clover term scaled by cSW = 1e6) are refused with NQ_EINVAL at
+    set-up instead of iterating on infinities, and FP32 inner iterations still take
+    them; the context then solves normally."""
+    L = SHAPES[1]
+    ctx, g, cl, geo, conf, ce, co, ref = setups(L)
+    big = nq.Clover(ctx, g, 1e6)
+    fs = ctx.field(nq.SPINCOLOR, nq.LX, nq.source_lx(L, 41, "gauss"))
+    fx = ctx.field(nq.SPINCOLOR, nq.LX)
+    for prec in ("f16", "f16h"):
+        with pytest.raises(nq.NqError) as e:
+            nq.tmclov_solve_eoprec(ctx, g, big, KAPPA, 0.01, 50, 1e-20, fx, fs, prec=prec)
+        assert e.value.code == nq.NQ_EINVAL, prec
+    try:
+        st = nq.tmclov_solve_eoprec(ctx, g, big, KAPPA, 0.01, 5, 1e-20, fx, fs, prec="f32")
+        assert st["rc"] in (nq.NQ_OK, nq.NQ_ENOCONV)
+    except nq.NqError as e:  # not refused for range; five iterations may not hold up
+        assert e.code != nq.NQ_EINVAL
+    big.close()
+    st = nq.tmclov_solve_eoprec(ctx, g, cl, KAPPA, 0.01, 20000, 1e-24, fx, fs, prec="f16")
+    assert st["rc"] == nq.NQ_OK and st["check_rel_res_sq"] < 1e-22
