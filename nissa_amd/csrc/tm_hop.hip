@@ -747,9 +747,12 @@ static int go(nq_ctx* ctx, int op, const SpinorView& out, const SpinorView& in, 
   }
   else {
     // one extra workgroup when it carries the finalisation of the previous reduction
-    fold_here = MODE == 0 && epi.fold_fin != FIN_NONE && nr == 1 && epi.st;
+    // (256 threads: the 1024-lane summation order); the kernel sees fold_fin only then
+    fold_here = MODE == 0 && epi.fold_fin != FIN_NONE && nr == 1 && epi.st && L.threads == 256;
+    HopEpi ke = epi;
+    if (!fold_here) ke.fold_fin = FIN_NONE;
     NQ_HOP_LAUNCH((k_tm_hop<SP, LK, MODE, XP, CLV>), dim3((unsigned)(L.blocks + (fold_here ? 1 : 0))),
-                  dim3(L.threads), g, op, out, in, U, anis, anis != 1.0 ? 1 : 0, epi, L.s, L.nchunks, ctx->partial);
+                  dim3(L.threads), g, op, out, in, U, anis, anis != 1.0 ? 1 : 0, ke, L.s, L.nchunks, ctx->partial);
   }
   prof_end_ext(ctx);
   NQ_HIP(hipGetLastError());
