@@ -766,18 +766,20 @@ static double csite(int sp) { return sp == SP_F64 ? 48.0 : 24.0; }
 static double lbytes(int lk) { return lk == LK_F64_18 ? 144.0 : 72.0; }
 
 // Right-hand sides per wavefront of the staggered hops: one for a single right-hand
-// side; 8 for FP32 batches of a multiple of 8 (the 16-system refinement groups:
-// 538 vs 601 us per hop at 32^3x64; 12 sources split 8 + 4 measured slower than
-// 4 + 4 + 4), else 4.  NQ_ST_RB = 1, 2, 4 or 8 overrides; every choice gives the
-// same bits.  The launch shape for RB: W = 4 chunks per workgroup, one RB group
-// per wavefront.
+// side; for FP32 batches of five or more, 6 (12 sources x 12 shifts at 32^3x64: the
+// multi-shift hop 1214 against 1245 us with RB = 4; the 16-system refinement CG's
+// fused tail 511 against 534-557 us with RB = 8 or 4); else 4 (FP64: RB = 6 takes
+// 208 VGPRs and 72 KB of LDS, two wavefronts per SIMD, and measured 2 % slower).
+// NQ_ST_RB = 1, 2, 4, 6 or 8 (8: FP32 only) overrides; every choice gives the same
+// bits.  The launch shape for RB: W = 4 chunks per workgroup, one RB group per
+// wavefront.
 static int st_rb(int nr, int sp)
 {
   if (nr <= 1) return 1;
   const char* e = getenv("NQ_ST_RB");
-  const int v = e ? atoi(e) : (sp == SP_F32 && nr % 8 == 0 ? 8 : 4);
+  const int v = e ? atoi(e) : (sp == SP_F32 && nr >= 5 ? 6 : 4);
   if (v == 8) return sp == SP_F32 ? 8 : 4;
-  return v == 1 || v == 2 ? v : 4;
+  return v == 1 || v == 2 || v == 6 ? v : 4;
 }
 static Shape st_shape(int nr, int RB)
 {
@@ -801,6 +803,8 @@ static void st_hop_go(nq_ctx* ctx, int RB, int op, const SpinorView& out, const 
   case 1: hipLaunchKernelGGL((k_st_hop<SP, LK, MODE, 1>), grid, blk, 0, ctx->stream, g, op, out, in, U, c1, c2, coef, aux,
                              write, sh, nr, nchunks, ctx->partial, skip_flags, skip_stride, skip_n); break;
   case 2: hipLaunchKernelGGL((k_st_hop<SP, LK, MODE, 2>), grid, blk, 0, ctx->stream, g, op, out, in, U, c1, c2, coef, aux,
+                             write, sh, nr, nchunks, ctx->partial, skip_flags, skip_stride, skip_n); break;
+  case 6: hipLaunchKernelGGL((k_st_hop<SP, LK, MODE, 6>), grid, blk, 0, ctx->stream, g, op, out, in, U, c1, c2, coef, aux,
                              write, sh, nr, nchunks, ctx->partial, skip_flags, skip_stride, skip_n); break;
   case 8:
     if constexpr (SP == SP_F32) {
@@ -874,10 +878,12 @@ int launch_st_cgtail(nq_ctx* ctx, const SpinorView& t, const LinkView& U, const 
   if (sp == SP_F64 && lk == LK_F64_18) {
     if (RB == 1) NQ_TL(SP_F64, LK_F64_18, 1);
     else if (RB == 2) NQ_TL(SP_F64, LK_F64_18, 2);
+    else if (RB == 6) NQ_TL(SP_F64, LK_F64_18, 6);
     else NQ_TL(SP_F64, LK_F64_18, 4);
   } else if (sp == SP_F32 && lk == LK_F32_18) {
     if (RB == 1) NQ_TL(SP_F32, LK_F32_18, 1);
     else if (RB == 2) NQ_TL(SP_F32, LK_F32_18, 2);
+    else if (RB == 6) NQ_TL(SP_F32, LK_F32_18, 6);
     else if (RB == 8) NQ_TL(SP_F32, LK_F32_18, 8);
     else NQ_TL(SP_F32, LK_F32_18, 4);
   } else return fail(NQ_EINVAL, "st CG tail: unsupported precision combination");
@@ -1107,10 +1113,12 @@ int launch_cgm_persistent(nq_ctx* ctx, const LinkView& U, const SpinorView& p, c
   if (sp == SP_F64 && lk == LK_F64_18) {
     if (RB == 1) rc = persistent_go<SP_F64, LK_F64_18, 1>(ctx, A);
     else if (RB == 2) rc = persistent_go<SP_F64, LK_F64_18, 2>(ctx, A);
+    else if (RB == 6) rc = persistent_go<SP_F64, LK_F64_18, 6>(ctx, A);
     else rc = persistent_go<SP_F64, LK_F64_18, 4>(ctx, A);
   } else if (sp == SP_F32 && lk == LK_F32_18) {
     if (RB == 1) rc = persistent_go<SP_F32, LK_F32_18, 1>(ctx, A);
     else if (RB == 2) rc = persistent_go<SP_F32, LK_F32_18, 2>(ctx, A);
+    else if (RB == 6) rc = persistent_go<SP_F32, LK_F32_18, 6>(ctx, A);
     else if (RB == 8) rc = persistent_go<SP_F32, LK_F32_18, 8>(ctx, A);
     else rc = persistent_go<SP_F32, LK_F32_18, 4>(ctx, A);
   } else {
@@ -1165,6 +1173,8 @@ int launch_cgm_iteration(nq_ctx* ctx, const LinkView& U, const SpinorView& p, co
   case 1: hipLaunchKernelGGL((k_cgm_hop2<SPV, LKV, 1>), grid2, blk2, 0, ctx->stream, g, t, U, r, sol, ps, sstride, st, \
                              sh, nr, L.nchunks, ctx->partial); break;                                             \
   case 2: hipLaunchKernelGGL((k_cgm_hop2<SPV, LKV, 2>), grid2, blk2, 0, ctx->stream, g, t, U, r, sol, ps, sstride, st, \
+                             sh, nr, L.nchunks, ctx->partial); break;                                             \
+  case 6: hipLaunchKernelGGL((k_cgm_hop2<SPV, LKV, 6>), grid2, blk2, 0, ctx->stream, g, t, U, r, sol, ps, sstride, st, \
                              sh, nr, L.nchunks, ctx->partial); break;                                             \
   default: hipLaunchKernelGGL((k_cgm_hop2<SPV, LKV, 4>), grid2, blk2, 0, ctx->stream, g, t, U, r, sol, ps, sstride,  \
                               st, sh, nr, L.nchunks, ctx->partial); break;                                        \
