@@ -21,7 +21,8 @@ if ROOT not in sys.path:
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--what", default="hop", choices=["hop", "hopmulti", "copy", "solve", "multisolve", "stagcgm"])
+    ap.add_argument("--what", default="hop",
+                    choices=["hop", "hopmulti", "copy", "solve", "multisolve", "stagcgm", "clover"])
     ap.add_argument("--nrhs", type=int, default=12)
     ap.add_argument("--prec", default="f64", choices=["f64", "f32", "f16", "f16h"])
     ap.add_argument("--n", type=int, default=20)
@@ -89,6 +90,22 @@ def main():
               f"worst {max(max(r['true_res_sq']) for r in res):.2e}")
         if a.n > 0:
             print_ledger(10)
+    elif a.what == "clover":
+        # twisted-clover eo solves (cSW 1.0, kappa 0.16, mu 0.003, residue 1e-20) on the
+        # same links; the first solve builds the clover term and its blocks
+        cl = nq.Clover(ctx, g, 1.0)
+        fs = ctx.field(nq.SPINCOLOR, nq.LX, nq.source_lx(L, 17, "gauss"))
+        fx = ctx.field(nq.SPINCOLOR, nq.LX)
+        nq.tmclov_solve_eoprec(ctx, g, cl, 0.16, 0.003, 100000, 1e-20, fx, fs, prec=a.prec)
+        for i in range(a.n):
+            ctx.profiling(a.ledger and i == a.n - 1)
+            st = nq.tmclov_solve_eoprec(ctx, g, cl, 0.16, 0.003, 100000, 1e-20, fx, fs, prec=a.prec)
+        ctx.profiling(False)
+        print(f"clover solve {a.prec}: {st['seconds']*1e3:.1f} ms, iters {st['iters']}, "
+              f"check {st['check_rel_res_sq']:.2e}")
+        if a.ledger:
+            print_ledger()
+        cl.close()
     elif a.what == "copy":
         fo2 = ctx.field(nq.SPINCOLOR, nq.ODD)
         for _ in range(a.n):
